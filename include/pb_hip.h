@@ -1,0 +1,263 @@
+/*
+ * pb_hip.h - C ABI of the MI355X (gfx950) bundle-adjustment backend.
+ *
+ * The backend runs the frame-BA nonlinear solve of PlaBundle's online point
+ * Schur path on one GPU: per-observation residual/Jacobian/robust weight,
+ * damped 3x3 point-block inversion, Schur reduction into camera-camera
+ * blocks, block-Jacobi PCG, point back-substitution and the Armijo
+ * acceptance loop, on a device-resident workspace that is reused across
+ * iterations and solves.
+ *
+ * Reference interfaces replaced (paths relative to the PlaBundle tree):
+ *   - internal::optimizePointsWithPlaMatrix(cameras, tracks, BAOptions, workspace)
+ *       src/internal/BundleAdjustPlaMatrix.h:27-30, src/internal/BundleAdjustPlaMatrix.cpp:162-631
+ *     -> pb_hip_upload + pb_hip_solve + pb_hip_download
+ *   - plamatrix::internal::SchurComplementSolverWorkspace<double> (the reusable
+ *     workspace handed through Solver::solve(problem, options, workspace))
+ *       include/plabundle/solver.h:13-32, src/solver.cpp:22-38
+ *     -> pb_ctx (created by pb_hip_create, owned by the caller)
+ *   - isPlaMatrixBackendAvailable(backend, device)
+ *       src/internal/BundleAdjustPlaMatrixRuntime.cpp:58-153
+ *     -> pb_hip_available
+ *   - validateAndNormalizeBundleAdjustOptions (gauge normalisation) and
+ *     prepareActiveProblem (block numbering)
+ *       src/internal/BundleAdjustValidation.cpp:371-896,
+ *       src/internal/BundleAdjustPlaMatrixProblem.cpp:29-278
+ *     -> run inside pb_hip_solve; results exposed by pb_hip_get_masks
+ *
+ * Conventions
+ *   - All functions return pb_status; no exception crosses the ABI.
+ *   - Host buffers passed in are borrowed for the duration of the call and
+ *     copied; the context owns every device allocation.
+ *   - One context per host thread; a context is not re-entrant. Each context
+ *     owns one HIP stream on the device it was created for.
+ *   - Camera rotations are camera-to-world, row-major 3x3. A world point X
+ *     maps to the camera frame as R^T (X - C); the camera looks along +Z.
+ *   - Per-camera intrinsics use the 13-slot parameter block of
+ *     CameraState::parameterBlock (src/internal/CameraState.cpp:164-185):
+ *       [fx, log(fy/fx), cx, cy, k1, k2, k3, p1, p2, b2, k4, p3, p4]
+ *     with pixel = (fx*xd + b2*yd + cx, fy*yd + cy).
+ */
+#ifndef PB_HIP_H
+#define PB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PB_ABI_VERSION 1
+#define PB_INTRINSIC_COUNT 13
+
+typedef enum pb_status {
+    PB_OK = 0,
+    PB_ERR_INVALID = 1,     /* bad arguments or problem data */
+    PB_ERR_DEVICE = 2,      /* HIP runtime error or no usable gfx950 device */
+    PB_ERR_OOM = 3,         /* device or host allocation failed */
+    PB_ERR_NUMERIC = 4,     /* numerical failure (non-finite state, invalid projection in a build) */
+    PB_ERR_CANCELLED = 5,   /* cancel flag observed */
+    PB_ERR_UNSUPPORTED = 6  /* configuration outside what this backend implements */
+} pb_status;
+
+/* Mirrors plabundle::SolveStatus (include/plabundle/backend.h:17-27). */
+typedef enum pb_solve_status {
+    PB_SOLVE_NOT_RUN = 0,
+    PB_SOLVE_SUCCESS = 1,
+    PB_SOLVE_NO_CONVERGENCE = 2,
+    PB_SOLVE_CANCELLED = 3,
+    PB_SOLVE_INVALID_INPUT = 4,
+    PB_SOLVE_UNSUPPORTED = 5,
+    PB_SOLVE_BACKEND_UNAVAILABLE = 6,
+    PB_SOLVE_NUMERICAL_FAILURE = 7
+} pb_solve_status;
+
+/* Mirrors plabundle::ImageRobustLoss (include/plabundle/options.h:17-22). */
+typedef enum pb_robust_loss {
+    PB_LOSS_LEAST_SQUARES = 0,
+    PB_LOSS_HUBER = 1,
+    PB_LOSS_CAUCHY = 2
+} pb_robust_loss;
+
+/* Mirrors plabundle::GaugePolicy (include/plabundle/problem.h:15-20). */
+typedef enum pb_gauge_policy {
+    PB_GAUGE_AUTO_ANCHOR = 0,
+    PB_GAUGE_REQUIRE_EXPLICIT = 1,
+    PB_GAUGE_CALLER_MANAGED = 2
+} pb_gauge_policy;
+
+/* Brown tangential convention of a frame camera (placamera::BrownTangentialConvention). */
+typedef enum pb_brown_convention {
+    PB_BROWN_OPENCV = 0,
+    PB_BROWN_METASHAPE = 1
+} pb_brown_convention;
+
+typedef enum pb_precision {
+    PB_PRECISION_FP64 = 0,  /* every stage in FP64 */
+    PB_PRECISION_MIXED = 1  /* FP32 Jacobians/Schur/PCG; FP64 cost, acceptance, point inverses */
+} pb_precision;
+
+/* Problem, structure-of-arrays, observations grouped by track.
+ * Mirrors plabundle::Problem (include/plabundle/problem.h:37-54) restricted to
+ * frame-pinhole cameras without rigs or survey constraints. */
+typedef struct pb_problem_view {
+    int32_t num_cameras;
+    const double* cam_rotation;     /* [num_cameras*9] camera-to-world, row-major */
+    const double* cam_center;       /* [num_cameras*3] world camera centres */
+    const double* cam_intrinsics;   /* [num_cameras*13] parameter blocks (slot order above) */
+    const int32_t* cam_convention;  /* [num_cameras] pb_brown_convention, or NULL = all OpenCV */
+    const int32_t* cam_image_size;  /* [num_cameras*2] width, height in pixels, or NULL */
+    const double* ref_intrinsics;   /* [num_cameras*13] shared-intrinsic reference cameras, or NULL = cam_intrinsics */
+    const int32_t* cam_group;       /* [num_cameras] calibration group ids (>= 0), or NULL = one group */
+
+    int32_t num_tracks;
+    const double* track_point;      /* [num_tracks*3] initial world points */
+    const int64_t* track_offset;    /* [num_tracks+1] observation ranges per track */
+    const int32_t* obs_camera;      /* [num_obs] camera index */
+    const double* obs_uv;           /* [num_obs*2] measured pixel (u, v) */
+    const double* obs_weight;       /* [num_obs] or NULL = 1 */
+    const double* obs_scale;        /* [num_obs] measurement scale, or NULL = 1 */
+
+    int32_t num_fixed_cameras;
+    const int32_t* fixed_cameras;
+    int32_t num_fixed_tracks;
+    const int32_t* fixed_tracks;
+
+    int32_t gauge_policy;           /* pb_gauge_policy */
+    int32_t gauge_anchor_camera;    /* reference gauge anchor (-1 = none) */
+    int32_t gauge_scale_camera;     /* reference gauge scale camera (-1 = none) */
+    double gauge_baseline;          /* reference gauge baseline (0 = none) */
+} pb_problem_view;
+
+/* Solve options. Mirrors the fields of plabundle::Options
+ * (include/plabundle/options.h:189-297) that the frame-BA path reads.
+ * pb_default_options fills the reference defaults. */
+typedef struct pb_solve_options {
+    int32_t max_iterations;              /* maxIterations = 20 */
+    int32_t robust_loss;                 /* imageRobustLoss = LeastSquares */
+    double robust_loss_scale;            /* imageRobustLossScalePixels = 1 */
+    double armijo_coefficient;           /* referenceArmijoCoefficient = 1e-3 */
+    int32_t line_search_steps;           /* referenceLineSearchSteps = 34 */
+    int32_t use_online_schur;            /* useReferenceOnlineSchur; this backend implements the online path */
+    double max_initial_track_rms;        /* maxInitialTrackRms = 100 */
+    int32_t refine_camera_pose;          /* refineCameraPose = true */
+    uint32_t shared_intrinsic_mask;      /* bit i: slot i refined per group (sharedIntrinsicParameterEnabled) */
+
+    /* CalibrationOptions bounds and priors (options.h:79-128). */
+    double min_focal_scale;              /* 0.5 */
+    double max_focal_scale;              /* 4.0 */
+    double min_aspect_scale;             /* 0.85 */
+    double max_aspect_scale;             /* 1.18 */
+    double max_principal_offset_fraction;/* 0.08 */
+    double focal_prior_sigma;            /* 0.35 (fraction of the focal reference) */
+    double principal_prior_sigma_fraction;/* 0.04 */
+    double aspect_prior_sigma;           /* 0.08 */
+    double max_abs[9];                   /* |k1| |k2| |k3| |p1| |p2| skew-fraction |k4| |p3| |p4| */
+    double prior_sigma[9];               /* k1 k2 k3 p1 p2 skew-fraction k4 p3 p4 */
+    double low_order_distortion_scale;   /* sharedLowOrderDistortionScale = 1 */
+
+    /* Backend controls. */
+    int32_t device;                      /* plaMatrixDevice */
+    int32_t precision;                   /* pb_precision */
+    double pcg_relative_tolerance;       /* 0 = path default (2e-10 FP64, 1e-3 mixed) */
+    double pcg_absolute_tolerance;       /* 0 = 1e-12 */
+    int32_t pcg_max_iterations;          /* 0 = max(200, 30 * primary blocks) */
+    int32_t pcg_check_interval;          /* host polls PCG convergence every N iterations (0 = 8) */
+    int32_t armijo_batch;                /* trial step lengths evaluated per cost pass (0 = 4) */
+    int32_t profile_kernels;             /* 1 = time kernels with HIP events (pb_hip_kernel_stats) */
+} pb_solve_options;
+
+/* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */
+typedef struct pb_solve_report {
+    int32_t solve_status;                /* pb_solve_status */
+    int32_t solution_usable;
+    int32_t converged;
+    int32_t iterations;
+    double initial_cost;
+    double final_cost;
+    int32_t accepted_steps;
+    int32_t rejected_steps;
+    int32_t linearizations;
+    int32_t objective_evaluations;
+    int32_t linear_iterations;
+    int32_t rejected_initial_tracks;
+    int32_t observation_count;
+    int32_t active_track_count;
+    int32_t camera_block_count;
+    int32_t intrinsic_block_count;
+    int32_t track_block_count;
+    int32_t primary_block_count;
+    int32_t online_schur_used;
+    int32_t mixed_precision_used;
+    int32_t gauge_anchor_camera;         /* after normalisation */
+    int32_t gauge_scale_camera;
+    double gauge_baseline;
+    double setup_seconds;                /* host preparation + upload of derived structures */
+    double solve_seconds;                /* first device launch of the solve to controller stop */
+    double assembly_seconds;
+    double linear_solve_seconds;
+    double back_substitution_seconds;
+    double objective_seconds;
+    double linear_tolerance;
+    char device_name[64];
+    char message[256];
+} pb_solve_report;
+
+typedef struct pb_ctx pb_ctx;
+
+/* Fills *options with the reference defaults (options.h:189-297) and the backend defaults. */
+void pb_default_options(pb_solve_options* options);
+
+/* 1 if a gfx950 device with the given index is usable, else 0 (never throws). */
+int pb_hip_available(int device);
+
+pb_status pb_hip_create(int device, pb_ctx** ctx);
+pb_status pb_hip_destroy(pb_ctx* ctx);
+
+/* Copies the problem into the context (host side). Device structures are built
+ * by the next pb_hip_solve; uploading again discards the previous problem. */
+pb_status pb_hip_upload(pb_ctx* ctx, const pb_problem_view* problem);
+
+/* Runs the nonlinear solve on the uploaded problem, starting from its initial
+ * state. cancel (may be NULL) is polled between trials; when it reads non-zero
+ * the solve stops and publishes nothing (state stays at the input). */
+pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_report* report,
+                       const volatile int32_t* cancel);
+
+/* Refined state of the last solve. Any pointer may be NULL.
+ *   cam_rotation [num_cameras*9], cam_center [num_cameras*3],
+ *   cam_intrinsics [num_cameras*13] (published per publishIntrinsics),
+ *   track_point [num_tracks*3], track_valid [num_tracks]. */
+pb_status pb_hip_download(pb_ctx* ctx, double* cam_rotation, double* cam_center, double* cam_intrinsics,
+                          double* track_point, int32_t* track_valid);
+
+/* Block numbering of the last solve (bit-exact with prepareActiveProblem).
+ * Arrays sized [num_cameras] / [num_tracks]; any pointer may be NULL. */
+pb_status pb_hip_get_masks(pb_ctx* ctx, int32_t* camera_block, int32_t* intrinsic_block_by_camera,
+                           int32_t* calibration_group_by_camera, int32_t* track_block, int32_t* active_track);
+
+/* Kernel timing collected when options.profile_kernels was set.
+ * name: kernel family ("linearize_points", "camera_direct", "schur_blocks",
+ * "pcg_spmv", "cost"). Returns launch count and total milliseconds. */
+pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, double* total_ms,
+                              double* bytes_per_launch);
+
+/* Reduced camera system at the uploaded initial state and the given damping
+ * (one linearisation, no solve), in the reference's 13-padded block layout:
+ * rhs [P*13] and the dense reduced matrix [P*13][P*13] (row-major) after
+ * finalizePrimaryDiagonal, plus the objective at that state. Intended for
+ * parity checks on small problems (P*13 <= 4096). */
+pb_status pb_hip_reduced_system(pb_ctx* ctx, const pb_solve_options* options, double damping,
+                                double* matrix, double* rhs, double* cost, int32_t* primary_blocks);
+
+/* Objective (image robust cost + intrinsic priors) at the uploaded initial state. */
+pb_status pb_hip_evaluate_cost(pb_ctx* ctx, const pb_solve_options* options, double* cost);
+
+/* Last error message of the context (never NULL). */
+const char* pb_hip_last_error(const pb_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PB_HIP_H */

@@ -1,0 +1,277 @@
+"""ctypes mirror of include/pb_hip.h (the C ABI of the MI355X backend).
+
+Only data layouts and the library loader live here. The structures must stay
+field-for-field identical to the header; tests/test_abi.py checks the symbol
+list and the struct sizes against the header.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+INTRINSIC_COUNT = 13
+
+# pb_status
+PB_OK = 0
+PB_ERR_INVALID = 1
+PB_ERR_DEVICE = 2
+PB_ERR_OOM = 3
+PB_ERR_NUMERIC = 4
+PB_ERR_CANCELLED = 5
+PB_ERR_UNSUPPORTED = 6
+
+STATUS_NAMES = {
+    PB_OK: "ok",
+    PB_ERR_INVALID: "invalid",
+    PB_ERR_DEVICE: "device",
+    PB_ERR_OOM: "out_of_memory",
+    PB_ERR_NUMERIC: "numeric",
+    PB_ERR_CANCELLED: "cancelled",
+    PB_ERR_UNSUPPORTED: "unsupported",
+}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+
+
+class ProblemView(C.Structure):
+    _fields_ = [
+        ("num_cameras", C.c_int32),
+        ("cam_rotation", _dp),
+        ("cam_center", _dp),
+        ("cam_intrinsics", _dp),
+        ("cam_convention", _ip),
+        ("cam_image_size", _ip),
+        ("ref_intrinsics", _dp),
+        ("cam_group", _ip),
+        ("num_tracks", C.c_int32),
+        ("track_point", _dp),
+        ("track_offset", _lp),
+        ("obs_camera", _ip),
+        ("obs_uv", _dp),
+        ("obs_weight", _dp),
+        ("obs_scale", _dp),
+        ("num_fixed_cameras", C.c_int32),
+        ("fixed_cameras", _ip),
+        ("num_fixed_tracks", C.c_int32),
+        ("fixed_tracks", _ip),
+        ("gauge_policy", C.c_int32),
+        ("gauge_anchor_camera", C.c_int32),
+        ("gauge_scale_camera", C.c_int32),
+        ("gauge_baseline", C.c_double),
+    ]
+
+
+class SolveOptionsC(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_int32),
+        ("robust_loss", C.c_int32),
+        ("robust_loss_scale", C.c_double),
+        ("armijo_coefficient", C.c_double),
+        ("line_search_steps", C.c_int32),
+        ("use_online_schur", C.c_int32),
+        ("max_initial_track_rms", C.c_double),
+        ("refine_camera_pose", C.c_int32),
+        ("shared_intrinsic_mask", C.c_uint32),
+        ("min_focal_scale", C.c_double),
+        ("max_focal_scale", C.c_double),
+        ("min_aspect_scale", C.c_double),
+        ("max_aspect_scale", C.c_double),
+        ("max_principal_offset_fraction", C.c_double),
+        ("focal_prior_sigma", C.c_double),
+        ("principal_prior_sigma_fraction", C.c_double),
+        ("aspect_prior_sigma", C.c_double),
+        ("max_abs", C.c_double * 9),
+        ("prior_sigma", C.c_double * 9),
+        ("low_order_distortion_scale", C.c_double),
+        ("device", C.c_int32),
+        ("precision", C.c_int32),
+        ("pcg_relative_tolerance", C.c_double),
+        ("pcg_absolute_tolerance", C.c_double),
+        ("pcg_max_iterations", C.c_int32),
+        ("pcg_check_interval", C.c_int32),
+        ("armijo_batch", C.c_int32),
+        ("profile_kernels", C.c_int32),
+    ]
+
+
+class SolveReportC(C.Structure):
+    _fields_ = [
+        ("solve_status", C.c_int32),
+        ("solution_usable", C.c_int32),
+        ("converged", C.c_int32),
+        ("iterations", C.c_int32),
+        ("initial_cost", C.c_double),
+        ("final_cost", C.c_double),
+        ("accepted_steps", C.c_int32),
+        ("rejected_steps", C.c_int32),
+        ("linearizations", C.c_int32),
+        ("objective_evaluations", C.c_int32),
+        ("linear_iterations", C.c_int32),
+        ("rejected_initial_tracks", C.c_int32),
+        ("observation_count", C.c_int32),
+        ("active_track_count", C.c_int32),
+        ("camera_block_count", C.c_int32),
+        ("intrinsic_block_count", C.c_int32),
+        ("track_block_count", C.c_int32),
+        ("primary_block_count", C.c_int32),
+        ("online_schur_used", C.c_int32),
+        ("mixed_precision_used", C.c_int32),
+        ("gauge_anchor_camera", C.c_int32),
+        ("gauge_scale_camera", C.c_int32),
+        ("gauge_baseline", C.c_double),
+        ("setup_seconds", C.c_double),
+        ("solve_seconds", C.c_double),
+        ("assembly_seconds", C.c_double),
+        ("linear_solve_seconds", C.c_double),
+        ("back_substitution_seconds", C.c_double),
+        ("objective_seconds", C.c_double),
+        ("linear_tolerance", C.c_double),
+        ("device_name", C.c_char * 64),
+        ("message", C.c_char * 256),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            value = getattr(self, name)
+            if isinstance(value, bytes):
+                value = value.decode("utf-8", "replace")
+            out[name] = value
+        return out
+
+
+# Exported symbols of the shared library, in header order.
+EXPORTED_SYMBOLS = (
+    "pb_default_options",
+    "pb_hip_available",
+    "pb_hip_create",
+    "pb_hip_destroy",
+    "pb_hip_upload",
+    "pb_hip_solve",
+    "pb_hip_download",
+    "pb_hip_get_masks",
+    "pb_hip_kernel_stats",
+    "pb_hip_reduced_system",
+    "pb_hip_evaluate_cost",
+    "pb_hip_last_error",
+)
+
+PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(PACKAGE_DIR, "libpb_hip.so")
+
+_lib = None
+
+
+class BackendLibraryMissing(RuntimeError):
+    """The HIP backend library has not been built (run __graft_entry__.build())."""
+
+
+def load_library(path: str | None = None):
+    """Loads libpb_hip.so from the package directory; fails loudly when it is absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    lib_path = path or LIBRARY_PATH
+    if not os.path.exists(lib_path):
+        raise BackendLibraryMissing(f"{lib_path} not found; build the HIP backend first")
+    lib = C.CDLL(lib_path, mode=C.RTLD_LOCAL)
+    lib.pb_default_options.argtypes = [C.POINTER(SolveOptionsC)]
+    lib.pb_default_options.restype = None
+    lib.pb_hip_available.argtypes = [C.c_int]
+    lib.pb_hip_available.restype = C.c_int
+    lib.pb_hip_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    lib.pb_hip_create.restype = C.c_int
+    lib.pb_hip_destroy.argtypes = [C.c_void_p]
+    lib.pb_hip_destroy.restype = C.c_int
+    lib.pb_hip_upload.argtypes = [C.c_void_p, C.POINTER(ProblemView)]
+    lib.pb_hip_upload.restype = C.c_int
+    lib.pb_hip_solve.argtypes = [C.c_void_p, C.POINTER(SolveOptionsC), C.POINTER(SolveReportC), _ip]
+    lib.pb_hip_solve.restype = C.c_int
+    lib.pb_hip_download.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _ip]
+    lib.pb_hip_download.restype = C.c_int
+    lib.pb_hip_get_masks.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip, _ip]
+    lib.pb_hip_get_masks.restype = C.c_int
+    lib.pb_hip_kernel_stats.argtypes = [C.c_void_p, C.c_char_p, _lp, _dp, _dp]
+    lib.pb_hip_kernel_stats.restype = C.c_int
+    lib.pb_hip_reduced_system.argtypes = [C.c_void_p, C.POINTER(SolveOptionsC), C.c_double, _dp, _dp, _dp, _ip]
+    lib.pb_hip_reduced_system.restype = C.c_int
+    lib.pb_hip_evaluate_cost.argtypes = [C.c_void_p, C.POINTER(SolveOptionsC), _dp]
+    lib.pb_hip_evaluate_cost.restype = C.c_int
+    lib.pb_hip_last_error.argtypes = [C.c_void_p]
+    lib.pb_hip_last_error.restype = C.c_char_p
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def default_options(lib=None) -> SolveOptionsC:
+    lib = lib or load_library()
+    opts = SolveOptionsC()
+    lib.pb_default_options(C.byref(opts))
+    return opts
+
+
+def ptr(array, ctype):
+    """Pointer to a contiguous numpy array (or NULL for None)."""
+    if array is None:
+        return C.cast(None, C.POINTER(ctype))
+    return array.ctypes.data_as(C.POINTER(ctype))
+
+
+def make_view(problem) -> tuple[ProblemView, list]:
+    """Builds a pb_problem_view over the arrays of a plabundle_amd.problem.Problem.
+
+    Returns the view and the list of arrays it points into (keep it alive)."""
+    keep = []
+
+    def arr(a, dtype):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return a
+
+    rot = arr(problem.cam_rotation, np.float64)
+    cen = arr(problem.cam_center, np.float64)
+    intr = arr(problem.cam_intrinsics, np.float64)
+    conv = arr(problem.cam_convention, np.int32)
+    size = arr(problem.cam_image_size, np.int32)
+    ref = arr(problem.ref_intrinsics, np.float64)
+    grp = arr(problem.cam_group, np.int32)
+    pts = arr(problem.track_point, np.float64)
+    off = arr(problem.track_offset, np.int64)
+    ocam = arr(problem.obs_camera, np.int32)
+    ouv = arr(problem.obs_uv, np.float64)
+    ow = arr(problem.obs_weight, np.float64)
+    osc = arr(problem.obs_scale, np.float64)
+    fc = arr(problem.fixed_cameras, np.int32)
+    ft = arr(problem.fixed_tracks, np.int32)
+    v = ProblemView()
+    v.num_cameras = int(rot.shape[0])
+    v.cam_rotation = ptr(rot, C.c_double)
+    v.cam_center = ptr(cen, C.c_double)
+    v.cam_intrinsics = ptr(intr, C.c_double)
+    v.cam_convention = ptr(conv, C.c_int32)
+    v.cam_image_size = ptr(size, C.c_int32)
+    v.ref_intrinsics = ptr(ref, C.c_double)
+    v.cam_group = ptr(grp, C.c_int32)
+    v.num_tracks = int(pts.shape[0])
+    v.track_point = ptr(pts, C.c_double)
+    v.track_offset = ptr(off, C.c_int64)
+    v.obs_camera = ptr(ocam, C.c_int32)
+    v.obs_uv = ptr(ouv, C.c_double)
+    v.obs_weight = ptr(ow, C.c_double)
+    v.obs_scale = ptr(osc, C.c_double)
+    v.num_fixed_cameras = 0 if fc is None else int(fc.size)
+    v.fixed_cameras = ptr(fc, C.c_int32)
+    v.num_fixed_tracks = 0 if ft is None else int(ft.size)
+    v.fixed_tracks = ptr(ft, C.c_int32)
+    v.gauge_policy = int(problem.gauge_policy)
+    v.gauge_anchor_camera = int(problem.gauge_anchor_camera)
+    v.gauge_scale_camera = int(problem.gauge_scale_camera)
+    v.gauge_baseline = float(problem.gauge_baseline)
+    return v, keep

@@ -1,0 +1,202 @@
+"""Solve options of the MI355X backend, mirroring plabundle::SolveOptions.
+
+Field names follow include/plabundle/options.h:58-186 so that code written
+against PlaBundle's structured options reads the same here. Only the fields the
+frame-BA path consumes are carried across the C ABI (pb_solve_options).
+"""
+from __future__ import annotations
+
+import enum
+from dataclasses import dataclass, field
+
+from . import _abi
+
+
+class Backend(enum.IntEnum):
+    """plabundle::Backend (include/plabundle/backend.h:8-15) plus the MI355X backend."""
+
+    Auto = 0
+    PlaMatrixCpu = 1
+    PlaMatrixCuda = 2
+    PlaMatrixVulkan = 3
+    PlaMatrixOpenCl = 4
+    Mi355x = 5
+
+
+class ImageRobustLoss(enum.IntEnum):
+    LeastSquares = 0
+    Huber = 1
+    Cauchy = 2
+
+
+class IntrinsicParameter(enum.IntEnum):
+    """Slot order of the 13-entry parameter block (options.h:24-45)."""
+
+    FocalLength = 0
+    FocalAspectRatio = 1
+    PrincipalPointX = 2
+    PrincipalPointY = 3
+    RadialK1 = 4
+    RadialK2 = 5
+    RadialK3 = 6
+    TangentialP1 = 7
+    TangentialP2 = 8
+    SkewB2 = 9
+    RadialK4 = 10
+    TangentialP3 = 11
+    TangentialP4 = 12
+
+
+class Precision(enum.IntEnum):
+    Fp64 = 0
+    Mixed = 1
+
+
+@dataclass
+class SolverControlOptions:
+    maxIterations: int = 20
+    imageRobustLoss: ImageRobustLoss = ImageRobustLoss.LeastSquares
+    imageRobustLossScalePixels: float = 1.0
+    referenceArmijoCoefficient: float = 1.0e-3
+    referenceLineSearchSteps: int = 34
+    useReferenceOnlineSchur: bool = True
+    enablePointFilter: bool = True
+    maxInitialTrackRms: float = 100.0
+    numThreads: int = 0
+    logIterationProgress: bool = True
+
+
+@dataclass
+class CalibrationOptions:
+    refineCameraPose: bool = True
+    refineSharedFocalLength: bool = False
+    refineSharedFocalAspectRatio: bool = False
+    refineSharedPrincipalPoint: bool = False
+    refineSharedModelCoefficients: bool = False
+    refineSharedRadialDistortion: bool = False
+    refineSharedHighOrderDistortion: bool = True
+    refineSharedMetashapeParameters: bool = False
+    useSharedIntrinsicParameterMask: bool = False
+    sharedIntrinsicParameterMask: list = field(default_factory=lambda: [True] * 13)
+    minSharedFocalScale: float = 0.5
+    maxSharedFocalScale: float = 4.0
+    minSharedFocalAspectScale: float = 0.85
+    maxSharedFocalAspectScale: float = 1.18
+    maxSharedPrincipalPointOffsetFraction: float = 0.08
+    sharedFocalPriorSigma: float = 0.35
+    sharedPrincipalPointPriorSigmaFraction: float = 0.04
+    sharedFocalAspectPriorSigma: float = 0.08
+    maxSharedRadialK1Abs: float = 0.35
+    maxSharedRadialK2Abs: float = 0.35
+    maxSharedRadialK3Abs: float = 0.35
+    maxSharedTangentialP1Abs: float = 0.02
+    maxSharedTangentialP2Abs: float = 0.02
+    maxSharedSkewFraction: float = 0.10
+    maxSharedRadialK4Abs: float = 0.35
+    maxSharedTangentialP3Abs: float = 0.35
+    maxSharedTangentialP4Abs: float = 0.35
+    sharedRadialK1PriorSigma: float = 0.15
+    sharedRadialK2PriorSigma: float = 0.15
+    sharedRadialK3PriorSigma: float = 0.15
+    sharedTangentialP1PriorSigma: float = 0.01
+    sharedTangentialP2PriorSigma: float = 0.01
+    sharedSkewPriorSigmaFraction: float = 0.02
+    sharedRadialK4PriorSigma: float = 0.15
+    sharedTangentialP3PriorSigma: float = 0.15
+    sharedTangentialP4PriorSigma: float = 0.15
+    sharedLowOrderDistortionScale: float = 1.0
+
+
+@dataclass
+class BackendOptions:
+    requested: Backend = Backend.Mi355x
+    plaMatrixDevice: int = 0
+    enablePlaMatrixMixedPrecision: bool = False
+    allowFallback: bool = False
+    # MI355X controls (no reference counterpart)
+    pcgRelativeTolerance: float = 0.0
+    pcgAbsoluteTolerance: float = 0.0
+    pcgMaxIterations: int = 0
+    pcgCheckInterval: int = 0
+    armijoBatch: int = 0
+    profileKernels: bool = False
+
+
+@dataclass
+class SolveOptions:
+    solver: SolverControlOptions = field(default_factory=SolverControlOptions)
+    calibration: CalibrationOptions = field(default_factory=CalibrationOptions)
+    backend: BackendOptions = field(default_factory=BackendOptions)
+
+
+def shared_intrinsic_parameter_enabled(calibration: CalibrationOptions, parameter: int) -> bool:
+    """sharedIntrinsicParameterEnabled (src/backend.cpp:8-47)."""
+    c = calibration
+    p = IntrinsicParameter(parameter)
+    radial = c.refineSharedModelCoefficients or c.refineSharedRadialDistortion
+    if p == IntrinsicParameter.FocalLength:
+        enabled = c.refineSharedFocalLength
+    elif p == IntrinsicParameter.FocalAspectRatio:
+        enabled = c.refineSharedFocalAspectRatio
+    elif p in (IntrinsicParameter.PrincipalPointX, IntrinsicParameter.PrincipalPointY):
+        enabled = c.refineSharedPrincipalPoint
+    elif p == IntrinsicParameter.RadialK1:
+        enabled = radial
+    elif p in (IntrinsicParameter.RadialK2, IntrinsicParameter.RadialK3,
+               IntrinsicParameter.TangentialP1, IntrinsicParameter.TangentialP2):
+        enabled = radial and c.refineSharedHighOrderDistortion
+    else:
+        enabled = c.refineSharedMetashapeParameters
+    return bool(enabled and (not c.useSharedIntrinsicParameterMask or c.sharedIntrinsicParameterMask[int(p)]))
+
+
+def shared_intrinsic_mask(calibration: CalibrationOptions) -> int:
+    mask = 0
+    for i in range(13):
+        if shared_intrinsic_parameter_enabled(calibration, i):
+            mask |= 1 << i
+    return mask
+
+
+def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.SolveOptionsC:
+    """Flattens SolveOptions into pb_solve_options (the makeCompatibilityOptions step)."""
+    s, c, b = options.solver, options.calibration, options.backend
+    o = _abi.SolveOptionsC()
+    o.max_iterations = int(s.maxIterations)
+    o.robust_loss = int(s.imageRobustLoss)
+    o.robust_loss_scale = float(s.imageRobustLossScalePixels)
+    o.armijo_coefficient = float(s.referenceArmijoCoefficient)
+    o.line_search_steps = int(s.referenceLineSearchSteps)
+    o.use_online_schur = 1 if s.useReferenceOnlineSchur else 0
+    o.max_initial_track_rms = float(s.maxInitialTrackRms)
+    o.refine_camera_pose = 1 if c.refineCameraPose else 0
+    o.shared_intrinsic_mask = shared_intrinsic_mask(c)
+    o.min_focal_scale = c.minSharedFocalScale
+    o.max_focal_scale = c.maxSharedFocalScale
+    o.min_aspect_scale = c.minSharedFocalAspectScale
+    o.max_aspect_scale = c.maxSharedFocalAspectScale
+    o.max_principal_offset_fraction = c.maxSharedPrincipalPointOffsetFraction
+    o.focal_prior_sigma = c.sharedFocalPriorSigma
+    o.principal_prior_sigma_fraction = c.sharedPrincipalPointPriorSigmaFraction
+    o.aspect_prior_sigma = c.sharedFocalAspectPriorSigma
+    for i, v in enumerate((c.maxSharedRadialK1Abs, c.maxSharedRadialK2Abs, c.maxSharedRadialK3Abs,
+                           c.maxSharedTangentialP1Abs, c.maxSharedTangentialP2Abs, c.maxSharedSkewFraction,
+                           c.maxSharedRadialK4Abs, c.maxSharedTangentialP3Abs, c.maxSharedTangentialP4Abs)):
+        o.max_abs[i] = v
+    for i, v in enumerate((c.sharedRadialK1PriorSigma, c.sharedRadialK2PriorSigma, c.sharedRadialK3PriorSigma,
+                           c.sharedTangentialP1PriorSigma, c.sharedTangentialP2PriorSigma,
+                           c.sharedSkewPriorSigmaFraction, c.sharedRadialK4PriorSigma,
+                           c.sharedTangentialP3PriorSigma, c.sharedTangentialP4PriorSigma)):
+        o.prior_sigma[i] = v
+    o.low_order_distortion_scale = c.sharedLowOrderDistortionScale
+    o.device = int(b.plaMatrixDevice)
+    if precision is None:
+        precision = Precision.Mixed if b.enablePlaMatrixMixedPrecision else Precision.Fp64
+    o.precision = int(precision)
+    o.pcg_relative_tolerance = float(b.pcgRelativeTolerance)
+    o.pcg_absolute_tolerance = float(b.pcgAbsoluteTolerance)
+    o.pcg_max_iterations = int(b.pcgMaxIterations)
+    o.pcg_check_interval = int(b.pcgCheckInterval)
+    o.armijo_batch = int(b.armijoBatch)
+    o.profile_kernels = 1 if b.profileKernels else 0
+    return o

@@ -1,0 +1,325 @@
+"""Seeded synthetic frame-BA problems.
+
+Workloads (sizes from the north-star configurations):
+  * ``native_three_cameras`` - the reference's own online-Schur test scene
+    (test/test_native_problem.cpp:51-94): three unit cameras, 24 tracks.
+  * ``ring_problem`` - 50 pinhole cameras on a 10 m circle looking inward,
+    4000x3000 px, f = 3000 px, one shared Brown group, 2000 points in a 4 m
+    cube, <= 8 views per point, 0.5 px noise, Huber 1 px (seed 1).
+  * ``aerial_problem`` - nadir aerial block: cameras in 3 strips at 300 m,
+    80 % forward / 60 % side overlap, 4000x3000 px, f = 3000 px, 4 shared
+    intrinsic groups, points on an fBm heightfield (+-20 m), geometric track
+    lengths (min 2, mean 7.5 after the visibility cap), 0.5 px noise, 1 %
+    outliers, Cauchy 1 px (seed 2; 1000 cameras / 200k points / ~1.5M obs at
+    full size).
+
+Random numbers come from numpy's PCG64 generator with the stated seed; the
+same numpy build (this image) reproduces every problem bit for bit.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .problem import GaugePolicy, Problem, intrinsic_block
+
+
+# --- small geometry helpers (numpy, float64) -------------------------------------------------
+def exp_so3(w: np.ndarray) -> np.ndarray:
+    """Rodrigues rotation matrices for axis-angle rows w (..., 3) -> (..., 3, 3)."""
+    w = np.asarray(w, dtype=np.float64)
+    th = np.linalg.norm(w, axis=-1)[..., None, None]
+    K = np.zeros(w.shape[:-1] + (3, 3))
+    K[..., 0, 1] = -w[..., 2]
+    K[..., 0, 2] = w[..., 1]
+    K[..., 1, 0] = w[..., 2]
+    K[..., 1, 2] = -w[..., 0]
+    K[..., 2, 0] = -w[..., 1]
+    K[..., 2, 1] = w[..., 0]
+    small = th < 1e-10
+    th_safe = np.where(small, 1.0, th)
+    A = np.where(small, 1.0, np.sin(th_safe) / th_safe)
+    B = np.where(small, 0.5, (1.0 - np.cos(th_safe)) / th_safe**2)
+    eye = np.broadcast_to(np.eye(3), K.shape)
+    return eye + A * K + B * (K @ K)
+
+
+def project_points(R: np.ndarray, C: np.ndarray, intr: np.ndarray, X: np.ndarray, metashape=None):
+    """Projects world points X (n,3) through cameras (R (n,3,3) c2w, C (n,3), intr (n,13)).
+
+    Returns (uv (n,2), depth (n,)). Same frame-camera model as the C ABI documents."""
+    d = X - C
+    xc = np.einsum("nij,ni->nj", R, d)
+    z = xc[:, 2]
+    x = xc[:, 0] / z
+    y = xc[:, 1] / z
+    fx = intr[:, 0]
+    fy = fx * np.exp(intr[:, 1])
+    k1, k2, k3, p1, p2, b2, k4, p3, p4 = (intr[:, i] for i in range(4, 13))
+    r2 = x * x + y * y
+    rad = 1 + r2 * (k1 + r2 * (k2 + r2 * (k3 + r2 * k4)))
+    if metashape is None:
+        metashape = np.zeros(len(x), dtype=bool)
+    tx_o = 2 * p1 * x * y + p2 * (r2 + 2 * x * x)
+    ty_o = p1 * (r2 + 2 * y * y) + 2 * p2 * x * y
+    T = 1 + p3 * r2 + p4 * r2 * r2
+    tx_m = (p1 * (r2 + 2 * x * x) + 2 * p2 * x * y) * T
+    ty_m = (p2 * (r2 + 2 * y * y) + 2 * p1 * x * y) * T
+    tx = np.where(metashape, tx_m, tx_o)
+    ty = np.where(metashape, ty_m, ty_o)
+    xd = x * rad + tx
+    yd = y * rad + ty
+    u = fx * xd + b2 * yd + intr[:, 2]
+    v = fy * yd + intr[:, 3]
+    return np.stack([u, v], axis=1), z
+
+
+def look_at_rotation(center, target, up=(0.0, 0.0, 1.0)) -> np.ndarray:
+    """Camera-to-world rotation whose columns are the camera right/down/forward axes."""
+    f = np.asarray(target, float) - np.asarray(center, float)
+    f /= np.linalg.norm(f)
+    r = np.cross(f, up)
+    r /= np.linalg.norm(r)
+    dn = np.cross(f, r)
+    return np.stack([r, dn, f], axis=1)
+
+
+def _finish(rot, cen, intr, pts, track_obs, **kwargs) -> Problem:
+    off = np.zeros(len(track_obs) + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(t[0]) for t in track_obs])
+    cam = np.concatenate([t[0] for t in track_obs]).astype(np.int32)
+    uv = np.concatenate([t[1] for t in track_obs]).reshape(-1, 2)
+    return Problem(
+        cam_rotation=np.ascontiguousarray(rot.reshape(-1, 9)),
+        cam_center=np.ascontiguousarray(cen),
+        cam_intrinsics=np.ascontiguousarray(intr),
+        track_point=np.ascontiguousarray(pts),
+        track_offset=off,
+        obs_camera=cam,
+        obs_uv=np.ascontiguousarray(uv),
+        obs_weight=np.ones(len(cam)),
+        obs_scale=np.ones(len(cam)),
+        **kwargs,
+    )
+
+
+# --- the reference's own online-Schur test scene ---------------------------------------------
+def native_three_cameras() -> Problem:
+    """test/test_native_problem.cpp:51-94: three f=100 cameras at x=0,1,2, camera 2 perturbed."""
+    intr = intrinsic_block(100.0, 100.0, 50.0, 50.0)
+    eye = np.eye(3)
+    truth_R = np.stack([eye, eye, eye])
+    truth_C = np.array([[0.0, 0, 0], [1.0, 0, 0], [2.0, 0, 0]])
+    intrs = np.stack([intr] * 3)
+    R = truth_R.copy()
+    C = truth_C.copy()
+    delta = np.array([0.001, -0.001, 0.0005, 0.02, -0.01, 0.005])
+    R[2] = exp_so3(delta[:3]) @ R[2]
+    C[2] = C[2] + delta[3:]
+    tracks = []
+    pts = []
+    for index in range(24):
+        truth = np.array([0.5 * (index % 4) - 0.75, 0.3 * ((index // 4) % 3) - 0.3, 5.0 + 0.2 * (index % 5)])
+        pts.append(truth + np.array([0.03, -0.02, 0.04]))
+        uv, _ = project_points(truth_R, truth_C, intrs, np.repeat(truth[None], 3, axis=0))
+        tracks.append((np.arange(3), uv))
+    return _finish(R, C, intrs, np.array(pts), tracks, fixed_cameras=np.array([0, 1], dtype=np.int32),
+                   gauge_policy=GaugePolicy.RequireExplicitGauge,
+                   meta={"workload": "native_three_cameras", "seed": None})
+
+
+# --- ring of cameras around a cube (configuration 1) ------------------------------------------
+def ring_problem(seed: int = 1, num_cameras: int = 50, num_points: int = 2000, max_views: int = 8,
+                 noise_px: float = 0.5, rot_sigma: float = 0.01, center_sigma: float = 0.05,
+                 point_sigma: float = 0.1) -> Problem:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    W, H, f = 4000, 3000, 3000.0
+    ang = 2 * np.pi * np.arange(num_cameras) / num_cameras
+    cen = np.stack([10 * np.cos(ang), 10 * np.sin(ang), np.zeros(num_cameras)], axis=1)
+    rot = np.stack([look_at_rotation(c, (0, 0, 0)) for c in cen])
+    dist = rng.normal(0.0, 1e-2, size=4)  # one shared Brown group: k1, k2, p1, p2 ~ N(0, sigma=1e-2)
+    intr1 = intrinsic_block(f, f, W / 2, H / 2, k1=dist[0], k2=dist[1], p1=dist[2], p2=dist[3])
+    intr = np.repeat(intr1[None], num_cameras, axis=0)
+    truth = rng.uniform(-2.0, 2.0, size=(num_points, 3))
+    tracks, pts = [], []
+    for p in range(num_points):
+        X = np.repeat(truth[p][None], num_cameras, axis=0)
+        uv, z = project_points(rot, cen, intr, X)
+        vis = np.nonzero((z > 0) & (uv[:, 0] >= 0) & (uv[:, 0] < W) & (uv[:, 1] >= 0) & (uv[:, 1] < H))[0]
+        vis = rng.permutation(vis)[:max_views]
+        if len(vis) < 2:
+            continue
+        vis = np.sort(vis)
+        obs = uv[vis] + rng.normal(0.0, noise_px, size=(len(vis), 2))
+        tracks.append((vis, obs))
+        pts.append(truth[p] + rng.normal(0.0, point_sigma, size=3))
+    # perturb every camera but the two gauge cameras 0 and 1 (as the reference benchmark does)
+    dR = exp_so3(rng.normal(0.0, rot_sigma, size=(num_cameras, 3)))
+    dC = rng.normal(0.0, center_sigma, size=(num_cameras, 3))
+    dR[:2] = np.eye(3)
+    dC[:2] = 0.0
+    rot0 = dR @ rot
+    cen0 = cen + dC
+    return _finish(rot0, cen0, intr, np.array(pts), tracks,
+                   cam_image_size=np.tile(np.array([W, H], dtype=np.int32), (num_cameras, 1)),
+                   cam_group=np.zeros(num_cameras, dtype=np.int32),
+                   fixed_cameras=np.array([0, 1], dtype=np.int32),
+                   gauge_policy=GaugePolicy.RequireExplicitGauge,
+                   meta={"workload": "ring", "seed": seed, "truth_rotation": rot, "truth_center": cen,
+                         "truth_points": truth})
+
+
+# --- aerial strip block (configuration 2) ------------------------------------------------------
+def _fbm_height(x, y, rng_params):
+    h = np.zeros_like(x)
+    for amp, kx, ky, ph in rng_params:
+        h += amp * np.sin(kx * x + ky * y + ph)
+    return h
+
+
+def aerial_problem(seed: int = 2, cameras_per_strip=(334, 333, 333), num_points: int = 200_000,
+                   mean_track_length: float = 7.5, outlier_fraction: float = 0.01, noise_px: float = 0.5,
+                   num_groups: int = 4, rot_sigma: float = 0.002, center_sigma: float = 0.2,
+                   point_sigma: float = 0.2, focal_sigma: float = 0.002, distortion_sigma: float = 0.01,
+                   altitude: float = 300.0) -> Problem:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    W, H, f = 4000, 3000, 3000.0
+    along = altitude * H / f          # 300 m footprint along track (image v axis)
+    across = altitude * W / f         # 400 m footprint across track (image u axis)
+    base = 0.2 * along                # 80 % forward overlap
+    spacing = 0.4 * across            # 60 % side overlap
+    strips = len(cameras_per_strip)
+    n_cam = int(sum(cameras_per_strip))
+    # cameras: all strips fly +x; camera right axis = +y world, down = +x world, forward = -z
+    nadir = np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, -1.0]])
+    cen, strip_of = [], []
+    for s, n in enumerate(cameras_per_strip):
+        for k in range(n):
+            cen.append([k * base, s * spacing, altitude])
+            strip_of.append(s)
+    cen = np.array(cen)
+    cen += rng.normal(0.0, 1.0, size=cen.shape) * np.array([2.0, 2.0, 1.0])  # flight irregularity
+    att = rng.normal(0.0, 0.02, size=(n_cam, 3))                            # attitude irregularity
+    rot = exp_so3(att) @ nadir
+    group = (np.arange(n_cam) * num_groups) // n_cam
+    # per-group truth intrinsics
+    g_f = f * (1.0 + rng.normal(0.0, focal_sigma, size=num_groups))
+    g_d = rng.normal(0.0, distortion_sigma, size=(num_groups, 2))           # k1, k2
+    g_t = rng.normal(0.0, 1e-4, size=(num_groups, 2))                       # p1, p2
+    intr_true = np.stack([intrinsic_block(g_f[g], g_f[g], W / 2, H / 2, k1=g_d[g, 0], k2=g_d[g, 1],
+                                          p1=g_t[g, 0], p2=g_t[g, 1]) for g in group])
+    intr_init = np.repeat(intrinsic_block(f, f, W / 2, H / 2)[None], n_cam, axis=0)
+    # terrain
+    octaves = []
+    for o in range(6):
+        amp = 20.0 * 0.55**o
+        k = 2 * np.pi / (2000.0 * 0.5**o)
+        th = rng.uniform(0, 2 * np.pi)
+        octaves.append((amp, k * np.cos(th), k * np.sin(th), rng.uniform(0, 2 * np.pi)))
+    norm = sum(a for a, *_ in octaves) / 20.0
+    octaves = [(a / norm, kx, ky, ph) for a, kx, ky, ph in octaves]
+    x_lo, x_hi = cen[:, 0].min() - along / 2, cen[:, 0].max() + along / 2
+    y_lo, y_hi = -across / 2, (strips - 1) * spacing + across / 2
+    # Track length L = min(2 + G, visible views) with G geometric; q is solved (once, on the first batch)
+    # so that the expected L over the batch equals the target mean.
+    q = None
+    half = int(np.ceil(along / base)) + 2
+    win = 2 * half + 1
+    strip_start = np.concatenate([[0], np.cumsum(cameras_per_strip)])
+    pts_true, cams_l, uv_l = [], [], []
+    have = 0
+    while have < num_points:
+        m = int((num_points - have) * 1.2) + 64
+        px = rng.uniform(x_lo, x_hi, size=m)
+        py = rng.uniform(y_lo, y_hi, size=m)
+        pz = _fbm_height(px, py, octaves)
+        X = np.stack([px, py, pz], axis=1)
+        # candidate cameras: a window of along-track indices in every strip
+        kc = np.floor(px / base).astype(np.int64)
+        cand = np.full((m, strips * win), -1, dtype=np.int64)
+        for s in range(strips):
+            k = kc[:, None] + np.arange(-half, half + 1)[None, :]
+            valid = (k >= 0) & (k < cameras_per_strip[s])
+            cand[:, s * win:(s + 1) * win] = np.where(valid, strip_start[s] + k, -1)
+        flat = cand.reshape(-1)
+        ok = flat >= 0
+        cidx = np.where(ok, flat, 0)
+        Xr = np.repeat(X, strips * win, axis=0)
+        uv, z = project_points(rot[cidx], cen[cidx], intr_true[cidx], Xr)
+        vis = ok & (z > 0) & (uv[:, 0] >= 0) & (uv[:, 0] < W) & (uv[:, 1] >= 0) & (uv[:, 1] < H)
+        vis = vis.reshape(m, -1)
+        uv = uv.reshape(m, -1, 2)
+        nvis = vis.sum(axis=1)
+        if q is None:
+            n2 = nvis[nvis >= 2].astype(np.float64)
+
+            def expected(qq):
+                return float(np.mean(2.0 + qq * (1.0 - qq ** (n2 - 2.0)) / (1.0 - qq)))
+
+            lo_q, hi_q = 1e-6, 1.0 - 1e-9
+            for _ in range(200):
+                mid = 0.5 * (lo_q + hi_q)
+                lo_q, hi_q = (mid, hi_q) if expected(mid) < mean_track_length else (lo_q, mid)
+            q = 0.5 * (lo_q + hi_q)
+        extra = rng.geometric(1.0 - q, size=m) - 1
+        keys = np.where(vis, rng.random(size=vis.shape), 2.0)
+        order = np.argsort(keys, axis=1)
+        L = np.minimum(2 + extra, nvis)
+        noise = rng.normal(0.0, noise_px, size=uv.shape)
+        outl = rng.random(size=vis.shape) < outlier_fraction
+        outl_uv = rng.uniform(0, 1, size=uv.shape) * np.array([W, H])
+        for i in np.nonzero(nvis >= 2)[0]:
+            if have >= num_points:
+                break
+            pick = order[i, : L[i]]
+            pick = pick[np.argsort(cand[i, pick])]
+            obs = uv[i, pick] + noise[i, pick]
+            o_mask = outl[i, pick]
+            obs[o_mask] = outl_uv[i, pick][o_mask]
+            pts_true.append(X[i])
+            cams_l.append(cand[i, pick])
+            uv_l.append(obs)
+            have += 1
+    tracks = list(zip(cams_l, uv_l))
+    pts_true = np.array(pts_true)
+    dR = exp_so3(rng.normal(0.0, rot_sigma, size=(n_cam, 3)))
+    dC = rng.normal(0.0, center_sigma, size=(n_cam, 3))
+    dR[:2] = np.eye(3)
+    dC[:2] = 0.0
+    rot0 = dR @ rot
+    cen0 = cen + dC
+    pts0 = pts_true + rng.normal(0.0, point_sigma, size=pts_true.shape)
+    return _finish(rot0, cen0, intr_init, pts0, tracks,
+                   cam_image_size=np.tile(np.array([W, H], dtype=np.int32), (n_cam, 1)),
+                   cam_group=group.astype(np.int32),
+                   fixed_cameras=np.array([0, 1], dtype=np.int32),
+                   gauge_policy=GaugePolicy.RequireExplicitGauge,
+                   meta={"workload": "aerial", "seed": seed, "geometric_q": q, "truth_rotation": rot, "truth_center": cen,
+                         "truth_points": pts_true, "truth_intrinsics": intr_true})
+
+
+def aerial_options(options=None):
+    """Solve options of the aerial workload: Cauchy 1 px, shared focal + Brown radial/tangential refined."""
+    from .options import ImageRobustLoss, SolveOptions
+
+    o = options or SolveOptions()
+    o.solver.imageRobustLoss = ImageRobustLoss.Cauchy
+    o.solver.imageRobustLossScalePixels = 1.0
+    o.solver.useReferenceOnlineSchur = True
+    o.calibration.refineSharedFocalLength = True
+    o.calibration.refineSharedRadialDistortion = True
+    o.calibration.refineSharedHighOrderDistortion = True
+    return o
+
+
+def ring_options(options=None, max_iterations: int = 10):
+    """Solve options of the ring workload: Huber 1 px, 10 iterations, intrinsics fixed."""
+    from .options import ImageRobustLoss, SolveOptions
+
+    o = options or SolveOptions()
+    o.solver.imageRobustLoss = ImageRobustLoss.Huber
+    o.solver.imageRobustLossScalePixels = 1.0
+    o.solver.maxIterations = max_iterations
+    o.solver.useReferenceOnlineSchur = True
+    return o
