@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark: time-to-converge and LM iter/s of the MI355X frame-BA backend.
+
+Workload: aerial block, 1000 cameras in 3 strips at 300 m, 200k points on an fBm
+terrain, ~1.5M observations (seed 2), Cauchy 1 px, shared focal + Brown
+distortion refined over 4 groups, 20 LM iterations (the reference default), online
+point Schur. A "step" is one complete solve from the initial state with the inputs
+resident in HBM (the problem structure is uploaded once, before the timed region).
+
+Reported: the mixed-precision path (FP32 Jacobians/Schur/PCG, FP64 cost and
+acceptance) as `value` (LM iter/s) and `ms_per_step` (time-to-converge), the plain
+FP64 GPU path beside it, the relative final-cost error against the FP64 CPU oracle
+(golden result in tests/golden/aerial_oracle.json), the HBM roofline of the dominant
+kernel (HIP events inside the library, on its stream) and the CPU oracle timed on a
+bounded sample of the same workload.
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "time-to-converge + LM iter/s, 1.5M obs, mixed vs FP64 GPU at <=1e-6 rel cost"
+
+
+def fingerprint(p) -> str:
+    import numpy as np
+
+    h = hashlib.sha256()
+    for a in (p.cam_rotation, p.cam_center, p.cam_intrinsics, p.track_point, p.track_offset, p.obs_camera, p.obs_uv):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--cpu-iterations", type=int, default=2, help="LM iterations of the CPU oracle sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--small", action="store_true", help="reduced problem for quick checks (not a bench line)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this backend runs on one MI355X; "
+              "the multi-GPU sharding plan is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    from plabundle_amd import options as PO
+    from plabundle_amd import synthetic as S
+    from plabundle_amd.solver import Solver, SolverWorkspace
+
+    t = time.time()
+    if args.small:
+        problem = S.aerial_problem(cameras_per_strip=(60, 60, 60), num_points=30000)
+    else:
+        problem = S.aerial_problem()
+    log(f"generated {problem.num_cameras} cameras, {problem.num_tracks} tracks, {problem.num_observations} "
+        f"observations in {time.time() - t:.1f}s")
+    ws = SolverWorkspace(0)
+    solver = Solver()
+
+    def solve(precision, profile):
+        o = S.aerial_options(PO.SolveOptions())
+        o.backend.profileKernels = profile
+        t0 = time.perf_counter()
+        r = solver.solve(problem, o, ws, precision=precision)
+        wall = time.perf_counter() - t0
+        if not r.usable():
+            raise RuntimeError(f"solve failed: {r.status.name} {r.backendMessage}")
+        return r, wall
+
+    for i in range(args.warmup):
+        r, wall = solve(PO.Precision.Mixed, False)
+        log(f"warmup {i}: mixed {r.report['iterations']} it, solve {r.report['solve_seconds']:.4f}s "
+            f"(setup {r.report['setup_seconds']:.2f}s), cost {r.finalCost:.10g}")
+
+    mixed = []
+    for i in range(args.steps):
+        r, wall = solve(PO.Precision.Mixed, True)
+        mixed.append(r)
+        log(f"step {i}: mixed {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
+            f"solve {r.report['solve_seconds']:.4f}s, cost {r.finalCost:.10g}")
+    stats = {}
+    for fam in ("linearize_points", "camera_direct", "schur_blocks", "pcg_spmv", "cost", "back_substitution"):
+        stats[fam] = ws.kernel_stats(fam)
+
+    fp64 = []
+    for i in range(args.fp64_steps):
+        r, wall = solve(PO.Precision.Fp64, False)
+        fp64.append(r)
+        log(f"fp64 {i}: {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
+            f"solve {r.report['solve_seconds']:.4f}s (setup {r.report['setup_seconds']:.2f}s), cost {r.finalCost:.10g}")
+
+    mixed_t = sum(r.report["solve_seconds"] for r in mixed) / len(mixed)
+    mixed_it = sum(r.report["iterations"] for r in mixed) / len(mixed)
+    fp64_t = sum(r.report["solve_seconds"] for r in fp64) / len(fp64) if fp64 else None
+    fp64_it = sum(r.report["iterations"] for r in fp64) / len(fp64) if fp64 else None
+
+    # dominant kernel family by total time in the timed region
+    dom = max(stats, key=lambda k: stats[k]["total_ms"])
+    s = stats[dom]
+    avg_ms = s["total_ms"] / max(1, s["launches"])
+    achieved = s["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "avg_launch_ms": round(avg_ms, 6), "launches": s["launches"],
+                "bytes_per_launch": s["bytes_per_launch"],
+                "families": {k: {"launches": v["launches"], "avg_ms": round(v["total_ms"] / max(1, v["launches"]), 6),
+                                 "GBps": round(v["bytes_per_launch"] / (v["total_ms"] / max(1, v["launches"]) * 1e-3)
+                                               / 1e9, 1) if v["total_ms"] > 0 else 0.0}
+                             for k, v in stats.items()}}
+
+    golden = None
+    gpath = os.path.join(ROOT, "tests", "golden", "aerial_oracle.json")
+    if not args.small and os.path.exists(gpath):
+        golden = json.load(open(gpath))
+        if golden.get("fingerprint") != fingerprint(problem):
+            log("golden fingerprint mismatch: the generated problem differs from the golden one")
+            golden = None
+    rel_oracle = abs(mixed[0].finalCost - golden["final_cost"]) / golden["final_cost"] if golden else None
+    rel_oracle64 = abs(fp64[0].finalCost - golden["final_cost"]) / golden["final_cost"] if golden and fp64 else None
+    rel_fp64 = abs(mixed[0].finalCost - fp64[0].finalCost) / fp64[0].finalCost if fp64 else None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        o = PO.to_c(S.aerial_options(PO.SolveOptions()))
+        o.max_iterations = args.cpu_iterations
+        threads = O.lib().oracle_max_threads()
+        orc = O.Oracle(problem, threads=threads)
+        t0 = time.perf_counter()
+        rep = orc.solve(o)
+        wall = time.perf_counter() - t0
+        cpu = {"value": round(rep["iterations"] / rep["solve_seconds"], 4), "unit": "LM iter/s", "cores": threads,
+               "kind": "port",
+               "sample": f"FP64 oracle, first {rep['iterations']} LM iterations of the same workload "
+                         f"({rep['solve_seconds']:.2f}s solve, {wall:.2f}s with setup)"}
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(mixed_it / mixed_t, 4),
+        "unit": "LM iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed_t * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (Jacobians/Schur/PCG) + fp64 (cost, acceptance, point inverses)",
+        "data": "synthetic",
+        "config": {
+            "workload": ("aerial block (reduced check size)" if args.small else
+                         "aerial block: 1000 cameras in 3 strips at 300 m, 200k points, 1.5M observations, seed 2"),
+            "observations": problem.num_observations,
+            "active_observations": mixed[0].report["observation_count"],
+            "cameras": problem.num_cameras,
+            "points": problem.num_tracks,
+            "loss": "Cauchy 1 px",
+            "refined_intrinsics": "shared focal + k1,k2,k3,p1,p2 over 4 groups",
+            "max_iterations": 20,
+            "path": "online point Schur, explicit 6x6-block reduced system, block-Jacobi PCG",
+        },
+        "time_to_converge_s": round(mixed_t, 6),
+        "lm_iterations": mixed_it,
+        "pcg_iterations": mixed[0].report["linear_iterations"],
+        "fp64_time_to_converge_s": round(fp64_t, 6) if fp64_t else None,
+        "fp64_lm_iter_per_s": round(fp64_it / fp64_t, 4) if fp64_t else None,
+        "fp64_pcg_iterations": fp64[0].report["linear_iterations"] if fp64 else None,
+        "speedup_mixed_vs_fp64": round(fp64_t / mixed_t, 4) if fp64_t else None,
+        "final_cost": mixed[0].finalCost,
+        "rel_cost_err_vs_oracle": rel_oracle,
+        "fp64_rel_cost_err_vs_oracle": rel_oracle64,
+        "rel_cost_err_vs_fp64_gpu": rel_fp64,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "device": mixed[0].report["device_name"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,338 @@
+// pb_device.h - device-side frame-camera model and per-observation linearisation for gfx950.
+//
+// One observation of a world point X by a frame camera (R camera-to-world, centre C) gives the
+// residual r = pi(X) - (u, v), the point Jacobian Jp (2x3, scaled by the point scale of the
+// online-Schur parameterisation), the pose Jacobian (2x6: rotation in the D = diag(1,-1,-1) target
+// frame, world centre) and the Jacobian of the group's active intrinsic slots. The math follows
+// BundleAdjustPlaMatrixProjection.cpp:176-285 (residual, Jacobian layout, weights) and
+// CameraState.cpp:387-481 (13-slot intrinsic layout and the focal/aspect chain rule).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#define PB_HD __host__ __device__ __forceinline__
+
+namespace pbdev
+{
+    constexpr int kSlots = 13;
+    constexpr int kMaxActive = 12;  // active intrinsic slots per group handled on device (2 units of 6)
+
+    // Effective camera used by every kernel of one pass (pose + intrinsics after the shared-group overlay).
+    struct Camera
+    {
+        double R[9];
+        double C[3];
+        double fx, fy, cx, cy;
+        double k1, k2, k3, k4, p1, p2, p3, p4, b2;
+        double basis[6];  // world-frame tangent directions of the gauge scale camera (D * basis)
+        int32_t metashape;
+        int32_t tangent;  // 1 for the gauge scale camera (centre moves on the baseline sphere)
+        int32_t pad[2];
+    };
+
+    struct Linearization
+    {
+        double r[2];
+        double Jp[6];    // 2x3, row-major
+        double Jc[12];   // 2x6, row-major [rotation | centre]
+        double Ji[2 * kMaxActive];  // 2 x K compact intrinsic Jacobian, row-major with stride kMaxActive
+        double w;        // normal weight (whitened weight * robust weight)
+        double cost;     // robust cost
+    };
+
+    PB_HD bool finite(double v) { return __builtin_isfinite(v); }
+
+    // s / atan(s) with s = |X| / 1000 (BundleAdjustPlaMatrixProjection.cpp:15-24).
+    PB_HD double pointScale(double x, double y, double z)
+    {
+        const double s = sqrt(x * x + y * y + z * z) / 1000.0;
+        if (!finite(s) || s == 0.0) return 1.0;
+        return s / atan(s);
+    }
+
+    // Robust loss (BundleAdjustPlaMatrixProjection.cpp:75-109): returns cost and IRLS weight.
+    PB_HD bool robust(double s, int loss, double scale, double& cost, double& weight)
+    {
+        if (!finite(s) || s < 0.0) return false;
+        const double d2 = scale * scale;
+        if (loss == 1)
+        {
+            if (s <= d2)
+            {
+                cost = 0.5 * s;
+                weight = 1.0;
+            }
+            else
+            {
+                const double n = sqrt(s);
+                cost = scale * n - 0.5 * d2;
+                weight = scale / n;
+            }
+        }
+        else if (loss == 2)
+        {
+            const double q = s / d2;
+            cost = 0.5 * d2 * log1p(q);
+            weight = 1.0 / (1.0 + q);
+        }
+        else
+        {
+            cost = 0.5 * s;
+            weight = 1.0;
+        }
+        return true;
+    }
+
+    // Distortion on the normalised plane: (x, y) -> (xd, yd) and, optionally, d(xd,yd)/d(x,y).
+    struct Distorted
+    {
+        double xd, yd;
+        double r2, rad, tx0, ty0, T;
+        double dxx, dxy, dyx, dyy;
+    };
+
+    template <bool kJacobian>
+    PB_HD Distorted distort(const Camera& c, double x, double y)
+    {
+        Distorted d;
+        const double r2 = x * x + y * y;
+        d.r2 = r2;
+        d.rad = 1.0 + r2 * (c.k1 + r2 * (c.k2 + r2 * (c.k3 + r2 * c.k4)));
+        if (c.metashape)
+        {
+            d.tx0 = c.p1 * (r2 + 2.0 * x * x) + 2.0 * c.p2 * x * y;
+            d.ty0 = c.p2 * (r2 + 2.0 * y * y) + 2.0 * c.p1 * x * y;
+            d.T = 1.0 + c.p3 * r2 + c.p4 * r2 * r2;
+        }
+        else
+        {
+            d.tx0 = 2.0 * c.p1 * x * y + c.p2 * (r2 + 2.0 * x * x);
+            d.ty0 = c.p1 * (r2 + 2.0 * y * y) + 2.0 * c.p2 * x * y;
+            d.T = 1.0;
+        }
+        d.xd = x * d.rad + d.tx0 * d.T;
+        d.yd = y * d.rad + d.ty0 * d.T;
+        if (kJacobian)
+        {
+            const double drad = c.k1 + r2 * (2.0 * c.k2 + r2 * (3.0 * c.k3 + r2 * 4.0 * c.k4));
+            double txx, txy, tyx, tyy;
+            if (c.metashape)
+            {
+                const double dT = 2.0 * (c.p3 + 2.0 * c.p4 * r2);
+                txx = (6.0 * c.p1 * x + 2.0 * c.p2 * y) * d.T + d.tx0 * dT * x;
+                txy = (2.0 * c.p1 * y + 2.0 * c.p2 * x) * d.T + d.tx0 * dT * y;
+                tyx = (2.0 * c.p2 * x + 2.0 * c.p1 * y) * d.T + d.ty0 * dT * x;
+                tyy = (6.0 * c.p2 * y + 2.0 * c.p1 * x) * d.T + d.ty0 * dT * y;
+            }
+            else
+            {
+                txx = 2.0 * c.p1 * y + 6.0 * c.p2 * x;
+                txy = 2.0 * c.p1 * x + 2.0 * c.p2 * y;
+                tyx = txy;
+                tyy = 6.0 * c.p1 * y + 2.0 * c.p2 * x;
+            }
+            d.dxx = d.rad + 2.0 * x * x * drad + txx;
+            d.dxy = 2.0 * x * y * drad + txy;
+            d.dyx = 2.0 * x * y * drad + tyx;
+            d.dyy = d.rad + 2.0 * y * y * drad + tyy;
+        }
+        return d;
+    }
+
+    // Projection only (cost passes). Forward depth must exceed 1e-9.
+    PB_HD bool project(const Camera& c, double X0, double X1, double X2, double& u, double& v)
+    {
+        const double d0 = X0 - c.C[0], d1 = X1 - c.C[1], d2 = X2 - c.C[2];
+        const double zc = c.R[2] * d0 + c.R[5] * d1 + c.R[8] * d2;
+        if (!(zc > 1e-9)) return false;
+        const double iz = 1.0 / zc;
+        const double x = (c.R[0] * d0 + c.R[3] * d1 + c.R[6] * d2) * iz;
+        const double y = (c.R[1] * d0 + c.R[4] * d1 + c.R[7] * d2) * iz;
+        const Distorted d = distort<false>(c, x, y);
+        u = c.fx * d.xd + c.b2 * d.yd + c.cx;
+        v = c.fy * d.yd + c.cy;
+        return finite(u) && finite(v);
+    }
+
+    // Derivative of (u, v) with respect to one intrinsic slot.
+    PB_HD void slotDerivative(const Camera& c, const Distorted& d, double x, double y, int slot,
+                                                   double& du, double& dv)
+    {
+        double ex = 0.0, ey = 0.0;
+        switch (slot)
+        {
+        case 0:  // fx at fixed aspect
+            du = d.xd;
+            dv = (c.fy / c.fx) * d.yd;
+            return;
+        case 1:  // log(fy / fx)
+            du = 0.0;
+            dv = c.fy * d.yd;
+            return;
+        case 2:
+            du = 1.0;
+            dv = 0.0;
+            return;
+        case 3:
+            du = 0.0;
+            dv = 1.0;
+            return;
+        case 9:  // skew b2
+            du = d.yd;
+            dv = 0.0;
+            return;
+        case 4: ex = x * d.r2; ey = y * d.r2; break;
+        case 5: ex = x * d.r2 * d.r2; ey = y * d.r2 * d.r2; break;
+        case 6: ex = x * d.r2 * d.r2 * d.r2; ey = y * d.r2 * d.r2 * d.r2; break;
+        case 10:
+        {
+            const double r8 = (d.r2 * d.r2) * (d.r2 * d.r2);
+            ex = x * r8;
+            ey = y * r8;
+            break;
+        }
+        case 7:
+            if (c.metashape) { ex = (d.r2 + 2.0 * x * x) * d.T; ey = 2.0 * x * y * d.T; }
+            else { ex = 2.0 * x * y; ey = d.r2 + 2.0 * y * y; }
+            break;
+        case 8:
+            if (c.metashape) { ex = 2.0 * x * y * d.T; ey = (d.r2 + 2.0 * y * y) * d.T; }
+            else { ex = d.r2 + 2.0 * x * x; ey = 2.0 * x * y; }
+            break;
+        case 11:
+            if (c.metashape) { ex = d.tx0 * d.r2; ey = d.ty0 * d.r2; }
+            break;
+        case 12:
+            if (c.metashape) { ex = d.tx0 * d.r2 * d.r2; ey = d.ty0 * d.r2 * d.r2; }
+            break;
+        default: break;
+        }
+        du = c.fx * ex + c.b2 * ey;
+        dv = c.fy * ey;
+    }
+
+    // Full linearisation of one observation. nActive/activeSlots select the intrinsic columns.
+    PB_HD bool linearize(const Camera& c, double X0, double X1, double X2, double ou, double ov,
+                                              double ow, int loss, double lscale, int nActive,
+                                              const int8_t* activeSlots, Linearization& L)
+    {
+        const double d0 = X0 - c.C[0], d1 = X1 - c.C[1], d2 = X2 - c.C[2];
+        const double zc = c.R[2] * d0 + c.R[5] * d1 + c.R[8] * d2;
+        if (!(zc > 1e-9)) return false;
+        const double iz = 1.0 / zc;
+        const double x = (c.R[0] * d0 + c.R[3] * d1 + c.R[6] * d2) * iz;
+        const double y = (c.R[1] * d0 + c.R[4] * d1 + c.R[7] * d2) * iz;
+        const Distorted d = distort<true>(c, x, y);
+        const double u = c.fx * d.xd + c.b2 * d.yd + c.cx;
+        const double v = c.fy * d.yd + c.cy;
+        L.r[0] = u - ou;
+        L.r[1] = v - ov;
+        // d(u,v)/d(x,y)
+        const double ux = c.fx * d.dxx + c.b2 * d.dyx;
+        const double uy = c.fx * d.dxy + c.b2 * d.dyy;
+        const double vx = c.fy * d.dyx;
+        const double vy = c.fy * d.dyy;
+        // camera-frame Jacobian rows, then world point Jacobian Jp = Jcam * R^T
+        const double cu[3] = {ux * iz, uy * iz, -(ux * x + uy * y) * iz};
+        const double cv[3] = {vx * iz, vy * iz, -(vx * x + vy * y) * iz};
+        double jp[6];
+#pragma unroll
+        for (int w = 0; w < 3; ++w)
+        {
+            jp[w] = cu[0] * c.R[w * 3 + 0] + cu[1] * c.R[w * 3 + 1] + cu[2] * c.R[w * 3 + 2];
+            jp[3 + w] = cv[0] * c.R[w * 3 + 0] + cv[1] * c.R[w * 3 + 1] + cv[2] * c.R[w * 3 + 2];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+        {
+            const double* p = jp + 3 * a;
+            double* J = L.Jc + 6 * a;
+            // rotation in the target frame: J_rot = -Jp [d]x D
+            J[0] = -p[1] * d2 + p[2] * d1;
+            J[1] = -p[0] * d2 + p[2] * d0;
+            J[2] = p[0] * d1 - p[1] * d0;
+            if (c.tangent)
+            {
+                J[3] = -(p[0] * c.basis[0] + p[1] * c.basis[1] + p[2] * c.basis[2]);
+                J[4] = -(p[0] * c.basis[3] + p[1] * c.basis[4] + p[2] * c.basis[5]);
+                J[5] = 0.0;
+            }
+            else
+            {
+                J[3] = -p[0];
+                J[4] = -p[1];
+                J[5] = -p[2];
+            }
+        }
+        const double ps = pointScale(X0, X1, X2);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) L.Jp[i] = jp[i] * ps;
+        for (int k = 0; k < nActive; ++k)
+        {
+            double du, dv;
+            slotDerivative(c, d, x, y, activeSlots[k], du, dv);
+            L.Ji[k] = du;
+            L.Ji[kMaxActive + k] = dv;
+        }
+        const double s = ow * (L.r[0] * L.r[0] + L.r[1] * L.r[1]);
+        double cost, rw;
+        if (!robust(s, loss, lscale, cost, rw)) return false;
+        L.w = ow * rw;
+        L.cost = cost;
+        return finite(L.w) && finite(cost) && finite(u) && finite(v);
+    }
+
+    // Reference pose retraction C2W' = D E^T D C2W, E = Rx(a) Ry(b) Rz(c) (Assembly.cpp:130-179).
+    PB_HD void retractRotation(const double* R, double a, double b, double cz, double* out)
+    {
+        const double sx = sin(a), cx = cos(a), sy = sin(b), cy = cos(b), sz = sin(cz), ccz = cos(cz);
+        // E = Rx Ry Rz
+        const double E[9] = {cy * ccz, -cy * sz, sy,
+                             sx * sy * ccz + cx * sz, -sx * sy * sz + cx * ccz, -sx * cy,
+                             -cx * sy * ccz + sx * sz, cx * sy * sz + sx * ccz, cx * cy};
+        const double sg[3] = {1.0, -1.0, -1.0};
+        double M[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) M[r * 3 + k] = sg[r] * E[k * 3 + r] * sg[k];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                out[r * 3 + k] = M[r * 3 + 0] * R[0 * 3 + k] + M[r * 3 + 1] * R[1 * 3 + k] + M[r * 3 + 2] * R[2 * 3 + k];
+    }
+
+    // Symmetric 3x3 inverse (row-major) with the singularity test |det| <= 1e-15 max|m|^3.
+    PB_HD bool inverse3(const double* m, double* inv)
+    {
+        double mx = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+        {
+            if (!finite(m[i])) return false;
+            mx = fmax(mx, fabs(m[i]));
+        }
+        const double c00 = m[4] * m[8] - m[5] * m[7];
+        const double c01 = m[5] * m[6] - m[3] * m[8];
+        const double c02 = m[3] * m[7] - m[4] * m[6];
+        const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+        if (!finite(det) || !(fabs(det) > 1e-15 * mx * mx * mx)) return false;
+        const double id = 1.0 / det;
+        inv[0] = c00 * id;
+        inv[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+        inv[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+        inv[3] = c01 * id;
+        inv[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+        inv[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+        inv[6] = c02 * id;
+        inv[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+        inv[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+            if (!finite(inv[i])) return false;
+        return true;
+    }
+}  // namespace pbdev

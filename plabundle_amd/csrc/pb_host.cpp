@@ -1,0 +1,1819 @@
+// pb_host.cpp - host side of the MI355X frame-BA backend: the C ABI of include/pb_hip.h.
+//
+// Host work per solve (C++20):
+//   1. options validation and gauge normalisation .... BundleAdjustValidation.cpp:371-896 (image-only subset)
+//   2. active problem and block numbering ............ BundleAdjustPlaMatrixProblem.cpp:29-278
+//   3. shared-intrinsic groups ....................... BundleAdjustPlaMatrixModel.cpp:84-231
+//   4. device layout: track-sorted observations, camera-sorted permutation, cross-block entries,
+//      Schur pair runs and the block-sparse reduced-system pattern (built once per problem)
+//   5. the online-Schur Gauss-Newton/Armijo controller BundleAdjustPlaMatrix.cpp:162-631, issuing the
+//      kernels of pb_kernels.hip on one HIP stream and reading back a few scalars per stage.
+#include "../../include/pb_hip.h"
+#include "pb_device.h"
+#include "pb_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <new>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace
+{
+    using Clock = std::chrono::steady_clock;
+    constexpr double kInf = std::numeric_limits<double>::infinity();
+
+    struct HipError : std::runtime_error
+    {
+        hipError_t code;
+        HipError(hipError_t c, const char* what) : std::runtime_error(what), code(c) {}
+    };
+
+    void check(hipError_t e, const char* where)
+    {
+        if (e != hipSuccess)
+        {
+            std::string msg = std::string(where) + ": " + hipGetErrorString(e);
+            throw HipError(e, msg.c_str());
+        }
+    }
+
+    double seconds(Clock::time_point a) { return std::chrono::duration<double>(Clock::now() - a).count(); }
+
+    // ------------------------------------------------------------------------------------------
+    // Input copy of one problem.
+    struct Input
+    {
+        int nCam = 0, nTracks = 0;
+        std::vector<double> R, C, slots, ref;
+        std::vector<int32_t> conv, group;
+        bool hasGroup = false;
+        std::vector<double> X;
+        std::vector<int64_t> off;
+        std::vector<int32_t> obsCam;
+        std::vector<double> uv, weight, scale;
+        std::vector<int32_t> fixedCams, fixedTracks;
+        int gaugePolicy = PB_GAUGE_AUTO_ANCHOR, gaugeAnchor = -1, gaugeScale = -1;
+        double gaugeBaseline = 0.0;
+    };
+
+    double sanitizedWeight(double w) { return std::isfinite(w) && w > 0.0 ? w : 0.0; }
+    double sanitizedScale(double s) { return std::isfinite(s) && s > 0.0 ? s : 1.0; }
+
+    bool usable(const Input& in, int64_t i)
+    {
+        const int c = in.obsCam[i];
+        return c >= 0 && c < in.nCam && std::isfinite(in.uv[2 * i]) && std::isfinite(in.uv[2 * i + 1]) &&
+               sanitizedWeight(in.weight[i]) > 0.0;
+    }
+
+    pbdev::Camera inputCamera(const Input& in, int c)
+    {
+        pbdev::Camera k{};
+        std::copy_n(&in.R[9 * c], 9, k.R);
+        std::copy_n(&in.C[3 * c], 3, k.C);
+        const double* s = &in.slots[13 * c];
+        k.fx = s[0];
+        k.fy = s[0] * std::exp(s[1]);
+        k.cx = s[2];
+        k.cy = s[3];
+        k.k1 = s[4];
+        k.k2 = s[5];
+        k.k3 = s[6];
+        k.p1 = s[7];
+        k.p2 = s[8];
+        k.b2 = s[9];
+        k.k4 = s[10];
+        k.metashape = in.conv[c] == PB_BROWN_METASHAPE;
+        k.p3 = k.metashape ? s[11] : 0.0;
+        k.p4 = k.metashape ? s[12] : 0.0;
+        return k;
+    }
+
+    uint32_t supportedMask(int conv) { return conv == PB_BROWN_METASHAPE ? 0x1FFFu : 0x7FFu; }
+
+    // ------------------------------------------------------------------------------------------
+    // Host preparation of one solve.
+    struct Prepared
+    {
+        std::vector<int32_t> fixedCams;
+        std::vector<int32_t> fixedTracks;  // sorted
+        int anchor = -1, scaleCam = -1;
+        double baseline = 0.0;
+        // active problem (bit-exact with prepareActiveProblem)
+        std::vector<char> activeTrack;
+        std::vector<int32_t> cameraBlock, intrinsicBlock, groupOfCam, trackBlock;
+        int cameraBlocks = 0, intrinsicBlocks = 0, primaryBlocks = 0, trackBlocks = 0;
+        int activeTracks = 0, observations = 0, rejectedInitial = 0;
+        // groups
+        int G = 0;
+        std::vector<double> gParams, gPrior, gInvSigma, gLower, gUpper;
+        std::vector<uint32_t> gEnabled;
+        std::vector<int8_t> gActive;
+        std::vector<int32_t> gK, gUnit, slotToCompact;
+        // units
+        int nUnits = 0, globalDim = 0;
+        std::vector<int32_t> unitWidth;
+    };
+
+    std::string normalizeGauge(const Input& in, const pb_solve_options& o, Prepared* p)
+    {
+        const int n = in.nCam;
+        std::set<int> fixed;
+        for (int c : in.fixedCams)
+        {
+            if (c < 0 || c >= n) return "invalid: fixed camera index out of range";
+            if (!fixed.insert(c).second) return "invalid: duplicate fixed camera index";
+        }
+        const bool requested = in.gaugeAnchor >= 0 || in.gaugeScale >= 0 || in.gaugeBaseline > 0.0;
+        if (requested && (in.gaugeAnchor < 0 || in.gaugeScale < 0 || in.gaugeAnchor >= n || in.gaugeScale >= n ||
+                          in.gaugeAnchor == in.gaugeScale || !std::isfinite(in.gaugeBaseline) ||
+                          in.gaugeBaseline <= 0.0))
+            return "invalid: reference gauge";
+        std::set<int> ft;
+        for (int t : in.fixedTracks)
+        {
+            if (t < 0 || t >= in.nTracks) return "invalid: fixed track index out of range";
+            if (!ft.insert(t).second) return "invalid: duplicate fixed track index";
+        }
+        p->fixedCams = in.fixedCams;
+        p->fixedTracks.assign(ft.begin(), ft.end());
+        p->anchor = in.gaugeAnchor;
+        p->scaleCam = in.gaugeScale;
+        p->baseline = in.gaugeBaseline;
+        if (!o.refine_camera_pose || in.gaugePolicy == PB_GAUGE_CALLER_MANAGED || n == 0) return {};
+        if (in.gaugePolicy == PB_GAUGE_REQUIRE_EXPLICIT)
+            return fixed.size() >= 2 ? std::string() : std::string("unsupported: incomplete gauge (fix two cameras)");
+        if (fixed.empty())
+        {
+            p->fixedCams.push_back(0);
+            fixed.insert(0);
+        }
+        if (fixed.size() < 2)
+        {
+            const int a = *fixed.begin();
+            int best = -1;
+            double bestD = 1e-20;
+            for (int c = 0; c < n; ++c)
+            {
+                if (fixed.count(c)) continue;
+                double d = 0.0;
+                for (int k = 0; k < 3; ++k) d += (in.C[3 * c + k] - in.C[3 * a + k]) * (in.C[3 * c + k] - in.C[3 * a + k]);
+                if (std::isfinite(d) && d > bestD)
+                {
+                    bestD = d;
+                    best = c;
+                }
+            }
+            if (best < 0) return "unsupported: no second camera with a non-zero baseline";
+            p->anchor = a;
+            p->scaleCam = best;
+            p->baseline = std::sqrt(bestD);
+        }
+        return {};
+    }
+
+    void prepareActive(const Input& in, const pb_solve_options& o, Prepared* p)
+    {
+        const int n = in.nCam;
+        p->activeTrack.assign(in.nTracks, 0);
+        p->cameraBlock.assign(n, -1);
+        p->intrinsicBlock.assign(n, -1);
+        p->groupOfCam.assign(n, 0);
+        p->trackBlock.assign(in.nTracks, -1);
+        std::vector<char> hasResidual(n, 0);
+        std::vector<pbdev::Camera> cams(n);
+        for (int c = 0; c < n; ++c) cams[c] = inputCamera(in, c);
+        for (int t = 0; t < in.nTracks; ++t)
+        {
+            const double* X = &in.X[3 * t];
+            if (!std::isfinite(X[0]) || !std::isfinite(X[1]) || !std::isfinite(X[2])) continue;
+            int first = -1, count = 0;
+            bool second = false, front = true;
+            double sq = 0.0;
+            for (int64_t i = in.off[t]; i < in.off[t + 1]; ++i)
+            {
+                if (!usable(in, i)) continue;
+                const int c = in.obsCam[i];
+                first = first < 0 ? c : first;
+                second = second || c != first;
+                double u, v;
+                if (!pbdev::project(cams[c], X[0], X[1], X[2], u, v))
+                {
+                    front = false;
+                    break;
+                }
+                const double du = u - in.uv[2 * i], dv = v - in.uv[2 * i + 1];
+                sq += sanitizedWeight(in.weight[i]) * (du * du + dv * dv);
+                ++count;
+            }
+            if (count < 2 || !second || !front) continue;
+            const double rms = std::sqrt(sq / (2.0 * count));
+            if (o.max_initial_track_rms > 0.0 && rms > o.max_initial_track_rms)
+            {
+                ++p->rejectedInitial;
+                continue;
+            }
+            p->activeTrack[t] = 1;
+            ++p->activeTracks;
+            p->observations += count;
+            for (int64_t i = in.off[t]; i < in.off[t + 1]; ++i)
+                if (usable(in, i)) hasResidual[in.obsCam[i]] = 1;
+        }
+        if (o.refine_camera_pose)
+        {
+            std::set<int> fixed(p->fixedCams.begin(), p->fixedCams.end());
+            for (int c = 0; c < n; ++c)
+                if (hasResidual[c] && !fixed.count(c)) p->cameraBlock[c] = p->cameraBlocks++;
+        }
+        p->primaryBlocks = p->cameraBlocks;
+        if (o.shared_intrinsic_mask & 0x1FFFu)
+        {
+            std::vector<int> ids;
+            for (int c = 0; c < n; ++c)
+            {
+                const int id = in.hasGroup ? in.group[c] : 0;
+                auto it = std::find(ids.begin(), ids.end(), id);
+                if (it == ids.end())
+                {
+                    p->groupOfCam[c] = static_cast<int>(ids.size());
+                    ids.push_back(id);
+                }
+                else
+                {
+                    p->groupOfCam[c] = static_cast<int>(it - ids.begin());
+                }
+            }
+            p->intrinsicBlocks = static_cast<int>(ids.size());
+            for (int c = 0; c < n; ++c) p->intrinsicBlock[c] = p->primaryBlocks + p->groupOfCam[c];
+            p->primaryBlocks += p->intrinsicBlocks;
+        }
+        for (int t = 0; t < in.nTracks; ++t)
+        {
+            if (!p->activeTrack[t]) continue;
+            if (std::binary_search(p->fixedTracks.begin(), p->fixedTracks.end(), t)) continue;
+            p->trackBlock[t] = p->trackBlocks++;
+        }
+    }
+
+    double median(std::vector<double> v, double fallback)
+    {
+        v.erase(std::remove_if(v.begin(), v.end(), [](double x) { return !std::isfinite(x); }), v.end());
+        if (v.empty()) return fallback;
+        std::sort(v.begin(), v.end());
+        const size_t k = v.size();
+        return k % 2 ? v[k / 2] : 0.5 * (v[k / 2 - 1] + v[k / 2]);
+    }
+
+    void prepareGroups(const Input& in, const pb_solve_options& o, Prepared* p)
+    {
+        const int G = p->intrinsicBlocks;
+        p->G = G;
+        p->gParams.assign(13 * G, 0.0);
+        p->gPrior.assign(13 * G, 0.0);
+        p->gInvSigma.assign(13 * G, 0.0);
+        p->gLower.assign(13 * G, -kInf);
+        p->gUpper.assign(13 * G, kInf);
+        p->gEnabled.assign(G, 0u);
+        p->gActive.assign(pbdev::kMaxActive * G, 0);
+        p->gK.assign(G, 0);
+        p->gUnit.assign(G, -1);
+        p->slotToCompact.assign(13 * G, -1);
+        const double low = std::max(1.0, o.low_order_distortion_scale);
+        for (int g = 0; g < G; ++g)
+        {
+            std::array<std::vector<double>, 13> init, ref;
+            uint32_t sup = 0x1FFFu;
+            for (int c = 0; c < in.nCam; ++c)
+            {
+                if (p->groupOfCam[c] != g) continue;
+                const double* s = &in.slots[13 * c];
+                const double* r = &in.ref[13 * c];
+                init[0].push_back(s[0]);
+                ref[0].push_back(r[0]);
+                init[1].push_back(std::exp(s[1]));
+                ref[1].push_back(std::exp(r[1]));
+                init[2].push_back(s[2] - r[2]);
+                init[3].push_back(s[3] - r[3]);
+                ref[2].push_back(0.0);
+                ref[3].push_back(0.0);
+                for (int k = 4; k < 13; ++k)
+                {
+                    init[k].push_back(s[k]);
+                    ref[k].push_back(r[k]);
+                }
+                sup &= supportedMask(in.conv[c]);
+            }
+            double* gp = &p->gParams[13 * g];
+            double* pr = &p->gPrior[13 * g];
+            double* lo = &p->gLower[13 * g];
+            double* hi = &p->gUpper[13 * g];
+            const uint32_t en = o.shared_intrinsic_mask & sup & 0x1FFFu;
+            p->gEnabled[g] = en;
+            const double fref = median(ref[0], 1.0);
+            const double aref = median(ref[1], 1.0);
+            gp[0] = std::max(1e-12, median(init[0], fref));
+            gp[1] = std::log(std::max(1e-12, median(init[1], aref)));
+            pr[0] = fref;
+            pr[1] = std::log(aref);
+            for (int k = 2; k < 13; ++k)
+            {
+                pr[k] = median(ref[k], 0.0);
+                gp[k] = median(init[k], pr[k]);
+            }
+            lo[0] = fref * o.min_focal_scale;
+            hi[0] = fref * o.max_focal_scale;
+            lo[1] = std::log(aref * o.min_aspect_scale);
+            hi[1] = std::log(aref * o.max_aspect_scale);
+            const double pl = fref * o.max_principal_offset_fraction;
+            lo[2] = lo[3] = -pl;
+            hi[2] = hi[3] = pl;
+            const double lim[9] = {o.max_abs[0] * low, o.max_abs[1], o.max_abs[2], o.max_abs[3] * low,
+                                   o.max_abs[4] * low, fref * o.max_abs[5], o.max_abs[6], o.max_abs[7], o.max_abs[8]};
+            for (int k = 0; k < 9; ++k)
+            {
+                lo[4 + k] = -lim[k];
+                hi[4 + k] = lim[k];
+            }
+            const double ps = std::max(1e-6, fref * o.principal_prior_sigma_fraction);
+            const double sig[13] = {o.focal_prior_sigma * fref, o.aspect_prior_sigma, ps, ps,
+                                    o.prior_sigma[0] * low, o.prior_sigma[1], o.prior_sigma[2],
+                                    o.prior_sigma[3] * low, o.prior_sigma[4] * low, fref * o.prior_sigma[5],
+                                    o.prior_sigma[6], o.prior_sigma[7], o.prior_sigma[8]};
+            int K = 0;
+            for (int k = 0; k < 13; ++k)
+            {
+                gp[k] = std::clamp(gp[k], lo[k], hi[k]);
+                const bool on = (en >> k) & 1u;
+                p->gInvSigma[13 * g + k] = on ? 1.0 / std::max(1e-9, sig[k]) : 0.0;
+                if (on)
+                {
+                    if (K >= pbdev::kMaxActive) throw std::invalid_argument("unsupported: more than 12 refined slots");
+                    p->gActive[pbdev::kMaxActive * g + K] = static_cast<int8_t>(k);
+                    p->slotToCompact[13 * g + k] = K;
+                    ++K;
+                }
+            }
+            p->gK[g] = K;
+        }
+        // units: pose units first (ids = camera block ids), then ceil(K/6) per group
+        int u = p->cameraBlocks;
+        p->unitWidth.assign(p->cameraBlocks, 6);
+        p->globalDim = 6 * p->cameraBlocks;
+        for (int g = 0; g < G; ++g)
+        {
+            if (p->gK[g] == 0) continue;
+            p->gUnit[g] = u;
+            const int m = (p->gK[g] + 5) / 6;
+            for (int j = 0; j < m; ++j) p->unitWidth.push_back(std::min(6, p->gK[g] - 6 * j));
+            u += m;
+            p->globalDim += p->gK[g];
+        }
+        p->nUnits = u;
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // Device layout of one prepared problem.
+    struct Layout
+    {
+        int nActive = 0, nObs = 0, nElim = 0, nEntries = 0, nBlocks = 0, nNnz = 0, nItems = 0;
+        long nPairs = 0;
+        int maxGroupK = 0;
+        std::vector<int32_t> activeTrackOf;  // active index -> input track
+        std::vector<int32_t> obsOffset, obsCam, obsPoint, obsPoseEntry, obsGroupEntry;
+        std::vector<double> obsUV, obsW, pointX;
+        std::vector<int32_t> pointElim, elimTrack;
+        std::vector<int32_t> camObsOffset, camObs, camUnit, camGroup, camDiagBlock, camGroupBlock;
+        std::vector<int32_t> groupCamOffset, groupCams, groupPairBlock;
+        std::vector<int32_t> entryOffset, entryUnit, entryPoint;
+        std::vector<int32_t> blockRow, blockCol, blockPosUp, blockPosLo, blockItemOffset;
+        std::vector<int32_t> itemBlock, itemBegin, pairE1, pairE2;
+        std::vector<int32_t> rowPtr, colIdx, diagPos;
+    };
+
+    void buildLayout(const Input& in, const Prepared& p, Layout* L)
+    {
+        // active tracks and their usable observations (track order)
+        std::vector<int32_t> activeIndex(in.nTracks, -1);
+        for (int t = 0; t < in.nTracks; ++t)
+            if (p.activeTrack[t])
+            {
+                activeIndex[t] = static_cast<int32_t>(L->activeTrackOf.size());
+                L->activeTrackOf.push_back(t);
+            }
+        L->nActive = static_cast<int>(L->activeTrackOf.size());
+        L->obsOffset.assign(L->nActive + 1, 0);
+        for (int a = 0; a < L->nActive; ++a)
+        {
+            const int t = L->activeTrackOf[a];
+            for (int64_t i = in.off[t]; i < in.off[t + 1]; ++i)
+            {
+                if (!usable(in, i)) continue;
+                L->obsCam.push_back(in.obsCam[i]);
+                L->obsPoint.push_back(a);
+                L->obsUV.push_back(in.uv[2 * i]);
+                L->obsUV.push_back(in.uv[2 * i + 1]);
+                const double ms = sanitizedScale(in.scale[i]);
+                L->obsW.push_back(sanitizedWeight(in.weight[i]) / (ms * ms));
+            }
+            L->obsOffset[a + 1] = static_cast<int32_t>(L->obsCam.size());
+            L->pointX.insert(L->pointX.end(), &in.X[3 * t], &in.X[3 * t] + 3);
+        }
+        L->nObs = static_cast<int>(L->obsCam.size());
+        L->pointElim.assign(L->nActive, -1);
+        for (int a = 0; a < L->nActive; ++a)
+        {
+            const int b = p.trackBlock[L->activeTrackOf[a]];
+            L->pointElim[a] = b;
+            if (b >= 0) L->elimTrack.push_back(a);
+        }
+        L->nElim = static_cast<int>(L->elimTrack.size());
+        // camera units and groups
+        const int n = in.nCam;
+        L->camUnit.assign(n, -1);
+        L->camGroup.assign(n, -1);
+        for (int c = 0; c < n; ++c)
+        {
+            L->camUnit[c] = p.cameraBlock[c];
+            if (p.intrinsicBlock[c] >= 0) L->camGroup[c] = p.groupOfCam[c];
+        }
+        for (int g = 0; g < p.G; ++g) L->maxGroupK = std::max(L->maxGroupK, p.gK[g]);
+        auto groupUnits = [&](int c, int* first) -> int
+        {
+            const int g = L->camGroup[c];
+            if (g < 0 || p.gK[g] == 0) return 0;
+            *first = p.gUnit[g];
+            return (p.gK[g] + 5) / 6;
+        };
+        // camera-sorted observation permutation
+        L->camObsOffset.assign(n + 1, 0);
+        for (int o = 0; o < L->nObs; ++o) ++L->camObsOffset[L->obsCam[o] + 1];
+        for (int c = 0; c < n; ++c) L->camObsOffset[c + 1] += L->camObsOffset[c];
+        L->camObs.assign(L->nObs, 0);
+        {
+            std::vector<int32_t> fill(L->camObsOffset.begin(), L->camObsOffset.end() - 1);
+            for (int o = 0; o < L->nObs; ++o) L->camObs[fill[L->obsCam[o]]++] = o;
+        }
+        // cross-block entries per eliminated point
+        L->obsPoseEntry.assign(L->nObs, -1);
+        L->obsGroupEntry.assign(L->nObs, -1);
+        L->entryOffset.assign(L->nElim + 1, 0);
+        std::vector<int32_t> units;
+        for (int b = 0; b < L->nElim; ++b)
+        {
+            const int a = L->elimTrack[b];
+            units.clear();
+            for (int o = L->obsOffset[a]; o < L->obsOffset[a + 1]; ++o)
+            {
+                const int c = L->obsCam[o];
+                if (L->camUnit[c] >= 0) units.push_back(L->camUnit[c]);
+                int g0 = 0;
+                const int m = groupUnits(c, &g0);
+                for (int j = 0; j < m; ++j) units.push_back(g0 + j);
+            }
+            std::sort(units.begin(), units.end());
+            units.erase(std::unique(units.begin(), units.end()), units.end());
+            const int base = static_cast<int>(L->entryUnit.size());
+            for (int u : units)
+            {
+                L->entryUnit.push_back(u);
+                L->entryPoint.push_back(b);
+            }
+            L->entryOffset[b + 1] = static_cast<int32_t>(L->entryUnit.size());
+            for (int o = L->obsOffset[a]; o < L->obsOffset[a + 1]; ++o)
+            {
+                const int c = L->obsCam[o];
+                if (L->camUnit[c] >= 0)
+                    L->obsPoseEntry[o] =
+                        base + static_cast<int>(std::lower_bound(units.begin(), units.end(), L->camUnit[c]) - units.begin());
+                int g0 = 0;
+                if (groupUnits(c, &g0) > 0)
+                    L->obsGroupEntry[o] =
+                        base + static_cast<int>(std::lower_bound(units.begin(), units.end(), g0) - units.begin());
+            }
+        }
+        L->nEntries = static_cast<int>(L->entryUnit.size());
+        // Schur pairs bucketed by row unit, then sorted by (column unit, first entry)
+        const int U = p.nUnits;
+        std::vector<int64_t> rowCount(U + 1, 0);
+        for (int b = 0; b < L->nElim; ++b)
+        {
+            const int e0 = L->entryOffset[b], e1 = L->entryOffset[b + 1];
+            for (int i = e0; i < e1; ++i) rowCount[L->entryUnit[i] + 1] += e1 - i;
+        }
+        for (int u = 0; u < U; ++u) rowCount[u + 1] += rowCount[u];
+        L->nPairs = rowCount[U];
+        if (L->nPairs > std::numeric_limits<int32_t>::max()) throw std::invalid_argument("unsupported: too many Schur pairs");
+        struct PairRec
+        {
+            int32_t col, e1, e2;
+        };
+        std::vector<PairRec> recs(L->nPairs);
+        {
+            std::vector<int64_t> fill(rowCount.begin(), rowCount.end() - 1);
+            for (int b = 0; b < L->nElim; ++b)
+            {
+                const int e0 = L->entryOffset[b], e1 = L->entryOffset[b + 1];
+                for (int i = e0; i < e1; ++i)
+                    for (int j = i; j < e1; ++j)
+                        recs[fill[L->entryUnit[i]]++] = {L->entryUnit[j], i, j};
+            }
+        }
+        for (int u = 0; u < U; ++u)
+            std::sort(recs.begin() + rowCount[u], recs.begin() + rowCount[u + 1],
+                      [](const PairRec& x, const PairRec& y) { return x.col != y.col ? x.col < y.col : x.e1 < y.e1; });
+        // direct-part blocks
+        std::vector<std::vector<int32_t>> direct(U);
+        for (int u = 0; u < U; ++u) direct[u].push_back(u);
+        for (int c = 0; c < n; ++c)
+        {
+            const int pu = L->camUnit[c];
+            int g0 = 0;
+            const int m = groupUnits(c, &g0);
+            if (pu >= 0)
+                for (int j = 0; j < m; ++j) direct[pu].push_back(g0 + j);
+        }
+        for (int g = 0; g < p.G; ++g)
+        {
+            if (p.gK[g] == 0) continue;
+            const int m = (p.gK[g] + 5) / 6;
+            for (int i = 0; i < m; ++i)
+                for (int j = i; j < m; ++j) direct[p.gUnit[g] + i].push_back(p.gUnit[g] + j);
+        }
+        // upper blocks in (row, col) order with their pair runs and work items
+        std::vector<int32_t> blockStart(U + 1, 0);
+        std::vector<int32_t> rowCols;
+        L->itemBegin.clear();
+        L->blockItemOffset.push_back(0);
+        int64_t pairOut = 0;
+        L->pairE1.resize(L->nPairs);
+        L->pairE2.resize(L->nPairs);
+        for (int u = 0; u < U; ++u)
+        {
+            rowCols.assign(direct[u].begin(), direct[u].end());
+            for (int64_t k = rowCount[u]; k < rowCount[u + 1]; ++k)
+                if (rowCols.empty() || rowCols.back() != recs[k].col) rowCols.push_back(recs[k].col);
+            std::sort(rowCols.begin(), rowCols.end());
+            rowCols.erase(std::unique(rowCols.begin(), rowCols.end()), rowCols.end());
+            int64_t k = rowCount[u];
+            for (int v : rowCols)
+            {
+                const int blk = static_cast<int>(L->blockRow.size());
+                L->blockRow.push_back(u);
+                L->blockCol.push_back(v);
+                const int64_t start = pairOut;
+                while (k < rowCount[u + 1] && recs[k].col == v)
+                {
+                    L->pairE1[pairOut] = recs[k].e1;
+                    L->pairE2[pairOut] = recs[k].e2;
+                    ++pairOut;
+                    ++k;
+                }
+                for (int64_t s = start; s < pairOut; s += pbk::kPairChunk)
+                {
+                    L->itemBlock.push_back(blk);
+                    L->itemBegin.push_back(static_cast<int32_t>(s));
+                }
+                L->blockItemOffset.push_back(static_cast<int32_t>(L->itemBlock.size()));
+            }
+            blockStart[u + 1] = static_cast<int32_t>(L->blockRow.size());
+        }
+        L->itemBegin.push_back(static_cast<int32_t>(pairOut));
+        L->nItems = static_cast<int>(L->itemBlock.size());
+        L->nBlocks = static_cast<int>(L->blockRow.size());
+        recs.clear();
+        recs.shrink_to_fit();
+        auto blockId = [&](int u, int v) -> int
+        {
+            const auto b = L->blockCol.begin() + blockStart[u];
+            const auto e = L->blockCol.begin() + blockStart[u + 1];
+            const auto it = std::lower_bound(b, e, v);
+            return (it != e && *it == v) ? static_cast<int>(it - L->blockCol.begin()) : -1;
+        };
+        // full block-sparse rows: lower part (from upper blocks (w, u), w < u) then the upper part
+        std::vector<std::vector<int32_t>> lower(U);
+        for (int blk = 0; blk < L->nBlocks; ++blk)
+            if (L->blockRow[blk] != L->blockCol[blk]) lower[L->blockCol[blk]].push_back(blk);
+        L->rowPtr.assign(U + 1, 0);
+        L->blockPosUp.assign(L->nBlocks, -1);
+        L->blockPosLo.assign(L->nBlocks, -1);
+        L->diagPos.assign(U, -1);
+        for (int u = 0; u < U; ++u)
+        {
+            for (int blk : lower[u])
+            {
+                L->blockPosLo[blk] = static_cast<int32_t>(L->colIdx.size());
+                L->colIdx.push_back(L->blockRow[blk]);
+            }
+            for (int blk = blockStart[u]; blk < blockStart[u + 1]; ++blk)
+            {
+                L->blockPosUp[blk] = static_cast<int32_t>(L->colIdx.size());
+                if (L->blockCol[blk] == u)
+                {
+                    L->diagPos[u] = L->blockPosUp[blk];
+                    L->blockPosLo[blk] = L->blockPosUp[blk];
+                }
+                L->colIdx.push_back(L->blockCol[blk]);
+            }
+            L->rowPtr[u + 1] = static_cast<int32_t>(L->colIdx.size());
+        }
+        L->nNnz = static_cast<int>(L->colIdx.size());
+        // direct-part block ids
+        L->camDiagBlock.assign(n, -1);
+        L->camGroupBlock.assign(2 * n, -1);
+        for (int c = 0; c < n; ++c)
+        {
+            const int pu = L->camUnit[c];
+            if (pu < 0) continue;
+            L->camDiagBlock[c] = blockId(pu, pu);
+            int g0 = 0;
+            const int m = groupUnits(c, &g0);
+            for (int j = 0; j < m; ++j) L->camGroupBlock[2 * c + j] = blockId(pu, g0 + j);
+        }
+        L->groupPairBlock.assign(4 * std::max(1, p.G), -1);
+        L->groupCamOffset.assign(p.G + 1, 0);
+        for (int g = 0; g < p.G; ++g)
+        {
+            if (p.gK[g] > 0)
+            {
+                const int m = (p.gK[g] + 5) / 6;
+                for (int i = 0; i < m; ++i)
+                    for (int j = i; j < m; ++j) L->groupPairBlock[4 * g + 2 * i + j] = blockId(p.gUnit[g] + i, p.gUnit[g] + j);
+            }
+            for (int c = 0; c < n; ++c)
+                if (L->camGroup[c] == g) L->groupCams.push_back(c);
+            L->groupCamOffset[g + 1] = static_cast<int32_t>(L->groupCams.size());
+        }
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // Device buffers.
+    struct DeviceAlloc
+    {
+        std::vector<void*> ptrs;
+        size_t bytes = 0;
+        template <typename V>
+        V* alloc(size_t n)
+        {
+            void* p = nullptr;
+            const size_t sz = std::max<size_t>(1, n) * sizeof(V);
+            const hipError_t e = hipMalloc(&p, sz);
+            if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+            check(e, "hipMalloc");
+            ptrs.push_back(p);
+            bytes += sz;
+            return static_cast<V*>(p);
+        }
+        template <typename V>
+        V* upload(const std::vector<V>& h, hipStream_t s)
+        {
+            V* d = alloc<V>(h.size());
+            if (!h.empty()) check(hipMemcpyAsync(d, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice, s), "upload");
+            return d;
+        }
+        void release()
+        {
+            for (void* p : ptrs) (void)hipFree(p);
+            ptrs.clear();
+            bytes = 0;
+        }
+    };
+
+    struct Family
+    {
+        int64_t launches = 0;
+        double ms = 0.0;
+        double bytes = 0.0;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    };
+
+    const char* const kFamilies[] = {"linearize_points", "camera_direct", "schur_blocks", "pcg_spmv", "cost",
+                                     "back_substitution"};
+    constexpr int kFamilyCount = sizeof(kFamilies) / sizeof(kFamilies[0]);
+}  // namespace
+
+struct pb_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error = "ok";
+    std::string deviceName;
+    Input in;
+    bool uploaded = false;
+    // prepared structure (kept while the problem and the structural options stay the same)
+    bool structureValid = false;
+    uint32_t structureMask = 0;
+    int structureRefinePose = -1;
+    double structureMaxRms = -1.0;
+    Prepared prep;
+    Layout lay;
+    DeviceAlloc dstruct;  // structure + state
+    DeviceAlloc dwork32, dwork64;
+    bool work32 = false, work64 = false;
+    pbk::Problem P;
+    pbk::Work<float> W32;
+    pbk::Work<double> W64;
+    int costBlocks = 0;
+    double* pinned = nullptr;  // host-pinned readback area
+    // last results
+    std::vector<double> outR, outC, outGroupParams, outPointX;
+    bool haveResult = false;
+    // profiling
+    bool profile = false;
+    double lastScalarBytes = 8.0;
+    std::array<Family, kFamilyCount> fam;
+    std::vector<hipEvent_t> eventPool;
+    size_t eventNext = 0;
+};
+
+namespace
+{
+    hipEvent_t takeEvent(pb_ctx* ctx)
+    {
+        if (ctx->eventNext == ctx->eventPool.size())
+        {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            ctx->eventPool.push_back(e);
+        }
+        return ctx->eventPool[ctx->eventNext++];
+    }
+
+    struct Span
+    {
+        pb_ctx* ctx;
+        int family;
+        hipEvent_t stop = nullptr;
+        Span(pb_ctx* c, int f) : ctx(c), family(f)
+        {
+            if (!ctx->profile) return;
+            hipEvent_t start = takeEvent(ctx);
+            stop = takeEvent(ctx);
+            check(hipEventRecord(start, ctx->stream), "hipEventRecord");
+            ctx->fam[family].pending.emplace_back(start, stop);
+        }
+        ~Span()
+        {
+            if (ctx->profile && stop) (void)hipEventRecord(stop, ctx->stream);
+        }
+    };
+
+    void collectProfile(pb_ctx* ctx)
+    {
+        if (!ctx->profile) return;
+        check(hipStreamSynchronize(ctx->stream), "sync");
+        for (Family& f : ctx->fam)
+        {
+            for (auto& pr : f.pending)
+            {
+                float ms = 0.0f;
+                check(hipEventElapsedTime(&ms, pr.first, pr.second), "hipEventElapsedTime");
+                f.ms += ms;
+                ++f.launches;
+            }
+            f.pending.clear();
+        }
+        ctx->eventNext = 0;
+    }
+
+    void copyIn(const pb_problem_view* v, Input* in)
+    {
+        Input r;
+        r.nCam = v->num_cameras;
+        r.nTracks = v->num_tracks;
+        const size_t n = static_cast<size_t>(r.nCam);
+        r.R.assign(v->cam_rotation, v->cam_rotation + 9 * n);
+        r.C.assign(v->cam_center, v->cam_center + 3 * n);
+        r.slots.assign(v->cam_intrinsics, v->cam_intrinsics + 13 * n);
+        r.ref = v->ref_intrinsics ? std::vector<double>(v->ref_intrinsics, v->ref_intrinsics + 13 * n) : r.slots;
+        r.conv = v->cam_convention ? std::vector<int32_t>(v->cam_convention, v->cam_convention + n)
+                                   : std::vector<int32_t>(n, PB_BROWN_OPENCV);
+        r.hasGroup = v->cam_group != nullptr;
+        if (r.hasGroup) r.group.assign(v->cam_group, v->cam_group + n);
+        r.X.assign(v->track_point, v->track_point + 3 * static_cast<size_t>(r.nTracks));
+        r.off.assign(v->track_offset, v->track_offset + r.nTracks + 1);
+        const int64_t no = r.off.back();
+        r.obsCam.assign(v->obs_camera, v->obs_camera + no);
+        r.uv.assign(v->obs_uv, v->obs_uv + 2 * no);
+        r.weight = v->obs_weight ? std::vector<double>(v->obs_weight, v->obs_weight + no) : std::vector<double>(no, 1.0);
+        r.scale = v->obs_scale ? std::vector<double>(v->obs_scale, v->obs_scale + no) : std::vector<double>(no, 1.0);
+        if (v->fixed_cameras) r.fixedCams.assign(v->fixed_cameras, v->fixed_cameras + v->num_fixed_cameras);
+        if (v->fixed_tracks) r.fixedTracks.assign(v->fixed_tracks, v->fixed_tracks + v->num_fixed_tracks);
+        r.gaugePolicy = v->gauge_policy;
+        r.gaugeAnchor = v->gauge_anchor_camera;
+        r.gaugeScale = v->gauge_scale_camera;
+        r.gaugeBaseline = v->gauge_baseline;
+        *in = std::move(r);
+    }
+
+    std::string validateInput(const pb_problem_view* v)
+    {
+        if (!v) return "problem is NULL";
+        if (v->num_cameras < 0 || v->num_tracks < 0) return "negative counts";
+        if (v->num_cameras > 0 && (!v->cam_rotation || !v->cam_center || !v->cam_intrinsics))
+            return "camera arrays are NULL";
+        if (v->num_tracks > 0 && (!v->track_point || !v->track_offset || !v->obs_camera || !v->obs_uv))
+            return "track arrays are NULL";
+        if (v->num_tracks > 0)
+        {
+            if (v->track_offset[0] != 0) return "track_offset[0] must be 0";
+            for (int t = 0; t < v->num_tracks; ++t)
+                if (v->track_offset[t + 1] < v->track_offset[t]) return "track_offset must be non-decreasing";
+        }
+        for (int c = 0; c < v->num_cameras; ++c)
+        {
+            for (int k = 0; k < 9; ++k)
+                if (!std::isfinite(v->cam_rotation[9 * c + k])) return "camera pose must be finite";
+            for (int k = 0; k < 3; ++k)
+                if (!std::isfinite(v->cam_center[3 * c + k])) return "camera pose must be finite";
+            const double fx = v->cam_intrinsics[13 * c];
+            if (!std::isfinite(fx) || fx <= 0.0 || !std::isfinite(v->cam_intrinsics[13 * c + 1]))
+                return "camera intrinsics must be finite and positive";
+            if (v->cam_convention && v->cam_convention[c] != PB_BROWN_OPENCV && v->cam_convention[c] != PB_BROWN_METASHAPE)
+                return "camera convention must be OpenCV or Metashape";
+            if (v->cam_group && v->cam_group[c] < 0) return "calibration group ids must be non-negative";
+        }
+        if (v->num_fixed_cameras > 0 && !v->fixed_cameras) return "fixed_cameras is NULL";
+        if (v->num_fixed_tracks > 0 && !v->fixed_tracks) return "fixed_tracks is NULL";
+        if (v->gauge_policy < 0 || v->gauge_policy > 2) return "gauge policy is invalid";
+        return {};
+    }
+
+    std::string validateOptions(const pb_solve_options* o)
+    {
+        if (!o) return "options are NULL";
+        if (o->max_iterations <= 0) return "max_iterations must be positive";
+        if (o->robust_loss < 0 || o->robust_loss > 2) return "robust loss is invalid";
+        if (!std::isfinite(o->robust_loss_scale) || o->robust_loss_scale <= 0.0) return "robust loss scale must be positive";
+        if (!(o->armijo_coefficient > 0.0) || !(o->armijo_coefficient < 1.0) || o->line_search_steps <= 0 ||
+            o->line_search_steps > 64)
+            return "Armijo parameters must satisfy coefficient in (0,1) and steps in [1,64]";
+        if (!std::isfinite(o->max_initial_track_rms) || o->max_initial_track_rms < 0.0)
+            return "max_initial_track_rms must be finite and non-negative";
+        if (o->shared_intrinsic_mask & ~0x1FFFu) return "shared_intrinsic_mask has bits beyond slot 12";
+        if ((o->shared_intrinsic_mask & 0x1FFEu) && !(o->shared_intrinsic_mask & 1u) && o->shared_intrinsic_mask != (1u << 4))
+            return "refining aspect, principal point or distortion requires refining the shared focal length";
+        if (o->precision != PB_PRECISION_FP64 && o->precision != PB_PRECISION_MIXED) return "precision is invalid";
+        if (o->pcg_relative_tolerance < 0.0 || o->pcg_absolute_tolerance < 0.0 || o->pcg_max_iterations < 0)
+            return "PCG controls must be non-negative";
+        return {};
+    }
+
+    // Builds the prepared structure and uploads it (setup; once per problem and structural options).
+    void ensureStructure(pb_ctx* ctx, const pb_solve_options& o)
+    {
+        if (ctx->structureValid && ctx->structureMask == o.shared_intrinsic_mask &&
+            ctx->structureRefinePose == o.refine_camera_pose && ctx->structureMaxRms == o.max_initial_track_rms)
+            return;
+        ctx->structureValid = false;
+        ctx->dstruct.release();
+        ctx->dwork32.release();
+        ctx->dwork64.release();
+        ctx->work32 = ctx->work64 = false;
+        Prepared p;
+        const std::string gerr = normalizeGauge(ctx->in, o, &p);
+        if (!gerr.empty()) throw std::invalid_argument(gerr);
+        prepareActive(ctx->in, o, &p);
+        prepareGroups(ctx->in, o, &p);
+        Layout L;
+        buildLayout(ctx->in, p, &L);
+        hipStream_t s = ctx->stream;
+        DeviceAlloc& A = ctx->dstruct;
+        pbk::Problem P;
+        P.nCam = ctx->in.nCam;
+        P.nGroup = p.G;
+        P.nObs = L.nObs;
+        P.nActive = L.nActive;
+        P.nElim = L.nElim;
+        P.nEntries = L.nEntries;
+        P.nUnits = p.nUnits;
+        P.nBlocks = L.nBlocks;
+        P.nNnz = L.nNnz;
+        P.nItems = L.nItems;
+        P.nPairs = L.nPairs;
+        P.maxGroupK = L.maxGroupK;
+        P.gaugeAnchor = p.anchor;
+        P.gaugeScale = (p.anchor >= 0 && p.scaleCam >= 0 && p.baseline > 0.0) ? p.scaleCam : -1;
+        P.gaugeBaseline = p.baseline;
+        P.globalDim = p.globalDim;
+        P.camR = A.upload(ctx->in.R, s);
+        P.camC = A.upload(ctx->in.C, s);
+        P.camSlots = A.upload(ctx->in.slots, s);
+        P.refSlots = A.upload(ctx->in.ref, s);
+        P.camConv = A.upload(ctx->in.conv, s);
+        P.camGroup = A.upload(L.camGroup, s);
+        P.camUnit = A.upload(L.camUnit, s);
+        P.cams = A.alloc<pbdev::Camera>(P.nCam);
+        P.camObsOffset = A.upload(L.camObsOffset, s);
+        P.camObs = A.upload(L.camObs, s);
+        P.camDiagBlock = A.upload(L.camDiagBlock, s);
+        P.camGroupBlock = A.upload(L.camGroupBlock, s);
+        P.groupParams = A.upload(p.gParams, s);
+        P.groupPrior = A.upload(p.gPrior, s);
+        P.groupInvSigma = A.upload(p.gInvSigma, s);
+        P.groupLower = A.upload(p.gLower, s);
+        P.groupUpper = A.upload(p.gUpper, s);
+        P.groupEnabled = A.upload(p.gEnabled, s);
+        P.groupActive = A.upload(p.gActive, s);
+        P.groupK = A.upload(p.gK, s);
+        P.groupUnit = A.upload(p.gUnit, s);
+        P.groupCamOffset = A.upload(L.groupCamOffset, s);
+        P.groupCams = A.upload(L.groupCams, s);
+        P.groupPairBlock = A.upload(L.groupPairBlock, s);
+        P.slotToCompact = A.upload(p.slotToCompact, s);
+        P.obsCam = A.upload(L.obsCam, s);
+        P.obsPoint = A.upload(L.obsPoint, s);
+        P.obsUV = A.upload(L.obsUV, s);
+        P.obsW = A.upload(L.obsW, s);
+        P.obsPoseEntry = A.upload(L.obsPoseEntry, s);
+        P.obsGroupEntry = A.upload(L.obsGroupEntry, s);
+        P.pointX = A.upload(L.pointX, s);
+        P.obsOffset = A.upload(L.obsOffset, s);
+        P.pointElim = A.upload(L.pointElim, s);
+        P.elimTrack = A.upload(L.elimTrack, s);
+        P.entryOffset = A.upload(L.entryOffset, s);
+        P.entryUnit = A.upload(L.entryUnit, s);
+        P.entryPoint = A.upload(L.entryPoint, s);
+        P.blockRow = A.upload(L.blockRow, s);
+        P.blockCol = A.upload(L.blockCol, s);
+        P.blockPosUp = A.upload(L.blockPosUp, s);
+        P.blockPosLo = A.upload(L.blockPosLo, s);
+        P.blockItemOffset = A.upload(L.blockItemOffset, s);
+        P.itemBlock = A.upload(L.itemBlock, s);
+        P.itemBegin = A.upload(L.itemBegin, s);
+        P.pairE1 = A.upload(L.pairE1, s);
+        P.pairE2 = A.upload(L.pairE2, s);
+        P.rowPtr = A.upload(L.rowPtr, s);
+        P.colIdx = A.upload(L.colIdx, s);
+        P.diagPos = A.upload(L.diagPos, s);
+        P.unitWidth = A.upload(p.unitWidth, s);
+        check(hipStreamSynchronize(s), "structure upload");
+        L.pairE1.clear();
+        L.pairE1.shrink_to_fit();
+        L.pairE2.clear();
+        L.pairE2.shrink_to_fit();
+        ctx->P = P;
+        ctx->prep = std::move(p);
+        ctx->lay = std::move(L);
+        ctx->structureValid = true;
+        ctx->structureMask = o.shared_intrinsic_mask;
+        ctx->structureRefinePose = o.refine_camera_pose;
+        ctx->structureMaxRms = o.max_initial_track_rms;
+        ctx->costBlocks = std::max(1, std::min(2048, (P.nObs + 255) / 256));
+    }
+
+    template <typename T>
+    void allocWork(pb_ctx* ctx, DeviceAlloc& A, pbk::Work<T>& W)
+    {
+        const pbk::Problem& P = ctx->P;
+        const int nbUnits = std::max(1, (P.nUnits + 255) / 256);
+        const int nbSpmv = std::max(1, (P.nUnits + 3) / 4);
+        const int nbElim = std::max(1, (P.nElim + 255) / 256);
+        const int nParts = std::max(std::max(nbUnits, nbSpmv), nbElim);
+        W.W = A.alloc<T>(size_t(P.nEntries) * 18);
+        W.Hinv = A.alloc<T>(size_t(P.nElim) * 9);
+        W.bp = A.alloc<double>(size_t(P.nElim) * 3);
+        W.y = A.alloc<double>(size_t(P.nElim) * 3);
+        W.dp = A.alloc<double>(size_t(P.nElim) * 3);
+        W.D = A.alloc<double>(size_t(P.nBlocks) * 36);
+        W.directRhs = A.alloc<double>(size_t(P.nUnits) * 6);
+        W.rhs = A.alloc<double>(size_t(P.nUnits) * 6);
+        W.camCost = A.alloc<double>(P.nCam);
+        W.camGroupPartial = A.alloc<double>(size_t(P.nCam) * pbk::kGroupPartialStride);
+        W.groupPriorCost = A.alloc<double>(std::max(1, P.nGroup));
+        W.itemPartial = A.alloc<T>(size_t(P.nItems) * 36);
+        W.itemPartialRhs = A.alloc<double>(size_t(P.nItems) * 6);
+        W.S = A.alloc<T>(size_t(P.nNnz) * 36);
+        W.Minv = A.alloc<T>(size_t(P.nUnits) * 36);
+        W.x = A.alloc<T>(size_t(P.nUnits) * 6);
+        W.r = A.alloc<T>(size_t(P.nUnits) * 6);
+        W.z = A.alloc<T>(size_t(P.nUnits) * 6);
+        W.p = A.alloc<T>(size_t(P.nUnits) * 6);
+        W.q = A.alloc<T>(size_t(P.nUnits) * 6);
+        W.partPq = A.alloc<double>(nParts);
+        W.partRz = A.alloc<double>(nParts);
+        W.partRr = A.alloc<double>(nParts);
+        W.partDd = A.alloc<double>(nParts);
+        W.partSs = A.alloc<double>(nParts);
+        W.scalars = A.alloc<double>(pbk::kScalarCount);
+        W.flags = A.alloc<int32_t>(pbk::kFlagCount);
+        W.iflags = A.alloc<int32_t>(pbk::kIflagCount);
+        W.candGroupParams = A.alloc<double>(size_t(pbk::kMaxAlpha) * std::max(1, P.nGroup) * 13);
+        W.candPriorCost = A.alloc<double>(size_t(pbk::kMaxAlpha) * std::max(1, P.nGroup));
+        W.candR = A.alloc<double>(size_t(pbk::kMaxAlpha) * P.nCam * 9);
+        W.candC = A.alloc<double>(size_t(pbk::kMaxAlpha) * P.nCam * 3);
+        W.candCams = A.alloc<pbdev::Camera>(size_t(pbk::kMaxAlpha) * P.nCam);
+        W.alphas = A.alloc<double>(pbk::kMaxAlpha);
+        W.trialPart = A.alloc<double>(size_t(ctx->costBlocks) * pbk::kMaxAlpha);
+        W.trialBad = A.alloc<int32_t>(size_t(ctx->costBlocks) * pbk::kMaxAlpha);
+        W.trialCost = A.alloc<double>(pbk::kMaxAlpha);
+        check(hipMemsetAsync(W.camGroupPartial, 0, sizeof(double) * size_t(P.nCam) * pbk::kGroupPartialStride,
+                             ctx->stream),
+              "memset");
+    }
+
+    template <typename T>
+    pbk::Work<T>& work(pb_ctx* ctx);
+    template <>
+    pbk::Work<float>& work<float>(pb_ctx* ctx)
+    {
+        if (!ctx->work32)
+        {
+            allocWork(ctx, ctx->dwork32, ctx->W32);
+            ctx->work32 = true;
+        }
+        return ctx->W32;
+    }
+    template <>
+    pbk::Work<double>& work<double>(pb_ctx* ctx)
+    {
+        if (!ctx->work64)
+        {
+            allocWork(ctx, ctx->dwork64, ctx->W64);
+            ctx->work64 = true;
+        }
+        return ctx->W64;
+    }
+
+    // Resets the device state (cameras, groups, points) to the uploaded initial state.
+    void resetState(pb_ctx* ctx)
+    {
+        hipStream_t s = ctx->stream;
+        const pbk::Problem& P = ctx->P;
+        check(hipMemcpyAsync(P.camR, ctx->in.R.data(), sizeof(double) * ctx->in.R.size(), hipMemcpyHostToDevice, s), "state");
+        check(hipMemcpyAsync(P.camC, ctx->in.C.data(), sizeof(double) * ctx->in.C.size(), hipMemcpyHostToDevice, s), "state");
+        if (!ctx->prep.gParams.empty())
+            check(hipMemcpyAsync(P.groupParams, ctx->prep.gParams.data(), sizeof(double) * ctx->prep.gParams.size(),
+                                 hipMemcpyHostToDevice, s),
+                  "state");
+        if (!ctx->lay.pointX.empty())
+            check(hipMemcpyAsync(P.pointX, ctx->lay.pointX.data(), sizeof(double) * ctx->lay.pointX.size(),
+                                 hipMemcpyHostToDevice, s),
+                  "state");
+    }
+
+    struct BuildResult
+    {
+        double cost = 0.0;
+        bool invalid = false, singular = false, notSpd = false;
+    };
+
+    template <typename T>
+    BuildResult build(pb_ctx* ctx, pbk::Work<T>& W, double damping)
+    {
+        using Lr = pbk::Launcher<T>;
+        hipStream_t s = ctx->stream;
+        const pbk::Problem& P = ctx->P;
+        check(hipMemsetAsync(W.flags, 0, sizeof(int32_t) * pbk::kFlagCount, s), "memset");
+        check(hipMemsetAsync(W.D, 0, sizeof(double) * size_t(std::max(1, P.nBlocks)) * 36, s), "memset");
+        check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
+        Lr::prepareCameras(P, s);
+        {
+            Span sp(ctx, 0);
+            Lr::linearizePoints(P, W, damping, s);
+        }
+        {
+            Span sp(ctx, 1);
+            Lr::cameraDirect(P, W, s);
+        }
+        {
+            Span sp(ctx, 2);
+            Lr::schurBlocks(P, W, damping, s);
+        }
+        double* h = ctx->pinned;
+        check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s), "readback");
+        check(hipMemcpyAsync(h + 16, W.flags, sizeof(int32_t) * pbk::kFlagCount, hipMemcpyDeviceToHost, s), "readback");
+        check(hipStreamSynchronize(s), "build");
+        const int32_t* f = reinterpret_cast<const int32_t*>(h + 16);
+        BuildResult r;
+        r.cost = h[pbk::kScalarCost];
+        r.invalid = f[pbk::kFlagInvalid] != 0;
+        r.singular = f[pbk::kFlagSingular] != 0;
+        r.notSpd = f[pbk::kFlagNotSpd] != 0;
+        return r;
+    }
+
+    struct PcgResult
+    {
+        bool converged = false, breakdown = false;
+        int iterations = 0;
+        double rnorm = 0.0;
+    };
+
+    template <typename T>
+    PcgResult pcg(pb_ctx* ctx, pbk::Work<T>& W, double relTol, double absTol, int maxIter, int checkEvery)
+    {
+        using Lr = pbk::Launcher<T>;
+        hipStream_t s = ctx->stream;
+        const pbk::Problem& P = ctx->P;
+        PcgResult res;
+        Lr::pcgStart(P, W, relTol, absTol, s);
+        double* h = ctx->pinned;
+        int32_t* hf = reinterpret_cast<int32_t*>(h + 24);
+        int it = 0;
+        while (true)
+        {
+            check(hipMemcpyAsync(hf, W.iflags, sizeof(int32_t) * pbk::kIflagCount, hipMemcpyDeviceToHost, s), "readback");
+            check(hipMemcpyAsync(h + 28, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
+                  "readback");
+            check(hipStreamSynchronize(s), "pcg");
+            if (hf[pbk::kIflagDone] || it >= maxIter) break;
+            const int chunk = std::min(checkEvery, maxIter - it);
+            for (int k = 0; k < chunk; ++k, ++it)
+            {
+                {
+                    Span sp(ctx, 3);
+                    Lr::pcgSpmv(P, W, s);
+                }
+                Lr::pcgUpdate(P, W, it, s);
+            }
+        }
+        res.breakdown = hf[pbk::kIflagBreakdown] != 0;
+        res.iterations = hf[pbk::kIflagIters];
+        res.rnorm = h[28 + pbk::kScalarRnorm];
+        res.converged = !res.breakdown && res.rnorm <= h[28 + pbk::kScalarThreshold];
+        return res;
+    }
+
+    void setMessage(pb_solve_report* r, const std::string& m)
+    {
+        std::snprintf(r->message, sizeof(r->message), "%s", m.c_str());
+    }
+
+    // The online-Schur controller (BundleAdjustPlaMatrix.cpp:162-631 with useReferenceOnlineSchur).
+    template <typename T>
+    void runSolve(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel)
+    {
+        using Lr = pbk::Launcher<T>;
+        pbk::Work<T>& W = work<T>(ctx);
+        hipStream_t s = ctx->stream;
+        const pbk::Problem& P = ctx->P;
+        const Prepared& pr = ctx->prep;
+        resetState(ctx);
+        check(hipStreamSynchronize(s), "reset");
+        const bool mixed = std::is_same<T, float>::value;
+        const double relTol = o.pcg_relative_tolerance > 0.0 ? o.pcg_relative_tolerance : (mixed ? 1e-3 : 2e-10);
+        const double absTol = o.pcg_absolute_tolerance > 0.0 ? o.pcg_absolute_tolerance : 1e-12;
+        const int maxPcg = o.pcg_max_iterations > 0 ? o.pcg_max_iterations : std::max(200, 30 * pr.primaryBlocks);
+        const int checkEvery = o.pcg_check_interval > 0 ? o.pcg_check_interval : 8;
+        const int batch = o.armijo_batch > 0 ? std::min(o.armijo_batch, pbk::kMaxAlpha) : 4;
+        rep->linear_tolerance = relTol;
+        rep->mixed_precision_used = mixed ? 1 : 0;
+        ctx->lastScalarBytes = sizeof(T);
+        rep->online_schur_used = 1;
+
+        const auto t0 = Clock::now();
+        double current = std::numeric_limits<double>::quiet_NaN();
+        double damping = 0.0;
+        int accepted = 0, rejected = 0, iterations = 0;
+        bool converged = pr.primaryBlocks == 0 && pr.trackBlocks == 0;
+        bool cancelled = false;
+        auto isCancelled = [&]() { return cancel && *cancel != 0; };
+        auto raise = [&]() { damping = damping == 0.0 ? 1e-3 : damping * 10.0; };
+        if (converged)
+        {
+            const BuildResult b = build(ctx, W, 0.0);
+            current = b.cost;
+            rep->initial_cost = current;
+            ++rep->objective_evaluations;
+        }
+        const int budget = std::max(1, o.max_iterations);
+        bool terminate = false;
+        double alphas[pbk::kMaxAlpha];
+        while (iterations < budget && !converged && !terminate)
+        {
+            if (isCancelled())
+            {
+                cancelled = true;
+                break;
+            }
+            bool relinearize = false;
+            while (iterations < budget && !converged && !relinearize)
+            {
+                if (isCancelled())
+                {
+                    cancelled = true;
+                    break;
+                }
+                auto ta = Clock::now();
+                const BuildResult b = build(ctx, W, damping);
+                rep->assembly_seconds += seconds(ta);
+                ++rep->linearizations;
+                ++rep->objective_evaluations;
+                current = b.cost;
+                if (rep->linearizations == 1) rep->initial_cost = current;
+                if (b.invalid) throw std::runtime_error("online Schur linearisation produced an invalid projection");
+                if (b.singular || b.notSpd)
+                {
+                    // singular point block, or a reduced diagonal block that is not positive definite
+                    // (a retryable "not positive definite" failure of the linear solve)
+                    ++iterations;
+                    ++rejected;
+                    raise();
+                    continue;
+                }
+                auto tl = Clock::now();
+                const PcgResult lin = pcg(ctx, W, relTol, absTol, maxPcg, checkEvery);
+                rep->linear_solve_seconds += seconds(tl);
+                rep->linear_iterations += lin.iterations;
+                if (lin.breakdown)
+                {
+                    ++iterations;
+                    ++rejected;
+                    raise();
+                    continue;
+                }
+                ++iterations;
+                if (!lin.converged)
+                {
+                    ++rejected;
+                    raise();
+                    continue;
+                }
+                auto tb = Clock::now();
+                {
+                    Span sp(ctx, 5);
+                    Lr::backSubstitute(P, W, s);
+                }
+                double* h = ctx->pinned;
+                check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
+                      "readback");
+                check(hipStreamSynchronize(s), "back-substitution");
+                rep->back_substitution_seconds += seconds(tb);
+                const double dd = h[pbk::kScalarDd];
+                const double rmsG = h[pbk::kScalarRmsGlobal], rmsP = h[pbk::kScalarRmsPoint];
+                if (!std::isfinite(dd))
+                {
+                    ++rejected;
+                    raise();
+                    continue;
+                }
+                // Armijo backtracking; several step lengths per cost pass, decided in sequential order
+                auto tobj = Clock::now();
+                double alpha = 1.0, cand = kInf;
+                bool armijo = false;
+                int acceptedIndex = -1;
+                int ls = 0;
+                bool firstPass = true;
+                while (ls < o.line_search_steps && !armijo)
+                {
+                    const int n = std::min(firstPass ? 1 : batch, o.line_search_steps - ls);
+                    firstPass = false;
+                    for (int k = 0; k < n; ++k) alphas[k] = alpha * std::ldexp(1.0, -k);
+                    check(hipMemcpyAsync(W.alphas, alphas, sizeof(double) * n, hipMemcpyHostToDevice, s), "alphas");
+                    {
+                        Span sp(ctx, 4);
+                        Lr::trialCosts(P, W, W.alphas, n, ctx->costBlocks, s);
+                    }
+                    check(hipMemcpyAsync(h, W.trialCost, sizeof(double) * n, hipMemcpyDeviceToHost, s), "readback");
+                    check(hipStreamSynchronize(s), "trial cost");
+                    for (int k = 0; k < n; ++k)
+                    {
+                        ++rep->objective_evaluations;
+                        ++ls;
+                        cand = h[k];
+                        const double limit = current - o.armijo_coefficient * alphas[k] * dd;
+                        if (std::isfinite(cand) && cand <= limit && cand < current)
+                        {
+                            armijo = true;
+                            acceptedIndex = k;
+                            alpha = alphas[k];
+                            break;
+                        }
+                    }
+                    if (!armijo) alpha = alphas[n - 1] * 0.5;
+                }
+                rep->objective_seconds += seconds(tobj);
+                if (armijo && std::isfinite(cand) && cand < current)
+                {
+                    Lr::accept(P, W, acceptedIndex, alpha, s);
+                    current = cand;
+                    ++accepted;
+                    damping /= 10.0;
+                    converged = alpha * rmsG <= 1e-6 && alpha * rmsP <= 1e-6;
+                    relinearize = !converged;
+                }
+                else
+                {
+                    ++rejected;
+                    terminate = true;
+                    relinearize = true;
+                }
+            }
+        }
+        check(hipStreamSynchronize(s), "solve");
+        rep->solve_seconds = seconds(t0);
+        rep->accepted_steps = accepted;
+        rep->rejected_steps = rejected;
+        rep->iterations = iterations;
+        rep->final_cost = current;
+        collectProfile(ctx);
+        if (cancelled)
+        {
+            rep->solve_status = PB_SOLVE_CANCELLED;
+            rep->solution_usable = 0;
+            setMessage(rep, "cancelled; no intermediate solution published");
+            ctx->haveResult = false;
+            return;
+        }
+        rep->converged = converged ? 1 : 0;
+        rep->solve_status = converged ? PB_SOLVE_SUCCESS : PB_SOLVE_NO_CONVERGENCE;
+        rep->solution_usable = std::isfinite(current) ? 1 : 0;
+        setMessage(rep, converged ? "converged (online Schur + block-Jacobi PCG + Armijo)"
+                                  : "iteration budget or line search exhausted");
+        // refined state back to the host
+        ctx->outR.resize(ctx->in.R.size());
+        ctx->outC.resize(ctx->in.C.size());
+        ctx->outGroupParams.resize(pr.gParams.size());
+        ctx->outPointX.resize(ctx->lay.pointX.size());
+        check(hipMemcpyAsync(ctx->outR.data(), P.camR, sizeof(double) * ctx->outR.size(), hipMemcpyDeviceToHost, s), "download");
+        check(hipMemcpyAsync(ctx->outC.data(), P.camC, sizeof(double) * ctx->outC.size(), hipMemcpyDeviceToHost, s), "download");
+        if (!ctx->outGroupParams.empty())
+            check(hipMemcpyAsync(ctx->outGroupParams.data(), P.groupParams, sizeof(double) * ctx->outGroupParams.size(),
+                                 hipMemcpyDeviceToHost, s),
+                  "download");
+        if (!ctx->outPointX.empty())
+            check(hipMemcpyAsync(ctx->outPointX.data(), P.pointX, sizeof(double) * ctx->outPointX.size(),
+                                 hipMemcpyDeviceToHost, s),
+                  "download");
+        check(hipStreamSynchronize(s), "download");
+        ctx->haveResult = true;
+    }
+
+    void fillReportCounts(const pb_ctx* ctx, pb_solve_report* r)
+    {
+        const Prepared& p = ctx->prep;
+        r->observation_count = p.observations;
+        r->active_track_count = p.activeTracks;
+        r->rejected_initial_tracks = p.rejectedInitial;
+        r->camera_block_count = p.cameraBlocks;
+        r->intrinsic_block_count = p.intrinsicBlocks;
+        r->track_block_count = p.trackBlocks;
+        r->primary_block_count = p.primaryBlocks;
+        r->gauge_anchor_camera = p.anchor;
+        r->gauge_scale_camera = p.scaleCam;
+        r->gauge_baseline = p.baseline;
+        std::snprintf(r->device_name, sizeof(r->device_name), "%s", ctx->deviceName.c_str());
+    }
+
+    pb_status fail(pb_ctx* ctx, pb_status st, const std::string& msg)
+    {
+        if (ctx) ctx->error = msg;
+        return st;
+    }
+
+    template <typename F>
+    pb_status guarded(pb_ctx* ctx, F&& f)
+    {
+        try
+        {
+            return f();
+        }
+        catch (const std::bad_alloc&)
+        {
+            return fail(ctx, PB_ERR_OOM, "out of memory");
+        }
+        catch (const HipError& e)
+        {
+            return fail(ctx, e.code == hipErrorOutOfMemory ? PB_ERR_OOM : PB_ERR_DEVICE, e.what());
+        }
+        catch (const std::invalid_argument& e)
+        {
+            const std::string m = e.what();
+            return fail(ctx, m.rfind("unsupported", 0) == 0 ? PB_ERR_UNSUPPORTED : PB_ERR_INVALID, m);
+        }
+        catch (const std::exception& e)
+        {
+            return fail(ctx, PB_ERR_NUMERIC, e.what());
+        }
+        catch (...)
+        {
+            return fail(ctx, PB_ERR_NUMERIC, "unknown error");
+        }
+    }
+
+    // Unit -> (reference 13-padded block, slot) mapping for dense exports.
+    void unitSlots(const pb_ctx* ctx, int u, int* block, int slots[6])
+    {
+        const Prepared& p = ctx->prep;
+        for (int a = 0; a < 6; ++a) slots[a] = -1;
+        if (u < p.cameraBlocks)
+        {
+            *block = u;
+            for (int a = 0; a < 6; ++a) slots[a] = a;
+            return;
+        }
+        for (int g = 0; g < p.G; ++g)
+        {
+            if (p.gK[g] == 0) continue;
+            const int m = (p.gK[g] + 5) / 6;
+            if (u >= p.gUnit[g] && u < p.gUnit[g] + m)
+            {
+                *block = p.cameraBlocks + g;
+                const int j = u - p.gUnit[g];
+                for (int a = 0; a < 6; ++a)
+                    if (6 * j + a < p.gK[g]) slots[a] = p.gActive[pbdev::kMaxActive * g + 6 * j + a];
+                return;
+            }
+        }
+        *block = -1;
+    }
+
+    template <typename T>
+    pb_status exportReduced(pb_ctx* ctx, const pb_solve_options& o, double damping, double* matrix, double* rhs,
+                            double* cost, int32_t* primaryBlocks)
+    {
+        pbk::Work<T>& W = work<T>(ctx);
+        resetState(ctx);
+        const BuildResult b = build(ctx, W, damping);
+        if (b.invalid) return fail(ctx, PB_ERR_NUMERIC, "invalid projection at the initial state");
+        if (cost) *cost = b.cost;
+        const int P13 = ctx->prep.primaryBlocks * 13;
+        if (primaryBlocks) *primaryBlocks = ctx->prep.primaryBlocks;
+        const pbk::Problem& P = ctx->P;
+        std::vector<T> S(size_t(P.nNnz) * 36);
+        std::vector<double> r(size_t(P.nUnits) * 6);
+        if (!S.empty())
+            check(hipMemcpy(S.data(), W.S, sizeof(T) * S.size(), hipMemcpyDeviceToHost), "export");
+        if (!r.empty()) check(hipMemcpy(r.data(), W.rhs, sizeof(double) * r.size(), hipMemcpyDeviceToHost), "export");
+        if (matrix)
+        {
+            if (static_cast<long>(P13) * P13 > (1l << 26)) return fail(ctx, PB_ERR_UNSUPPORTED, "reduced system too large to export");
+            std::fill(matrix, matrix + size_t(P13) * P13, 0.0);
+            for (int i = 0; i < P13; ++i) matrix[size_t(i) * P13 + i] = 1.0;
+            for (int u = 0; u < P.nUnits; ++u)
+            {
+                int bu, su[6];
+                unitSlots(ctx, u, &bu, su);
+                for (int k = ctx->lay.rowPtr[u]; k < ctx->lay.rowPtr[u + 1]; ++k)
+                {
+                    const int v = ctx->lay.colIdx[k];
+                    int bv, sv[6];
+                    unitSlots(ctx, v, &bv, sv);
+                    for (int a = 0; a < 6; ++a)
+                    {
+                        if (su[a] < 0) continue;
+                        for (int c = 0; c < 6; ++c)
+                        {
+                            if (sv[c] < 0) continue;
+                            matrix[size_t(bu * 13 + su[a]) * P13 + bv * 13 + sv[c]] = double(S[size_t(k) * 36 + a * 6 + c]);
+                        }
+                    }
+                }
+            }
+        }
+        if (rhs)
+        {
+            std::fill(rhs, rhs + P13, 0.0);
+            for (int u = 0; u < P.nUnits; ++u)
+            {
+                int bu, su[6];
+                unitSlots(ctx, u, &bu, su);
+                for (int a = 0; a < 6; ++a)
+                    if (su[a] >= 0) rhs[bu * 13 + su[a]] = r[u * 6 + a];
+            }
+        }
+        return PB_OK;
+    }
+}  // namespace
+
+extern "C" {
+
+void pb_default_options(pb_solve_options* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->max_iterations = 20;
+    o->robust_loss = PB_LOSS_LEAST_SQUARES;
+    o->robust_loss_scale = 1.0;
+    o->armijo_coefficient = 1e-3;
+    o->line_search_steps = 34;
+    o->use_online_schur = 1;
+    o->max_initial_track_rms = 100.0;
+    o->refine_camera_pose = 1;
+    o->shared_intrinsic_mask = 0;
+    o->min_focal_scale = 0.5;
+    o->max_focal_scale = 4.0;
+    o->min_aspect_scale = 0.85;
+    o->max_aspect_scale = 1.18;
+    o->max_principal_offset_fraction = 0.08;
+    o->focal_prior_sigma = 0.35;
+    o->principal_prior_sigma_fraction = 0.04;
+    o->aspect_prior_sigma = 0.08;
+    const double mx[9] = {0.35, 0.35, 0.35, 0.02, 0.02, 0.10, 0.35, 0.35, 0.35};
+    const double sg[9] = {0.15, 0.15, 0.15, 0.01, 0.01, 0.02, 0.15, 0.15, 0.15};
+    for (int i = 0; i < 9; ++i)
+    {
+        o->max_abs[i] = mx[i];
+        o->prior_sigma[i] = sg[i];
+    }
+    o->low_order_distortion_scale = 1.0;
+    o->device = 0;
+    o->precision = PB_PRECISION_FP64;
+}
+
+int pb_hip_available(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+pb_status pb_hip_create(int device, pb_ctx** out)
+{
+    if (!out) return PB_ERR_INVALID;
+    *out = nullptr;
+    pb_ctx* ctx = nullptr;
+    try
+    {
+        ctx = new pb_ctx();
+    }
+    catch (const std::bad_alloc&)
+    {
+        return PB_ERR_OOM;
+    }
+    const pb_status st = guarded(ctx, [&]() -> pb_status {
+        if (!pb_hip_available(device)) return fail(ctx, PB_ERR_DEVICE, "no gfx950 device at this index");
+        ctx->device = device;
+        check(hipSetDevice(device), "hipSetDevice");
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        ctx->deviceName = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+        check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        void* p = nullptr;
+        check(hipHostMalloc(&p, 4096, hipHostMallocDefault), "hipHostMalloc");
+        ctx->pinned = static_cast<double*>(p);
+        return PB_OK;
+    });
+    if (st != PB_OK)
+    {
+        pb_hip_destroy(ctx);
+        return st;
+    }
+    *out = ctx;
+    return PB_OK;
+}
+
+pb_status pb_hip_destroy(pb_ctx* ctx)
+{
+    if (!ctx) return PB_OK;
+    if (ctx->stream)
+    {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    ctx->dstruct.release();
+    ctx->dwork32.release();
+    ctx->dwork64.release();
+    for (hipEvent_t e : ctx->eventPool) (void)hipEventDestroy(e);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PB_OK;
+}
+
+pb_status pb_hip_upload(pb_ctx* ctx, const pb_problem_view* problem)
+{
+    if (!ctx) return PB_ERR_INVALID;
+    const std::string err = validateInput(problem);
+    if (!err.empty()) return fail(ctx, PB_ERR_INVALID, err);
+    return guarded(ctx, [&]() -> pb_status {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        copyIn(problem, &ctx->in);
+        ctx->structureValid = false;
+        ctx->haveResult = false;
+        ctx->uploaded = true;
+        return PB_OK;
+    });
+}
+
+pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_report* report,
+                       const volatile int32_t* cancel)
+{
+    if (!ctx || !report) return PB_ERR_INVALID;
+    std::memset(report, 0, sizeof(*report));
+    report->gauge_anchor_camera = report->gauge_scale_camera = -1;
+    const std::string oerr = validateOptions(options);
+    if (!oerr.empty())
+    {
+        report->solve_status = PB_SOLVE_INVALID_INPUT;
+        setMessage(report, oerr);
+        return fail(ctx, PB_ERR_INVALID, oerr);
+    }
+    if (!options->use_online_schur)
+    {
+        report->solve_status = PB_SOLVE_UNSUPPORTED;
+        setMessage(report, "the MI355X backend implements the online point-Schur path only");
+        return fail(ctx, PB_ERR_UNSUPPORTED, report->message);
+    }
+    if (!ctx->uploaded)
+    {
+        report->solve_status = PB_SOLVE_INVALID_INPUT;
+        setMessage(report, "no problem uploaded");
+        return fail(ctx, PB_ERR_INVALID, "no problem uploaded");
+    }
+    ctx->profile = options->profile_kernels != 0;
+    const pb_status st = guarded(ctx, [&]() -> pb_status {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (cancel && *cancel)
+        {
+            report->solve_status = PB_SOLVE_CANCELLED;
+            setMessage(report, "cancelled before start");
+            ctx->haveResult = false;
+            return PB_ERR_CANCELLED;
+        }
+        const auto ts = Clock::now();
+        ensureStructure(ctx, *options);
+        fillReportCounts(ctx, report);
+        ctx->P.loss = options->robust_loss;
+        ctx->P.lossScale = options->robust_loss_scale;
+        if (ctx->in.nCam == 0 || ctx->in.nTracks == 0)
+        {
+            report->solve_status = PB_SOLVE_INVALID_INPUT;
+            setMessage(report, "empty cameras or tracks");
+            return PB_ERR_INVALID;
+        }
+        if (ctx->prep.observations == 0)
+        {
+            report->solve_status = PB_SOLVE_INVALID_INPUT;
+            setMessage(report, "no usable tracks");
+            return PB_ERR_INVALID;
+        }
+        if (options->precision == PB_PRECISION_MIXED)
+            (void)work<float>(ctx);
+        else
+            (void)work<double>(ctx);
+        check(hipStreamSynchronize(ctx->stream), "setup");
+        report->setup_seconds = seconds(ts);
+        if (options->precision == PB_PRECISION_MIXED)
+            runSolve<float>(ctx, *options, report, cancel);
+        else
+            runSolve<double>(ctx, *options, report, cancel);
+        if (report->solve_status == PB_SOLVE_CANCELLED) return PB_ERR_CANCELLED;
+        return PB_OK;
+    });
+    if (st != PB_OK && st != PB_ERR_CANCELLED && report->solve_status == PB_SOLVE_NOT_RUN)
+    {
+        report->solve_status = st == PB_ERR_UNSUPPORTED  ? PB_SOLVE_UNSUPPORTED
+                               : st == PB_ERR_INVALID    ? PB_SOLVE_INVALID_INPUT
+                               : st == PB_ERR_DEVICE     ? PB_SOLVE_BACKEND_UNAVAILABLE
+                                                         : PB_SOLVE_NUMERICAL_FAILURE;
+        setMessage(report, ctx->error);
+    }
+    if (st == PB_ERR_NUMERIC)
+    {
+        report->solve_status = PB_SOLVE_NUMERICAL_FAILURE;
+        setMessage(report, "numerical failure: " + ctx->error);
+        ctx->haveResult = false;
+    }
+    return st;
+}
+
+pb_status pb_hip_download(pb_ctx* ctx, double* R, double* C, double* intr, double* X, int32_t* valid)
+{
+    if (!ctx || !ctx->uploaded) return PB_ERR_INVALID;
+    const Input& in = ctx->in;
+    const bool res = ctx->haveResult;
+    const Prepared& p = ctx->prep;
+    for (int c = 0; c < in.nCam; ++c)
+    {
+        if (R) std::copy_n(res ? &ctx->outR[9 * c] : &in.R[9 * c], 9, R + 9 * c);
+        if (C) std::copy_n(res ? &ctx->outC[3 * c] : &in.C[3 * c], 3, C + 3 * c);
+        if (intr)
+        {
+            // publishIntrinsics (BundleAdjustPlaMatrixModel.cpp:271-352) without transition priors
+            double pub[13];
+            std::copy_n(&in.slots[13 * c], 13, pub);
+            if (res && p.G > 0 && p.intrinsicBlock[c] >= 0)
+            {
+                const int g = p.groupOfCam[c];
+                const uint32_t en = p.gEnabled[g];
+                const double* gp = &ctx->outGroupParams[13 * g];
+                const double* ref = &in.ref[13 * c];
+                pub[0] = (en & 1u) ? gp[0] : in.slots[13 * c];
+                pub[1] = (en & 2u) ? gp[1] : in.slots[13 * c + 1];
+                pub[2] = (en & 4u) ? ref[2] + gp[2] : in.slots[13 * c + 2];
+                pub[3] = (en & 8u) ? ref[3] + gp[3] : in.slots[13 * c + 3];
+                for (int k = 4; k < 13; ++k)
+                    if ((en >> k) & 1u) pub[k] = gp[k];
+            }
+            std::copy_n(pub, 13, intr + 13 * c);
+        }
+    }
+    std::vector<int32_t> activeIndex(in.nTracks, -1);
+    if (ctx->structureValid)
+        for (size_t a = 0; a < ctx->lay.activeTrackOf.size(); ++a) activeIndex[ctx->lay.activeTrackOf[a]] = int32_t(a);
+    for (int t = 0; t < in.nTracks; ++t)
+    {
+        const int a = activeIndex[t];
+        if (X) std::copy_n((res && a >= 0) ? &ctx->outPointX[3 * a] : &in.X[3 * t], 3, X + 3 * t);
+        if (valid) valid[t] = (ctx->structureValid && p.activeTrack[t]) ? 1 : 0;
+    }
+    return PB_OK;
+}
+
+pb_status pb_hip_get_masks(pb_ctx* ctx, int32_t* cameraBlock, int32_t* intrinsicBlock, int32_t* groupOfCam,
+                           int32_t* trackBlock, int32_t* activeTrack)
+{
+    if (!ctx || !ctx->structureValid) return fail(ctx, PB_ERR_INVALID, "no prepared problem (solve first)");
+    const Prepared& p = ctx->prep;
+    for (int c = 0; c < ctx->in.nCam; ++c)
+    {
+        if (cameraBlock) cameraBlock[c] = p.cameraBlock[c];
+        if (intrinsicBlock) intrinsicBlock[c] = p.intrinsicBlock[c];
+        if (groupOfCam) groupOfCam[c] = p.groupOfCam[c];
+    }
+    for (int t = 0; t < ctx->in.nTracks; ++t)
+    {
+        if (trackBlock) trackBlock[t] = p.trackBlock[t];
+        if (activeTrack) activeTrack[t] = p.activeTrack[t];
+    }
+    return PB_OK;
+}
+
+pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, double* total_ms,
+                              double* bytes_per_launch)
+{
+    if (!ctx || !name) return PB_ERR_INVALID;
+    for (int f = 0; f < kFamilyCount; ++f)
+    {
+        if (std::strcmp(kFamilies[f], name) != 0) continue;
+        if (launches) *launches = ctx->fam[f].launches;
+        if (total_ms) *total_ms = ctx->fam[f].ms;
+        if (bytes_per_launch)
+        {
+            // algorithmic bytes: each array element the kernel family must touch, counted once
+            const pbk::Problem& P = ctx->P;
+            const double t = ctx->lastScalarBytes;
+            double b = 0.0;
+            switch (f)
+            {
+            case 0:  // obs (cam, uv, w, two entry ids) + point X + W out + Hinv/bp/y out
+                b = P.nObs * (4.0 + 16 + 8 + 8) + P.nActive * 4.0 * 2 + P.nElim * (24.0 + 4 + 8 + 9 * t + 48) +
+                    P.nEntries * 18 * t;
+                break;
+            case 1:  // camera permutation + obs (point id, uv, w) + point X
+                b = P.nObs * (4.0 + 4 + 16 + 8) + P.nActive * 24.0;
+                break;
+            case 2:  // pairs + entry points + W + Hinv + y + S out
+                b = P.nPairs * 8.0 + P.nEntries * (4.0 + 18 * t) + P.nElim * (9 * t + 24) + P.nNnz * 36 * t;
+                break;
+            case 3:  // S + column ids + vectors
+                b = P.nNnz * (36 * t + 4) + P.nUnits * 6 * t * 6;
+                break;
+            case 4:  // obs (cam, point, uv, w) + point X, dp, elim id
+                b = P.nObs * (4.0 + 4 + 16 + 8) + P.nActive * (24.0 + 4) + P.nElim * 24.0;
+                break;
+            case 5:  // entries W + units + point bp/Hinv + dp out
+                b = P.nEntries * (18 * t + 4) + P.nElim * (24.0 + 9 * t + 24 + 8);
+                break;
+            default:
+                break;
+            }
+            *bytes_per_launch = b;
+        }
+        return PB_OK;
+    }
+    return fail(ctx, PB_ERR_INVALID, std::string("unknown kernel family ") + name);
+}
+
+pb_status pb_hip_reduced_system(pb_ctx* ctx, const pb_solve_options* options, double damping, double* matrix,
+                                double* rhs, double* cost, int32_t* primary_blocks)
+{
+    if (!ctx || !ctx->uploaded) return PB_ERR_INVALID;
+    const std::string oerr = validateOptions(options);
+    if (!oerr.empty()) return fail(ctx, PB_ERR_INVALID, oerr);
+    return guarded(ctx, [&]() -> pb_status {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        ensureStructure(ctx, *options);
+        ctx->P.loss = options->robust_loss;
+        ctx->P.lossScale = options->robust_loss_scale;
+        ctx->profile = false;
+        if (options->precision == PB_PRECISION_MIXED)
+            return exportReduced<float>(ctx, *options, damping, matrix, rhs, cost, primary_blocks);
+        return exportReduced<double>(ctx, *options, damping, matrix, rhs, cost, primary_blocks);
+    });
+}
+
+pb_status pb_hip_evaluate_cost(pb_ctx* ctx, const pb_solve_options* options, double* cost)
+{
+    if (!cost) return PB_ERR_INVALID;
+    return pb_hip_reduced_system(ctx, options, 0.0, nullptr, nullptr, cost, nullptr);
+}
+
+const char* pb_hip_last_error(const pb_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+}  // extern "C"

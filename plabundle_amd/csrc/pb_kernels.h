@@ -1,0 +1,187 @@
+// pb_kernels.h - device data layout of one uploaded problem and the kernel launchers.
+//
+// Layout (all arrays device-resident, allocated once per problem and reused across solves):
+//   units      6-wide primary parameter groups: one per camera pose block (unit id = reference
+//              camera block id), then ceil(K/6) per intrinsic group with K active slots
+//   entries    per eliminated point, one 6x3 cross block W = sum w J_unit^T Jp per distinct unit
+//   pairs      (entry, entry) pairs of one point, sorted by reduced-system block; the Schur
+//              complement of a block is the sum over its pair run
+//   S          reduced system as block-sparse rows of 6x6 blocks (both triangles stored)
+#pragma once
+
+#include "pb_device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pbk
+{
+    constexpr int kMaxAlpha = 8;             // step lengths evaluated per trial-cost pass
+    constexpr int kGroupPartialStride = 96;  // >= K(K+1)/2 + K for K <= 12
+    constexpr int kPairChunk = 2048;         // pairs per Schur work item
+
+    enum Flag
+    {
+        kFlagInvalid = 0,   // an observation failed to project during linearisation
+        kFlagSingular = 1,  // a damped point block failed the 3x3 inverse
+        kFlagNotSpd = 2,    // a 6x6 diagonal block of the reduced system is not SPD
+        kFlagCount = 4
+    };
+
+    enum Iflag
+    {
+        kIflagDone = 0,
+        kIflagIters = 1,
+        kIflagBreakdown = 2,
+        kIflagCount = 4
+    };
+
+    enum Scalar
+    {
+        kScalarCost = 0,
+        kScalarRz0,
+        kScalarRz1,
+        kScalarBnorm,
+        kScalarThreshold,
+        kScalarRnorm,
+        kScalarDd,
+        kScalarRmsGlobal,
+        kScalarRmsPoint,
+        kScalarCount = 16
+    };
+
+    struct Problem
+    {
+        int nCam = 0, nGroup = 0, nObs = 0, nActive = 0, nElim = 0, nEntries = 0;
+        int nUnits = 0, nBlocks = 0, nNnz = 0, nItems = 0;
+        long nPairs = 0;
+        int maxGroupK = 0;
+        int loss = 0;
+        double lossScale = 1.0;
+        int gaugeAnchor = -1, gaugeScale = -1;
+        double gaugeBaseline = 0.0;
+        int globalDim = 0;
+
+        // cameras
+        double* camR = nullptr;  // state, camera-to-world [nCam*9]
+        double* camC = nullptr;  // state, centres [nCam*3]
+        const double* camSlots = nullptr;
+        const double* refSlots = nullptr;
+        const int32_t* camConv = nullptr;
+        const int32_t* camGroup = nullptr;  // group index when intrinsic blocks exist, else -1
+        const int32_t* camUnit = nullptr;   // pose unit or -1
+        pbdev::Camera* cams = nullptr;      // effective cameras of the current pass
+        const int32_t* camObsOffset = nullptr;
+        const int32_t* camObs = nullptr;
+        const int32_t* camDiagBlock = nullptr;
+        const int32_t* camGroupBlock = nullptr;  // [nCam*2]
+
+        // intrinsic groups
+        double* groupParams = nullptr;  // [nGroup*13]
+        const double* groupPrior = nullptr;
+        const double* groupInvSigma = nullptr;
+        const double* groupLower = nullptr;
+        const double* groupUpper = nullptr;
+        const uint32_t* groupEnabled = nullptr;
+        const int8_t* groupActive = nullptr;  // [nGroup*12] compact active slot list
+        const int32_t* groupK = nullptr;
+        const int32_t* groupUnit = nullptr;
+        const int32_t* groupCamOffset = nullptr;
+        const int32_t* groupCams = nullptr;
+        const int32_t* groupPairBlock = nullptr;  // [nGroup*4] (unit i, unit j) block ids, i <= j < 2
+        const int32_t* slotToCompact = nullptr;   // [nGroup*13]
+
+        // observations (usable observations of active tracks, track order)
+        const int32_t* obsCam = nullptr;
+        const int32_t* obsPoint = nullptr;  // active track index
+        const double* obsUV = nullptr;
+        const double* obsW = nullptr;       // weight / scale^2
+        const int32_t* obsPoseEntry = nullptr;
+        const int32_t* obsGroupEntry = nullptr;
+
+        // active tracks
+        double* pointX = nullptr;  // [nActive*3]
+        const int32_t* obsOffset = nullptr;  // [nActive+1]
+        const int32_t* pointElim = nullptr;  // eliminated index or -1 (fixed track)
+        const int32_t* elimTrack = nullptr;  // [nElim] active track index
+
+        // cross-block entries
+        const int32_t* entryOffset = nullptr;  // [nElim+1]
+        const int32_t* entryUnit = nullptr;
+        const int32_t* entryPoint = nullptr;  // eliminated index
+
+        // reduced system structure
+        const int32_t* blockRow = nullptr;  // upper blocks (row <= col)
+        const int32_t* blockCol = nullptr;
+        const int32_t* blockPosUp = nullptr;  // position of (row, col) in the block-sparse rows
+        const int32_t* blockPosLo = nullptr;  // position of (col, row)
+        const int32_t* blockItemOffset = nullptr;
+        const int32_t* itemBlock = nullptr;
+        const int32_t* itemBegin = nullptr;  // [nItems+1] pair ranges
+        const int32_t* pairE1 = nullptr;
+        const int32_t* pairE2 = nullptr;
+        const int32_t* rowPtr = nullptr;
+        const int32_t* colIdx = nullptr;
+        const int32_t* diagPos = nullptr;
+        const int32_t* unitWidth = nullptr;  // active columns of a unit (6, or the group remainder)
+    };
+
+    template <typename T>
+    struct Work
+    {
+        T* W = nullptr;       // [nEntries*18]
+        T* Hinv = nullptr;    // [nElim*9]
+        double* bp = nullptr; // [nElim*3]
+        double* y = nullptr;  // Hinv * bp
+        double* dp = nullptr; // point step
+        double* D = nullptr;  // direct part per upper block [nBlocks*36]
+        double* directRhs = nullptr;  // [nUnits*6]
+        double* rhs = nullptr;        // reduced right-hand side
+        double* camCost = nullptr;
+        double* camGroupPartial = nullptr;  // [nCam*kGroupPartialStride]
+        double* groupPriorCost = nullptr;
+        T* itemPartial = nullptr;       // [nItems*36]
+        double* itemPartialRhs = nullptr;  // [nItems*6]
+        T* S = nullptr;     // [nNnz*36]
+        T* Minv = nullptr;  // [nUnits*36]
+        T* x = nullptr;
+        T* r = nullptr;
+        T* z = nullptr;
+        T* p = nullptr;
+        T* q = nullptr;
+        double* partPq = nullptr;
+        double* partRz = nullptr;
+        double* partRr = nullptr;
+        double* partDd = nullptr;
+        double* partSs = nullptr;
+        double* scalars = nullptr;
+        int32_t* flags = nullptr;
+        int32_t* iflags = nullptr;
+        double* candGroupParams = nullptr;  // [kMaxAlpha*nGroup*13]
+        double* candPriorCost = nullptr;
+        double* candR = nullptr;
+        double* candC = nullptr;
+        pbdev::Camera* candCams = nullptr;  // [kMaxAlpha*nCam]
+        double* alphas = nullptr;           // [kMaxAlpha]
+        double* trialPart = nullptr;        // [costBlocks*kMaxAlpha]
+        int32_t* trialBad = nullptr;
+        double* trialCost = nullptr;        // [kMaxAlpha]
+    };
+
+    template <typename T>
+    struct Launcher
+    {
+        static void prepareCameras(const Problem& P, hipStream_t s);
+        static void linearizePoints(const Problem& P, const Work<T>& W, double damping, hipStream_t s);
+        static void cameraDirect(const Problem& P, const Work<T>& W, hipStream_t s);
+        static void schurBlocks(const Problem& P, const Work<T>& W, double damping, hipStream_t s);
+        static int pcgStart(const Problem& P, const Work<T>& W, double relTol, double absTol, hipStream_t s);
+        static void pcgSpmv(const Problem& P, const Work<T>& W, hipStream_t s);
+        static void pcgUpdate(const Problem& P, const Work<T>& W, int it, hipStream_t s);
+        static void backSubstitute(const Problem& P, const Work<T>& W, hipStream_t s);
+        static void trialCosts(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, int costBlocks,
+                               hipStream_t s);
+        static void accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s);
+    };
+}  // namespace pbk

@@ -1,0 +1,205 @@
+"""Python host mirror of plabundle::Solver for the MI355X backend.
+
+Solver.solve(problem, options, workspace) follows include/plabundle/solver.h:34-54:
+the problem is immutable, the result is returned by value and owns the refined
+state, the workspace is caller-owned and must not be shared by concurrent
+solves, and errors travel in Result.status plus a message (only a missing HIP
+library raises, because there is no silent CPU path here).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .options import Backend, Precision, SolveOptions, to_c
+from .problem import Problem
+
+
+class SolveStatus(enum.IntEnum):
+    """plabundle::SolveStatus (include/plabundle/backend.h:17-27)."""
+
+    NotRun = 0
+    Success = 1
+    NoConvergence = 2
+    Cancelled = 3
+    InvalidInput = 4
+    UnsupportedConfiguration = 5
+    BackendUnavailable = 6
+    NumericalFailure = 7
+
+
+@dataclass
+class Result:
+    requestedBackend: Backend = Backend.Mi355x
+    usedBackend: Backend = Backend.Mi355x
+    status: SolveStatus = SolveStatus.NotRun
+    solutionUsable: bool = False
+    usedGpu: bool = False
+    backendMessage: str = ""
+    observationCount: int = 0
+    report: dict = field(default_factory=dict)
+    cam_rotation: np.ndarray | None = None
+    cam_center: np.ndarray | None = None
+    cam_intrinsics: np.ndarray | None = None
+    track_point: np.ndarray | None = None
+    track_valid: np.ndarray | None = None
+    abi_status: int = _abi.PB_OK
+
+    def usable(self) -> bool:
+        return self.solutionUsable and self.status in (SolveStatus.Success, SolveStatus.NoConvergence)
+
+    @property
+    def initialCost(self) -> float:
+        return self.report.get("initial_cost", float("nan"))
+
+    @property
+    def finalCost(self) -> float:
+        return self.report.get("final_cost", float("nan"))
+
+
+class SolverWorkspace:
+    """Reusable device workspace (the SolverWorkspace analogue, include/plabundle/solver.h:13-32)."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+        self._lib = _abi.load_library()
+        self._ctx = C.c_void_p()
+        self._uploaded_id = None
+        st = self._lib.pb_hip_create(device, C.byref(self._ctx))
+        if st != _abi.PB_OK:
+            self._ctx = None
+            raise RuntimeError(f"pb_hip_create failed: {_abi.STATUS_NAMES.get(st, st)}")
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def last_error(self) -> str:
+        return self._lib.pb_hip_last_error(self._ctx).decode("utf-8", "replace") if self._ctx else "closed"
+
+    def upload(self, problem: Problem) -> int:
+        view, keep = _abi.make_view(problem)
+        st = self._lib.pb_hip_upload(self._ctx, C.byref(view))
+        del keep
+        if st == _abi.PB_OK:
+            self._uploaded_id = id(problem)
+        return st
+
+    def clear(self):
+        self._uploaded_id = None
+
+    def close(self):
+        if self._ctx:
+            self._lib.pb_hip_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- inspection entry points used by parity tests -------------------------------------
+    def reduced_system(self, copts, damping: float, primary_blocks: int):
+        n = primary_blocks * 13
+        A = np.zeros((n, n))
+        b = np.zeros(n)
+        cost = C.c_double()
+        P = C.c_int32()
+        st = self._lib.pb_hip_reduced_system(self._ctx, C.byref(copts), float(damping), _abi.ptr(A, C.c_double),
+                                             _abi.ptr(b, C.c_double), C.byref(cost), C.byref(P))
+        if st != _abi.PB_OK:
+            raise RuntimeError(f"pb_hip_reduced_system: {self.last_error()}")
+        return A, b, cost.value
+
+    def evaluate_cost(self, copts) -> float:
+        cost = C.c_double()
+        st = self._lib.pb_hip_evaluate_cost(self._ctx, C.byref(copts), C.byref(cost))
+        if st != _abi.PB_OK:
+            raise RuntimeError(f"pb_hip_evaluate_cost: {self.last_error()}")
+        return cost.value
+
+    def masks(self, problem: Problem) -> dict:
+        nc, nt = problem.num_cameras, problem.num_tracks
+        out = {k: np.zeros(n, dtype=np.int32) for k, n in (
+            ("camera_block", nc), ("intrinsic_block_by_camera", nc), ("calibration_group_by_camera", nc),
+            ("track_block", nt), ("active_track", nt))}
+        st = self._lib.pb_hip_get_masks(self._ctx, *(_abi.ptr(out[k], C.c_int32) for k in (
+            "camera_block", "intrinsic_block_by_camera", "calibration_group_by_camera", "track_block",
+            "active_track")))
+        if st != _abi.PB_OK:
+            raise RuntimeError(f"pb_hip_get_masks: {self.last_error()}")
+        return out
+
+    def kernel_stats(self, name: str) -> dict:
+        launches = C.c_int64()
+        ms = C.c_double()
+        nbytes = C.c_double()
+        st = self._lib.pb_hip_kernel_stats(self._ctx, name.encode(), C.byref(launches), C.byref(ms), C.byref(nbytes))
+        if st != _abi.PB_OK:
+            raise RuntimeError(self.last_error())
+        return {"launches": launches.value, "total_ms": ms.value, "bytes_per_launch": nbytes.value}
+
+
+class Solver:
+    """plabundle::Solver restricted to the MI355X backend."""
+
+    @staticmethod
+    def isBackendAvailable(backend: Backend = Backend.Mi355x, deviceIndex: int = 0) -> bool:
+        if backend != Backend.Mi355x:
+            return False
+        try:
+            lib = _abi.load_library()
+        except _abi.BackendLibraryMissing:
+            return False
+        return bool(lib.pb_hip_available(int(deviceIndex)))
+
+    def solve(self, problem: Problem, options: SolveOptions | None = None, workspace: SolverWorkspace | None = None,
+              precision: Precision | None = None, cancel=None) -> Result:
+        options = options or SolveOptions()
+        result = Result(requestedBackend=options.backend.requested, usedBackend=Backend.Mi355x)
+        if options.backend.requested not in (Backend.Mi355x, Backend.Auto):
+            result.status = SolveStatus.UnsupportedConfiguration
+            result.backendMessage = "this package provides the MI355X backend only"
+            return result
+        lib = _abi.load_library()  # raises when the HIP library is missing: no silent CPU fallback
+        if not lib.pb_hip_available(int(options.backend.plaMatrixDevice)):
+            result.status = SolveStatus.BackendUnavailable
+            result.backendMessage = f"no gfx950 device at index {options.backend.plaMatrixDevice}"
+            return result
+        ws = workspace or SolverWorkspace(options.backend.plaMatrixDevice)
+        if ws._uploaded_id != id(problem):
+            st = ws.upload(problem)
+            if st != _abi.PB_OK:
+                result.status = SolveStatus.InvalidInput
+                result.backendMessage = ws.last_error()
+                result.abi_status = st
+                return result
+        copts = to_c(options, precision)
+        rep = _abi.SolveReportC()
+        flag = None
+        if cancel is not None:
+            flag = np.array([1 if cancel else 0], dtype=np.int32)
+        st = lib.pb_hip_solve(ws.handle, C.byref(copts), C.byref(rep), _abi.ptr(flag, C.c_int32))
+        result.abi_status = st
+        result.report = rep.as_dict()
+        result.status = SolveStatus(rep.solve_status)
+        result.solutionUsable = bool(rep.solution_usable)
+        result.backendMessage = result.report["message"]
+        result.observationCount = rep.observation_count
+        result.usedGpu = st in (_abi.PB_OK,)
+        nc, nt = problem.num_cameras, problem.num_tracks
+        R = np.zeros((nc, 9))
+        Cc = np.zeros((nc, 3))
+        I = np.zeros((nc, 13))
+        X = np.zeros((nt, 3))
+        V = np.zeros(nt, dtype=np.int32)
+        lib.pb_hip_download(ws.handle, _abi.ptr(R, C.c_double), _abi.ptr(Cc, C.c_double), _abi.ptr(I, C.c_double),
+                            _abi.ptr(X, C.c_double), _abi.ptr(V, C.c_int32))
+        result.cam_rotation, result.cam_center, result.cam_intrinsics = R, Cc, I
+        result.track_point, result.track_valid = X, V
+        return result

@@ -1,0 +1,83 @@
+"""CPU checks of the C ABI: the library loads, exports every function include/pb_hip.h declares,
+and the ctypes structures match the C layouts (compiled here with gcc)."""
+import os
+import re
+import subprocess
+import tempfile
+
+import ctypes as C
+import pytest
+
+from plabundle_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pb_hip.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:pb_status|void|int|const char\*)\s+(pb_\w+)\s*\(", text, re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_abi.LIBRARY_PATH):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "plabundle_amd", "csrc")], check=True)
+    return _abi.load_library()
+
+
+def test_header_functions_are_exported(lib):
+    declared = _declared_functions()
+    assert set(declared) == set(_abi.EXPORTED_SYMBOLS)
+    out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIBRARY_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\sT\s(pb_\w+)$", out, re.M))
+    assert set(declared) <= exported
+    for name in declared:
+        assert hasattr(lib, name)
+
+
+def test_struct_layouts_match_header():
+    src = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "pb_hip.h"
+int main(void) {
+  printf("%zu %zu %zu\n", sizeof(pb_problem_view), sizeof(pb_solve_options), sizeof(pb_solve_report));
+  printf("%zu %zu %zu\n", offsetof(pb_problem_view, gauge_baseline), offsetof(pb_solve_options, profile_kernels),
+         offsetof(pb_solve_report, message));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        lines = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    sizes = [int(x) for x in lines[0].split()]
+    offs = [int(x) for x in lines[1].split()]
+    assert sizes == [C.sizeof(_abi.ProblemView), C.sizeof(_abi.SolveOptionsC), C.sizeof(_abi.SolveReportC)]
+    assert offs == [_abi.ProblemView.gauge_baseline.offset, _abi.SolveOptionsC.profile_kernels.offset,
+                    _abi.SolveReportC.message.offset]
+
+
+def test_default_options_are_the_reference_defaults(lib):
+    from oracle import oracle as O
+
+    o = _abi.default_options(lib)
+    ref = O.default_options()
+    for name, _ in _abi.SolveOptionsC._fields_:
+        a, b = getattr(o, name), getattr(ref, name)
+        if hasattr(a, "__len__"):
+            assert list(a) == list(b), name
+        elif name not in ("precision",):
+            assert a == b, name
+
+
+def test_invalid_calls_fail_cleanly_without_a_device(lib):
+    # no context -> invalid; a missing device is reported, never a crash
+    assert lib.pb_hip_upload(None, None) == _abi.PB_ERR_INVALID
+    assert lib.pb_hip_destroy(None) == _abi.PB_OK
+    ctx = C.c_void_p()
+    if not lib.pb_hip_available(0):
+        assert lib.pb_hip_create(0, C.byref(ctx)) == _abi.PB_ERR_DEVICE

@@ -1,0 +1,169 @@
+"""GPU parity of the MI355X backend against the FP64 oracle, through the C ABI.
+
+Bars: integer block numbering bit-exact; FP64 path reduced systems within 1e-10 of the
+oracle's (relative to the matrix scale) and final costs within 1e-9 relative with the same
+iteration count; mixed path final cost within 1e-6 relative.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from plabundle_amd import _abi
+from plabundle_amd import options as PO
+from plabundle_amd import synthetic as S
+from plabundle_amd.solver import Solver, SolverWorkspace, SolveStatus
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ws():
+    lib = _abi.load_library()
+    assert lib.pb_hip_available(0), "no gfx950 device"
+    w = SolverWorkspace(0)
+    yield w
+    w.close()
+
+
+def _ring(intrinsics=False, n=10, pts=200, seed=7):
+    p = S.ring_problem(seed=seed, num_cameras=n, num_points=pts, max_views=6)
+    o = S.ring_options(PO.SolveOptions(), max_iterations=10)
+    if intrinsics:
+        o.calibration.refineSharedFocalLength = True
+        o.calibration.refineSharedRadialDistortion = True
+    return p, o
+
+
+def _aerial(n=(20, 20, 20), pts=4000, seed=2):
+    p = S.aerial_problem(seed=seed, cameras_per_strip=n, num_points=pts)
+    o = S.aerial_options(PO.SolveOptions())
+    return p, o
+
+
+@pytest.mark.parametrize("intrinsics", [False, True])
+def test_masks_bit_exact(ws, oracle_mod, intrinsics):
+    _say(f"case masks intrinsics={intrinsics}")
+    p, o = _ring(intrinsics)
+    orc = oracle_mod.Oracle(p, threads=2)
+    orc.prepare(PO.to_c(o))
+    ws.upload(p)
+    ws.evaluate_cost(PO.to_c(o, PO.Precision.Fp64))
+    a, b = ws.masks(p), orc.masks()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+@pytest.mark.parametrize("intrinsics", [False, True])
+@pytest.mark.parametrize("damping", [0.0, 1e-2])
+def test_reduced_system_fp64(ws, oracle_mod, intrinsics, damping):
+    _say(f"case reduced fp64 intrinsics={intrinsics} damping={damping}")
+    p, o = _ring(intrinsics)
+    co = PO.to_c(o, PO.Precision.Fp64)
+    orc = oracle_mod.Oracle(p, threads=2)
+    prep = orc.prepare(co)
+    P = prep["primary_block_count"]
+    A0, b0, c0, _ = orc.reduced_system(P, damping)
+    ws.upload(p)
+    A, b, c = ws.reduced_system(co, damping, P)
+    scale = np.abs(A0).max()
+    np.testing.assert_allclose(A, A0, atol=1e-10 * scale, rtol=0)
+    np.testing.assert_allclose(b, b0, atol=1e-10 * np.abs(b0).max(), rtol=0)
+    assert abs(c - c0) <= 1e-12 * c0
+
+
+def test_reduced_system_mixed(ws, oracle_mod):
+    _say("case reduced mixed")
+    p, o = _ring(True)
+    co = PO.to_c(o, PO.Precision.Mixed)
+    orc = oracle_mod.Oracle(p, threads=2)
+    prep = orc.prepare(co)
+    P = prep["primary_block_count"]
+    A0, b0, c0, _ = orc.reduced_system(P, 0.0)
+    ws.upload(p)
+    A, b, c = ws.reduced_system(co, 0.0, P)
+    np.testing.assert_allclose(A, A0, atol=2e-5 * np.abs(A0).max(), rtol=0)
+    # the Schur right-hand side correction sums FP32 cross blocks: ~1e-7 relative per term
+    np.testing.assert_allclose(b, b0, atol=1e-7 * np.abs(b0).max(), rtol=0)
+    assert abs(c - c0) <= 1e-12 * c0  # cost stays FP64
+
+
+def test_native_three_camera_scene(ws, oracle_mod):
+    _say("case native three cameras")
+    p = S.native_three_cameras()
+    o = PO.SolveOptions()
+    o.solver.maxIterations = 8
+    r = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
+    assert r.report["online_schur_used"] == 1 and r.usable(), r.backendMessage
+    assert r.finalCost < r.initialCost
+    ref = oracle_mod.Oracle(p, threads=1).solve(PO.to_c(o))
+    assert r.report["iterations"] == ref["iterations"]
+    np.testing.assert_allclose(r.cam_center[2], [2.0, 0.0, 0.0], atol=1e-6)
+
+
+@pytest.mark.parametrize("intrinsics", [False, True])
+def test_full_solve_fp64_matches_oracle(ws, oracle_mod, intrinsics):
+    _say(f"case solve fp64 ring intrinsics={intrinsics}")
+    p, o = _ring(intrinsics, n=16, pts=500)
+    r = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
+    orc = oracle_mod.Oracle(p, threads=4)
+    ref = orc.solve(PO.to_c(o))
+    st = orc.state()
+    assert r.status == SolveStatus(ref["solve_status"])
+    assert r.report["iterations"] == ref["iterations"]
+    assert r.report["accepted_steps"] == ref["accepted_steps"]
+    assert abs(r.finalCost - ref["final_cost"]) <= 1e-9 * ref["final_cost"]
+    np.testing.assert_allclose(r.cam_center, st["cam_center"], atol=1e-7)
+    np.testing.assert_allclose(r.track_point, st["track_point"], atol=1e-6)
+    np.testing.assert_allclose(r.cam_intrinsics, st["cam_intrinsics"], rtol=1e-7, atol=1e-9)
+    np.testing.assert_array_equal(r.track_valid, st["track_valid"])
+
+
+def test_full_solve_mixed_within_north_star_tolerance(ws, oracle_mod):
+    _say("case solve mixed aerial-small")
+    p, o = _aerial()
+    r = Solver().solve(p, o, ws, precision=PO.Precision.Mixed)
+    ref = oracle_mod.Oracle(p, threads=4).solve(PO.to_c(o))
+    assert r.usable(), r.backendMessage
+    rel = abs(r.finalCost - ref["final_cost"]) / ref["final_cost"]
+    _say(f"mixed rel cost error {rel:.3e} iterations {r.report['iterations']} vs {ref['iterations']}")
+    assert rel <= 1e-6
+
+
+def test_aerial_fp64_matches_oracle(ws, oracle_mod):
+    _say("case solve fp64 aerial-small")
+    p, o = _aerial()
+    r = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
+    ref = oracle_mod.Oracle(p, threads=4).solve(PO.to_c(o))
+    rel = abs(r.finalCost - ref["final_cost"]) / ref["final_cost"]
+    _say(f"fp64 rel cost error {rel:.3e}")
+    assert r.report["iterations"] == ref["iterations"]
+    assert rel <= 1e-9
+
+
+def test_cancel_publishes_nothing(ws):
+    _say("case cancel")
+    p, o = _ring()
+    r = Solver().solve(p, o, ws, precision=PO.Precision.Fp64, cancel=True)
+    assert r.status == SolveStatus.Cancelled and not r.usable()
+    np.testing.assert_array_equal(r.cam_center, p.cam_center)
+    np.testing.assert_array_equal(r.track_point, p.track_point)
+
+
+def test_invalid_problem_rejected(ws):
+    _say("case invalid")
+    p, o = _ring()
+    bad = p.copy()
+    bad.track_offset = bad.track_offset.copy()
+    bad.track_offset[0] = 1
+    assert ws.upload(bad) == _abi.PB_ERR_INVALID
+    p2 = p.copy()
+    p2.fixed_cameras = np.array([0], dtype=np.int32)
+    r = Solver().solve(p2, o, ws, precision=PO.Precision.Fp64)
+    assert r.status == SolveStatus.UnsupportedConfiguration
