@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=2, help="LM iterations of the CPU oracle sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="reduced problem for quick checks (not a bench line)")
+    ap.add_argument("--mixed-rtol", type=float, default=0.0, help="PCG relative tolerance of the mixed path (0 = default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this backend runs on one MI355X; "
@@ -73,6 +74,8 @@ def main():
     def solve(precision, profile):
         o = S.aerial_options(PO.SolveOptions())
         o.backend.profileKernels = profile
+        if precision == PO.Precision.Mixed and args.mixed_rtol > 0:
+            o.backend.pcgRelativeTolerance = args.mixed_rtol
         t0 = time.perf_counter()
         r = solver.solve(problem, o, ws, precision=precision)
         wall = time.perf_counter() - t0
@@ -92,7 +95,8 @@ def main():
         log(f"step {i}: mixed {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
             f"solve {r.report['solve_seconds']:.4f}s, cost {r.finalCost:.10g}")
     stats = {}
-    for fam in ("linearize_points", "camera_direct", "schur_blocks", "pcg_spmv", "cost", "back_substitution"):
+    for fam in ("linearize_points", "camera_direct", "schur_blocks", "pcg_spmv", "pcg_solve", "cost",
+                "back_substitution"):
         stats[fam] = ws.kernel_stats(fam)
 
     fp64 = []

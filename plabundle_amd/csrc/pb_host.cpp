@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -699,7 +700,7 @@ namespace
     };
 
     const char* const kFamilies[] = {"linearize_points", "camera_direct", "schur_blocks", "pcg_spmv", "cost",
-                                     "back_substitution"};
+                                     "back_substitution", "pcg_solve"};
     constexpr int kFamilyCount = sizeof(kFamilies) / sizeof(kFamilies[0]);
 }  // namespace
 
@@ -731,7 +732,10 @@ struct pb_ctx
     bool haveResult = false;
     // profiling
     bool profile = false;
+    bool noPersistent = false;
+    int64_t pcgIterationsProfiled = 0;
     double lastScalarBytes = 8.0;
+    std::vector<int32_t> pcgRowStart;
     std::array<Family, kFamilyCount> fam;
     std::vector<hipEvent_t> eventPool;
     size_t eventNext = 0;
@@ -907,6 +911,33 @@ namespace
         P.gaugeScale = (p.anchor >= 0 && p.scaleCam >= 0 && p.baseline > 0.0) ? p.scaleCam : -1;
         P.gaugeBaseline = p.baseline;
         P.globalDim = p.globalDim;
+        // persistent PCG: one block per group of unit rows, balanced by stored blocks, at most the
+        // number of co-resident blocks of either precision
+        {
+            const char* force = std::getenv("PB_PCG_MULTIKERNEL");
+            const int co = std::min(pbk::Launcher<float>::pcgPersistentBlocks(), pbk::Launcher<double>::pcgPersistentBlocks());
+            int blocks = std::min(256, std::min(co, std::max(1, p.nUnits / 2)));
+            if (force && force[0] == '1') blocks = 0;
+            ctx->pcgRowStart.clear();
+            if (blocks > 0 && p.nUnits > 0)
+            {
+                const long total = L.nNnz + 2L * p.nUnits;
+                ctx->pcgRowStart.push_back(0);
+                long acc = 0;
+                int b = 1;
+                for (int u = 0; u < p.nUnits; ++u)
+                {
+                    acc += (L.rowPtr[u + 1] - L.rowPtr[u]) + 2;
+                    if (b < blocks && acc >= total * b / blocks && u + 1 < p.nUnits)
+                    {
+                        ctx->pcgRowStart.push_back(u + 1);
+                        ++b;
+                    }
+                }
+                ctx->pcgRowStart.push_back(p.nUnits);
+                P.pcgBlocks = static_cast<int>(ctx->pcgRowStart.size()) - 1;
+            }
+        }
         P.camR = A.upload(ctx->in.R, s);
         P.camC = A.upload(ctx->in.C, s);
         P.camSlots = A.upload(ctx->in.slots, s);
@@ -958,6 +989,7 @@ namespace
         P.colIdx = A.upload(L.colIdx, s);
         P.diagPos = A.upload(L.diagPos, s);
         P.unitWidth = A.upload(p.unitWidth, s);
+        P.pcgRowStart = A.upload(ctx->pcgRowStart, s);
         check(hipStreamSynchronize(s), "structure upload");
         L.pairE1.clear();
         L.pairE1.shrink_to_fit();
@@ -980,7 +1012,7 @@ namespace
         const int nbUnits = std::max(1, (P.nUnits + 255) / 256);
         const int nbSpmv = std::max(1, (P.nUnits + 3) / 4);
         const int nbElim = std::max(1, (P.nElim + 255) / 256);
-        const int nParts = std::max(std::max(nbUnits, nbSpmv), nbElim);
+        const int nParts = std::max(std::max(std::max(nbUnits, nbSpmv), nbElim), 3 * std::max(1, P.pcgBlocks));
         W.W = A.alloc<T>(size_t(P.nEntries) * 18);
         W.Hinv = A.alloc<T>(size_t(P.nElim) * 9);
         W.bp = A.alloc<double>(size_t(P.nElim) * 3);
@@ -1001,6 +1033,8 @@ namespace
         W.z = A.alloc<T>(size_t(P.nUnits) * 6);
         W.p = A.alloc<T>(size_t(P.nUnits) * 6);
         W.q = A.alloc<T>(size_t(P.nUnits) * 6);
+        W.u = A.alloc<T>(size_t(P.nUnits) * 6);
+        W.w = A.alloc<T>(size_t(P.nUnits) * 6);
         W.partPq = A.alloc<double>(nParts);
         W.partRz = A.alloc<double>(nParts);
         W.partRr = A.alloc<double>(nParts);
@@ -1118,9 +1152,33 @@ namespace
         hipStream_t s = ctx->stream;
         const pbk::Problem& P = ctx->P;
         PcgResult res;
-        Lr::pcgStart(P, W, relTol, absTol, s);
         double* h = ctx->pinned;
         int32_t* hf = reinterpret_cast<int32_t*>(h + 24);
+        if (P.pcgBlocks > 0 && !ctx->noPersistent)
+        {
+            hipError_t e;
+            {
+                Span sp(ctx, 6);
+                e = Lr::pcgPersistent(P, W, P.pcgBlocks, maxIter, relTol, absTol, s);
+            }
+            if (e == hipSuccess)
+            {
+                check(hipMemcpyAsync(hf, W.iflags, sizeof(int32_t) * pbk::kIflagCount, hipMemcpyDeviceToHost, s),
+                      "readback");
+                check(hipMemcpyAsync(h + 28, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
+                      "readback");
+                check(hipStreamSynchronize(s), "pcg");
+                res.breakdown = hf[pbk::kIflagBreakdown] != 0;
+                res.iterations = hf[pbk::kIflagIters];
+                res.rnorm = h[28 + pbk::kScalarRnorm];
+                res.converged = !res.breakdown && hf[pbk::kIflagDone] != 0;
+                if (ctx->profile) ctx->pcgIterationsProfiled += res.iterations;
+                return res;
+            }
+            (void)hipGetLastError();
+            ctx->noPersistent = true;  // cooperative launch unavailable: multi-kernel path from now on
+        }
+        Lr::pcgStart(P, W, relTol, absTol, s);
         int it = 0;
         while (true)
         {
@@ -1295,6 +1353,11 @@ namespace
                     if (!armijo) alpha = alphas[n - 1] * 0.5;
                 }
                 rep->objective_seconds += seconds(tobj);
+                if (std::getenv("PB_TRACE"))
+                    std::fprintf(stderr,
+                                 "pb it=%d cost=%.12g cand=%.12g alpha=%g pcg=%d rnorm=%.3e rms_g=%.3e rms_p=%.3e "
+                                 "dd=%.4e lambda=%g\n",
+                                 iterations, current, cand, alpha, lin.iterations, lin.rnorm, rmsG, rmsP, dd, damping);
                 if (armijo && std::isfinite(cand) && cand < current)
                 {
                     Lr::accept(P, W, acceptedIndex, alpha, s);
@@ -1780,6 +1843,13 @@ pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, 
             case 5:  // entries W + units + point bp/Hinv + dp out
                 b = P.nEntries * (18 * t + 4) + P.nElim * (24.0 + 9 * t + 24 + 8);
                 break;
+            case 6:  // per PCG iteration: S + column ids + preconditioner + vectors, times iterations per launch
+            {
+                const double perIt = P.nNnz * (36 * t + 4) + P.nUnits * (36 * t + 12 * 6 * t);
+                const double its = ctx->fam[f].launches ? double(ctx->pcgIterationsProfiled) / ctx->fam[f].launches : 0.0;
+                b = perIt * its;
+                break;
+            }
             default:
                 break;
             }

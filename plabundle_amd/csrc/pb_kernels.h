@@ -125,6 +125,8 @@ namespace pbk
         const int32_t* colIdx = nullptr;
         const int32_t* diagPos = nullptr;
         const int32_t* unitWidth = nullptr;  // active columns of a unit (6, or the group remainder)
+        const int32_t* pcgRowStart = nullptr;  // [pcgBlocks+1] unit rows of each persistent-PCG block
+        int pcgBlocks = 0;
     };
 
     template <typename T>
@@ -150,6 +152,8 @@ namespace pbk
         T* z = nullptr;
         T* p = nullptr;
         T* q = nullptr;
+        T* u = nullptr;  // preconditioned residual (persistent PCG)
+        T* w = nullptr;  // S u (persistent PCG)
         double* partPq = nullptr;
         double* partRz = nullptr;
         double* partRr = nullptr;
@@ -179,6 +183,9 @@ namespace pbk
         static int pcgStart(const Problem& P, const Work<T>& W, double relTol, double absTol, hipStream_t s);
         static void pcgSpmv(const Problem& P, const Work<T>& W, hipStream_t s);
         static void pcgUpdate(const Problem& P, const Work<T>& W, int it, hipStream_t s);
+        static int pcgPersistentBlocks();
+        static hipError_t pcgPersistent(const Problem& P, const Work<T>& W, int blocks, int maxIter, double relTol,
+                                        double absTol, hipStream_t s);
         static void backSubstitute(const Problem& P, const Work<T>& W, hipStream_t s);
         static void trialCosts(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, int costBlocks,
                                hipStream_t s);

@@ -16,6 +16,7 @@
 #include "pb_device.h"
 #include "pb_kernels.h"
 
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 namespace pbk
@@ -747,6 +748,250 @@ namespace pbk
     }
 
     // ---------------------------------------------------------------------------------------------
+    // Persistent preconditioned conjugate gradients (Chronopoulos-Gear form): one SpMV and one pair of
+    // dot products per iteration, two grid barriers per iteration, all iterations in one cooperative
+    // launch. Every block derives the scalars from the same partials in the same order, so all blocks
+    // take identical decisions and execute the same number of barriers; the loop is bounded by maxIter.
+    template <typename T>
+    __device__ __forceinline__ void applyM(const Work<T>& Wk, int u, const T* r, T* out)
+    {
+        const T* M = Wk.Minv + u * 36;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+        {
+            T z = T(0);
+#pragma unroll
+            for (int j = 0; j < 6; ++j) z += M[i * 6 + j] * r[j];
+            out[i] = z;
+        }
+    }
+
+    // w = S u for the rows of this block (wave per row, 60 lanes: row element = lane % 6, block slot = lane / 6);
+    // returns this thread's share of (w.u).
+    template <typename T>
+    __device__ __forceinline__ double spmvRows(const Problem& P, const Work<T>& Wk, int ub, int ue, T (*red)[64])
+    {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        double wu = 0.0;
+        for (int u0 = ub; u0 < ue; u0 += 4)
+        {
+            const int u = u0 + wv;
+            T acc = T(0);
+            if (u < ue && lane < 60)
+            {
+                const int row = lane % 6;
+                for (int k = P.rowPtr[u] + lane / 6; k < P.rowPtr[u + 1]; k += 10)
+                {
+                    const T* Sb = Wk.S + k * 36 + row * 6;
+                    const T* pv = Wk.u + P.colIdx[k] * 6;
+                    acc += Sb[0] * pv[0] + Sb[1] * pv[1] + Sb[2] * pv[2] + Sb[3] * pv[3] + Sb[4] * pv[4] + Sb[5] * pv[5];
+                }
+            }
+            red[wv][lane] = acc;
+            __syncthreads();
+            if (u < ue && lane < 6)
+            {
+                T sum = T(0);
+                for (int q = 0; q < 10; ++q) sum += red[wv][lane + 6 * q];
+                Wk.w[u * 6 + lane] = sum;
+                wu += double(sum) * double(Wk.u[u * 6 + lane]);
+            }
+            __syncthreads();
+        }
+        return wu;
+    }
+
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_pcg_persistent(Problem P, Work<T> Wk, int maxIter, double relTol,
+                                                            double absTol)
+    {
+        namespace cg = cooperative_groups;
+        cg::grid_group grid = cg::this_grid();
+        __shared__ T red[4][64];
+        __shared__ double lds[3][4];
+        __shared__ double bc[4];
+        const int nb = gridDim.x;
+        const int ub = P.pcgRowStart[blockIdx.x], ue = P.pcgRowStart[blockIdx.x + 1];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        // block partial of three values -> Wk.partPq[3 * block + k]
+        auto publish = [&](double a, double b, double c) {
+            a = waveSum(a);
+            b = waveSum(b);
+            c = waveSum(c);
+            if (lane == 0)
+            {
+                lds[0][wv] = a;
+                lds[1][wv] = b;
+                lds[2][wv] = c;
+            }
+            __syncthreads();
+            if (threadIdx.x < 3)
+            {
+                const int k = threadIdx.x;
+                Wk.partPq[3 * blockIdx.x + k] = lds[k][0] + lds[k][1] + lds[k][2] + lds[k][3];
+            }
+            __syncthreads();
+        };
+        // every block sums the partials with the same fixed-order wave reduction, so all blocks hold
+        // bitwise identical scalars
+        auto gather = [&](double* out3) {
+            if (wv == 0)
+            {
+                double a = 0.0, b = 0.0, c = 0.0;
+                for (int i = lane; i < nb; i += 64)
+                {
+                    a += Wk.partPq[3 * i];
+                    b += Wk.partPq[3 * i + 1];
+                    c += Wk.partPq[3 * i + 2];
+                }
+                a = waveSum(a);
+                b = waveSum(b);
+                c = waveSum(c);
+                if (lane == 0)
+                {
+                    bc[0] = a;
+                    bc[1] = b;
+                    bc[2] = c;
+                }
+            }
+            __syncthreads();
+            out3[0] = bc[0];
+            out3[1] = bc[1];
+            out3[2] = bc[2];
+            __syncthreads();
+        };
+
+        // x = 0, r = b, u = M r
+        double ru = 0.0, rr = 0.0;
+        for (int i = threadIdx.x; i < (ue - ub); i += blockDim.x)
+        {
+            const int u = ub + i;
+            T r[6], z[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+            {
+                r[k] = T(Wk.rhs[u * 6 + k]);
+                Wk.r[u * 6 + k] = r[k];
+                Wk.x[u * 6 + k] = T(0);
+            }
+            applyM(Wk, u, r, z);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) Wk.u[u * 6 + k] = z[k];
+        }
+        grid.sync();
+        double wu = spmvRows(P, Wk, ub, ue, red);
+        for (int i = threadIdx.x; i < (ue - ub); i += blockDim.x)
+        {
+            const int u = ub + i;
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+            {
+                const double rv = double(Wk.r[u * 6 + k]);
+                ru += rv * double(Wk.u[u * 6 + k]);
+                rr += rv * rv;
+            }
+        }
+        publish(ru, wu, rr);
+        grid.sync();
+        double sc[3];
+        gather(sc);
+        double gamma = sc[0], delta = sc[1];
+        const double bnorm = sqrt(sc[2]);
+        const double thr = fmax(relTol * bnorm, absTol);
+        double rnorm = bnorm;
+        int iters = 0;
+        bool done = rnorm <= thr;
+        bool breakdown = false;
+        double eta = delta;
+        double alpha = 0.0;
+        if (!done)
+        {
+            breakdown = !(eta > 0.0) || !pbdev::finite(eta);
+            alpha = breakdown ? 0.0 : gamma / eta;
+        }
+        // p = u, s = w
+        for (int i = threadIdx.x; i < (ue - ub) * 6; i += blockDim.x)
+        {
+            const int e = ub * 6 + i;
+            Wk.p[e] = Wk.u[e];
+            Wk.q[e] = Wk.w[e];
+        }
+        while (!done && !breakdown && iters < maxIter)
+        {
+            // x += alpha p; r -= alpha s; u = M r (rows of this block)
+            const T a = T(alpha);
+            for (int i = threadIdx.x; i < (ue - ub); i += blockDim.x)
+            {
+                const int u = ub + i;
+                T r[6], z[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                {
+                    Wk.x[u * 6 + k] += a * Wk.p[u * 6 + k];
+                    r[k] = Wk.r[u * 6 + k] - a * Wk.q[u * 6 + k];
+                    Wk.r[u * 6 + k] = r[k];
+                }
+                applyM(Wk, u, r, z);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) Wk.u[u * 6 + k] = z[k];
+            }
+            grid.sync();
+            wu = spmvRows(P, Wk, ub, ue, red);
+            ru = 0.0;
+            rr = 0.0;
+            for (int i = threadIdx.x; i < (ue - ub); i += blockDim.x)
+            {
+                const int u = ub + i;
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                {
+                    const double rv = double(Wk.r[u * 6 + k]);
+                    ru += rv * double(Wk.u[u * 6 + k]);
+                    rr += rv * rv;
+                }
+            }
+            publish(ru, wu, rr);
+            grid.sync();
+            gather(sc);
+            ++iters;
+            rnorm = sqrt(sc[2]);
+            if (rnorm <= thr)
+            {
+                done = true;
+                break;
+            }
+            const double gnew = sc[0];
+            delta = sc[1];
+            const double beta = gnew / gamma;
+            eta = delta - beta * gnew / alpha;
+            if (!(eta > 0.0) || !pbdev::finite(eta) || !pbdev::finite(beta))
+            {
+                breakdown = true;
+                break;
+            }
+            alpha = gnew / eta;
+            gamma = gnew;
+            // p = u + beta p; s = w + beta s
+            const T b = T(beta);
+            for (int i = threadIdx.x; i < (ue - ub) * 6; i += blockDim.x)
+            {
+                const int e = ub * 6 + i;
+                Wk.p[e] = Wk.u[e] + b * Wk.p[e];
+                Wk.q[e] = Wk.w[e] + b * Wk.q[e];
+            }
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            Wk.iflags[kIflagDone] = done ? 1 : 0;
+            Wk.iflags[kIflagIters] = iters;
+            Wk.iflags[kIflagBreakdown] = breakdown ? 1 : 0;
+            Wk.scalars[kScalarRnorm] = rnorm;
+            Wk.scalars[kScalarThreshold] = thr;
+            Wk.scalars[kScalarBnorm] = bnorm;
+        }
+    }
+
+    // ---------------------------------------------------------------------------------------------
     // Back-substitution dp = Hinv (bp - sum_e W_e^T dc_e); partials of bp.dp and |dp|^2.
     template <typename T>
     __global__ void k_back_substitute(Problem P, Work<T> Wk)
@@ -1106,6 +1351,28 @@ namespace pbk
         const int nbVec = std::max(1, blocksFor(P.nUnits, 256));
         k_pcg_update<T><<<nbVec, 256, 0, s>>>(P, W, it, nbSpmv);
         k_pcg_direction<T><<<nbVec, 256, 0, s>>>(P, W, it, nbVec);
+    }
+
+    template <typename T>
+    int Launcher<T>::pcgPersistentBlocks()
+    {
+        int perCu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_pcg_persistent<T>, 256, 0) != hipSuccess) return 0;
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+        return perCu * cus;
+    }
+
+    template <typename T>
+    hipError_t Launcher<T>::pcgPersistent(const Problem& P, const Work<T>& W, int blocks, int maxIter, double relTol,
+                                          double absTol, hipStream_t s)
+    {
+        Problem Pc = P;
+        Work<T> Wc = W;
+        void* args[] = {&Pc, &Wc, &maxIter, &relTol, &absTol};
+        return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<T>), dim3(blocks), dim3(256),
+                                          args, 0, s);
     }
 
     template <typename T>
