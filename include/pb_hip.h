@@ -165,6 +165,9 @@ typedef struct pb_solve_options {
     int32_t pcg_check_interval;          /* host polls PCG convergence every N iterations (0 = 8) */
     int32_t armijo_batch;                /* trial step lengths evaluated per cost pass (0 = 4) */
     int32_t profile_kernels;             /* 1 = time kernels with HIP events (pb_hip_kernel_stats) */
+    double pcg_model_eta;                /* inexact-Newton stop: end PCG when i*(Q_i - Q_{i-1})/Q_i < eta for the
+                                            quadratic model Q (0 = off; negative = path default: off for FP64,
+                                            0.1 for mixed) */
 } pb_solve_options;
 
 /* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */

@@ -94,6 +94,7 @@ def default_options() -> _abi.SolveOptionsC:
     for i, v in enumerate((0.15, 0.15, 0.15, 0.01, 0.01, 0.02, 0.15, 0.15, 0.15)):
         o.prior_sigma[i] = v
     o.low_order_distortion_scale = 1.0
+    o.pcg_model_eta = -1.0
     return o
 
 

@@ -95,6 +95,7 @@ class SolveOptionsC(C.Structure):
         ("pcg_check_interval", C.c_int32),
         ("armijo_batch", C.c_int32),
         ("profile_kernels", C.c_int32),
+        ("pcg_model_eta", C.c_double),
     ]
 
 

@@ -401,6 +401,7 @@ namespace
         std::vector<int32_t> blockRow, blockCol, blockPosUp, blockPosLo, blockItemOffset;
         std::vector<int32_t> itemBlock, itemBegin, pairE1, pairE2;
         std::vector<int32_t> rowPtr, colIdx, diagPos;
+        std::vector<int32_t> segRow, segBegin, segFirst, rowSegOffset;
     };
 
     void buildLayout(const Input& in, const Prepared& p, Layout* L)
@@ -630,6 +631,22 @@ namespace
             L->rowPtr[u + 1] = static_cast<int32_t>(L->colIdx.size());
         }
         L->nNnz = static_cast<int>(L->colIdx.size());
+        // SpMV segments: long rows (intrinsic units couple with every camera of a group) are split so that
+        // no wave walks more than kSpmvSegment blocks
+        L->rowSegOffset.assign(U + 1, 0);
+        for (int u = 0; u < U; ++u)
+        {
+            bool first = true;
+            for (int k = L->rowPtr[u]; k < L->rowPtr[u + 1]; k += pbk::kSpmvSegment)
+            {
+                L->segRow.push_back(u);
+                L->segBegin.push_back(k);
+                L->segFirst.push_back(first ? 1 : 0);
+                first = false;
+            }
+            L->rowSegOffset[u + 1] = static_cast<int32_t>(L->segRow.size());
+        }
+        L->segBegin.push_back(L->nNnz);
         // direct-part block ids
         L->camDiagBlock.assign(n, -1);
         L->camGroupBlock.assign(2 * n, -1);
@@ -914,10 +931,10 @@ namespace
         // persistent PCG: one block per group of unit rows, balanced by stored blocks, at most the
         // number of co-resident blocks of either precision
         {
-            const char* force = std::getenv("PB_PCG_MULTIKERNEL");
+            const char* force = std::getenv("PB_PCG_PERSISTENT");
             const int co = std::min(pbk::Launcher<float>::pcgPersistentBlocks(), pbk::Launcher<double>::pcgPersistentBlocks());
             int blocks = std::min(256, std::min(co, std::max(1, p.nUnits / 2)));
-            if (force && force[0] == '1') blocks = 0;
+            if (!force || force[0] != '1') blocks = 0;  // the two-launch PCG is the default
             ctx->pcgRowStart.clear();
             if (blocks > 0 && p.nUnits > 0)
             {
@@ -990,6 +1007,11 @@ namespace
         P.diagPos = A.upload(L.diagPos, s);
         P.unitWidth = A.upload(p.unitWidth, s);
         P.pcgRowStart = A.upload(ctx->pcgRowStart, s);
+        P.nSegments = static_cast<int>(L.segRow.size());
+        P.segRow = A.upload(L.segRow, s);
+        P.segBegin = A.upload(L.segBegin, s);
+        P.segFirst = A.upload(L.segFirst, s);
+        P.rowSegOffset = A.upload(L.rowSegOffset, s);
         check(hipStreamSynchronize(s), "structure upload");
         L.pairE1.clear();
         L.pairE1.shrink_to_fit();
@@ -1010,7 +1032,7 @@ namespace
     {
         const pbk::Problem& P = ctx->P;
         const int nbUnits = std::max(1, (P.nUnits + 255) / 256);
-        const int nbSpmv = std::max(1, (P.nUnits + 3) / 4);
+        const int nbSpmv = std::max(1, (P.nSegments + 3) / 4);
         const int nbElim = std::max(1, (P.nElim + 255) / 256);
         const int nParts = std::max(std::max(std::max(nbUnits, nbSpmv), nbElim), 3 * std::max(1, P.pcgBlocks));
         W.W = A.alloc<T>(size_t(P.nEntries) * 18);
@@ -1033,6 +1055,8 @@ namespace
         W.z = A.alloc<T>(size_t(P.nUnits) * 6);
         W.p = A.alloc<T>(size_t(P.nUnits) * 6);
         W.q = A.alloc<T>(size_t(P.nUnits) * 6);
+        W.p2 = A.alloc<T>(size_t(P.nUnits) * 6);
+        W.qseg = A.alloc<T>(size_t(std::max(1, P.nSegments)) * 6);
         W.u = A.alloc<T>(size_t(P.nUnits) * 6);
         W.w = A.alloc<T>(size_t(P.nUnits) * 6);
         W.partPq = A.alloc<double>(nParts);
@@ -1140,13 +1164,14 @@ namespace
 
     struct PcgResult
     {
-        bool converged = false, breakdown = false;
+        bool converged = false, breakdown = false, modelStop = false;
         int iterations = 0;
         double rnorm = 0.0;
     };
 
     template <typename T>
-    PcgResult pcg(pb_ctx* ctx, pbk::Work<T>& W, double relTol, double absTol, int maxIter, int checkEvery)
+    PcgResult pcg(pb_ctx* ctx, pbk::Work<T>& W, double relTol, double absTol, int maxIter, int checkEvery,
+                  double modelEta)
     {
         using Lr = pbk::Launcher<T>;
         hipStream_t s = ctx->stream;
@@ -1178,29 +1203,37 @@ namespace
             (void)hipGetLastError();
             ctx->noPersistent = true;  // cooperative launch unavailable: multi-kernel path from now on
         }
-        Lr::pcgStart(P, W, relTol, absTol, s);
+        const int nbVec = Lr::pcgStart(P, W, relTol, absTol, s);
+        (void)nbVec;
         int it = 0;
         while (true)
         {
+            if (it >= maxIter)
+            {
+                // convergence test of the last update (the first half of the next iteration makes it)
+                Lr::pcgSpmv(P, W, it, relTol, absTol, s);
+            }
             check(hipMemcpyAsync(hf, W.iflags, sizeof(int32_t) * pbk::kIflagCount, hipMemcpyDeviceToHost, s), "readback");
             check(hipMemcpyAsync(h + 28, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
                   "readback");
             check(hipStreamSynchronize(s), "pcg");
-            if (hf[pbk::kIflagDone] || it >= maxIter) break;
+            if (hf[pbk::kIflagDone] || hf[pbk::kIflagModelStop] || it >= maxIter) break;
             const int chunk = std::min(checkEvery, maxIter - it);
             for (int k = 0; k < chunk; ++k, ++it)
             {
                 {
                     Span sp(ctx, 3);
-                    Lr::pcgSpmv(P, W, s);
+                    Lr::pcgSpmv(P, W, it, relTol, absTol, s);
                 }
-                Lr::pcgUpdate(P, W, it, s);
+                Lr::pcgUpdate(P, W, it, modelEta, s);
             }
         }
         res.breakdown = hf[pbk::kIflagBreakdown] != 0;
         res.iterations = hf[pbk::kIflagIters];
         res.rnorm = h[28 + pbk::kScalarRnorm];
-        res.converged = !res.breakdown && res.rnorm <= h[28 + pbk::kScalarThreshold];
+        res.modelStop = hf[pbk::kIflagModelStop] != 0;
+        res.converged = !res.breakdown && (res.modelStop || (hf[pbk::kIflagDone] != 0 &&
+                                                             res.rnorm <= h[28 + pbk::kScalarThreshold]));
         return res;
     }
 
@@ -1226,6 +1259,7 @@ namespace
         const int maxPcg = o.pcg_max_iterations > 0 ? o.pcg_max_iterations : std::max(200, 30 * pr.primaryBlocks);
         const int checkEvery = o.pcg_check_interval > 0 ? o.pcg_check_interval : 8;
         const int batch = o.armijo_batch > 0 ? std::min(o.armijo_batch, pbk::kMaxAlpha) : 4;
+        const double modelEta = o.pcg_model_eta >= 0.0 ? o.pcg_model_eta : (mixed ? 0.1 : 0.0);
         rep->linear_tolerance = relTol;
         rep->mixed_precision_used = mixed ? 1 : 0;
         ctx->lastScalarBytes = sizeof(T);
@@ -1282,7 +1316,7 @@ namespace
                     continue;
                 }
                 auto tl = Clock::now();
-                const PcgResult lin = pcg(ctx, W, relTol, absTol, maxPcg, checkEvery);
+                const PcgResult lin = pcg(ctx, W, relTol, absTol, maxPcg, checkEvery, modelEta);
                 rep->linear_solve_seconds += seconds(tl);
                 rep->linear_iterations += lin.iterations;
                 if (lin.breakdown)
@@ -1584,6 +1618,7 @@ void pb_default_options(pb_solve_options* o)
     o->low_order_distortion_scale = 1.0;
     o->device = 0;
     o->precision = PB_PRECISION_FP64;
+    o->pcg_model_eta = -1.0;
 }
 
 int pb_hip_available(int device)

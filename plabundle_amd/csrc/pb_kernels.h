@@ -20,6 +20,7 @@ namespace pbk
     constexpr int kMaxAlpha = 8;             // step lengths evaluated per trial-cost pass
     constexpr int kGroupPartialStride = 96;  // >= K(K+1)/2 + K for K <= 12
     constexpr int kPairChunk = 2048;         // pairs per Schur work item
+    constexpr int kSpmvSegment = 40;         // stored 6x6 blocks per SpMV wave
 
     enum Flag
     {
@@ -34,6 +35,7 @@ namespace pbk
         kIflagDone = 0,
         kIflagIters = 1,
         kIflagBreakdown = 2,
+        kIflagModelStop = 3,  // PCG ended by the quadratic-model test
         kIflagCount = 4
     };
 
@@ -48,6 +50,8 @@ namespace pbk
         kScalarDd,
         kScalarRmsGlobal,
         kScalarRmsPoint,
+        kScalarQ0,  // quadratic model value, double-buffered by iteration parity
+        kScalarQ1,
         kScalarCount = 16
     };
 
@@ -127,6 +131,12 @@ namespace pbk
         const int32_t* unitWidth = nullptr;  // active columns of a unit (6, or the group remainder)
         const int32_t* pcgRowStart = nullptr;  // [pcgBlocks+1] unit rows of each persistent-PCG block
         int pcgBlocks = 0;
+        // SpMV row segments of at most kSpmvSegment stored blocks
+        int nSegments = 0;
+        const int32_t* segRow = nullptr;
+        const int32_t* segBegin = nullptr;    // [nSegments+1]
+        const int32_t* segFirst = nullptr;    // 1 for the first segment of a row
+        const int32_t* rowSegOffset = nullptr;  // [nUnits+1]
     };
 
     template <typename T>
@@ -152,6 +162,8 @@ namespace pbk
         T* z = nullptr;
         T* p = nullptr;
         T* q = nullptr;
+        T* p2 = nullptr;    // second direction buffer (p alternates between p and p2)
+        T* qseg = nullptr;  // [nSegments*6] row-segment products of S p
         T* u = nullptr;  // preconditioned residual (persistent PCG)
         T* w = nullptr;  // S u (persistent PCG)
         double* partPq = nullptr;
@@ -181,8 +193,8 @@ namespace pbk
         static void cameraDirect(const Problem& P, const Work<T>& W, hipStream_t s);
         static void schurBlocks(const Problem& P, const Work<T>& W, double damping, hipStream_t s);
         static int pcgStart(const Problem& P, const Work<T>& W, double relTol, double absTol, hipStream_t s);
-        static void pcgSpmv(const Problem& P, const Work<T>& W, hipStream_t s);
-        static void pcgUpdate(const Problem& P, const Work<T>& W, int it, hipStream_t s);
+        static void pcgSpmv(const Problem& P, const Work<T>& W, int it, double relTol, double absTol, hipStream_t s);
+        static void pcgUpdate(const Problem& P, const Work<T>& W, int it, double modelEta, hipStream_t s);
         static int pcgPersistentBlocks();
         static hipError_t pcgPersistent(const Problem& P, const Work<T>& W, int blocks, int maxIter, double relTol,
                                         double absTol, hipStream_t s);

@@ -576,7 +576,24 @@ namespace pbk
         return s;
     }
 
-    // x = 0, r = b, z = M r, p = z; partials of r.z and r.r; ||b||^2 partial.
+    // Sum of n per-block partials by wave 0 of the calling block (fixed lane order and butterfly),
+    // broadcast to the block. Every block gets bitwise the same value.
+    __device__ __forceinline__ double blockSumPartials(const double* part, int n, double* slot)
+    {
+        if ((threadIdx.x >> 6) == 0)
+        {
+            double v = 0.0;
+            for (int i = threadIdx.x; i < n; i += kWave) v += part[i];
+            v = waveSum(v);
+            if (threadIdx.x == 0) *slot = v;
+        }
+        __syncthreads();
+        const double r = *slot;
+        __syncthreads();
+        return r;
+    }
+
+    // x = 0, r = b, z = M r; partials of r.z and r.r (= ||b||^2).
     template <typename T>
     __global__ void k_pcg_start(Problem P, Work<T> Wk)
     {
@@ -592,6 +609,7 @@ namespace pbk
                 r[i] = T(Wk.rhs[u * 6 + i]);
                 Wk.r[u * 6 + i] = r[i];
                 Wk.x[u * 6 + i] = T(0);
+                Wk.p[u * 6 + i] = T(0);  // p_old of iteration 0 (beta = 0)
             }
 #pragma unroll
             for (int i = 0; i < 6; ++i)
@@ -600,7 +618,6 @@ namespace pbk
 #pragma unroll
                 for (int j = 0; j < 6; ++j) z += Wk.Minv[u * 36 + i * 6 + j] * r[j];
                 Wk.z[u * 6 + i] = z;
-                Wk.p[u * 6 + i] = z;
                 rz += double(r[i]) * double(z);
                 rr += double(r[i]) * double(r[i]);
             }
@@ -613,88 +630,125 @@ namespace pbk
             Wk.partRz[blockIdx.x] = s1;
             Wk.partRr[blockIdx.x] = s2;
         }
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            Wk.iflags[kIflagDone] = 0;
+            Wk.iflags[kIflagIters] = 0;
+            Wk.iflags[kIflagBreakdown] = 0;
+            Wk.iflags[kIflagModelStop] = 0;
+            Wk.scalars[kScalarQ0] = 0.0;
+        }
     }
 
-    // Scalars after the start: rz[0], ||b||, convergence threshold, done flag.
+    // Iteration it, first half: beta from the previous r.z, the new direction p = z + beta p_old
+    // (formed on the fly for every column read), q segments of S p and partials of p.q.
+    // One wave per row segment of at most kSpmvSegment blocks.
     template <typename T>
-    __global__ void k_pcg_start_scalars(Problem P, Work<T> Wk, int nParts, double relTol, double absTol)
+    __global__ void __launch_bounds__(256) k_pcg_spmv(Problem P, Work<T> Wk, int it, int nVecParts, double relTol,
+                                                      double absTol)
     {
-        if (threadIdx.x != 0 || blockIdx.x != 0) return;
-        const double rz = sumPartials(Wk.partRz, nParts);
-        const double rr = sumPartials(Wk.partRr, nParts);
-        const double bn = sqrt(rr);
-        double* sc = Wk.scalars;
-        sc[kScalarRz0] = rz;
-        sc[kScalarRz1] = 0.0;
-        sc[kScalarBnorm] = bn;
-        const double thr = fmax(relTol * bn, absTol);
-        sc[kScalarThreshold] = thr;
-        sc[kScalarRnorm] = bn;
-        Wk.iflags[kIflagDone] = (bn <= thr) ? 1 : 0;
-        Wk.iflags[kIflagIters] = 0;
-        Wk.iflags[kIflagBreakdown] = 0;
-    }
-
-    // q = S p (block-sparse rows of 6x6 blocks); partial p.q. One wave per block row.
-    template <typename T>
-    __global__ void __launch_bounds__(256) k_pcg_spmv(Problem P, Work<T> Wk)
-    {
-        if (Wk.iflags[kIflagDone]) return;
+        if (Wk.iflags[kIflagDone] || Wk.iflags[kIflagModelStop]) return;
+        __shared__ double slot[2];
         __shared__ T red[4][64];
         __shared__ double lds[8];
-        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int u = blockIdx.x * 4 + wv;
-        double pq = 0.0;
-        if (u < P.nUnits)
+        const double rzCur = blockSumPartials(Wk.partRz, nVecParts, slot);
+        const double rr = blockSumPartials(Wk.partRr, nVecParts, slot + 1);
+        double thr;
+        double beta = 0.0;
+        if (it == 0)
         {
-            // lanes 0..59: row = lane % 6, block slot = lane / 6 (stride 10 blocks)
-            T acc = T(0);
+            thr = fmax(relTol * sqrt(rr), absTol);
+        }
+        else
+        {
+            thr = Wk.scalars[kScalarThreshold];
+            const double rzPrev = Wk.scalars[((it - 1) & 1) ? kScalarRz1 : kScalarRz0];
+            beta = rzPrev != 0.0 ? rzCur / rzPrev : 0.0;
+        }
+        const bool conv = sqrt(rr) <= thr;
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            Wk.scalars[(it & 1) ? kScalarRz1 : kScalarRz0] = rzCur;
+            Wk.scalars[kScalarRnorm] = sqrt(rr);
+            if (it == 0)
+            {
+                Wk.scalars[kScalarThreshold] = thr;
+                Wk.scalars[kScalarBnorm] = sqrt(rr);
+            }
+            Wk.iflags[kIflagIters] = it;
+            if (conv) Wk.iflags[kIflagDone] = 1;
+        }
+        if (conv) return;  // every block takes the same decision
+        const T* pOld = (it & 1) ? Wk.p2 : Wk.p;
+        T* pNew = (it & 1) ? Wk.p : Wk.p2;
+        const T bt = T(beta);
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int seg = blockIdx.x * 4 + wv;
+        double pq = 0.0;
+        T acc = T(0);
+        int u = -1;
+        if (seg < P.nSegments)
+        {
+            u = P.segRow[seg];
             if (lane < 60)
             {
                 const int row = lane % 6;
-                for (int k = P.rowPtr[u] + lane / 6; k < P.rowPtr[u + 1]; k += 10)
+                for (int k = P.segBegin[seg] + lane / 6; k < P.segBegin[seg + 1]; k += 10)
                 {
                     const int col = P.colIdx[k];
                     const T* Sb = Wk.S + k * 36 + row * 6;
-                    const T* pv = Wk.p + col * 6;
-                    acc += Sb[0] * pv[0] + Sb[1] * pv[1] + Sb[2] * pv[2] + Sb[3] * pv[3] + Sb[4] * pv[4] + Sb[5] * pv[5];
+                    const T* zc = Wk.z + col * 6;
+                    const T* pc = pOld + col * 6;
+                    T a = T(0);
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) a += Sb[j] * (zc[j] + bt * pc[j]);
+                    acc += a;
                 }
             }
-            red[wv][lane] = acc;
         }
+        red[wv][lane] = acc;
         __syncthreads();
-        if (u < P.nUnits && lane < 6)
+        if (seg < P.nSegments && lane < 6)
         {
-            T s = T(0);
-            for (int q = 0; q < 10; ++q) s += red[wv][lane + 6 * q];
-            Wk.q[u * 6 + lane] = s;
-            pq = double(s) * double(Wk.p[u * 6 + lane]);
+            T q = T(0);
+            for (int k = 0; k < 10; ++k) q += red[wv][lane + 6 * k];
+            Wk.qseg[seg * 6 + lane] = q;
+            const T pn = Wk.z[u * 6 + lane] + bt * pOld[u * 6 + lane];
+            if (P.segFirst[seg]) pNew[u * 6 + lane] = pn;
+            pq = double(pn) * double(q);
         }
         const double tot = blockSum256(pq, lds);
         if (threadIdx.x == 0) Wk.partPq[blockIdx.x] = tot;
     }
 
-    // alpha = rz / p.q; x += alpha p; r -= alpha q; z = M r; partials r.z and r.r.
+    // Iteration it, second half: alpha = r.z / p.q; q = sum of row segments; x += alpha p; r -= alpha q;
+    // z = M r; partials of r.z and r.r for the next iteration.
     template <typename T>
-    __global__ void k_pcg_update(Problem P, Work<T> Wk, int it, int nPq)
+    __global__ void k_pcg_update(Problem P, Work<T> Wk, int it, int nPq, double modelEta)
     {
-        if (Wk.iflags[kIflagDone]) return;
+        if (Wk.iflags[kIflagDone] || Wk.iflags[kIflagModelStop]) return;
+        __shared__ double slot[1];
         __shared__ double lds[8];
-        const double pq = sumPartials(Wk.partPq, nPq);
+        const double pq = blockSumPartials(Wk.partPq, nPq, slot);
         const double rz = Wk.scalars[(it & 1) ? kScalarRz1 : kScalarRz0];
         const bool breakdown = !(pq > 0.0) || !pbdev::finite(pq);
         const double alpha = breakdown ? 0.0 : rz / pq;
+        const T* pNew = (it & 1) ? Wk.p : Wk.p2;
         const int u = blockIdx.x * blockDim.x + threadIdx.x;
         double rzn = 0.0, rr = 0.0;
         if (u < P.nUnits && !breakdown)
         {
+            T q[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+            for (int sg = P.rowSegOffset[u]; sg < P.rowSegOffset[u + 1]; ++sg)
+#pragma unroll
+                for (int i = 0; i < 6; ++i) q[i] += Wk.qseg[sg * 6 + i];
             T r[6];
             const T a = T(alpha);
 #pragma unroll
             for (int i = 0; i < 6; ++i)
             {
-                Wk.x[u * 6 + i] += a * Wk.p[u * 6 + i];
-                r[i] = Wk.r[u * 6 + i] - a * Wk.q[u * 6 + i];
+                Wk.x[u * 6 + i] += a * pNew[u * 6 + i];
+                r[i] = Wk.r[u * 6 + i] - a * q[i];
                 Wk.r[u * 6 + i] = r[i];
             }
 #pragma unroll
@@ -715,35 +769,30 @@ namespace pbk
         {
             Wk.partRz[blockIdx.x] = s1;
             Wk.partRr[blockIdx.x] = s2;
-            if (breakdown && blockIdx.x == 0) Wk.iflags[kIflagBreakdown] = 1;
-        }
-    }
-
-    // beta = rz_new / rz; p = z + beta p; convergence test on ||r||. Every block derives the same
-    // decision from the same partials; block 0 publishes the scalars for the next launches.
-    template <typename T>
-    __global__ void k_pcg_direction(Problem P, Work<T> Wk, int it, int nParts)
-    {
-        if (Wk.iflags[kIflagDone]) return;
-        const double rzn = sumPartials(Wk.partRz, nParts);
-        const double rr = sumPartials(Wk.partRr, nParts);
-        const double rz = Wk.scalars[(it & 1) ? kScalarRz1 : kScalarRz0];
-        const double beta = rz != 0.0 ? rzn / rz : 0.0;
-        const bool brk = Wk.iflags[kIflagBreakdown] != 0;
-        const bool conv = sqrt(rr) <= Wk.scalars[kScalarThreshold];
-        const int u = blockIdx.x * blockDim.x + threadIdx.x;
-        if (u < P.nUnits && !brk && !conv)
-        {
-            const T bt = T(beta);
-#pragma unroll
-            for (int i = 0; i < 6; ++i) Wk.p[u * 6 + i] = Wk.z[u * 6 + i] + bt * Wk.p[u * 6 + i];
         }
         if (blockIdx.x == 0 && threadIdx.x == 0)
         {
-            Wk.scalars[((it + 1) & 1) ? kScalarRz1 : kScalarRz0] = rzn;
-            Wk.scalars[kScalarRnorm] = sqrt(rr);
-            Wk.iflags[kIflagIters] = brk ? it : it + 1;
-            if (brk || conv) Wk.iflags[kIflagDone] = 1;
+            if (breakdown)
+            {
+                Wk.iflags[kIflagBreakdown] = 1;
+                Wk.iflags[kIflagDone] = 1;
+                Wk.iflags[kIflagIters] = it;
+            }
+            else
+            {
+                // quadratic model Q(x) = x'Sx/2 - b'x decreases by alpha r.z/2 per step; truncated-Newton test
+                // i (Q_i - Q_{i-1}) / Q_i < eta
+                const double dq = -0.5 * alpha * rz;
+                const double qNew = Wk.scalars[(it & 1) ? kScalarQ1 : kScalarQ0] + dq;
+                Wk.scalars[((it + 1) & 1) ? kScalarQ1 : kScalarQ0] = qNew;
+                // Done is not raised here: sibling blocks of this launch may still be reading it. The next
+                // SpMV launch and the host treat ModelStop as done.
+                if (modelEta > 0.0 && qNew < 0.0 && double(it + 1) * dq / qNew < modelEta)
+                {
+                    Wk.iflags[kIflagModelStop] = 1;
+                    Wk.iflags[kIflagIters] = it + 1;
+                }
+            }
         }
     }
 
@@ -1334,23 +1383,22 @@ namespace pbk
     {
         const int nb = std::max(1, blocksFor(P.nUnits, 256));
         k_pcg_start<T><<<nb, 256, 0, s>>>(P, W);
-        k_pcg_start_scalars<T><<<1, 64, 0, s>>>(P, W, nb, relTol, absTol);
         return nb;
     }
 
     template <typename T>
-    void Launcher<T>::pcgSpmv(const Problem& P, const Work<T>& W, hipStream_t s)
+    void Launcher<T>::pcgSpmv(const Problem& P, const Work<T>& W, int it, double relTol, double absTol, hipStream_t s)
     {
-        k_pcg_spmv<T><<<std::max(1, blocksFor(P.nUnits, 4)), 256, 0, s>>>(P, W);
+        const int nbVec = std::max(1, blocksFor(P.nUnits, 256));
+        k_pcg_spmv<T><<<std::max(1, blocksFor(P.nSegments, 4)), 256, 0, s>>>(P, W, it, nbVec, relTol, absTol);
     }
 
     template <typename T>
-    void Launcher<T>::pcgUpdate(const Problem& P, const Work<T>& W, int it, hipStream_t s)
+    void Launcher<T>::pcgUpdate(const Problem& P, const Work<T>& W, int it, double modelEta, hipStream_t s)
     {
-        const int nbSpmv = std::max(1, blocksFor(P.nUnits, 4));
+        const int nbSpmv = std::max(1, blocksFor(P.nSegments, 4));
         const int nbVec = std::max(1, blocksFor(P.nUnits, 256));
-        k_pcg_update<T><<<nbVec, 256, 0, s>>>(P, W, it, nbSpmv);
-        k_pcg_direction<T><<<nbVec, 256, 0, s>>>(P, W, it, nbVec);
+        k_pcg_update<T><<<nbVec, 256, 0, s>>>(P, W, it, nbSpmv, modelEta);
     }
 
     template <typename T>

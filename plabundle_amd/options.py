@@ -120,6 +120,7 @@ class BackendOptions:
     pcgCheckInterval: int = 0
     armijoBatch: int = 0
     profileKernels: bool = False
+    pcgModelEta: float = -1.0  # < 0: path default (off for FP64, 0.1 for mixed)
 
 
 @dataclass
@@ -199,4 +200,5 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.pcg_check_interval = int(b.pcgCheckInterval)
     o.armijo_batch = int(b.armijoBatch)
     o.profile_kernels = 1 if b.profileKernels else 0
+    o.pcg_model_eta = float(b.pcgModelEta)
     return o
