@@ -51,6 +51,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="reduced problem for quick checks (not a bench line)")
     ap.add_argument("--mixed-rtol", type=float, default=0.0, help="PCG relative tolerance of the mixed path (0 = default)")
+    ap.add_argument("--mixed-eta", type=float, default=-1.0,
+                    help="quadratic-model PCG stop of the mixed path (<0 = default, 0 = off)")
+    ap.add_argument("--fp64-inexact-steps", type=int, default=1,
+                    help="FP64 solves with the mixed path's PCG stopping rule (isolates the precision effect)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this backend runs on one MI355X; "
@@ -71,11 +75,15 @@ def main():
     ws = SolverWorkspace(0)
     solver = Solver()
 
-    def solve(precision, profile):
+    def solve(precision, profile, inexact=False):
         o = S.aerial_options(PO.SolveOptions())
         o.backend.profileKernels = profile
-        if precision == PO.Precision.Mixed and args.mixed_rtol > 0:
-            o.backend.pcgRelativeTolerance = args.mixed_rtol
+        if precision == PO.Precision.Mixed or inexact:
+            if args.mixed_rtol > 0:
+                o.backend.pcgRelativeTolerance = args.mixed_rtol
+            elif inexact:
+                o.backend.pcgRelativeTolerance = 1e-3
+            o.backend.pcgModelEta = args.mixed_eta if args.mixed_eta >= 0 else 0.1
         t0 = time.perf_counter()
         r = solver.solve(problem, o, ws, precision=precision)
         wall = time.perf_counter() - t0
@@ -95,8 +103,7 @@ def main():
         log(f"step {i}: mixed {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
             f"solve {r.report['solve_seconds']:.4f}s, cost {r.finalCost:.10g}")
     stats = {}
-    for fam in ("linearize_points", "camera_direct", "schur_blocks", "pcg_spmv", "pcg_solve", "cost",
-                "back_substitution"):
+    for fam in ("linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost", "back_substitution"):
         stats[fam] = ws.kernel_stats(fam)
 
     fp64 = []
@@ -105,6 +112,13 @@ def main():
         fp64.append(r)
         log(f"fp64 {i}: {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
             f"solve {r.report['solve_seconds']:.4f}s (setup {r.report['setup_seconds']:.2f}s), cost {r.finalCost:.10g}")
+
+    fp64i = []
+    for i in range(args.fp64_inexact_steps):
+        r, wall = solve(PO.Precision.Fp64, False, inexact=True)
+        fp64i.append(r)
+        log(f"fp64 inexact {i}: {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
+            f"solve {r.report['solve_seconds']:.4f}s, cost {r.finalCost:.10g}")
 
     mixed_t = sum(r.report["solve_seconds"] for r in mixed) / len(mixed)
     mixed_it = sum(r.report["iterations"] for r in mixed) / len(mixed)
@@ -135,6 +149,8 @@ def main():
     rel_oracle = abs(mixed[0].finalCost - golden["final_cost"]) / golden["final_cost"] if golden else None
     rel_oracle64 = abs(fp64[0].finalCost - golden["final_cost"]) / golden["final_cost"] if golden and fp64 else None
     rel_fp64 = abs(mixed[0].finalCost - fp64[0].finalCost) / fp64[0].finalCost if fp64 else None
+    fp64i_t = sum(r.report["solve_seconds"] for r in fp64i) / len(fp64i) if fp64i else None
+    rel_oracle64i = abs(fp64i[0].finalCost - golden["final_cost"]) / golden["final_cost"] if golden and fp64i else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -189,6 +205,11 @@ def main():
         "rel_cost_err_vs_oracle": rel_oracle,
         "fp64_rel_cost_err_vs_oracle": rel_oracle64,
         "rel_cost_err_vs_fp64_gpu": rel_fp64,
+        "meets_1e-6": (rel_oracle <= 1e-6) if rel_oracle is not None else None,
+        "fp64_inexact_time_to_converge_s": round(fp64i_t, 6) if fp64i_t else None,
+        "fp64_inexact_pcg_iterations": fp64i[0].report["linear_iterations"] if fp64i else None,
+        "fp64_inexact_rel_cost_err_vs_oracle": rel_oracle64i,
+        "speedup_mixed_vs_fp64_inexact": round(fp64i_t / mixed_t, 4) if fp64i_t else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "device": mixed[0].report["device_name"],

@@ -401,7 +401,7 @@ namespace
         std::vector<int32_t> blockRow, blockCol, blockPosUp, blockPosLo, blockItemOffset;
         std::vector<int32_t> itemBlock, itemBegin, pairE1, pairE2;
         std::vector<int32_t> rowPtr, colIdx, diagPos;
-        std::vector<int32_t> segRow, segBegin, segFirst, rowSegOffset;
+        std::vector<int32_t> cgWaveRow, cgWaveBegin, cgWaveEnd, cgWaveLead;
     };
 
     void buildLayout(const Input& in, const Prepared& p, Layout* L)
@@ -631,22 +631,61 @@ namespace
             L->rowPtr[u + 1] = static_cast<int32_t>(L->colIdx.size());
         }
         L->nNnz = static_cast<int>(L->colIdx.size());
-        // SpMV segments: long rows (intrinsic units couple with every camera of a group) are split so that
-        // no wave walks more than kSpmvSegment blocks
-        L->rowSegOffset.assign(U + 1, 0);
+        // fused-PCG wave slots: a row gets one wave per kCgSegment stored blocks (at most kCgWaves, all in
+        // one workgroup); rows are packed into workgroups of kCgWaves slots in row order
+        L->cgWaveRow.clear();
+        L->cgWaveBegin.clear();
+        L->cgWaveEnd.clear();
+        L->cgWaveLead.clear();
+        auto padPack = [&]() {
+            while (L->cgWaveRow.size() % pbk::kCgWaves)
+            {
+                L->cgWaveRow.push_back(-1);
+                L->cgWaveBegin.push_back(0);
+                L->cgWaveEnd.push_back(0);
+                L->cgWaveLead.push_back(0);
+            }
+        };
         for (int u = 0; u < U; ++u)
         {
-            bool first = true;
-            for (int k = L->rowPtr[u]; k < L->rowPtr[u + 1]; k += pbk::kSpmvSegment)
+            const int b0 = L->rowPtr[u], n = L->rowPtr[u + 1] - b0;
+            const int w = std::max(1, std::min(pbk::kCgWaves, (n + pbk::kCgSegment - 1) / pbk::kCgSegment));
+            const int len = (n + w - 1) / w;
+            const int used = static_cast<int>(L->cgWaveRow.size() % pbk::kCgWaves);
+            if (used + w > pbk::kCgWaves) padPack();
+            for (int j = 0; j < w; ++j)
             {
-                L->segRow.push_back(u);
-                L->segBegin.push_back(k);
-                L->segFirst.push_back(first ? 1 : 0);
-                first = false;
+                L->cgWaveRow.push_back(u);
+                L->cgWaveBegin.push_back(b0 + std::min(n, j * len));
+                L->cgWaveEnd.push_back(b0 + std::min(n, (j + 1) * len));
+                L->cgWaveLead.push_back(j == 0 ? w : 0);
             }
-            L->rowSegOffset[u + 1] = static_cast<int32_t>(L->segRow.size());
         }
-        L->segBegin.push_back(L->nNnz);
+        padPack();
+        // the kernel relies on these: every row owned by exactly one lead slot whose row slots follow it in the
+        // same workgroup, and slot ranges inside the row
+        {
+            std::vector<int> owners(U, 0);
+            const int nSlots = static_cast<int>(L->cgWaveRow.size());
+            for (int sl = 0; sl < nSlots; ++sl)
+            {
+                const int u = L->cgWaveRow[sl], lead = L->cgWaveLead[sl];
+                if (u < 0) continue;
+                if (u >= U || L->cgWaveBegin[sl] < L->rowPtr[u] || L->cgWaveEnd[sl] > L->rowPtr[u + 1] ||
+                    L->cgWaveBegin[sl] > L->cgWaveEnd[sl])
+                    throw std::logic_error("PCG wave slot range");
+                if (lead > 0)
+                {
+                    ++owners[u];
+                    if (sl % pbk::kCgWaves + lead > pbk::kCgWaves) throw std::logic_error("PCG row crosses workgroup");
+                    for (int j = 1; j < lead; ++j)
+                        if (L->cgWaveRow[sl + j] != u || L->cgWaveLead[sl + j] != 0)
+                            throw std::logic_error("PCG row slots not contiguous");
+                }
+            }
+            for (int u = 0; u < U; ++u)
+                if (owners[u] != 1) throw std::logic_error("PCG row without a unique owner");
+        }
         // direct-part block ids
         L->camDiagBlock.assign(n, -1);
         L->camGroupBlock.assign(2 * n, -1);
@@ -716,8 +755,16 @@ namespace
         std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     };
 
-    const char* const kFamilies[] = {"linearize_points", "camera_direct", "schur_blocks", "pcg_spmv", "cost",
-                                     "back_substitution", "pcg_solve"};
+    const char* const kFamilies[] = {"linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost",
+                                     "back_substitution"};
+    // pinned readback area (in doubles): scalars of a build / step, flags, PCG scalars, PCG flags
+    constexpr int kPinFlags = 32;
+    constexpr int kPinPcg = 40;
+    constexpr int kPinPcgFlags = 72;
+    static_assert(pbk::kScalarCount <= kPinFlags && kPinPcg + pbk::kScalarCount <= kPinPcgFlags, "pinned layout");
+    // PCG launches are timed on every kPcgSampleEvery-th iteration only, so that the events do not
+    // stretch the timed solve; the family average is the average over the sampled launches
+    constexpr int kPcgSampleEvery = 16;
     constexpr int kFamilyCount = sizeof(kFamilies) / sizeof(kFamilies[0]);
 }  // namespace
 
@@ -749,10 +796,8 @@ struct pb_ctx
     bool haveResult = false;
     // profiling
     bool profile = false;
-    bool noPersistent = false;
     int64_t pcgIterationsProfiled = 0;
     double lastScalarBytes = 8.0;
-    std::vector<int32_t> pcgRowStart;
     std::array<Family, kFamilyCount> fam;
     std::vector<hipEvent_t> eventPool;
     size_t eventNext = 0;
@@ -776,9 +821,9 @@ namespace
         pb_ctx* ctx;
         int family;
         hipEvent_t stop = nullptr;
-        Span(pb_ctx* c, int f) : ctx(c), family(f)
+        Span(pb_ctx* c, int f, bool sample = true) : ctx(c), family(f)
         {
-            if (!ctx->profile) return;
+            if (!ctx->profile || !sample) return;
             hipEvent_t start = takeEvent(ctx);
             stop = takeEvent(ctx);
             check(hipEventRecord(start, ctx->stream), "hipEventRecord");
@@ -928,33 +973,6 @@ namespace
         P.gaugeScale = (p.anchor >= 0 && p.scaleCam >= 0 && p.baseline > 0.0) ? p.scaleCam : -1;
         P.gaugeBaseline = p.baseline;
         P.globalDim = p.globalDim;
-        // persistent PCG: one block per group of unit rows, balanced by stored blocks, at most the
-        // number of co-resident blocks of either precision
-        {
-            const char* force = std::getenv("PB_PCG_PERSISTENT");
-            const int co = std::min(pbk::Launcher<float>::pcgPersistentBlocks(), pbk::Launcher<double>::pcgPersistentBlocks());
-            int blocks = std::min(256, std::min(co, std::max(1, p.nUnits / 2)));
-            if (!force || force[0] != '1') blocks = 0;  // the two-launch PCG is the default
-            ctx->pcgRowStart.clear();
-            if (blocks > 0 && p.nUnits > 0)
-            {
-                const long total = L.nNnz + 2L * p.nUnits;
-                ctx->pcgRowStart.push_back(0);
-                long acc = 0;
-                int b = 1;
-                for (int u = 0; u < p.nUnits; ++u)
-                {
-                    acc += (L.rowPtr[u + 1] - L.rowPtr[u]) + 2;
-                    if (b < blocks && acc >= total * b / blocks && u + 1 < p.nUnits)
-                    {
-                        ctx->pcgRowStart.push_back(u + 1);
-                        ++b;
-                    }
-                }
-                ctx->pcgRowStart.push_back(p.nUnits);
-                P.pcgBlocks = static_cast<int>(ctx->pcgRowStart.size()) - 1;
-            }
-        }
         P.camR = A.upload(ctx->in.R, s);
         P.camC = A.upload(ctx->in.C, s);
         P.camSlots = A.upload(ctx->in.slots, s);
@@ -1006,12 +1024,11 @@ namespace
         P.colIdx = A.upload(L.colIdx, s);
         P.diagPos = A.upload(L.diagPos, s);
         P.unitWidth = A.upload(p.unitWidth, s);
-        P.pcgRowStart = A.upload(ctx->pcgRowStart, s);
-        P.nSegments = static_cast<int>(L.segRow.size());
-        P.segRow = A.upload(L.segRow, s);
-        P.segBegin = A.upload(L.segBegin, s);
-        P.segFirst = A.upload(L.segFirst, s);
-        P.rowSegOffset = A.upload(L.rowSegOffset, s);
+        P.cgBlocks = static_cast<int>(L.cgWaveRow.size()) / pbk::kCgWaves;
+        P.cgWaveRow = A.upload(L.cgWaveRow, s);
+        P.cgWaveBegin = A.upload(L.cgWaveBegin, s);
+        P.cgWaveEnd = A.upload(L.cgWaveEnd, s);
+        P.cgWaveLead = A.upload(L.cgWaveLead, s);
         check(hipStreamSynchronize(s), "structure upload");
         L.pairE1.clear();
         L.pairE1.shrink_to_fit();
@@ -1031,10 +1048,8 @@ namespace
     void allocWork(pb_ctx* ctx, DeviceAlloc& A, pbk::Work<T>& W)
     {
         const pbk::Problem& P = ctx->P;
-        const int nbUnits = std::max(1, (P.nUnits + 255) / 256);
-        const int nbSpmv = std::max(1, (P.nSegments + 3) / 4);
         const int nbElim = std::max(1, (P.nElim + 255) / 256);
-        const int nParts = std::max(std::max(std::max(nbUnits, nbSpmv), nbElim), 3 * std::max(1, P.pcgBlocks));
+        const int nParts = nbElim;  // back-substitution block partials
         W.W = A.alloc<T>(size_t(P.nEntries) * 18);
         W.Hinv = A.alloc<T>(size_t(P.nElim) * 9);
         W.bp = A.alloc<double>(size_t(P.nElim) * 3);
@@ -1051,17 +1066,14 @@ namespace
         W.S = A.alloc<T>(size_t(P.nNnz) * 36);
         W.Minv = A.alloc<T>(size_t(P.nUnits) * 36);
         W.x = A.alloc<T>(size_t(P.nUnits) * 6);
-        W.r = A.alloc<T>(size_t(P.nUnits) * 6);
-        W.z = A.alloc<T>(size_t(P.nUnits) * 6);
         W.p = A.alloc<T>(size_t(P.nUnits) * 6);
-        W.q = A.alloc<T>(size_t(P.nUnits) * 6);
-        W.p2 = A.alloc<T>(size_t(P.nUnits) * 6);
-        W.qseg = A.alloc<T>(size_t(std::max(1, P.nSegments)) * 6);
-        W.u = A.alloc<T>(size_t(P.nUnits) * 6);
-        W.w = A.alloc<T>(size_t(P.nUnits) * 6);
-        W.partPq = A.alloc<double>(nParts);
-        W.partRz = A.alloc<double>(nParts);
-        W.partRr = A.alloc<double>(nParts);
+        for (int k = 0; k < 2; ++k)
+        {
+            W.cgR[k] = A.alloc<T>(size_t(P.nUnits) * 6);
+            W.cgW[k] = A.alloc<T>(size_t(P.nUnits) * 6);
+            W.cgS[k] = A.alloc<T>(size_t(P.nUnits) * 6);
+            W.cgPart[k] = A.alloc<double>(size_t(std::max(1, P.cgBlocks)) * 3);
+        }
         W.partDd = A.alloc<double>(nParts);
         W.partSs = A.alloc<double>(nParts);
         W.scalars = A.alloc<double>(pbk::kScalarCount);
@@ -1151,9 +1163,10 @@ namespace
         }
         double* h = ctx->pinned;
         check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s), "readback");
-        check(hipMemcpyAsync(h + 16, W.flags, sizeof(int32_t) * pbk::kFlagCount, hipMemcpyDeviceToHost, s), "readback");
+        check(hipMemcpyAsync(h + kPinFlags, W.flags, sizeof(int32_t) * pbk::kFlagCount, hipMemcpyDeviceToHost, s),
+              "readback");
         check(hipStreamSynchronize(s), "build");
-        const int32_t* f = reinterpret_cast<const int32_t*>(h + 16);
+        const int32_t* f = reinterpret_cast<const int32_t*>(h + kPinFlags);
         BuildResult r;
         r.cost = h[pbk::kScalarCost];
         r.invalid = f[pbk::kFlagInvalid] != 0;
@@ -1177,63 +1190,35 @@ namespace
         hipStream_t s = ctx->stream;
         const pbk::Problem& P = ctx->P;
         PcgResult res;
-        double* h = ctx->pinned;
-        int32_t* hf = reinterpret_cast<int32_t*>(h + 24);
-        if (P.pcgBlocks > 0 && !ctx->noPersistent)
-        {
-            hipError_t e;
-            {
-                Span sp(ctx, 6);
-                e = Lr::pcgPersistent(P, W, P.pcgBlocks, maxIter, relTol, absTol, s);
-            }
-            if (e == hipSuccess)
-            {
-                check(hipMemcpyAsync(hf, W.iflags, sizeof(int32_t) * pbk::kIflagCount, hipMemcpyDeviceToHost, s),
-                      "readback");
-                check(hipMemcpyAsync(h + 28, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
-                      "readback");
-                check(hipStreamSynchronize(s), "pcg");
-                res.breakdown = hf[pbk::kIflagBreakdown] != 0;
-                res.iterations = hf[pbk::kIflagIters];
-                res.rnorm = h[28 + pbk::kScalarRnorm];
-                res.converged = !res.breakdown && hf[pbk::kIflagDone] != 0;
-                if (ctx->profile) ctx->pcgIterationsProfiled += res.iterations;
-                return res;
-            }
-            (void)hipGetLastError();
-            ctx->noPersistent = true;  // cooperative launch unavailable: multi-kernel path from now on
-        }
-        const int nbVec = Lr::pcgStart(P, W, relTol, absTol, s);
-        (void)nbVec;
+        double* h = ctx->pinned + kPinPcg;
+        int32_t* hf = reinterpret_cast<int32_t*>(ctx->pinned + kPinPcgFlags);
+        auto readback = [&]() {
+            check(hipMemcpyAsync(hf, W.iflags, sizeof(int32_t) * pbk::kIflagCount, hipMemcpyDeviceToHost, s), "readback");
+            check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s), "readback");
+            check(hipStreamSynchronize(s), "pcg");
+        };
+        // fused iterations; the host looks at the flags after chunks that grow with the iteration count
+        // (launches after the stop return at once)
+        Lr::pcgInit(P, W, relTol, absTol, s);
         int it = 0;
         while (true)
         {
-            if (it >= maxIter)
+            const int chunk = std::min(std::max(checkEvery, it / 4), 256);
+            const int last = std::min(it + chunk, maxIter + 1);
+            for (; it < last; ++it)
             {
-                // convergence test of the last update (the first half of the next iteration makes it)
-                Lr::pcgSpmv(P, W, it, relTol, absTol, s);
+                Span sp(ctx, 3, it % kPcgSampleEvery == 0);
+                Lr::pcgIteration(P, W, it, maxIter, relTol, absTol, modelEta, s);
             }
-            check(hipMemcpyAsync(hf, W.iflags, sizeof(int32_t) * pbk::kIflagCount, hipMemcpyDeviceToHost, s), "readback");
-            check(hipMemcpyAsync(h + 28, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
-                  "readback");
-            check(hipStreamSynchronize(s), "pcg");
-            if (hf[pbk::kIflagDone] || hf[pbk::kIflagModelStop] || it >= maxIter) break;
-            const int chunk = std::min(checkEvery, maxIter - it);
-            for (int k = 0; k < chunk; ++k, ++it)
-            {
-                {
-                    Span sp(ctx, 3);
-                    Lr::pcgSpmv(P, W, it, relTol, absTol, s);
-                }
-                Lr::pcgUpdate(P, W, it, modelEta, s);
-            }
+            readback();
+            if (hf[pbk::kIflagDone0] || hf[pbk::kIflagDone1] || it > maxIter) break;
         }
         res.breakdown = hf[pbk::kIflagBreakdown] != 0;
         res.iterations = hf[pbk::kIflagIters];
-        res.rnorm = h[28 + pbk::kScalarRnorm];
+        res.rnorm = h[pbk::kScalarRnorm];
         res.modelStop = hf[pbk::kIflagModelStop] != 0;
-        res.converged = !res.breakdown && (res.modelStop || (hf[pbk::kIflagDone] != 0 &&
-                                                             res.rnorm <= h[28 + pbk::kScalarThreshold]));
+        res.converged = !res.breakdown && (res.modelStop || res.rnorm <= h[pbk::kScalarThreshold]);
+        if (ctx->profile) ctx->pcgIterationsProfiled += res.iterations;
         return res;
     }
 
@@ -1869,8 +1854,8 @@ pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, 
             case 2:  // pairs + entry points + W + Hinv + y + S out
                 b = P.nPairs * 8.0 + P.nEntries * (4.0 + 18 * t) + P.nElim * (9 * t + 24) + P.nNnz * 36 * t;
                 break;
-            case 3:  // S + column ids + vectors
-                b = P.nNnz * (36 * t + 4) + P.nUnits * 6 * t * 6;
+            case 3:  // S + column ids + preconditioner + r, w, s in + p, x in/out + r, w, s out
+                b = P.nNnz * (36 * t + 4) + P.nUnits * (36 * t + 10 * 6 * t);
                 break;
             case 4:  // obs (cam, point, uv, w) + point X, dp, elim id
                 b = P.nObs * (4.0 + 4 + 16 + 8) + P.nActive * (24.0 + 4) + P.nElim * 24.0;
@@ -1878,13 +1863,6 @@ pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, 
             case 5:  // entries W + units + point bp/Hinv + dp out
                 b = P.nEntries * (18 * t + 4) + P.nElim * (24.0 + 9 * t + 24 + 8);
                 break;
-            case 6:  // per PCG iteration: S + column ids + preconditioner + vectors, times iterations per launch
-            {
-                const double perIt = P.nNnz * (36 * t + 4) + P.nUnits * (36 * t + 12 * 6 * t);
-                const double its = ctx->fam[f].launches ? double(ctx->pcgIterationsProfiled) / ctx->fam[f].launches : 0.0;
-                b = perIt * its;
-                break;
-            }
             default:
                 break;
             }

@@ -20,7 +20,8 @@ namespace pbk
     constexpr int kMaxAlpha = 8;             // step lengths evaluated per trial-cost pass
     constexpr int kGroupPartialStride = 96;  // >= K(K+1)/2 + K for K <= 12
     constexpr int kPairChunk = 2048;         // pairs per Schur work item
-    constexpr int kSpmvSegment = 40;         // stored 6x6 blocks per SpMV wave
+    constexpr int kCgWaves = 8;              // waves per workgroup of the fused PCG iteration
+    constexpr int kCgSegment = 10;           // stored blocks per wave before a row gets another wave
 
     enum Flag
     {
@@ -32,27 +33,32 @@ namespace pbk
 
     enum Iflag
     {
-        kIflagDone = 0,
-        kIflagIters = 1,
-        kIflagBreakdown = 2,
-        kIflagModelStop = 3,  // PCG ended by the quadratic-model test
-        kIflagCount = 4
+        kIflagDone0 = 0,  // PCG stopped, by launch parity (a launch reads one and writes the other)
+        kIflagDone1 = 1,
+        kIflagIters = 2,
+        kIflagBreakdown = 3,
+        kIflagModelStop = 4,  // PCG ended by the quadratic-model test
+        kIflagCount = 8
     };
 
     enum Scalar
     {
         kScalarCost = 0,
-        kScalarRz0,
-        kScalarRz1,
         kScalarBnorm,
         kScalarThreshold,
         kScalarRnorm,
         kScalarDd,
         kScalarRmsGlobal,
         kScalarRmsPoint,
-        kScalarQ0,  // quadratic model value, double-buffered by iteration parity
+        kScalarQ0,  // quadratic model value after `it` steps, double-buffered by iteration parity
         kScalarQ1,
-        kScalarCount = 16
+        kScalarDq0,  // model decrease of the last step
+        kScalarDq1,
+        kScalarGamma0,  // (r, M r) of the last step
+        kScalarGamma1,
+        kScalarAlpha0,  // step length of the last step
+        kScalarAlpha1,
+        kScalarCount = 24
     };
 
     struct Problem
@@ -129,14 +135,13 @@ namespace pbk
         const int32_t* colIdx = nullptr;
         const int32_t* diagPos = nullptr;
         const int32_t* unitWidth = nullptr;  // active columns of a unit (6, or the group remainder)
-        const int32_t* pcgRowStart = nullptr;  // [pcgBlocks+1] unit rows of each persistent-PCG block
-        int pcgBlocks = 0;
-        // SpMV row segments of at most kSpmvSegment stored blocks
-        int nSegments = 0;
-        const int32_t* segRow = nullptr;
-        const int32_t* segBegin = nullptr;    // [nSegments+1]
-        const int32_t* segFirst = nullptr;    // 1 for the first segment of a row
-        const int32_t* rowSegOffset = nullptr;  // [nUnits+1]
+        // fused PCG: kCgWaves wave slots per workgroup; a slot walks [begin, end) of one block row, and the
+        // first slot of a row (lead = number of slots of the row, all in the same workgroup) owns the row
+        int cgBlocks = 0;
+        const int32_t* cgWaveRow = nullptr;  // [cgBlocks*kCgWaves], -1 for an empty slot
+        const int32_t* cgWaveBegin = nullptr;
+        const int32_t* cgWaveEnd = nullptr;
+        const int32_t* cgWaveLead = nullptr;
     };
 
     template <typename T>
@@ -157,18 +162,12 @@ namespace pbk
         double* itemPartialRhs = nullptr;  // [nItems*6]
         T* S = nullptr;     // [nNnz*36]
         T* Minv = nullptr;  // [nUnits*36]
-        T* x = nullptr;
-        T* r = nullptr;
-        T* z = nullptr;
-        T* p = nullptr;
-        T* q = nullptr;
-        T* p2 = nullptr;    // second direction buffer (p alternates between p and p2)
-        T* qseg = nullptr;  // [nSegments*6] row-segment products of S p
-        T* u = nullptr;  // preconditioned residual (persistent PCG)
-        T* w = nullptr;  // S u (persistent PCG)
-        double* partPq = nullptr;
-        double* partRz = nullptr;
-        double* partRr = nullptr;
+        T* x = nullptr;  // PCG solution
+        T* p = nullptr;  // PCG direction
+        T* cgR[2] = {nullptr, nullptr};  // fused PCG: residual, by iteration parity
+        T* cgW[2] = {nullptr, nullptr};  // S M r
+        T* cgS[2] = {nullptr, nullptr};  // S p
+        double* cgPart[2] = {nullptr, nullptr};  // [cgBlocks*3] block partials of (r,Mr), (w,Mr), (r,r)
         double* partDd = nullptr;
         double* partSs = nullptr;
         double* scalars = nullptr;
@@ -192,12 +191,9 @@ namespace pbk
         static void linearizePoints(const Problem& P, const Work<T>& W, double damping, hipStream_t s);
         static void cameraDirect(const Problem& P, const Work<T>& W, hipStream_t s);
         static void schurBlocks(const Problem& P, const Work<T>& W, double damping, hipStream_t s);
-        static int pcgStart(const Problem& P, const Work<T>& W, double relTol, double absTol, hipStream_t s);
-        static void pcgSpmv(const Problem& P, const Work<T>& W, int it, double relTol, double absTol, hipStream_t s);
-        static void pcgUpdate(const Problem& P, const Work<T>& W, int it, double modelEta, hipStream_t s);
-        static int pcgPersistentBlocks();
-        static hipError_t pcgPersistent(const Problem& P, const Work<T>& W, int blocks, int maxIter, double relTol,
-                                        double absTol, hipStream_t s);
+        static void pcgInit(const Problem& P, const Work<T>& W, double relTol, double absTol, hipStream_t s);
+        static void pcgIteration(const Problem& P, const Work<T>& W, int it, int maxIt, double relTol, double absTol,
+                                 double modelEta, hipStream_t s);
         static void backSubstitute(const Problem& P, const Work<T>& W, hipStream_t s);
         static void trialCosts(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, int costBlocks,
                                hipStream_t s);

@@ -16,7 +16,6 @@
 #include "pb_device.h"
 #include "pb_kernels.h"
 
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 namespace pbk
@@ -569,13 +568,6 @@ namespace pbk
         return s;
     }
 
-    __device__ __forceinline__ double sumPartials(const double* part, int n)
-    {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += part[i];
-        return s;
-    }
-
     // Sum of n per-block partials by wave 0 of the calling block (fixed lane order and butterfly),
     // broadcast to the block. Every block gets bitwise the same value.
     __device__ __forceinline__ double blockSumPartials(const double* part, int n, double* slot)
@@ -593,450 +585,282 @@ namespace pbk
         return r;
     }
 
-    // x = 0, r = b, z = M r; partials of r.z and r.r (= ||b||^2).
+    // ---------------------------------------------------------------------------------------------
+    // Fused PCG iteration (Chronopoulos-Gear recurrences): one launch per iteration.
+    //
+    // State entering iteration it (parity rd = it & 1): r_it, w_it = S M r_it, s_{it-1}, p_{it-1}, x_it and the
+    // block partials of gamma = (r, M r), delta = (w, M r), rho = (r, r). Every block sums the partials in the
+    // same order, so all blocks derive bitwise the same alpha, beta and stop decision. The owner wave of a
+    // unit row updates s, p, x, r of its row; the SpMV w_{it+1} = S M r_{it+1} needs r_{it+1} of every column
+    // unit, which each wave recomputes from the previous-parity vectors with the same fused operations as
+    // the owner (so the values agree bit for bit). r, s, w are double-buffered by parity; p and x are
+    // touched by their owner only. it = -1 primes w_0 and the partials of iteration 0 (alpha = beta = 0).
     template <typename T>
-    __global__ void k_pcg_start(Problem P, Work<T> Wk)
+    __device__ __forceinline__ T fmaT(T a, T b, T c);
+    template <>
+    __device__ __forceinline__ float fmaT<float>(float a, float b, float c)
     {
-        __shared__ double lds[8];
-        const int u = blockIdx.x * blockDim.x + threadIdx.x;
-        double rz = 0.0, rr = 0.0;
-        if (u < P.nUnits)
-        {
-            T r[6];
-#pragma unroll
-            for (int i = 0; i < 6; ++i)
-            {
-                r[i] = T(Wk.rhs[u * 6 + i]);
-                Wk.r[u * 6 + i] = r[i];
-                Wk.x[u * 6 + i] = T(0);
-                Wk.p[u * 6 + i] = T(0);  // p_old of iteration 0 (beta = 0)
-            }
-#pragma unroll
-            for (int i = 0; i < 6; ++i)
-            {
-                T z = T(0);
-#pragma unroll
-                for (int j = 0; j < 6; ++j) z += Wk.Minv[u * 36 + i * 6 + j] * r[j];
-                Wk.z[u * 6 + i] = z;
-                rz += double(r[i]) * double(z);
-                rr += double(r[i]) * double(r[i]);
-            }
-        }
-        const double s1 = blockSum256(rz, lds);
-        __syncthreads();
-        const double s2 = blockSum256(rr, lds + 4);
-        if (threadIdx.x == 0)
-        {
-            Wk.partRz[blockIdx.x] = s1;
-            Wk.partRr[blockIdx.x] = s2;
-        }
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-        {
-            Wk.iflags[kIflagDone] = 0;
-            Wk.iflags[kIflagIters] = 0;
-            Wk.iflags[kIflagBreakdown] = 0;
-            Wk.iflags[kIflagModelStop] = 0;
-            Wk.scalars[kScalarQ0] = 0.0;
-        }
+        return __builtin_fmaf(a, b, c);
+    }
+    template <>
+    __device__ __forceinline__ double fmaT<double>(double a, double b, double c)
+    {
+        return __builtin_fma(a, b, c);
     }
 
-    // Iteration it, first half: beta from the previous r.z, the new direction p = z + beta p_old
-    // (formed on the fly for every column read), q segments of S p and partials of p.q.
-    // One wave per row segment of at most kSpmvSegment blocks.
+    // s_it, r_{it+1} entries i6 of unit col, then (M r_{it+1}) entry i6 through the six lanes base..base+5 that
+    // hold the unit. Every lane of the wave must call it (it shuffles).
     template <typename T>
-    __global__ void __launch_bounds__(256) k_pcg_spmv(Problem P, Work<T> Wk, int it, int nVecParts, double relTol,
-                                                      double absTol)
+    __device__ __forceinline__ void nextResidual(const Work<T>& Wk, const T* rO, const T* wO, const T* sO, int col,
+                                                 int i6, int base, T a, T bt, T& sn, T& rn, T& un)
     {
-        if (Wk.iflags[kIflagDone] || Wk.iflags[kIflagModelStop]) return;
-        __shared__ double slot[2];
-        __shared__ T red[4][64];
-        __shared__ double lds[8];
-        const double rzCur = blockSumPartials(Wk.partRz, nVecParts, slot);
-        const double rr = blockSumPartials(Wk.partRr, nVecParts, slot + 1);
-        double thr;
-        double beta = 0.0;
-        if (it == 0)
-        {
-            thr = fmax(relTol * sqrt(rr), absTol);
-        }
-        else
-        {
-            thr = Wk.scalars[kScalarThreshold];
-            const double rzPrev = Wk.scalars[((it - 1) & 1) ? kScalarRz1 : kScalarRz0];
-            beta = rzPrev != 0.0 ? rzCur / rzPrev : 0.0;
-        }
-        const bool conv = sqrt(rr) <= thr;
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-        {
-            Wk.scalars[(it & 1) ? kScalarRz1 : kScalarRz0] = rzCur;
-            Wk.scalars[kScalarRnorm] = sqrt(rr);
-            if (it == 0)
-            {
-                Wk.scalars[kScalarThreshold] = thr;
-                Wk.scalars[kScalarBnorm] = sqrt(rr);
-            }
-            Wk.iflags[kIflagIters] = it;
-            if (conv) Wk.iflags[kIflagDone] = 1;
-        }
-        if (conv) return;  // every block takes the same decision
-        const T* pOld = (it & 1) ? Wk.p2 : Wk.p;
-        T* pNew = (it & 1) ? Wk.p : Wk.p2;
-        const T bt = T(beta);
-        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int seg = blockIdx.x * 4 + wv;
-        double pq = 0.0;
-        T acc = T(0);
-        int u = -1;
-        if (seg < P.nSegments)
-        {
-            u = P.segRow[seg];
-            if (lane < 60)
-            {
-                const int row = lane % 6;
-                for (int k = P.segBegin[seg] + lane / 6; k < P.segBegin[seg + 1]; k += 10)
-                {
-                    const int col = P.colIdx[k];
-                    const T* Sb = Wk.S + k * 36 + row * 6;
-                    const T* zc = Wk.z + col * 6;
-                    const T* pc = pOld + col * 6;
-                    T a = T(0);
+        const int ci = col * 6 + i6;
+        T m[6];
 #pragma unroll
-                    for (int j = 0; j < 6; ++j) a += Sb[j] * (zc[j] + bt * pc[j]);
-                    acc += a;
+        for (int j = 0; j < 6; ++j) m[j] = Wk.Minv[col * 36 + i6 * 6 + j];
+        sn = fmaT(bt, sO[ci], wO[ci]);
+        rn = fmaT(-a, sn, rO[ci]);
+        T acc = T(0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+        {
+            const T rj = __shfl(rn, min(base + j, kWave - 1), kWave);
+            acc = fmaT(m[j], rj, acc);
+        }
+        un = acc;
+    }
+
+    template <typename T>
+    __global__ void __launch_bounds__(64 * kCgWaves) k_pcg_iter(Problem P, Work<T> Wk, int it, int maxIt, int nParts,
+                                                                double relTol, double absTol, double modelEta)
+    {
+        __shared__ double red3[kCgWaves][3];
+        __shared__ T red[kCgWaves][kWave];
+        __shared__ T segq[kCgWaves][6];
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int rd = it & 1, wr = rd ^ 1;  // prime (it = -1) reads parity 1 and writes parity 0
+        // One memory round trip for everything the launch decides on: the stop flag of the previous launch,
+        // the block partials (all threads of the block load them), the scalars and this wave's slot. A launch
+        // reads only what earlier launches wrote, so all blocks and waves see the same values.
+        const int prevDone = Wk.iflags[kIflagDone0 + rd];
+        const int slot = blockIdx.x * kCgWaves + wv;
+        const int row = P.cgWaveRow[slot];
+        const int kb = P.cgWaveBegin[slot], ke = P.cgWaveEnd[slot];
+        const int lead = P.cgWaveLead[slot];  // waves of the row when this is its first wave
+        double g = 0.0, d = 0.0, q = 0.0;
+        double thrPrev = 0.0, qIt = 0.0, dqPrev = 0.0, gPrev = 1.0, aPrev = 1.0;
+        if (it >= 0)
+        {
+            const double* part = Wk.cgPart[rd];
+            for (int i = threadIdx.x; i < nParts; i += 64 * kCgWaves)
+            {
+                g += part[3 * i];
+                d += part[3 * i + 1];
+                q += part[3 * i + 2];
+            }
+            thrPrev = Wk.scalars[kScalarThreshold];
+            qIt = Wk.scalars[kScalarQ0 + rd];
+            dqPrev = Wk.scalars[kScalarDq0 + rd];
+            gPrev = Wk.scalars[kScalarGamma0 + rd];
+            aPrev = Wk.scalars[kScalarAlpha0 + rd];
+        }
+        if (prevDone)
+        {
+            if (blockIdx.x == 0 && threadIdx.x == 0) Wk.iflags[kIflagDone0 + wr] = 1;
+            return;
+        }
+        double alpha = 0.0, beta = 0.0;
+        if (it >= 0)
+        {
+            g = waveSum(g);
+            d = waveSum(d);
+            q = waveSum(q);
+            if (lane == 0)
+            {
+                red3[wv][0] = g;
+                red3[wv][1] = d;
+                red3[wv][2] = q;
+            }
+            __syncthreads();
+            double gamma = 0.0, delta = 0.0, rho = 0.0;
+            for (int w2 = 0; w2 < kCgWaves; ++w2)
+            {
+                gamma += red3[w2][0];
+                delta += red3[w2][1];
+                rho += red3[w2][2];
+            }
+            __syncthreads();  // red3 is reused for this launch's partials
+            const double rnorm = sqrt(rho);
+            const double thr = it == 0 ? fmax(relTol * rnorm, absTol) : thrPrev;
+            const bool conv = rnorm <= thr;
+            // truncated-Newton test of the previous step on the quadratic model Q(x) = x'Sx/2 - b'x:
+            // it (Q_it - Q_{it-1}) / Q_it < eta
+            const bool modelStop = !conv && it > 0 && modelEta > 0.0 && qIt < 0.0 && double(it) * dqPrev / qIt < modelEta;
+            double den = delta;
+            if (it > 0)
+            {
+                beta = gamma / gPrev;
+                den = delta - beta * gamma / aPrev;
+            }
+            alpha = gamma / den;
+            const bool breakdown = !conv && !modelStop &&
+                                   (!(den > 0.0) || !(gamma > 0.0) || !pbdev::finite(alpha) || !pbdev::finite(beta));
+            const bool stop = conv || modelStop || breakdown || it >= maxIt;
+            if (blockIdx.x == 0 && threadIdx.x == 0)
+            {
+                Wk.scalars[kScalarRnorm] = rnorm;
+                if (it == 0)
+                {
+                    Wk.scalars[kScalarThreshold] = thr;
+                    Wk.scalars[kScalarBnorm] = rnorm;
+                }
+                if (stop)
+                {
+                    Wk.iflags[kIflagIters] = it;
+                    Wk.iflags[kIflagModelStop] = modelStop ? 1 : 0;
+                    Wk.iflags[kIflagBreakdown] = breakdown ? 1 : 0;
+                    Wk.iflags[kIflagDone0 + wr] = 1;
+                }
+                else
+                {
+                    const double dq = -0.5 * alpha * gamma;
+                    Wk.scalars[kScalarQ0 + wr] = qIt + dq;
+                    Wk.scalars[kScalarDq0 + wr] = dq;
+                    Wk.scalars[kScalarGamma0 + wr] = gamma;
+                    Wk.scalars[kScalarAlpha0 + wr] = alpha;
                 }
             }
+            if (stop) return;  // every block takes the same decision
+        }
+        else if (blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            Wk.scalars[kScalarQ0] = 0.0;
+            Wk.scalars[kScalarDq0] = 0.0;
+        }
+        const T a = T(alpha), bt = T(beta);
+        const T* rO = Wk.cgR[rd];
+        const T* wO = Wk.cgW[rd];
+        const T* sO = Wk.cgS[rd];
+        const int i6 = lane % 6, bslot = lane / 6, base = 6 * bslot;
+        // SpMV segment of the row: ten 6x6 blocks per step, lane (bslot, i6) takes row i6 of block bslot
+        T acc = T(0);
+        if (row >= 0)
+        {
+            const int nSteps = (ke - kb + 9) / 10;
+            for (int st = 0; st < nSteps; ++st)
+            {
+                const int k = kb + st * 10 + bslot;
+                const bool act = lane < 60 && k < ke;
+                const int kk = act ? k : kb;
+                const int col = P.colIdx[kk];
+                T Srow[6];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) Srow[j] = Wk.S[size_t(kk) * 36 + i6 * 6 + j];
+                T sn, rn, un;
+                nextResidual(Wk, rO, wO, sO, col, i6, base, a, bt, sn, rn, un);
+                T sum = T(0);
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+                {
+                    const T uj = __shfl(un, min(base + j, kWave - 1), kWave);
+                    sum = fmaT(Srow[j], uj, sum);
+                }
+                if (act) acc += sum;
+            }
+        }
+        // owner update of row `row` (lanes 0..5 hold its six entries)
+        T rOwn = T(0), uOwn = T(0);
+        if (lead)
+        {
+            const bool own = lane < 6;
+            const int gi = row * 6 + (own ? lane : 0);
+            const T rOld = rO[gi], pOld = Wk.p[gi], xOld = Wk.x[gi];
+            T uOld = T(0);
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+            {
+                const T rj = __shfl(rOld, j, kWave);
+                uOld = fmaT(Wk.Minv[row * 36 + (own ? lane : 0) * 6 + j], rj, uOld);
+            }
+            T sn, rn, un;
+            nextResidual(Wk, rO, wO, sO, row, own ? lane : 0, 0, a, bt, sn, rn, un);
+            if (own)
+            {
+                const T pn = fmaT(bt, pOld, uOld);
+                Wk.cgS[wr][gi] = sn;
+                Wk.cgR[wr][gi] = rn;
+                Wk.p[gi] = pn;
+                Wk.x[gi] = fmaT(a, pn, xOld);
+            }
+            rOwn = rn;
+            uOwn = un;
         }
         red[wv][lane] = acc;
         __syncthreads();
-        if (seg < P.nSegments && lane < 6)
+        if (lane < 6)
         {
-            T q = T(0);
-            for (int k = 0; k < 10; ++k) q += red[wv][lane + 6 * k];
-            Wk.qseg[seg * 6 + lane] = q;
-            const T pn = Wk.z[u * 6 + lane] + bt * pOld[u * 6 + lane];
-            if (P.segFirst[seg]) pNew[u * 6 + lane] = pn;
-            pq = double(pn) * double(q);
+            T s6 = T(0);
+#pragma unroll
+            for (int b = 0; b < 10; ++b) s6 += red[wv][b * 6 + lane];
+            segq[wv][lane] = s6;
         }
-        const double tot = blockSum256(pq, lds);
-        if (threadIdx.x == 0) Wk.partPq[blockIdx.x] = tot;
-    }
-
-    // Iteration it, second half: alpha = r.z / p.q; q = sum of row segments; x += alpha p; r -= alpha q;
-    // z = M r; partials of r.z and r.r for the next iteration.
-    template <typename T>
-    __global__ void k_pcg_update(Problem P, Work<T> Wk, int it, int nPq, double modelEta)
-    {
-        if (Wk.iflags[kIflagDone] || Wk.iflags[kIflagModelStop]) return;
-        __shared__ double slot[1];
-        __shared__ double lds[8];
-        const double pq = blockSumPartials(Wk.partPq, nPq, slot);
-        const double rz = Wk.scalars[(it & 1) ? kScalarRz1 : kScalarRz0];
-        const bool breakdown = !(pq > 0.0) || !pbdev::finite(pq);
-        const double alpha = breakdown ? 0.0 : rz / pq;
-        const T* pNew = (it & 1) ? Wk.p : Wk.p2;
-        const int u = blockIdx.x * blockDim.x + threadIdx.x;
-        double rzn = 0.0, rr = 0.0;
-        if (u < P.nUnits && !breakdown)
-        {
-            T q[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-            for (int sg = P.rowSegOffset[u]; sg < P.rowSegOffset[u + 1]; ++sg)
-#pragma unroll
-                for (int i = 0; i < 6; ++i) q[i] += Wk.qseg[sg * 6 + i];
-            T r[6];
-            const T a = T(alpha);
-#pragma unroll
-            for (int i = 0; i < 6; ++i)
-            {
-                Wk.x[u * 6 + i] += a * pNew[u * 6 + i];
-                r[i] = Wk.r[u * 6 + i] - a * q[i];
-                Wk.r[u * 6 + i] = r[i];
-            }
-#pragma unroll
-            for (int i = 0; i < 6; ++i)
-            {
-                T z = T(0);
-#pragma unroll
-                for (int j = 0; j < 6; ++j) z += Wk.Minv[u * 36 + i * 6 + j] * r[j];
-                Wk.z[u * 6 + i] = z;
-                rzn += double(r[i]) * double(z);
-                rr += double(r[i]) * double(r[i]);
-            }
-        }
-        const double s1 = blockSum256(rzn, lds);
         __syncthreads();
-        const double s2 = blockSum256(rr, lds + 4);
+        double gg = 0.0, dd = 0.0, rr = 0.0;
+        if (lead && lane < 6)
+        {
+            T wsum = T(0);
+            for (int w2 = 0; w2 < lead; ++w2) wsum += segq[wv + w2][lane];
+            Wk.cgW[wr][row * 6 + lane] = wsum;
+            gg = double(rOwn) * double(uOwn);
+            dd = double(wsum) * double(uOwn);
+            rr = double(rOwn) * double(rOwn);
+        }
+        gg = waveSum(gg);
+        dd = waveSum(dd);
+        rr = waveSum(rr);
+        if (lane == 0)
+        {
+            red3[wv][0] = gg;
+            red3[wv][1] = dd;
+            red3[wv][2] = rr;
+        }
+        __syncthreads();
         if (threadIdx.x == 0)
         {
-            Wk.partRz[blockIdx.x] = s1;
-            Wk.partRr[blockIdx.x] = s2;
-        }
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-        {
-            if (breakdown)
+            double G = 0.0, Dl = 0.0, R = 0.0;
+            for (int w2 = 0; w2 < kCgWaves; ++w2)
             {
-                Wk.iflags[kIflagBreakdown] = 1;
-                Wk.iflags[kIflagDone] = 1;
-                Wk.iflags[kIflagIters] = it;
+                G += red3[w2][0];
+                Dl += red3[w2][1];
+                R += red3[w2][2];
             }
-            else
-            {
-                // quadratic model Q(x) = x'Sx/2 - b'x decreases by alpha r.z/2 per step; truncated-Newton test
-                // i (Q_i - Q_{i-1}) / Q_i < eta
-                const double dq = -0.5 * alpha * rz;
-                const double qNew = Wk.scalars[(it & 1) ? kScalarQ1 : kScalarQ0] + dq;
-                Wk.scalars[((it + 1) & 1) ? kScalarQ1 : kScalarQ0] = qNew;
-                // Done is not raised here: sibling blocks of this launch may still be reading it. The next
-                // SpMV launch and the host treat ModelStop as done.
-                if (modelEta > 0.0 && qNew < 0.0 && double(it + 1) * dq / qNew < modelEta)
-                {
-                    Wk.iflags[kIflagModelStop] = 1;
-                    Wk.iflags[kIflagIters] = it + 1;
-                }
-            }
+            double* part = Wk.cgPart[wr];
+            part[3 * blockIdx.x] = G;
+            part[3 * blockIdx.x + 1] = Dl;
+            part[3 * blockIdx.x + 2] = R;
         }
     }
 
-    // ---------------------------------------------------------------------------------------------
-    // Persistent preconditioned conjugate gradients (Chronopoulos-Gear form): one SpMV and one pair of
-    // dot products per iteration, two grid barriers per iteration, all iterations in one cooperative
-    // launch. Every block derives the scalars from the same partials in the same order, so all blocks
-    // take identical decisions and execute the same number of barriers; the loop is bounded by maxIter.
+    // r = b and zero s, w, p, x (parity 1, read by the priming launch); resets the PCG flags.
     template <typename T>
-    __device__ __forceinline__ void applyM(const Work<T>& Wk, int u, const T* r, T* out)
+    __global__ void k_pcg_init(Problem P, Work<T> Wk)
     {
-        const T* M = Wk.Minv + u * 36;
-#pragma unroll
-        for (int i = 0; i < 6; ++i)
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < P.nUnits * 6)
         {
-            T z = T(0);
-#pragma unroll
-            for (int j = 0; j < 6; ++j) z += M[i * 6 + j] * r[j];
-            out[i] = z;
+            Wk.cgR[1][i] = T(Wk.rhs[i]);
+            Wk.cgW[1][i] = T(0);
+            Wk.cgS[1][i] = T(0);
+            Wk.p[i] = T(0);
+            Wk.x[i] = T(0);
         }
-    }
-
-    // w = S u for the rows of this block (wave per row, 60 lanes: row element = lane % 6, block slot = lane / 6);
-    // returns this thread's share of (w.u).
-    template <typename T>
-    __device__ __forceinline__ double spmvRows(const Problem& P, const Work<T>& Wk, int ub, int ue, T (*red)[64])
-    {
-        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        double wu = 0.0;
-        for (int u0 = ub; u0 < ue; u0 += 4)
+        if (i == 0)
         {
-            const int u = u0 + wv;
-            T acc = T(0);
-            if (u < ue && lane < 60)
-            {
-                const int row = lane % 6;
-                for (int k = P.rowPtr[u] + lane / 6; k < P.rowPtr[u + 1]; k += 10)
-                {
-                    const T* Sb = Wk.S + k * 36 + row * 6;
-                    const T* pv = Wk.u + P.colIdx[k] * 6;
-                    acc += Sb[0] * pv[0] + Sb[1] * pv[1] + Sb[2] * pv[2] + Sb[3] * pv[3] + Sb[4] * pv[4] + Sb[5] * pv[5];
-                }
-            }
-            red[wv][lane] = acc;
-            __syncthreads();
-            if (u < ue && lane < 6)
-            {
-                T sum = T(0);
-                for (int q = 0; q < 10; ++q) sum += red[wv][lane + 6 * q];
-                Wk.w[u * 6 + lane] = sum;
-                wu += double(sum) * double(Wk.u[u * 6 + lane]);
-            }
-            __syncthreads();
-        }
-        return wu;
-    }
-
-    template <typename T>
-    __global__ void __launch_bounds__(256) k_pcg_persistent(Problem P, Work<T> Wk, int maxIter, double relTol,
-                                                            double absTol)
-    {
-        namespace cg = cooperative_groups;
-        cg::grid_group grid = cg::this_grid();
-        __shared__ T red[4][64];
-        __shared__ double lds[3][4];
-        __shared__ double bc[4];
-        const int nb = gridDim.x;
-        const int ub = P.pcgRowStart[blockIdx.x], ue = P.pcgRowStart[blockIdx.x + 1];
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        // block partial of three values -> Wk.partPq[3 * block + k]
-        auto publish = [&](double a, double b, double c) {
-            a = waveSum(a);
-            b = waveSum(b);
-            c = waveSum(c);
-            if (lane == 0)
-            {
-                lds[0][wv] = a;
-                lds[1][wv] = b;
-                lds[2][wv] = c;
-            }
-            __syncthreads();
-            if (threadIdx.x < 3)
-            {
-                const int k = threadIdx.x;
-                Wk.partPq[3 * blockIdx.x + k] = lds[k][0] + lds[k][1] + lds[k][2] + lds[k][3];
-            }
-            __syncthreads();
-        };
-        // every block sums the partials with the same fixed-order wave reduction, so all blocks hold
-        // bitwise identical scalars
-        auto gather = [&](double* out3) {
-            if (wv == 0)
-            {
-                double a = 0.0, b = 0.0, c = 0.0;
-                for (int i = lane; i < nb; i += 64)
-                {
-                    a += Wk.partPq[3 * i];
-                    b += Wk.partPq[3 * i + 1];
-                    c += Wk.partPq[3 * i + 2];
-                }
-                a = waveSum(a);
-                b = waveSum(b);
-                c = waveSum(c);
-                if (lane == 0)
-                {
-                    bc[0] = a;
-                    bc[1] = b;
-                    bc[2] = c;
-                }
-            }
-            __syncthreads();
-            out3[0] = bc[0];
-            out3[1] = bc[1];
-            out3[2] = bc[2];
-            __syncthreads();
-        };
-
-        // x = 0, r = b, u = M r
-        double ru = 0.0, rr = 0.0;
-        for (int i = threadIdx.x; i < (ue - ub); i += blockDim.x)
-        {
-            const int u = ub + i;
-            T r[6], z[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k)
-            {
-                r[k] = T(Wk.rhs[u * 6 + k]);
-                Wk.r[u * 6 + k] = r[k];
-                Wk.x[u * 6 + k] = T(0);
-            }
-            applyM(Wk, u, r, z);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) Wk.u[u * 6 + k] = z[k];
-        }
-        grid.sync();
-        double wu = spmvRows(P, Wk, ub, ue, red);
-        for (int i = threadIdx.x; i < (ue - ub); i += blockDim.x)
-        {
-            const int u = ub + i;
-#pragma unroll
-            for (int k = 0; k < 6; ++k)
-            {
-                const double rv = double(Wk.r[u * 6 + k]);
-                ru += rv * double(Wk.u[u * 6 + k]);
-                rr += rv * rv;
-            }
-        }
-        publish(ru, wu, rr);
-        grid.sync();
-        double sc[3];
-        gather(sc);
-        double gamma = sc[0], delta = sc[1];
-        const double bnorm = sqrt(sc[2]);
-        const double thr = fmax(relTol * bnorm, absTol);
-        double rnorm = bnorm;
-        int iters = 0;
-        bool done = rnorm <= thr;
-        bool breakdown = false;
-        double eta = delta;
-        double alpha = 0.0;
-        if (!done)
-        {
-            breakdown = !(eta > 0.0) || !pbdev::finite(eta);
-            alpha = breakdown ? 0.0 : gamma / eta;
-        }
-        // p = u, s = w
-        for (int i = threadIdx.x; i < (ue - ub) * 6; i += blockDim.x)
-        {
-            const int e = ub * 6 + i;
-            Wk.p[e] = Wk.u[e];
-            Wk.q[e] = Wk.w[e];
-        }
-        while (!done && !breakdown && iters < maxIter)
-        {
-            // x += alpha p; r -= alpha s; u = M r (rows of this block)
-            const T a = T(alpha);
-            for (int i = threadIdx.x; i < (ue - ub); i += blockDim.x)
-            {
-                const int u = ub + i;
-                T r[6], z[6];
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                {
-                    Wk.x[u * 6 + k] += a * Wk.p[u * 6 + k];
-                    r[k] = Wk.r[u * 6 + k] - a * Wk.q[u * 6 + k];
-                    Wk.r[u * 6 + k] = r[k];
-                }
-                applyM(Wk, u, r, z);
-#pragma unroll
-                for (int k = 0; k < 6; ++k) Wk.u[u * 6 + k] = z[k];
-            }
-            grid.sync();
-            wu = spmvRows(P, Wk, ub, ue, red);
-            ru = 0.0;
-            rr = 0.0;
-            for (int i = threadIdx.x; i < (ue - ub); i += blockDim.x)
-            {
-                const int u = ub + i;
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                {
-                    const double rv = double(Wk.r[u * 6 + k]);
-                    ru += rv * double(Wk.u[u * 6 + k]);
-                    rr += rv * rv;
-                }
-            }
-            publish(ru, wu, rr);
-            grid.sync();
-            gather(sc);
-            ++iters;
-            rnorm = sqrt(sc[2]);
-            if (rnorm <= thr)
-            {
-                done = true;
-                break;
-            }
-            const double gnew = sc[0];
-            delta = sc[1];
-            const double beta = gnew / gamma;
-            eta = delta - beta * gnew / alpha;
-            if (!(eta > 0.0) || !pbdev::finite(eta) || !pbdev::finite(beta))
-            {
-                breakdown = true;
-                break;
-            }
-            alpha = gnew / eta;
-            gamma = gnew;
-            // p = u + beta p; s = w + beta s
-            const T b = T(beta);
-            for (int i = threadIdx.x; i < (ue - ub) * 6; i += blockDim.x)
-            {
-                const int e = ub * 6 + i;
-                Wk.p[e] = Wk.u[e] + b * Wk.p[e];
-                Wk.q[e] = Wk.w[e] + b * Wk.q[e];
-            }
-        }
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-        {
-            Wk.iflags[kIflagDone] = done ? 1 : 0;
-            Wk.iflags[kIflagIters] = iters;
-            Wk.iflags[kIflagBreakdown] = breakdown ? 1 : 0;
-            Wk.scalars[kScalarRnorm] = rnorm;
-            Wk.scalars[kScalarThreshold] = thr;
-            Wk.scalars[kScalarBnorm] = bnorm;
+            Wk.iflags[kIflagDone0] = 0;
+            Wk.iflags[kIflagDone1] = 0;
+            Wk.iflags[kIflagIters] = 0;
+            Wk.iflags[kIflagBreakdown] = 0;
+            Wk.iflags[kIflagModelStop] = 0;
         }
     }
 
@@ -1100,13 +924,14 @@ namespace pbk
                 if (a < lim) gs += x * x;
             }
         }
+        __shared__ double slot[2];
+        const double ddp = blockSumPartials(Wk.partDd, nParts, slot);
+        const double ssp = blockSumPartials(Wk.partSs, nParts, slot + 1);
         const double s1 = blockSum256(dd, lds);
         __syncthreads();
         const double s2 = blockSum256(gs, lds + 4);
         if (threadIdx.x == 0)
         {
-            const double ddp = sumPartials(Wk.partDd, nParts);
-            const double ssp = sumPartials(Wk.partSs, nParts);
             Wk.scalars[kScalarDd] = s1 + ddp;
             Wk.scalars[kScalarRmsGlobal] = P.globalDim > 0 ? sqrt(s2 / P.globalDim) : 0.0;
             Wk.scalars[kScalarRmsPoint] = P.nElim > 0 ? sqrt(ssp / (3.0 * P.nElim)) : 0.0;
@@ -1295,19 +1120,25 @@ namespace pbk
     }
 
     template <typename T, int NA>
-    __global__ void k_trial_reduce(Problem P, Work<T> Wk, int nAlpha, int nParts)
+    __global__ void __launch_bounds__(64 * NA) k_trial_reduce(Problem P, Work<T> Wk, int nAlpha, int nParts)
     {
-        const int a = threadIdx.x;
+        // one wave per step length: lane-strided partial sums in a fixed order, then the wave butterfly
+        const int a = threadIdx.x >> 6, lane = threadIdx.x & 63;
         if (a >= nAlpha) return;
         double s = 0.0;
         int bb = 0;
-        for (int i = 0; i < nParts; ++i)
+        for (int i = lane; i < nParts; i += kWave)
         {
             s += Wk.trialPart[i * NA + a];
             bb |= Wk.trialBad[i * NA + a];
         }
-        for (int g = 0; g < P.nGroup; ++g) s += Wk.candPriorCost[a * P.nGroup + g];
-        Wk.trialCost[a] = bb ? __builtin_inf() : s;
+        s = waveSum(s);
+        const bool bad = __any(bb != 0);
+        if (lane == 0)
+        {
+            for (int g = 0; g < P.nGroup; ++g) s += Wk.candPriorCost[a * P.nGroup + g];
+            Wk.trialCost[a] = bad ? __builtin_inf() : s;
+        }
     }
 
     // Commit candidate ai: cameras, groups and points.
@@ -1379,48 +1210,18 @@ namespace pbk
     }
 
     template <typename T>
-    int Launcher<T>::pcgStart(const Problem& P, const Work<T>& W, double relTol, double absTol, hipStream_t s)
+    void Launcher<T>::pcgInit(const Problem& P, const Work<T>& W, double relTol, double absTol, hipStream_t s)
     {
-        const int nb = std::max(1, blocksFor(P.nUnits, 256));
-        k_pcg_start<T><<<nb, 256, 0, s>>>(P, W);
-        return nb;
+        k_pcg_init<T><<<std::max(1, blocksFor(long(P.nUnits) * 6, 256)), 256, 0, s>>>(P, W);
+        k_pcg_iter<T><<<std::max(1, P.cgBlocks), 64 * kCgWaves, 0, s>>>(P, W, -1, 0, P.cgBlocks, relTol, absTol, 0.0);
     }
 
     template <typename T>
-    void Launcher<T>::pcgSpmv(const Problem& P, const Work<T>& W, int it, double relTol, double absTol, hipStream_t s)
+    void Launcher<T>::pcgIteration(const Problem& P, const Work<T>& W, int it, int maxIt, double relTol, double absTol,
+                                   double modelEta, hipStream_t s)
     {
-        const int nbVec = std::max(1, blocksFor(P.nUnits, 256));
-        k_pcg_spmv<T><<<std::max(1, blocksFor(P.nSegments, 4)), 256, 0, s>>>(P, W, it, nbVec, relTol, absTol);
-    }
-
-    template <typename T>
-    void Launcher<T>::pcgUpdate(const Problem& P, const Work<T>& W, int it, double modelEta, hipStream_t s)
-    {
-        const int nbSpmv = std::max(1, blocksFor(P.nSegments, 4));
-        const int nbVec = std::max(1, blocksFor(P.nUnits, 256));
-        k_pcg_update<T><<<nbVec, 256, 0, s>>>(P, W, it, nbSpmv, modelEta);
-    }
-
-    template <typename T>
-    int Launcher<T>::pcgPersistentBlocks()
-    {
-        int perCu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_pcg_persistent<T>, 256, 0) != hipSuccess) return 0;
-        int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) != hipSuccess) return 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-        return perCu * cus;
-    }
-
-    template <typename T>
-    hipError_t Launcher<T>::pcgPersistent(const Problem& P, const Work<T>& W, int blocks, int maxIter, double relTol,
-                                          double absTol, hipStream_t s)
-    {
-        Problem Pc = P;
-        Work<T> Wc = W;
-        void* args[] = {&Pc, &Wc, &maxIter, &relTol, &absTol};
-        return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent<T>), dim3(blocks), dim3(256),
-                                          args, 0, s);
+        k_pcg_iter<T><<<std::max(1, P.cgBlocks), 64 * kCgWaves, 0, s>>>(P, W, it, maxIt, P.cgBlocks, relTol, absTol,
+                                                                        modelEta);
     }
 
     template <typename T>
@@ -1438,7 +1239,7 @@ namespace pbk
         if (P.nGroup > 0) k_trial_groups<T><<<blocksFor(long(nAlpha) * P.nGroup, 64), 64, 0, s>>>(P, W, alphas, nAlpha);
         k_trial_cameras<T><<<blocksFor(long(nAlpha) * P.nCam, 64), 64, 0, s>>>(P, W, alphas, nAlpha);
         k_trial_cost<T, kMaxAlpha><<<costBlocks, 256, 0, s>>>(P, W, alphas, nAlpha);
-        k_trial_reduce<T, kMaxAlpha><<<1, 64, 0, s>>>(P, W, nAlpha, costBlocks);
+        k_trial_reduce<T, kMaxAlpha><<<1, 64 * kMaxAlpha, 0, s>>>(P, W, nAlpha, costBlocks);
     }
 
     template <typename T>
