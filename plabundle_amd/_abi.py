@@ -176,7 +176,7 @@ def load_library(path: str | None = None):
     global _lib
     if _lib is not None and path is None:
         return _lib
-    lib_path = path or LIBRARY_PATH
+    lib_path = path or os.environ.get("PLABUNDLE_AMD_LIBRARY") or LIBRARY_PATH
     if not os.path.exists(lib_path):
         raise BackendLibraryMissing(f"{lib_path} not found; build the HIP backend first")
     lib = C.CDLL(lib_path, mode=C.RTLD_LOCAL)

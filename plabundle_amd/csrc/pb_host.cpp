@@ -1025,10 +1025,22 @@ namespace
         P.diagPos = A.upload(L.diagPos, s);
         P.unitWidth = A.upload(p.unitWidth, s);
         P.cgBlocks = static_cast<int>(L.cgWaveRow.size()) / pbk::kCgWaves;
-        P.cgWaveRow = A.upload(L.cgWaveRow, s);
-        P.cgWaveBegin = A.upload(L.cgWaveBegin, s);
-        P.cgWaveEnd = A.upload(L.cgWaveEnd, s);
-        P.cgWaveLead = A.upload(L.cgWaveLead, s);
+        {
+            std::vector<int32_t> slots(4 * L.cgWaveRow.size());
+            for (size_t i = 0; i < L.cgWaveRow.size(); ++i)
+            {
+                slots[4 * i] = L.cgWaveRow[i];
+                slots[4 * i + 1] = L.cgWaveBegin[i];
+                slots[4 * i + 2] = L.cgWaveEnd[i];
+                slots[4 * i + 3] = L.cgWaveLead[i];
+            }
+            P.cgSlot = reinterpret_cast<const int4*>(A.upload(slots, s));
+            std::vector<int32_t> direct(pbk::kCgDirect * L.cgWaveRow.size(), -1);
+            for (size_t i = 0; i < L.cgWaveRow.size(); ++i)
+                for (int j = 0; j < pbk::kCgDirect && L.cgWaveBegin[i] + j < L.cgWaveEnd[i]; ++j)
+                    direct[pbk::kCgDirect * i + j] = L.colIdx[L.cgWaveBegin[i] + j];
+            P.cgColDirect = A.upload(direct, s);
+        }
         check(hipStreamSynchronize(s), "structure upload");
         L.pairE1.clear();
         L.pairE1.shrink_to_fit();
@@ -1070,9 +1082,12 @@ namespace
         for (int k = 0; k < 2; ++k)
         {
             W.cgR[k] = A.alloc<T>(size_t(P.nUnits) * 6);
+            W.cgU[k] = A.alloc<T>(size_t(P.nUnits) * 6);
             W.cgW[k] = A.alloc<T>(size_t(P.nUnits) * 6);
+            W.cgMW[k] = A.alloc<T>(size_t(P.nUnits) * 6);
             W.cgS[k] = A.alloc<T>(size_t(P.nUnits) * 6);
-            W.cgPart[k] = A.alloc<double>(size_t(std::max(1, P.cgBlocks)) * 3);
+            W.cgMS[k] = A.alloc<T>(size_t(P.nUnits) * 6);
+            W.cgPart[k] = A.alloc<double>(size_t(P.nUnits) * 3);
         }
         W.partDd = A.alloc<double>(nParts);
         W.partSs = A.alloc<double>(nParts);
@@ -1197,6 +1212,11 @@ namespace
             check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s), "readback");
             check(hipStreamSynchronize(s), "pcg");
         };
+        if (P.cgBlocks == 0)  // no primary unknowns: the zero step solves the empty system
+        {
+            res.converged = true;
+            return res;
+        }
         // fused iterations; the host looks at the flags after chunks that grow with the iteration count
         // (launches after the stop return at once)
         Lr::pcgInit(P, W, relTol, absTol, s);
@@ -1854,8 +1874,8 @@ pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, 
             case 2:  // pairs + entry points + W + Hinv + y + S out
                 b = P.nPairs * 8.0 + P.nEntries * (4.0 + 18 * t) + P.nElim * (9 * t + 24) + P.nNnz * 36 * t;
                 break;
-            case 3:  // S + column ids + preconditioner + r, w, s in + p, x in/out + r, w, s out
-                b = P.nNnz * (36 * t + 4) + P.nUnits * (36 * t + 10 * 6 * t);
+            case 3:  // S + column ids + preconditioner + r, u, w, Mw, s, Ms in and out + p, x in and out
+                b = P.nNnz * (36 * t + 4) + P.nUnits * (36 * t + 16 * 6 * t);
                 break;
             case 4:  // obs (cam, point, uv, w) + point X, dp, elim id
                 b = P.nObs * (4.0 + 4 + 16 + 8) + P.nActive * (24.0 + 4) + P.nElim * 24.0;

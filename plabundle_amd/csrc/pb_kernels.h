@@ -21,7 +21,14 @@ namespace pbk
     constexpr int kGroupPartialStride = 96;  // >= K(K+1)/2 + K for K <= 12
     constexpr int kPairChunk = 2048;         // pairs per Schur work item
     constexpr int kCgWaves = 8;              // waves per workgroup of the fused PCG iteration
-    constexpr int kCgSegment = 10;           // stored blocks per wave before a row gets another wave
+    // SpMV steps per wave whose loads are issued before the scalars are known (more steps cost registers and
+    // with them the occupancy that lets the whole grid be resident at once)
+    template <typename T>
+    constexpr int cgPrefetch() { return sizeof(T) == 4 ? 6 : 4; }
+    constexpr int kCgAlways = 2;             // steps every wave loads (kCgSegment / 10)
+    constexpr int kCgPartLoads = 4;          // unrolled row-partial loads per thread (rows up to 4 * 64 * kCgWaves)
+    constexpr int kCgDirect = 60;            // column ids per slot stored by slot (the prefetched steps)
+    constexpr int kCgSegment = 20;           // stored blocks per wave before a row gets another wave
 
     enum Flag
     {
@@ -54,10 +61,11 @@ namespace pbk
         kScalarQ1,
         kScalarDq0,  // model decrease of the last step
         kScalarDq1,
-        kScalarGamma0,  // (r, M r) of the last step
-        kScalarGamma1,
-        kScalarAlpha0,  // step length of the last step
-        kScalarAlpha1,
+        kScalarInvGamma0,  // 1 / (r, M r) of the last step
+        kScalarInvGamma1,
+        kScalarInvAlpha0,  // 1 / step length of the last step
+        kScalarInvAlpha1,
+        kScalarThreshold2,  // squared PCG residual threshold
         kScalarCount = 24
     };
 
@@ -138,10 +146,8 @@ namespace pbk
         // fused PCG: kCgWaves wave slots per workgroup; a slot walks [begin, end) of one block row, and the
         // first slot of a row (lead = number of slots of the row, all in the same workgroup) owns the row
         int cgBlocks = 0;
-        const int32_t* cgWaveRow = nullptr;  // [cgBlocks*kCgWaves], -1 for an empty slot
-        const int32_t* cgWaveBegin = nullptr;
-        const int32_t* cgWaveEnd = nullptr;
-        const int32_t* cgWaveLead = nullptr;
+        const int4* cgSlot = nullptr;  // [cgBlocks*kCgWaves] {row (-1: empty), begin, end, lead}
+        const int32_t* cgColDirect = nullptr;  // [cgBlocks*kCgWaves*kCgDirect] column of block begin+i, -1 past end
     };
 
     template <typename T>
@@ -164,10 +170,13 @@ namespace pbk
         T* Minv = nullptr;  // [nUnits*36]
         T* x = nullptr;  // PCG solution
         T* p = nullptr;  // PCG direction
-        T* cgR[2] = {nullptr, nullptr};  // fused PCG: residual, by iteration parity
-        T* cgW[2] = {nullptr, nullptr};  // S M r
-        T* cgS[2] = {nullptr, nullptr};  // S p
-        double* cgPart[2] = {nullptr, nullptr};  // [cgBlocks*3] block partials of (r,Mr), (w,Mr), (r,r)
+        T* cgR[2] = {nullptr, nullptr};   // fused PCG vectors by iteration parity: residual r
+        T* cgU[2] = {nullptr, nullptr};   // u = M r
+        T* cgW[2] = {nullptr, nullptr};   // w = S u
+        T* cgMW[2] = {nullptr, nullptr};  // M w
+        T* cgS[2] = {nullptr, nullptr};   // s = S p
+        T* cgMS[2] = {nullptr, nullptr};  // M s
+        double* cgPart[2] = {nullptr, nullptr};  // [nUnits*3] row partials of (r,Mr), (w,Mr), (r,r)
         double* partDd = nullptr;
         double* partSs = nullptr;
         double* scalars = nullptr;

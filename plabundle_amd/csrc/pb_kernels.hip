@@ -588,13 +588,16 @@ namespace pbk
     // ---------------------------------------------------------------------------------------------
     // Fused PCG iteration (Chronopoulos-Gear recurrences): one launch per iteration.
     //
-    // State entering iteration it (parity rd = it & 1): r_it, w_it = S M r_it, s_{it-1}, p_{it-1}, x_it and the
-    // block partials of gamma = (r, M r), delta = (w, M r), rho = (r, r). Every block sums the partials in the
-    // same order, so all blocks derive bitwise the same alpha, beta and stop decision. The owner wave of a
-    // unit row updates s, p, x, r of its row; the SpMV w_{it+1} = S M r_{it+1} needs r_{it+1} of every column
-    // unit, which each wave recomputes from the previous-parity vectors with the same fused operations as
-    // the owner (so the values agree bit for bit). r, s, w are double-buffered by parity; p and x are
-    // touched by their owner only. it = -1 primes w_0 and the partials of iteration 0 (alpha = beta = 0).
+    // State entering iteration it (parity rd = it & 1), per unit row: r_it, u_it = M r_it, w_it = S u~_it,
+    // Mw_it = M w_it, s_{it-1} = S p_{it-1} (by recurrence), Ms_{it-1} = M s_{it-1}, p_{it-1}, x_it, and the
+    // row partials of gamma = (r, u), delta = (w, u), rho = (r, r). Every block sums the partials in the same
+    // order, so all blocks derive bitwise the same alpha, beta and stop decision. The owner wave of a row
+    // updates s = w + beta s, p = u + beta p, x += alpha p, r -= alpha s and stores u = M r, M s and, after the
+    // row's SpMV, w and M w (M is block diagonal, so the owner holds its block). The SpMV w_{it+1} = S u~_{it+1}
+    // needs u_{it+1} of every column unit; a wave evaluates it as u~ = u - alpha (Mw + beta Ms) from the
+    // column's previous-parity values, which equals M r_{it+1} up to the rounding of one step (no
+    // preconditioner block is read per stored block). r, u, w, Mw, s, Ms are double-buffered by parity; p and
+    // x are touched by their owner only. it = -1 primes w_0 and the partials of iteration 0 (alpha = beta = 0).
     template <typename T>
     __device__ __forceinline__ T fmaT(T a, T b, T c);
     template <>
@@ -608,26 +611,21 @@ namespace pbk
         return __builtin_fma(a, b, c);
     }
 
-    // s_it, r_{it+1} entries i6 of unit col, then (M r_{it+1}) entry i6 through the six lanes base..base+5 that
-    // hold the unit. Every lane of the wave must call it (it shuffles).
+    // x_{it+1} = x - alpha (y + beta z): r from (r, w, s) and u from (u, Mw, Ms)
     template <typename T>
-    __device__ __forceinline__ void nextResidual(const Work<T>& Wk, const T* rO, const T* wO, const T* sO, int col,
-                                                 int i6, int base, T a, T bt, T& sn, T& rn, T& un)
+    __device__ __forceinline__ T cgNext(T x, T y, T z, T a, T bt)
     {
-        const int ci = col * 6 + i6;
-        T m[6];
-#pragma unroll
-        for (int j = 0; j < 6; ++j) m[j] = Wk.Minv[col * 36 + i6 * 6 + j];
-        sn = fmaT(bt, sO[ci], wO[ci]);
-        rn = fmaT(-a, sn, rO[ci]);
+        return fmaT(-a, fmaT(bt, z, y), x);
+    }
+
+    // sum_j m[j] v_j where v_j is held by lane base + j. Every lane of the wave must call it (it shuffles).
+    template <typename T>
+    __device__ __forceinline__ T rowTimes(const T (&m)[6], T v, int base)
+    {
         T acc = T(0);
 #pragma unroll
-        for (int j = 0; j < 6; ++j)
-        {
-            const T rj = __shfl(rn, min(base + j, kWave - 1), kWave);
-            acc = fmaT(m[j], rj, acc);
-        }
-        un = acc;
+        for (int j = 0; j < 6; ++j) acc = fmaT(m[j], __shfl(v, min(base + j, kWave - 1), kWave), acc);
+        return acc;
     }
 
     template <typename T>
@@ -639,31 +637,95 @@ namespace pbk
         __shared__ T segq[kCgWaves][6];
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int rd = it & 1, wr = rd ^ 1;  // prime (it = -1) reads parity 1 and writes parity 0
-        // One memory round trip for everything the launch decides on: the stop flag of the previous launch,
-        // the block partials (all threads of the block load them), the scalars and this wave's slot. A launch
-        // reads only what earlier launches wrote, so all blocks and waves see the same values.
-        const int prevDone = Wk.iflags[kIflagDone0 + rd];
+        const int i6 = lane % 6, bslot = lane / 6, base = 6 * bslot;
+        constexpr int kCgPrefetch = cgPrefetch<T>();
+
+        // Round trip 1: everything that depends on nothing - the stop flag of the previous launch, the row
+        // partials (every thread of the block loads its share), the scalars, this wave's slot and the column ids
+        // of its first steps. A launch reads only what earlier launches wrote, so all blocks and waves see the
+        // same values. Loads are unconditional (clamped to valid addresses and masked at their use) so that the
+        // compiler issues them together instead of waiting at branch joins.
         const int slot = blockIdx.x * kCgWaves + wv;
-        const int row = P.cgWaveRow[slot];
-        const int kb = P.cgWaveBegin[slot], ke = P.cgWaveEnd[slot];
-        const int lead = P.cgWaveLead[slot];  // waves of the row when this is its first wave
-        double g = 0.0, d = 0.0, q = 0.0;
-        double thrPrev = 0.0, qIt = 0.0, dqPrev = 0.0, gPrev = 1.0, aPrev = 1.0;
+        const int prevDone = Wk.iflags[kIflagDone0 + rd];
+        const int4 si = P.cgSlot[slot];
+        int col[kCgPrefetch];
+#pragma unroll
+        for (int st = 0; st < kCgPrefetch; ++st) col[st] = P.cgColDirect[slot * kCgDirect + st * 10 + min(bslot, 9)];
+        double pg[kCgPartLoads], pd[kCgPartLoads], pq[kCgPartLoads];
+        double thr2 = 0.0, qIt = 0.0, dqPrev = 0.0, invGammaPrev = 0.0, invAlphaPrev = 0.0;
+        const double* part = Wk.cgPart[rd];
+#pragma unroll
+        for (int k = 0; k < kCgPartLoads; ++k)
+        {
+            const int i = min(int(threadIdx.x) + k * 64 * kCgWaves, nParts - 1);
+            pg[k] = part[3 * i];
+            pd[k] = part[3 * i + 1];
+            pq[k] = part[3 * i + 2];
+        }
         if (it >= 0)
         {
-            const double* part = Wk.cgPart[rd];
-            for (int i = threadIdx.x; i < nParts; i += 64 * kCgWaves)
+            thr2 = Wk.scalars[kScalarThreshold2];
+            qIt = Wk.scalars[kScalarQ0 + rd];
+            dqPrev = Wk.scalars[kScalarDq0 + rd];
+            invGammaPrev = Wk.scalars[kScalarInvGamma0 + rd];
+            invAlphaPrev = Wk.scalars[kScalarInvAlpha0 + rd];
+        }
+        const int row = si.x, kb = si.y, ke = si.z, lead = si.w;  // lead: waves of the row on its first wave
+        const int nSteps = row >= 0 ? (ke - kb + 9) / 10 : 0;      // uniform over the wave
+        const int rowSafe = row >= 0 ? row : 0;
+        const T* uO = Wk.cgU[rd];
+        const T* mwO = Wk.cgMW[rd];
+        const T* msO = Wk.cgMS[rd];
+
+        // Round trip 2: S rows and the column units' u, Mw, Ms, and the owner's own entries. The first kCgAlways
+        // steps and the owner entries are loaded by every wave (so the compiler can count them and the partials
+        // below wait for nothing else); further steps only by waves that have them. Lanes past the segment load
+        // their own row's data and are masked at the use.
+        bool act[kCgPrefetch];
+        T Srow[kCgPrefetch][6], uC[kCgPrefetch], mwC[kCgPrefetch], msC[kCgPrefetch];
+        auto loadStep = [&](int st) {
+            const int k = act[st] ? kb + st * 10 + bslot : kb;
+            const int c = act[st] ? col[st] : rowSafe;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) Srow[st][j] = Wk.S[size_t(k) * 36 + i6 * 6 + j];
+            uC[st] = uO[c * 6 + i6];
+            mwC[st] = mwO[c * 6 + i6];
+            msC[st] = msO[c * 6 + i6];
+        };
+#pragma unroll
+        for (int st = 0; st < kCgPrefetch; ++st) act[st] = lane < 60 && col[st] >= 0;
+#pragma unroll
+        for (int st = 0; st < kCgAlways; ++st) loadStep(st);
+        const bool own = lead && lane < 6;
+        const int oi = lane < 6 ? lane : 0;
+        const int gi = rowSafe * 6 + oi;
+        const T rOld = Wk.cgR[rd][gi], uOld = uO[gi], wOld = Wk.cgW[rd][gi];
+        const T sOld = Wk.cgS[rd][gi], pOld = Wk.p[gi], xOld = Wk.x[gi];
+        T mOwn[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) mOwn[j] = Wk.Minv[rowSafe * 36 + oi * 6 + j];
+#pragma unroll
+        for (int st = kCgAlways; st < kCgPrefetch; ++st)
+            if (st < nSteps) loadStep(st);
+        double g = 0.0, d = 0.0, q = 0.0;
+        if (it >= 0)
+        {
+#pragma unroll
+            for (int k = 0; k < kCgPartLoads; ++k)
+                if (int(threadIdx.x) + k * 64 * kCgWaves < nParts)
+                {
+                    g += pg[k];
+                    d += pd[k];
+                    q += pq[k];
+                }
+            for (int i = threadIdx.x + kCgPartLoads * 64 * kCgWaves; i < nParts; i += 64 * kCgWaves)
             {
                 g += part[3 * i];
                 d += part[3 * i + 1];
                 q += part[3 * i + 2];
             }
-            thrPrev = Wk.scalars[kScalarThreshold];
-            qIt = Wk.scalars[kScalarQ0 + rd];
-            dqPrev = Wk.scalars[kScalarDq0 + rd];
-            gPrev = Wk.scalars[kScalarGamma0 + rd];
-            aPrev = Wk.scalars[kScalarAlpha0 + rd];
         }
+
         if (prevDone)
         {
             if (blockIdx.x == 0 && threadIdx.x == 0) Wk.iflags[kIflagDone0 + wr] = 1;
@@ -683,24 +745,30 @@ namespace pbk
             }
             __syncthreads();
             double gamma = 0.0, delta = 0.0, rho = 0.0;
+#pragma unroll
             for (int w2 = 0; w2 < kCgWaves; ++w2)
             {
                 gamma += red3[w2][0];
                 delta += red3[w2][1];
                 rho += red3[w2][2];
             }
-            __syncthreads();  // red3 is reused for this launch's partials
-            const double rnorm = sqrt(rho);
-            const double thr = it == 0 ? fmax(relTol * rnorm, absTol) : thrPrev;
-            const bool conv = rnorm <= thr;
+            // the threshold, the reciprocals and the model values come from the previous launch, so that the
+            // only division on the path of every block is alpha's
+            double thrSq = thr2;
+            if (it == 0)
+            {
+                const double thr = fmax(relTol * sqrt(rho), absTol);
+                thrSq = thr * thr;
+            }
+            const bool conv = rho <= thrSq;
             // truncated-Newton test of the previous step on the quadratic model Q(x) = x'Sx/2 - b'x:
-            // it (Q_it - Q_{it-1}) / Q_it < eta
-            const bool modelStop = !conv && it > 0 && modelEta > 0.0 && qIt < 0.0 && double(it) * dqPrev / qIt < modelEta;
+            // it (Q_it - Q_{it-1}) / Q_it < eta, multiplied out with Q_it < 0
+            const bool modelStop = !conv && it > 0 && modelEta > 0.0 && qIt < 0.0 && double(it) * dqPrev > modelEta * qIt;
             double den = delta;
             if (it > 0)
             {
-                beta = gamma / gPrev;
-                den = delta - beta * gamma / aPrev;
+                beta = gamma * invGammaPrev;
+                den = delta - beta * gamma * invAlphaPrev;
             }
             alpha = gamma / den;
             const bool breakdown = !conv && !modelStop &&
@@ -708,11 +776,12 @@ namespace pbk
             const bool stop = conv || modelStop || breakdown || it >= maxIt;
             if (blockIdx.x == 0 && threadIdx.x == 0)
             {
-                Wk.scalars[kScalarRnorm] = rnorm;
+                Wk.scalars[kScalarRnorm] = sqrt(rho);
                 if (it == 0)
                 {
-                    Wk.scalars[kScalarThreshold] = thr;
-                    Wk.scalars[kScalarBnorm] = rnorm;
+                    Wk.scalars[kScalarThreshold] = sqrt(thrSq);
+                    Wk.scalars[kScalarThreshold2] = thrSq;
+                    Wk.scalars[kScalarBnorm] = sqrt(rho);
                 }
                 if (stop)
                 {
@@ -726,8 +795,8 @@ namespace pbk
                     const double dq = -0.5 * alpha * gamma;
                     Wk.scalars[kScalarQ0 + wr] = qIt + dq;
                     Wk.scalars[kScalarDq0 + wr] = dq;
-                    Wk.scalars[kScalarGamma0 + wr] = gamma;
-                    Wk.scalars[kScalarAlpha0 + wr] = alpha;
+                    Wk.scalars[kScalarInvGamma0 + wr] = 1.0 / gamma;
+                    Wk.scalars[kScalarInvAlpha0 + wr] = 1.0 / alpha;
                 }
             }
             if (stop) return;  // every block takes the same decision
@@ -738,65 +807,53 @@ namespace pbk
             Wk.scalars[kScalarDq0] = 0.0;
         }
         const T a = T(alpha), bt = T(beta);
-        const T* rO = Wk.cgR[rd];
-        const T* wO = Wk.cgW[rd];
-        const T* sO = Wk.cgS[rd];
-        const int i6 = lane % 6, bslot = lane / 6, base = 6 * bslot;
-        // SpMV segment of the row: ten 6x6 blocks per step, lane (bslot, i6) takes row i6 of block bslot
+
+        // SpMV segment: w_{it+1} = S u_{it+1}, ten 6x6 blocks per step, lane (bslot, i6) takes row i6 of block bslot
         T acc = T(0);
-        if (row >= 0)
+#pragma unroll
+        for (int st = 0; st < kCgPrefetch; ++st)
         {
-            const int nSteps = (ke - kb + 9) / 10;
-            for (int st = 0; st < nSteps; ++st)
+            if (st < nSteps)
             {
-                const int k = kb + st * 10 + bslot;
-                const bool act = lane < 60 && k < ke;
-                const int kk = act ? k : kb;
-                const int col = P.colIdx[kk];
-                T Srow[6];
-#pragma unroll
-                for (int j = 0; j < 6; ++j) Srow[j] = Wk.S[size_t(kk) * 36 + i6 * 6 + j];
-                T sn, rn, un;
-                nextResidual(Wk, rO, wO, sO, col, i6, base, a, bt, sn, rn, un);
-                T sum = T(0);
-#pragma unroll
-                for (int j = 0; j < 6; ++j)
-                {
-                    const T uj = __shfl(un, min(base + j, kWave - 1), kWave);
-                    sum = fmaT(Srow[j], uj, sum);
-                }
-                if (act) acc += sum;
+                const T un = cgNext(uC[st], mwC[st], msC[st], a, bt);
+                const T sum = rowTimes(Srow[st], un, base);
+                if (act[st]) acc += sum;
             }
         }
-        // owner update of row `row` (lanes 0..5 hold its six entries)
-        T rOwn = T(0), uOwn = T(0);
-        if (lead)
+        for (int st = kCgPrefetch; st < nSteps; ++st)  // rows longer than the prefetch window
         {
-            const bool own = lane < 6;
-            const int gi = row * 6 + (own ? lane : 0);
-            const T rOld = rO[gi], pOld = Wk.p[gi], xOld = Wk.x[gi];
-            T uOld = T(0);
+            const int k = kb + st * 10 + bslot;
+            const bool on = lane < 60 && k < ke;
+            const int kk = on ? k : kb;
+            const int c = on ? P.colIdx[kk] : rowSafe;
+            T Sr[6];
 #pragma unroll
-            for (int j = 0; j < 6; ++j)
-            {
-                const T rj = __shfl(rOld, j, kWave);
-                uOld = fmaT(Wk.Minv[row * 36 + (own ? lane : 0) * 6 + j], rj, uOld);
-            }
-            T sn, rn, un;
-            nextResidual(Wk, rO, wO, sO, row, own ? lane : 0, 0, a, bt, sn, rn, un);
-            if (own)
-            {
-                const T pn = fmaT(bt, pOld, uOld);
-                Wk.cgS[wr][gi] = sn;
-                Wk.cgR[wr][gi] = rn;
-                Wk.p[gi] = pn;
-                Wk.x[gi] = fmaT(a, pn, xOld);
-            }
-            rOwn = rn;
-            uOwn = un;
+            for (int j = 0; j < 6; ++j) Sr[j] = Wk.S[size_t(kk) * 36 + i6 * 6 + j];
+            const T un = cgNext(uO[c * 6 + i6], mwO[c * 6 + i6], msO[c * 6 + i6], a, bt);
+            const T sum = rowTimes(Sr, un, base);
+            if (on) acc += sum;
         }
+
+        // owner update of row `row` (lanes 0..5 hold its six entries). The owner stores u = M r and M s computed
+        // from its own block, so the column recurrence above starts every iteration from exact values and its
+        // rounding does not accumulate.
+        const T sn = fmaT(bt, sOld, wOld);
+        const T rn = cgNext(rOld, wOld, sOld, a, bt);
+        const T un = rowTimes(mOwn, rn, 0);
+        const T msn = rowTimes(mOwn, sn, 0);
+        if (own)
+        {
+            const T pn = fmaT(bt, pOld, uOld);
+            Wk.cgS[wr][gi] = sn;
+            Wk.cgMS[wr][gi] = msn;
+            Wk.cgR[wr][gi] = rn;
+            Wk.cgU[wr][gi] = un;
+            Wk.p[gi] = pn;
+            Wk.x[gi] = fmaT(a, pn, xOld);
+        }
+        // row sums: ten block slots of this wave (same wave, no barrier), then the row's waves
         red[wv][lane] = acc;
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
         if (lane < 6)
         {
             T s6 = T(0);
@@ -805,52 +862,56 @@ namespace pbk
             segq[wv][lane] = s6;
         }
         __syncthreads();
-        double gg = 0.0, dd = 0.0, rr = 0.0;
-        if (lead && lane < 6)
+        if (lead)
         {
             T wsum = T(0);
-            for (int w2 = 0; w2 < lead; ++w2) wsum += segq[wv + w2][lane];
-            Wk.cgW[wr][row * 6 + lane] = wsum;
-            gg = double(rOwn) * double(uOwn);
-            dd = double(wsum) * double(uOwn);
-            rr = double(rOwn) * double(rOwn);
-        }
-        gg = waveSum(gg);
-        dd = waveSum(dd);
-        rr = waveSum(rr);
-        if (lane == 0)
-        {
-            red3[wv][0] = gg;
-            red3[wv][1] = dd;
-            red3[wv][2] = rr;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0)
-        {
-            double G = 0.0, Dl = 0.0, R = 0.0;
-            for (int w2 = 0; w2 < kCgWaves; ++w2)
+            if (lane < 6)
+                for (int w2 = 0; w2 < lead; ++w2) wsum += segq[wv + w2][lane];
+            const T mw = rowTimes(mOwn, wsum, 0);  // M w_{it+1}, row of the owner's block
+            double gg = 0.0, dd = 0.0, rr = 0.0;
+            if (own)
             {
-                G += red3[w2][0];
-                Dl += red3[w2][1];
-                R += red3[w2][2];
+                Wk.cgW[wr][gi] = wsum;
+                Wk.cgMW[wr][gi] = mw;
+                gg = double(rn) * double(un);
+                dd = double(wsum) * double(un);
+                rr = double(rn) * double(rn);
             }
-            double* part = Wk.cgPart[wr];
-            part[3 * blockIdx.x] = G;
-            part[3 * blockIdx.x + 1] = Dl;
-            part[3 * blockIdx.x + 2] = R;
+            // lanes 0..7 (6 and 7 hold zero): three butterfly steps, lane 0 writes the row partials
+#pragma unroll
+            for (int off = 1; off < 8; off <<= 1)
+            {
+                gg += __shfl_xor(gg, off, kWave);
+                dd += __shfl_xor(dd, off, kWave);
+                rr += __shfl_xor(rr, off, kWave);
+            }
+            if (lane == 0)
+            {
+                double* outp = Wk.cgPart[wr];
+                outp[3 * row] = gg;
+                outp[3 * row + 1] = dd;
+                outp[3 * row + 2] = rr;
+            }
         }
     }
 
-    // r = b and zero s, w, p, x (parity 1, read by the priming launch); resets the PCG flags.
+    // r = b, u = M b and zero w, Mw, s, Ms, p, x (parity 1, read by the priming launch); resets the PCG flags.
     template <typename T>
     __global__ void k_pcg_init(Problem P, Work<T> Wk)
     {
         const int i = blockIdx.x * blockDim.x + threadIdx.x;
         if (i < P.nUnits * 6)
         {
+            const int u = i / 6, row = i % 6;
+            T z = T(0);
+#pragma unroll
+            for (int j = 0; j < 6; ++j) z = fmaT(Wk.Minv[u * 36 + row * 6 + j], T(Wk.rhs[u * 6 + j]), z);
             Wk.cgR[1][i] = T(Wk.rhs[i]);
+            Wk.cgU[1][i] = z;
             Wk.cgW[1][i] = T(0);
+            Wk.cgMW[1][i] = T(0);
             Wk.cgS[1][i] = T(0);
+            Wk.cgMS[1][i] = T(0);
             Wk.p[i] = T(0);
             Wk.x[i] = T(0);
         }
@@ -1212,15 +1273,17 @@ namespace pbk
     template <typename T>
     void Launcher<T>::pcgInit(const Problem& P, const Work<T>& W, double relTol, double absTol, hipStream_t s)
     {
+        if (P.cgBlocks == 0) return;
         k_pcg_init<T><<<std::max(1, blocksFor(long(P.nUnits) * 6, 256)), 256, 0, s>>>(P, W);
-        k_pcg_iter<T><<<std::max(1, P.cgBlocks), 64 * kCgWaves, 0, s>>>(P, W, -1, 0, P.cgBlocks, relTol, absTol, 0.0);
+        k_pcg_iter<T><<<std::max(1, P.cgBlocks), 64 * kCgWaves, 0, s>>>(P, W, -1, 0, P.nUnits, relTol, absTol, 0.0);
     }
 
     template <typename T>
     void Launcher<T>::pcgIteration(const Problem& P, const Work<T>& W, int it, int maxIt, double relTol, double absTol,
                                    double modelEta, hipStream_t s)
     {
-        k_pcg_iter<T><<<std::max(1, P.cgBlocks), 64 * kCgWaves, 0, s>>>(P, W, it, maxIt, P.cgBlocks, relTol, absTol,
+        if (P.cgBlocks == 0) return;
+        k_pcg_iter<T><<<std::max(1, P.cgBlocks), 64 * kCgWaves, 0, s>>>(P, W, it, maxIt, P.nUnits, relTol, absTol,
                                                                         modelEta);
     }
 
