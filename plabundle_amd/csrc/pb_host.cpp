@@ -747,12 +747,18 @@ namespace
         }
     };
 
+    struct TimedSpan
+    {
+        hipEvent_t start, stop;
+        int launches;
+    };
+
     struct Family
     {
         int64_t launches = 0;
         double ms = 0.0;
         double bytes = 0.0;
-        std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+        std::vector<TimedSpan> pending;
     };
 
     const char* const kFamilies[] = {"linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost",
@@ -762,9 +768,9 @@ namespace
     constexpr int kPinPcg = 40;
     constexpr int kPinPcgFlags = 72;
     static_assert(pbk::kScalarCount <= kPinFlags && kPinPcg + pbk::kScalarCount <= kPinPcgFlags, "pinned layout");
-    // PCG launches are timed on every kPcgSampleEvery-th iteration only, so that the events do not
-    // stretch the timed solve; the family average is the average over the sampled launches
-    constexpr int kPcgSampleEvery = 16;
+    // PCG iterations are timed per chunk of back-to-back launches between two host polls (a pair of events
+    // per chunk, so the events do not stretch the solve); chunks that contain the stop are not counted, so
+    // every timed launch did a full iteration
     constexpr int kFamilyCount = sizeof(kFamilies) / sizeof(kFamilies[0]);
 }  // namespace
 
@@ -827,7 +833,7 @@ namespace
             hipEvent_t start = takeEvent(ctx);
             stop = takeEvent(ctx);
             check(hipEventRecord(start, ctx->stream), "hipEventRecord");
-            ctx->fam[family].pending.emplace_back(start, stop);
+            ctx->fam[family].pending.push_back({start, stop, 1});
         }
         ~Span()
         {
@@ -841,12 +847,12 @@ namespace
         check(hipStreamSynchronize(ctx->stream), "sync");
         for (Family& f : ctx->fam)
         {
-            for (auto& pr : f.pending)
+            for (const TimedSpan& sp : f.pending)
             {
                 float ms = 0.0f;
-                check(hipEventElapsedTime(&ms, pr.first, pr.second), "hipEventElapsedTime");
+                check(hipEventElapsedTime(&ms, sp.start, sp.stop), "hipEventElapsedTime");
                 f.ms += ms;
-                ++f.launches;
+                f.launches += sp.launches;
             }
             f.pending.clear();
         }
@@ -1225,13 +1231,19 @@ namespace
         {
             const int chunk = std::min(std::max(checkEvery, it / 4), 256);
             const int last = std::min(it + chunk, maxIter + 1);
-            for (; it < last; ++it)
+            TimedSpan ts{nullptr, nullptr, last - it};
+            if (ctx->profile)
             {
-                Span sp(ctx, 3, it % kPcgSampleEvery == 0);
-                Lr::pcgIteration(P, W, it, maxIter, relTol, absTol, modelEta, s);
+                ts.start = takeEvent(ctx);
+                ts.stop = takeEvent(ctx);
+                check(hipEventRecord(ts.start, s), "hipEventRecord");
             }
+            for (; it < last; ++it) Lr::pcgIteration(P, W, it, maxIter, relTol, absTol, modelEta, s);
+            if (ctx->profile) check(hipEventRecord(ts.stop, s), "hipEventRecord");
             readback();
-            if (hf[pbk::kIflagDone0] || hf[pbk::kIflagDone1] || it > maxIter) break;
+            const bool done = hf[pbk::kIflagDone0] || hf[pbk::kIflagDone1] || it > maxIter;
+            if (ctx->profile && !done) ctx->fam[3].pending.push_back(ts);
+            if (done) break;
         }
         res.breakdown = hf[pbk::kIflagBreakdown] != 0;
         res.iterations = hf[pbk::kIflagIters];
@@ -1874,8 +1886,9 @@ pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, 
             case 2:  // pairs + entry points + W + Hinv + y + S out
                 b = P.nPairs * 8.0 + P.nEntries * (4.0 + 18 * t) + P.nElim * (9 * t + 24) + P.nNnz * 36 * t;
                 break;
-            case 3:  // S + column ids + preconditioner + r, u, w, Mw, s, Ms in and out + p, x in and out
-                b = P.nNnz * (36 * t + 4) + P.nUnits * (36 * t + 16 * 6 * t);
+            case 3:  // S + column ids; per unit the preconditioner block, 17 six-vectors (r, u, w, s, p, x in;
+                     // column u, Mw, Ms; r, u, w, Mw, s, Ms, p, x out) and the partials in and out
+                b = P.nNnz * (36 * t + 4) + P.nUnits * (36 * t + 17 * 6 * t + 48);
                 break;
             case 4:  // obs (cam, point, uv, w) + point X, dp, elim id
                 b = P.nObs * (4.0 + 4 + 16 + 8) + P.nActive * (24.0 + 4) + P.nElim * 24.0;
