@@ -214,6 +214,8 @@ namespace pbdev
     }
 
     // Full linearisation of one observation. nActive/activeSlots select the intrinsic columns.
+    // KM: compile-time bound on the active intrinsic slots (0, 6 or 12) so that Ji lives in registers.
+    template <int KM>
     PB_HD bool linearize(const Camera& c, double X0, double X1, double X2, double ou, double ov,
                                               double ow, int loss, double lscale, int nActive,
                                               const int8_t* activeSlots, Linearization& L)
@@ -269,10 +271,12 @@ namespace pbdev
         const double ps = pointScale(X0, X1, X2);
 #pragma unroll
         for (int i = 0; i < 6; ++i) L.Jp[i] = jp[i] * ps;
-        for (int k = 0; k < nActive; ++k)
+        // fixed trip count so that Ji stays in registers (a runtime bound would index it dynamically)
+#pragma unroll
+        for (int k = 0; k < KM; ++k)
         {
-            double du, dv;
-            slotDerivative(c, d, x, y, activeSlots[k], du, dv);
+            double du = 0.0, dv = 0.0;
+            if (k < nActive) slotDerivative(c, d, x, y, activeSlots[k], du, dv);
             L.Ji[k] = du;
             L.Ji[kMaxActive + k] = dv;
         }

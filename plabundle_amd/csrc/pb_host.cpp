@@ -406,14 +406,29 @@ namespace
 
     void buildLayout(const Input& in, const Prepared& p, Layout* L)
     {
-        // active tracks and their usable observations (track order)
-        std::vector<int32_t> activeIndex(in.nTracks, -1);
+        // Active tracks and their usable observations. The device order of the active tracks (and with it of
+        // the eliminated points, their cross-block entries and observations) is sorted by the lowest camera
+        // that observes the track: cameras are numbered along the block, so the points shared by a pair of
+        // cameras sit close together and the Schur pairs of one block read neighbouring entries. The order
+        // is internal; block numbering and masks keep the reference's track order.
+        std::vector<int32_t> minCam;
         for (int t = 0; t < in.nTracks; ++t)
             if (p.activeTrack[t])
             {
-                activeIndex[t] = static_cast<int32_t>(L->activeTrackOf.size());
+                int m = std::numeric_limits<int>::max();
+                for (int64_t i = in.off[t]; i < in.off[t + 1]; ++i)
+                    if (usable(in, i)) m = std::min(m, in.obsCam[i]);
                 L->activeTrackOf.push_back(t);
+                minCam.push_back(m);
             }
+        {
+            std::vector<int32_t> order(L->activeTrackOf.size());
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return minCam[x] < minCam[y]; });
+            std::vector<int32_t> sorted(order.size());
+            for (size_t i = 0; i < order.size(); ++i) sorted[i] = L->activeTrackOf[order[i]];
+            L->activeTrackOf.swap(sorted);
+        }
         L->nActive = static_cast<int>(L->activeTrackOf.size());
         L->obsOffset.assign(L->nActive + 1, 0);
         for (int a = 0; a < L->nActive; ++a)
@@ -435,11 +450,11 @@ namespace
         L->nObs = static_cast<int>(L->obsCam.size());
         L->pointElim.assign(L->nActive, -1);
         for (int a = 0; a < L->nActive; ++a)
-        {
-            const int b = p.trackBlock[L->activeTrackOf[a]];
-            L->pointElim[a] = b;
-            if (b >= 0) L->elimTrack.push_back(a);
-        }
+            if (p.trackBlock[L->activeTrackOf[a]] >= 0)
+            {
+                L->pointElim[a] = static_cast<int32_t>(L->elimTrack.size());
+                L->elimTrack.push_back(a);
+            }
         L->nElim = static_cast<int>(L->elimTrack.size());
         // camera units and groups
         const int n = in.nCam;

@@ -111,8 +111,8 @@ namespace pbk
 
     // ---------------------------------------------------------------------------------------------
     // Point side of the linearisation: Hpp, bp, cross blocks W (6x3 per entry), damped inverse.
-    template <typename T>
-    __global__ void k_linearize_points(Problem P, Work<T> Wk, double damping)
+    template <typename T, int KM>
+    __global__ void __launch_bounds__(128) k_linearize_points(Problem P, Work<T> Wk, double damping)
     {
         const int b = blockIdx.x * blockDim.x + threadIdx.x;
         if (b >= P.nElim) return;
@@ -130,7 +130,7 @@ namespace pbk
             const int g = P.camGroup[c];
             const int K = g >= 0 ? P.groupK[g] : 0;
             const int8_t* act = g >= 0 ? P.groupActive + g * kMaxActive : nullptr;
-            if (!pbdev::linearize(P.cams[c], X0, X1, X2, P.obsUV[2 * o], P.obsUV[2 * o + 1], P.obsW[o], P.loss,
+            if (!pbdev::linearize<KM>(P.cams[c], X0, X1, X2, P.obsUV[2 * o], P.obsUV[2 * o + 1], P.obsW[o], P.loss,
                                   P.lossScale, K, act, L))
             {
                 setFlag(Wk.flags, kFlagInvalid);
@@ -157,8 +157,10 @@ namespace pbk
             const int eg = P.obsGroupEntry[o];
             if (eg >= 0)
             {
-                for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int k = 0; k < KM; ++k)
                 {
+                    if (k >= K) break;
                     T* Wg = Wk.W + (eg + k / 6) * 18 + (k % 6) * 3;
 #pragma unroll
                     for (int i = 0; i < 3; ++i)
@@ -223,7 +225,7 @@ namespace pbk
         {
             const int o = P.camObs[k];
             const int t = P.obsPoint[o];
-            if (!pbdev::linearize(cam, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2], P.obsUV[2 * o],
+            if (!pbdev::linearize<KM>(cam, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2], P.obsUV[2 * o],
                                   P.obsUV[2 * o + 1], P.obsW[o], P.loss, P.lossScale, KM > 0 ? K : 0, act, L))
             {
                 setFlag(Wk.flags, kFlagInvalid);
@@ -1237,7 +1239,13 @@ namespace pbk
     template <typename T>
     void Launcher<T>::linearizePoints(const Problem& P, const Work<T>& W, double damping, hipStream_t s)
     {
-        if (P.nElim > 0) k_linearize_points<T><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
+        if (P.nElim == 0) return;
+        if (P.maxGroupK == 0)
+            k_linearize_points<T, 0><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
+        else if (P.maxGroupK <= 6)
+            k_linearize_points<T, 6><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
+        else
+            k_linearize_points<T, 12><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
     }
 
     template <typename T>
