@@ -1891,15 +1891,15 @@ pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, 
             double b = 0.0;
             switch (f)
             {
-            case 0:  // obs (cam, uv, w, two entry ids) + point X + W out + Hinv/bp/y out
-                b = P.nObs * (4.0 + 16 + 8 + 8) + P.nActive * 4.0 * 2 + P.nElim * (24.0 + 4 + 8 + 9 * t + 48) +
+            case 0:  // obs (cam, uv, w, two entry ids) + point X + cross blocks out + L, bp, z out
+                b = P.nObs * (4.0 + 16 + 8 + 8) + P.nActive * 4.0 * 2 + P.nElim * (24.0 + 4 + 6 * t + 48) +
                     P.nEntries * 18 * t;
                 break;
             case 1:  // camera permutation + obs (point id, uv, w) + point X
                 b = P.nObs * (4.0 + 4 + 16 + 8) + P.nActive * 24.0;
                 break;
-            case 2:  // pairs + entry points + W + Hinv + y + S out
-                b = P.nPairs * 8.0 + P.nEntries * (4.0 + 18 * t) + P.nElim * (9 * t + 24) + P.nNnz * 36 * t;
+            case 2:  // pairs + cross blocks U + z of the points of diagonal blocks + S out
+                b = P.nPairs * 8.0 + P.nEntries * (4.0 + 18 * t) + P.nElim * 24.0 + P.nNnz * 36 * t;
                 break;
             case 3:  // S + column ids; per unit the preconditioner block, 17 six-vectors (r, u, w, s, p, x in;
                      // column u, Mw, Ms; r, u, w, Mw, s, Ms, p, x out) and the partials in and out
@@ -1908,8 +1908,8 @@ pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, 
             case 4:  // obs (cam, point, uv, w) + point X, dp, elim id
                 b = P.nObs * (4.0 + 4 + 16 + 8) + P.nActive * (24.0 + 4) + P.nElim * 24.0;
                 break;
-            case 5:  // entries W + units + point bp/Hinv + dp out
-                b = P.nEntries * (18 * t + 4) + P.nElim * (24.0 + 9 * t + 24 + 8);
+            case 5:  // cross blocks U + units + point z, L, bp + dp out
+                b = P.nEntries * (18 * t + 4) + P.nElim * (24.0 + 6 * t + 24 + 24);
                 break;
             default:
                 break;

@@ -20,6 +20,7 @@ namespace pbk
     constexpr int kMaxAlpha = 8;             // step lengths evaluated per trial-cost pass
     constexpr int kGroupPartialStride = 96;  // >= K(K+1)/2 + K for K <= 12
     constexpr int kPairChunk = 2048;         // pairs per Schur work item
+    constexpr int kXcds = 8;                 // accelerator dies (XCDs) of an MI355X, each with its own L2
     constexpr int kCgWaves = 8;              // waves per workgroup of the fused PCG iteration
     // SpMV steps per wave whose loads are issued before the scalars are known (more steps cost registers and
     // with them the occupancy that lets the whole grid be resident at once)
@@ -153,10 +154,10 @@ namespace pbk
     template <typename T>
     struct Work
     {
-        T* W = nullptr;       // [nEntries*18]
-        T* Hinv = nullptr;    // [nElim*9]
+        T* W = nullptr;       // [nEntries*18] cross blocks W, replaced by U = W L once the point is eliminated
+        T* Hinv = nullptr;    // [nElim*9] first 6: lower factor L of Hpp^-1 = L L^T (L00, L10, L11, L20, L21, L22)
         double* bp = nullptr; // [nElim*3]
-        double* y = nullptr;  // Hinv * bp
+        double* y = nullptr;  // z = L^T bp
         double* dp = nullptr; // point step
         double* D = nullptr;  // direct part per upper block [nBlocks*36]
         double* directRhs = nullptr;  // [nUnits*6]

@@ -26,6 +26,14 @@ namespace pbk
 
     constexpr int kWave = 64;
 
+    // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on XCD b % kXcds). For a grid whose
+    // size is a multiple of kXcds this gives XCD x the contiguous range [x * n / kXcds, (x + 1) * n / kXcds) of
+    // work indices, so neighbouring work items, which share data, share an L2.
+    __device__ __forceinline__ int xcdLocal(int b, int n)
+    {
+        return (b % kXcds) * (n / kXcds) + b / kXcds;
+    }
+
     template <typename V>
     __device__ __forceinline__ V waveSum(V v)
     {
@@ -181,13 +189,50 @@ namespace pbk
 #pragma unroll
             for (int i = 0; i < 9; ++i) inv[i] = 0.0;
         }
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Wk.Hinv[9 * b + i] = T(inv[i]);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
+        // Factor the inverse, Hpp^-1 = L L^T (L lower), and store U = W L in place of each cross block W: then
+        // W Hpp^-1 W'^T = U U'^T, W Hpp^-1 bp = U z with z = L^T bp, and Hpp^-1 (bp - W^T dc) = L (z - U^T dc).
+        double Lf[6];  // L00, L10, L11, L20, L21, L22
         {
-            Wk.bp[3 * b + i] = bp[i];
-            Wk.y[3 * b + i] = inv[3 * i] * bp[0] + inv[3 * i + 1] * bp[1] + inv[3 * i + 2] * bp[2];
+            const double l00 = sqrt(inv[0]);
+            const double l10 = inv[3] / l00, l20 = inv[6] / l00;
+            const double l11 = sqrt(inv[4] - l10 * l10);
+            const double l21 = (inv[7] - l20 * l10) / l11;
+            const double l22 = sqrt(inv[8] - l20 * l20 - l21 * l21);
+            Lf[0] = l00;
+            Lf[1] = l10;
+            Lf[2] = l11;
+            Lf[3] = l20;
+            Lf[4] = l21;
+            Lf[5] = l22;
+            bool ok = true;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) ok = ok && pbdev::finite(Lf[i]);
+            if (!ok)
+            {
+                setFlag(Wk.flags, kFlagSingular);
+#pragma unroll
+                for (int i = 0; i < 6; ++i) Lf[i] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) Wk.Hinv[9 * b + i] = T(Lf[i]);
+        Wk.bp[3 * b] = bp[0];
+        Wk.bp[3 * b + 1] = bp[1];
+        Wk.bp[3 * b + 2] = bp[2];
+        Wk.y[3 * b] = Lf[0] * bp[0] + Lf[1] * bp[1] + Lf[3] * bp[2];
+        Wk.y[3 * b + 1] = Lf[2] * bp[1] + Lf[4] * bp[2];
+        Wk.y[3 * b + 2] = Lf[5] * bp[2];
+        for (int e = e0; e < e1; ++e)
+        {
+            T* We = Wk.W + e * 18;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+            {
+                const double w0 = double(We[a * 3]), w1 = double(We[a * 3 + 1]), w2 = double(We[a * 3 + 2]);
+                We[a * 3] = T(w0 * Lf[0] + w1 * Lf[1] + w2 * Lf[3]);
+                We[a * 3 + 1] = T(w1 * Lf[2] + w2 * Lf[4]);
+                We[a * 3 + 2] = T(w2 * Lf[5]);
+            }
         }
     }
 
@@ -410,7 +455,7 @@ namespace pbk
     template <typename T>
     __global__ void __launch_bounds__(256) k_schur_partial(Problem P, Work<T> Wk)
     {
-        const int item = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+        const int item = xcdLocal(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave);
         if (item >= P.nItems) return;
         const int lane = threadIdx.x & 63;
         const int blk = P.itemBlock[item];
@@ -419,37 +464,30 @@ namespace pbk
 #pragma unroll
         for (int i = 0; i < 36; ++i) acc[i] = T(0);
         double accr[6] = {0, 0, 0, 0, 0, 0};
+        // the cross blocks hold U = W L (Hpp^-1 = L L^T), so each pair contributes -U1 U2^T and the diagonal blocks
+        // the right-hand side correction U1 z (z = L^T bp)
         for (int k = P.itemBegin[item] + lane; k < P.itemBegin[item + 1]; k += kWave)
         {
             const int e1 = P.pairE1[k], e2 = P.pairE2[k];
-            const int p = P.entryPoint[e1];
-            T w1[18], w2[18], hi[9];
+            T u1[18], u2[18];
 #pragma unroll
             for (int i = 0; i < 18; ++i)
             {
-                w1[i] = Wk.W[e1 * 18 + i];
-                w2[i] = Wk.W[e2 * 18 + i];
+                u1[i] = Wk.W[e1 * 18 + i];
+                u2[i] = Wk.W[e2 * 18 + i];
             }
-#pragma unroll
-            for (int i = 0; i < 9; ++i) hi[i] = Wk.Hinv[9 * p + i];
-            // t = Hinv * W2^T (3x6)
-            T tm[18];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 6; ++j)
-                    tm[i * 6 + j] = hi[i * 3] * w2[j * 3] + hi[i * 3 + 1] * w2[j * 3 + 1] + hi[i * 3 + 2] * w2[j * 3 + 2];
 #pragma unroll
             for (int a = 0; a < 6; ++a)
 #pragma unroll
                 for (int j = 0; j < 6; ++j)
-                    acc[a * 6 + j] -= w1[a * 3] * tm[j] + w1[a * 3 + 1] * tm[6 + j] + w1[a * 3 + 2] * tm[12 + j];
+                    acc[a * 6 + j] -= u1[a * 3] * u2[j * 3] + u1[a * 3 + 1] * u2[j * 3 + 1] + u1[a * 3 + 2] * u2[j * 3 + 2];
             if (diag)
             {
-                const double y0 = Wk.y[3 * p], y1 = Wk.y[3 * p + 1], y2 = Wk.y[3 * p + 2];
+                const int p = P.entryPoint[e1];
+                const double z0 = Wk.y[3 * p], z1 = Wk.y[3 * p + 1], z2 = Wk.y[3 * p + 2];
 #pragma unroll
                 for (int a = 0; a < 6; ++a)
-                    accr[a] += double(w1[a * 3]) * y0 + double(w1[a * 3 + 1]) * y1 + double(w1[a * 3 + 2]) * y2;
+                    accr[a] += double(u1[a * 3]) * z0 + double(u1[a * 3 + 1]) * z1 + double(u1[a * 3 + 2]) * z2;
             }
         }
 #pragma unroll
@@ -937,27 +975,29 @@ namespace pbk
         double dd = 0.0, ss = 0.0;
         if (b < P.nElim)
         {
-            double z[3] = {Wk.bp[3 * b], Wk.bp[3 * b + 1], Wk.bp[3 * b + 2]};
+            // dp = Hpp^-1 (bp - sum W^T dc) = L (z - sum U^T dc)
+            double z[3] = {Wk.y[3 * b], Wk.y[3 * b + 1], Wk.y[3 * b + 2]};
             for (int e = P.entryOffset[b]; e < P.entryOffset[b + 1]; ++e)
             {
                 const int u = P.entryUnit[e];
-                const T* Wp = Wk.W + e * 18;
+                const T* Up = Wk.W + e * 18;
 #pragma unroll
                 for (int a = 0; a < 6; ++a)
                 {
                     const double xa = double(Wk.x[u * 6 + a]);
 #pragma unroll
-                    for (int i = 0; i < 3; ++i) z[i] -= double(Wp[a * 3 + i]) * xa;
+                    for (int i = 0; i < 3; ++i) z[i] -= double(Up[a * 3 + i]) * xa;
                 }
             }
+            const T* Lp = Wk.Hinv + 9 * b;
+            const double d[3] = {double(Lp[0]) * z[0], double(Lp[1]) * z[0] + double(Lp[2]) * z[1],
+                                 double(Lp[3]) * z[0] + double(Lp[4]) * z[1] + double(Lp[5]) * z[2]};
 #pragma unroll
             for (int i = 0; i < 3; ++i)
             {
-                const double d = double(Wk.Hinv[9 * b + 3 * i]) * z[0] + double(Wk.Hinv[9 * b + 3 * i + 1]) * z[1] +
-                                 double(Wk.Hinv[9 * b + 3 * i + 2]) * z[2];
-                Wk.dp[3 * b + i] = d;
-                dd += Wk.bp[3 * b + i] * d;
-                ss += d * d;
+                Wk.dp[3 * b + i] = d[i];
+                dd += Wk.bp[3 * b + i] * d[i];
+                ss += d[i] * d[i];
             }
         }
         const double s1 = blockSum256(dd, lds);
@@ -1273,7 +1313,11 @@ namespace pbk
     template <typename T>
     void Launcher<T>::schurBlocks(const Problem& P, const Work<T>& W, double damping, hipStream_t s)
     {
-        if (P.nItems > 0) k_schur_partial<T><<<blocksFor(P.nItems, 4), 256, 0, s>>>(P, W);
+        if (P.nItems > 0)
+        {
+            const int nb = (blocksFor(P.nItems, 4) + kXcds - 1) / kXcds * kXcds;  // a multiple of kXcds (xcdLocal)
+            k_schur_partial<T><<<nb, 256, 0, s>>>(P, W);
+        }
         if (P.nBlocks > 0) k_schur_finalize<T><<<blocksFor(P.nBlocks, 4), 256, 0, s>>>(P, W, damping);
         if (P.nUnits > 0) k_block_jacobi<T><<<blocksFor(P.nUnits, 64), 64, 0, s>>>(P, W);
     }
