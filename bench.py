@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--mixed-rtol", type=float, default=0.0, help="PCG relative tolerance of the mixed path (0 = default)")
     ap.add_argument("--mixed-eta", type=float, default=-1.0,
                     help="quadratic-model PCG stop of the mixed path (<0 = default, 0 = off)")
+    ap.add_argument("--mixed-pcg-max", type=int, default=0,
+                    help="PCG iteration cap per LM step of the mixed path (0 = default)")
     ap.add_argument("--fp64-inexact-steps", type=int, default=1,
                     help="FP64 solves with the mixed path's PCG stopping rule (isolates the precision effect)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -119,6 +121,8 @@ def main():
     ws = SolverWorkspace(0)
     solver = Solver()
 
+    truncation_cap = {}
+
     def solve(precision, profile, inexact=False):
         o = S.aerial_options(PO.SolveOptions())
         o.backend.profileKernels = profile
@@ -128,6 +132,10 @@ def main():
             elif inexact:
                 o.backend.pcgRelativeTolerance = 1e-3
             o.backend.pcgModelEta = args.mixed_eta if args.mixed_eta >= 0 else 0.1
+            if args.mixed_pcg_max > 0:
+                o.backend.pcgMaxIterations = args.mixed_pcg_max
+            elif inexact and "cap" in truncation_cap:
+                o.backend.pcgMaxIterations = truncation_cap["cap"]  # the mixed path's default cap
         t0 = time.perf_counter()
         r = solver.solve(problem, o, ws, precision=precision)
         wall = time.perf_counter() - t0
@@ -157,6 +165,8 @@ def main():
         log(f"fp64 {i}: {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
             f"solve {r.report['solve_seconds']:.4f}s (setup {r.report['setup_seconds']:.2f}s), cost {r.finalCost:.10g}")
 
+    # the FP64 leg with the mixed path's stopping rule uses the mixed default cap max(200, 0.6 P) too
+    truncation_cap["cap"] = max(200, 3 * mixed[0].report["primary_block_count"] // 5)
     fp64i = []
     for i in range(args.fp64_inexact_steps):
         r, wall = solve(PO.Precision.Fp64, False, inexact=True)
