@@ -168,6 +168,8 @@ typedef struct pb_solve_options {
     double pcg_model_eta;                /* inexact-Newton stop: end PCG when i*(Q_i - Q_{i-1})/Q_i < eta for the
                                             quadratic model Q (0 = off; negative = path default: off for FP64,
                                             0.1 for mixed) */
+    double step_rms_tolerance;           /* convergence when both step RMS values are at or below it
+                                            (0 = 1e-6, BundleAdjustPlaMatrix.cpp:446-466) */
 } pb_solve_options;
 
 /* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */
@@ -257,6 +259,18 @@ pb_status pb_hip_reduced_system(pb_ctx* ctx, const pb_solve_options* options, do
 pb_status pb_hip_evaluate_cost(pb_ctx* ctx, const pb_solve_options* options, double* cost);
 
 /* Last error message of the context (never NULL). */
+/* Adjoint at the converged state of the last pb_hip_solve on this context (no reference counterpart in
+ * PlaBundle; the backward entry of the MI355X backend). For a scalar loss l of the refined state, given
+ * dl/dC of the camera centres (dl_dcenter [num_cameras*3] or NULL) and dl/dX of the points
+ * (dl_dpoint [num_tracks*3] or NULL), writes dl/d(u, v) of every observation (dl_dobs [num_obs*2], input
+ * order; unusable or inactive observations get 0). Implicit-function theorem on the optimality condition:
+ * H lambda = dl/dtheta is solved in FP64 through the online point Schur, with H = sum J^T M J and
+ * M = d(w_n r)/dr the observation's exact weight matrix (its eigenvalue along r clamped at zero), then
+ * dl/dy_i = M_i J_i lambda. Second derivatives of the projection are not included. dl_dcontrol must be NULL:
+ * this backend has no control-point constraints (PB_ERR_UNSUPPORTED otherwise). */
+pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* dl_dpoint, double* dl_dobs,
+                          double* dl_dcontrol);
+
 const char* pb_hip_last_error(const pb_ctx* ctx);
 
 #ifdef __cplusplus

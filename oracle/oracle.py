@@ -95,6 +95,7 @@ def default_options() -> _abi.SolveOptionsC:
         o.prior_sigma[i] = v
     o.low_order_distortion_scale = 1.0
     o.pcg_model_eta = -1.0
+    o.step_rms_tolerance = 0.0
     return o
 
 

@@ -1763,7 +1763,9 @@ namespace
                         current = cand;
                         ++accepted;
                         damping /= 10.0;
-                        converged = alpha * rms.global <= 1e-6 && alpha * rms.point <= 1e-6;
+                        // step-RMS stop (BundleAdjustPlaMatrix.cpp:446-466), threshold 1e-6 unless the options set one
+                        const double tol = o.opt.o.step_rms_tolerance > 0.0 ? o.opt.o.step_rms_tolerance : 1e-6;
+                        converged = alpha * rms.global <= tol && alpha * rms.point <= tol;
                         relinearize = !converged;
                     }
                     else

@@ -96,6 +96,7 @@ class SolveOptionsC(C.Structure):
         ("armijo_batch", C.c_int32),
         ("profile_kernels", C.c_int32),
         ("pcg_model_eta", C.c_double),
+        ("step_rms_tolerance", C.c_double),
     ]
 
 
@@ -158,6 +159,7 @@ EXPORTED_SYMBOLS = (
     "pb_hip_kernel_stats",
     "pb_hip_reduced_system",
     "pb_hip_evaluate_cost",
+    "pb_hip_backward",
     "pb_hip_last_error",
 )
 
@@ -202,6 +204,8 @@ def load_library(path: str | None = None):
     lib.pb_hip_reduced_system.restype = C.c_int
     lib.pb_hip_evaluate_cost.argtypes = [C.c_void_p, C.POINTER(SolveOptionsC), _dp]
     lib.pb_hip_evaluate_cost.restype = C.c_int
+    lib.pb_hip_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    lib.pb_hip_backward.restype = C.c_int
     lib.pb_hip_last_error.argtypes = [C.c_void_p]
     lib.pb_hip_last_error.restype = C.c_char_p
     if path is None:

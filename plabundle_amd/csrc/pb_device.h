@@ -288,6 +288,61 @@ namespace pbdev
         return finite(L.w) && finite(cost) && finite(u) && finite(v);
     }
 
+    // Second derivative rho''(s) of the per-observation cost of robust() (cost = rho(s), weight = 2 rho'(s)).
+    PB_HD double robustCurvature(double s, int loss, double scale)
+    {
+        const double d2 = scale * scale;
+        if (loss == 1) return s <= d2 ? 0.0 : -0.25 * scale / (s * sqrt(s));
+        if (loss == 2)
+        {
+            const double q = 1.0 + s / d2;
+            return -0.5 / (d2 * q * q);
+        }
+        return 0.0;
+    }
+
+    // Symmetric square root Q (q00, q01, q11) of an observation's exact weight matrix for the adjoint:
+    // the gradient of the cost with respect to the residual is w_n r (w_n = ow * 2 rho'), and
+    // M = d(w_n r)/dr = w_n I + 4 ow^2 rho''(s) r r^T; its eigenvalue along r is clamped at zero so that the
+    // adjoint system stays positive semi-definite (it is never negative for least squares and Huber).
+    PB_HD void adjointWeightRoot(const Linearization& L, double ow, int loss, double scale, double q[3])
+    {
+        const double n2 = L.r[0] * L.r[0] + L.r[1] * L.r[1];
+        const double sw = sqrt(fmax(L.w, 0.0));
+        if (!(n2 > 0.0))
+        {
+            q[0] = sw;
+            q[1] = 0.0;
+            q[2] = sw;
+            return;
+        }
+        const double c = 4.0 * ow * ow * robustCurvature(ow * n2, loss, scale);
+        const double sm = sqrt(fmax(0.0, L.w + c * n2));
+        const double a = L.r[0] * L.r[0] / n2, b = L.r[0] * L.r[1] / n2, d = L.r[1] * L.r[1] / n2;
+        q[0] = sw * (1.0 - a) + sm * a;
+        q[1] = (sm - sw) * b;
+        q[2] = sw * (1.0 - d) + sm * d;
+    }
+
+    // J <- Q J for every Jacobian row pair, weight 1: the normal equations of the transformed rows are
+    // sum J^T M J.
+    template <int KM>
+    PB_HD void applyWeightRoot(const double q[3], Linearization& L)
+    {
+        auto tr = [&](double& j0, double& j1) {
+            const double a = j0, b = j1;
+            j0 = q[0] * a + q[1] * b;
+            j1 = q[1] * a + q[2] * b;
+        };
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tr(L.Jp[i], L.Jp[3 + i]);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) tr(L.Jc[i], L.Jc[6 + i]);
+#pragma unroll
+        for (int k = 0; k < KM; ++k) tr(L.Ji[k], L.Ji[kMaxActive + k]);
+        L.w = 1.0;
+    }
+
     // Reference pose retraction C2W' = D E^T D C2W, E = Rx(a) Ry(b) Rz(c) (Assembly.cpp:130-179).
     PB_HD void retractRotation(const double* R, double a, double b, double cz, double* out)
     {

@@ -392,6 +392,7 @@ namespace
         long nPairs = 0;
         int maxGroupK = 0;
         std::vector<int32_t> activeTrackOf;  // active index -> input track
+        std::vector<int64_t> obsInput;       // active observation -> input observation index
         std::vector<int32_t> obsOffset, obsCam, obsPoint, obsPoseEntry, obsGroupEntry;
         std::vector<double> obsUV, obsW, pointX;
         std::vector<int32_t> pointElim, elimTrack;
@@ -437,6 +438,7 @@ namespace
             for (int64_t i = in.off[t]; i < in.off[t + 1]; ++i)
             {
                 if (!usable(in, i)) continue;
+                L->obsInput.push_back(i);
                 L->obsCam.push_back(in.obsCam[i]);
                 L->obsPoint.push_back(a);
                 L->obsUV.push_back(in.uv[2 * i]);
@@ -815,6 +817,7 @@ struct pb_ctx
     // last results
     std::vector<double> outR, outC, outGroupParams, outPointX;
     bool haveResult = false;
+    pb_solve_options lastOptions{};  // options of the last solve (the adjoint linearises with them)
     // profiling
     bool profile = false;
     int64_t pcgIterationsProfiled = 0;
@@ -1299,6 +1302,7 @@ namespace
         const int checkEvery = o.pcg_check_interval > 0 ? o.pcg_check_interval : 8;
         const int batch = o.armijo_batch > 0 ? std::min(o.armijo_batch, pbk::kMaxAlpha) : 4;
         const double modelEta = o.pcg_model_eta >= 0.0 ? o.pcg_model_eta : (mixed ? 0.1 : 0.0);
+        const double rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
         rep->linear_tolerance = relTol;
         rep->mixed_precision_used = mixed ? 1 : 0;
         ctx->lastScalarBytes = sizeof(T);
@@ -1437,7 +1441,7 @@ namespace
                     current = cand;
                     ++accepted;
                     damping /= 10.0;
-                    converged = alpha * rmsG <= 1e-6 && alpha * rmsP <= 1e-6;
+                    converged = alpha * rmsG <= rmsTol && alpha * rmsP <= rmsTol;
                     relinearize = !converged;
                 }
                 else
@@ -1795,6 +1799,7 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
             (void)work<double>(ctx);
         check(hipStreamSynchronize(ctx->stream), "setup");
         report->setup_seconds = seconds(ts);
+        ctx->lastOptions = *options;
         if (options->precision == PB_PRECISION_MIXED)
             runSolve<float>(ctx, *options, report, cancel);
         else
@@ -1817,6 +1822,77 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
         ctx->haveResult = false;
     }
     return st;
+}
+
+pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* dl_dpoint, double* dl_dobs,
+                          double* dl_dcontrol)
+{
+    if (!ctx || !dl_dobs) return PB_ERR_INVALID;
+    if (dl_dcontrol) return fail(ctx, PB_ERR_UNSUPPORTED, "unsupported: no control-point constraints in this backend");
+    if (!ctx->uploaded || !ctx->haveResult)
+        return fail(ctx, PB_ERR_INVALID, "pb_hip_backward needs the result of a pb_hip_solve on this context");
+    return guarded(ctx, [&]() -> pb_status {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        using Lr = pbk::Launcher<double>;
+        pbk::Work<double>& W = work<double>(ctx);
+        pbk::Problem& P = ctx->P;
+        const Layout& L = ctx->lay;
+        hipStream_t s = ctx->stream;
+        P.loss = ctx->lastOptions.robust_loss;
+        P.lossScale = ctx->lastOptions.robust_loss_scale;
+        // loss gradients in device order (cameras as given, points in elimination order)
+        std::vector<double> hc(3 * size_t(std::max(1, P.nCam)), 0.0), hp(3 * size_t(std::max(1, P.nElim)), 0.0);
+        if (dl_dcenter) std::copy_n(dl_dcenter, 3 * size_t(P.nCam), hc.begin());
+        if (dl_dpoint)
+            for (int b = 0; b < P.nElim; ++b)
+                std::copy_n(dl_dpoint + 3 * size_t(L.activeTrackOf[L.elimTrack[b]]), 3, hp.begin() + 3 * b);
+        DeviceAlloc tmp;
+        const double* dC = tmp.upload(hc, s);
+        const double* dP = tmp.upload(hp, s);
+        double* dOut = tmp.alloc<double>(2 * size_t(std::max(1, P.nObs)));
+        // the normal equations at the converged state with the adjoint weight roots, no damping
+        P.adjoint = 1;
+        BuildResult br;
+        try
+        {
+            br = build(ctx, W, 0.0);
+        }
+        catch (...)
+        {
+            P.adjoint = 0;
+            throw;
+        }
+        P.adjoint = 0;
+        if (br.invalid || br.singular || br.notSpd)
+        {
+            tmp.release();
+            return fail(ctx, PB_ERR_NUMERIC, "adjoint system is singular or has an invalid projection");
+        }
+        check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
+        Lr::adjointRhs(P, W, dC, dP, s);
+        Lr::schurBlocks(P, W, 0.0, s);
+        const PcgResult lin = pcg(ctx, W, 1e-12, 1e-300, std::max(200, 30 * ctx->prep.primaryBlocks), 8, 0.0);
+        if (!lin.converged)
+        {
+            tmp.release();
+            return fail(ctx, PB_ERR_NUMERIC, "adjoint PCG did not converge");
+        }
+        Lr::backSubstitute(P, W, s);
+        Lr::adjointObservations(P, W, dOut, s);
+        std::vector<double> out(2 * size_t(P.nObs));
+        if (P.nObs > 0)
+            check(hipMemcpyAsync(out.data(), dOut, sizeof(double) * out.size(), hipMemcpyDeviceToHost, s), "readback");
+        check(hipStreamSynchronize(s), "adjoint");
+        tmp.release();
+        const int64_t nIn = ctx->in.off.empty() ? 0 : ctx->in.off.back();
+        std::fill_n(dl_dobs, 2 * size_t(nIn), 0.0);
+        for (int o = 0; o < P.nObs; ++o)
+        {
+            dl_dobs[2 * L.obsInput[o]] = out[2 * o];
+            dl_dobs[2 * L.obsInput[o] + 1] = out[2 * o + 1];
+        }
+        return PB_OK;
+    });
 }
 
 pb_status pb_hip_download(pb_ctx* ctx, double* R, double* C, double* intr, double* X, int32_t* valid)

@@ -81,6 +81,7 @@ namespace pbk
         int gaugeAnchor = -1, gaugeScale = -1;
         double gaugeBaseline = 0.0;
         int globalDim = 0;
+        int adjoint = 0;  // 1: linearisations use the adjoint weight root (pb_hip_backward builds)
 
         // cameras
         double* camR = nullptr;  // state, camera-to-world [nCam*9]
@@ -205,6 +206,9 @@ namespace pbk
         static void pcgIteration(const Problem& P, const Work<T>& W, int it, int maxIt, double relTol, double absTol,
                                  double modelEta, hipStream_t s);
         static void backSubstitute(const Problem& P, const Work<T>& W, hipStream_t s);
+        static void adjointRhs(const Problem& P, const Work<T>& W, const double* dlCenter, const double* dlPoint,
+                               hipStream_t s);
+        static void adjointObservations(const Problem& P, const Work<T>& W, double* out, hipStream_t s);
         static void trialCosts(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, int costBlocks,
                                hipStream_t s);
         static void accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s);

@@ -144,6 +144,12 @@ namespace pbk
                 setFlag(Wk.flags, kFlagInvalid);
                 continue;
             }
+            if (P.adjoint)
+            {
+                double q[3];
+                pbdev::adjointWeightRoot(L, P.obsW[o], P.loss, P.lossScale, q);
+                pbdev::applyWeightRoot<KM>(q, L);
+            }
             const double w = L.w;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
@@ -275,6 +281,12 @@ namespace pbk
             {
                 setFlag(Wk.flags, kFlagInvalid);
                 continue;
+            }
+            if (P.adjoint)
+            {
+                double q[3];
+                pbdev::adjointWeightRoot(L, P.obsW[o], P.loss, P.lossScale, q);
+                pbdev::applyWeightRoot<KM>(q, L);
             }
             const double w = L.w;
             A.cost += L.cost;
@@ -1264,6 +1276,106 @@ namespace pbk
     }
 
     // ---------------------------------------------------------------------------------------------
+    // Adjoint at convergence (pb_hip_backward). With the system built on the adjoint weight roots, H lambda =
+    // d loss / d theta is solved through the same Schur path: the camera side of the right-hand side is the
+    // loss gradient in the step parameters, the point side enters as z = L^T (s grad_X) (the point step is
+    // X += s dp); then d loss / d y_i = M_i J_i lambda for every observation.
+    template <typename T>
+    __global__ void k_adjoint_rhs_cameras(Problem P, Work<T> Wk, const double* dlCenter)
+    {
+        const int c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= P.nCam) return;
+        const int u = P.camUnit[c];
+        if (u < 0) return;
+        const double g0 = dlCenter[3 * c], g1 = dlCenter[3 * c + 1], g2 = dlCenter[3 * c + 2];
+        double* out = Wk.directRhs + 6 * u;
+        out[0] = out[1] = out[2] = 0.0;  // no rotation gradient
+        const Camera& cam = P.cams[c];
+        if (cam.tangent)
+        {
+            // the gauge scale camera moves along its two tangent directions (basis holds dC / dt)
+            out[3] = cam.basis[0] * g0 + cam.basis[1] * g1 + cam.basis[2] * g2;
+            out[4] = cam.basis[3] * g0 + cam.basis[4] * g1 + cam.basis[5] * g2;
+            out[5] = 0.0;
+        }
+        else
+        {
+            out[3] = g0;
+            out[4] = g1;
+            out[5] = g2;
+        }
+    }
+
+    template <typename T>
+    __global__ void k_adjoint_rhs_points(Problem P, Work<T> Wk, const double* dlPoint)
+    {
+        const int b = blockIdx.x * blockDim.x + threadIdx.x;
+        if (b >= P.nElim) return;
+        const int t = P.elimTrack[b];
+        const double sc = pbdev::pointScale(P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2]);
+        const double g0 = sc * dlPoint[3 * b], g1 = sc * dlPoint[3 * b + 1], g2 = sc * dlPoint[3 * b + 2];
+        const T* Lp = Wk.Hinv + 9 * b;
+        Wk.y[3 * b] = double(Lp[0]) * g0 + double(Lp[1]) * g1 + double(Lp[3]) * g2;
+        Wk.y[3 * b + 1] = double(Lp[2]) * g1 + double(Lp[4]) * g2;
+        Wk.y[3 * b + 2] = double(Lp[5]) * g2;
+    }
+
+    template <typename T, int KM>
+    __global__ void k_adjoint_observations(Problem P, Work<T> Wk, double* out)
+    {
+        const int o = blockIdx.x * blockDim.x + threadIdx.x;
+        if (o >= P.nObs) return;
+        const int c = P.obsCam[o];
+        const int a = P.obsPoint[o];
+        const int b = P.pointElim[a];
+        const int g = P.camGroup[c];
+        const int K = g >= 0 ? P.groupK[g] : 0;
+        const int8_t* act = g >= 0 ? P.groupActive + g * kMaxActive : nullptr;
+        Linearization L;
+        if (!pbdev::linearize<KM>(P.cams[c], P.pointX[3 * a], P.pointX[3 * a + 1], P.pointX[3 * a + 2],
+                                  P.obsUV[2 * o], P.obsUV[2 * o + 1], P.obsW[o], P.loss, P.lossScale, K, act, L))
+        {
+            out[2 * o] = out[2 * o + 1] = 0.0;
+            return;
+        }
+        double q[3];
+        pbdev::adjointWeightRoot(L, P.obsW[o], P.loss, P.lossScale, q);
+        double v0 = 0.0, v1 = 0.0;  // J lambda
+        const int u = P.camUnit[c];
+        if (u >= 0)
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+            {
+                const double xk = double(Wk.x[6 * u + k]);
+                v0 += L.Jc[k] * xk;
+                v1 += L.Jc[6 + k] * xk;
+            }
+        if (g >= 0)
+        {
+            const int gu = P.groupUnit[g];
+#pragma unroll
+            for (int k = 0; k < KM; ++k)
+                if (k < K)
+                {
+                    const double xk = double(Wk.x[6 * (gu + k / 6) + k % 6]);
+                    v0 += L.Ji[k] * xk;
+                    v1 += L.Ji[kMaxActive + k] * xk;
+                }
+        }
+        if (b >= 0)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                v0 += L.Jp[i] * Wk.dp[3 * b + i];
+                v1 += L.Jp[3 + i] * Wk.dp[3 * b + i];
+            }
+        // M v = Q (Q v)
+        const double t0 = q[0] * v0 + q[1] * v1, t1 = q[1] * v0 + q[2] * v1;
+        out[2 * o] = q[0] * t0 + q[1] * t1;
+        out[2 * o + 1] = q[1] * t0 + q[2] * t1;
+    }
+
+    // ---------------------------------------------------------------------------------------------
     // Launchers
     namespace
     {
@@ -1345,6 +1457,27 @@ namespace pbk
         const int nb = std::max(1, blocksFor(P.nElim, 256));
         k_back_substitute<T><<<nb, 256, 0, s>>>(P, W);
         k_step_stats<T><<<1, 256, 0, s>>>(P, W, nb);
+    }
+
+    template <typename T>
+    void Launcher<T>::adjointRhs(const Problem& P, const Work<T>& W, const double* dlCenter, const double* dlPoint,
+                                 hipStream_t s)
+    {
+        if (P.nCam > 0) k_adjoint_rhs_cameras<T><<<blocksFor(P.nCam, 128), 128, 0, s>>>(P, W, dlCenter);
+        if (P.nElim > 0) k_adjoint_rhs_points<T><<<blocksFor(P.nElim, 256), 256, 0, s>>>(P, W, dlPoint);
+    }
+
+    template <typename T>
+    void Launcher<T>::adjointObservations(const Problem& P, const Work<T>& W, double* out, hipStream_t s)
+    {
+        if (P.nObs == 0) return;
+        const int nb = blocksFor(P.nObs, 128);
+        if (P.maxGroupK == 0)
+            k_adjoint_observations<T, 0><<<nb, 128, 0, s>>>(P, W, out);
+        else if (P.maxGroupK <= 6)
+            k_adjoint_observations<T, 6><<<nb, 128, 0, s>>>(P, W, out);
+        else
+            k_adjoint_observations<T, 12><<<nb, 128, 0, s>>>(P, W, out);
     }
 
     template <typename T>

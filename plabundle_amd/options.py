@@ -121,6 +121,7 @@ class BackendOptions:
     armijoBatch: int = 0
     profileKernels: bool = False
     pcgModelEta: float = -1.0  # < 0: path default (off for FP64, 0.1 for mixed)
+    stepRmsTolerance: float = 0.0  # 0: the reference's 1e-6
 
 
 @dataclass
@@ -201,4 +202,5 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.armijo_batch = int(b.armijoBatch)
     o.profile_kernels = 1 if b.profileKernels else 0
     o.pcg_model_eta = float(b.pcgModelEta)
+    o.step_rms_tolerance = float(b.stepRmsTolerance)
     return o

@@ -135,6 +135,22 @@ class SolverWorkspace:
             raise RuntimeError(f"pb_hip_get_masks: {self.last_error()}")
         return out
 
+    def backward(self, problem: Problem, dl_dcenter=None, dl_dpoint=None):
+        """Adjoint of the last solve on this workspace: dl/d(u, v) per observation ([num_observations, 2])
+        for a loss with gradients dl/dC ([num_cameras, 3]) and dl/dX ([num_tracks, 3]) at the refined state."""
+        dc = None if dl_dcenter is None else np.ascontiguousarray(dl_dcenter, dtype=np.float64).reshape(-1, 3)
+        dp = None if dl_dpoint is None else np.ascontiguousarray(dl_dpoint, dtype=np.float64).reshape(-1, 3)
+        if dc is not None and dc.shape[0] != problem.num_cameras:
+            raise ValueError("dl_dcenter must have one row per camera")
+        if dp is not None and dp.shape[0] != problem.num_tracks:
+            raise ValueError("dl_dpoint must have one row per track")
+        out = np.zeros((problem.num_observations, 2))
+        st = self._lib.pb_hip_backward(self._ctx, _abi.ptr(dc, C.c_double), _abi.ptr(dp, C.c_double),
+                                       _abi.ptr(out, C.c_double), None)
+        if st != _abi.PB_OK:
+            raise RuntimeError(f"pb_hip_backward: {_abi.STATUS_NAMES.get(st, st)}: {self.last_error()}")
+        return out
+
     def kernel_stats(self, name: str) -> dict:
         launches = C.c_int64()
         ms = C.c_double()

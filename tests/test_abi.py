@@ -43,7 +43,7 @@ def test_struct_layouts_match_header():
 #include "pb_hip.h"
 int main(void) {
   printf("%zu %zu %zu\n", sizeof(pb_problem_view), sizeof(pb_solve_options), sizeof(pb_solve_report));
-  printf("%zu %zu %zu\n", offsetof(pb_problem_view, gauge_baseline), offsetof(pb_solve_options, pcg_model_eta),
+  printf("%zu %zu %zu\n", offsetof(pb_problem_view, gauge_baseline), offsetof(pb_solve_options, step_rms_tolerance),
          offsetof(pb_solve_report, message));
   return 0;
 }
@@ -57,7 +57,7 @@ int main(void) {
     sizes = [int(x) for x in lines[0].split()]
     offs = [int(x) for x in lines[1].split()]
     assert sizes == [C.sizeof(_abi.ProblemView), C.sizeof(_abi.SolveOptionsC), C.sizeof(_abi.SolveReportC)]
-    assert offs == [_abi.ProblemView.gauge_baseline.offset, _abi.SolveOptionsC.pcg_model_eta.offset,
+    assert offs == [_abi.ProblemView.gauge_baseline.offset, _abi.SolveOptionsC.step_rms_tolerance.offset,
                     _abi.SolveReportC.message.offset]
 
 
