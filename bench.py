@@ -80,6 +80,62 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
 
 
+def batch_leg(num_windows: int, precisions, oracle_sample: int) -> dict:
+    """Batched local windows through pb_hip_solve_batch: 512 windows of U{8..64} cameras, U{500..8000} points,
+    track lengths U{2..12} (seed 4). Reports the batch solve time, the per-window device time to converge
+    (p50/p99) and the final-cost error of a sample of windows against the FP64 oracle."""
+    import numpy as np
+
+    from plabundle_amd import options as PO
+    from plabundle_amd import synthetic as S
+    from plabundle_amd.solver import Solver, SolverWorkspace
+
+    t = time.time()
+    wins = S.batch_windows(seed=4, num_windows=num_windows)
+    nobs = int(sum(w.num_observations for w in wins))
+    log(f"batch: generated {num_windows} windows, {nobs} observations in {time.time() - t:.1f}s")
+    o = S.window_options(PO.SolveOptions(), max_iterations=10)
+    ws = SolverWorkspace(0)
+    out = {"workload": f"{num_windows} local windows: cameras U{{8..64}}, points U{{500..8000}}, track length "
+                       "U{2..12}, one shared focal + Brown group per window, Huber 1 px, 10 LM iterations, seed 4",
+           "windows": num_windows, "observations": nobs,
+           "linear_solve": "dense FP64 Cholesky of each window's reduced camera system (one workgroup per window)"}
+    sample = list(np.random.default_rng(4).choice(num_windows, size=min(oracle_sample, num_windows), replace=False))
+    refs = {}
+    if sample:
+        from oracle import oracle as O
+
+        co = PO.to_c(o)
+        for w in sample:
+            refs[w] = O.Oracle(wins[w], threads=O.lib().oracle_max_threads()).solve(co)["final_cost"]
+    for prec in precisions:
+        name = "mixed" if prec == PO.Precision.Mixed else "fp64"
+        res, summ = Solver().solve_batch(wins, o, ws, precision=prec)
+        if summ["abi_status"] != 0:
+            out[name] = {"error": summ["message"]}
+            continue
+        dev = np.array([r.report["device_seconds"] for r in res])
+        its = np.array([r.report["iterations"] for r in res])
+        work = float(sum(r.report["observation_count"] * r.report["iterations"] for r in res))
+        errs = [abs(res[w].finalCost - refs[w]) / refs[w] for w in sample]
+        out[name] = {
+            "solve_s": round(summ["solve_seconds"], 4),
+            "setup_s": round(summ["setup_seconds"], 2),
+            "rounds": summ["iterations"],
+            "converged_windows": int(sum(r.report["converged"] for r in res)),
+            "window_time_to_converge_p50_ms": round(float(np.percentile(dev, 50)) * 1e3, 3),
+            "window_time_to_converge_p99_ms": round(float(np.percentile(dev, 99)) * 1e3, 3),
+            "window_time_to_converge_max_ms": round(float(dev.max()) * 1e3, 3),
+            "window_lm_iterations_mean": round(float(its.mean()), 3),
+            "windows_per_s": round(num_windows / summ["solve_seconds"], 2),
+            "observation_iterations_per_s": round(work / summ["solve_seconds"], 1),
+            "oracle_sample_windows": [int(w) for w in sample],
+            "oracle_sample_max_rel_cost_err": float(max(errs)) if errs else None,
+        }
+        log(f"batch {name}: {out[name]}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +152,12 @@ def main():
                     help="PCG iteration cap per LM step of the mixed path (0 = default)")
     ap.add_argument("--fp64-inexact-steps", type=int, default=1,
                     help="FP64 solves with the mixed path's PCG stopping rule (isolates the precision effect)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--batch-windows", type=int, default=512,
+                    help="windows of the batched local-BA leg (0 = skip)")
+    ap.add_argument("--batch-fp64", type=int, default=1, help="also solve the batch in FP64")
+    ap.add_argument("--batch-oracle-sample", type=int, default=8,
+                    help="windows of the batch checked against the FP64 oracle")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only copy this many MiB device-to-device three times (the counter calibration run)")
     args = ap.parse_args()
@@ -174,6 +236,19 @@ def main():
         log(f"fp64 inexact {i}: {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
             f"solve {r.report['solve_seconds']:.4f}s, cost {r.finalCost:.10g}")
 
+    # adjoint at the last mixed solve's state: l = sum |C - T|^2, gradients for every observation
+    adjoint = None
+    if not args.no_adjoint:
+        import numpy as np
+
+        r = solve(PO.Precision.Mixed, False)[0]
+        T = r.cam_center + np.random.default_rng(5).normal(0.0, 0.05, r.cam_center.shape)
+        t0 = time.perf_counter()
+        g = ws.backward(problem, 2.0 * (r.cam_center - T))
+        adjoint = {"seconds": round(time.perf_counter() - t0, 4), "loss": "sum |C - T|^2 over refined centres",
+                   "gradient_entries": int(g.size), "nonzero": int(np.count_nonzero(g))}
+        log(f"adjoint: {adjoint}")
+
     mixed_t = sum(r.report["solve_seconds"] for r in mixed) / len(mixed)
     mixed_it = sum(r.report["iterations"] for r in mixed) / len(mixed)
     fp64_t = sum(r.report["solve_seconds"] for r in fp64) / len(fp64) if fp64 else None
@@ -211,6 +286,15 @@ def main():
     rel_fp64 = abs(mixed[0].finalCost - fp64[0].finalCost) / fp64[0].finalCost if fp64 else None
     fp64i_t = sum(r.report["solve_seconds"] for r in fp64i) / len(fp64i) if fp64i else None
     rel_oracle64i = abs(fp64i[0].finalCost - golden["final_cost"]) / golden["final_cost"] if golden and fp64i else None
+
+    batch = None
+    if args.batch_windows > 0 and not args.small:
+        try:
+            precs = [PO.Precision.Mixed] + ([PO.Precision.Fp64] if args.batch_fp64 else [])
+            batch = batch_leg(args.batch_windows, precs, args.batch_oracle_sample)
+        except Exception as e:  # the batch leg reports its failure; the headline line still prints
+            batch = {"error": f"{type(e).__name__}: {e}"}
+            log(f"batch leg failed: {batch['error']}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -270,6 +354,8 @@ def main():
         "fp64_inexact_pcg_iterations": fp64i[0].report["linear_iterations"] if fp64i else None,
         "fp64_inexact_rel_cost_err_vs_oracle": rel_oracle64i,
         "speedup_mixed_vs_fp64_inexact": round(fp64i_t / mixed_t, 4) if fp64i_t else None,
+        "adjoint": adjoint,
+        "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "device": mixed[0].report["device_name"],

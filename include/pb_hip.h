@@ -258,7 +258,6 @@ pb_status pb_hip_reduced_system(pb_ctx* ctx, const pb_solve_options* options, do
 /* Objective (image robust cost + intrinsic priors) at the uploaded initial state. */
 pb_status pb_hip_evaluate_cost(pb_ctx* ctx, const pb_solve_options* options, double* cost);
 
-/* Last error message of the context (never NULL). */
 /* Adjoint at the converged state of the last pb_hip_solve on this context (no reference counterpart in
  * PlaBundle; the backward entry of the MI355X backend). For a scalar loss l of the refined state, given
  * dl/dC of the camera centres (dl_dcenter [num_cameras*3] or NULL) and dl/dX of the points
@@ -271,6 +270,53 @@ pb_status pb_hip_evaluate_cost(pb_ctx* ctx, const pb_solve_options* options, dou
 pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* dl_dpoint, double* dl_dobs,
                           double* dl_dcontrol);
 
+/* Batched local windows: independent problems solved in one call. Each window is a complete problem (its own
+ * camera and track numbering, calibration groups, fixed blocks and gauge) and is solved as pb_hip_upload +
+ * pb_hip_solve would solve it alone, with the same options, except that each LM step solves the window's
+ * reduced camera system exactly (dense FP64 Cholesky, at most 85 primary units per window) instead of by PCG;
+ * the PCG options are not used. No reference counterpart: PlaBundle solves windows one Solver::solve at a time
+ * (src/solver.cpp:275-302 per window); this entry is the batched driver of the MI355X backend. */
+typedef struct pb_batch_view {
+    int32_t num_windows;
+    const pb_problem_view* windows;  /* [num_windows] */
+} pb_batch_view;
+
+/* Outcome of one window (the per-window fields of pb_solve_report). */
+typedef struct pb_window_report {
+    int32_t solve_status;          /* pb_solve_status */
+    int32_t solution_usable;
+    int32_t converged;
+    int32_t iterations;
+    int32_t accepted_steps;
+    int32_t rejected_steps;
+    int32_t linearizations;
+    int32_t objective_evaluations;
+    int32_t observation_count;
+    int32_t active_track_count;
+    int32_t camera_block_count;
+    int32_t primary_block_count;
+    double initial_cost;
+    double final_cost;
+    double device_seconds;         /* device clock from the start of the batch to the window's last step */
+} pb_window_report;
+
+/* Solves every window of the batch. The windows share each device launch (one block-diagonal problem) and
+ * keep separate LM controllers on the device (damping, line search, acceptance and stopping per window);
+ * a window stops when it converges, exhausts its line search or reaches max_iterations, and later launches
+ * skip it. reports [num_windows] receives each window's outcome; summary receives the batch totals
+ * (iterations = controller rounds, costs = sums over the windows, solve_seconds = host wall time of the
+ * rounds, setup_seconds = host preparation and upload). A window whose input is invalid fails the whole call
+ * with PB_ERR_INVALID / PB_ERR_UNSUPPORTED and a message naming the window. cancel is polled between rounds;
+ * a cancelled batch publishes nothing. The context then holds the batch: pb_hip_batch_download reads a
+ * window, pb_hip_download the concatenated state; pb_hip_solve and pb_hip_backward need a new pb_hip_upload. */
+pb_status pb_hip_solve_batch(pb_ctx* ctx, const pb_batch_view* batch, const pb_solve_options* options,
+                             pb_window_report* reports, pb_solve_report* summary, const volatile int32_t* cancel);
+
+/* Refined state of one window of the last batch, in the window's own numbering (as pb_hip_download). */
+pb_status pb_hip_batch_download(pb_ctx* ctx, int32_t window, double* cam_rotation, double* cam_center,
+                                double* cam_intrinsics, double* track_point, int32_t* track_valid);
+
+/* Last error message of the context (never NULL). */
 const char* pb_hip_last_error(const pb_ctx* ctx);
 
 #ifdef __cplusplus

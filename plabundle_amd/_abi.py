@@ -146,6 +146,36 @@ class SolveReportC(C.Structure):
         return out
 
 
+class BatchView(C.Structure):
+    _fields_ = [
+        ("num_windows", C.c_int32),
+        ("windows", C.POINTER(ProblemView)),
+    ]
+
+
+class WindowReportC(C.Structure):
+    _fields_ = [
+        ("solve_status", C.c_int32),
+        ("solution_usable", C.c_int32),
+        ("converged", C.c_int32),
+        ("iterations", C.c_int32),
+        ("accepted_steps", C.c_int32),
+        ("rejected_steps", C.c_int32),
+        ("linearizations", C.c_int32),
+        ("objective_evaluations", C.c_int32),
+        ("observation_count", C.c_int32),
+        ("active_track_count", C.c_int32),
+        ("camera_block_count", C.c_int32),
+        ("primary_block_count", C.c_int32),
+        ("initial_cost", C.c_double),
+        ("final_cost", C.c_double),
+        ("device_seconds", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # Exported symbols of the shared library, in header order.
 EXPORTED_SYMBOLS = (
     "pb_default_options",
@@ -160,6 +190,8 @@ EXPORTED_SYMBOLS = (
     "pb_hip_reduced_system",
     "pb_hip_evaluate_cost",
     "pb_hip_backward",
+    "pb_hip_solve_batch",
+    "pb_hip_batch_download",
     "pb_hip_last_error",
 )
 
@@ -206,6 +238,11 @@ def load_library(path: str | None = None):
     lib.pb_hip_evaluate_cost.restype = C.c_int
     lib.pb_hip_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     lib.pb_hip_backward.restype = C.c_int
+    lib.pb_hip_solve_batch.argtypes = [C.c_void_p, C.POINTER(BatchView), C.POINTER(SolveOptionsC),
+                                       C.POINTER(WindowReportC), C.POINTER(SolveReportC), _ip]
+    lib.pb_hip_solve_batch.restype = C.c_int
+    lib.pb_hip_batch_download.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _ip]
+    lib.pb_hip_batch_download.restype = C.c_int
     lib.pb_hip_last_error.argtypes = [C.c_void_p]
     lib.pb_hip_last_error.restype = C.c_char_p
     if path is None:

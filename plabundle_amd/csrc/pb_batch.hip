@@ -1,0 +1,693 @@
+// pb_batch.hip - batched windows (pb_hip_solve_batch): per-window LM control and the dense window solve.
+//
+// The windows of a batch form one block-diagonal problem. Linearisation, Schur reduction and point
+// back-substitution run through the shared grid kernels (every launch covers every window); the kernels here
+// keep one LM controller per window on the device (BundleAdjustPlaMatrix.cpp:162-631 with
+// useReferenceOnlineSchur, run per window) and solve each window's reduced camera system exactly with a dense
+// FP64 Cholesky factorisation, one workgroup per window.
+//
+// Round of every running window:
+//   k_batch_round_begin   activity, flags and line-search state of the round
+//   (shared build)        effective cameras, linearisation, camera terms, Schur blocks with the window damping
+//   k_batch_after_build   objective of the linearisation point, build failures -> reject or stop
+//   k_batch_factor        dense S of the window, Cholesky, forward/backward substitution -> camera step
+//   (shared back-subst.)  point steps
+//   k_batch_step_stats    directional derivative and step RMS values per window
+//   trial passes          k_batch_alphas, trial states (shared), k_batch_trial_cost, k_batch_decide
+//   k_batch_control       accept / reject, damping update, convergence and budget tests
+//   k_batch_apply         commits the accepted trial state of each window
+#include "pb_common.h"
+#include "pb_device.h"
+#include "pb_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+
+namespace pbk
+{
+    namespace
+    {
+        int gridFor(long n, int per) { return static_cast<int>(std::max<long>(1, (n + per - 1) / per)); }
+
+        __device__ __forceinline__ unsigned long long deviceClock() { return wall_clock64(); }
+    }  // namespace
+
+    __global__ void k_batch_start(Batch B)
+    {
+        if (threadIdx.x == 0 && blockIdx.x == 0) B.doneClock[B.nWin] = deviceClock();
+    }
+
+    __global__ void k_batch_round_begin(Problem P, Batch B)
+    {
+        const int w = blockIdx.x * blockDim.x + threadIdx.x;
+        if (w == 0) B.counts[0] = 0;
+        if (w >= B.nWin) return;
+        B.active[w] = B.status[w] == kWinRunning ? 1 : 0;
+        B.reject[w] = 0;
+        B.searching[w] = 0;
+        B.nAlpha[w] = 0;
+        B.acceptIdx[w] = -1;
+#pragma unroll
+        for (int f = 0; f < kFlagCount; ++f) P.winFlags[w * kFlagCount + f] = 0;
+    }
+
+    // Objective at the linearisation point (camera costs in camera order plus the group priors) and the
+    // reaction to build failures, as the controller does after each build.
+    template <typename T>
+    __global__ void k_batch_after_build(Problem P, Work<T> Wk, Batch B)
+    {
+        const int lane = threadIdx.x & 63;
+        const int w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+        if (w >= B.nWin || !B.active[w]) return;
+        double v = 0.0;
+        for (int c = B.camOffset[w] + lane; c < B.camOffset[w + 1]; c += kWave) v += Wk.camCost[c];
+        v = waveSum(v);
+        if (lane != 0) return;
+        for (int g = B.groupOffset[w]; g < B.groupOffset[w + 1]; ++g) v += P.groupK[g] > 0 ? Wk.groupPriorCost[g] : 0.0;
+        B.cost[w] = v;
+        int32_t* cnt = B.counters + w * kWinCounterCount;
+        ++cnt[kWinLinearizations];
+        ++cnt[kWinEvaluations];
+        if (cnt[kWinLinearizations] == 1) B.initialCost[w] = v;
+        const int32_t* f = P.winFlags + w * kFlagCount;
+        const bool empty = B.poseUnitOffset[w + 1] == B.poseUnitOffset[w] &&
+                           B.groupUnitOffset[w + 1] == B.groupUnitOffset[w] && B.elimOffset[w + 1] == B.elimOffset[w];
+        if (f[kFlagInvalid])
+        {
+            // an invalid projection in a build fails the solve (the single-problem path reports NUMERIC)
+            B.status[w] = kWinNumeric;
+            B.active[w] = 0;
+            B.doneClock[w] = deviceClock();
+        }
+        else if (empty)
+        {
+            // nothing to refine: converged at the initial objective
+            B.status[w] = kWinConverged;
+            B.active[w] = 0;
+            B.doneClock[w] = deviceClock();
+        }
+        else if (f[kFlagSingular] || f[kFlagNotSpd])
+        {
+            B.reject[w] = 1;
+        }
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    // Dense window solve. The window's reduced system (6 * units rows, both triangles of the block-sparse S)
+    // is gathered into a column-major FP64 matrix A, factored in place by a left-looking blocked Cholesky with
+    // column panels of kDensePanel, and solved by blocked forward and backward substitution. Thread t owns
+    // the panel rows j0 + t + r * kDenseThreads (r < kDenseRows). A failed pivot (not positive, or not
+    // finite) rejects the step, as a breakdown of the iterative solve does on the single-problem path.
+    template <int R>
+    __device__ __forceinline__ void panelUpdate(const double* A, const double* Bt, int n, int j0, int tid,
+                                                double (&acc)[kDenseRows][kDensePanel])
+    {
+        int row[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) row[r] = j0 + tid + r * kDenseThreads;
+        for (int k = 0; k < j0; ++k)
+        {
+            double a[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) a[r] = row[r] < n ? A[size_t(k) * n + row[r]] : 0.0;
+            const double2* b2 = reinterpret_cast<const double2*>(Bt + k * kDensePanel);
+#pragma unroll
+            for (int q = 0; q < kDensePanel / 2; ++q)
+            {
+                const double2 b = b2[q];
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                {
+                    acc[r][2 * q] -= a[r] * b.x;
+                    acc[r][2 * q + 1] -= a[r] * b.y;
+                }
+            }
+        }
+    }
+
+    template <typename T>
+    __global__ void __launch_bounds__(kDenseThreads) k_batch_factor(Problem P, Work<T> Wk, Batch B)
+    {
+        extern __shared__ double lds[];
+        const int w = blockIdx.x;
+        if (!B.active[w] || B.reject[w]) return;
+        const int pu0 = B.poseUnitOffset[w], pu1 = B.poseUnitOffset[w + 1];
+        const int gu0 = B.groupUnitOffset[w];
+        const int nPose = pu1 - pu0;
+        const int nU = nPose + (B.groupUnitOffset[w + 1] - gu0);
+        const int n = 6 * nU;
+        if (n == 0) return;
+        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+        constexpr int kWaves = kDenseThreads / kWave;
+        constexpr int NB = kDensePanel;
+        double* Bt = lds;                         // [maxDim][NB]  L(j0 + jj, k) at k * NB + jj
+        double* Dg = Bt + size_t(B.maxDim) * NB;  // [NB][NB]      diagonal block of the panel, row-major
+        double* vec = Dg + NB * NB;               // [maxDim]      right-hand side / solution
+        double* red = vec + B.maxDim;             // [kWaves][NB]  backward-substitution partials
+        __shared__ int failed;
+        if (tid == 0) failed = 0;
+        double* A = B.dense + B.denseOffset[w];
+        auto unitOf = [&](int r) { return r < nPose ? pu0 + r : gu0 + (r - nPose); };
+        auto localOf = [&](int u) { return (u >= pu0 && u < pu1) ? u - pu0 : nPose + (u - gu0); };
+
+        // gather: zero, then one wave per block row writes the row's 6x6 blocks (lane = element)
+        for (long i = tid; i < long(n) * n; i += kDenseThreads) A[i] = 0.0;
+        for (int i = tid; i < n; i += kDenseThreads) vec[i] = Wk.rhs[unitOf(i / 6) * 6 + i % 6];
+        __syncthreads();
+        for (int r = wv; r < nU; r += kWaves)
+        {
+            const int u = unitOf(r);
+            for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k)
+            {
+                const int c = localOf(P.colIdx[k]);
+                if (lane < 36)
+                {
+                    const int i = lane / 6, j = lane % 6;
+                    A[size_t(6 * c + j) * n + 6 * r + i] = double(Wk.S[size_t(k) * 36 + lane]);
+                }
+            }
+        }
+        __syncthreads();
+
+        // factorisation
+        for (int j0 = 0; j0 < n; j0 += NB)
+        {
+            const int jb = min(NB, n - j0);
+            for (int q = tid; q < j0 * NB; q += kDenseThreads)
+            {
+                const int k = q / NB, jj = q % NB;
+                Bt[q] = jj < jb ? A[size_t(k) * n + j0 + jj] : 0.0;
+            }
+            __syncthreads();
+            double acc[kDenseRows][NB];
+#pragma unroll
+            for (int r = 0; r < kDenseRows; ++r)
+            {
+                const int i = j0 + tid + r * kDenseThreads;
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj) acc[r][jj] = (i < n && jj < jb) ? A[size_t(j0 + jj) * n + i] : 0.0;
+            }
+            // rows of this wave beyond the first kDenseThreads of the panel (wave-uniform)
+            if (j0 + kDenseThreads + (tid & ~63) < n)
+                panelUpdate<2>(A, Bt, n, j0, tid, acc);
+            else if (j0 + (tid & ~63) < n)
+                panelUpdate<1>(A, Bt, n, j0, tid, acc);
+            // the diagonal block goes to LDS; wave 0 factors it in registers (lane l holds row l)
+#pragma unroll
+            for (int r = 0; r < kDenseRows; ++r)
+            {
+                const int i = j0 + tid + r * kDenseThreads;
+                if (i < j0 + jb)
+                {
+#pragma unroll
+                    for (int jj = 0; jj < NB; ++jj) Dg[(i - j0) * NB + jj] = acc[r][jj];
+                }
+            }
+            __syncthreads();
+            if (wv == 0)
+            {
+                double rowv[NB];
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj)
+                    rowv[jj] = lane < jb ? (jj < jb ? Dg[lane * NB + jj] : 0.0) : (jj == lane ? 1.0 : 0.0);
+                bool bad = false;
+                // columns j >= jb of a short last panel are identity (rows of lanes >= jb too): no effect
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+                {
+                    const double d = __shfl(rowv[j], j, kWave);
+                    const bool ok = d > 0.0 && pbdev::finite(d);
+                    bad = bad || (j < jb && !ok);
+                    const double ljj = ok ? sqrt(d) : 1.0;
+                    if (lane == j) rowv[j] = ljj;
+                    if (lane > j) rowv[j] /= ljj;
+#pragma unroll
+                    for (int q = j + 1; q < NB; ++q)
+                    {
+                        const double lq = __shfl(rowv[j], q, kWave);
+                        if (lane > j) rowv[q] -= rowv[j] * lq;
+                    }
+                }
+                if (lane < jb)
+                {
+#pragma unroll
+                    for (int jj = 0; jj < NB; ++jj) Dg[lane * NB + jj] = jj <= lane ? rowv[jj] : 0.0;
+                }
+                if (lane == 0 && bad) failed = 1;
+            }
+            __syncthreads();
+            // rows below the diagonal block: L(i, panel) = acc L_D^-T (forward substitution over the panel)
+#pragma unroll
+            for (int r = 0; r < kDenseRows; ++r)
+            {
+                const int i = j0 + tid + r * kDenseThreads;
+                if (i >= n) continue;
+                if (i < j0 + jb)
+                {
+#pragma unroll
+                    for (int jj = 0; jj < NB; ++jj) acc[r][jj] = Dg[(i - j0) * NB + jj];
+                }
+                else
+                {
+#pragma unroll
+                    for (int j = 0; j < NB; ++j)
+                    {
+                        double s = acc[r][j];
+#pragma unroll
+                        for (int q = 0; q < j; ++q) s -= acc[r][q] * Dg[j * NB + q];
+                        acc[r][j] = j < jb ? s / Dg[j * NB + j] : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj)
+                    if (jj < jb) A[size_t(j0 + jj) * n + i] = acc[r][jj];
+            }
+            __syncthreads();
+        }
+        if (failed)
+        {
+            if (tid == 0) B.reject[w] = 1;
+            return;
+        }
+
+        // forward substitution L y = b
+        for (int j0 = 0; j0 < n; j0 += NB)
+        {
+            const int jb = min(NB, n - j0);
+            if (wv == 0)
+            {
+                double rowv[NB];
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj)
+                    rowv[jj] = (lane < jb && jj <= lane) ? A[size_t(j0 + jj) * n + j0 + lane] : 0.0;
+                double val = lane < jb ? vec[j0 + lane] : 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+                {
+                    if (j < jb)
+                    {
+                        const double xj = __shfl(val, j, kWave) / __shfl(rowv[j], j, kWave);
+                        if (lane > j) val -= rowv[j] * xj;
+                        if (lane == j) val = xj;
+                    }
+                }
+                if (lane < jb) vec[j0 + lane] = val;
+            }
+            __syncthreads();
+            for (int i = j0 + jb + tid; i < n; i += kDenseThreads)
+            {
+                double s = vec[i];
+                for (int jj = 0; jj < jb; ++jj) s -= A[size_t(j0 + jj) * n + i] * vec[j0 + jj];
+                vec[i] = s;
+            }
+            __syncthreads();
+        }
+        // backward substitution L^T x = y, panels from the last
+        for (int j0 = ((n - 1) / NB) * NB; j0 >= 0; j0 -= NB)
+        {
+            const int jb = min(NB, n - j0);
+            double part[NB];
+#pragma unroll
+            for (int jj = 0; jj < NB; ++jj) part[jj] = 0.0;
+            for (int i = j0 + jb + tid; i < n; i += kDenseThreads)
+            {
+                const double xi = vec[i];
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj)
+                    if (jj < jb) part[jj] += A[size_t(j0 + jj) * n + i] * xi;
+            }
+#pragma unroll
+            for (int jj = 0; jj < NB; ++jj)
+            {
+                const double s = waveSum(part[jj]);
+                if (lane == 0) red[wv * NB + jj] = s;
+            }
+            // the panel's diagonal block, row-major
+            for (int q = tid; q < NB * NB; q += kDenseThreads)
+            {
+                const int i = q / NB, j = q % NB;
+                Dg[q] = (i < jb && j <= i) ? A[size_t(j0 + j) * n + j0 + i] : 0.0;
+            }
+            __syncthreads();
+            if (wv == 0)
+            {
+                double val = 0.0;
+                if (lane < jb)
+                {
+                    val = vec[j0 + lane];
+                    for (int q = 0; q < kWaves; ++q) val -= red[q * NB + lane];
+                }
+#pragma unroll
+                for (int j = NB - 1; j >= 0; --j)
+                {
+                    if (j < jb)
+                    {
+                        const double xj = __shfl(val, j, kWave) / Dg[j * NB + j];
+                        if (lane < j) val -= Dg[j * NB + (lane & (NB - 1))] * xj;
+                        if (lane == j) val = xj;
+                    }
+                }
+                if (lane < jb) vec[j0 + lane] = val;
+            }
+            __syncthreads();
+        }
+        for (int i = tid; i < n; i += kDenseThreads) Wk.x[unitOf(i / 6) * 6 + i % 6] = T(vec[i]);
+    }
+
+    // Directional decrease dc.directRhs + sum bp.dp and the two step RMS values of each window
+    // (BundleAdjustPlaMatrix.cpp:77-158), in a fixed reduction order.
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_batch_step_stats(Problem P, Work<T> Wk, Batch B)
+    {
+        const int w = blockIdx.x;
+        if (!B.active[w] || B.reject[w]) return;
+        double dd = 0.0, gs = 0.0, ss = 0.0;
+        const int pu0 = B.poseUnitOffset[w], nPose = B.poseUnitOffset[w + 1] - pu0;
+        const int gu0 = B.groupUnitOffset[w], nU = nPose + B.groupUnitOffset[w + 1] - gu0;
+        for (int r = threadIdx.x; r < nU; r += blockDim.x)
+        {
+            const int u = r < nPose ? pu0 + r : gu0 + (r - nPose);
+            const int lim = P.unitWidth[u];
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+            {
+                const double x = double(Wk.x[u * 6 + a]);
+                dd += x * Wk.directRhs[u * 6 + a];
+                if (a < lim) gs += x * x;
+            }
+        }
+        for (int b = B.elimOffset[w] + threadIdx.x; b < B.elimOffset[w + 1]; b += blockDim.x)
+        {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                const double d = Wk.dp[3 * b + i];
+                dd += Wk.bp[3 * b + i] * d;
+                ss += d * d;
+            }
+        }
+        __shared__ double red[3][4];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        dd = waveSum(dd);
+        gs = waveSum(gs);
+        ss = waveSum(ss);
+        if (lane == 0)
+        {
+            red[0][wv] = dd;
+            red[1][wv] = gs;
+            red[2][wv] = ss;
+        }
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        dd = gs = ss = 0.0;
+        for (int q = 0; q < int(blockDim.x / kWave); ++q)
+        {
+            dd += red[0][q];
+            gs += red[1][q];
+            ss += red[2][q];
+        }
+        const int gd = B.globalDim[w], ne = B.elimOffset[w + 1] - B.elimOffset[w];
+        B.dd[w] = dd;
+        B.rmsG[w] = gd > 0 ? sqrt(gs / gd) : 0.0;
+        B.rmsP[w] = ne > 0 ? sqrt(ss / (3.0 * ne)) : 0.0;
+        if (!pbdev::finite(dd))
+        {
+            B.reject[w] = 1;
+            return;
+        }
+        B.searching[w] = 1;
+        B.lsCount[w] = 0;
+        B.alphaNext[w] = 1.0;
+    }
+
+    // Step lengths of the next trial pass: alphaNext * 2^-k for the windows still searching.
+    __global__ void k_batch_alphas(Batch B, int maxAlpha)
+    {
+        const int w = blockIdx.x * blockDim.x + threadIdx.x;
+        if (w == 0) B.counts[1] = 0;
+        if (w >= B.nWin) return;
+        if (!B.searching[w])
+        {
+            B.nAlpha[w] = 0;
+            return;
+        }
+        const int n = min(maxAlpha, B.lineSearchSteps - B.lsCount[w]);
+        for (int k = 0; k < n; ++k) B.alpha[w * kMaxAlpha + k] = ldexp(B.alphaNext[w], -k);
+        B.nAlpha[w] = n;
+    }
+
+    // Trial objectives: each workgroup covers one chunk of one window's observations.
+    template <typename T, int NA>
+    __global__ void __launch_bounds__(256) k_batch_trial_cost(Problem P, Work<T> Wk, Batch B)
+    {
+        const int ch = blockIdx.x;
+        const int w = B.chunkWin[ch];
+        const int nA = B.nAlpha[w];
+        if (nA == 0) return;
+        double acc[NA], al[NA];
+        int bad[NA];
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+        {
+            acc[a] = 0.0;
+            bad[a] = 0;
+            al[a] = a < nA ? B.alpha[w * kMaxAlpha + a] : 0.0;
+        }
+        for (int o = B.chunkBegin[ch] + threadIdx.x; o < B.chunkEnd[ch]; o += blockDim.x)
+            trialObservation<NA>(P, Wk.candCams, Wk.dp, o, al, nA, acc, bad);
+        __shared__ double lds[NA][4];
+        __shared__ int lbad[NA][4];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+        {
+            const double s = waveSum(acc[a]);
+            const int anyBad = __any(bad[a]) ? 1 : 0;
+            if (lane == 0)
+            {
+                lds[a][wv] = s;
+                lbad[a][wv] = anyBad;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < NA)
+        {
+            const int a = threadIdx.x;
+            double s = 0.0;
+            int bb = 0;
+            for (int q = 0; q < int(blockDim.x / kWave); ++q)
+            {
+                s += lds[a][q];
+                bb |= lbad[a][q];
+            }
+            B.trialPart[ch * kMaxAlpha + a] = s;
+            B.trialBad[ch * kMaxAlpha + a] = bb;
+        }
+    }
+
+    // Sums each window's chunk partials per step length (lane-strided, then the wave butterfly) and runs the
+    // Armijo test over the pass's step lengths in sequential order (the first that passes is accepted, as in
+    // the reference's backtracking loop).
+    template <typename T, int NA>
+    __global__ void k_batch_decide(Problem P, Work<T> Wk, Batch B)
+    {
+        const int lane = threadIdx.x & 63;
+        const int w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+        if (w >= B.nWin) return;
+        const int nA = B.nAlpha[w];
+        if (nA == 0) return;
+        double s[NA];
+        int bad[NA];
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+        {
+            s[a] = 0.0;
+            bad[a] = 0;
+        }
+        for (int k = B.winChunkOffset[w] + lane; k < B.winChunkOffset[w + 1]; k += kWave)
+        {
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+            {
+                if (a >= nA) break;
+                s[a] += B.trialPart[k * kMaxAlpha + a];
+                bad[a] |= B.trialBad[k * kMaxAlpha + a];
+            }
+        }
+        bool anyBad[NA];
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+        {
+            s[a] = waveSum(s[a]);
+            anyBad[a] = __any(bad[a] != 0);
+        }
+        if (lane != 0) return;
+        int32_t* cnt = B.counters + w * kWinCounterCount;
+        const double current = B.cost[w], dd = B.dd[w];
+        bool accepted = false;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+        {
+            if (a >= nA || accepted) break;
+            double cand = s[a];
+            for (int g = B.groupOffset[w]; g < B.groupOffset[w + 1]; ++g) cand += Wk.candPriorCost[a * P.nGroup + g];
+            if (anyBad[a]) cand = __builtin_inf();
+            ++cnt[kWinEvaluations];
+            ++B.lsCount[w];
+            const double alpha = B.alpha[w * kMaxAlpha + a];
+            const double limit = current - B.armijo * alpha * dd;
+            if (pbdev::finite(cand) && cand <= limit && cand < current)
+            {
+                accepted = true;
+                B.acceptIdx[w] = a;
+                B.acceptAlpha[w] = alpha;
+                B.acceptCost[w] = cand;
+            }
+        }
+        if (accepted || B.lsCount[w] >= B.lineSearchSteps)
+        {
+            B.searching[w] = 0;
+            return;
+        }
+        B.alphaNext[w] = B.alpha[w * kMaxAlpha + nA - 1] * 0.5;
+        atomicAdd(B.counts + 1, 1);
+    }
+
+    // Controller step of each running window: counts, damping, acceptance, convergence and budget.
+    __global__ void k_batch_control(Batch B)
+    {
+        const int w = blockIdx.x * blockDim.x + threadIdx.x;
+        if (w >= B.nWin || !B.active[w]) return;
+        int32_t* cnt = B.counters + w * kWinCounterCount;
+        ++cnt[kWinIterations];
+        double& lambda = B.damping[w];
+        if (B.reject[w])
+        {
+            ++cnt[kWinRejected];
+            lambda = lambda == 0.0 ? 1e-3 : lambda * 10.0;
+        }
+        else if (B.acceptIdx[w] >= 0)
+        {
+            ++cnt[kWinAccepted];
+            B.cost[w] = B.acceptCost[w];
+            lambda /= 10.0;
+            const double a = B.acceptAlpha[w];
+            if (a * B.rmsG[w] <= B.rmsTol && a * B.rmsP[w] <= B.rmsTol) B.status[w] = kWinConverged;
+        }
+        else
+        {
+            // line search exhausted: the controller stops (BundleAdjustPlaMatrix.cpp:532-536)
+            ++cnt[kWinRejected];
+            B.status[w] = kWinNoConvergence;
+        }
+        if (B.status[w] == kWinRunning && cnt[kWinIterations] >= B.maxIterations) B.status[w] = kWinNoConvergence;
+        if (B.status[w] != kWinRunning)
+            B.doneClock[w] = deviceClock();
+        else
+            atomicAdd(B.counts, 1);
+    }
+
+    // Commits the accepted candidate of each window: cameras, group parameters and points.
+    template <typename T>
+    __global__ void k_batch_apply(Problem P, Work<T> Wk, Batch B)
+    {
+        const int id = blockIdx.x * blockDim.x + threadIdx.x;
+        if (id < P.nCam)
+        {
+            const int ai = B.acceptIdx[P.camWin[id]];
+            if (ai >= 0)
+            {
+                for (int i = 0; i < 9; ++i) P.camR[9 * id + i] = Wk.candR[(ai * P.nCam + id) * 9 + i];
+                for (int i = 0; i < 3; ++i) P.camC[3 * id + i] = Wk.candC[(ai * P.nCam + id) * 3 + i];
+            }
+        }
+        if (id < P.nGroup * 13)
+        {
+            const int ai = B.acceptIdx[P.groupWin[id / 13]];
+            if (ai >= 0) P.groupParams[id] = Wk.candGroupParams[ai * P.nGroup * 13 + id];
+        }
+        if (id < P.nElim)
+        {
+            const int w = P.elimWin[id];
+            if (B.acceptIdx[w] >= 0)
+            {
+                const double alpha = B.acceptAlpha[w];
+                const int t = P.elimTrack[id];
+                const double s = pbdev::pointScale(P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2]);
+                for (int i = 0; i < 3; ++i) P.pointX[3 * t + i] += s * alpha * Wk.dp[3 * id + i];
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    template <typename T>
+    void BatchLauncher<T>::start(const Batch& B, hipStream_t s)
+    {
+        k_batch_start<<<1, 64, 0, s>>>(B);
+    }
+
+    template <typename T>
+    void BatchLauncher<T>::roundBegin(const Problem& P, const Work<T>&, const Batch& B, hipStream_t s)
+    {
+        k_batch_round_begin<<<gridFor(B.nWin, 256), 256, 0, s>>>(P, B);
+    }
+
+    template <typename T>
+    void BatchLauncher<T>::build(const Problem& P, const Work<T>& W, hipStream_t s)
+    {
+        // the shared kernels read each window's damping and activity from P
+        Launcher<T>::prepareCameras(P, s);
+        Launcher<T>::linearizePoints(P, W, 0.0, s);
+        Launcher<T>::cameraDirect(P, W, s);
+        Launcher<T>::schurBlocks(P, W, 0.0, s);
+    }
+
+    template <typename T>
+    void BatchLauncher<T>::afterBuild(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s)
+    {
+        k_batch_after_build<T><<<gridFor(B.nWin, 4), 256, 0, s>>>(P, W, B);
+    }
+
+    template <typename T>
+    void BatchLauncher<T>::factor(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s)
+    {
+        const size_t lds = sizeof(double) * (size_t(B.maxDim) * kDensePanel + kDensePanel * kDensePanel + B.maxDim +
+                                             (kDenseThreads / kWave) * kDensePanel);
+        if (lds > 64 * 1024)
+        {
+            // above the default dynamic LDS limit (kMaxDense = 512 needs 72 KiB of the CU's 160 KiB)
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_factor<T>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            if (e != hipSuccess) throw std::runtime_error("hipFuncSetAttribute (dense window solve LDS)");
+        }
+        k_batch_factor<T><<<B.nWin, kDenseThreads, lds, s>>>(P, W, B);
+    }
+
+    template <typename T>
+    void BatchLauncher<T>::backSubstitute(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s)
+    {
+        Launcher<T>::pointSteps(P, W, s);
+        k_batch_step_stats<T><<<B.nWin, 256, 0, s>>>(P, W, B);
+    }
+
+    template <typename T>
+    void BatchLauncher<T>::trialPass(const Problem& P, const Work<T>& W, const Batch& B, int maxAlpha, hipStream_t s)
+    {
+        k_batch_alphas<<<gridFor(B.nWin, 256), 256, 0, s>>>(B, maxAlpha);
+        Launcher<T>::trialStates(P, W, nullptr, maxAlpha, s);  // step lengths per window from P.winAlpha
+        if (B.nChunks > 0) k_batch_trial_cost<T, kMaxAlpha><<<B.nChunks, 256, 0, s>>>(P, W, B);
+        k_batch_decide<T, kMaxAlpha><<<gridFor(B.nWin, 4), 256, 0, s>>>(P, W, B);
+    }
+
+    template <typename T>
+    void BatchLauncher<T>::finishRound(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s)
+    {
+        k_batch_control<<<gridFor(B.nWin, 256), 256, 0, s>>>(B);
+        const long n = std::max<long>(std::max<long>(P.nCam, long(P.nGroup) * 13), P.nElim);
+        if (n > 0) k_batch_apply<T><<<gridFor(n, 256), 256, 0, s>>>(P, W, B);
+    }
+
+    template struct BatchLauncher<float>;
+    template struct BatchLauncher<double>;
+}  // namespace pbk

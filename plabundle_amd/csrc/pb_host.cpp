@@ -392,6 +392,7 @@ namespace
         long nPairs = 0;
         int maxGroupK = 0;
         std::vector<int32_t> activeTrackOf;  // active index -> input track
+        std::vector<int32_t> activeIndexOf;  // input track -> active index or -1
         std::vector<int64_t> obsInput;       // active observation -> input observation index
         std::vector<int32_t> obsOffset, obsCam, obsPoint, obsPoseEntry, obsGroupEntry;
         std::vector<double> obsUV, obsW, pointX;
@@ -431,6 +432,8 @@ namespace
             L->activeTrackOf.swap(sorted);
         }
         L->nActive = static_cast<int>(L->activeTrackOf.size());
+        L->activeIndexOf.assign(in.nTracks, -1);
+        for (int a = 0; a < L->nActive; ++a) L->activeIndexOf[L->activeTrackOf[a]] = a;
         L->obsOffset.assign(L->nActive + 1, 0);
         for (int a = 0; a < L->nActive; ++a)
         {
@@ -789,6 +792,18 @@ namespace
     // per chunk, so the events do not stretch the solve); chunks that contain the stop are not counted, so
     // every timed launch did a full iteration
     constexpr int kFamilyCount = sizeof(kFamilies) / sizeof(kFamilies[0]);
+
+    // Host side of a batched solve: window ranges of the combined input and the per-window counts.
+    struct BatchHost
+    {
+        int nWin = 0;
+        std::vector<int32_t> camBase, trackBase;  // [nWin+1] input camera / track ranges of the windows
+        std::vector<int32_t> gaugeAnchor, gaugeScale;  // per window after normalisation (window numbering), -1 = none
+        std::vector<double> gaugeBaseline;
+        std::vector<int32_t> observations, activeTracks, cameraBlocks, primaryBlocks;
+        pbk::Batch B;
+        double clockHz = 1e8;
+    };
 }  // namespace
 
 struct pb_ctx
@@ -808,6 +823,9 @@ struct pb_ctx
     Layout lay;
     DeviceAlloc dstruct;  // structure + state
     DeviceAlloc dwork32, dwork64;
+    DeviceAlloc dbatch;      // per-window controller state of a batched solve
+    bool batchMode = false;  // the context holds a batch (pb_hip_solve_batch); single-problem entries refuse it
+    BatchHost bh;
     bool work32 = false, work64 = false;
     pbk::Problem P;
     pbk::Work<float> W32;
@@ -960,17 +978,16 @@ namespace
         return {};
     }
 
+    void releaseStructure(pb_ctx* ctx);
+    void uploadStructure(pb_ctx* ctx, Prepared&& p, Layout&& L, const pb_solve_options& o);
+
     // Builds the prepared structure and uploads it (setup; once per problem and structural options).
     void ensureStructure(pb_ctx* ctx, const pb_solve_options& o)
     {
         if (ctx->structureValid && ctx->structureMask == o.shared_intrinsic_mask &&
             ctx->structureRefinePose == o.refine_camera_pose && ctx->structureMaxRms == o.max_initial_track_rms)
             return;
-        ctx->structureValid = false;
-        ctx->dstruct.release();
-        ctx->dwork32.release();
-        ctx->dwork64.release();
-        ctx->work32 = ctx->work64 = false;
+        releaseStructure(ctx);
         Prepared p;
         const std::string gerr = normalizeGauge(ctx->in, o, &p);
         if (!gerr.empty()) throw std::invalid_argument(gerr);
@@ -978,6 +995,22 @@ namespace
         prepareGroups(ctx->in, o, &p);
         Layout L;
         buildLayout(ctx->in, p, &L);
+        uploadStructure(ctx, std::move(p), std::move(L), o);
+    }
+
+    void releaseStructure(pb_ctx* ctx)
+    {
+        ctx->structureValid = false;
+        ctx->dstruct.release();
+        ctx->dwork32.release();
+        ctx->dwork64.release();
+        ctx->dbatch.release();
+        ctx->work32 = ctx->work64 = false;
+    }
+
+    // Device copy of a prepared problem and its layout (setup).
+    void uploadStructure(pb_ctx* ctx, Prepared&& p, Layout&& L, const pb_solve_options& o)
+    {
         hipStream_t s = ctx->stream;
         DeviceAlloc& A = ctx->dstruct;
         pbk::Problem P;
@@ -1626,6 +1659,444 @@ namespace
         }
         return PB_OK;
     }
+
+    // ------------------------------------------------------------------------------------------
+    // Batched windows (pb_hip_solve_batch).
+    //
+    // The windows are concatenated into one input (cameras, calibration groups and tracks numbered window by
+    // window, observation cameras re-indexed) after each window's gauge is normalised on its own, and the
+    // combined input is prepared and laid out as one problem. Its reduced system is block-diagonal by window,
+    // and tracks, eliminated points, observations, pose units and group units are contiguous per window
+    // (camera blocks and groups are numbered in camera order, tracks are ordered by their lowest camera).
+    void prepareBatch(pb_ctx* ctx, const pb_batch_view* batch, const pb_solve_options& o)
+    {
+        releaseStructure(ctx);
+        ctx->batchMode = false;
+        ctx->uploaded = false;
+        ctx->haveResult = false;
+        const int nWin = batch->num_windows;
+        BatchHost bh;
+        bh.nWin = nWin;
+        Input in;
+        in.hasGroup = true;
+        in.gaugePolicy = PB_GAUGE_CALLER_MANAGED;
+        in.off.assign(1, 0);
+        std::vector<int32_t> fixedCams, fixedTracks;
+        bh.camBase.assign(1, 0);
+        bh.trackBase.assign(1, 0);
+        int nextGroup = 0;
+        for (int w = 0; w < nWin; ++w)
+        {
+            const pb_problem_view* v = &batch->windows[w];
+            auto windowError = [&](const std::string& m) {
+                const bool unsupported = m.rfind("unsupported", 0) == 0;
+                return std::invalid_argument((unsupported ? "unsupported: window " : "window ") + std::to_string(w) +
+                                             ": " + m);
+            };
+            const std::string verr = validateInput(v);
+            if (!verr.empty()) throw windowError(verr);
+            Input wi;
+            copyIn(v, &wi);
+            Prepared pw;
+            const std::string gerr = normalizeGauge(wi, o, &pw);
+            if (!gerr.empty()) throw windowError(gerr);
+            const int cb = in.nCam, tb = in.nTracks;
+            const int64_t ob = in.off.back();
+            std::vector<std::pair<int32_t, int32_t>> ids;  // window group id -> combined id
+            for (int c = 0; c < wi.nCam; ++c)
+            {
+                const int32_t id = wi.hasGroup ? wi.group[c] : 0;
+                auto it = std::find_if(ids.begin(), ids.end(), [&](const auto& e) { return e.first == id; });
+                if (it == ids.end())
+                {
+                    ids.emplace_back(id, nextGroup++);
+                    in.group.push_back(ids.back().second);
+                }
+                else
+                {
+                    in.group.push_back(it->second);
+                }
+            }
+            in.R.insert(in.R.end(), wi.R.begin(), wi.R.end());
+            in.C.insert(in.C.end(), wi.C.begin(), wi.C.end());
+            in.slots.insert(in.slots.end(), wi.slots.begin(), wi.slots.end());
+            in.ref.insert(in.ref.end(), wi.ref.begin(), wi.ref.end());
+            in.conv.insert(in.conv.end(), wi.conv.begin(), wi.conv.end());
+            in.X.insert(in.X.end(), wi.X.begin(), wi.X.end());
+            for (int t = 0; t < wi.nTracks; ++t) in.off.push_back(ob + wi.off[t + 1]);
+            for (int32_t c : wi.obsCam) in.obsCam.push_back(c >= 0 && c < wi.nCam ? c + cb : -1);
+            in.uv.insert(in.uv.end(), wi.uv.begin(), wi.uv.end());
+            in.weight.insert(in.weight.end(), wi.weight.begin(), wi.weight.end());
+            in.scale.insert(in.scale.end(), wi.scale.begin(), wi.scale.end());
+            for (int32_t c : pw.fixedCams) fixedCams.push_back(c + cb);
+            for (int32_t t : pw.fixedTracks) fixedTracks.push_back(t + tb);
+            const bool gauge = pw.anchor >= 0 && pw.scaleCam >= 0 && pw.baseline > 0.0;
+            bh.gaugeAnchor.push_back(gauge ? pw.anchor : -1);
+            bh.gaugeScale.push_back(gauge ? pw.scaleCam : -1);
+            bh.gaugeBaseline.push_back(gauge ? pw.baseline : 0.0);
+            in.nCam += wi.nCam;
+            in.nTracks += wi.nTracks;
+            bh.camBase.push_back(in.nCam);
+            bh.trackBase.push_back(in.nTracks);
+        }
+        ctx->in = std::move(in);
+        Prepared p;
+        p.fixedCams = fixedCams;
+        p.fixedTracks = fixedTracks;  // ascending: window by window, each window's list sorted
+        prepareActive(ctx->in, o, &p);
+        prepareGroups(ctx->in, o, &p);
+        Layout L;
+        buildLayout(ctx->in, p, &L);
+        uploadStructure(ctx, std::move(p), std::move(L), o);
+
+        // window tables
+        const Prepared& pp = ctx->prep;
+        const Layout& LL = ctx->lay;
+        const int nCam = ctx->in.nCam;
+        auto windowOfTrack = [&](int t) {
+            return static_cast<int32_t>(std::upper_bound(bh.trackBase.begin(), bh.trackBase.end(), t) -
+                                        bh.trackBase.begin()) - 1;
+        };
+        auto offsetsOf = [&](const std::vector<int32_t>& win, int base, const char* what) {
+            std::vector<int32_t> off(nWin + 1, 0);
+            for (size_t i = 0; i < win.size(); ++i)
+            {
+                if (i > 0 && win[i] < win[i - 1])
+                    throw std::logic_error(std::string("batch ") + what + " not grouped by window");
+                ++off[win[i] + 1];
+            }
+            off[0] = base;
+            for (int w = 0; w < nWin; ++w) off[w + 1] += off[w];
+            return off;
+        };
+        std::vector<int32_t> camWin(nCam);
+        for (int w = 0; w < nWin; ++w)
+            for (int c = bh.camBase[w]; c < bh.camBase[w + 1]; ++c) camWin[c] = w;
+        std::vector<int32_t> groupWin(pp.G, 0);
+        for (int c = 0; c < nCam; ++c)
+            if (pp.intrinsicBlock[c] >= 0) groupWin[pp.groupOfCam[c]] = camWin[c];
+        std::vector<int32_t> elimWin(LL.nElim), obsWin(LL.nObs);
+        for (int b = 0; b < LL.nElim; ++b) elimWin[b] = windowOfTrack(LL.activeTrackOf[LL.elimTrack[b]]);
+        for (int i = 0; i < LL.nObs; ++i) obsWin[i] = windowOfTrack(LL.activeTrackOf[LL.obsPoint[i]]);
+        std::vector<int32_t> unitWin(pp.nUnits, 0), poseWin, groupUnitWin;
+        for (int c = 0; c < nCam; ++c)
+            if (LL.camUnit[c] >= 0)
+            {
+                if (LL.camUnit[c] != static_cast<int32_t>(poseWin.size()))
+                    throw std::logic_error("batch pose units not in camera order");
+                unitWin[LL.camUnit[c]] = camWin[c];
+                poseWin.push_back(camWin[c]);
+            }
+        for (int g = 0; g < pp.G; ++g)
+        {
+            if (pp.gK[g] == 0) continue;
+            for (int j = 0; j < (pp.gK[g] + 5) / 6; ++j)
+            {
+                if (pp.gUnit[g] + j != pp.cameraBlocks + static_cast<int32_t>(groupUnitWin.size()))
+                    throw std::logic_error("batch group units not in group order");
+                unitWin[pp.gUnit[g] + j] = groupWin[g];
+                groupUnitWin.push_back(groupWin[g]);
+            }
+        }
+        const std::vector<int32_t> groupOffset = offsetsOf(groupWin, 0, "groups");
+        const std::vector<int32_t> elimOffset = offsetsOf(elimWin, 0, "points");
+        const std::vector<int32_t> obsOffset = offsetsOf(obsWin, 0, "observations");
+        const std::vector<int32_t> poseUnitOffset = offsetsOf(poseWin, 0, "pose units");
+        const std::vector<int32_t> groupUnitOffset = offsetsOf(groupUnitWin, pp.cameraBlocks, "group units");
+        std::vector<int32_t> globalDim(nWin), chunkWin, chunkBegin, chunkEnd, winChunkOffset(1, 0);
+        std::vector<int64_t> denseOffset(1, 0);
+        int maxDim = 0;
+        bh.observations.assign(nWin, 0);
+        bh.activeTracks.assign(nWin, 0);
+        bh.cameraBlocks.assign(nWin, 0);
+        bh.primaryBlocks.assign(nWin, 0);
+        for (int a = 0; a < LL.nActive; ++a) ++bh.activeTracks[windowOfTrack(LL.activeTrackOf[a])];
+        for (int w = 0; w < nWin; ++w)
+        {
+            const int poseUnits = poseUnitOffset[w + 1] - poseUnitOffset[w];
+            const int groupUnits = groupUnitOffset[w + 1] - groupUnitOffset[w];
+            int dim = 6 * poseUnits;
+            for (int g = groupOffset[w]; g < groupOffset[w + 1]; ++g) dim += pp.gK[g];
+            globalDim[w] = dim;
+            const int n = 6 * (poseUnits + groupUnits);
+            if (n > pbk::kMaxDense)
+                throw std::invalid_argument("unsupported: window " + std::to_string(w) + " has " +
+                                            std::to_string(poseUnits + groupUnits) +
+                                            " primary units; the batched dense solve takes at most " +
+                                            std::to_string(pbk::kMaxDense / 6));
+            maxDim = std::max(maxDim, n);
+            denseOffset.push_back(denseOffset.back() + int64_t(n) * n);
+            for (int b = obsOffset[w]; b < obsOffset[w + 1]; b += pbk::kTrialChunk)
+            {
+                chunkWin.push_back(w);
+                chunkBegin.push_back(b);
+                chunkEnd.push_back(std::min(obsOffset[w + 1], b + pbk::kTrialChunk));
+            }
+            winChunkOffset.push_back(static_cast<int32_t>(chunkWin.size()));
+            bh.observations[w] = obsOffset[w + 1] - obsOffset[w];
+            bh.cameraBlocks[w] = poseUnits;
+            bh.primaryBlocks[w] = poseUnits + (groupOffset[w + 1] - groupOffset[w]);
+        }
+        std::vector<int32_t> camGaugeAnchor(nCam, -1);
+        for (int w = 0; w < nWin; ++w)
+            if (bh.gaugeScale[w] >= 0)
+                camGaugeAnchor[bh.camBase[w] + bh.gaugeScale[w]] = bh.camBase[w] + bh.gaugeAnchor[w];
+
+        hipStream_t s = ctx->stream;
+        DeviceAlloc& A = ctx->dbatch;
+        pbk::Problem& P = ctx->P;
+        pbk::Batch& B = bh.B;
+        P.nWin = nWin;
+        P.camWin = A.upload(camWin, s);
+        P.groupWin = A.upload(groupWin, s);
+        P.elimWin = A.upload(elimWin, s);
+        P.unitWin = A.upload(unitWin, s);
+        P.camGaugeAnchor = A.upload(camGaugeAnchor, s);
+        P.winBaseline = A.upload(bh.gaugeBaseline, s);
+        P.winFlags = A.alloc<int32_t>(size_t(nWin) * pbk::kFlagCount);
+        B.nWin = nWin;
+        B.nChunks = static_cast<int>(chunkWin.size());
+        B.maxDim = maxDim;
+        B.camOffset = A.upload(bh.camBase, s);
+        B.groupOffset = A.upload(groupOffset, s);
+        B.elimOffset = A.upload(elimOffset, s);
+        B.poseUnitOffset = A.upload(poseUnitOffset, s);
+        B.groupUnitOffset = A.upload(groupUnitOffset, s);
+        B.globalDim = A.upload(globalDim, s);
+        B.denseOffset = A.upload(denseOffset, s);
+        B.chunkWin = A.upload(chunkWin, s);
+        B.chunkBegin = A.upload(chunkBegin, s);
+        B.chunkEnd = A.upload(chunkEnd, s);
+        B.winChunkOffset = A.upload(winChunkOffset, s);
+        B.status = A.alloc<int32_t>(nWin);
+        B.active = A.alloc<int32_t>(nWin);
+        B.reject = A.alloc<int32_t>(nWin);
+        B.damping = A.alloc<double>(nWin);
+        B.cost = A.alloc<double>(nWin);
+        B.initialCost = A.alloc<double>(nWin);
+        B.counters = A.alloc<int32_t>(size_t(nWin) * pbk::kWinCounterCount);
+        B.dd = A.alloc<double>(nWin);
+        B.rmsG = A.alloc<double>(nWin);
+        B.rmsP = A.alloc<double>(nWin);
+        B.searching = A.alloc<int32_t>(nWin);
+        B.lsCount = A.alloc<int32_t>(nWin);
+        B.alphaNext = A.alloc<double>(nWin);
+        B.alpha = A.alloc<double>(size_t(nWin) * pbk::kMaxAlpha);
+        B.nAlpha = A.alloc<int32_t>(nWin);
+        B.acceptIdx = A.alloc<int32_t>(nWin);
+        B.acceptAlpha = A.alloc<double>(nWin);
+        B.acceptCost = A.alloc<double>(nWin);
+        B.dense = A.alloc<double>(size_t(denseOffset.back()));
+        B.trialPart = A.alloc<double>(size_t(B.nChunks) * pbk::kMaxAlpha);
+        B.trialBad = A.alloc<int32_t>(size_t(B.nChunks) * pbk::kMaxAlpha);
+        B.doneClock = A.alloc<unsigned long long>(nWin + 1);
+        B.counts = A.alloc<int32_t>(2);
+        P.winDamping = B.damping;
+        P.winActive = B.active;
+        P.winAlpha = B.alpha;
+        P.winNAlpha = B.nAlpha;
+        int khz = 0;
+        check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device), "hipDeviceGetAttribute");
+        bh.clockHz = khz > 0 ? 1e3 * khz : 1e8;
+        check(hipStreamSynchronize(s), "batch upload");
+        ctx->bh = std::move(bh);
+        ctx->uploaded = true;
+        ctx->batchMode = true;
+    }
+
+    // One LM controller per window, all windows advancing one iteration per round
+    // (BundleAdjustPlaMatrix.cpp:162-631 with useReferenceOnlineSchur, per window).
+    template <typename T>
+    void runBatch(pb_ctx* ctx, const pb_solve_options& o, pb_window_report* reports, pb_solve_report* summary,
+                  const volatile int32_t* cancel)
+    {
+        using BL = pbk::BatchLauncher<T>;
+        pbk::Work<T>& W = work<T>(ctx);
+        hipStream_t s = ctx->stream;
+        const pbk::Problem& P = ctx->P;
+        BatchHost& bh = ctx->bh;
+        pbk::Batch& B = bh.B;
+        const int nWin = bh.nWin;
+        resetState(ctx);
+        B.maxIterations = std::max(1, o.max_iterations);
+        B.lineSearchSteps = o.line_search_steps;
+        B.armijo = o.armijo_coefficient;
+        B.rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
+        const int passAlpha = o.armijo_batch > 0 ? std::min(o.armijo_batch, pbk::kMaxAlpha) : 4;
+        // controller state (zero = running, no damping, no counts); direct blocks that no camera writes stay zero
+        check(hipMemsetAsync(B.status, 0, sizeof(int32_t) * nWin, s), "memset");
+        check(hipMemsetAsync(B.damping, 0, sizeof(double) * nWin, s), "memset");
+        check(hipMemsetAsync(B.counters, 0, sizeof(int32_t) * size_t(nWin) * pbk::kWinCounterCount, s), "memset");
+        check(hipMemsetAsync(B.cost, 0, sizeof(double) * nWin, s), "memset");
+        check(hipMemsetAsync(B.initialCost, 0, sizeof(double) * nWin, s), "memset");
+        check(hipMemsetAsync(B.doneClock, 0, sizeof(unsigned long long) * (nWin + 1), s), "memset");
+        check(hipMemsetAsync(W.D, 0, sizeof(double) * size_t(std::max(1, P.nBlocks)) * 36, s), "memset");
+        check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
+        check(hipStreamSynchronize(s), "reset");
+        const auto t0 = Clock::now();
+        BL::start(B, s);
+        int32_t* hc = reinterpret_cast<int32_t*>(ctx->pinned + kPinFlags);
+        int rounds = 0, passes = 0;
+        bool cancelled = false;
+        // every running window spends one iteration per round, so the budget ends the loop
+        while (rounds <= B.maxIterations)
+        {
+            if (cancel && *cancel)
+            {
+                cancelled = true;
+                break;
+            }
+            BL::roundBegin(P, W, B, s);
+            BL::build(P, W, s);
+            BL::afterBuild(P, W, B, s);
+            BL::factor(P, W, B, s);
+            BL::backSubstitute(P, W, B, s);
+            // trial passes until no window is still searching (each pass spends at least one of a window's
+            // line-search steps, so at most line_search_steps passes)
+            for (int pass = 0; pass <= o.line_search_steps; ++pass)
+            {
+                BL::trialPass(P, W, B, pass == 0 ? 1 : passAlpha, s);
+                ++passes;
+                check(hipMemcpyAsync(hc, B.counts + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s), "readback");
+                check(hipStreamSynchronize(s), "trial pass");
+                if (hc[0] == 0) break;
+            }
+            BL::finishRound(P, W, B, s);
+            check(hipMemcpyAsync(hc, B.counts, sizeof(int32_t), hipMemcpyDeviceToHost, s), "readback");
+            check(hipStreamSynchronize(s), "round");
+            ++rounds;
+            if (hc[0] == 0) break;
+        }
+        check(hipStreamSynchronize(s), "batch");
+        summary->solve_seconds = seconds(t0);
+        summary->iterations = rounds;
+        summary->objective_evaluations = passes;
+        std::vector<int32_t> status(nWin), counters(size_t(nWin) * pbk::kWinCounterCount);
+        std::vector<double> cost(nWin), initial(nWin);
+        std::vector<unsigned long long> clk(nWin + 1);
+        check(hipMemcpyAsync(status.data(), B.status, sizeof(int32_t) * nWin, hipMemcpyDeviceToHost, s), "readback");
+        check(hipMemcpyAsync(counters.data(), B.counters, sizeof(int32_t) * counters.size(), hipMemcpyDeviceToHost, s),
+              "readback");
+        check(hipMemcpyAsync(cost.data(), B.cost, sizeof(double) * nWin, hipMemcpyDeviceToHost, s), "readback");
+        check(hipMemcpyAsync(initial.data(), B.initialCost, sizeof(double) * nWin, hipMemcpyDeviceToHost, s),
+              "readback");
+        check(hipMemcpyAsync(clk.data(), B.doneClock, sizeof(unsigned long long) * (nWin + 1), hipMemcpyDeviceToHost, s),
+              "readback");
+        check(hipStreamSynchronize(s), "readback");
+        int converged = 0;
+        for (int w = 0; w < nWin; ++w)
+        {
+            pb_window_report& r = reports[w];
+            const int32_t* cnt = &counters[size_t(w) * pbk::kWinCounterCount];
+            r.iterations = cnt[pbk::kWinIterations];
+            r.accepted_steps = cnt[pbk::kWinAccepted];
+            r.rejected_steps = cnt[pbk::kWinRejected];
+            r.linearizations = cnt[pbk::kWinLinearizations];
+            r.objective_evaluations = cnt[pbk::kWinEvaluations];
+            r.observation_count = bh.observations[w];
+            r.active_track_count = bh.activeTracks[w];
+            r.camera_block_count = bh.cameraBlocks[w];
+            r.primary_block_count = bh.primaryBlocks[w];
+            r.initial_cost = initial[w];
+            r.final_cost = cost[w];
+            r.device_seconds = (clk[w] > 0 && clk[w] >= clk[nWin]) ? double(clk[w] - clk[nWin]) / bh.clockHz : 0.0;
+            if (cancelled)
+            {
+                r.solve_status = PB_SOLVE_CANCELLED;
+                continue;
+            }
+            if (bh.observations[w] == 0)
+            {
+                // as pb_hip_solve on the window alone: no usable tracks is invalid input
+                r.solve_status = PB_SOLVE_INVALID_INPUT;
+                continue;
+            }
+            r.converged = status[w] == pbk::kWinConverged ? 1 : 0;
+            r.solve_status = status[w] == pbk::kWinConverged ? PB_SOLVE_SUCCESS
+                             : status[w] == pbk::kWinNumeric ? PB_SOLVE_NUMERICAL_FAILURE
+                                                             : PB_SOLVE_NO_CONVERGENCE;
+            r.solution_usable = status[w] != pbk::kWinNumeric && std::isfinite(cost[w]) ? 1 : 0;
+            converged += r.converged;
+            summary->accepted_steps += r.accepted_steps;
+            summary->rejected_steps += r.rejected_steps;
+            summary->linearizations += r.linearizations;
+            summary->initial_cost += r.initial_cost;
+            summary->final_cost += r.final_cost;
+        }
+        if (cancelled)
+        {
+            summary->solve_status = PB_SOLVE_CANCELLED;
+            setMessage(summary, "cancelled; no intermediate solution published");
+            ctx->haveResult = false;
+            return;
+        }
+        summary->converged = converged == nWin ? 1 : 0;
+        summary->solution_usable = converged > 0 ? 1 : 0;
+        summary->solve_status = converged == nWin ? PB_SOLVE_SUCCESS : PB_SOLVE_NO_CONVERGENCE;
+        setMessage(summary, std::to_string(nWin) + " windows, " + std::to_string(converged) + " converged in " +
+                                std::to_string(rounds) + " rounds (dense window solves + Armijo per window)");
+        ctx->outR.resize(ctx->in.R.size());
+        ctx->outC.resize(ctx->in.C.size());
+        ctx->outGroupParams.resize(ctx->prep.gParams.size());
+        ctx->outPointX.resize(ctx->lay.pointX.size());
+        check(hipMemcpyAsync(ctx->outR.data(), P.camR, sizeof(double) * ctx->outR.size(), hipMemcpyDeviceToHost, s),
+              "download");
+        check(hipMemcpyAsync(ctx->outC.data(), P.camC, sizeof(double) * ctx->outC.size(), hipMemcpyDeviceToHost, s),
+              "download");
+        if (!ctx->outGroupParams.empty())
+            check(hipMemcpyAsync(ctx->outGroupParams.data(), P.groupParams, sizeof(double) * ctx->outGroupParams.size(),
+                                 hipMemcpyDeviceToHost, s),
+                  "download");
+        if (!ctx->outPointX.empty())
+            check(hipMemcpyAsync(ctx->outPointX.data(), P.pointX, sizeof(double) * ctx->outPointX.size(),
+                                 hipMemcpyDeviceToHost, s),
+                  "download");
+        check(hipStreamSynchronize(s), "download");
+        ctx->haveResult = true;
+    }
+
+    // Refined (or, before a solve, initial) state of input cameras [c0, c1) and tracks [t0, t1), written from
+    // index 0 of each output (publishIntrinsics, BundleAdjustPlaMatrixModel.cpp:271-352, without transition
+    // priors).
+    void downloadRange(pb_ctx* ctx, int c0, int c1, int t0, int t1, double* R, double* C, double* intr, double* X,
+                       int32_t* valid)
+    {
+        const Input& in = ctx->in;
+        const bool res = ctx->haveResult;
+        const Prepared& p = ctx->prep;
+        for (int c = c0; c < c1; ++c)
+        {
+            const int k = c - c0;
+            if (R) std::copy_n(res ? &ctx->outR[9 * c] : &in.R[9 * c], 9, R + 9 * k);
+            if (C) std::copy_n(res ? &ctx->outC[3 * c] : &in.C[3 * c], 3, C + 3 * k);
+            if (intr)
+            {
+                double pub[13];
+                std::copy_n(&in.slots[13 * c], 13, pub);
+                if (res && p.G > 0 && p.intrinsicBlock[c] >= 0)
+                {
+                    const int g = p.groupOfCam[c];
+                    const uint32_t en = p.gEnabled[g];
+                    const double* gp = &ctx->outGroupParams[13 * g];
+                    const double* ref = &in.ref[13 * c];
+                    pub[0] = (en & 1u) ? gp[0] : in.slots[13 * c];
+                    pub[1] = (en & 2u) ? gp[1] : in.slots[13 * c + 1];
+                    pub[2] = (en & 4u) ? ref[2] + gp[2] : in.slots[13 * c + 2];
+                    pub[3] = (en & 8u) ? ref[3] + gp[3] : in.slots[13 * c + 3];
+                    for (int q = 4; q < 13; ++q)
+                        if ((en >> q) & 1u) pub[q] = gp[q];
+                }
+                std::copy_n(pub, 13, intr + 13 * k);
+            }
+        }
+        for (int t = t0; t < t1; ++t)
+        {
+            const int a = ctx->structureValid ? ctx->lay.activeIndexOf[t] : -1;
+            const int k = t - t0;
+            if (X) std::copy_n((res && a >= 0) ? &ctx->outPointX[3 * a] : &in.X[3 * t], 3, X + 3 * k);
+            if (valid) valid[k] = (ctx->structureValid && p.activeTrack[t]) ? 1 : 0;
+        }
+    }
 }  // namespace
 
 extern "C" {
@@ -1734,6 +2205,8 @@ pb_status pb_hip_upload(pb_ctx* ctx, const pb_problem_view* problem)
     return guarded(ctx, [&]() -> pb_status {
         check(hipSetDevice(ctx->device), "hipSetDevice");
         copyIn(problem, &ctx->in);
+        if (ctx->batchMode) releaseStructure(ctx);
+        ctx->batchMode = false;
         ctx->structureValid = false;
         ctx->haveResult = false;
         ctx->uploaded = true;
@@ -1760,7 +2233,7 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
         setMessage(report, "the MI355X backend implements the online point-Schur path only");
         return fail(ctx, PB_ERR_UNSUPPORTED, report->message);
     }
-    if (!ctx->uploaded)
+    if (!ctx->uploaded || ctx->batchMode)
     {
         report->solve_status = PB_SOLVE_INVALID_INPUT;
         setMessage(report, "no problem uploaded");
@@ -1829,7 +2302,7 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
 {
     if (!ctx || !dl_dobs) return PB_ERR_INVALID;
     if (dl_dcontrol) return fail(ctx, PB_ERR_UNSUPPORTED, "unsupported: no control-point constraints in this backend");
-    if (!ctx->uploaded || !ctx->haveResult)
+    if (!ctx->uploaded || !ctx->haveResult || ctx->batchMode)
         return fail(ctx, PB_ERR_INVALID, "pb_hip_backward needs the result of a pb_hip_solve on this context");
     return guarded(ctx, [&]() -> pb_status {
         check(hipSetDevice(ctx->device), "hipSetDevice");
@@ -1871,7 +2344,9 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
         check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
         Lr::adjointRhs(P, W, dC, dP, s);
         Lr::schurBlocks(P, W, 0.0, s);
-        const PcgResult lin = pcg(ctx, W, 1e-12, 1e-300, std::max(200, 30 * ctx->prep.primaryBlocks), 8, 0.0);
+        // FP64 PCG to the FP64 solve path's accuracy class (relative residual 1e-10; 1e-12 is not reachable on
+        // aerial blocks whose reduced systems have condition numbers near 1e14)
+        const PcgResult lin = pcg(ctx, W, 1e-10, 1e-300, std::max(200, 30 * ctx->prep.primaryBlocks), 8, 0.0);
         if (!lin.converged)
         {
             tmp.release();
@@ -1898,44 +2373,85 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
 pb_status pb_hip_download(pb_ctx* ctx, double* R, double* C, double* intr, double* X, int32_t* valid)
 {
     if (!ctx || !ctx->uploaded) return PB_ERR_INVALID;
-    const Input& in = ctx->in;
-    const bool res = ctx->haveResult;
-    const Prepared& p = ctx->prep;
-    for (int c = 0; c < in.nCam; ++c)
-    {
-        if (R) std::copy_n(res ? &ctx->outR[9 * c] : &in.R[9 * c], 9, R + 9 * c);
-        if (C) std::copy_n(res ? &ctx->outC[3 * c] : &in.C[3 * c], 3, C + 3 * c);
-        if (intr)
-        {
-            // publishIntrinsics (BundleAdjustPlaMatrixModel.cpp:271-352) without transition priors
-            double pub[13];
-            std::copy_n(&in.slots[13 * c], 13, pub);
-            if (res && p.G > 0 && p.intrinsicBlock[c] >= 0)
-            {
-                const int g = p.groupOfCam[c];
-                const uint32_t en = p.gEnabled[g];
-                const double* gp = &ctx->outGroupParams[13 * g];
-                const double* ref = &in.ref[13 * c];
-                pub[0] = (en & 1u) ? gp[0] : in.slots[13 * c];
-                pub[1] = (en & 2u) ? gp[1] : in.slots[13 * c + 1];
-                pub[2] = (en & 4u) ? ref[2] + gp[2] : in.slots[13 * c + 2];
-                pub[3] = (en & 8u) ? ref[3] + gp[3] : in.slots[13 * c + 3];
-                for (int k = 4; k < 13; ++k)
-                    if ((en >> k) & 1u) pub[k] = gp[k];
-            }
-            std::copy_n(pub, 13, intr + 13 * c);
-        }
-    }
-    std::vector<int32_t> activeIndex(in.nTracks, -1);
-    if (ctx->structureValid)
-        for (size_t a = 0; a < ctx->lay.activeTrackOf.size(); ++a) activeIndex[ctx->lay.activeTrackOf[a]] = int32_t(a);
-    for (int t = 0; t < in.nTracks; ++t)
-    {
-        const int a = activeIndex[t];
-        if (X) std::copy_n((res && a >= 0) ? &ctx->outPointX[3 * a] : &in.X[3 * t], 3, X + 3 * t);
-        if (valid) valid[t] = (ctx->structureValid && p.activeTrack[t]) ? 1 : 0;
-    }
+    downloadRange(ctx, 0, ctx->in.nCam, 0, ctx->in.nTracks, R, C, intr, X, valid);
     return PB_OK;
+}
+
+pb_status pb_hip_batch_download(pb_ctx* ctx, int32_t window, double* R, double* C, double* intr, double* X,
+                                int32_t* valid)
+{
+    if (!ctx) return PB_ERR_INVALID;
+    if (!ctx->batchMode) return fail(ctx, PB_ERR_INVALID, "the context holds no batch");
+    if (window < 0 || window >= ctx->bh.nWin) return fail(ctx, PB_ERR_INVALID, "window index out of range");
+    const BatchHost& bh = ctx->bh;
+    downloadRange(ctx, bh.camBase[window], bh.camBase[window + 1], bh.trackBase[window], bh.trackBase[window + 1], R,
+                  C, intr, X, valid);
+    return PB_OK;
+}
+
+pb_status pb_hip_solve_batch(pb_ctx* ctx, const pb_batch_view* batch, const pb_solve_options* options,
+                             pb_window_report* reports, pb_solve_report* summary, const volatile int32_t* cancel)
+{
+    if (!ctx || !summary) return PB_ERR_INVALID;
+    std::memset(summary, 0, sizeof(*summary));
+    summary->gauge_anchor_camera = summary->gauge_scale_camera = -1;
+    auto invalid = [&](pb_status st, const std::string& m) {
+        summary->solve_status = st == PB_ERR_UNSUPPORTED ? PB_SOLVE_UNSUPPORTED : PB_SOLVE_INVALID_INPUT;
+        setMessage(summary, m);
+        return fail(ctx, st, m);
+    };
+    if (!batch || batch->num_windows <= 0 || !batch->windows) return invalid(PB_ERR_INVALID, "batch has no windows");
+    if (!reports) return invalid(PB_ERR_INVALID, "reports is NULL");
+    std::memset(reports, 0, sizeof(pb_window_report) * size_t(batch->num_windows));
+    const std::string oerr = validateOptions(options);
+    if (!oerr.empty()) return invalid(PB_ERR_INVALID, oerr);
+    if (!options->use_online_schur)
+        return invalid(PB_ERR_UNSUPPORTED, "the MI355X backend implements the online point-Schur path only");
+    ctx->profile = false;
+    const pb_status st = guarded(ctx, [&]() -> pb_status {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (cancel && *cancel)
+        {
+            summary->solve_status = PB_SOLVE_CANCELLED;
+            setMessage(summary, "cancelled before start");
+            return PB_ERR_CANCELLED;
+        }
+        const auto ts = Clock::now();
+        prepareBatch(ctx, batch, *options);
+        fillReportCounts(ctx, summary);
+        ctx->P.loss = options->robust_loss;
+        ctx->P.lossScale = options->robust_loss_scale;
+        if (ctx->prep.observations == 0)
+        {
+            summary->solve_status = PB_SOLVE_INVALID_INPUT;
+            setMessage(summary, "no usable tracks");
+            return PB_ERR_INVALID;
+        }
+        const bool mixed = options->precision == PB_PRECISION_MIXED;
+        if (mixed)
+            (void)work<float>(ctx);
+        else
+            (void)work<double>(ctx);
+        check(hipStreamSynchronize(ctx->stream), "setup");
+        summary->setup_seconds = seconds(ts);
+        summary->online_schur_used = 1;
+        summary->mixed_precision_used = mixed ? 1 : 0;
+        ctx->lastOptions = *options;
+        if (mixed)
+            runBatch<float>(ctx, *options, reports, summary, cancel);
+        else
+            runBatch<double>(ctx, *options, reports, summary, cancel);
+        return summary->solve_status == PB_SOLVE_CANCELLED ? PB_ERR_CANCELLED : PB_OK;
+    });
+    if (st != PB_OK && st != PB_ERR_CANCELLED && summary->solve_status == PB_SOLVE_NOT_RUN)
+    {
+        summary->solve_status = st == PB_ERR_UNSUPPORTED  ? PB_SOLVE_UNSUPPORTED
+                                : st == PB_ERR_INVALID    ? PB_SOLVE_INVALID_INPUT
+                                : st == PB_ERR_DEVICE     ? PB_SOLVE_BACKEND_UNAVAILABLE
+                                                          : PB_SOLVE_NUMERICAL_FAILURE;
+        setMessage(summary, ctx->error);
+    }
+    return st;
 }
 
 pb_status pb_hip_get_masks(pb_ctx* ctx, int32_t* cameraBlock, int32_t* intrinsicBlock, int32_t* groupOfCam,
@@ -2008,6 +2524,7 @@ pb_status pb_hip_reduced_system(pb_ctx* ctx, const pb_solve_options* options, do
                                 double* rhs, double* cost, int32_t* primary_blocks)
 {
     if (!ctx || !ctx->uploaded) return PB_ERR_INVALID;
+    if (ctx->batchMode) return fail(ctx, PB_ERR_INVALID, "the context holds a batch; upload a problem first");
     const std::string oerr = validateOptions(options);
     if (!oerr.empty()) return fail(ctx, PB_ERR_INVALID, oerr);
     return guarded(ctx, [&]() -> pb_status {

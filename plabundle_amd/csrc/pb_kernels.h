@@ -150,6 +150,93 @@ namespace pbk
         int cgBlocks = 0;
         const int4* cgSlot = nullptr;  // [cgBlocks*kCgWaves] {row (-1: empty), begin, end, lead}
         const int32_t* cgColDirect = nullptr;  // [cgBlocks*kCgWaves*kCgDirect] column of block begin+i, -1 past end
+
+        // Batched windows (pb_hip_solve_batch): independent problems solved together as one block-diagonal
+        // problem whose cameras, groups, tracks and units are numbered window by window. nWin = 0 for a
+        // single solve; then every pointer below is null and the scalar gauge/damping arguments apply.
+        int nWin = 0;
+        const int32_t* camWin = nullptr;          // [nCam]
+        const int32_t* groupWin = nullptr;        // [nGroup]
+        const int32_t* elimWin = nullptr;         // [nElim]
+        const int32_t* unitWin = nullptr;         // [nUnits]
+        const int32_t* camGaugeAnchor = nullptr;  // [nCam] anchor camera of a window's gauge scale camera, else -1
+        const double* winBaseline = nullptr;      // [nWin] gauge baseline of the window
+        const double* winDamping = nullptr;       // [nWin] LM damping of the current round
+        int32_t* winFlags = nullptr;              // [nWin*kFlagCount] build flags per window
+        const int32_t* winActive = nullptr;       // [nWin] 1: the window builds and solves this round
+        const double* winAlpha = nullptr;         // [nWin*kMaxAlpha] trial step lengths of the current pass
+        const int32_t* winNAlpha = nullptr;       // [nWin] trial step lengths of the window in the current pass
+    };
+
+    // Per-window LM controller state and window-local structure of a batched solve (device arrays).
+    enum WinStatus
+    {
+        kWinRunning = 0,
+        kWinConverged = 1,
+        kWinNoConvergence = 2,  // iteration budget or line search exhausted
+        kWinNumeric = 3         // invalid projection during a build
+    };
+
+    enum WinCounter
+    {
+        kWinIterations = 0,
+        kWinAccepted,
+        kWinRejected,
+        kWinLinearizations,
+        kWinEvaluations,
+        kWinCounterCount = 8
+    };
+
+    constexpr int kDensePanel = 16;  // column panel of the dense window factorisation
+    constexpr int kDenseRows = 2;    // matrix rows per thread of the factorisation workgroup
+    constexpr int kDenseThreads = 256;
+    constexpr int kMaxDense = kDenseRows * kDenseThreads;  // largest window system (6 * primary units)
+    constexpr int kTrialChunk = 1024;  // observations per workgroup of a batched trial-cost pass
+
+    struct Batch
+    {
+        int nWin = 0;
+        int nChunks = 0;
+        int maxDim = 0;  // largest window system dimension
+        // structure
+        const int32_t* camOffset = nullptr;        // [nWin+1]
+        const int32_t* groupOffset = nullptr;      // [nWin+1]
+        const int32_t* elimOffset = nullptr;       // [nWin+1]
+        const int32_t* poseUnitOffset = nullptr;   // [nWin+1] pose units [a, b) of window w
+        const int32_t* groupUnitOffset = nullptr;  // [nWin+1] intrinsic-group units
+        const int32_t* globalDim = nullptr;        // [nWin] active primary parameters (step RMS)
+        const int64_t* denseOffset = nullptr;      // [nWin+1] dense system storage (doubles)
+        const int32_t* chunkWin = nullptr;         // [nChunks]
+        const int32_t* chunkBegin = nullptr;       // [nChunks] observation range [chunkBegin[k], chunkEnd[k])
+        const int32_t* chunkEnd = nullptr;         // [nChunks]
+        const int32_t* winChunkOffset = nullptr;   // [nWin+1]
+        // controls
+        int maxIterations = 0, lineSearchSteps = 0;
+        double armijo = 0.0, rmsTol = 0.0;
+        // state
+        int32_t* status = nullptr;   // WinStatus
+        int32_t* active = nullptr;   // Problem::winActive
+        int32_t* reject = nullptr;   // step of this round rejected before the line search
+        double* damping = nullptr;   // Problem::winDamping
+        double* cost = nullptr;      // objective at the current state
+        double* initialCost = nullptr;
+        int32_t* counters = nullptr;  // [nWin*kWinCounterCount]
+        double* dd = nullptr;
+        double* rmsG = nullptr;
+        double* rmsP = nullptr;
+        int32_t* searching = nullptr;
+        int32_t* lsCount = nullptr;
+        double* alphaNext = nullptr;
+        double* alpha = nullptr;    // Problem::winAlpha
+        int32_t* nAlpha = nullptr;  // Problem::winNAlpha
+        int32_t* acceptIdx = nullptr;
+        double* acceptAlpha = nullptr;
+        double* acceptCost = nullptr;
+        double* dense = nullptr;
+        double* trialPart = nullptr;   // [nChunks*kMaxAlpha]
+        int32_t* trialBad = nullptr;
+        unsigned long long* doneClock = nullptr;  // [nWin+1]; the last entry is the start of the solve
+        int32_t* counts = nullptr;  // [2] windows running after the round, windows searching after a pass
     };
 
     template <typename T>
@@ -206,11 +293,29 @@ namespace pbk
         static void pcgIteration(const Problem& P, const Work<T>& W, int it, int maxIt, double relTol, double absTol,
                                  double modelEta, hipStream_t s);
         static void backSubstitute(const Problem& P, const Work<T>& W, hipStream_t s);
+        static void pointSteps(const Problem& P, const Work<T>& W, hipStream_t s);  // back-substitution only
+        static void trialStates(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, hipStream_t s);
         static void adjointRhs(const Problem& P, const Work<T>& W, const double* dlCenter, const double* dlPoint,
                                hipStream_t s);
         static void adjointObservations(const Problem& P, const Work<T>& W, double* out, hipStream_t s);
         static void trialCosts(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, int costBlocks,
                                hipStream_t s);
         static void accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s);
+    };
+
+    // Batched windows: one LM round of every running window per call sequence
+    //   roundBegin, build (shared kernels), afterBuild, factor, backSubstitute (shared), stepStats,
+    //   { trialPass } while windows search, control, apply.
+    template <typename T>
+    struct BatchLauncher
+    {
+        static void start(const Batch& B, hipStream_t s);
+        static void roundBegin(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
+        static void build(const Problem& P, const Work<T>& W, hipStream_t s);
+        static void afterBuild(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
+        static void factor(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
+        static void backSubstitute(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
+        static void trialPass(const Problem& P, const Work<T>& W, const Batch& B, int maxAlpha, hipStream_t s);
+        static void finishRound(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
     };
 }  // namespace pbk

@@ -15,6 +15,7 @@
 // order), so results are bitwise reproducible run to run. No float atomics are used.
 #include "pb_device.h"
 #include "pb_kernels.h"
+#include "pb_common.h"
 
 #include <hip/hip_runtime.h>
 
@@ -24,28 +25,12 @@ namespace pbk
     using pbdev::kMaxActive;
     using pbdev::Linearization;
 
-    constexpr int kWave = 64;
-
     // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on XCD b % kXcds). For a grid whose
     // size is a multiple of kXcds this gives XCD x the contiguous range [x * n / kXcds, (x + 1) * n / kXcds) of
     // work indices, so neighbouring work items, which share data, share an L2.
     __device__ __forceinline__ int xcdLocal(int b, int n)
     {
         return (b % kXcds) * (n / kXcds) + b / kXcds;
-    }
-
-    template <typename V>
-    __device__ __forceinline__ V waveSum(V v)
-    {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-        return v;
-    }
-
-    __device__ __forceinline__ void setFlag(int32_t* flags, int index)
-    {
-        // integer flag: any writer stores the same value, the order does not matter
-        atomicOr(flags + index, 1);
     }
 
     // ---------------------------------------------------------------------------------------------
@@ -80,7 +65,7 @@ namespace pbk
         out.metashape = meta ? 1 : 0;
         out.tangent = 0;
         for (int i = 0; i < 6; ++i) out.basis[i] = 0.0;
-        if (c == P.gaugeScale && anchorC)
+        if (anchorC)
         {
             // reference gauge tangent basis (Assembly.cpp:69-99) in target coordinates D = diag(1,-1,-1)
             const double o[3] = {anchorC[0], -anchorC[1], -anchorC[2]};
@@ -108,12 +93,43 @@ namespace pbk
         }
     }
 
+    // Anchor camera of the gauge when c is the gauge scale camera (of its window in a batch), else -1.
+    __device__ __forceinline__ int gaugeAnchorOf(const Problem& P, int c)
+    {
+        if (P.camGaugeAnchor) return P.camGaugeAnchor[c];
+        return (P.gaugeScale >= 0 && c == P.gaugeScale) ? P.gaugeAnchor : -1;
+    }
+
+    __device__ __forceinline__ double gaugeBaselineOf(const Problem& P, int c)
+    {
+        return P.nWin ? P.winBaseline[P.camWin[c]] : P.gaugeBaseline;
+    }
+
+    // LM damping and build-flag words of the window that owns item i (the scalar / global ones when unbatched)
+    __device__ __forceinline__ double dampingOf(const Problem& P, const int32_t* winOf, int i, double damping)
+    {
+        return P.nWin ? P.winDamping[winOf[i]] : damping;
+    }
+
+    template <typename T>
+    __device__ __forceinline__ int32_t* flagsOf(const Problem& P, const Work<T>& Wk, const int32_t* winOf, int i)
+    {
+        return P.nWin ? P.winFlags + kFlagCount * winOf[i] : Wk.flags;
+    }
+
+    // true when item i belongs to a window that does not build this round
+    __device__ __forceinline__ bool idleWindow(const Problem& P, const int32_t* winOf, int i)
+    {
+        return P.nWin && !P.winActive[winOf[i]];
+    }
+
     __global__ void k_prepare_cameras(Problem P)
     {
         const int c = blockIdx.x * blockDim.x + threadIdx.x;
         if (c >= P.nCam) return;
-        const double* anchorC = P.gaugeScale >= 0 ? P.camC + 3 * P.gaugeAnchor : nullptr;
-        const double* scaleC = P.gaugeScale >= 0 ? P.camC + 3 * P.gaugeScale : nullptr;
+        const int anchor = gaugeAnchorOf(P, c);
+        const double* anchorC = anchor >= 0 ? P.camC + 3 * anchor : nullptr;
+        const double* scaleC = anchor >= 0 ? P.camC + 3 * c : nullptr;
         makeCamera(P, c, P.camR + 9 * c, P.camC + 3 * c, P.groupParams, anchorC, scaleC, P.cams[c]);
     }
 
@@ -123,7 +139,9 @@ namespace pbk
     __global__ void __launch_bounds__(128) k_linearize_points(Problem P, Work<T> Wk, double damping)
     {
         const int b = blockIdx.x * blockDim.x + threadIdx.x;
-        if (b >= P.nElim) return;
+        if (b >= P.nElim || idleWindow(P, P.elimWin, b)) return;
+        damping = dampingOf(P, P.elimWin, b, damping);
+        int32_t* flags = flagsOf(P, Wk, P.elimWin, b);
         const int t = P.elimTrack[b];
         const double X0 = P.pointX[3 * t], X1 = P.pointX[3 * t + 1], X2 = P.pointX[3 * t + 2];
         const int e0 = P.entryOffset[b], e1 = P.entryOffset[b + 1];
@@ -141,7 +159,7 @@ namespace pbk
             if (!pbdev::linearize<KM>(P.cams[c], X0, X1, X2, P.obsUV[2 * o], P.obsUV[2 * o + 1], P.obsW[o], P.loss,
                                   P.lossScale, K, act, L))
             {
-                setFlag(Wk.flags, kFlagInvalid);
+                setFlag(flags, kFlagInvalid);
                 continue;
             }
             if (P.adjoint)
@@ -191,7 +209,7 @@ namespace pbk
         double inv[9];
         if (!pbdev::inverse3(Hd, inv))
         {
-            setFlag(Wk.flags, kFlagSingular);
+            setFlag(flags, kFlagSingular);
 #pragma unroll
             for (int i = 0; i < 9; ++i) inv[i] = 0.0;
         }
@@ -215,7 +233,7 @@ namespace pbk
             for (int i = 0; i < 6; ++i) ok = ok && pbdev::finite(Lf[i]);
             if (!ok)
             {
-                setFlag(Wk.flags, kFlagSingular);
+                setFlag(flags, kFlagSingular);
 #pragma unroll
                 for (int i = 0; i < 6; ++i) Lf[i] = 0.0;
             }
@@ -260,6 +278,7 @@ namespace pbk
     {
         using Acc = CamAcc<T, KM>;
         const int c = blockIdx.x;
+        if (idleWindow(P, P.camWin, c)) return;
         const int pu = P.camUnit[c];
         const int g = P.camGroup[c];
         const int K = g >= 0 ? P.groupK[g] : 0;
@@ -279,7 +298,7 @@ namespace pbk
             if (!pbdev::linearize<KM>(cam, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2], P.obsUV[2 * o],
                                   P.obsUV[2 * o + 1], P.obsW[o], P.loss, P.lossScale, KM > 0 ? K : 0, act, L))
             {
-                setFlag(Wk.flags, kFlagInvalid);
+                setFlag(flagsOf(P, Wk, P.camWin, c), kFlagInvalid);
                 continue;
             }
             if (P.adjoint)
@@ -397,7 +416,7 @@ namespace pbk
     {
         const int g = blockIdx.x;
         const int K = P.groupK[g];
-        if (K == 0) return;
+        if (K == 0 || idleWindow(P, P.groupWin, g)) return;
         constexpr int kSym = KM * (KM + 1) / 2;
         __shared__ double red[kSym + KM];
         const double* gp = P.groupParams + g * 13;
@@ -471,6 +490,7 @@ namespace pbk
         if (item >= P.nItems) return;
         const int lane = threadIdx.x & 63;
         const int blk = P.itemBlock[item];
+        if (P.nWin && idleWindow(P, P.unitWin, P.blockRow[blk])) return;
         const bool diag = P.blockRow[blk] == P.blockCol[blk];
         T acc[36];
 #pragma unroll
@@ -530,6 +550,8 @@ namespace pbk
         if (blk >= P.nBlocks) return;
         const int lane = threadIdx.x & 63;
         const int u = P.blockRow[blk], v = P.blockCol[blk];
+        if (idleWindow(P, P.unitWin, u)) return;
+        damping = dampingOf(P, P.unitWin, u, damping);
         const bool diag = u == v;
         if (lane < 36)
         {
@@ -985,7 +1007,7 @@ namespace pbk
         __shared__ double lds[8];
         const int b = blockIdx.x * blockDim.x + threadIdx.x;
         double dd = 0.0, ss = 0.0;
-        if (b < P.nElim)
+        if (b < P.nElim && !idleWindow(P, P.elimWin, b))
         {
             // dp = Hpp^-1 (bp - sum W^T dc) = L (z - sum U^T dc)
             double z[3] = {Wk.y[3 * b], Wk.y[3 * b + 1], Wk.y[3 * b + 2]};
@@ -1061,7 +1083,13 @@ namespace pbk
         const int id = blockIdx.x * blockDim.x + threadIdx.x;
         if (id >= nAlpha * P.nGroup) return;
         const int ai = id / P.nGroup, g = id % P.nGroup;
-        const double a = alphas[ai];
+        double a = alphas ? alphas[ai] : 0.0;
+        if (P.nWin)
+        {
+            const int w = P.groupWin[g];
+            if (ai >= P.winNAlpha[w]) return;
+            a = P.winAlpha[w * kMaxAlpha + ai];
+        }
         const uint32_t en = P.groupEnabled[g];
         double* out = Wk.candGroupParams + (ai * P.nGroup + g) * 13;
         double prior = 0.0;
@@ -1088,19 +1116,25 @@ namespace pbk
         const int id = blockIdx.x * blockDim.x + threadIdx.x;
         if (id >= nAlpha * P.nCam) return;
         const int ai = id / P.nCam, c = id % P.nCam;
-        const double a = alphas[ai];
+        double a = alphas ? alphas[ai] : 0.0;
+        if (P.nWin)
+        {
+            const int w = P.camWin[c];
+            if (ai >= P.winNAlpha[w]) return;
+            a = P.winAlpha[w * kMaxAlpha + ai];
+        }
         const int u = P.camUnit[c];
         double R[9], Cc[3];
         const double* R0 = P.camR + 9 * c;
         const double* C0 = P.camC + 3 * c;
+        const int anchor = gaugeAnchorOf(P, c);
         double anchorC[3], scaleC[3];
-        const bool gauge = P.gaugeScale >= 0;
-        if (gauge)
+        if (anchor >= 0)
         {
             for (int i = 0; i < 3; ++i)
             {
-                anchorC[i] = P.camC[3 * P.gaugeAnchor + i];
-                scaleC[i] = P.camC[3 * P.gaugeScale + i];
+                anchorC[i] = P.camC[3 * anchor + i];
+                scaleC[i] = C0[i];
             }
         }
         if (u >= 0)
@@ -1109,7 +1143,7 @@ namespace pbk
 #pragma unroll
             for (int i = 0; i < 6; ++i) d[i] = a * double(Wk.x[u * 6 + i]);
             pbdev::retractRotation(R0, d[0], d[1], d[2], R);
-            if (c == P.gaugeScale)
+            if (anchor >= 0)
             {
                 // centre moves along the tangent basis, then back onto the baseline sphere
                 // (Assembly.cpp:101-128)
@@ -1130,7 +1164,7 @@ namespace pbk
                 }
                 else
                 {
-                    const double s = P.gaugeBaseline / len;
+                    const double s = gaugeBaselineOf(P, c) / len;
                     const double t[3] = {o[0] + dir[0] * s, o[1] + dir[1] * s, o[2] + dir[2] * s};
                     Cc[0] = t[0];
                     Cc[1] = -t[1];
@@ -1168,42 +1202,11 @@ namespace pbk
             acc[a] = 0.0;
             bad[a] = 0;
         }
-        for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < P.nObs; o += gridDim.x * blockDim.x)
-        {
-            const int t = P.obsPoint[o];
-            const int c = P.obsCam[o];
-            const int b = P.pointElim[t];
-            const double X0 = P.pointX[3 * t], X1 = P.pointX[3 * t + 1], X2 = P.pointX[3 * t + 2];
-            double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-            if (b >= 0)
-            {
-                const double s = pbdev::pointScale(X0, X1, X2);
-                d0 = s * Wk.dp[3 * b];
-                d1 = s * Wk.dp[3 * b + 1];
-                d2 = s * Wk.dp[3 * b + 2];
-            }
-            const double ou = P.obsUV[2 * o], ov = P.obsUV[2 * o + 1], ow = P.obsW[o];
+        double al[NA];
 #pragma unroll
-            for (int a = 0; a < NA; ++a)
-            {
-                if (a >= nAlpha) break;
-                const double al = alphas[a];
-                double u, v;
-                if (!pbdev::project(Wk.candCams[a * P.nCam + c], X0 + al * d0, X1 + al * d1, X2 + al * d2, u, v))
-                {
-                    bad[a] = 1;
-                    continue;
-                }
-                const double ru = u - ou, rv = v - ov;
-                double cost, w;
-                if (!pbdev::robust(ow * (ru * ru + rv * rv), P.loss, P.lossScale, cost, w))
-                {
-                    bad[a] = 1;
-                    continue;
-                }
-                acc[a] += cost;
-            }
-        }
+        for (int a = 0; a < NA; ++a) al[a] = a < nAlpha ? alphas[a] : 0.0;
+        for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < P.nObs; o += gridDim.x * blockDim.x)
+            trialObservation<NA>(P, Wk.candCams, Wk.dp, o, al, nAlpha, acc, bad);
         __shared__ double lds[NA][4];
         __shared__ int lbad[NA][4];
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1460,6 +1463,19 @@ namespace pbk
     }
 
     template <typename T>
+    void Launcher<T>::pointSteps(const Problem& P, const Work<T>& W, hipStream_t s)
+    {
+        if (P.nElim > 0) k_back_substitute<T><<<blocksFor(P.nElim, 256), 256, 0, s>>>(P, W);
+    }
+
+    template <typename T>
+    void Launcher<T>::trialStates(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, hipStream_t s)
+    {
+        if (P.nGroup > 0) k_trial_groups<T><<<blocksFor(long(nAlpha) * P.nGroup, 64), 64, 0, s>>>(P, W, alphas, nAlpha);
+        k_trial_cameras<T><<<blocksFor(long(nAlpha) * P.nCam, 64), 64, 0, s>>>(P, W, alphas, nAlpha);
+    }
+
+    template <typename T>
     void Launcher<T>::adjointRhs(const Problem& P, const Work<T>& W, const double* dlCenter, const double* dlPoint,
                                  hipStream_t s)
     {
@@ -1484,8 +1500,7 @@ namespace pbk
     void Launcher<T>::trialCosts(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha,
                                  int costBlocks, hipStream_t s)
     {
-        if (P.nGroup > 0) k_trial_groups<T><<<blocksFor(long(nAlpha) * P.nGroup, 64), 64, 0, s>>>(P, W, alphas, nAlpha);
-        k_trial_cameras<T><<<blocksFor(long(nAlpha) * P.nCam, 64), 64, 0, s>>>(P, W, alphas, nAlpha);
+        trialStates(P, W, alphas, nAlpha, s);
         k_trial_cost<T, kMaxAlpha><<<costBlocks, 256, 0, s>>>(P, W, alphas, nAlpha);
         k_trial_reduce<T, kMaxAlpha><<<1, 64 * kMaxAlpha, 0, s>>>(P, W, nAlpha, costBlocks);
     }

@@ -219,3 +219,60 @@ class Solver:
         result.cam_rotation, result.cam_center, result.cam_intrinsics = R, Cc, I
         result.track_point, result.track_valid = X, V
         return result
+
+    def solve_batch(self, problems: list[Problem], options: SolveOptions | None = None,
+                    workspace: SolverWorkspace | None = None, precision: Precision | None = None,
+                    cancel=None) -> tuple[list[Result], dict]:
+        """Solves independent windows in one call (pb_hip_solve_batch).
+
+        Each window is solved as solve() would solve it alone with the same options, except that every LM step
+        solves the window's reduced camera system exactly (dense Cholesky) instead of by PCG. Returns one
+        Result per window (report = the window's pb_window_report) and the batch summary (pb_solve_report).
+        The workspace then holds the batch; the next solve() uploads its problem again."""
+        options = options or SolveOptions()
+        lib = _abi.load_library()  # raises when the HIP library is missing: no silent CPU fallback
+        results = [Result(requestedBackend=options.backend.requested) for _ in problems]
+        if not problems:
+            return results, {"solve_status": int(SolveStatus.InvalidInput), "message": "batch has no windows"}
+        if not lib.pb_hip_available(int(options.backend.plaMatrixDevice)):
+            for r in results:
+                r.status = SolveStatus.BackendUnavailable
+                r.backendMessage = f"no gfx950 device at index {options.backend.plaMatrixDevice}"
+            return results, {"solve_status": int(SolveStatus.BackendUnavailable)}
+        ws = workspace or SolverWorkspace(options.backend.plaMatrixDevice)
+        ws.clear()
+        views = (_abi.ProblemView * len(problems))()
+        keep = []
+        for i, p in enumerate(problems):
+            v, k = _abi.make_view(p)
+            views[i] = v
+            keep.append(k)
+        batch = _abi.BatchView(len(problems), C.cast(views, C.POINTER(_abi.ProblemView)))
+        copts = to_c(options, precision)
+        reports = (_abi.WindowReportC * len(problems))()
+        summary = _abi.SolveReportC()
+        flag = None if cancel is None else np.array([1 if cancel else 0], dtype=np.int32)
+        st = lib.pb_hip_solve_batch(ws.handle, C.byref(batch), C.byref(copts), reports, C.byref(summary),
+                                    _abi.ptr(flag, C.c_int32))
+        del keep
+        out = summary.as_dict()
+        out["abi_status"] = st
+        for w, (p, r) in enumerate(zip(problems, results)):
+            rep = reports[w]
+            r.abi_status = st
+            r.report = rep.as_dict()
+            r.status = SolveStatus(rep.solve_status) if st == _abi.PB_OK else SolveStatus(summary.solve_status)
+            r.solutionUsable = st == _abi.PB_OK and bool(rep.solution_usable)
+            r.usedGpu = st == _abi.PB_OK
+            r.backendMessage = out["message"] if st == _abi.PB_OK else ws.last_error()
+            r.observationCount = rep.observation_count
+            if st != _abi.PB_OK:
+                continue
+            nc, nt = p.num_cameras, p.num_tracks
+            R, Cc, I = np.zeros((nc, 9)), np.zeros((nc, 3)), np.zeros((nc, 13))
+            X, V = np.zeros((nt, 3)), np.zeros(nt, dtype=np.int32)
+            lib.pb_hip_batch_download(ws.handle, w, _abi.ptr(R, C.c_double), _abi.ptr(Cc, C.c_double),
+                                      _abi.ptr(I, C.c_double), _abi.ptr(X, C.c_double), _abi.ptr(V, C.c_int32))
+            r.cam_rotation, r.cam_center, r.cam_intrinsics = R, Cc, I
+            r.track_point, r.track_valid = X, V
+        return results, out

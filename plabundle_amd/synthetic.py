@@ -323,3 +323,92 @@ def ring_options(options=None, max_iterations: int = 10):
     o.solver.maxIterations = max_iterations
     o.solver.useReferenceOnlineSchur = True
     return o
+
+
+# --- batched local windows (configuration 4) ----------------------------------------------------
+def window_problem(rng: np.random.Generator, num_cameras: int, num_points: int, track_range=(2, 12),
+                   noise_px: float = 0.5, rot_sigma: float = 0.01, center_sigma: float = 0.05,
+                   point_sigma: float = 0.1, arc_step_deg: float = 3.0) -> Problem:
+    """One local window: cameras on an arc of a 10 m circle (arc_step_deg apart, heights ~ N(0, 0.5 m)) looking
+    at a 4 m cube of points, 4000x3000 px, f = 3000 px, one shared Brown group (k1, k2, p1, p2 ~ N(0, 1e-2)).
+    Track length L ~ U{track_range} capped by the visible cameras, which are drawn at random; 0.5 px noise;
+    cameras 0 and 1 fixed (explicit gauge), the others and the points perturbed as the ring workload."""
+    W, H, f = 4000, 3000, 3000.0
+    step = np.deg2rad(arc_step_deg)
+    ang = rng.uniform(0.0, 2 * np.pi) + step * (np.arange(num_cameras) - 0.5 * (num_cameras - 1))
+    cen = np.stack([10 * np.cos(ang), 10 * np.sin(ang), rng.normal(0.0, 0.5, num_cameras)], axis=1)
+    rot = np.stack([look_at_rotation(c, (0, 0, 0)) for c in cen])
+    dist = rng.normal(0.0, 1e-2, size=4)
+    intr1 = intrinsic_block(f, f, W / 2, H / 2, k1=dist[0], k2=dist[1], p1=dist[2], p2=dist[3])
+    intr = np.repeat(intr1[None], num_cameras, axis=0)
+    truth = rng.uniform(-2.0, 2.0, size=(num_points, 3))
+    # every point in every camera: (points x cameras)
+    pc = np.repeat(np.arange(num_cameras)[None, :], num_points, axis=0).reshape(-1)
+    uv, z = project_points(rot[pc], cen[pc], intr[pc], np.repeat(truth, num_cameras, axis=0))
+    vis = ((z > 0) & (uv[:, 0] >= 0) & (uv[:, 0] < W) & (uv[:, 1] >= 0) & (uv[:, 1] < H)).reshape(num_points, -1)
+    uv = uv.reshape(num_points, num_cameras, 2)
+    L = np.minimum(rng.integers(track_range[0], track_range[1] + 1, size=num_points), vis.sum(axis=1))
+    keys = np.where(vis, rng.random(size=vis.shape), 2.0)
+    order = np.argsort(keys, axis=1)
+    noise = rng.normal(0.0, noise_px, size=uv.shape)
+    keep = np.nonzero(L >= 2)[0]
+    # the first L visible cameras of each kept point in the random order, then in camera order
+    m = min(int(track_range[1]), num_cameras)
+    first = order[keep, :m]
+    chosen = np.arange(m)[None, :] < L[keep][:, None]
+    picked = np.sort(np.where(chosen, first, num_cameras), axis=1)
+    valid = picked < num_cameras
+    rows = np.repeat(keep, valid.sum(axis=1))
+    cam = picked[valid].astype(np.int32)
+    obs_uv = uv[rows, cam] + noise[rows, cam]
+    off = np.zeros(len(keep) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(valid.sum(axis=1))
+    truth = truth[keep]
+    dR = exp_so3(rng.normal(0.0, rot_sigma, size=(num_cameras, 3)))
+    dC = rng.normal(0.0, center_sigma, size=(num_cameras, 3))
+    dR[:2] = np.eye(3)
+    dC[:2] = 0.0
+    pts0 = truth + rng.normal(0.0, point_sigma, size=truth.shape)
+    return Problem(
+        cam_rotation=np.ascontiguousarray((dR @ rot).reshape(-1, 9)),
+        cam_center=np.ascontiguousarray(cen + dC),
+        cam_intrinsics=np.ascontiguousarray(intr),
+        track_point=np.ascontiguousarray(pts0),
+        track_offset=off,
+        obs_camera=cam,
+        obs_uv=np.ascontiguousarray(obs_uv),
+        obs_weight=np.ones(len(cam)),
+        obs_scale=np.ones(len(cam)),
+        cam_image_size=np.tile(np.array([W, H], dtype=np.int32), (num_cameras, 1)),
+        cam_group=np.zeros(num_cameras, dtype=np.int32),
+        fixed_cameras=np.array([0, 1], dtype=np.int32),
+        gauge_policy=GaugePolicy.RequireExplicitGauge,
+        meta={"workload": "window", "truth_rotation": rot, "truth_center": cen, "truth_points": truth})
+
+
+def batch_windows(seed: int = 4, num_windows: int = 512, cameras=(8, 64), points=(500, 8000),
+                  track_range=(2, 12), noise_px: float = 0.5) -> list[Problem]:
+    """Batched local BA: num_windows independent windows with cameras ~ U{cameras}, points ~ U{points} and
+    track lengths ~ U{track_range} (seed 4; 512 windows, ~15M observations at full size)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(num_windows):
+        nc = int(rng.integers(cameras[0], cameras[1] + 1))
+        npts = int(rng.integers(points[0], points[1] + 1))
+        out.append(window_problem(rng, nc, npts, track_range=track_range, noise_px=noise_px))
+    return out
+
+
+def window_options(options=None, max_iterations: int = 10):
+    """Solve options of the batched windows: Huber 1 px, shared focal + Brown (k1, k2, k3, p1, p2) refined."""
+    from .options import ImageRobustLoss, SolveOptions
+
+    o = options or SolveOptions()
+    o.solver.imageRobustLoss = ImageRobustLoss.Huber
+    o.solver.imageRobustLossScalePixels = 1.0
+    o.solver.maxIterations = max_iterations
+    o.solver.useReferenceOnlineSchur = True
+    o.calibration.refineSharedFocalLength = True
+    o.calibration.refineSharedRadialDistortion = True
+    o.calibration.refineSharedHighOrderDistortion = True
+    return o

@@ -45,6 +45,8 @@ int main(void) {
   printf("%zu %zu %zu\n", sizeof(pb_problem_view), sizeof(pb_solve_options), sizeof(pb_solve_report));
   printf("%zu %zu %zu\n", offsetof(pb_problem_view, gauge_baseline), offsetof(pb_solve_options, step_rms_tolerance),
          offsetof(pb_solve_report, message));
+  printf("%zu %zu %zu %zu\n", sizeof(pb_batch_view), offsetof(pb_batch_view, windows), sizeof(pb_window_report),
+         offsetof(pb_window_report, device_seconds));
   return 0;
 }
 """
@@ -59,6 +61,9 @@ int main(void) {
     assert sizes == [C.sizeof(_abi.ProblemView), C.sizeof(_abi.SolveOptionsC), C.sizeof(_abi.SolveReportC)]
     assert offs == [_abi.ProblemView.gauge_baseline.offset, _abi.SolveOptionsC.step_rms_tolerance.offset,
                     _abi.SolveReportC.message.offset]
+    batch = [int(x) for x in lines[2].split()]
+    assert batch == [C.sizeof(_abi.BatchView), _abi.BatchView.windows.offset, C.sizeof(_abi.WindowReportC),
+                     _abi.WindowReportC.device_seconds.offset]
 
 
 def test_default_options_are_the_reference_defaults(lib):
@@ -77,6 +82,8 @@ def test_default_options_are_the_reference_defaults(lib):
 def test_invalid_calls_fail_cleanly_without_a_device(lib):
     # no context -> invalid; a missing device is reported, never a crash
     assert lib.pb_hip_upload(None, None) == _abi.PB_ERR_INVALID
+    assert lib.pb_hip_solve_batch(None, None, None, None, None, None) == _abi.PB_ERR_INVALID
+    assert lib.pb_hip_batch_download(None, 0, None, None, None, None, None) == _abi.PB_ERR_INVALID
     assert lib.pb_hip_destroy(None) == _abi.PB_OK
     ctx = C.c_void_p()
     if not lib.pb_hip_available(0):

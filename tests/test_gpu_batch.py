@@ -1,0 +1,158 @@
+"""GPU parity of the batched-window entry (pb_hip_solve_batch) against the FP64 oracle, through the C ABI.
+
+Each window of a batch must come out as the oracle solves that window alone (same options): the oracle's
+reduced solve is a dense Cholesky, as the batched path's, so the FP64 bar is the single-problem one (final
+cost within 1e-9 relative, same iteration and accepted-step counts, refined state within 1e-7 m / 1e-6 m);
+the mixed bar is 1e-6 relative final cost.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from plabundle_amd import _abi
+from plabundle_amd import options as PO
+from plabundle_amd import synthetic as S
+from plabundle_amd.problem import GaugePolicy
+from plabundle_amd.solver import Solver, SolverWorkspace, SolveStatus
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ws():
+    lib = _abi.load_library()
+    assert lib.pb_hip_available(0), "no gfx950 device"
+    w = SolverWorkspace(0)
+    yield w
+    w.close()
+
+
+def _windows(n=6, seed=11):
+    return S.batch_windows(seed=seed, num_windows=n, cameras=(8, 20), points=(150, 500))
+
+
+@pytest.mark.parametrize("intrinsics", [True, False])
+def test_batch_fp64_matches_oracle_per_window(ws, oracle_mod, intrinsics):
+    _say(f"case batch fp64 intrinsics={intrinsics}")
+    wins = _windows()
+    o = S.window_options(PO.SolveOptions(), max_iterations=10)
+    if not intrinsics:
+        o.calibration.refineSharedFocalLength = False
+        o.calibration.refineSharedRadialDistortion = False
+    res, summary = Solver().solve_batch(wins, o, ws, precision=PO.Precision.Fp64)
+    assert summary["abi_status"] == _abi.PB_OK, summary["message"]
+    co = PO.to_c(o, PO.Precision.Fp64)
+    for w, (p, r) in enumerate(zip(wins, res)):
+        orc = oracle_mod.Oracle(p, threads=2)
+        ref = orc.solve(co)
+        st = orc.state()
+        rel = abs(r.finalCost - ref["final_cost"]) / ref["final_cost"]
+        _say(f"window {w}: {p.num_cameras} cameras {p.num_observations} obs, {r.report['iterations']} it "
+             f"(oracle {ref['iterations']}), rel cost {rel:.2e}, {r.report['device_seconds'] * 1e3:.2f} ms")
+        assert r.status == SolveStatus(ref["solve_status"])
+        assert r.report["iterations"] == ref["iterations"]
+        assert r.report["accepted_steps"] == ref["accepted_steps"]
+        assert abs(r.report["initial_cost"] - ref["initial_cost"]) <= 1e-12 * ref["initial_cost"]
+        assert rel <= 1e-9
+        np.testing.assert_allclose(r.cam_center, st["cam_center"], atol=1e-7)
+        np.testing.assert_allclose(r.track_point, st["track_point"], atol=1e-6)
+        np.testing.assert_allclose(r.cam_intrinsics, st["cam_intrinsics"], rtol=1e-7, atol=1e-9)
+        np.testing.assert_array_equal(r.track_valid, st["track_valid"])
+
+
+def test_batch_mixed_within_north_star_tolerance(ws, oracle_mod):
+    # The mixed windows assemble S in FP32, so a window's step-RMS stop (1e-6) can fall one iteration earlier
+    # than the oracle's; the last FP64 step of such a window still changes the cost by ~1e-6 relative. Parity
+    # is then checked against the oracle run with the window's own iteration count (1e-6), and the full-run
+    # gap is bounded at 1e-5.
+    _say("case batch mixed")
+    wins = _windows(seed=12)
+    o = S.window_options(PO.SolveOptions(), max_iterations=10)
+    res, summary = Solver().solve_batch(wins, o, ws, precision=PO.Precision.Mixed)
+    assert summary["abi_status"] == _abi.PB_OK, summary["message"]
+    co = PO.to_c(o)
+    for w, (p, r) in enumerate(zip(wins, res)):
+        ref = oracle_mod.Oracle(p, threads=2).solve(co)
+        rel = abs(r.finalCost - ref["final_cost"]) / ref["final_cost"]
+        its = r.report["iterations"]
+        _say(f"window {w}: mixed rel cost {rel:.2e}, {its} it (oracle {ref['iterations']})")
+        assert r.usable()
+        if its < ref["iterations"]:
+            co_k = PO.to_c(S.window_options(PO.SolveOptions(), max_iterations=its))
+            ref_k = oracle_mod.Oracle(p, threads=2).solve(co_k)
+            rel_k = abs(r.finalCost - ref_k["final_cost"]) / ref_k["final_cost"]
+            _say(f"window {w}: oracle after {its} iterations: rel cost {rel_k:.2e}")
+            assert rel_k <= 1e-6
+            assert rel <= 1e-5
+        else:
+            assert rel <= 1e-6
+
+
+def test_batch_auto_anchor_gauge_matches_single_solves(ws):
+    # AutoAnchor: each window fixes its camera 0 and moves its farthest camera on the baseline sphere (the
+    # per-window gauge scale camera); the batch must match the single-problem path window by window
+    _say("case batch auto-anchor vs single")
+    wins = []
+    for p in _windows(n=4, seed=13):
+        q = p.copy()
+        q.fixed_cameras = None
+        q.gauge_policy = GaugePolicy.AutoAnchor
+        wins.append(q)
+    o = S.window_options(PO.SolveOptions(), max_iterations=8)
+    res, summary = Solver().solve_batch(wins, o, ws, precision=PO.Precision.Fp64)
+    assert summary["abi_status"] == _abi.PB_OK, summary["message"]
+    for w, (p, r) in enumerate(zip(wins, res)):
+        single = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
+        rel = abs(r.finalCost - single.finalCost) / single.finalCost
+        _say(f"window {w}: batch {r.finalCost:.10g} single {single.finalCost:.10g} rel {rel:.2e}")
+        assert single.report["gauge_scale_camera"] >= 0
+        assert r.report["iterations"] == single.report["iterations"]
+        assert rel <= 1e-8
+        np.testing.assert_allclose(r.cam_center, single.cam_center, atol=1e-6)
+
+
+def test_batch_window_without_usable_tracks(ws, oracle_mod):
+    _say("case batch empty window")
+    wins = _windows(n=3, seed=14)
+    bad = wins[1].copy()
+    bad.obs_uv = np.full_like(bad.obs_uv, np.nan)
+    wins[1] = bad
+    o = S.window_options(PO.SolveOptions(), max_iterations=10)
+    res, summary = Solver().solve_batch(wins, o, ws, precision=PO.Precision.Fp64)
+    assert summary["abi_status"] == _abi.PB_OK, summary["message"]
+    assert res[1].status == SolveStatus.InvalidInput and not res[1].usable()
+    co = PO.to_c(o, PO.Precision.Fp64)
+    for w in (0, 2):
+        ref = oracle_mod.Oracle(wins[w], threads=2).solve(co)
+        assert abs(res[w].finalCost - ref["final_cost"]) <= 1e-9 * ref["final_cost"]
+
+
+def test_batch_rejects_oversized_and_invalid_windows(ws):
+    _say("case batch rejects")
+    wins = _windows(n=2, seed=15)
+    big = S.window_problem(np.random.Generator(np.random.PCG64(3)), 90, 300)
+    o = S.window_options(PO.SolveOptions(), max_iterations=5)
+    res, summary = Solver().solve_batch([wins[0], big], o, ws, precision=PO.Precision.Fp64)
+    assert summary["abi_status"] == _abi.PB_ERR_UNSUPPORTED
+    assert "window 1" in res[0].backendMessage
+    bad = wins[1].copy()
+    bad.track_offset = bad.track_offset.copy()
+    bad.track_offset[0] = 1
+    res, summary = Solver().solve_batch([wins[0], bad], o, ws, precision=PO.Precision.Fp64)
+    assert summary["abi_status"] == _abi.PB_ERR_INVALID
+    assert "window 1" in res[0].backendMessage
+
+
+def test_batch_cancel_publishes_nothing(ws):
+    _say("case batch cancel")
+    wins = _windows(n=2, seed=16)
+    o = S.window_options(PO.SolveOptions(), max_iterations=5)
+    res, summary = Solver().solve_batch(wins, o, ws, precision=PO.Precision.Fp64, cancel=True)
+    assert summary["solve_status"] == int(SolveStatus.Cancelled)
+    assert all(not r.usable() for r in res)
