@@ -291,6 +291,7 @@ typedef struct pb_window_report {
     int32_t rejected_steps;
     int32_t linearizations;
     int32_t objective_evaluations;
+    int32_t fp64_iterations;       /* iterations assembled in FP64 (all of them in an FP64 batch) */
     int32_t observation_count;
     int32_t active_track_count;
     int32_t camera_block_count;
@@ -303,7 +304,8 @@ typedef struct pb_window_report {
 /* Solves every window of the batch. The windows share each device launch (one block-diagonal problem) and
  * keep separate LM controllers on the device (damping, line search, acceptance and stopping per window);
  * a window stops when it converges, exhausts its line search or reaches max_iterations, and later launches
- * skip it. reports [num_windows] receives each window's outcome; summary receives the batch totals
+ * skip it. PB_PRECISION_MIXED assembles a window's system in FP32 while its accepted steps still lower the
+ * objective by at least 1e-2 (relative) and in FP64 afterwards; a window's stop is only taken in FP64. reports [num_windows] receives each window's outcome; summary receives the batch totals
  * (iterations = controller rounds, costs = sums over the windows, solve_seconds = host wall time of the
  * rounds, setup_seconds = host preparation and upload). A window whose input is invalid fails the whole call
  * with PB_ERR_INVALID / PB_ERR_UNSUPPORTED and a message naming the window. cancel is polled between rounds;

@@ -163,6 +163,7 @@ class WindowReportC(C.Structure):
         ("rejected_steps", C.c_int32),
         ("linearizations", C.c_int32),
         ("objective_evaluations", C.c_int32),
+        ("fp64_iterations", C.c_int32),
         ("observation_count", C.c_int32),
         ("active_track_count", C.c_int32),
         ("camera_block_count", C.c_int32),

@@ -14,8 +14,10 @@
 //   (shared back-subst.)  point steps
 //   k_batch_step_stats    directional derivative and step RMS values per window
 //   trial passes          k_batch_alphas, trial states (shared), k_batch_trial_cost, k_batch_decide
-//   k_batch_control       accept / reject, damping update, convergence and budget tests
+//   k_batch_control       accept / reject, damping update, convergence and budget tests, precision phase
 //   k_batch_apply         commits the accepted trial state of each window
+// A mixed batch runs each round twice: the kernels instantiated for FP32 assembly on the windows in the FP32
+// phase, then the FP64 kernels on the windows in the FP64 phase (every window ends in FP64).
 #include "pb_common.h"
 #include "pb_device.h"
 #include "pb_kernels.h"
@@ -38,12 +40,11 @@ namespace pbk
         if (threadIdx.x == 0 && blockIdx.x == 0) B.doneClock[B.nWin] = deviceClock();
     }
 
-    __global__ void k_batch_round_begin(Problem P, Batch B)
+    __global__ void k_batch_round_begin(Problem P, Batch B, int phase)
     {
         const int w = blockIdx.x * blockDim.x + threadIdx.x;
-        if (w == 0) B.counts[0] = 0;
         if (w >= B.nWin) return;
-        B.active[w] = B.status[w] == kWinRunning ? 1 : 0;
+        B.active[w] = (B.status[w] == kWinRunning && B.phase[w] == phase) ? 1 : 0;
         B.reject[w] = 0;
         B.searching[w] = 0;
         B.nAlpha[w] = 0;
@@ -425,7 +426,7 @@ namespace pbk
     __global__ void k_batch_alphas(Batch B, int maxAlpha)
     {
         const int w = blockIdx.x * blockDim.x + threadIdx.x;
-        if (w == 0) B.counts[1] = 0;
+        if (w == 0) B.counts[2] = 0;
         if (w >= B.nWin) return;
         if (!B.searching[w])
         {
@@ -551,41 +552,82 @@ namespace pbk
             return;
         }
         B.alphaNext[w] = B.alpha[w * kMaxAlpha + nA - 1] * 0.5;
-        atomicAdd(B.counts + 1, 1);
+        atomicAdd(B.counts + 2, 1);
     }
 
-    // Controller step of each running window: counts, damping, acceptance, convergence and budget.
-    __global__ void k_batch_control(Batch B)
+    // Controller step of each window of this phase: counts, damping, acceptance, convergence and budget. In the
+    // FP32 phase of a mixed batch the stop and the failure exits move the window to FP64 instead.
+    __global__ void k_batch_control(Batch B, int phase)
     {
         const int w = blockIdx.x * blockDim.x + threadIdx.x;
         if (w >= B.nWin || !B.active[w]) return;
         int32_t* cnt = B.counters + w * kWinCounterCount;
         ++cnt[kWinIterations];
+        if (phase == 1) ++cnt[kWinFp64Iterations];
         double& lambda = B.damping[w];
         if (B.reject[w])
         {
             ++cnt[kWinRejected];
             lambda = lambda == 0.0 ? 1e-3 : lambda * 10.0;
+            if (phase == 0) B.phase[w] = 1;
         }
         else if (B.acceptIdx[w] >= 0)
         {
             ++cnt[kWinAccepted];
+            const double before = B.cost[w];
             B.cost[w] = B.acceptCost[w];
             lambda /= 10.0;
             const double a = B.acceptAlpha[w];
-            if (a * B.rmsG[w] <= B.rmsTol && a * B.rmsP[w] <= B.rmsTol) B.status[w] = kWinConverged;
+            const bool stop = a * B.rmsG[w] <= B.rmsTol && a * B.rmsP[w] <= B.rmsTol;
+            if (phase == 0)
+            {
+                if (stop || before - B.acceptCost[w] < B.switchDecrease * before) B.phase[w] = 1;
+            }
+            else if (stop)
+            {
+                B.status[w] = kWinConverged;
+            }
         }
         else
         {
             // line search exhausted: the controller stops (BundleAdjustPlaMatrix.cpp:532-536)
             ++cnt[kWinRejected];
-            B.status[w] = kWinNoConvergence;
+            if (phase == 0)
+                B.phase[w] = 1;
+            else
+                B.status[w] = kWinNoConvergence;
         }
         if (B.status[w] == kWinRunning && cnt[kWinIterations] >= B.maxIterations) B.status[w] = kWinNoConvergence;
-        if (B.status[w] != kWinRunning)
-            B.doneClock[w] = deviceClock();
-        else
-            atomicAdd(B.counts, 1);
+        if (B.status[w] != kWinRunning) B.doneClock[w] = deviceClock();
+    }
+
+    // Running windows per phase (one workgroup, fixed reduction order).
+    __global__ void k_batch_count(Batch B)
+    {
+        __shared__ int red[2][4];
+        int c0 = 0, c1 = 0;
+        for (int w = threadIdx.x; w < B.nWin; w += blockDim.x)
+            if (B.status[w] == kWinRunning) (B.phase[w] == 0 ? c0 : c1) += 1;
+        c0 = waveSum(c0);
+        c1 = waveSum(c1);
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane == 0)
+        {
+            red[0][wv] = c0;
+            red[1][wv] = c1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            int s0 = 0, s1 = 0;
+            for (int q = 0; q < int(blockDim.x / kWave); ++q)
+            {
+                s0 += red[0][q];
+                s1 += red[1][q];
+            }
+            B.counts[0] = s0;
+            B.counts[1] = s1;
+        }
     }
 
     // Commits the accepted candidate of each window: cameras, group parameters and points.
@@ -628,9 +670,15 @@ namespace pbk
     }
 
     template <typename T>
-    void BatchLauncher<T>::roundBegin(const Problem& P, const Work<T>&, const Batch& B, hipStream_t s)
+    void BatchLauncher<T>::count(const Batch& B, hipStream_t s)
     {
-        k_batch_round_begin<<<gridFor(B.nWin, 256), 256, 0, s>>>(P, B);
+        k_batch_count<<<1, 256, 0, s>>>(B);
+    }
+
+    template <typename T>
+    void BatchLauncher<T>::roundBegin(const Problem& P, const Work<T>&, const Batch& B, int phase, hipStream_t s)
+    {
+        k_batch_round_begin<<<gridFor(B.nWin, 256), 256, 0, s>>>(P, B, phase);
     }
 
     template <typename T>
@@ -681,9 +729,9 @@ namespace pbk
     }
 
     template <typename T>
-    void BatchLauncher<T>::finishRound(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s)
+    void BatchLauncher<T>::finishRound(const Problem& P, const Work<T>& W, const Batch& B, int phase, hipStream_t s)
     {
-        k_batch_control<<<gridFor(B.nWin, 256), 256, 0, s>>>(B);
+        k_batch_control<<<gridFor(B.nWin, 256), 256, 0, s>>>(B, phase);
         const long n = std::max<long>(std::max<long>(P.nCam, long(P.nGroup) * 13), P.nElim);
         if (n > 0) k_batch_apply<T><<<gridFor(n, 256), 256, 0, s>>>(P, W, B);
     }

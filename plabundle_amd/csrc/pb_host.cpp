@@ -13,6 +13,8 @@
 #include "pb_kernels.h"
 
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <array>
@@ -792,6 +794,11 @@ namespace
     // per chunk, so the events do not stretch the solve); chunks that contain the stop are not counted, so
     // every timed launch did a full iteration
     constexpr int kFamilyCount = sizeof(kFamilies) / sizeof(kFamilies[0]);
+    constexpr long kDenseAdjointMax = 24576;  // rows of the dense adjoint factorisation (4.8 GB of FP64)
+    // a mixed batch moves a window to FP64 assembly once an FP32 step lowers its objective by less than this
+    // (relative): the FP32 rounding of S (1e-7 of entries whose Schur complement cancels to 1e-8 of them on
+    // windows that refine their intrinsics) leaves steps that only serve the first, large decreases
+    constexpr double kMixedSwitchDecrease = 1e-2;
 
     // Host side of a batched solve: window ranges of the combined input and the per-window counts.
     struct BatchHost
@@ -836,6 +843,7 @@ struct pb_ctx
     std::vector<double> outR, outC, outGroupParams, outPointX;
     bool haveResult = false;
     pb_solve_options lastOptions{};  // options of the last solve (the adjoint linearises with them)
+    rocblas_handle blas = nullptr;   // dense factorisation of the adjoint system (created on first use)
     // profiling
     bool profile = false;
     int64_t pcgIterationsProfiled = 0;
@@ -1890,7 +1898,8 @@ namespace
         B.trialPart = A.alloc<double>(size_t(B.nChunks) * pbk::kMaxAlpha);
         B.trialBad = A.alloc<int32_t>(size_t(B.nChunks) * pbk::kMaxAlpha);
         B.doneClock = A.alloc<unsigned long long>(nWin + 1);
-        B.counts = A.alloc<int32_t>(2);
+        B.counts = A.alloc<int32_t>(4);
+        B.phase = A.alloc<int32_t>(nWin);
         P.winDamping = B.damping;
         P.winActive = B.active;
         P.winAlpha = B.alpha;
@@ -1904,14 +1913,56 @@ namespace
         ctx->batchMode = true;
     }
 
-    // One LM controller per window, all windows advancing one iteration per round
-    // (BundleAdjustPlaMatrix.cpp:162-631 with useReferenceOnlineSchur, per window).
+    // One LM iteration of every running window of one precision phase (the kernels of that precision).
     template <typename T>
-    void runBatch(pb_ctx* ctx, const pb_solve_options& o, pb_window_report* reports, pb_solve_report* summary,
-                  const volatile int32_t* cancel)
+    int batchIteration(pb_ctx* ctx, pbk::Work<T>& W, int phase, const pb_solve_options& o, int passAlpha,
+                       int32_t* hc)
     {
         using BL = pbk::BatchLauncher<T>;
-        pbk::Work<T>& W = work<T>(ctx);
+        hipStream_t s = ctx->stream;
+        const pbk::Problem& P = ctx->P;
+        const pbk::Batch& B = ctx->bh.B;
+        BL::roundBegin(P, W, B, phase, s);
+        BL::build(P, W, s);
+        BL::afterBuild(P, W, B, s);
+        BL::factor(P, W, B, s);
+        BL::backSubstitute(P, W, B, s);
+        // trial passes until no window is still searching (each pass spends at least one of a window's
+        // line-search steps, so at most line_search_steps passes)
+        int passes = 0;
+        for (int pass = 0; pass <= o.line_search_steps; ++pass)
+        {
+            BL::trialPass(P, W, B, pass == 0 ? 1 : passAlpha, s);
+            ++passes;
+            check(hipMemcpyAsync(hc + 2, B.counts + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s), "readback");
+            check(hipStreamSynchronize(s), "trial pass");
+            if (hc[2] == 0) break;
+        }
+        BL::finishRound(P, W, B, phase, s);
+        BL::count(B, s);
+        check(hipMemcpyAsync(hc, B.counts, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s), "readback");
+        check(hipStreamSynchronize(s), "round");
+        return passes;
+    }
+
+    template <typename T>
+    void zeroDirect(pb_ctx* ctx, pbk::Work<T>& W)
+    {
+        const pbk::Problem& P = ctx->P;
+        hipStream_t s = ctx->stream;
+        check(hipMemsetAsync(W.D, 0, sizeof(double) * size_t(std::max(1, P.nBlocks)) * 36, s), "memset");
+        check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
+    }
+
+    // One LM controller per window (BundleAdjustPlaMatrix.cpp:162-631 with useReferenceOnlineSchur, per window).
+    // A round gives every running window one iteration: FP64 batches run the FP64 kernels; mixed batches run
+    // the FP32 kernels on the windows still in the FP32 phase, then the FP64 kernels on the windows in the FP64
+    // phase (a window that moves to FP64 in a round takes its first FP64 iteration in the same round).
+    void runBatch(pb_ctx* ctx, const pb_solve_options& o, bool mixed, pb_window_report* reports,
+                  pb_solve_report* summary, const volatile int32_t* cancel)
+    {
+        pbk::Work<double>& W64 = work<double>(ctx);
+        pbk::Work<float>* W32 = mixed ? &work<float>(ctx) : nullptr;
         hipStream_t s = ctx->stream;
         const pbk::Problem& P = ctx->P;
         BatchHost& bh = ctx->bh;
@@ -1922,50 +1973,37 @@ namespace
         B.lineSearchSteps = o.line_search_steps;
         B.armijo = o.armijo_coefficient;
         B.rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
+        B.switchDecrease = kMixedSwitchDecrease;
         const int passAlpha = o.armijo_batch > 0 ? std::min(o.armijo_batch, pbk::kMaxAlpha) : 4;
         // controller state (zero = running, no damping, no counts); direct blocks that no camera writes stay zero
         check(hipMemsetAsync(B.status, 0, sizeof(int32_t) * nWin, s), "memset");
+        check(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(B.phase), mixed ? 0 : 1, nWin, s), "memset");
         check(hipMemsetAsync(B.damping, 0, sizeof(double) * nWin, s), "memset");
         check(hipMemsetAsync(B.counters, 0, sizeof(int32_t) * size_t(nWin) * pbk::kWinCounterCount, s), "memset");
         check(hipMemsetAsync(B.cost, 0, sizeof(double) * nWin, s), "memset");
         check(hipMemsetAsync(B.initialCost, 0, sizeof(double) * nWin, s), "memset");
         check(hipMemsetAsync(B.doneClock, 0, sizeof(unsigned long long) * (nWin + 1), s), "memset");
-        check(hipMemsetAsync(W.D, 0, sizeof(double) * size_t(std::max(1, P.nBlocks)) * 36, s), "memset");
-        check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
+        zeroDirect(ctx, W64);
+        if (W32) zeroDirect(ctx, *W32);
         check(hipStreamSynchronize(s), "reset");
         const auto t0 = Clock::now();
-        BL::start(B, s);
+        pbk::BatchLauncher<double>::start(B, s);
         int32_t* hc = reinterpret_cast<int32_t*>(ctx->pinned + kPinFlags);
+        hc[0] = mixed ? nWin : 0;
+        hc[1] = mixed ? 0 : nWin;
         int rounds = 0, passes = 0;
         bool cancelled = false;
-        // every running window spends one iteration per round, so the budget ends the loop
-        while (rounds <= B.maxIterations)
+        // every running window spends at least one iteration per round, so the budget ends the loop
+        while (rounds <= B.maxIterations && hc[0] + hc[1] > 0)
         {
             if (cancel && *cancel)
             {
                 cancelled = true;
                 break;
             }
-            BL::roundBegin(P, W, B, s);
-            BL::build(P, W, s);
-            BL::afterBuild(P, W, B, s);
-            BL::factor(P, W, B, s);
-            BL::backSubstitute(P, W, B, s);
-            // trial passes until no window is still searching (each pass spends at least one of a window's
-            // line-search steps, so at most line_search_steps passes)
-            for (int pass = 0; pass <= o.line_search_steps; ++pass)
-            {
-                BL::trialPass(P, W, B, pass == 0 ? 1 : passAlpha, s);
-                ++passes;
-                check(hipMemcpyAsync(hc, B.counts + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s), "readback");
-                check(hipStreamSynchronize(s), "trial pass");
-                if (hc[0] == 0) break;
-            }
-            BL::finishRound(P, W, B, s);
-            check(hipMemcpyAsync(hc, B.counts, sizeof(int32_t), hipMemcpyDeviceToHost, s), "readback");
-            check(hipStreamSynchronize(s), "round");
+            if (W32 && hc[0] > 0) passes += batchIteration<float>(ctx, *W32, 0, o, passAlpha, hc);
+            if (hc[1] > 0) passes += batchIteration<double>(ctx, W64, 1, o, passAlpha, hc);
             ++rounds;
-            if (hc[0] == 0) break;
         }
         check(hipStreamSynchronize(s), "batch");
         summary->solve_seconds = seconds(t0);
@@ -1993,6 +2031,7 @@ namespace
             r.rejected_steps = cnt[pbk::kWinRejected];
             r.linearizations = cnt[pbk::kWinLinearizations];
             r.objective_evaluations = cnt[pbk::kWinEvaluations];
+            r.fp64_iterations = cnt[pbk::kWinFp64Iterations];
             r.observation_count = bh.observations[w];
             r.active_track_count = bh.activeTracks[w];
             r.camera_block_count = bh.cameraBlocks[w];
@@ -2191,6 +2230,7 @@ pb_status pb_hip_destroy(pb_ctx* ctx)
     ctx->dwork32.release();
     ctx->dwork64.release();
     for (hipEvent_t e : ctx->eventPool) (void)hipEventDestroy(e);
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -2344,13 +2384,44 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
         check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
         Lr::adjointRhs(P, W, dC, dP, s);
         Lr::schurBlocks(P, W, 0.0, s);
-        // FP64 PCG to the FP64 solve path's accuracy class (relative residual 1e-10; 1e-12 is not reachable on
-        // aerial blocks whose reduced systems have condition numbers near 1e14)
-        const PcgResult lin = pcg(ctx, W, 1e-10, 1e-300, std::max(200, 30 * ctx->prep.primaryBlocks), 8, 0.0);
-        if (!lin.converged)
+        // H lambda = g is solved once per call, exactly: a dense FP64 Cholesky of the reduced system (rocSOLVER)
+        // up to kDenseAdjointMax rows (the aerial block has 6,024; its condition number near 1e14 keeps PCG
+        // from a tight residual), FP64 PCG to a 1e-10 relative residual above that
+        const long n = 6L * P.nUnits;
+        if (n > 0 && n <= kDenseAdjointMax)
         {
-            tmp.release();
-            return fail(ctx, PB_ERR_NUMERIC, "adjoint PCG did not converge");
+            double* A = tmp.alloc<double>(size_t(n) * n);
+            double* rhs = tmp.alloc<double>(n);
+            rocblas_int* info = tmp.alloc<rocblas_int>(1);
+            check(hipMemsetAsync(A, 0, sizeof(double) * size_t(n) * n, s), "memset");
+            Lr::denseSystem(P, W, A, rhs, s);
+            if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+                throw std::runtime_error("rocblas_create_handle failed");
+            if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success ||
+                rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, rocblas_int(n), A, rocblas_int(n), info) !=
+                    rocblas_status_success)
+                throw std::runtime_error("rocsolver_dpotrf failed");
+            rocblas_int hinfo = 0;
+            check(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s), "readback");
+            check(hipStreamSynchronize(s), "adjoint factorisation");
+            if (hinfo != 0)
+            {
+                tmp.release();
+                return fail(ctx, PB_ERR_NUMERIC, "adjoint system is not positive definite");
+            }
+            if (rocsolver_dpotrs(ctx->blas, rocblas_fill_lower, rocblas_int(n), 1, A, rocblas_int(n), rhs,
+                                 rocblas_int(n)) != rocblas_status_success)
+                throw std::runtime_error("rocsolver_dpotrs failed");
+            check(hipMemcpyAsync(W.x, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
+        }
+        else
+        {
+            const PcgResult lin = pcg(ctx, W, 1e-10, 1e-300, std::max(200, 30 * ctx->prep.primaryBlocks), 8, 0.0);
+            if (!lin.converged)
+            {
+                tmp.release();
+                return fail(ctx, PB_ERR_NUMERIC, "adjoint PCG did not converge");
+            }
         }
         Lr::backSubstitute(P, W, s);
         Lr::adjointObservations(P, W, dOut, s);
@@ -2428,19 +2499,14 @@ pb_status pb_hip_solve_batch(pb_ctx* ctx, const pb_batch_view* batch, const pb_s
             return PB_ERR_INVALID;
         }
         const bool mixed = options->precision == PB_PRECISION_MIXED;
-        if (mixed)
-            (void)work<float>(ctx);
-        else
-            (void)work<double>(ctx);
+        (void)work<double>(ctx);
+        if (mixed) (void)work<float>(ctx);
         check(hipStreamSynchronize(ctx->stream), "setup");
         summary->setup_seconds = seconds(ts);
         summary->online_schur_used = 1;
         summary->mixed_precision_used = mixed ? 1 : 0;
         ctx->lastOptions = *options;
-        if (mixed)
-            runBatch<float>(ctx, *options, reports, summary, cancel);
-        else
-            runBatch<double>(ctx, *options, reports, summary, cancel);
+        runBatch(ctx, *options, mixed, reports, summary, cancel);
         return summary->solve_status == PB_SOLVE_CANCELLED ? PB_ERR_CANCELLED : PB_OK;
     });
     if (st != PB_OK && st != PB_ERR_CANCELLED && summary->solve_status == PB_SOLVE_NOT_RUN)

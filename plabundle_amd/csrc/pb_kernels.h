@@ -184,6 +184,7 @@ namespace pbk
         kWinRejected,
         kWinLinearizations,
         kWinEvaluations,
+        kWinFp64Iterations,  // iterations built and solved with FP64 assembly
         kWinCounterCount = 8
     };
 
@@ -213,8 +214,13 @@ namespace pbk
         // controls
         int maxIterations = 0, lineSearchSteps = 0;
         double armijo = 0.0, rmsTol = 0.0;
+        // Mixed batches start every window with FP32 assembly (phase 0) and move a window to FP64 assembly
+        // (phase 1) once an accepted FP32 step lowers its objective by less than switchDecrease (relative), or
+        // would meet the step-RMS stop, or an FP32 step fails; the stop is only taken in FP64.
+        double switchDecrease = 0.0;
         // state
         int32_t* status = nullptr;   // WinStatus
+        int32_t* phase = nullptr;    // 0: FP32 assembly, 1: FP64 assembly
         int32_t* active = nullptr;   // Problem::winActive
         int32_t* reject = nullptr;   // step of this round rejected before the line search
         double* damping = nullptr;   // Problem::winDamping
@@ -236,7 +242,7 @@ namespace pbk
         double* trialPart = nullptr;   // [nChunks*kMaxAlpha]
         int32_t* trialBad = nullptr;
         unsigned long long* doneClock = nullptr;  // [nWin+1]; the last entry is the start of the solve
-        int32_t* counts = nullptr;  // [2] windows running after the round, windows searching after a pass
+        int32_t* counts = nullptr;  // [4] running windows in phase 0 / phase 1 after a round, windows searching
     };
 
     template <typename T>
@@ -298,24 +304,27 @@ namespace pbk
         static void adjointRhs(const Problem& P, const Work<T>& W, const double* dlCenter, const double* dlPoint,
                                hipStream_t s);
         static void adjointObservations(const Problem& P, const Work<T>& W, double* out, hipStream_t s);
+        // dense column-major copy of S (A, zero-filled by the caller) and of the reduced right-hand side
+        static void denseSystem(const Problem& P, const Work<T>& W, double* A, double* b, hipStream_t s);
         static void trialCosts(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, int costBlocks,
                                hipStream_t s);
         static void accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s);
     };
 
-    // Batched windows: one LM round of every running window per call sequence
+    // Batched windows: one LM iteration of every running window of one precision phase per call sequence
     //   roundBegin, build (shared kernels), afterBuild, factor, backSubstitute (shared), stepStats,
-    //   { trialPass } while windows search, control, apply.
+    //   { trialPass } while windows search, finishRound (control, apply), count.
     template <typename T>
     struct BatchLauncher
     {
         static void start(const Batch& B, hipStream_t s);
-        static void roundBegin(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
+        static void count(const Batch& B, hipStream_t s);  // running windows per phase -> counts[0], counts[1]
+        static void roundBegin(const Problem& P, const Work<T>& W, const Batch& B, int phase, hipStream_t s);
         static void build(const Problem& P, const Work<T>& W, hipStream_t s);
         static void afterBuild(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
         static void factor(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
         static void backSubstitute(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
         static void trialPass(const Problem& P, const Work<T>& W, const Batch& B, int maxAlpha, hipStream_t s);
-        static void finishRound(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
+        static void finishRound(const Problem& P, const Work<T>& W, const Batch& B, int phase, hipStream_t s);
     };
 }  // namespace pbk

@@ -1323,6 +1323,27 @@ namespace pbk
         Wk.y[3 * b + 2] = double(Lp[5]) * g2;
     }
 
+    // Dense copy of the reduced system (column-major, leading dimension 6 * nUnits) and of its right-hand side:
+    // one wave per block row, lanes over the 36 elements of each stored block.
+    template <typename T>
+    __global__ void k_dense_system(Problem P, Work<T> Wk, double* A, double* b)
+    {
+        const int u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+        if (u >= P.nUnits) return;
+        const int lane = threadIdx.x & 63;
+        const long n = 6L * P.nUnits;
+        for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k)
+        {
+            const int v = P.colIdx[k];
+            if (lane < 36)
+            {
+                const int i = lane / 6, j = lane % 6;
+                A[(6L * v + j) * n + 6 * u + i] = double(Wk.S[size_t(k) * 36 + lane]);
+            }
+        }
+        if (lane < 6) b[6 * u + lane] = Wk.rhs[6 * u + lane];
+    }
+
     template <typename T, int KM>
     __global__ void k_adjoint_observations(Problem P, Work<T> Wk, double* out)
     {
@@ -1494,6 +1515,12 @@ namespace pbk
             k_adjoint_observations<T, 6><<<nb, 128, 0, s>>>(P, W, out);
         else
             k_adjoint_observations<T, 12><<<nb, 128, 0, s>>>(P, W, out);
+    }
+
+    template <typename T>
+    void Launcher<T>::denseSystem(const Problem& P, const Work<T>& W, double* A, double* b, hipStream_t s)
+    {
+        if (P.nUnits > 0) k_dense_system<T><<<blocksFor(P.nUnits, 4), 256, 0, s>>>(P, W, A, b);
     }
 
     template <typename T>

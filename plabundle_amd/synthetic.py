@@ -399,8 +399,9 @@ def batch_windows(seed: int = 4, num_windows: int = 512, cameras=(8, 64), points
     return out
 
 
-def window_options(options=None, max_iterations: int = 10):
-    """Solve options of the batched windows: Huber 1 px, shared focal + Brown (k1, k2, k3, p1, p2) refined."""
+def window_options(options=None, max_iterations: int = 20):
+    """Solve options of the batched windows: Huber 1 px, shared focal + Brown (k1, k2, k3, p1, p2) refined,
+    the reference's default iteration budget."""
     from .options import ImageRobustLoss, SolveOptions
 
     o = options or SolveOptions()

@@ -53,6 +53,7 @@ def test_batch_fp64_matches_oracle_per_window(ws, oracle_mod, intrinsics):
         ref = orc.solve(co)
         st = orc.state()
         rel = abs(r.finalCost - ref["final_cost"]) / ref["final_cost"]
+        assert r.report["fp64_iterations"] == r.report["iterations"]
         _say(f"window {w}: {p.num_cameras} cameras {p.num_observations} obs, {r.report['iterations']} it "
              f"(oracle {ref['iterations']}), rel cost {rel:.2e}, {r.report['device_seconds'] * 1e3:.2f} ms")
         assert r.status == SolveStatus(ref["solve_status"])
@@ -67,31 +68,30 @@ def test_batch_fp64_matches_oracle_per_window(ws, oracle_mod, intrinsics):
 
 
 def test_batch_mixed_within_north_star_tolerance(ws, oracle_mod):
-    # The mixed windows assemble S in FP32, so a window's step-RMS stop (1e-6) can fall one iteration earlier
-    # than the oracle's; the last FP64 step of such a window still changes the cost by ~1e-6 relative. Parity
-    # is then checked against the oracle run with the window's own iteration count (1e-6), and the full-run
-    # gap is bounded at 1e-5.
+    # Mixed windows assemble in FP32 while their steps still lower the objective by 1e-2 or more and finish in
+    # FP64 (a window's stop is only taken in FP64). Windows that both the batch and the oracle end as converged
+    # must agree to the north-star bar; a window that the reference itself ends by an exhausted line search
+    # (NoConvergence after a few iterations) stops wherever its trajectory stalls, so it is held to 1e-4.
     _say("case batch mixed")
-    wins = _windows(seed=12)
-    o = S.window_options(PO.SolveOptions(), max_iterations=10)
+    wins = _windows(seed=12) + _windows(n=6, seed=17)
+    o = S.window_options(PO.SolveOptions(), max_iterations=20)
     res, summary = Solver().solve_batch(wins, o, ws, precision=PO.Precision.Mixed)
     assert summary["abi_status"] == _abi.PB_OK, summary["message"]
     co = PO.to_c(o)
+    both = 0
     for w, (p, r) in enumerate(zip(wins, res)):
         ref = oracle_mod.Oracle(p, threads=2).solve(co)
         rel = abs(r.finalCost - ref["final_cost"]) / ref["final_cost"]
-        its = r.report["iterations"]
-        _say(f"window {w}: mixed rel cost {rel:.2e}, {its} it (oracle {ref['iterations']})")
+        _say(f"window {w}: mixed rel cost {rel:.2e}, {r.report['iterations']} it ({r.report['fp64_iterations']} "
+             f"in FP64) status {r.status.name}; oracle {ref['iterations']} it status {ref['solve_status']}")
         assert r.usable()
-        if its < ref["iterations"]:
-            co_k = PO.to_c(S.window_options(PO.SolveOptions(), max_iterations=its))
-            ref_k = oracle_mod.Oracle(p, threads=2).solve(co_k)
-            rel_k = abs(r.finalCost - ref_k["final_cost"]) / ref_k["final_cost"]
-            _say(f"window {w}: oracle after {its} iterations: rel cost {rel_k:.2e}")
-            assert rel_k <= 1e-6
-            assert rel <= 1e-5
-        else:
+        assert r.report["fp64_iterations"] >= 1
+        if r.status == SolveStatus.Success and ref["solve_status"] == int(SolveStatus.Success):
+            both += 1
             assert rel <= 1e-6
+        else:
+            assert rel <= 1e-4
+    assert both >= len(wins) // 2
 
 
 def test_batch_auto_anchor_gauge_matches_single_solves(ws):
