@@ -107,21 +107,29 @@ namespace pbk
         int row[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) row[r] = j0 + tid + r * kDenseThreads;
-        for (int k = 0; k < j0; ++k)
+        // the loads of kDenseLoads columns are issued together, so their latency overlaps (j0 is a multiple of
+        // the panel width, itself a multiple of kDenseLoads)
+        for (int k0 = 0; k0 < j0; k0 += kDenseLoads)
         {
-            double a[R];
+            double a[kDenseLoads][R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) a[r] = row[r] < n ? A[size_t(k) * n + row[r]] : 0.0;
-            const double2* b2 = reinterpret_cast<const double2*>(Bt + k * kDensePanel);
+            for (int u = 0; u < kDenseLoads; ++u)
 #pragma unroll
-            for (int q = 0; q < kDensePanel / 2; ++q)
+                for (int r = 0; r < R; ++r) a[u][r] = row[r] < n ? A[size_t(k0 + u) * n + row[r]] : 0.0;
+#pragma unroll
+            for (int u = 0; u < kDenseLoads; ++u)
             {
-                const double2 b = b2[q];
+                const double2* b2 = reinterpret_cast<const double2*>(Bt + (k0 + u) * kDensePanel);
 #pragma unroll
-                for (int r = 0; r < R; ++r)
+                for (int q = 0; q < kDensePanel / 2; ++q)
                 {
-                    acc[r][2 * q] -= a[r] * b.x;
-                    acc[r][2 * q + 1] -= a[r] * b.y;
+                    const double2 b = b2[q];
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                    {
+                        acc[r][2 * q] -= a[u][r] * b.x;
+                        acc[r][2 * q + 1] -= a[u][r] * b.y;
+                    }
                 }
             }
         }
@@ -152,21 +160,33 @@ namespace pbk
         auto unitOf = [&](int r) { return r < nPose ? pu0 + r : gu0 + (r - nPose); };
         auto localOf = [&](int u) { return (u >= pu0 && u < pu1) ? u - pu0 : nPose + (u - gu0); };
 
-        // gather: zero, then one wave per block row writes the row's 6x6 blocks (lane = element)
+        // gather: zero, then every element of the stored blocks of the window's rows (the pose rows and the
+        // group rows are two contiguous ranges of the block-sparse rows) goes to its place, all loads independent
         for (long i = tid; i < long(n) * n; i += kDenseThreads) A[i] = 0.0;
         for (int i = tid; i < n; i += kDenseThreads) vec[i] = Wk.rhs[unitOf(i / 6) * 6 + i % 6];
         __syncthreads();
-        for (int r = wv; r < nU; r += kWaves)
         {
-            const int u = unitOf(r);
-            for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k)
+            const int kp0 = P.rowPtr[pu0], nbp = P.rowPtr[pu1] - kp0;
+            const int kg0 = P.rowPtr[gu0], nbg = P.rowPtr[B.groupUnitOffset[w + 1]] - kg0;
+            const long total = long(nbp + nbg) * 36;
+            for (long e0 = tid; e0 < total; e0 += kDenseLoads * kDenseThreads)
             {
-                const int c = localOf(P.colIdx[k]);
-                if (lane < 36)
+                double v[kDenseLoads];
+                size_t at[kDenseLoads];
+#pragma unroll
+                for (int u = 0; u < kDenseLoads; ++u)
                 {
-                    const int i = lane / 6, j = lane % 6;
-                    A[size_t(6 * c + j) * n + 6 * r + i] = double(Wk.S[size_t(k) * 36 + lane]);
+                    const long e = e0 + long(u) * kDenseThreads;
+                    const long ec = e < total ? e : total - 1;
+                    const int bi = int(ec / 36), el = int(ec % 36);
+                    const int k = bi < nbp ? kp0 + bi : kg0 + (bi - nbp);
+                    const int r = localOf(P.nnzRow[k]), c = localOf(P.colIdx[k]);
+                    at[u] = size_t(6 * c + el % 6) * n + 6 * r + el / 6;
+                    v[u] = double(Wk.S[size_t(k) * 36 + el]);
                 }
+#pragma unroll
+                for (int u = 0; u < kDenseLoads; ++u)
+                    if (e0 + long(u) * kDenseThreads < total) A[at[u]] = v[u];
             }
         }
         __syncthreads();
@@ -175,10 +195,19 @@ namespace pbk
         for (int j0 = 0; j0 < n; j0 += NB)
         {
             const int jb = min(NB, n - j0);
-            for (int q = tid; q < j0 * NB; q += kDenseThreads)
+            // staging: groups of kDenseLoads independent loads per thread before the LDS stores
+            for (int q0 = tid; q0 < j0 * NB; q0 += kDenseLoads * kDenseThreads)
             {
-                const int k = q / NB, jj = q % NB;
-                Bt[q] = jj < jb ? A[size_t(k) * n + j0 + jj] : 0.0;
+                double v[kDenseLoads];
+#pragma unroll
+                for (int u = 0; u < kDenseLoads; ++u)
+                {
+                    const int q = q0 + u * kDenseThreads, k = q / NB, jj = q % NB;
+                    v[u] = (q < j0 * NB && jj < jb) ? A[size_t(k) * n + j0 + jj] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < kDenseLoads; ++u)
+                    if (q0 + u * kDenseThreads < j0 * NB) Bt[q0 + u * kDenseThreads] = v[u];
             }
             __syncthreads();
             double acc[kDenseRows][NB];
@@ -299,7 +328,9 @@ namespace pbk
             for (int i = j0 + jb + tid; i < n; i += kDenseThreads)
             {
                 double s = vec[i];
-                for (int jj = 0; jj < jb; ++jj) s -= A[size_t(j0 + jj) * n + i] * vec[j0 + jj];
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj)
+                    if (jj < jb) s -= A[size_t(j0 + jj) * n + i] * vec[j0 + jj];
                 vec[i] = s;
             }
             __syncthreads();

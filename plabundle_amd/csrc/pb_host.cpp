@@ -796,8 +796,8 @@ namespace
     constexpr int kFamilyCount = sizeof(kFamilies) / sizeof(kFamilies[0]);
     constexpr long kDenseAdjointMax = 24576;  // rows of the dense adjoint factorisation (4.8 GB of FP64)
     // a mixed batch moves a window to FP64 assembly once an FP32 step lowers its objective by less than this
-    // (relative): the FP32 rounding of S (1e-7 of entries whose Schur complement cancels to 1e-8 of them on
-    // windows that refine their intrinsics) leaves steps that only serve the first, large decreases
+    // (relative): FP32 steps carry the early, large decreases; near the minimum the FP32 rounding of the
+    // reduced system limits them, so the last iterations of every window assemble in FP64
     constexpr double kMixedSwitchDecrease = 1e-2;
 
     // Host side of a batched solve: window ranges of the combined input and the per-window counts.
@@ -1862,6 +1862,12 @@ namespace
         P.camGaugeAnchor = A.upload(camGaugeAnchor, s);
         P.winBaseline = A.upload(bh.gaugeBaseline, s);
         P.winFlags = A.alloc<int32_t>(size_t(nWin) * pbk::kFlagCount);
+        {
+            std::vector<int32_t> nnzRow(std::max(1, LL.nNnz));
+            for (int u = 0; u < pp.nUnits; ++u)
+                for (int k = LL.rowPtr[u]; k < LL.rowPtr[u + 1]; ++k) nnzRow[k] = u;
+            P.nnzRow = A.upload(nnzRow, s);
+        }
         B.nWin = nWin;
         B.nChunks = static_cast<int>(chunkWin.size());
         B.maxDim = maxDim;

@@ -166,6 +166,7 @@ namespace pbk
         const int32_t* winActive = nullptr;       // [nWin] 1: the window builds and solves this round
         const double* winAlpha = nullptr;         // [nWin*kMaxAlpha] trial step lengths of the current pass
         const int32_t* winNAlpha = nullptr;       // [nWin] trial step lengths of the window in the current pass
+        const int32_t* nnzRow = nullptr;          // [nNnz] block row (unit) of each stored block
     };
 
     // Per-window LM controller state and window-local structure of a batched solve (device arrays).
@@ -189,6 +190,8 @@ namespace pbk
     };
 
     constexpr int kDensePanel = 16;  // column panel of the dense window factorisation
+    constexpr int kDenseLoads = 16;  // global loads a thread of the dense window solve issues together
+    static_assert(kDensePanel % kDenseLoads == 0, "panel starts are multiples of the load group");
     constexpr int kDenseRows = 2;    // matrix rows per thread of the factorisation workgroup
     constexpr int kDenseThreads = 256;
     constexpr int kMaxDense = kDenseRows * kDenseThreads;  // largest window system (6 * primary units)
