@@ -318,6 +318,46 @@ pb_status pb_hip_solve_batch(pb_ctx* ctx, const pb_batch_view* batch, const pb_s
 pb_status pb_hip_batch_download(pb_ctx* ctx, int32_t window, double* cam_rotation, double* cam_center,
                                 double* cam_intrinsics, double* track_point, int32_t* track_valid);
 
+/* Post-solve quality. Mirrors SolverControlOptions::enablePointFilter / filterMaxReprojError /
+ * filterSigmaFactor (include/plabundle/options.h:66-68) and the image part of QualityGateOptions
+ * (options.h:168-177). pb_default_quality_options fills the reference defaults. */
+typedef struct pb_quality_options {
+    int32_t enable_point_filter;            /* true */
+    double filter_max_reproj_error;         /* 2.5 px: floor of the adaptive threshold */
+    double filter_sigma_factor;             /* 3.0: threshold = max(floor, factor * median track RMS) */
+    int32_t enable_quality_gate;            /* QualityGateOptions::enabled = true */
+    double max_accepted_rms_growth;         /* 1.25 (0 disables only the RMS-growth check) */
+    double min_accepted_valid_track_ratio;  /* 0.60 */
+} pb_quality_options;
+
+/* Mirrors the image fields of plabundle::QualitySummary (include/plabundle/result.h:50-56) and the outcome of
+ * finalizeBundleAdjustResult + resultFailsQualityGate. */
+typedef struct pb_quality_report {
+    int32_t total_tracks;
+    int32_t optimized_tracks;       /* tracks valid after the point filter */
+    double valid_track_ratio;       /* optimized_tracks / total_tracks */
+    double mean_rms_before;         /* strict reprojection RMS (px) over the optimized tracks, input state */
+    double mean_rms_after;          /* the same at the refined state (inf when no track is optimized) */
+    double median_rms_after;        /* median strict RMS of the filter candidates */
+    double filter_threshold;        /* px (inf when the filter is off) */
+    int32_t solve_status;           /* the solve's status after the zero-track downgrade (pb_solve_status) */
+    int32_t solution_usable;
+    int32_t quality_gate_rejected;  /* 1 when the image quality gate rejects the result */
+    char message[128];
+} pb_quality_report;
+
+void pb_default_quality_options(pb_quality_options* options);
+
+/* Quality of the last pb_hip_solve, computed on the device: the strict reprojection RMS of every track at the
+ * input and at the refined state (a track with a projection behind a camera, an invalid weighted observation or
+ * fewer than two distinct cameras has RMS inf), the adaptive point filter, the track statistics and the image
+ * quality gate. Replaces finalizeBundleAdjustResult (src/internal/BundleAdjustQuality.cpp:302-502) and the
+ * image checks of resultFailsQualityGate (src/solver.cpp:123-171). Per-track outputs (input track order, any
+ * pointer may be NULL): track_rms_before / track_rms_after [num_tracks], track_valid [num_tracks] (1 when the
+ * track is kept by the filter). */
+pb_status pb_hip_quality(pb_ctx* ctx, const pb_quality_options* options, pb_quality_report* report,
+                         double* track_rms_before, double* track_rms_after, int32_t* track_valid);
+
 /* Last error message of the context (never NULL). */
 const char* pb_hip_last_error(const pb_ctx* ctx);
 

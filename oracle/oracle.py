@@ -51,6 +51,11 @@ def lib():
     L.oracle_compute_step.restype = C.c_int
     L.oracle_cost_after_step.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
     L.oracle_cost_after_step.restype = C.c_double
+    L.oracle_default_quality_options.argtypes = [C.POINTER(_abi.QualityOptionsC)]
+    L.oracle_default_quality_options.restype = None
+    L.oracle_quality.argtypes = [C.c_void_p, C.POINTER(_abi.QualityOptionsC), C.POINTER(_abi.QualityReportC), _dp,
+                                 _dp, _ip]
+    L.oracle_quality.restype = C.c_int
     L.oracle_initial_cost.argtypes = [C.c_void_p]
     L.oracle_initial_cost.restype = C.c_double
     L.oracle_project.argtypes = [_dp, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp]
@@ -69,6 +74,13 @@ def lib():
 
 def _p(a, t=C.c_double):
     return _abi.ptr(a, t)
+
+
+def default_quality_options() -> _abi.QualityOptionsC:
+    """Reference quality defaults (options.h:66-68, 168-177) without loading the HIP library."""
+    q = _abi.QualityOptionsC()
+    lib().oracle_default_quality_options(C.byref(q))
+    return q
 
 
 def default_options() -> _abi.SolveOptionsC:
@@ -180,6 +192,19 @@ class Oracle:
         primary = np.ascontiguousarray(primary, dtype=np.float64)
         elim = np.ascontiguousarray(elim, dtype=np.float64)
         return lib().oracle_cost_after_step(self.ctx, _p(primary), _p(elim), alpha)
+
+    def quality(self, qopts=None) -> dict:
+        """finalizeBundleAdjustResult + the image quality gate on the last solve (per-track arrays included)."""
+        if qopts is None:
+            qopts = default_quality_options()
+        n = self.problem.num_tracks
+        before, after = np.zeros(n), np.zeros(n)
+        valid = np.zeros(n, dtype=np.int32)
+        rep = _abi.QualityReportC()
+        lib().oracle_quality(self.ctx, C.byref(qopts), C.byref(rep), _p(before), _p(after), _p(valid, C.c_int32))
+        out = rep.as_dict()
+        out.update(track_rms_before=before, track_rms_after=after, track_valid=valid)
+        return out
 
     def initial_cost(self) -> float:
         return lib().oracle_initial_cost(self.ctx)

@@ -1867,9 +1867,54 @@ namespace
         o.prepared = true;
         return true;
     }
+
+    // --- post-solve quality (BundleAdjustQuality.cpp:302-502, solver.cpp:123-171) -------------------
+    // strictTrackRms (BundleAdjustQuality.cpp:302-363): whitened weights w / s^2; any weighted observation with an
+    // invalid camera or pixel, or whose projection fails, makes the whole track unusable (inf); fewer than two
+    // residuals or a single distinct camera also give inf.
+    double strictTrackRms(const std::vector<Camera>& cams, const Track& tr, const Vec3& X)
+    {
+        const double inf = std::numeric_limits<double>::infinity();
+        if (!std::isfinite(X[0]) || !std::isfinite(X[1]) || !std::isfinite(X[2])) return inf;
+        double sum = 0.0;
+        int residuals = 0, first = -1;
+        bool second = false;
+        for (const Observation& ob : tr.obs)
+        {
+            const double sc = sanitizedScale(ob);
+            const double w = sanitizedWeight(ob) / (sc * sc);
+            if (!(w > 0.0)) continue;
+            if (ob.cam < 0 || ob.cam >= static_cast<int>(cams.size()) || !std::isfinite(ob.u) || !std::isfinite(ob.v))
+                return inf;
+            ProjectionOut pr;
+            if (!project(cams[ob.cam], X, false, &pr)) return inf;
+            const double du = pr.uv[0] - ob.u, dv = pr.uv[1] - ob.v;
+            sum += w * (du * du + dv * dv);
+            residuals += 2;
+            if (first < 0)
+                first = ob.cam;
+            else if (ob.cam != first)
+                second = true;
+        }
+        if (residuals < 4 || !second) return inf;
+        return std::sqrt(sum / residuals);
+    }
+
+    // adaptivePointFilterThreshold (BundleAdjustQuality.cpp:373-392)
+    double adaptiveThreshold(const std::vector<double>& rms, double floorPx, double factor)
+    {
+        std::vector<double> finite;
+        for (double v : rms)
+            if (std::isfinite(v) && v >= 0.0) finite.push_back(v);
+        double threshold = std::isfinite(floorPx) && floorPx >= 0.0 ? floorPx : 0.0;
+        if (finite.empty() || !std::isfinite(factor) || !(factor > 0.0)) return threshold;
+        return std::max(threshold, factor * medianOr(finite, 0.0));
+    }
 }  // namespace
 
 extern "C" {
+
+
 
 struct oracle_ctx
 {
@@ -2163,6 +2208,102 @@ int oracle_linearize(const double* R, const double* C, const double* slots, int 
     *normal_weight = L.w;
     *robust_cost = L.cost;
     return 1;
+}
+
+
+void oracle_default_quality_options(pb_quality_options* q)
+{
+    q->enable_point_filter = 1;
+    q->filter_max_reproj_error = 2.5;
+    q->filter_sigma_factor = 3.0;
+    q->enable_quality_gate = 1;
+    q->max_accepted_rms_growth = 1.25;
+    q->min_accepted_valid_track_ratio = 0.60;
+}
+
+// finalizeBundleAdjustResult (BundleAdjustQuality.cpp:394-502) + the image checks of resultFailsQualityGate
+// (solver.cpp:123-171) on the oracle's last solve.
+int oracle_quality(oracle_ctx* ctx, const pb_quality_options* q, pb_quality_report* rep, double* rmsBefore,
+                   double* rmsAfter, int32_t* validOut)
+{
+    Oracle& o = ctx->o;
+    const double inf = std::numeric_limits<double>::infinity();
+    const std::size_t nT = o.pb.tracks.size();
+    std::memset(rep, 0, sizeof(*rep));
+    const State& st = o.state.cams.empty() ? o.init : o.state;
+    const std::vector<Camera> refined = (o.prepared && !st.cams.empty()) ? effectiveCameras(o.pb, o.act, st)
+                                                                         : o.pb.cams;
+    std::vector<double> before(nT), after(nT, inf), candidates;
+    std::vector<char> valid(nT, 0);
+    for (std::size_t t = 0; t < nT; ++t)
+    {
+        const Track& tr = o.pb.tracks[t];
+        before[t] = strictTrackRms(o.pb.cams, tr, tr.X);
+        if (!o.prepared || !o.act.activeTrack[t]) continue;
+        const Vec3& X = st.points.empty() ? tr.X : st.points[t];
+        after[t] = strictTrackRms(refined, tr, X);
+        valid[t] = std::isfinite(after[t]) ? 1 : 0;
+        if (valid[t]) candidates.push_back(after[t]);
+    }
+    const double threshold = q->enable_point_filter
+                                 ? adaptiveThreshold(candidates, q->filter_max_reproj_error, q->filter_sigma_factor)
+                                 : inf;
+    int optimized = 0;
+    double sumBefore = 0.0, sumAfter = 0.0;
+    for (std::size_t t = 0; t < nT; ++t)
+    {
+        if (valid[t] && after[t] > threshold) valid[t] = 0;
+        if (!valid[t]) continue;
+        ++optimized;
+        sumBefore += before[t];
+        sumAfter += after[t];
+    }
+    rep->total_tracks = static_cast<int32_t>(nT);
+    rep->optimized_tracks = optimized;
+    rep->mean_rms_before = optimized > 0 ? sumBefore / optimized : inf;
+    rep->mean_rms_after = optimized > 0 ? sumAfter / optimized : inf;
+    rep->valid_track_ratio = nT > 0 ? double(optimized) / double(nT) : 0.0;
+    rep->median_rms_after = medianOr(candidates, 0.0);
+    rep->filter_threshold = threshold;
+    rep->solve_status = o.report.solve_status;
+    rep->solution_usable = o.report.solution_usable;
+    if (optimized == 0 &&
+        (rep->solve_status == PB_SOLVE_SUCCESS || rep->solve_status == PB_SOLVE_NO_CONVERGENCE))
+    {
+        rep->solve_status = PB_SOLVE_NUMERICAL_FAILURE;
+        rep->solution_usable = 0;
+        std::snprintf(rep->message, sizeof(rep->message), "final quality check kept no valid track");
+    }
+    if (q->enable_quality_gate)
+    {
+        const char* why = nullptr;
+        const double growth = std::max(0.0, q->max_accepted_rms_growth);
+        if (!rep->solution_usable)
+            why = "quality gate rejected a non-usable solution";
+        else if (nT > 0 && !std::isfinite(rep->mean_rms_after))
+            why = "quality gate rejected a non-finite final RMS";
+        else if (nT > 0 && rep->valid_track_ratio < std::max(0.0, q->min_accepted_valid_track_ratio))
+            why = "quality gate rejected a low valid-track ratio";
+        else if (nT > 0 && growth > 0.0 && std::isfinite(rep->mean_rms_before))
+        {
+            if (rep->mean_rms_before > 1e-12 && rep->mean_rms_after > rep->mean_rms_before * growth)
+                why = "quality gate rejected excessive reprojection RMS growth";
+            else if (rep->mean_rms_before <= 1e-12 && rep->mean_rms_after > 1e-9)
+                why = "quality gate rejected growth from a zero-residual input";
+        }
+        if (why)
+        {
+            rep->quality_gate_rejected = 1;
+            std::snprintf(rep->message, sizeof(rep->message), "%s", why);
+        }
+    }
+    for (std::size_t t = 0; t < nT; ++t)
+    {
+        if (rmsBefore) rmsBefore[t] = before[t];
+        if (rmsAfter) rmsAfter[t] = after[t];
+        if (validOut) validOut[t] = valid[t];
+    }
+    return rep->quality_gate_rejected;
 }
 
 }  // extern "C"

@@ -177,6 +177,40 @@ class WindowReportC(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class QualityOptionsC(C.Structure):
+    _fields_ = [
+        ("enable_point_filter", C.c_int32),
+        ("filter_max_reproj_error", C.c_double),
+        ("filter_sigma_factor", C.c_double),
+        ("enable_quality_gate", C.c_int32),
+        ("max_accepted_rms_growth", C.c_double),
+        ("min_accepted_valid_track_ratio", C.c_double),
+    ]
+
+
+class QualityReportC(C.Structure):
+    _fields_ = [
+        ("total_tracks", C.c_int32),
+        ("optimized_tracks", C.c_int32),
+        ("valid_track_ratio", C.c_double),
+        ("mean_rms_before", C.c_double),
+        ("mean_rms_after", C.c_double),
+        ("median_rms_after", C.c_double),
+        ("filter_threshold", C.c_double),
+        ("solve_status", C.c_int32),
+        ("solution_usable", C.c_int32),
+        ("quality_gate_rejected", C.c_int32),
+        ("message", C.c_char * 128),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            value = getattr(self, name)
+            out[name] = value.decode("utf-8", "replace") if isinstance(value, bytes) else value
+        return out
+
+
 # Exported symbols of the shared library, in header order.
 EXPORTED_SYMBOLS = (
     "pb_default_options",
@@ -193,6 +227,8 @@ EXPORTED_SYMBOLS = (
     "pb_hip_backward",
     "pb_hip_solve_batch",
     "pb_hip_batch_download",
+    "pb_default_quality_options",
+    "pb_hip_quality",
     "pb_hip_last_error",
 )
 
@@ -244,6 +280,10 @@ def load_library(path: str | None = None):
     lib.pb_hip_solve_batch.restype = C.c_int
     lib.pb_hip_batch_download.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _ip]
     lib.pb_hip_batch_download.restype = C.c_int
+    lib.pb_default_quality_options.argtypes = [C.POINTER(QualityOptionsC)]
+    lib.pb_default_quality_options.restype = None
+    lib.pb_hip_quality.argtypes = [C.c_void_p, C.POINTER(QualityOptionsC), C.POINTER(QualityReportC), _dp, _dp, _ip]
+    lib.pb_hip_quality.restype = C.c_int
     lib.pb_hip_last_error.argtypes = [C.c_void_p]
     lib.pb_hip_last_error.restype = C.c_char_p
     if path is None:

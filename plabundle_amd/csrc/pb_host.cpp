@@ -831,6 +831,9 @@ struct pb_ctx
     DeviceAlloc dstruct;  // structure + state
     DeviceAlloc dwork32, dwork64;
     DeviceAlloc dbatch;      // per-window controller state of a batched solve
+    DeviceAlloc dquality;    // post-solve quality buffers (pb_hip_quality)
+    pbk::Quality Q;
+    int lastSolveStatus = PB_SOLVE_NOT_RUN, lastUsable = 0;
     bool batchMode = false;  // the context holds a batch (pb_hip_solve_batch); single-problem entries refuse it
     BatchHost bh;
     bool work32 = false, work64 = false;
@@ -1013,6 +1016,8 @@ namespace
         ctx->dwork32.release();
         ctx->dwork64.release();
         ctx->dbatch.release();
+        ctx->dquality.release();
+        ctx->Q = pbk::Quality{};
         ctx->work32 = ctx->work64 = false;
     }
 
@@ -1089,6 +1094,25 @@ namespace
         P.colIdx = A.upload(L.colIdx, s);
         P.diagPos = A.upload(L.diagPos, s);
         P.unitWidth = A.upload(p.unitWidth, s);
+        {
+            // post-solve quality: input cameras (own intrinsics), input points, strict-RMS exclusions
+            std::vector<pbdev::Camera> inCams(std::max(1, ctx->in.nCam));
+            for (int c = 0; c < ctx->in.nCam; ++c) inCams[c] = inputCamera(ctx->in, c);
+            P.inputCams = A.upload(inCams, s);
+            P.pointX0 = A.upload(L.pointX, s);
+            std::vector<int8_t> bad(std::max(1, L.nActive), 0);
+            for (int a = 0; a < L.nActive; ++a)
+            {
+                const int t = L.activeTrackOf[a];
+                for (int64_t i = ctx->in.off[t]; i < ctx->in.off[t + 1]; ++i)
+                {
+                    const double sc = sanitizedScale(ctx->in.scale[i]);
+                    const bool weighted = sanitizedWeight(ctx->in.weight[i]) / (sc * sc) > 0.0;
+                    if (weighted && !usable(ctx->in, i)) bad[a] = 1;
+                }
+            }
+            P.trackStrictBad = A.upload(bad, s);
+        }
         P.cgBlocks = static_cast<int>(L.cgWaveRow.size()) / pbk::kCgWaves;
         {
             std::vector<int32_t> slots(4 * L.cgWaveRow.size());
@@ -1132,8 +1156,9 @@ namespace
         W.bp = A.alloc<double>(size_t(P.nElim) * 3);
         W.y = A.alloc<double>(size_t(P.nElim) * 3);
         W.dp = A.alloc<double>(size_t(P.nElim) * 3);
-        W.D = A.alloc<double>(size_t(P.nBlocks) * 36);
-        W.directRhs = A.alloc<double>(size_t(P.nUnits) * 6);
+        // sized as the builds clear them (at least one block / unit, also for problems with nothing to refine)
+        W.D = A.alloc<double>(size_t(std::max(1, P.nBlocks)) * 36);
+        W.directRhs = A.alloc<double>(size_t(std::max(1, P.nUnits)) * 6);
         W.rhs = A.alloc<double>(size_t(P.nUnits) * 6);
         W.camCost = A.alloc<double>(P.nCam);
         W.camGroupPartial = A.alloc<double>(size_t(P.nCam) * pbk::kGroupPartialStride);
@@ -2100,6 +2125,37 @@ namespace
         ctx->haveResult = true;
     }
 
+    // strictTrackRms (BundleAdjustQuality.cpp:302-363) of an input track at the input state, on the host (used for
+    // the tracks that the solve did not activate).
+    double hostStrictRms(const Input& in, const std::vector<pbdev::Camera>& cams, int t)
+    {
+        const double inf = std::numeric_limits<double>::infinity();
+        const double* X = &in.X[3 * t];
+        if (!std::isfinite(X[0]) || !std::isfinite(X[1]) || !std::isfinite(X[2])) return inf;
+        double sum = 0.0;
+        int residuals = 0, first = -1;
+        bool second = false;
+        for (int64_t i = in.off[t]; i < in.off[t + 1]; ++i)
+        {
+            const double sc = sanitizedScale(in.scale[i]);
+            const double w = sanitizedWeight(in.weight[i]) / (sc * sc);
+            if (!(w > 0.0)) continue;
+            if (!usable(in, i)) return inf;
+            const int c = in.obsCam[i];
+            double u, v;
+            if (!pbdev::project(cams[c], X[0], X[1], X[2], u, v)) return inf;
+            const double du = u - in.uv[2 * i], dv = v - in.uv[2 * i + 1];
+            sum += w * (du * du + dv * dv);
+            residuals += 2;
+            if (first < 0)
+                first = c;
+            else if (c != first)
+                second = true;
+        }
+        if (residuals < 4 || !second) return inf;
+        return std::sqrt(sum / residuals);
+    }
+
     // Refined (or, before a solve, initial) state of input cameras [c0, c1) and tracks [t0, t1), written from
     // index 0 of each output (publishIntrinsics, BundleAdjustPlaMatrixModel.cpp:271-352, without transition
     // priors).
@@ -2235,6 +2291,8 @@ pb_status pb_hip_destroy(pb_ctx* ctx)
     ctx->dstruct.release();
     ctx->dwork32.release();
     ctx->dwork64.release();
+    ctx->dbatch.release();
+    ctx->dquality.release();
     for (hipEvent_t e : ctx->eventPool) (void)hipEventDestroy(e);
     if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -2323,6 +2381,8 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
             runSolve<float>(ctx, *options, report, cancel);
         else
             runSolve<double>(ctx, *options, report, cancel);
+        ctx->lastSolveStatus = report->solve_status;
+        ctx->lastUsable = report->solution_usable;
         if (report->solve_status == PB_SOLVE_CANCELLED) return PB_ERR_CANCELLED;
         return PB_OK;
     });
@@ -2615,6 +2675,134 @@ pb_status pb_hip_evaluate_cost(pb_ctx* ctx, const pb_solve_options* options, dou
 {
     if (!cost) return PB_ERR_INVALID;
     return pb_hip_reduced_system(ctx, options, 0.0, nullptr, nullptr, cost, nullptr);
+}
+
+void pb_default_quality_options(pb_quality_options* q)
+{
+    if (!q) return;
+    std::memset(q, 0, sizeof(*q));
+    q->enable_point_filter = 1;
+    q->filter_max_reproj_error = 2.5;
+    q->filter_sigma_factor = 3.0;
+    q->enable_quality_gate = 1;
+    q->max_accepted_rms_growth = 1.25;
+    q->min_accepted_valid_track_ratio = 0.60;
+}
+
+pb_status pb_hip_quality(pb_ctx* ctx, const pb_quality_options* q, pb_quality_report* rep, double* rmsBefore,
+                         double* rmsAfter, int32_t* validOut)
+{
+    if (!ctx || !q || !rep) return PB_ERR_INVALID;
+    std::memset(rep, 0, sizeof(*rep));
+    if (!ctx->uploaded || !ctx->haveResult || ctx->batchMode || !ctx->structureValid)
+        return fail(ctx, PB_ERR_INVALID, "pb_hip_quality needs the result of a pb_hip_solve on this context");
+    return guarded(ctx, [&]() -> pb_status {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        hipStream_t s = ctx->stream;
+        const pbk::Problem& P = ctx->P;
+        const Layout& L = ctx->lay;
+        const Input& in = ctx->in;
+        pbk::Quality& Q = ctx->Q;
+        if (!Q.out)
+        {
+            DeviceAlloc& A = ctx->dquality;
+            const size_t n = size_t(std::max(1, P.nActive));
+            Q.before = A.alloc<double>(n);
+            Q.after = A.alloc<double>(n);
+            Q.key = A.alloc<double>(n);
+            Q.sorted = A.alloc<double>(n);
+            Q.valid = A.alloc<int32_t>(n);
+            Q.out = A.alloc<double>(8);
+            Q.tempBytes = std::max<size_t>(1, pbk::QualityLauncher::sortScratchBytes(P.nActive));
+            Q.temp = A.alloc<char>(Q.tempBytes);
+        }
+        // effective cameras of the refined state (the solve's last accepted step moved them after the last build)
+        pbk::Launcher<double>::prepareCameras(P, s);
+        pbk::QualityLauncher::run(P, Q, q->enable_point_filter, q->filter_max_reproj_error, q->filter_sigma_factor, s);
+        double out[8] = {};
+        check(hipMemcpyAsync(out, Q.out, sizeof(out), hipMemcpyDeviceToHost, s), "readback");
+        std::vector<double> before, after;
+        std::vector<int32_t> valid;
+        const bool perTrack = rmsBefore || rmsAfter || validOut;
+        if (perTrack && P.nActive > 0)
+        {
+            before.resize(P.nActive);
+            after.resize(P.nActive);
+            valid.resize(P.nActive);
+            check(hipMemcpyAsync(before.data(), Q.before, sizeof(double) * P.nActive, hipMemcpyDeviceToHost, s), "readback");
+            check(hipMemcpyAsync(after.data(), Q.after, sizeof(double) * P.nActive, hipMemcpyDeviceToHost, s), "readback");
+            check(hipMemcpyAsync(valid.data(), Q.valid, sizeof(int32_t) * P.nActive, hipMemcpyDeviceToHost, s), "readback");
+        }
+        check(hipStreamSynchronize(s), "quality");
+        const double inf = std::numeric_limits<double>::infinity();
+        const int kept = static_cast<int>(out[3]);
+        rep->total_tracks = in.nTracks;
+        rep->optimized_tracks = kept;
+        rep->mean_rms_before = kept > 0 ? out[4] / kept : inf;
+        rep->mean_rms_after = kept > 0 ? out[5] / kept : inf;
+        rep->valid_track_ratio = in.nTracks > 0 ? double(kept) / double(in.nTracks) : 0.0;
+        rep->median_rms_after = out[1];
+        rep->filter_threshold = out[2];
+        rep->solve_status = ctx->lastSolveStatus;
+        rep->solution_usable = ctx->lastUsable;
+        // finalizeBundleAdjustResult (BundleAdjustQuality.cpp:490-501): no kept track downgrades a usable solve
+        if (kept == 0 && (rep->solve_status == PB_SOLVE_SUCCESS || rep->solve_status == PB_SOLVE_NO_CONVERGENCE))
+        {
+            rep->solve_status = PB_SOLVE_NUMERICAL_FAILURE;
+            rep->solution_usable = 0;
+            std::snprintf(rep->message, sizeof(rep->message), "final quality check kept no valid track");
+        }
+        // resultFailsQualityGate, image checks (solver.cpp:123-171)
+        if (q->enable_quality_gate)
+        {
+            const char* why = nullptr;
+            const double growth = std::max(0.0, q->max_accepted_rms_growth);
+            const bool any = in.nTracks > 0;
+            if (!rep->solution_usable)
+                why = "quality gate rejected a non-usable solution";
+            else if (any && !std::isfinite(rep->mean_rms_after))
+                why = "quality gate rejected a non-finite final RMS";
+            else if (any && rep->valid_track_ratio < std::max(0.0, q->min_accepted_valid_track_ratio))
+                why = "quality gate rejected a low valid-track ratio";
+            else if (any && growth > 0.0 && std::isfinite(rep->mean_rms_before))
+            {
+                if (rep->mean_rms_before > 1e-12 && rep->mean_rms_after > rep->mean_rms_before * growth)
+                    why = "quality gate rejected excessive reprojection RMS growth";
+                else if (rep->mean_rms_before <= 1e-12 && rep->mean_rms_after > 1e-9)
+                    why = "quality gate rejected growth from a zero-residual input";
+            }
+            if (why)
+            {
+                rep->quality_gate_rejected = 1;
+                std::snprintf(rep->message, sizeof(rep->message), "%s", why);
+            }
+        }
+        if (perTrack)
+        {
+            // inactive tracks: input-state RMS on the host (they are never refined), refined RMS inf
+            std::vector<pbdev::Camera> inCams;
+            if (rmsBefore)
+            {
+                inCams.resize(in.nCam);
+                for (int c = 0; c < in.nCam; ++c) inCams[c] = inputCamera(in, c);
+            }
+            for (int t = 0; t < in.nTracks; ++t)
+            {
+                const int a = L.activeIndexOf[t];
+                if (a >= 0)
+                {
+                    if (rmsBefore) rmsBefore[t] = before[a];
+                    if (rmsAfter) rmsAfter[t] = after[a];
+                    if (validOut) validOut[t] = valid[a];
+                    continue;
+                }
+                if (rmsAfter) rmsAfter[t] = inf;
+                if (validOut) validOut[t] = 0;
+                if (rmsBefore) rmsBefore[t] = hostStrictRms(in, inCams, t);
+            }
+        }
+        return PB_OK;
+    });
 }
 
 const char* pb_hip_last_error(const pb_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }

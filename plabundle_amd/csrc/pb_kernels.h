@@ -151,6 +151,11 @@ namespace pbk
         const int4* cgSlot = nullptr;  // [cgBlocks*kCgWaves] {row (-1: empty), begin, end, lead}
         const int32_t* cgColDirect = nullptr;  // [cgBlocks*kCgWaves*kCgDirect] column of block begin+i, -1 past end
 
+        // post-solve quality (pb_hip_quality)
+        const pbdev::Camera* inputCams = nullptr;  // [nCam] input cameras with their own intrinsics
+        const double* pointX0 = nullptr;           // [nActive*3] input points
+        const int8_t* trackStrictBad = nullptr;    // [nActive] 1: a weighted observation has an invalid camera or pixel
+
         // Batched windows (pb_hip_solve_batch): independent problems solved together as one block-diagonal
         // problem whose cameras, groups, tracks and units are numbered window by window. nWin = 0 for a
         // single solve; then every pointer below is null and the scalar gauge/damping arguments apply.
@@ -312,6 +317,25 @@ namespace pbk
         static void trialCosts(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, int costBlocks,
                                hipStream_t s);
         static void accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s);
+    };
+
+    // Post-solve quality buffers (device): strict track RMS at the input and refined state, sort keys, filter.
+    struct Quality
+    {
+        double* before = nullptr;  // [nActive]
+        double* after = nullptr;   // [nActive]
+        double* key = nullptr;     // [nActive] refined RMS of filter candidates, inf otherwise
+        double* sorted = nullptr;  // [nActive]
+        int32_t* valid = nullptr;  // [nActive] kept by the filter
+        double* out = nullptr;     // [8] candidates, median, threshold, kept, sum before, sum after
+        void* temp = nullptr;      // radix-sort scratch
+        size_t tempBytes = 0;
+    };
+
+    struct QualityLauncher
+    {
+        static size_t sortScratchBytes(int n);
+        static void run(const Problem& P, const Quality& Q, int filter, double floorPx, double factor, hipStream_t s);
     };
 
     // Batched windows: one LM iteration of every running window of one precision phase per call sequence

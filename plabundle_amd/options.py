@@ -61,6 +61,8 @@ class SolverControlOptions:
     referenceLineSearchSteps: int = 34
     useReferenceOnlineSchur: bool = True
     enablePointFilter: bool = True
+    filterMaxReprojError: float = 2.5
+    filterSigmaFactor: float = 3.0
     maxInitialTrackRms: float = 100.0
     numThreads: int = 0
     logIterationProgress: bool = True
@@ -125,10 +127,20 @@ class BackendOptions:
 
 
 @dataclass
+class QualityGateOptions:
+    """plabundle::QualityGateOptions (include/plabundle/options.h:168-177), image checks."""
+
+    enabled: bool = True
+    maxAcceptedRmsGrowth: float = 1.25
+    minAcceptedValidTrackRatio: float = 0.60
+
+
+@dataclass
 class SolveOptions:
     solver: SolverControlOptions = field(default_factory=SolverControlOptions)
     calibration: CalibrationOptions = field(default_factory=CalibrationOptions)
     backend: BackendOptions = field(default_factory=BackendOptions)
+    quality: QualityGateOptions = field(default_factory=QualityGateOptions)
 
 
 def shared_intrinsic_parameter_enabled(calibration: CalibrationOptions, parameter: int) -> bool:
@@ -204,3 +216,15 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.pcg_model_eta = float(b.pcgModelEta)
     o.step_rms_tolerance = float(b.stepRmsTolerance)
     return o
+
+
+def quality_to_c(options: SolveOptions) -> _abi.QualityOptionsC:
+    """The post-solve quality fields of SolveOptions as pb_quality_options."""
+    q = _abi.QualityOptionsC()
+    q.enable_point_filter = 1 if options.solver.enablePointFilter else 0
+    q.filter_max_reproj_error = float(options.solver.filterMaxReprojError)
+    q.filter_sigma_factor = float(options.solver.filterSigmaFactor)
+    q.enable_quality_gate = 1 if options.quality.enabled else 0
+    q.max_accepted_rms_growth = float(options.quality.maxAcceptedRmsGrowth)
+    q.min_accepted_valid_track_ratio = float(options.quality.minAcceptedValidTrackRatio)
+    return q

@@ -48,6 +48,9 @@ class Result:
     track_point: np.ndarray | None = None
     track_valid: np.ndarray | None = None
     abi_status: int = _abi.PB_OK
+    quality: dict | None = None
+    qualityGateRejected: bool = False
+    qualityGateMessage: str = ""
 
     def usable(self) -> bool:
         return self.solutionUsable and self.status in (SolveStatus.Success, SolveStatus.NoConvergence)
@@ -69,6 +72,7 @@ class SolverWorkspace:
         self._lib = _abi.load_library()
         self._ctx = C.c_void_p()
         self._uploaded_id = None
+        self._num_tracks = 0
         st = self._lib.pb_hip_create(device, C.byref(self._ctx))
         if st != _abi.PB_OK:
             self._ctx = None
@@ -87,6 +91,7 @@ class SolverWorkspace:
         del keep
         if st == _abi.PB_OK:
             self._uploaded_id = id(problem)
+            self._num_tracks = problem.num_tracks
         return st
 
     def clear(self):
@@ -151,6 +156,21 @@ class SolverWorkspace:
             raise RuntimeError(f"pb_hip_backward: {_abi.STATUS_NAMES.get(st, st)}: {self.last_error()}")
         return out
 
+    def quality(self, qopts) -> dict:
+        """Post-solve quality of the last solve on this workspace (pb_hip_quality): the report fields plus
+        per-track strict RMS before/after and the filter's validity, in input track order."""
+        n = self._num_tracks
+        before, after = np.zeros(n), np.zeros(n)
+        valid = np.zeros(n, dtype=np.int32)
+        rep = _abi.QualityReportC()
+        st = self._lib.pb_hip_quality(self._ctx, C.byref(qopts), C.byref(rep), _abi.ptr(before, C.c_double),
+                                      _abi.ptr(after, C.c_double), _abi.ptr(valid, C.c_int32))
+        if st != _abi.PB_OK:
+            raise RuntimeError(f"pb_hip_quality: {_abi.STATUS_NAMES.get(st, st)}: {self.last_error()}")
+        out = rep.as_dict()
+        out.update(track_rms_before=before, track_rms_after=after, track_valid=valid)
+        return out
+
     def kernel_stats(self, name: str) -> dict:
         launches = C.c_int64()
         ms = C.c_double()
@@ -175,7 +195,11 @@ class Solver:
         return bool(lib.pb_hip_available(int(deviceIndex)))
 
     def solve(self, problem: Problem, options: SolveOptions | None = None, workspace: SolverWorkspace | None = None,
-              precision: Precision | None = None, cancel=None) -> Result:
+              precision: Precision | None = None, cancel=None, quality: bool = False) -> Result:
+        """With quality=True the post-solve quality runs on the device after the solve (finalizeBundleAdjustResult
+        and the image quality gate of src/solver.cpp:123-171): Result.quality holds its report, a solve that keeps
+        no track is downgraded to NumericalFailure, and a gate rejection sets qualityGateRejected and makes the
+        result unusable (the reference then falls back or publishes nothing)."""
         options = options or SolveOptions()
         result = Result(requestedBackend=options.backend.requested, usedBackend=Backend.Mi355x)
         if options.backend.requested not in (Backend.Mi355x, Backend.Auto):
@@ -218,6 +242,15 @@ class Solver:
                             _abi.ptr(X, C.c_double), _abi.ptr(V, C.c_int32))
         result.cam_rotation, result.cam_center, result.cam_intrinsics = R, Cc, I
         result.track_point, result.track_valid = X, V
+        if quality and st == _abi.PB_OK and result.usable():
+            from .options import quality_to_c
+
+            q = ws.quality(quality_to_c(options))
+            result.quality = q
+            result.status = SolveStatus(q["solve_status"])
+            result.solutionUsable = bool(q["solution_usable"]) and not q["quality_gate_rejected"]
+            result.qualityGateRejected = bool(q["quality_gate_rejected"])
+            result.qualityGateMessage = q["message"]
         return result
 
     def solve_batch(self, problems: list[Problem], options: SolveOptions | None = None,

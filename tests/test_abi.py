@@ -47,6 +47,8 @@ int main(void) {
          offsetof(pb_solve_report, message));
   printf("%zu %zu %zu %zu\n", sizeof(pb_batch_view), offsetof(pb_batch_view, windows), sizeof(pb_window_report),
          offsetof(pb_window_report, device_seconds));
+  printf("%zu %zu %zu %zu\n", sizeof(pb_quality_options), offsetof(pb_quality_options, min_accepted_valid_track_ratio),
+         sizeof(pb_quality_report), offsetof(pb_quality_report, message));
   return 0;
 }
 """
@@ -64,6 +66,9 @@ int main(void) {
     batch = [int(x) for x in lines[2].split()]
     assert batch == [C.sizeof(_abi.BatchView), _abi.BatchView.windows.offset, C.sizeof(_abi.WindowReportC),
                      _abi.WindowReportC.device_seconds.offset]
+    quality = [int(x) for x in lines[3].split()]
+    assert quality == [C.sizeof(_abi.QualityOptionsC), _abi.QualityOptionsC.min_accepted_valid_track_ratio.offset,
+                       C.sizeof(_abi.QualityReportC), _abi.QualityReportC.message.offset]
 
 
 def test_default_options_are_the_reference_defaults(lib):
