@@ -358,6 +358,79 @@ void pb_default_quality_options(pb_quality_options* options);
 pb_status pb_hip_quality(pb_ctx* ctx, const pb_quality_options* options, pb_quality_report* report,
                          double* track_rms_before, double* track_rms_after, int32_t* track_valid);
 
+/* Solver-neutral outcome of one solve keyed by caller-owned ids, for legacy/new comparisons. Mirrors
+ * plabundle::PhotoAlignmentOutcome (include/plabundle/photo_alignment.h:40-53); the arrays are borrowed. */
+typedef struct pb_outcome_view {
+    int32_t termination_status;       /* pb_solve_status */
+    const char* termination_reason;   /* NULL = "" */
+    int32_t requested_backend;        /* plabundle::Backend numbering */
+    int32_t used_backend;
+    int32_t solution_usable;
+    int32_t backend_fallback;
+    int32_t quality_gate_rejected;
+    double mean_rms_before;
+    double mean_rms_after;
+    int32_t num_cameras;
+    const char* const* camera_ids;    /* [num_cameras] non-empty, unique */
+    const double* cam_rotation;       /* [num_cameras*9] camera-to-world, row-major */
+    const double* cam_center;         /* [num_cameras*3] */
+    const double* cam_intrinsics;     /* [num_cameras*13] */
+    const int32_t* cam_image_size;    /* [num_cameras*2] width, height (> 0) */
+    int32_t num_tracks;
+    const char* const* track_ids;     /* [num_tracks] non-empty, unique */
+    const double* track_point;        /* [num_tracks*3] */
+    const int32_t* track_valid;       /* [num_tracks] */
+    const double* track_rms_before;   /* [num_tracks] */
+    const double* track_rms_after;    /* [num_tracks] */
+} pb_outcome_view;
+
+/* Mirrors plabundle::PhotoAlignmentComparisonTolerance (photo_alignment.h:61-71). */
+typedef struct pb_comparison_tolerance {
+    double rms;                        /* 1e-8 px */
+    double camera_center_meters;       /* 1e-8 */
+    double camera_rotation_degrees;    /* 1e-8 */
+    double point_meters;               /* 1e-8 */
+    int32_t require_termination_status_match;  /* 1 */
+    int32_t require_termination_reason_match;  /* 1 */
+    int32_t require_requested_backend_match;   /* 1 */
+    int32_t require_used_backend_match;        /* 1 */
+} pb_comparison_tolerance;
+
+/* Mirrors plabundle::PhotoAlignmentComparison (photo_alignment.h:86-103); per-id lists are optional outputs of
+ * pb_compare_outcomes. */
+typedef struct pb_comparison {
+    int32_t comparable;
+    int32_t equivalent;
+    int32_t termination_status_matches;
+    int32_t termination_reason_matches;
+    int32_t requested_backend_matches;
+    int32_t used_backend_matches;
+    double rms_before_difference;
+    double rms_after_difference;
+    double maximum_camera_center_difference_meters;
+    double maximum_camera_rotation_difference_degrees;
+    double maximum_point_difference_meters;
+    int32_t valid_track_mask_mismatches;  /* number of ids whose valid flags differ */
+    char message[128];
+} pb_comparison;
+
+void pb_default_comparison_tolerance(pb_comparison_tolerance* tolerance);
+
+/* validatePhotoAlignmentOutcome (src/photo_alignment.cpp:193-223): finite mean RMS values, non-empty unique ids,
+ * every camera with a valid image size, finite non-zero focal lengths and a finite centre, every track with a
+ * finite point and finite RMS values. PB_ERR_INVALID with the reason in error (error_size bytes) otherwise. */
+pb_status pb_validate_outcome(const pb_outcome_view* outcome, char* error, int32_t error_size);
+
+/* comparePhotoAlignmentOutcomes (src/photo_alignment.cpp:225-318): matches cameras and tracks by id (the id sets
+ * must be equal), the centre distance and rotation angle per camera, the point distance per track valid on both
+ * sides, the valid-mask mismatches, and equivalence under the tolerance. Optional per-id outputs in the
+ * reference outcome's order: camera_center_difference / camera_rotation_difference_degrees [num_cameras],
+ * point_difference [num_tracks] (0 unless valid on both sides), valid_mismatch [num_tracks]. */
+pb_status pb_compare_outcomes(const pb_outcome_view* reference, const pb_outcome_view* candidate,
+                              const pb_comparison_tolerance* tolerance, pb_comparison* comparison,
+                              double* camera_center_difference, double* camera_rotation_difference_degrees,
+                              double* point_difference, int32_t* valid_mismatch);
+
 /* Last error message of the context (never NULL). */
 const char* pb_hip_last_error(const pb_ctx* ctx);
 

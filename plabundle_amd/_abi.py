@@ -211,6 +211,70 @@ class QualityReportC(C.Structure):
         return out
 
 
+class OutcomeView(C.Structure):
+    _fields_ = [
+        ("termination_status", C.c_int32),
+        ("termination_reason", C.c_char_p),
+        ("requested_backend", C.c_int32),
+        ("used_backend", C.c_int32),
+        ("solution_usable", C.c_int32),
+        ("backend_fallback", C.c_int32),
+        ("quality_gate_rejected", C.c_int32),
+        ("mean_rms_before", C.c_double),
+        ("mean_rms_after", C.c_double),
+        ("num_cameras", C.c_int32),
+        ("camera_ids", C.POINTER(C.c_char_p)),
+        ("cam_rotation", _dp),
+        ("cam_center", _dp),
+        ("cam_intrinsics", _dp),
+        ("cam_image_size", _ip),
+        ("num_tracks", C.c_int32),
+        ("track_ids", C.POINTER(C.c_char_p)),
+        ("track_point", _dp),
+        ("track_valid", _ip),
+        ("track_rms_before", _dp),
+        ("track_rms_after", _dp),
+    ]
+
+
+class ComparisonToleranceC(C.Structure):
+    _fields_ = [
+        ("rms", C.c_double),
+        ("camera_center_meters", C.c_double),
+        ("camera_rotation_degrees", C.c_double),
+        ("point_meters", C.c_double),
+        ("require_termination_status_match", C.c_int32),
+        ("require_termination_reason_match", C.c_int32),
+        ("require_requested_backend_match", C.c_int32),
+        ("require_used_backend_match", C.c_int32),
+    ]
+
+
+class ComparisonC(C.Structure):
+    _fields_ = [
+        ("comparable", C.c_int32),
+        ("equivalent", C.c_int32),
+        ("termination_status_matches", C.c_int32),
+        ("termination_reason_matches", C.c_int32),
+        ("requested_backend_matches", C.c_int32),
+        ("used_backend_matches", C.c_int32),
+        ("rms_before_difference", C.c_double),
+        ("rms_after_difference", C.c_double),
+        ("maximum_camera_center_difference_meters", C.c_double),
+        ("maximum_camera_rotation_difference_degrees", C.c_double),
+        ("maximum_point_difference_meters", C.c_double),
+        ("valid_track_mask_mismatches", C.c_int32),
+        ("message", C.c_char * 128),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            value = getattr(self, name)
+            out[name] = value.decode("utf-8", "replace") if isinstance(value, bytes) else value
+        return out
+
+
 # Exported symbols of the shared library, in header order.
 EXPORTED_SYMBOLS = (
     "pb_default_options",
@@ -229,6 +293,9 @@ EXPORTED_SYMBOLS = (
     "pb_hip_batch_download",
     "pb_default_quality_options",
     "pb_hip_quality",
+    "pb_default_comparison_tolerance",
+    "pb_validate_outcome",
+    "pb_compare_outcomes",
     "pb_hip_last_error",
 )
 
@@ -284,6 +351,13 @@ def load_library(path: str | None = None):
     lib.pb_default_quality_options.restype = None
     lib.pb_hip_quality.argtypes = [C.c_void_p, C.POINTER(QualityOptionsC), C.POINTER(QualityReportC), _dp, _dp, _ip]
     lib.pb_hip_quality.restype = C.c_int
+    lib.pb_default_comparison_tolerance.argtypes = [C.POINTER(ComparisonToleranceC)]
+    lib.pb_default_comparison_tolerance.restype = None
+    lib.pb_validate_outcome.argtypes = [C.POINTER(OutcomeView), C.c_char_p, C.c_int32]
+    lib.pb_validate_outcome.restype = C.c_int
+    lib.pb_compare_outcomes.argtypes = [C.POINTER(OutcomeView), C.POINTER(OutcomeView),
+                                        C.POINTER(ComparisonToleranceC), C.POINTER(ComparisonC), _dp, _dp, _dp, _ip]
+    lib.pb_compare_outcomes.restype = C.c_int
     lib.pb_hip_last_error.argtypes = [C.c_void_p]
     lib.pb_hip_last_error.restype = C.c_char_p
     if path is None:

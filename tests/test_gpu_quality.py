@@ -88,3 +88,42 @@ def test_quality_requires_a_solve(ws):
     import ctypes as C
 
     assert lib.pb_hip_quality(ws.handle, C.byref(q), C.byref(rep), None, None, None) == _abi.PB_ERR_INVALID
+
+
+def test_outcome_of_gpu_solve_matches_oracle_outcome(ws, oracle_mod):
+    # id-keyed outcome comparison (comparePhotoAlignmentOutcomes) of the GPU and the oracle solve, ids shuffled
+    from plabundle_amd import photo_alignment as PA
+
+    _say("case outcome comparison")
+    p = S.ring_problem(seed=9, num_cameras=16, num_points=600, max_views=6)
+    o = S.ring_options(PO.SolveOptions(), max_iterations=10)
+    pa = PA.PhotoAlignmentProblem(p, [f"IMG_{i:03d}" for i in range(p.num_cameras)],
+                                  [f"tie-{i}" for i in range(p.num_tracks)])
+    gpu = PA.makePhotoAlignmentOutcome(pa, Solver().solve(p, o, ws, precision=PO.Precision.Fp64, quality=True))
+    orc = oracle_mod.Oracle(p, threads=4)
+    rep = orc.solve(PO.to_c(o))
+    st = orc.state()
+    q = orc.quality(PO.quality_to_c(o))
+    ref = PA.PhotoAlignmentOutcome(
+        terminationStatus=rep["solve_status"], terminationReason="oracle", requestedBackend=gpu.requestedBackend,
+        usedBackend=gpu.usedBackend, solutionUsable=bool(q["solution_usable"]),
+        meanRmsBefore=q["mean_rms_before"], meanRmsAfter=q["mean_rms_after"], camera_ids=pa.cameraIds,
+        cam_rotation=st["cam_rotation"], cam_center=st["cam_center"], cam_intrinsics=st["cam_intrinsics"],
+        cam_image_size=p.cam_image_size, track_ids=pa.trackIds, track_point=st["track_point"],
+        track_valid=q["track_valid"], track_rms_before=q["track_rms_before"], track_rms_after=q["track_rms_after"])
+    assert PA.validatePhotoAlignmentOutcome(ref) == (True, "")
+    perm_c = np.random.default_rng(1).permutation(p.num_cameras)
+    perm_t = np.random.default_rng(2).permutation(p.num_tracks)
+    shuffled = PA.PhotoAlignmentOutcome(**{k: v for k, v in gpu.__dict__.items()})
+    shuffled.camera_ids = [gpu.camera_ids[i] for i in perm_c]
+    for name in ("cam_rotation", "cam_center", "cam_intrinsics", "cam_image_size"):
+        setattr(shuffled, name, getattr(gpu, name)[perm_c])
+    shuffled.track_ids = [gpu.track_ids[i] for i in perm_t]
+    for name in ("track_point", "track_valid", "track_rms_before", "track_rms_after"):
+        setattr(shuffled, name, getattr(gpu, name)[perm_t])
+    tol = PA.PhotoAlignmentComparisonTolerance(rms=1e-8, cameraCenterMeters=1e-7, cameraRotationDegrees=1e-5,
+                                               pointMeters=1e-6, requireTerminationReasonMatch=False)
+    c = PA.comparePhotoAlignmentOutcomes(ref, shuffled, tol)
+    _say(f"outcome: center {c['maximum_camera_center_difference_meters']:.2e} m, rotation "
+         f"{c['maximum_camera_rotation_difference_degrees']:.2e} deg, point {c['maximum_point_difference_meters']:.2e} m")
+    assert c["comparable"] and c["equivalent"], c["message"]
