@@ -262,6 +262,23 @@ def main():
                    "gradient_entries": int(g.size), "nonzero": int(np.count_nonzero(g))}
         log(f"adjoint: {adjoint}")
 
+    # post-solve quality on the device after a mixed solve (finalizeBundleAdjustResult + image quality gate)
+    quality = None
+    if not args.no_adjoint:
+        r = solve(PO.Precision.Mixed, False)[0]
+        qo = PO.quality_to_c(S.aerial_options(PO.SolveOptions()))
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            q = ws.quality(qo)
+            times.append(time.perf_counter() - t0)
+        quality = {"seconds": round(min(times), 5), "total_tracks": q["total_tracks"],
+                   "optimized_tracks": q["optimized_tracks"], "valid_track_ratio": round(q["valid_track_ratio"], 5),
+                   "mean_rms_before_px": round(q["mean_rms_before"], 5), "mean_rms_after_px": round(q["mean_rms_after"], 5),
+                   "filter_threshold_px": round(q["filter_threshold"], 5),
+                   "quality_gate_rejected": bool(q["quality_gate_rejected"])}
+        log(f"quality: {quality}")
+
     mixed_t = sum(r.report["solve_seconds"] for r in mixed) / len(mixed)
     mixed_it = sum(r.report["iterations"] for r in mixed) / len(mixed)
     fp64_t = sum(r.report["solve_seconds"] for r in fp64) / len(fp64) if fp64 else None
@@ -368,6 +385,7 @@ def main():
         "fp64_inexact_rel_cost_err_vs_oracle": rel_oracle64i,
         "speedup_mixed_vs_fp64_inexact": round(fp64i_t / mixed_t, 4) if fp64i_t else None,
         "adjoint": adjoint,
+        "quality": quality,
         "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
