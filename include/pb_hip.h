@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define PB_ABI_VERSION 1
+#define PB_ABI_VERSION 2
 #define PB_INTRINSIC_COUNT 13
 
 typedef enum pb_status {
@@ -97,9 +97,19 @@ typedef enum pb_precision {
     PB_PRECISION_MIXED = 1  /* FP32 Jacobians/Schur/PCG; FP64 cost, acceptance, point inverses */
 } pb_precision;
 
+/* Uncertainty form of a control point (plabundle::ControlPointConstraint::Uncertainty,
+ * include/plabundle/constraints.h:81-87). */
+typedef enum pb_control_uncertainty {
+    PB_CONTROL_ISOTROPIC_SIGMA = 0,  /* sigma (m) on every axis */
+    PB_CONTROL_COVARIANCE = 1,       /* row-major 3x3 covariance (m^2) */
+    PB_CONTROL_SQRT_INFORMATION = 2  /* row-major 3x3 square-root information (1/m) */
+} pb_control_uncertainty;
+
 /* Problem, structure-of-arrays, observations grouped by track.
  * Mirrors plabundle::Problem (include/plabundle/problem.h:37-54) restricted to
- * frame-pinhole cameras without rigs or survey constraints. */
+ * frame-pinhole cameras without rigs, with the per-track survey constraints of
+ * plabundle::Track (control points and LiDAR point-to-plane constraints,
+ * include/plabundle/constraints.h:136-141) as flat arrays keyed by track. */
 typedef struct pb_problem_view {
     int32_t num_cameras;
     const double* cam_rotation;     /* [num_cameras*9] camera-to-world, row-major */
@@ -127,6 +137,24 @@ typedef struct pb_problem_view {
     int32_t gauge_anchor_camera;    /* reference gauge anchor (-1 = none) */
     int32_t gauge_scale_camera;     /* reference gauge scale camera (-1 = none) */
     double gauge_baseline;          /* reference gauge baseline (0 = none) */
+
+    /* Control points (ControlPointConstraint, constraints.h:73-88). A track may carry several; within a track
+     * they are taken in array order. Any of the optional arrays may be NULL. */
+    int32_t num_control_points;
+    const int32_t* cp_track;        /* [n] track index */
+    const double* cp_point;         /* [n*3] surveyed position (solver frame) */
+    const double* cp_sigma;         /* [n] sigmaMeters of an isotropic constraint, or NULL = 1 */
+    const double* cp_weight;        /* [n] weight (> 0), or NULL = 1 */
+    const int32_t* cp_uncertainty;  /* [n] pb_control_uncertainty, or NULL = all isotropic */
+    const double* cp_matrix;        /* [n*9] covariance or square-root information, or NULL when all isotropic */
+
+    /* LiDAR point-to-plane constraints (LaserPlaneConstraint, constraints.h:39-46): the residual is
+     * sqrt(w) * normal . (X - point), the normal taken as given (not normalised). */
+    int32_t num_laser_planes;
+    const int32_t* lp_track;        /* [n] track index */
+    const double* lp_point;         /* [n*3] point on the plane */
+    const double* lp_normal;        /* [n*3] plane normal (non-zero) */
+    const double* lp_weight;        /* [n] weight (> 0), or NULL = 1 */
 } pb_problem_view;
 
 /* Solve options. Mirrors the fields of plabundle::Options
@@ -138,7 +166,9 @@ typedef struct pb_solve_options {
     double robust_loss_scale;            /* imageRobustLossScalePixels = 1 */
     double armijo_coefficient;           /* referenceArmijoCoefficient = 1e-3 */
     int32_t line_search_steps;           /* referenceLineSearchSteps = 34 */
-    int32_t use_online_schur;            /* useReferenceOnlineSchur; this backend implements the online path */
+    int32_t use_online_schur;            /* useReferenceOnlineSchur: 1 = online point Schur (path B) when the
+                                            problem allows it (no laser planes, canUseReferenceOnlineSchur,
+                                            ReferenceSchur.cpp:667-673); 0 = assembled damped Schur (path A) */
     double max_initial_track_rms;        /* maxInitialTrackRms = 100 */
     int32_t refine_camera_pose;          /* refineCameraPose = true */
     uint32_t shared_intrinsic_mask;      /* bit i: slot i refined per group (sharedIntrinsicParameterEnabled) */
@@ -170,6 +200,13 @@ typedef struct pb_solve_options {
                                             0.1 for mixed) */
     double step_rms_tolerance;           /* convergence when both step RMS values are at or below it
                                             (0 = 1e-6, BundleAdjustPlaMatrix.cpp:446-466) */
+
+    /* ConstraintOptions (options.h:122-135) of the constraints this backend implements. A constraint kind is
+     * enabled when the problem carries at least one (Conversion.cpp:103-106). */
+    double laser_plane_weight;           /* laserPlaneWeight = 1 */
+    double laser_huber_delta;            /* laserHuberDeltaMeters = 0.2 (scaled by sqrt(weight) per plane) */
+    double control_point_weight;         /* controlPointWeight = 1 */
+    double control_point_huber_delta;    /* controlPointHuberDeltaMeters = 0.2 (on the whitened residual) */
 } pb_solve_options;
 
 /* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */
@@ -192,7 +229,7 @@ typedef struct pb_solve_report {
     int32_t intrinsic_block_count;
     int32_t track_block_count;
     int32_t primary_block_count;
-    int32_t online_schur_used;
+    int32_t online_schur_used;           /* 1: path B (online point Schur); 0: path A (assembled damped Schur) */
     int32_t mixed_precision_used;
     int32_t gauge_anchor_camera;         /* after normalisation */
     int32_t gauge_scale_camera;
@@ -265,8 +302,9 @@ pb_status pb_hip_evaluate_cost(pb_ctx* ctx, const pb_solve_options* options, dou
  * order; unusable or inactive observations get 0). Implicit-function theorem on the optimality condition:
  * H lambda = dl/dtheta is solved in FP64 through the online point Schur, with H = sum J^T M J and
  * M = d(w_n r)/dr the observation's exact weight matrix (its eigenvalue along r clamped at zero), then
- * dl/dy_i = M_i J_i lambda. Second derivatives of the projection are not included. dl_dcontrol must be NULL:
- * this backend has no control-point constraints (PB_ERR_UNSUPPORTED otherwise). */
+ * dl/dy_i = M_i J_i lambda. Second derivatives of the projection are not included. dl_dcontrol (may be NULL)
+ * receives dl/dP of every control point's surveyed position [num_control_points*3] (0 for a control point whose
+ * track is not eliminated or whose constraint is skipped); laser planes enter H like control points. */
 pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* dl_dpoint, double* dl_dobs,
                           double* dl_dcontrol);
 

@@ -68,6 +68,8 @@ def lib():
     L.oracle_linearize.argtypes = [_dp, _dp, _dp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                    C.c_int, C.c_double, C.c_int, C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp]
     L.oracle_linearize.restype = C.c_int
+    L.oracle_control_whitening.argtypes = [C.c_int, C.c_double, _dp, _dp]
+    L.oracle_control_whitening.restype = C.c_int
     _lib = L
     return L
 
@@ -108,6 +110,10 @@ def default_options() -> _abi.SolveOptionsC:
     o.low_order_distortion_scale = 1.0
     o.pcg_model_eta = -1.0
     o.step_rms_tolerance = 0.0
+    o.laser_plane_weight = 1.0
+    o.laser_huber_delta = 0.2
+    o.control_point_weight = 1.0
+    o.control_point_huber_delta = 0.2
     return o
 
 
@@ -261,3 +267,11 @@ def linearize(R, Cc, slots, conv, X, u, v, weight=1.0, scale=1.0, loss=0, loss_s
         return None
     return {"residual": r, "Jp": Jp.reshape(2, 3), "Jpose": Jpose.reshape(2, 6), "Jint": Jint.reshape(2, 13),
             "weight": w.value, "cost": c.value}
+
+
+def control_whitening(kind, sigma=1.0, matrix=None):
+    """Whitening matrix A of one control point (residual sqrt(w) A (X - P)), or None when it is rejected."""
+    m = None if matrix is None else np.ascontiguousarray(matrix, dtype=np.float64).reshape(9)
+    A = np.zeros(9)
+    ok = lib().oracle_control_whitening(int(kind), float(sigma), _p(m), _p(A))
+    return A.reshape(3, 3) if ok else None

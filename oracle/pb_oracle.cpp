@@ -81,11 +81,96 @@ namespace
         double u = 0.0, v = 0.0, weight = 1.0, scale = 1.0;
     };
 
+    // Survey constraints of one track (constraints.h:39-46, 73-88). Control points keep their whitening
+    // matrix A (residual sqrt(w) A (X - P)); laser planes their plane point and (unnormalised) normal.
+    struct ControlPoint
+    {
+        int index = -1;  // position in the caller's control-point list
+        Vec3 P{};
+        Mat3 A{};
+        double weight = 1.0;
+    };
+
+    struct LaserPlane
+    {
+        Vec3 p{};
+        Vec3 n{};
+        double weight = 1.0;
+    };
+
     struct Track
     {
         Vec3 X{};
         std::vector<Observation> obs;
+        std::vector<LaserPlane> planes;
+        std::vector<ControlPoint> controls;
     };
+
+    // makeControlPointWhitening (src/control_point.cpp:91-165): isotropic -> diag(1/sigma); covariance S = L L^T
+    // -> A = L^-1; square-root information -> A as given, with A^T A required positive definite. The Cholesky
+    // test of the reference (every pivot L_ii^2 above a threshold) is restated with threshold 0.
+    bool cholesky3(const Mat3& m, Mat3* L)
+    {
+        for (double v : m)
+            if (!std::isfinite(v)) return false;
+        *L = Mat3{};
+        for (int i = 0; i < 3; ++i)
+        {
+            for (int j = 0; j <= i; ++j)
+            {
+                double s = 0.5 * (m[i * 3 + j] + m[j * 3 + i]);
+                for (int k = 0; k < j; ++k) s -= (*L)[i * 3 + k] * (*L)[j * 3 + k];
+                if (i == j)
+                {
+                    if (!(s > 0.0) || !std::isfinite(s)) return false;
+                    (*L)[i * 3 + i] = std::sqrt(s);
+                }
+                else
+                {
+                    (*L)[i * 3 + j] = s / (*L)[j * 3 + j];
+                }
+            }
+        }
+        return true;
+    }
+
+    bool controlWhitening(int kind, double sigma, const double* m, Mat3* A)
+    {
+        *A = Mat3{};
+        if (kind == PB_CONTROL_ISOTROPIC_SIGMA)
+        {
+            if (!std::isfinite(sigma) || sigma <= 0.0) return false;
+            (*A)[0] = (*A)[4] = (*A)[8] = 1.0 / sigma;
+            return true;
+        }
+        if (!m) return false;
+        Mat3 M;
+        std::copy(m, m + 9, M.begin());
+        if (kind == PB_CONTROL_COVARIANCE)
+        {
+            Mat3 L;
+            if (!cholesky3(M, &L)) return false;
+            // A = L^-1 (lower triangular)
+            (*A)[0] = 1.0 / L[0];
+            (*A)[4] = 1.0 / L[4];
+            (*A)[8] = 1.0 / L[8];
+            (*A)[3] = -L[3] * (*A)[0] / L[4];
+            (*A)[7] = -L[7] * (*A)[4] / L[8];
+            (*A)[6] = -(L[6] * (*A)[0] + L[7] * (*A)[3]) / L[8];
+            return true;
+        }
+        if (kind == PB_CONTROL_SQRT_INFORMATION)
+        {
+            Mat3 I{}, L;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k) I[i * 3 + j] += M[k * 3 + i] * M[k * 3 + j];
+            if (!cholesky3(I, &L)) return false;
+            *A = M;
+            return true;
+        }
+        return false;
+    }
 
     // --- observation sanitising (BundleAdjustValidation.cpp:304-326) -------------------------
     double sanitizedWeight(const Observation& o)
@@ -449,6 +534,60 @@ namespace
         return std::isfinite(c);
     }
 
+    // --- survey constraints (BundleAdjustPlaMatrixConstraints.cpp:17-29, 66-116) ----------------
+    // Residual r = J (X - target) with J = sqrt(w) A (control point, 3 rows) or sqrt(w) n^T (laser plane, 1
+    // row), robustified by Huber on |r|^2: delta_cp for control points, delta_l * sqrt(w) for laser planes.
+    struct ConstraintLin
+    {
+        int rows = 0;
+        double r[3] = {0.0, 0.0, 0.0};
+        double J[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double w = 0.0;
+        double cost = 0.0;
+    };
+
+    bool finishConstraint(ConstraintLin* c, double delta)
+    {
+        double s = 0.0;
+        for (int i = 0; i < c->rows; ++i) s += c->r[i] * c->r[i];
+        double cost = 0.0, w = 0.0;
+        if (!robustLoss(s, PB_LOSS_HUBER, delta, &cost, &w)) return false;
+        c->w = w;
+        c->cost = cost;
+        return std::isfinite(w) && std::isfinite(cost);
+    }
+
+    bool linearizeControl(const ControlPoint& k, const Vec3& X, const pb_solve_options& o, ConstraintLin* c)
+    {
+        const double w = o.control_point_weight * k.weight;
+        if (!(w > 0.0)) return false;
+        const double sw = std::sqrt(w);
+        *c = ConstraintLin{};
+        c->rows = 3;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+            {
+                c->J[i * 3 + j] = sw * k.A[i * 3 + j];
+                c->r[i] += c->J[i * 3 + j] * (X[j] - k.P[j]);
+            }
+        return finishConstraint(c, o.control_point_huber_delta);
+    }
+
+    bool linearizePlane(const LaserPlane& k, const Vec3& X, const pb_solve_options& o, ConstraintLin* c)
+    {
+        const double w = o.laser_plane_weight * k.weight;
+        if (!(w > 0.0)) return false;
+        const double sw = std::sqrt(w);
+        *c = ConstraintLin{};
+        c->rows = 1;
+        for (int j = 0; j < 3; ++j)
+        {
+            c->J[j] = sw * k.n[j];
+            c->r[0] += c->J[j] * (X[j] - k.p[j]);
+        }
+        return finishConstraint(c, o.laser_huber_delta * sw);
+    }
+
     // --- problem, options, active problem -----------------------------------------------------
     struct Options
     {
@@ -459,7 +598,10 @@ namespace
         int gaugeScale = -1;
         double gaugeBaseline = 0.0;
         int gaugePolicy = PB_GAUGE_AUTO_ANCHOR;
-        bool referenceParam() const { return o.use_online_schur != 0; }
+        bool controlPoints = false;  // enableControlPointConstraints (Conversion.cpp:106)
+        bool laserPlanes = false;    // enableLaserPlaneConstraints (Conversion.cpp:103)
+        // useReferencePointParameterization (Assembly.cpp:24-29)
+        bool referenceParam() const { return o.use_online_schur != 0 && !controlPoints && !laserPlanes; }
         bool enabled(int slot) const { return (o.shared_intrinsic_mask >> slot) & 1u; }
     };
 
@@ -507,6 +649,7 @@ namespace
         int gaugePolicy = PB_GAUGE_AUTO_ANCHOR;
         int gaugeAnchor = -1, gaugeScale = -1;
         double gaugeBaseline = 0.0;
+        std::string constraintError;  // validateProblem's verdict on the survey constraints (problem.cpp:241-259)
     };
 
     bool hasSharedIntrinsics(const Options& opt)
@@ -516,9 +659,68 @@ namespace
         return false;
     }
 
+    // hasUsableTrackObservations (BundleAdjustValidation.cpp:94-106): two distinct cameras with a finite
+    // pixel and a positive weight (the camera index is only checked for being non-negative).
+    bool usableForGauge(const Track& tr)
+    {
+        std::set<int> cams;
+        for (const Observation& ob : tr.obs)
+            if (ob.cam >= 0 && std::isfinite(ob.u) && std::isfinite(ob.v) && std::isfinite(ob.weight) &&
+                ob.weight > 0.0)
+                cams.insert(ob.cam);
+        return cams.size() >= 2;
+    }
+
+    // hasCompleteControlPointGauge (BundleAdjustValidation.cpp:181-240): three non-collinear targets.
+    bool completeControlGauge(const Problem& pb)
+    {
+        std::vector<Vec3> pts;
+        for (const Track& tr : pb.tracks)
+        {
+            if (tr.controls.empty() || !usableForGauge(tr)) continue;
+            for (const ControlPoint& k : tr.controls) pts.push_back(k.P);
+        }
+        for (std::size_t a = 0; a < pts.size(); ++a)
+            for (std::size_t b = a + 1; b < pts.size(); ++b)
+            {
+                const Vec3 ab{pts[b][0] - pts[a][0], pts[b][1] - pts[a][1], pts[b][2] - pts[a][2]};
+                const double ab2 = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+                if (ab2 <= 1e-16) continue;
+                for (std::size_t c = b + 1; c < pts.size(); ++c)
+                {
+                    const Vec3 ac{pts[c][0] - pts[a][0], pts[c][1] - pts[a][1], pts[c][2] - pts[a][2]};
+                    const double ac2 = ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
+                    const Vec3 x{ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2],
+                                 ab[0] * ac[1] - ab[1] * ac[0]};
+                    const double x2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+                    if (ac2 > 1e-16 && x2 > 1e-12 * ab2 * ac2) return true;
+                }
+            }
+        return false;
+    }
+
+    // hasControlPointBaselineToFixedCamera (BundleAdjustValidation.cpp:147-179).
+    bool controlBaselineToFixed(const Problem& pb, const std::set<int>& fixed)
+    {
+        for (const Track& tr : pb.tracks)
+        {
+            if (tr.controls.empty() || !usableForGauge(tr)) continue;
+            for (const ControlPoint& k : tr.controls)
+                for (int f : fixed)
+                {
+                    const Vec3& c = pb.cams[f].C;
+                    if (!std::isfinite(c[0]) || !std::isfinite(c[1]) || !std::isfinite(c[2])) continue;
+                    const double d = (k.P[0] - c[0]) * (k.P[0] - c[0]) + (k.P[1] - c[1]) * (k.P[1] - c[1]) +
+                                     (k.P[2] - c[2]) * (k.P[2] - c[2]);
+                    if (d > 1e-16) return true;
+                }
+        }
+        return false;
+    }
+
     // validateAndNormalizeBundleAdjustOptions gauge part (BundleAdjustValidation.cpp:767-896) for
-    // image-only problems: no pose priors, control points or scale bars, so the problem has
-    // neither absolute pose nor absolute scale.
+    // problems without pose priors or scale bars: control points (three non-collinear targets) give
+    // absolute pose and scale; a target away from a fixed camera's centre gives absolute scale.
     std::string normalizeGauge(const Problem& pb, Options* opt)
     {
         const int n = static_cast<int>(pb.cams.size());
@@ -546,17 +748,23 @@ namespace
         opt->gaugeBaseline = pb.gaugeBaseline;
         opt->gaugePolicy = pb.gaugePolicy;
         if (!opt->o.refine_camera_pose || pb.gaugePolicy == PB_GAUGE_CALLER_MANAGED || n == 0) return {};
+        const bool complete = opt->controlPoints && completeControlGauge(pb);
+        const bool absolutePose = complete;
+        bool absoluteScale = complete || (opt->controlPoints && controlBaselineToFixed(pb, fixed));
         if (pb.gaugePolicy == PB_GAUGE_REQUIRE_EXPLICIT)
         {
-            if (fixed.size() >= 2) return {};
-            return "unsupported: incomplete gauge (fix two cameras)";
+            if ((absolutePose && absoluteScale) || fixed.size() >= 2 || (fixed.size() == 1 && absoluteScale))
+                return {};
+            return "unsupported: incomplete gauge (fix two cameras, or one camera with absolute scale)";
         }
+        if (absolutePose && absoluteScale) return {};
         if (fixed.empty())
         {
             opt->fixedCameras.push_back(0);
             fixed.insert(0);
+            absoluteScale = absoluteScale || (opt->controlPoints && controlBaselineToFixed(pb, fixed));
         }
-        if (fixed.size() < 2)
+        if (!absoluteScale && fixed.size() < 2)
         {
             const int anchor = *fixed.begin();
             const Vec3 a = pb.cams[anchor].C;
@@ -1143,6 +1351,34 @@ namespace
                         }
                     }
                 }
+                // point-only survey constraints: laser planes (path A, Assembly.cpp:656-671), then control
+                // points (ReferenceSchur.cpp:331-346, accumulatePointOnlyConstraint :241-259); the cost counts
+                // for every active track, the normal terms only for an eliminated point; a constraint whose
+                // linearisation fails is skipped
+                auto addConstraint = [&](const ConstraintLin& c)
+                {
+                    cost += c.cost;
+                    if (!elim) return;
+                    for (int col = 0; col < kE; ++col)
+                        for (int r = 0; r < c.rows; ++r)
+                        {
+                            const double pv = c.J[r * kE + col];
+                            bp[col] -= c.w * pv * c.r[r];
+                            for (int k = 0; k < kE; ++k) Hpp[col * kE + k] += c.w * pv * c.J[r * kE + k];
+                        }
+                };
+                if (opt.laserPlanes)
+                    for (const LaserPlane& k : o.pb.tracks[t].planes)
+                    {
+                        ConstraintLin c;
+                        if (linearizePlane(k, X, opt.o, &c)) addConstraint(c);
+                    }
+                if (opt.controlPoints)
+                    for (const ControlPoint& k : o.pb.tracks[t].controls)
+                    {
+                        ConstraintLin c;
+                        if (linearizeControl(k, X, opt.o, &c)) addConstraint(c);
+                    }
                 if (!elim) continue;
                 // eliminatePoint (ReferenceSchur.cpp:350-490); damping form Hpp_ii *= (1 + lambda)
                 Mat3 damped = Hpp;
@@ -1620,6 +1856,20 @@ namespace
                     }
                     c += oc;
                 }
+                if (bad[th]) break;
+                // survey constraint costs (Assembly.cpp:656-690)
+                if (o.opt.laserPlanes)
+                    for (const LaserPlane& k : o.pb.tracks[t].planes)
+                    {
+                        ConstraintLin cl;
+                        if (linearizePlane(k, st.points[t], o.opt.o, &cl)) c += cl.cost;
+                    }
+                if (o.opt.controlPoints)
+                    for (const ControlPoint& k : o.pb.tracks[t].controls)
+                    {
+                        ConstraintLin cl;
+                        if (linearizeControl(k, st.points[t], o.opt.o, &cl)) c += cl.cost;
+                    }
             }
             part[th] = c;
         }
@@ -1644,9 +1894,185 @@ namespace
         std::snprintf(r->message, sizeof(r->message), "%s", m.c_str());
     }
 
+    // canUseReferenceOnlineSchur (ReferenceSchur.cpp:667-673) for frame cameras without rigs, laser ranges,
+    // scale bars, pose priors or a camera plane.
+    bool onlineSchur(const Oracle& o)
+    {
+        return o.opt.o.use_online_schur != 0 && o.act.primaryBlockCount > 0 && !o.opt.laserPlanes;
+    }
+
+    void finishReport(Oracle& o, const State& st, double current, bool converged, bool cancelled, bool failed,
+                      int accepted, int rejected, int iterations, std::chrono::steady_clock::time_point t0)
+    {
+        pb_solve_report& rep = o.report;
+        rep.solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        rep.accepted_steps = accepted;
+        rep.rejected_steps = rejected;
+        rep.iterations = iterations;
+        rep.final_cost = current;
+        rep.online_schur_used = onlineSchur(o) ? 1 : 0;
+        if (cancelled)
+        {
+            o.state = o.init;
+            rep.solve_status = PB_SOLVE_CANCELLED;
+            rep.solution_usable = 0;
+            setMessage(&rep, "cancelled; no intermediate solution published");
+            return;
+        }
+        if (failed)
+        {
+            o.state = o.init;
+            return;
+        }
+        rep.converged = converged;
+        rep.solve_status = converged ? PB_SOLVE_SUCCESS : PB_SOLVE_NO_CONVERGENCE;
+        rep.solution_usable = std::isfinite(current);
+        setMessage(&rep, converged ? "converged" : "iteration budget or line search exhausted");
+        o.state = st;
+    }
+
+    // optimizePointsWithPlaMatrix, assembled branch (path A, BundleAdjustPlaMatrix.cpp:239-556 with the online
+    // Schur off): one linearisation per outer step; after a rejected step the same equations are solved again
+    // with more damping; the directional derivative comes from an objective probe at alpha = 1e-4 (:473-486);
+    // any decrease is accepted (:517-519); a small step ends the solve without being applied (:451-465).
+    // Unpinned choices (PlaMatrix's solveDampedSchurComplement is external): the damped system is the online
+    // path's (Hpp and reduced diagonals times 1 + lambda), and a singular damped point block or a reduced
+    // system that is not positive definite is a linear-solver exception, which this path rethrows (:357-364).
+    void solveAssembled(Oracle& o, const volatile int32_t* cancel)
+    {
+        pb_solve_report& rep = o.report;
+        const Active& a = o.act;
+        const auto t0 = std::chrono::steady_clock::now();
+        State st = o.init;
+        double current = std::numeric_limits<double>::quiet_NaN();
+        double damping = 0.0;
+        int accepted = 0, rejected = 0, iterations = 0;
+        bool converged = a.primaryBlockCount == 0 && a.trackBlockCount == 0;
+        bool cancelled = false, failed = false;
+        auto isCancelled = [&]() { return cancel && *cancel != 0; };
+        auto raise = [&]() { damping = damping == 0.0 ? 1e-3 : damping * 10.0; };
+        const double tol = o.opt.o.step_rms_tolerance > 0.0 ? o.opt.o.step_rms_tolerance : 1e-6;
+        rep.linear_tolerance = 1e-12;
+        try
+        {
+            if (converged)
+            {
+                current = evaluateObjective(o, st);
+                rep.initial_cost = current;
+                ++rep.objective_evaluations;
+            }
+            const int budget = std::max(1, o.opt.o.max_iterations);
+            while (iterations < budget && !converged)
+            {
+                if (isCancelled())
+                {
+                    cancelled = true;
+                    break;
+                }
+                // buildNormalEquations at the current state (:249-269)
+                const auto ta = std::chrono::steady_clock::now();
+                Reduced R = buildReduced(o, st, damping);
+                current = R.cost;
+                rep.assembly_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - ta).count();
+                ++rep.linearizations;
+                ++rep.objective_evaluations;
+                if (rep.linearizations == 1) rep.initial_cost = current;
+                bool relinearize = false;
+                bool first = true;
+                while (iterations < budget && !converged && !relinearize)
+                {
+                    if (isCancelled())
+                    {
+                        cancelled = true;
+                        break;
+                    }
+                    // the same linearisation point, damped with the current damping
+                    if (!first) R = buildReduced(o, st, damping);
+                    first = false;
+                    if (R.singular) throw std::runtime_error("singular damped point block");
+                    std::vector<double> primary;
+                    double rel = 0.0;
+                    const auto tl = std::chrono::steady_clock::now();
+                    if (!solveReduced(R, a.cameraBlockCount, &primary, &rel))
+                        throw std::runtime_error("reduced system not positive definite");
+                    rep.linear_solve_seconds +=
+                        std::chrono::duration<double>(std::chrono::steady_clock::now() - tl).count();
+                    ++iterations;
+                    if (!(rel <= 1e-12))
+                    {
+                        ++rejected;
+                        raise();
+                        continue;
+                    }
+                    std::vector<double> elim;
+                    double ddUnused = 0.0;
+                    if (!backSubstitute(o, primary, R, &elim, &ddUnused))
+                        throw std::runtime_error("singular damped point block");
+                    const StepRms rms = stepRms(o, st, primary, elim);
+                    if (rms.global <= tol && rms.point <= tol)
+                    {
+                        converged = true;
+                        continue;
+                    }
+                    const auto tobj = std::chrono::steady_clock::now();
+                    State trial = st;
+                    applyStep(o, primary, elim, &trial, 1e-4);
+                    const double probe = evaluateObjective(o, trial);
+                    ++rep.objective_evaluations;
+                    double dd = 0.0;
+                    if (std::isfinite(probe) && probe < current) dd = (current - probe) / 1e-4;
+                    double alpha = 1.0, cand = kInf;
+                    for (int ls = 0; ls < o.opt.o.line_search_steps; ++ls)
+                    {
+                        trial = st;
+                        applyStep(o, primary, elim, &trial, alpha);
+                        cand = evaluateObjective(o, trial);
+                        ++rep.objective_evaluations;
+                        const double limit = dd > 0.0 ? current - o.opt.o.armijo_coefficient * alpha * dd : current;
+                        if (std::isfinite(cand) && cand <= limit && cand < current) break;
+                        alpha *= 0.5;
+                    }
+                    rep.objective_seconds +=
+                        std::chrono::duration<double>(std::chrono::steady_clock::now() - tobj).count();
+                    if (std::getenv("PB_ORACLE_TRACE"))
+                        std::fprintf(stderr, "oracle A it=%d cost=%.10g cand=%.10g alpha=%g rms_g=%.3e rms_p=%.3e "
+                                     "dd=%.4e lambda=%g\n",
+                                     iterations, current, cand, alpha, rms.global, rms.point, dd, damping);
+                    if (std::isfinite(cand) && cand < current)
+                    {
+                        st = trial;
+                        current = cand;
+                        ++accepted;
+                        damping /= 10.0;
+                        // alpha is the accepted length, or half the last one tried when no length met Armijo
+                        converged = alpha * rms.global <= tol && alpha * rms.point <= tol;
+                        relinearize = !converged;
+                    }
+                    else
+                    {
+                        ++rejected;
+                        raise();
+                    }
+                }
+            }
+        }
+        catch (const std::exception& e)
+        {
+            failed = true;
+            rep.solve_status = PB_SOLVE_NUMERICAL_FAILURE;
+            setMessage(&rep, std::string("numerical failure: ") + e.what());
+        }
+        finishReport(o, st, current, converged, cancelled, failed, accepted, rejected, iterations, t0);
+    }
+
     // optimizePointsWithPlaMatrix, online Schur branch (BundleAdjustPlaMatrix.cpp:162-631).
     void solve(Oracle& o, const volatile int32_t* cancel)
     {
+        if (!onlineSchur(o))
+        {
+            solveAssembled(o, cancel);
+            return;
+        }
         pb_solve_report& rep = o.report;
         const Active& a = o.act;
         const auto t0 = std::chrono::steady_clock::now();
@@ -1783,30 +2209,7 @@ namespace
             rep.solve_status = PB_SOLVE_NUMERICAL_FAILURE;
             setMessage(&rep, std::string("numerical failure: ") + e.what());
         }
-        rep.solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        rep.accepted_steps = accepted;
-        rep.rejected_steps = rejected;
-        rep.iterations = iterations;
-        rep.final_cost = current;
-        rep.online_schur_used = 1;
-        if (cancelled)
-        {
-            o.state = o.init;
-            rep.solve_status = PB_SOLVE_CANCELLED;
-            rep.solution_usable = 0;
-            setMessage(&rep, "cancelled; no intermediate solution published");
-            return;
-        }
-        if (failed)
-        {
-            o.state = o.init;
-            return;
-        }
-        rep.converged = converged;
-        rep.solve_status = converged ? PB_SOLVE_SUCCESS : PB_SOLVE_NO_CONVERGENCE;
-        rep.solution_usable = std::isfinite(current);
-        setMessage(&rep, converged ? "converged (online Schur + Armijo)" : "iteration budget or line search exhausted");
-        o.state = st;
+        finishReport(o, st, current, converged, cancelled, failed, accepted, rejected, iterations, t0);
     }
 
     bool prepare(Oracle& o, const pb_solve_options* opts)
@@ -1824,6 +2227,25 @@ namespace
         {
             o.report.solve_status = PB_SOLVE_INVALID_INPUT;
             setMessage(&o.report, "invalid options");
+            return false;
+        }
+        if (!o.pb.constraintError.empty())
+        {
+            o.report.solve_status = PB_SOLVE_INVALID_INPUT;
+            setMessage(&o.report, o.pb.constraintError);
+            return false;
+        }
+        for (const Track& tr : o.pb.tracks)
+        {
+            o.opt.controlPoints = o.opt.controlPoints || !tr.controls.empty();
+            o.opt.laserPlanes = o.opt.laserPlanes || !tr.planes.empty();
+        }
+        // BundleAdjustValidation.cpp:589-595
+        if (o.opt.laserPlanes && (!std::isfinite(so.laser_plane_weight) || so.laser_plane_weight <= 0.0 ||
+                                  !std::isfinite(so.laser_huber_delta) || so.laser_huber_delta < 0.0))
+        {
+            o.report.solve_status = PB_SOLVE_INVALID_INPUT;
+            setMessage(&o.report, "invalid laser-plane weight or Huber delta");
             return false;
         }
         const std::string gerr = normalizeGauge(o.pb, &o.opt);
@@ -1966,7 +2388,62 @@ oracle_ctx* oracle_create(const pb_problem_view* v)
     pb.gaugeAnchor = v->gauge_anchor_camera;
     pb.gaugeScale = v->gauge_scale_camera;
     pb.gaugeBaseline = v->gauge_baseline;
+    // survey constraints, validated as validateProblem does (problem.cpp:241-259)
+    auto finite3 = [](const double* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); };
+    for (int i = 0; i < v->num_control_points; ++i)
+    {
+        const int t = v->cp_track[i];
+        ControlPoint k;
+        k.index = i;
+        std::copy(v->cp_point + 3 * i, v->cp_point + 3 * i + 3, k.P.begin());
+        k.weight = v->cp_weight ? v->cp_weight[i] : 1.0;
+        const int kind = v->cp_uncertainty ? v->cp_uncertainty[i] : PB_CONTROL_ISOTROPIC_SIGMA;
+        const double sigma = v->cp_sigma ? v->cp_sigma[i] : 1.0;
+        const double* m = v->cp_matrix ? v->cp_matrix + 9 * i : nullptr;
+        if (t < 0 || t >= v->num_tracks)
+        {
+            pb.constraintError = "control point track index out of range";
+            continue;
+        }
+        if (!finite3(v->cp_point + 3 * i) || !std::isfinite(k.weight) || k.weight <= 0.0 ||
+            !controlWhitening(kind, sigma, m, &k.A))
+        {
+            pb.constraintError = "track contains an invalid control-point constraint";
+            continue;
+        }
+        pb.tracks[t].controls.push_back(k);
+    }
+    for (int i = 0; i < v->num_laser_planes; ++i)
+    {
+        const int t = v->lp_track[i];
+        LaserPlane k;
+        std::copy(v->lp_point + 3 * i, v->lp_point + 3 * i + 3, k.p.begin());
+        std::copy(v->lp_normal + 3 * i, v->lp_normal + 3 * i + 3, k.n.begin());
+        k.weight = v->lp_weight ? v->lp_weight[i] : 1.0;
+        if (t < 0 || t >= v->num_tracks)
+        {
+            pb.constraintError = "laser plane track index out of range";
+            continue;
+        }
+        const double nn = std::hypot(k.n[0], k.n[1], k.n[2]);
+        if (!finite3(k.p.data()) || !finite3(k.n.data()) || !std::isfinite(nn) || nn <= 0.0 ||
+            !std::isfinite(k.weight) || k.weight <= 0.0)
+        {
+            pb.constraintError = "track contains an invalid laser-plane constraint";
+            continue;
+        }
+        pb.tracks[t].planes.push_back(k);
+    }
     return ctx;
+}
+
+// Whitening matrix of one control point (makeControlPointWhitening); returns 0 when it is rejected.
+int oracle_control_whitening(int kind, double sigma, const double* matrix, double* A)
+{
+    Mat3 W;
+    if (!controlWhitening(kind, sigma, matrix, &W)) return 0;
+    std::copy(W.begin(), W.end(), A);
+    return 1;
 }
 
 void oracle_destroy(oracle_ctx* ctx) { delete ctx; }

@@ -62,6 +62,18 @@ class ProblemView(C.Structure):
         ("gauge_anchor_camera", C.c_int32),
         ("gauge_scale_camera", C.c_int32),
         ("gauge_baseline", C.c_double),
+        ("num_control_points", C.c_int32),
+        ("cp_track", _ip),
+        ("cp_point", _dp),
+        ("cp_sigma", _dp),
+        ("cp_weight", _dp),
+        ("cp_uncertainty", _ip),
+        ("cp_matrix", _dp),
+        ("num_laser_planes", C.c_int32),
+        ("lp_track", _ip),
+        ("lp_point", _dp),
+        ("lp_normal", _dp),
+        ("lp_weight", _dp),
     ]
 
 
@@ -97,6 +109,10 @@ class SolveOptionsC(C.Structure):
         ("profile_kernels", C.c_int32),
         ("pcg_model_eta", C.c_double),
         ("step_rms_tolerance", C.c_double),
+        ("laser_plane_weight", C.c_double),
+        ("laser_huber_delta", C.c_double),
+        ("control_point_weight", C.c_double),
+        ("control_point_huber_delta", C.c_double),
     ]
 
 
@@ -431,4 +447,18 @@ def make_view(problem) -> tuple[ProblemView, list]:
     v.gauge_anchor_camera = int(problem.gauge_anchor_camera)
     v.gauge_scale_camera = int(problem.gauge_scale_camera)
     v.gauge_baseline = float(problem.gauge_baseline)
+    cp_track = arr(getattr(problem, "cp_track", None), np.int32)
+    v.num_control_points = 0 if cp_track is None else int(cp_track.size)
+    v.cp_track = ptr(cp_track, C.c_int32)
+    v.cp_point = ptr(arr(getattr(problem, "cp_point", None), np.float64), C.c_double)
+    v.cp_sigma = ptr(arr(getattr(problem, "cp_sigma", None), np.float64), C.c_double)
+    v.cp_weight = ptr(arr(getattr(problem, "cp_weight", None), np.float64), C.c_double)
+    v.cp_uncertainty = ptr(arr(getattr(problem, "cp_uncertainty", None), np.int32), C.c_int32)
+    v.cp_matrix = ptr(arr(getattr(problem, "cp_matrix", None), np.float64), C.c_double)
+    lp_track = arr(getattr(problem, "lp_track", None), np.int32)
+    v.num_laser_planes = 0 if lp_track is None else int(lp_track.size)
+    v.lp_track = ptr(lp_track, C.c_int32)
+    v.lp_point = ptr(arr(getattr(problem, "lp_point", None), np.float64), C.c_double)
+    v.lp_normal = ptr(arr(getattr(problem, "lp_normal", None), np.float64), C.c_double)
+    v.lp_weight = ptr(arr(getattr(problem, "lp_weight", None), np.float64), C.c_double)
     return v, keep

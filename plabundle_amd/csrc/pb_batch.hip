@@ -687,7 +687,7 @@ namespace pbk
             {
                 const double alpha = B.acceptAlpha[w];
                 const int t = P.elimTrack[id];
-                const double s = pbdev::pointScale(P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2]);
+                const double s = stepScale(P, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2]);
                 for (int i = 0; i < 3; ++i) P.pointX[3 * t + i] += s * alpha * Wk.dp[3 * id + i];
             }
         }

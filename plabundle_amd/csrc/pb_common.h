@@ -27,6 +27,30 @@ namespace pbk
         atomicOr(flags + index, 1);
     }
 
+    // Point scale of the step parameterisation (applyStep, Assembly.cpp:1213-1232).
+    __device__ __forceinline__ double stepScale(const Problem& P, double X0, double X1, double X2)
+    {
+        return P.refParam ? pbdev::pointScale(X0, X1, X2) : 1.0;
+    }
+
+    // Survey constraint k at point X (BundleAdjustPlaMatrixConstraints.cpp:66-116): residual r = J (X - target),
+    // Huber cost and IRLS weight. False when the loss is not finite; the reference then skips the constraint.
+    __device__ __forceinline__ bool constraintAt(const Problem& P, int k, double X0, double X1, double X2, double r[3],
+                                                 double& cost, double& w)
+    {
+        const int rows = P.consRows[k];
+        const double* J = P.consJ + 9 * k;
+        const double d0 = X0 - P.consTarget[3 * k], d1 = X1 - P.consTarget[3 * k + 1], d2 = X2 - P.consTarget[3 * k + 2];
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            r[i] = i < rows ? J[3 * i] * d0 + J[3 * i + 1] * d1 + J[3 * i + 2] * d2 : 0.0;
+            s += r[i] * r[i];
+        }
+        return pbdev::huber(s, P.consDelta[k], cost, w);
+    }
+
     // Robust cost of observation o at the trial states alpha_a (a < nAlpha): candidate cameras candCams
     // [NA*nCam] and the point moved by alpha_a * s * dp (applyStep, Assembly.cpp:1010-1247). Adds the cost to
     // acc[a]; a projection behind the camera or an invalid loss argument marks bad[a].
@@ -41,7 +65,7 @@ namespace pbk
         double d0 = 0.0, d1 = 0.0, d2 = 0.0;
         if (b >= 0)
         {
-            const double s = pbdev::pointScale(X0, X1, X2);
+            const double s = stepScale(P, X0, X1, X2);
             d0 = s * dp[3 * b];
             d1 = s * dp[3 * b + 1];
             d2 = s * dp[3 * b + 2];
@@ -65,6 +89,32 @@ namespace pbk
                 continue;
             }
             acc[a] += cost;
+        }
+    }
+
+    // Cost of survey constraint k at the trial states alpha_a: the point moves by alpha_a * s * dp. A constraint
+    // whose loss is not finite adds nothing (the reference skips it).
+    template <int NA>
+    __device__ __forceinline__ void trialConstraint(const Problem& P, const double* dp, int k, const double* al,
+                                                    int nAlpha, double* acc)
+    {
+        const int t = P.consPoint[k];
+        const int b = P.pointElim[t];
+        const double X0 = P.pointX[3 * t], X1 = P.pointX[3 * t + 1], X2 = P.pointX[3 * t + 2];
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+        if (b >= 0)
+        {
+            const double s = stepScale(P, X0, X1, X2);
+            d0 = s * dp[3 * b];
+            d1 = s * dp[3 * b + 1];
+            d2 = s * dp[3 * b + 2];
+        }
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+        {
+            if (a >= nAlpha) break;
+            double r[3], cost, w;
+            if (constraintAt(P, k, X0 + al[a] * d0, X1 + al[a] * d1, X2 + al[a] * d2, r, cost, w)) acc[a] += cost;
         }
     }
 }  // namespace pbk

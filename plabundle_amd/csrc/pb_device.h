@@ -1,9 +1,11 @@
 // pb_device.h - device-side frame-camera model and per-observation linearisation for gfx950.
 //
 // One observation of a world point X by a frame camera (R camera-to-world, centre C) gives the
-// residual r = pi(X) - (u, v), the point Jacobian Jp (2x3, scaled by the point scale of the
-// online-Schur parameterisation), the pose Jacobian (2x6: rotation in the D = diag(1,-1,-1) target
-// frame, world centre) and the Jacobian of the group's active intrinsic slots. The math follows
+// residual r = pi(X) - (u, v), the point Jacobian Jp (2x3), the pose Jacobian (2x6: rotation, world
+// centre) and the Jacobian of the group's active intrinsic slots. Two parameterisations
+// (Assembly.cpp:24-29): the reference one of the online Schur without survey constraints (rotation in
+// the D = diag(1,-1,-1) target frame, Euler retraction, point step scaled by s/atan s) and the default
+// one (world-frame left rotation increment, Exp retraction, unscaled point step). The math follows
 // BundleAdjustPlaMatrixProjection.cpp:176-285 (residual, Jacobian layout, weights) and
 // CameraState.cpp:387-481 (13-slot intrinsic layout and the focal/aspect chain rule).
 #pragma once
@@ -218,7 +220,7 @@ namespace pbdev
     template <int KM>
     PB_HD bool linearize(const Camera& c, double X0, double X1, double X2, double ou, double ov,
                                               double ow, int loss, double lscale, int nActive,
-                                              const int8_t* activeSlots, Linearization& L)
+                                              const int8_t* activeSlots, Linearization& L, bool refParam = true)
     {
         const double d0 = X0 - c.C[0], d1 = X1 - c.C[1], d2 = X2 - c.C[2];
         const double zc = c.R[2] * d0 + c.R[5] * d1 + c.R[8] * d2;
@@ -251,8 +253,9 @@ namespace pbdev
         {
             const double* p = jp + 3 * a;
             double* J = L.Jc + 6 * a;
-            // rotation in the target frame: J_rot = -Jp [d]x D
-            J[0] = -p[1] * d2 + p[2] * d1;
+            // reference: rotation in the target frame, J_rot = -Jp [d]x D; default: world-frame left
+            // increment, J_rot = Jp [d]x (the two differ in the sign of the first column)
+            J[0] = refParam ? -p[1] * d2 + p[2] * d1 : p[1] * d2 - p[2] * d1;
             J[1] = -p[0] * d2 + p[2] * d0;
             J[2] = p[0] * d1 - p[1] * d0;
             if (c.tangent)
@@ -268,7 +271,7 @@ namespace pbdev
                 J[5] = -p[2];
             }
         }
-        const double ps = pointScale(X0, X1, X2);
+        const double ps = refParam ? pointScale(X0, X1, X2) : 1.0;
 #pragma unroll
         for (int i = 0; i < 6; ++i) L.Jp[i] = jp[i] * ps;
         // fixed trip count so that Ji stays in registers (a runtime bound would index it dynamically)
@@ -324,6 +327,17 @@ namespace pbdev
         q[2] = sw * (1.0 - d) + sm * d;
     }
 
+    // Symmetric root of a survey constraint's exact weight matrix d(w r)/dr = w I + 4 rho''(s) r r^T (Huber on
+    // s = |r|^2, 1 or 3 rows): Q = a I + (b - a) r r^T / s with a = sqrt(w), b = sqrt(max(0, w + 4 rho'' s))
+    // (zero along r on the linear branch). Returns the pair (a, b - a) scaled into q[0], q[1].
+    PB_HD void constraintWeightRoot(double s, double w, double delta, double q[2])
+    {
+        const double a = sqrt(fmax(w, 0.0));
+        const double b = sqrt(fmax(0.0, w + 4.0 * robustCurvature(s, 1, delta) * s));
+        q[0] = a;
+        q[1] = s > 0.0 ? (b - a) / s : 0.0;
+    }
+
     // J <- Q J for every Jacobian row pair, weight 1: the normal equations of the transformed rows are
     // sum J^T M J.
     template <int KM>
@@ -362,6 +376,39 @@ namespace pbdev
 #pragma unroll
             for (int k = 0; k < 3; ++k)
                 out[r * 3 + k] = M[r * 3 + 0] * R[0 * 3 + k] + M[r * 3 + 1] * R[1 * 3 + k] + M[r * 3 + 2] * R[2 * 3 + k];
+    }
+
+    // Default pose retraction (PlaCamera applyPoseDelta, CameraState.cpp:374-380): C2W' = Exp([w]x) C2W
+    // with the Rodrigues form of Exp.
+    PB_HD void retractExp(const double* R, double wx, double wy, double wz, double* out)
+    {
+        const double t2 = wx * wx + wy * wy + wz * wz;
+        double E[9];
+        if (t2 < 1e-20)
+        {
+            E[0] = 1.0; E[1] = -wz; E[2] = wy;
+            E[3] = wz; E[4] = 1.0; E[5] = -wx;
+            E[6] = -wy; E[7] = wx; E[8] = 1.0;
+        }
+        else
+        {
+            const double t = sqrt(t2);
+            const double sc = sin(t) / t, cc = (1.0 - cos(t)) / t2;
+            E[0] = 1.0 - cc * (wy * wy + wz * wz); E[1] = cc * wx * wy - sc * wz; E[2] = cc * wx * wz + sc * wy;
+            E[3] = cc * wx * wy + sc * wz; E[4] = 1.0 - cc * (wx * wx + wz * wz); E[5] = cc * wy * wz - sc * wx;
+            E[6] = cc * wx * wz - sc * wy; E[7] = cc * wy * wz + sc * wx; E[8] = 1.0 - cc * (wx * wx + wy * wy);
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                out[r * 3 + k] = E[r * 3 + 0] * R[0 * 3 + k] + E[r * 3 + 1] * R[1 * 3 + k] + E[r * 3 + 2] * R[2 * 3 + k];
+    }
+
+    // Huber loss of a survey constraint (BundleAdjustPlaMatrixConstraints.cpp:17-29): s = |r|^2.
+    PB_HD bool huber(double s, double delta, double& cost, double& weight)
+    {
+        return finite(delta) && delta > 0.0 && robust(s, 1, delta, cost, weight) && finite(cost) && finite(weight);
     }
 
     // Symmetric 3x3 inverse (row-major) with the singularity test |det| <= 1e-15 max|m|^3.

@@ -69,6 +69,18 @@ namespace
         std::vector<int32_t> fixedCams, fixedTracks;
         int gaugePolicy = PB_GAUGE_AUTO_ANCHOR, gaugeAnchor = -1, gaugeScale = -1;
         double gaugeBaseline = 0.0;
+        // survey constraints (per-track control points and laser planes), in caller order per kind
+        struct Constraint
+        {
+            int track = -1;
+            bool plane = false;
+            int index = -1;     // position in the caller's list of its kind
+            double A[9] = {};   // control point: whitening matrix (row-major); laser plane: normal in row 0
+            double target[3] = {};
+            double weight = 1.0;
+        };
+        std::vector<Constraint> cons;
+        int nControl = 0, nPlanes = 0;
     };
 
     double sanitizedWeight(double w) { return std::isfinite(w) && w > 0.0 ? w : 0.0; }
@@ -130,6 +142,78 @@ namespace
         std::vector<int32_t> unitWidth;
     };
 
+    // Two distinct cameras with a finite pixel and a positive weight (hasUsableTrackObservations,
+    // BundleAdjustValidation.cpp:94-106; only the sign of the camera index is checked there).
+    bool trackUsableForGauge(const Input& in, int t)
+    {
+        int first = -1;
+        for (int64_t i = in.off[t]; i < in.off[t + 1]; ++i)
+        {
+            if (in.obsCam[i] < 0 || !std::isfinite(in.uv[2 * i]) || !std::isfinite(in.uv[2 * i + 1]) ||
+                !std::isfinite(in.weight[i]) || !(in.weight[i] > 0.0))
+                continue;
+            if (first < 0)
+                first = in.obsCam[i];
+            else if (in.obsCam[i] != first)
+                return true;
+        }
+        return false;
+    }
+
+    // Control-point targets that count for the gauge (on tracks usable for it).
+    std::vector<const Input::Constraint*> gaugeControls(const Input& in)
+    {
+        std::vector<const Input::Constraint*> out;
+        for (const Input::Constraint& k : in.cons)
+            if (!k.plane && trackUsableForGauge(in, k.track)) out.push_back(&k);
+        return out;
+    }
+
+    // Three targets spanning a triangle fix pose and scale (hasCompleteControlPointGauge, :181-240).
+    bool controlGaugeComplete(const std::vector<const Input::Constraint*>& ks)
+    {
+        const size_t n = ks.size();
+        for (size_t a = 0; a < n; ++a)
+            for (size_t b = a + 1; b < n; ++b)
+            {
+                double u[3], u2 = 0.0;
+                for (int i = 0; i < 3; ++i)
+                {
+                    u[i] = ks[b]->target[i] - ks[a]->target[i];
+                    u2 += u[i] * u[i];
+                }
+                if (u2 <= 1e-16) continue;
+                for (size_t c = b + 1; c < n; ++c)
+                {
+                    double v[3], v2 = 0.0;
+                    for (int i = 0; i < 3; ++i)
+                    {
+                        v[i] = ks[c]->target[i] - ks[a]->target[i];
+                        v2 += v[i] * v[i];
+                    }
+                    const double x0 = u[1] * v[2] - u[2] * v[1], x1 = u[2] * v[0] - u[0] * v[2],
+                                 x2 = u[0] * v[1] - u[1] * v[0];
+                    if (v2 > 1e-16 && x0 * x0 + x1 * x1 + x2 * x2 > 1e-12 * u2 * v2) return true;
+                }
+            }
+        return false;
+    }
+
+    // A target away from a fixed camera's centre gives the scale (hasControlPointBaselineToFixedCamera, :147-179).
+    bool controlScaleFromFixed(const Input& in, const std::vector<const Input::Constraint*>& ks, const std::set<int>& fixed)
+    {
+        for (const Input::Constraint* k : ks)
+            for (int c : fixed)
+            {
+                const double* C = &in.C[3 * c];
+                if (!std::isfinite(C[0]) || !std::isfinite(C[1]) || !std::isfinite(C[2])) continue;
+                double d = 0.0;
+                for (int i = 0; i < 3; ++i) d += (k->target[i] - C[i]) * (k->target[i] - C[i]);
+                if (d > 1e-16) return true;
+            }
+        return false;
+    }
+
     std::string normalizeGauge(const Input& in, const pb_solve_options& o, Prepared* p)
     {
         const int n = in.nCam;
@@ -156,14 +240,23 @@ namespace
         p->scaleCam = in.gaugeScale;
         p->baseline = in.gaugeBaseline;
         if (!o.refine_camera_pose || in.gaugePolicy == PB_GAUGE_CALLER_MANAGED || n == 0) return {};
+        // control points: three non-collinear targets give absolute pose and scale; a target away from a fixed
+        // camera's centre gives absolute scale (BundleAdjustValidation.cpp:826-850)
+        const std::vector<const Input::Constraint*> ks = gaugeControls(in);
+        const bool absolutePose = controlGaugeComplete(ks);
+        bool absoluteScale = absolutePose || controlScaleFromFixed(in, ks, fixed);
         if (in.gaugePolicy == PB_GAUGE_REQUIRE_EXPLICIT)
-            return fixed.size() >= 2 ? std::string() : std::string("unsupported: incomplete gauge (fix two cameras)");
+            return (absolutePose && absoluteScale) || fixed.size() >= 2 || (fixed.size() == 1 && absoluteScale)
+                       ? std::string()
+                       : std::string("unsupported: incomplete gauge (fix two cameras, or one with an absolute scale)");
+        if (absolutePose && absoluteScale) return {};
         if (fixed.empty())
         {
             p->fixedCams.push_back(0);
             fixed.insert(0);
+            absoluteScale = absoluteScale || controlScaleFromFixed(in, ks, fixed);
         }
-        if (fixed.size() < 2)
+        if (!absoluteScale && fixed.size() < 2)
         {
             const int a = *fixed.begin();
             int best = -1;
@@ -832,6 +925,7 @@ struct pb_ctx
     DeviceAlloc dwork32, dwork64;
     DeviceAlloc dbatch;      // per-window controller state of a batched solve
     DeviceAlloc dquality;    // post-solve quality buffers (pb_hip_quality)
+    DeviceAlloc dcons;       // survey-constraint table of the current solve (weights and thresholds are options)
     pbk::Quality Q;
     int lastSolveStatus = PB_SOLVE_NOT_RUN, lastUsable = 0;
     bool batchMode = false;  // the context holds a batch (pb_hip_solve_batch); single-problem entries refuse it
@@ -906,6 +1000,112 @@ namespace
         ctx->eventNext = 0;
     }
 
+    // Cholesky factor (lower, row-major) of a symmetric 3x3 matrix; false unless every pivot is positive.
+    bool cholesky3(const double* m, double* L)
+    {
+        for (int i = 0; i < 9; ++i)
+        {
+            if (!std::isfinite(m[i])) return false;
+            L[i] = 0.0;
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j <= i; ++j)
+            {
+                double v = 0.5 * (m[3 * i + j] + m[3 * j + i]);
+                for (int k = 0; k < j; ++k) v -= L[3 * i + k] * L[3 * j + k];
+                if (i == j)
+                {
+                    if (!(v > 0.0)) return false;
+                    L[3 * i + i] = std::sqrt(v);
+                }
+                else
+                {
+                    L[3 * i + j] = v / L[3 * j + j];
+                }
+            }
+        return true;
+    }
+
+    // Whitening matrix of a control point (makeControlPointWhitening, src/control_point.cpp:91-165): residual
+    // A (X - P) with A = diag(1/sigma), the inverse Cholesky factor of a covariance, or a square-root
+    // information matrix whose A^T A is positive definite.
+    bool controlWhitening(int kind, double sigma, const double* m, double* A)
+    {
+        for (int i = 0; i < 9; ++i) A[i] = 0.0;
+        if (kind == PB_CONTROL_ISOTROPIC_SIGMA)
+        {
+            if (!std::isfinite(sigma) || sigma <= 0.0) return false;
+            A[0] = A[4] = A[8] = 1.0 / sigma;
+            return true;
+        }
+        if (!m) return false;
+        double L[9];
+        if (kind == PB_CONTROL_COVARIANCE)
+        {
+            if (!cholesky3(m, L)) return false;
+            // forward substitution L A = I, column by column
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 3; ++r)
+                {
+                    double v = r == c ? 1.0 : 0.0;
+                    for (int k = 0; k < r; ++k) v -= L[3 * r + k] * A[3 * k + c];
+                    A[3 * r + c] = v / L[3 * r + r];
+                }
+            return true;
+        }
+        if (kind == PB_CONTROL_SQRT_INFORMATION)
+        {
+            double I[9] = {};
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k) I[3 * i + j] += m[3 * k + i] * m[3 * k + j];
+            if (!cholesky3(I, L)) return false;
+            std::copy(m, m + 9, A);
+            return true;
+        }
+        return false;
+    }
+
+    // The survey constraints of a view, validated as validateProblem does (src/problem.cpp:241-259).
+    std::string copyConstraints(const pb_problem_view* v, Input* r)
+    {
+        auto finite3 = [](const double* q) { return std::isfinite(q[0]) && std::isfinite(q[1]) && std::isfinite(q[2]); };
+        for (int i = 0; i < v->num_control_points; ++i)
+        {
+            Input::Constraint k;
+            k.track = v->cp_track[i];
+            k.index = i;
+            std::copy(v->cp_point + 3 * i, v->cp_point + 3 * i + 3, k.target);
+            k.weight = v->cp_weight ? v->cp_weight[i] : 1.0;
+            if (k.track < 0 || k.track >= v->num_tracks) return "control point " + std::to_string(i) + ": track out of range";
+            const int kind = v->cp_uncertainty ? v->cp_uncertainty[i] : PB_CONTROL_ISOTROPIC_SIGMA;
+            if (!finite3(k.target) || !std::isfinite(k.weight) || k.weight <= 0.0 ||
+                !controlWhitening(kind, v->cp_sigma ? v->cp_sigma[i] : 1.0, v->cp_matrix ? v->cp_matrix + 9 * i : nullptr,
+                                  k.A))
+                return "track contains an invalid control-point constraint (control point " + std::to_string(i) + ")";
+            r->cons.push_back(k);
+        }
+        for (int i = 0; i < v->num_laser_planes; ++i)
+        {
+            Input::Constraint k;
+            k.track = v->lp_track[i];
+            k.plane = true;
+            k.index = i;
+            std::copy(v->lp_point + 3 * i, v->lp_point + 3 * i + 3, k.target);
+            std::copy(v->lp_normal + 3 * i, v->lp_normal + 3 * i + 3, k.A);
+            k.weight = v->lp_weight ? v->lp_weight[i] : 1.0;
+            if (k.track < 0 || k.track >= v->num_tracks) return "laser plane " + std::to_string(i) + ": track out of range";
+            const double nn = std::hypot(k.A[0], k.A[1], k.A[2]);
+            if (!finite3(k.target) || !finite3(k.A) || !std::isfinite(nn) || nn <= 0.0 || !std::isfinite(k.weight) ||
+                k.weight <= 0.0)
+                return "track contains an invalid laser-plane constraint (laser plane " + std::to_string(i) + ")";
+            r->cons.push_back(k);
+        }
+        r->nControl = v->num_control_points;
+        r->nPlanes = v->num_laser_planes;
+        return {};
+    }
+
     void copyIn(const pb_problem_view* v, Input* in)
     {
         Input r;
@@ -933,6 +1133,8 @@ namespace
         r.gaugeAnchor = v->gauge_anchor_camera;
         r.gaugeScale = v->gauge_scale_camera;
         r.gaugeBaseline = v->gauge_baseline;
+        const std::string err = copyConstraints(v, &r);
+        if (!err.empty()) throw std::invalid_argument(err);
         *in = std::move(r);
     }
 
@@ -966,6 +1168,12 @@ namespace
         if (v->num_fixed_cameras > 0 && !v->fixed_cameras) return "fixed_cameras is NULL";
         if (v->num_fixed_tracks > 0 && !v->fixed_tracks) return "fixed_tracks is NULL";
         if (v->gauge_policy < 0 || v->gauge_policy > 2) return "gauge policy is invalid";
+        if (v->num_control_points < 0 || v->num_laser_planes < 0) return "negative constraint counts";
+        if (v->num_control_points > 0 && (!v->cp_track || !v->cp_point)) return "control-point arrays are NULL";
+        if (v->num_control_points > 0 && v->cp_uncertainty && !v->cp_matrix)
+            for (int i = 0; i < v->num_control_points; ++i)
+                if (v->cp_uncertainty[i] != PB_CONTROL_ISOTROPIC_SIGMA) return "cp_matrix is NULL";
+        if (v->num_laser_planes > 0 && (!v->lp_track || !v->lp_point || !v->lp_normal)) return "laser-plane arrays are NULL";
         return {};
     }
 
@@ -1017,6 +1225,7 @@ namespace
         ctx->dwork64.release();
         ctx->dbatch.release();
         ctx->dquality.release();
+        ctx->dcons.release();
         ctx->Q = pbk::Quality{};
         ctx->work32 = ctx->work64 = false;
     }
@@ -1238,6 +1447,76 @@ namespace
                   "state");
     }
 
+    // canUseReferenceOnlineSchur (ReferenceSchur.cpp:667-673) for frame cameras without rigs, laser ranges, scale
+    // bars, pose priors or a camera plane: the online point Schur (path B) unless laser planes are present or
+    // nothing primary is refined; otherwise the assembled damped Schur (path A).
+    bool onlineSchur(const pb_ctx* ctx, const pb_solve_options& o)
+    {
+        return o.use_online_schur != 0 && ctx->prep.primaryBlocks > 0 && ctx->in.nPlanes == 0;
+    }
+
+    // Per-solve device settings that follow the options: robust loss, the parameterisation
+    // (useReferencePointParameterization, Assembly.cpp:24-29) and the survey-constraint table, rebuilt each solve
+    // because its weights and Huber thresholds are options. Constraints are grouped by active track, laser planes
+    // before control points and each kind in caller order (Assembly.cpp:656-690); a constraint whose weight
+    // product is not positive is left out, as its linearisation would fail (Constraints.cpp:72-76, 94-98).
+    void applySolveOptions(pb_ctx* ctx, const pb_solve_options& o)
+    {
+        pbk::Problem& P = ctx->P;
+        P.loss = o.robust_loss;
+        P.lossScale = o.robust_loss_scale;
+        P.refParam = (o.use_online_schur && ctx->in.nControl == 0 && ctx->in.nPlanes == 0) ? 1 : 0;
+        ctx->dcons.release();
+        P.nCons = 0;
+        P.consOffset = P.consPoint = P.consRows = P.consIndex = nullptr;
+        P.consJ = P.consTarget = P.consDelta = nullptr;
+        if (ctx->in.cons.empty()) return;
+        const Layout& L = ctx->lay;
+        struct Key
+        {
+            int active, plane, index, k;
+        };
+        std::vector<Key> keys;
+        for (int k = 0; k < static_cast<int>(ctx->in.cons.size()); ++k)
+        {
+            const Input::Constraint& c = ctx->in.cons[k];
+            const int a = L.activeIndexOf[c.track];
+            const double w = (c.plane ? o.laser_plane_weight : o.control_point_weight) * c.weight;
+            if (a < 0 || !(w > 0.0) || !std::isfinite(w)) continue;
+            keys.push_back({a, c.plane ? 0 : 1, c.index, k});
+        }
+        std::sort(keys.begin(), keys.end(), [](const Key& x, const Key& y) {
+            return x.active != y.active ? x.active < y.active : x.plane != y.plane ? x.plane < y.plane : x.index < y.index;
+        });
+        const int n = static_cast<int>(keys.size());
+        std::vector<int32_t> offset(L.nActive + 1, 0), point(n), rows(n), index(n);
+        std::vector<double> J(size_t(n) * 9, 0.0), target(size_t(n) * 3), delta(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const Input::Constraint& c = ctx->in.cons[keys[i].k];
+            const double sw = std::sqrt((c.plane ? o.laser_plane_weight : o.control_point_weight) * c.weight);
+            ++offset[keys[i].active + 1];
+            point[i] = keys[i].active;
+            rows[i] = c.plane ? 1 : 3;
+            index[i] = c.plane ? -1 : c.index;
+            for (int q = 0; q < (c.plane ? 3 : 9); ++q) J[9 * i + q] = sw * c.A[q];
+            std::copy(c.target, c.target + 3, &target[3 * i]);
+            delta[i] = c.plane ? o.laser_huber_delta * sw : o.control_point_huber_delta;
+        }
+        for (int a = 0; a < L.nActive; ++a) offset[a + 1] += offset[a];
+        if (n == 0) return;
+        hipStream_t s = ctx->stream;
+        DeviceAlloc& A = ctx->dcons;
+        P.nCons = n;
+        P.consOffset = A.upload(offset, s);
+        P.consPoint = A.upload(point, s);
+        P.consRows = A.upload(rows, s);
+        P.consIndex = A.upload(index, s);
+        P.consJ = A.upload(J, s);
+        P.consTarget = A.upload(target, s);
+        P.consDelta = A.upload(delta, s);
+    }
+
     struct BuildResult
     {
         double cost = 0.0;
@@ -1346,10 +1625,20 @@ namespace
         std::snprintf(r->message, sizeof(r->message), "%s", m.c_str());
     }
 
+    template <typename T>
+    void runSolveAssembled(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel);
+    template <typename T>
+    void publishState(pb_ctx* ctx, pb_solve_report* rep, double current, bool converged, bool cancelled);
+
     // The online-Schur controller (BundleAdjustPlaMatrix.cpp:162-631 with useReferenceOnlineSchur).
     template <typename T>
     void runSolve(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel)
     {
+        if (!onlineSchur(ctx, o))
+        {
+            runSolveAssembled<T>(ctx, o, rep, cancel);
+            return;
+        }
         using Lr = pbk::Launcher<T>;
         pbk::Work<T>& W = work<T>(ctx);
         hipStream_t s = ctx->stream;
@@ -1523,6 +1812,16 @@ namespace
         rep->accepted_steps = accepted;
         rep->rejected_steps = rejected;
         rep->iterations = iterations;
+        publishState<T>(ctx, rep, current, converged, cancelled);
+    }
+
+    // Final report fields and the refined state back to the host (both controllers).
+    template <typename T>
+    void publishState(pb_ctx* ctx, pb_solve_report* rep, double current, bool converged, bool cancelled)
+    {
+        hipStream_t s = ctx->stream;
+        const pbk::Problem& P = ctx->P;
+        const Prepared& pr = ctx->prep;
         rep->final_cost = current;
         collectProfile(ctx);
         if (cancelled)
@@ -1536,7 +1835,8 @@ namespace
         rep->converged = converged ? 1 : 0;
         rep->solve_status = converged ? PB_SOLVE_SUCCESS : PB_SOLVE_NO_CONVERGENCE;
         rep->solution_usable = std::isfinite(current) ? 1 : 0;
-        setMessage(rep, converged ? "converged (online Schur + block-Jacobi PCG + Armijo)"
+        setMessage(rep, converged ? (rep->online_schur_used ? "converged (online Schur + block-Jacobi PCG + Armijo)"
+                                                            : "converged (assembled Schur + block-Jacobi PCG + line search)")
                                   : "iteration budget or line search exhausted");
         // refined state back to the host
         ctx->outR.resize(ctx->in.R.size());
@@ -1555,6 +1855,192 @@ namespace
                   "download");
         check(hipStreamSynchronize(s), "download");
         ctx->haveResult = true;
+    }
+
+    // The assembled-Schur controller (path A: BundleAdjustPlaMatrix.cpp:239-556 with the online Schur off).
+    // One linearisation per outer step; a rejected step solves the same equations again with more damping (the
+    // build is repeated at the unchanged state, which reproduces them); the directional derivative comes from an
+    // objective probe at alpha = 1e-4 (:473-486) evaluated in the same cost pass as alpha = 1; any decrease is
+    // accepted (:517-519), and a small step ends the solve without being applied (:451-465). A singular damped
+    // point block is a linear-solver exception, which this path does not retry (:357-364); a reduced diagonal
+    // block that is not positive definite counts as a solve that did not converge.
+    template <typename T>
+    void runSolveAssembled(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel)
+    {
+        using Lr = pbk::Launcher<T>;
+        pbk::Work<T>& W = work<T>(ctx);
+        hipStream_t s = ctx->stream;
+        const pbk::Problem& P = ctx->P;
+        const Prepared& pr = ctx->prep;
+        resetState(ctx);
+        check(hipStreamSynchronize(s), "reset");
+        const bool mixed = std::is_same<T, float>::value;
+        const double relTol = o.pcg_relative_tolerance > 0.0 ? o.pcg_relative_tolerance : (mixed ? 1e-3 : 1e-12);
+        const double absTol = o.pcg_absolute_tolerance > 0.0 ? o.pcg_absolute_tolerance : 1e-12;
+        const int maxPcg = o.pcg_max_iterations > 0 ? o.pcg_max_iterations
+                                                    : (mixed ? std::max(200, 3 * pr.primaryBlocks / 5)
+                                                             : std::max(200, 30 * pr.primaryBlocks));
+        const int checkEvery = o.pcg_check_interval > 0 ? o.pcg_check_interval : 8;
+        const int batch = o.armijo_batch > 0 ? std::min(o.armijo_batch, pbk::kMaxAlpha) : 4;
+        const double modelEta = o.pcg_model_eta >= 0.0 ? o.pcg_model_eta : (mixed ? 0.1 : 0.0);
+        const double rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
+        constexpr double kProbe = 1e-4;
+        rep->linear_tolerance = relTol;
+        rep->mixed_precision_used = mixed ? 1 : 0;
+        rep->online_schur_used = 0;
+        ctx->lastScalarBytes = sizeof(T);
+
+        const auto t0 = Clock::now();
+        double current = std::numeric_limits<double>::quiet_NaN();
+        double damping = 0.0;
+        int accepted = 0, rejected = 0, iterations = 0;
+        bool converged = pr.primaryBlocks == 0 && pr.trackBlocks == 0;
+        bool cancelled = false;
+        auto isCancelled = [&]() { return cancel && *cancel != 0; };
+        auto raise = [&]() { damping = damping == 0.0 ? 1e-3 : damping * 10.0; };
+        auto timedBuild = [&]() {
+            auto ta = Clock::now();
+            const BuildResult b = build(ctx, W, damping);
+            rep->assembly_seconds += seconds(ta);
+            if (b.invalid) throw std::runtime_error("assembled Schur linearisation produced an invalid projection");
+            return b;
+        };
+        if (converged)
+        {
+            const BuildResult b = build(ctx, W, 0.0);
+            current = b.cost;
+            rep->initial_cost = current;
+            ++rep->objective_evaluations;
+        }
+        const int budget = std::max(1, o.max_iterations);
+        double alphas[pbk::kMaxAlpha];
+        double* h = ctx->pinned;
+        while (iterations < budget && !converged)
+        {
+            if (isCancelled())
+            {
+                cancelled = true;
+                break;
+            }
+            BuildResult b = timedBuild();
+            ++rep->linearizations;
+            ++rep->objective_evaluations;
+            current = b.cost;
+            if (rep->linearizations == 1) rep->initial_cost = current;
+            bool relinearize = false, fresh = true;
+            while (iterations < budget && !converged && !relinearize)
+            {
+                if (isCancelled())
+                {
+                    cancelled = true;
+                    break;
+                }
+                if (!fresh) b = timedBuild();  // same linearisation point, the current damping
+                fresh = false;
+                if (b.singular) throw std::runtime_error("linear solve failed: singular damped point block");
+                PcgResult lin;
+                if (!b.notSpd)
+                {
+                    auto tl = Clock::now();
+                    lin = pcg(ctx, W, relTol, absTol, maxPcg, checkEvery, modelEta);
+                    rep->linear_solve_seconds += seconds(tl);
+                    rep->linear_iterations += lin.iterations;
+                }
+                ++iterations;
+                if (b.notSpd || lin.breakdown || !lin.converged)
+                {
+                    ++rejected;
+                    raise();
+                    continue;
+                }
+                auto tb = Clock::now();
+                {
+                    Span sp(ctx, 5);
+                    Lr::backSubstitute(P, W, s);
+                }
+                check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
+                      "readback");
+                check(hipStreamSynchronize(s), "back-substitution");
+                rep->back_substitution_seconds += seconds(tb);
+                const double rmsG = h[pbk::kScalarRmsGlobal], rmsP = h[pbk::kScalarRmsPoint];
+                if (!std::isfinite(rmsG) || !std::isfinite(rmsP))
+                    throw std::runtime_error("linear solve produced a non-finite step");
+                if (rmsG <= rmsTol && rmsP <= rmsTol)
+                {
+                    converged = true;  // the small step is not applied
+                    continue;
+                }
+                // probe + line search: the first pass evaluates alpha = 1e-4 and 1, later passes `batch` halvings
+                auto tobj = Clock::now();
+                double alpha = 1.0, cand = kInf, candAlpha = 0.0, dd = 0.0;
+                int candIndex = -1, ls = 0;
+                bool armijo = false, probed = false;
+                while (ls < o.line_search_steps && !armijo)
+                {
+                    int n = 0;
+                    if (!probed) alphas[n++] = kProbe;
+                    const int trials = std::min(probed ? batch : 1, o.line_search_steps - ls);
+                    for (int k = 0; k < trials; ++k) alphas[n++] = alpha * std::ldexp(1.0, -k);
+                    check(hipMemcpyAsync(W.alphas, alphas, sizeof(double) * n, hipMemcpyHostToDevice, s), "alphas");
+                    {
+                        Span sp(ctx, 4);
+                        Lr::trialCosts(P, W, W.alphas, n, ctx->costBlocks, s);
+                    }
+                    check(hipMemcpyAsync(h, W.trialCost, sizeof(double) * n, hipMemcpyDeviceToHost, s), "readback");
+                    check(hipStreamSynchronize(s), "trial cost");
+                    int k = 0;
+                    if (!probed)
+                    {
+                        ++rep->objective_evaluations;
+                        if (std::isfinite(h[0]) && h[0] < current) dd = (current - h[0]) / kProbe;
+                        probed = true;
+                        k = 1;
+                    }
+                    for (; k < n; ++k)
+                    {
+                        ++rep->objective_evaluations;
+                        ++ls;
+                        cand = h[k];
+                        candIndex = k;
+                        candAlpha = alphas[k];
+                        const double limit = dd > 0.0 ? current - o.armijo_coefficient * alphas[k] * dd : current;
+                        if (std::isfinite(cand) && cand <= limit && cand < current)
+                        {
+                            armijo = true;
+                            alpha = alphas[k];
+                            break;
+                        }
+                        alpha = alphas[k] * 0.5;
+                    }
+                }
+                rep->objective_seconds += seconds(tobj);
+                if (std::getenv("PB_TRACE"))
+                    std::fprintf(stderr, "pb A it=%d cost=%.12g cand=%.12g alpha=%g rms_g=%.3e rms_p=%.3e dd=%.4e lambda=%g\n",
+                                 iterations, current, cand, alpha, rmsG, rmsP, dd, damping);
+                if (std::isfinite(cand) && cand < current)
+                {
+                    // the state of the last length tried; the stop test uses alpha, which is half of it when no
+                    // length met the Armijo condition (:505-529)
+                    Lr::accept(P, W, candIndex, candAlpha, s);
+                    current = cand;
+                    ++accepted;
+                    damping /= 10.0;
+                    converged = alpha * rmsG <= rmsTol && alpha * rmsP <= rmsTol;
+                    relinearize = !converged;
+                }
+                else
+                {
+                    ++rejected;
+                    raise();
+                }
+            }
+        }
+        check(hipStreamSynchronize(s), "solve");
+        rep->solve_seconds = seconds(t0);
+        rep->accepted_steps = accepted;
+        rep->rejected_steps = rejected;
+        rep->iterations = iterations;
+        publishState<T>(ctx, rep, current, converged, cancelled);
     }
 
     void fillReportCounts(const pb_ctx* ctx, pb_solve_report* r)
@@ -1728,6 +2214,8 @@ namespace
             };
             const std::string verr = validateInput(v);
             if (!verr.empty()) throw windowError(verr);
+            if (v->num_control_points > 0 || v->num_laser_planes > 0)
+                throw windowError("unsupported: survey constraints in a batched window");
             Input wi;
             copyIn(v, &wi);
             Prepared pw;
@@ -2213,6 +2701,10 @@ void pb_default_options(pb_solve_options* o)
     o->line_search_steps = 34;
     o->use_online_schur = 1;
     o->max_initial_track_rms = 100.0;
+    o->laser_plane_weight = 1.0;
+    o->laser_huber_delta = 0.2;
+    o->control_point_weight = 1.0;
+    o->control_point_huber_delta = 0.2;
     o->refine_camera_pose = 1;
     o->shared_intrinsic_mask = 0;
     o->min_focal_scale = 0.5;
@@ -2308,6 +2800,9 @@ pb_status pb_hip_upload(pb_ctx* ctx, const pb_problem_view* problem)
     if (!err.empty()) return fail(ctx, PB_ERR_INVALID, err);
     return guarded(ctx, [&]() -> pb_status {
         check(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->uploaded = false;
+        ctx->structureValid = false;
+        ctx->haveResult = false;
         copyIn(problem, &ctx->in);
         if (ctx->batchMode) releaseStructure(ctx);
         ctx->batchMode = false;
@@ -2331,17 +2826,19 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
         setMessage(report, oerr);
         return fail(ctx, PB_ERR_INVALID, oerr);
     }
-    if (!options->use_online_schur)
-    {
-        report->solve_status = PB_SOLVE_UNSUPPORTED;
-        setMessage(report, "the MI355X backend implements the online point-Schur path only");
-        return fail(ctx, PB_ERR_UNSUPPORTED, report->message);
-    }
     if (!ctx->uploaded || ctx->batchMode)
     {
         report->solve_status = PB_SOLVE_INVALID_INPUT;
         setMessage(report, "no problem uploaded");
         return fail(ctx, PB_ERR_INVALID, "no problem uploaded");
+    }
+    // BundleAdjustValidation.cpp:589-595
+    if (ctx->in.nPlanes > 0 && (!std::isfinite(options->laser_plane_weight) || options->laser_plane_weight <= 0.0 ||
+                                !std::isfinite(options->laser_huber_delta) || options->laser_huber_delta < 0.0))
+    {
+        report->solve_status = PB_SOLVE_INVALID_INPUT;
+        setMessage(report, "laser-plane weight must be positive and its Huber threshold finite and non-negative");
+        return fail(ctx, PB_ERR_INVALID, report->message);
     }
     ctx->profile = options->profile_kernels != 0;
     const pb_status st = guarded(ctx, [&]() -> pb_status {
@@ -2356,8 +2853,7 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
         const auto ts = Clock::now();
         ensureStructure(ctx, *options);
         fillReportCounts(ctx, report);
-        ctx->P.loss = options->robust_loss;
-        ctx->P.lossScale = options->robust_loss_scale;
+        applySolveOptions(ctx, *options);
         if (ctx->in.nCam == 0 || ctx->in.nTracks == 0)
         {
             report->solve_status = PB_SOLVE_INVALID_INPUT;
@@ -2407,7 +2903,6 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
                           double* dl_dcontrol)
 {
     if (!ctx || !dl_dobs) return PB_ERR_INVALID;
-    if (dl_dcontrol) return fail(ctx, PB_ERR_UNSUPPORTED, "unsupported: no control-point constraints in this backend");
     if (!ctx->uploaded || !ctx->haveResult || ctx->batchMode)
         return fail(ctx, PB_ERR_INVALID, "pb_hip_backward needs the result of a pb_hip_solve on this context");
     return guarded(ctx, [&]() -> pb_status {
@@ -2417,8 +2912,7 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
         pbk::Problem& P = ctx->P;
         const Layout& L = ctx->lay;
         hipStream_t s = ctx->stream;
-        P.loss = ctx->lastOptions.robust_loss;
-        P.lossScale = ctx->lastOptions.robust_loss_scale;
+        applySolveOptions(ctx, ctx->lastOptions);
         // loss gradients in device order (cameras as given, points in elimination order)
         std::vector<double> hc(3 * size_t(std::max(1, P.nCam)), 0.0), hp(3 * size_t(std::max(1, P.nElim)), 0.0);
         if (dl_dcenter) std::copy_n(dl_dcenter, 3 * size_t(P.nCam), hc.begin());
@@ -2429,6 +2923,8 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
         const double* dC = tmp.upload(hc, s);
         const double* dP = tmp.upload(hp, s);
         double* dOut = tmp.alloc<double>(2 * size_t(std::max(1, P.nObs)));
+        double* dCtl = tmp.alloc<double>(3 * size_t(std::max(1, ctx->in.nControl)));
+        check(hipMemsetAsync(dCtl, 0, sizeof(double) * 3 * size_t(std::max(1, ctx->in.nControl)), s), "memset");
         // the normal equations at the converged state with the adjoint weight roots, no damping
         P.adjoint = 1;
         BuildResult br;
@@ -2491,9 +2987,13 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
         }
         Lr::backSubstitute(P, W, s);
         Lr::adjointObservations(P, W, dOut, s);
+        Lr::adjointConstraints(P, W, dCtl, s);
         std::vector<double> out(2 * size_t(P.nObs));
         if (P.nObs > 0)
             check(hipMemcpyAsync(out.data(), dOut, sizeof(double) * out.size(), hipMemcpyDeviceToHost, s), "readback");
+        if (dl_dcontrol && ctx->in.nControl > 0)
+            check(hipMemcpyAsync(dl_dcontrol, dCtl, sizeof(double) * 3 * size_t(ctx->in.nControl), hipMemcpyDeviceToHost, s),
+                  "readback");
         check(hipStreamSynchronize(s), "adjoint");
         tmp.release();
         const int64_t nIn = ctx->in.off.empty() ? 0 : ctx->in.off.back();
@@ -2662,8 +3162,7 @@ pb_status pb_hip_reduced_system(pb_ctx* ctx, const pb_solve_options* options, do
     return guarded(ctx, [&]() -> pb_status {
         check(hipSetDevice(ctx->device), "hipSetDevice");
         ensureStructure(ctx, *options);
-        ctx->P.loss = options->robust_loss;
-        ctx->P.lossScale = options->robust_loss_scale;
+        applySolveOptions(ctx, *options);
         ctx->profile = false;
         if (options->precision == PB_PRECISION_MIXED)
             return exportReduced<float>(ctx, *options, damping, matrix, rhs, cost, primary_blocks);

@@ -82,6 +82,20 @@ namespace pbk
         double gaugeBaseline = 0.0;
         int globalDim = 0;
         int adjoint = 0;  // 1: linearisations use the adjoint weight root (pb_hip_backward builds)
+        // 1: reference parameterisation (online Schur without survey constraints: target-frame Euler rotation,
+        // point step scaled by s/atan s); 0: default (world-frame Exp rotation, unscaled point step)
+        int refParam = 1;
+
+        // Survey constraints on points (control points, laser planes), grouped by active track: residual
+        // r = J (X - target) with 1 or 3 rows, Huber(|r|^2, delta). nCons = 0 when the problem has none.
+        int nCons = 0;
+        const int32_t* consOffset = nullptr;  // [nActive+1]
+        const int32_t* consPoint = nullptr;   // [nCons] active track index
+        const int32_t* consRows = nullptr;    // [nCons] 1 or 3
+        const double* consJ = nullptr;        // [nCons*9] rows x 3, row-major (sqrt(weight) * whitening / normal)
+        const double* consTarget = nullptr;   // [nCons*3]
+        const double* consDelta = nullptr;    // [nCons] Huber threshold
+        const int32_t* consIndex = nullptr;   // [nCons] caller's control-point index, -1 for a laser plane
 
         // cameras
         double* camR = nullptr;  // state, camera-to-world [nCam*9]
@@ -312,6 +326,8 @@ namespace pbk
         static void adjointRhs(const Problem& P, const Work<T>& W, const double* dlCenter, const double* dlPoint,
                                hipStream_t s);
         static void adjointObservations(const Problem& P, const Work<T>& W, double* out, hipStream_t s);
+        // d loss / d target of every control point (out [control points * 3], zero-filled by the caller)
+        static void adjointConstraints(const Problem& P, const Work<T>& W, double* out, hipStream_t s);
         // dense column-major copy of S (A, zero-filled by the caller) and of the reduced right-hand side
         static void denseSystem(const Problem& P, const Work<T>& W, double* A, double* b, hipStream_t s);
         static void trialCosts(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, int costBlocks,

@@ -157,7 +157,7 @@ namespace pbk
             const int K = g >= 0 ? P.groupK[g] : 0;
             const int8_t* act = g >= 0 ? P.groupActive + g * kMaxActive : nullptr;
             if (!pbdev::linearize<KM>(P.cams[c], X0, X1, X2, P.obsUV[2 * o], P.obsUV[2 * o + 1], P.obsW[o], P.loss,
-                                  P.lossScale, K, act, L))
+                                  P.lossScale, K, act, L, P.refParam != 0))
             {
                 setFlag(flags, kFlagInvalid);
                 continue;
@@ -200,6 +200,50 @@ namespace pbk
                 }
             }
         }
+        // point-only survey constraints (accumulatePointOnlyConstraint, ReferenceSchur.cpp:241-259): w J^T J and
+        // -w J^T r; the adjoint build uses the root Q of the exact weight matrix instead (J <- Q J, weight 1)
+        if (P.nCons)
+            for (int k = P.consOffset[t]; k < P.consOffset[t + 1]; ++k)
+            {
+                double r[3], cost, w;
+                if (!constraintAt(P, k, X0, X1, X2, r, cost, w)) continue;
+                const int rows = P.consRows[k];
+                double J[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) J[i] = P.consJ[9 * k + i];
+                if (P.adjoint)
+                {
+                    double q[2];
+                    const double ss = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                    pbdev::constraintWeightRoot(ss, w, P.consDelta[k], q);
+                    double QJ[9];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j)
+                        {
+                            double v = q[0] * J[3 * i + j];
+#pragma unroll
+                            for (int m = 0; m < 3; ++m) v += q[1] * r[i] * r[m] * J[3 * m + j];
+                            QJ[3 * i + j] = i < rows ? v : 0.0;
+                        }
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) J[i] = QJ[i];
+                    w = 1.0;
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                {
+                    if (i >= rows) break;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                    {
+                        bp[c] -= w * J[3 * i + c] * r[i];
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) H[c * 3 + j] += w * J[3 * i + c] * J[3 * i + j];
+                    }
+                }
+            }
         double Hd[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) Hd[i] = H[i];
@@ -296,7 +340,8 @@ namespace pbk
             const int o = P.camObs[k];
             const int t = P.obsPoint[o];
             if (!pbdev::linearize<KM>(cam, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2], P.obsUV[2 * o],
-                                  P.obsUV[2 * o + 1], P.obsW[o], P.loss, P.lossScale, KM > 0 ? K : 0, act, L))
+                                  P.obsUV[2 * o + 1], P.obsW[o], P.loss, P.lossScale, KM > 0 ? K : 0, act, L,
+                                  P.refParam != 0))
             {
                 setFlag(flagsOf(P, Wk, P.camWin, c), kFlagInvalid);
                 continue;
@@ -470,6 +515,13 @@ namespace pbk
         __shared__ double part[256];
         double v = 0.0;
         for (int c = threadIdx.x; c < P.nCam; c += blockDim.x) v += Wk.camCost[c];
+        // survey constraints of every active track (Assembly.cpp:656-690; ReferenceSchur.cpp:331-346)
+        for (int k = threadIdx.x; k < P.nCons; k += blockDim.x)
+        {
+            const int t = P.consPoint[k];
+            double r[3], cost, w;
+            if (constraintAt(P, k, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2], r, cost, w)) v += cost;
+        }
         part[threadIdx.x] = v;
         __syncthreads();
         if (threadIdx.x == 0)
@@ -1142,8 +1194,13 @@ namespace pbk
             double d[6];
 #pragma unroll
             for (int i = 0; i < 6; ++i) d[i] = a * double(Wk.x[u * 6 + i]);
-            pbdev::retractRotation(R0, d[0], d[1], d[2], R);
-            if (anchor >= 0)
+            if (P.refParam)
+                pbdev::retractRotation(R0, d[0], d[1], d[2], R);
+            else
+                pbdev::retractExp(R0, d[0], d[1], d[2], R);
+            // the default parameterisation applies every centre step in world axes, the gauge scale camera's
+            // included (applyDeltaPose, Assembly.cpp:1133-1136)
+            if (anchor >= 0 && P.refParam)
             {
                 // centre moves along the tangent basis, then back onto the baseline sphere
                 // (Assembly.cpp:101-128)
@@ -1207,6 +1264,8 @@ namespace pbk
         for (int a = 0; a < NA; ++a) al[a] = a < nAlpha ? alphas[a] : 0.0;
         for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < P.nObs; o += gridDim.x * blockDim.x)
             trialObservation<NA>(P, Wk.candCams, Wk.dp, o, al, nAlpha, acc, bad);
+        for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < P.nCons; k += gridDim.x * blockDim.x)
+            trialConstraint<NA>(P, Wk.dp, k, al, nAlpha, acc);
         __shared__ double lds[NA][4];
         __shared__ int lbad[NA][4];
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1273,7 +1332,7 @@ namespace pbk
         if (id < P.nElim)
         {
             const int t = P.elimTrack[id];
-            const double s = pbdev::pointScale(P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2]);
+            const double s = stepScale(P, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2]);
             for (int i = 0; i < 3; ++i) P.pointX[3 * t + i] += s * alpha * Wk.dp[3 * id + i];
         }
     }
@@ -1315,7 +1374,7 @@ namespace pbk
         const int b = blockIdx.x * blockDim.x + threadIdx.x;
         if (b >= P.nElim) return;
         const int t = P.elimTrack[b];
-        const double sc = pbdev::pointScale(P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2]);
+        const double sc = stepScale(P, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2]);
         const double g0 = sc * dlPoint[3 * b], g1 = sc * dlPoint[3 * b + 1], g2 = sc * dlPoint[3 * b + 2];
         const T* Lp = Wk.Hinv + 9 * b;
         Wk.y[3 * b] = double(Lp[0]) * g0 + double(Lp[1]) * g1 + double(Lp[3]) * g2;
@@ -1357,7 +1416,8 @@ namespace pbk
         const int8_t* act = g >= 0 ? P.groupActive + g * kMaxActive : nullptr;
         Linearization L;
         if (!pbdev::linearize<KM>(P.cams[c], P.pointX[3 * a], P.pointX[3 * a + 1], P.pointX[3 * a + 2],
-                                  P.obsUV[2 * o], P.obsUV[2 * o + 1], P.obsW[o], P.loss, P.lossScale, K, act, L))
+                                  P.obsUV[2 * o], P.obsUV[2 * o + 1], P.obsW[o], P.loss, P.lossScale, K, act, L,
+                                  P.refParam != 0))
         {
             out[2 * o] = out[2 * o + 1] = 0.0;
             return;
@@ -1397,6 +1457,50 @@ namespace pbk
         const double t0 = q[0] * v0 + q[1] * v1, t1 = q[1] * v0 + q[2] * v1;
         out[2 * o] = q[0] * t0 + q[1] * t1;
         out[2 * o + 1] = q[1] * t0 + q[2] * t1;
+    }
+
+    // d loss / d P of control point k: its residual is r = J (X - P), so d g_X / d P = -J^T M J and
+    // d loss / d P = J^T M J lambda_X with lambda_X = s dp the point part of the adjoint solution.
+    template <typename T>
+    __global__ void k_adjoint_constraints(Problem P, Work<T> Wk, double* out)
+    {
+        const int k = blockIdx.x * blockDim.x + threadIdx.x;
+        if (k >= P.nCons) return;
+        const int slot = P.consIndex[k];
+        if (slot < 0) return;
+        const int t = P.consPoint[k];
+        const int b = P.pointElim[t];
+        const double X0 = P.pointX[3 * t], X1 = P.pointX[3 * t + 1], X2 = P.pointX[3 * t + 2];
+        double g[3] = {0.0, 0.0, 0.0};
+        double r[3], cost, w;
+        if (b >= 0 && constraintAt(P, k, X0, X1, X2, r, cost, w))
+        {
+            const int rows = P.consRows[k];
+            const double* J = P.consJ + 9 * k;
+            const double s = stepScale(P, X0, X1, X2);
+            const double l[3] = {s * Wk.dp[3 * b], s * Wk.dp[3 * b + 1], s * Wk.dp[3 * b + 2]};
+            double v[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) v[i] = i < rows ? J[3 * i] * l[0] + J[3 * i + 1] * l[1] + J[3 * i + 2] * l[2] : 0.0;
+            double q[2];
+            const double ss = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            pbdev::constraintWeightRoot(ss, w, P.consDelta[k], q);
+            // M v = Q (Q v), Q v = q0 v + q1 r (r . v)
+#pragma unroll
+            for (int pass = 0; pass < 2; ++pass)
+            {
+                const double rv = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) v[i] = q[0] * v[i] + q[1] * r[i] * rv;
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    if (i < rows) g[j] += J[3 * i + j] * v[i];
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out[3 * slot + j] = g[j];
     }
 
     // ---------------------------------------------------------------------------------------------
@@ -1515,6 +1619,12 @@ namespace pbk
             k_adjoint_observations<T, 6><<<nb, 128, 0, s>>>(P, W, out);
         else
             k_adjoint_observations<T, 12><<<nb, 128, 0, s>>>(P, W, out);
+    }
+
+    template <typename T>
+    void Launcher<T>::adjointConstraints(const Problem& P, const Work<T>& W, double* out, hipStream_t s)
+    {
+        if (P.nCons > 0) k_adjoint_constraints<T><<<blocksFor(P.nCons, 128), 128, 0, s>>>(P, W, out);
     }
 
     template <typename T>

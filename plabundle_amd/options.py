@@ -110,6 +110,16 @@ class CalibrationOptions:
 
 
 @dataclass
+class ConstraintOptions:
+    """plabundle::ConstraintOptions (include/plabundle/options.h:122-135), the kinds this backend implements."""
+
+    laserPlaneWeight: float = 1.0
+    laserHuberDeltaMeters: float = 0.2
+    controlPointWeight: float = 1.0
+    controlPointHuberDeltaMeters: float = 0.2
+
+
+@dataclass
 class BackendOptions:
     requested: Backend = Backend.Mi355x
     plaMatrixDevice: int = 0
@@ -141,6 +151,7 @@ class SolveOptions:
     calibration: CalibrationOptions = field(default_factory=CalibrationOptions)
     backend: BackendOptions = field(default_factory=BackendOptions)
     quality: QualityGateOptions = field(default_factory=QualityGateOptions)
+    constraints: ConstraintOptions = field(default_factory=ConstraintOptions)
 
 
 def shared_intrinsic_parameter_enabled(calibration: CalibrationOptions, parameter: int) -> bool:
@@ -215,6 +226,11 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.profile_kernels = 1 if b.profileKernels else 0
     o.pcg_model_eta = float(b.pcgModelEta)
     o.step_rms_tolerance = float(b.stepRmsTolerance)
+    k = options.constraints
+    o.laser_plane_weight = float(k.laserPlaneWeight)
+    o.laser_huber_delta = float(k.laserHuberDeltaMeters)
+    o.control_point_weight = float(k.controlPointWeight)
+    o.control_point_huber_delta = float(k.controlPointHuberDeltaMeters)
     return o
 
 

@@ -25,6 +25,13 @@ class BrownConvention(enum.IntEnum):
     Metashape = 1
 
 
+class ControlPointUncertainty(enum.IntEnum):
+    """plabundle::ControlPointConstraint::Uncertainty (include/plabundle/constraints.h:81-87)."""
+    IsotropicSigma = 0
+    Covariance = 1
+    SqrtInformation = 2
+
+
 @dataclass
 class Problem:
     cam_rotation: np.ndarray            # (Nc, 9) camera-to-world, row-major
@@ -46,7 +53,26 @@ class Problem:
     gauge_anchor_camera: int = -1
     gauge_scale_camera: int = -1
     gauge_baseline: float = 0.0
+    # Survey constraints of plabundle::Track (constraints.h:39-46, 73-88, 136-141) as flat arrays keyed by track.
+    cp_track: np.ndarray | None = None        # (Ncp,) int32 track index
+    cp_point: np.ndarray | None = None        # (Ncp, 3) surveyed position
+    cp_sigma: np.ndarray | None = None        # (Ncp,) sigmaMeters (isotropic)
+    cp_weight: np.ndarray | None = None       # (Ncp,)
+    cp_uncertainty: np.ndarray | None = None  # (Ncp,) ControlPointUncertainty
+    cp_matrix: np.ndarray | None = None       # (Ncp, 9) covariance or square-root information, row-major
+    lp_track: np.ndarray | None = None        # (Nlp,) int32 track index
+    lp_point: np.ndarray | None = None        # (Nlp, 3) point on the plane
+    lp_normal: np.ndarray | None = None       # (Nlp, 3) plane normal
+    lp_weight: np.ndarray | None = None       # (Nlp,)
     meta: dict = field(default_factory=dict)
+
+    @property
+    def num_control_points(self) -> int:
+        return 0 if self.cp_track is None else int(np.asarray(self.cp_track).size)
+
+    @property
+    def num_laser_planes(self) -> int:
+        return 0 if self.lp_track is None else int(np.asarray(self.lp_track).size)
 
     @property
     def num_cameras(self) -> int:

@@ -140,9 +140,10 @@ class SolverWorkspace:
             raise RuntimeError(f"pb_hip_get_masks: {self.last_error()}")
         return out
 
-    def backward(self, problem: Problem, dl_dcenter=None, dl_dpoint=None):
+    def backward(self, problem: Problem, dl_dcenter=None, dl_dpoint=None, with_control: bool = False):
         """Adjoint of the last solve on this workspace: dl/d(u, v) per observation ([num_observations, 2])
-        for a loss with gradients dl/dC ([num_cameras, 3]) and dl/dX ([num_tracks, 3]) at the refined state."""
+        for a loss with gradients dl/dC ([num_cameras, 3]) and dl/dX ([num_tracks, 3]) at the refined state.
+        with_control=True also returns dl/dP of every control point's surveyed position ([num_control_points, 3])."""
         dc = None if dl_dcenter is None else np.ascontiguousarray(dl_dcenter, dtype=np.float64).reshape(-1, 3)
         dp = None if dl_dpoint is None else np.ascontiguousarray(dl_dpoint, dtype=np.float64).reshape(-1, 3)
         if dc is not None and dc.shape[0] != problem.num_cameras:
@@ -150,11 +151,12 @@ class SolverWorkspace:
         if dp is not None and dp.shape[0] != problem.num_tracks:
             raise ValueError("dl_dpoint must have one row per track")
         out = np.zeros((problem.num_observations, 2))
+        ctl = np.zeros((max(1, problem.num_control_points), 3)) if with_control else None
         st = self._lib.pb_hip_backward(self._ctx, _abi.ptr(dc, C.c_double), _abi.ptr(dp, C.c_double),
-                                       _abi.ptr(out, C.c_double), None)
+                                       _abi.ptr(out, C.c_double), _abi.ptr(ctl, C.c_double))
         if st != _abi.PB_OK:
             raise RuntimeError(f"pb_hip_backward: {_abi.STATUS_NAMES.get(st, st)}: {self.last_error()}")
-        return out
+        return (out, ctl[: problem.num_control_points]) if with_control else out
 
     def quality(self, qopts) -> dict:
         """Post-solve quality of the last solve on this workspace (pb_hip_quality): the report fields plus

@@ -142,7 +142,7 @@ def ring_problem(seed: int = 1, num_cameras: int = 50, num_points: int = 2000, m
     intr1 = intrinsic_block(f, f, W / 2, H / 2, k1=dist[0], k2=dist[1], p1=dist[2], p2=dist[3])
     intr = np.repeat(intr1[None], num_cameras, axis=0)
     truth = rng.uniform(-2.0, 2.0, size=(num_points, 3))
-    tracks, pts = [], []
+    tracks, pts, truth_index = [], [], []
     for p in range(num_points):
         X = np.repeat(truth[p][None], num_cameras, axis=0)
         uv, z = project_points(rot, cen, intr, X)
@@ -154,6 +154,7 @@ def ring_problem(seed: int = 1, num_cameras: int = 50, num_points: int = 2000, m
         obs = uv[vis] + rng.normal(0.0, noise_px, size=(len(vis), 2))
         tracks.append((vis, obs))
         pts.append(truth[p] + rng.normal(0.0, point_sigma, size=3))
+        truth_index.append(p)
     # perturb every camera but the two gauge cameras 0 and 1 (as the reference benchmark does)
     dR = exp_so3(rng.normal(0.0, rot_sigma, size=(num_cameras, 3)))
     dC = rng.normal(0.0, center_sigma, size=(num_cameras, 3))
@@ -167,7 +168,53 @@ def ring_problem(seed: int = 1, num_cameras: int = 50, num_points: int = 2000, m
                    fixed_cameras=np.array([0, 1], dtype=np.int32),
                    gauge_policy=GaugePolicy.RequireExplicitGauge,
                    meta={"workload": "ring", "seed": seed, "truth_rotation": rot, "truth_center": cen,
-                         "truth_points": truth})
+                         "truth_points": truth[np.array(truth_index, dtype=np.int64)]})
+
+
+def add_survey_constraints(problem: Problem, seed: int = 11, num_control: int = 6, num_planes: int = 0,
+                           control_sigma: float = 0.02, plane_noise: float = 0.01, kinds=(0,),
+                           truth_points=None) -> Problem:
+    """Control points and LiDAR point-to-plane constraints on distinct tracks of a copy of `problem`.
+
+    Control points are surveyed truth positions plus N(0, control_sigma) per axis with the uncertainty forms in
+    `kinds` (cycled: 0 isotropic sigma, 1 covariance, 2 square-root information of a random SPD matrix whose
+    axis sigmas are about control_sigma). Laser planes pass within N(0, plane_noise) of the truth point with a
+    random unit normal. Truth points default to problem.meta["truth_points"] (one per track)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = problem.copy()
+    truth = np.asarray(truth_points if truth_points is not None else problem.meta["truth_points"], dtype=np.float64)
+    order = rng.permutation(problem.num_tracks)
+    cp_tracks = np.sort(order[:num_control]).astype(np.int32)
+    lp_tracks = np.sort(order[num_control:num_control + num_planes]).astype(np.int32)
+    if num_control:
+        n = num_control
+        kind = np.array([kinds[i % len(kinds)] for i in range(n)], dtype=np.int32)
+        point = truth[cp_tracks] + rng.normal(0.0, control_sigma, size=(n, 3))
+        matrix = np.zeros((n, 9))
+        for i in range(n):
+            Q = exp_so3(rng.normal(0.0, 0.5, size=3))
+            sig = control_sigma * rng.uniform(0.7, 1.5, size=3)
+            cov = Q @ np.diag(sig**2) @ Q.T
+            if kind[i] == 1:
+                matrix[i] = cov.reshape(9)
+            elif kind[i] == 2:
+                matrix[i] = (np.diag(1.0 / sig) @ Q.T).reshape(9)  # W^T W = cov^-1
+        out.cp_track = cp_tracks
+        out.cp_point = point
+        out.cp_sigma = np.full(n, control_sigma)
+        out.cp_weight = rng.uniform(0.5, 2.0, size=n)
+        out.cp_uncertainty = kind
+        out.cp_matrix = matrix
+    if num_planes:
+        n = num_planes
+        nrm = rng.normal(size=(n, 3))
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        tang = np.cross(nrm, rng.normal(size=(n, 3)))
+        out.lp_track = lp_tracks
+        out.lp_normal = nrm
+        out.lp_point = truth[lp_tracks] + tang + nrm * rng.normal(0.0, plane_noise, size=(n, 1))
+        out.lp_weight = rng.uniform(50.0, 200.0, size=n)
+    return out
 
 
 # --- aerial strip block (configuration 2) ------------------------------------------------------

@@ -28,8 +28,37 @@ def weight_matrix(r, w_obs, w_n, loss, d):
     return w_n * (np.eye(2) - P) + mu * P
 
 
-def adjoint_reference(O, problem, state, masks, prep, copts, dl_dC, dl_dX=None, priors=None):
-    """dl/d(u, v) per observation ([num_observations, 2]) at `state` (dict of the oracle / workspace download)."""
+def constraint_rows(O, problem, copts):
+    """Survey constraints as (track, J (rows x 3), target, delta, control index or -1): J = sqrt(w) A for a control
+    point, sqrt(w) n^T for a laser plane; delta is the Huber threshold (BundleAdjustPlaMatrixConstraints.cpp)."""
+    out = []
+    for i in range(problem.num_laser_planes):
+        sw = np.sqrt(copts.laser_plane_weight * problem.lp_weight[i])
+        out.append((int(problem.lp_track[i]), sw * problem.lp_normal[i][None, :], problem.lp_point[i],
+                    copts.laser_huber_delta * sw, -1))
+    for i in range(problem.num_control_points):
+        A = O.control_whitening(int(problem.cp_uncertainty[i]), float(problem.cp_sigma[i]), problem.cp_matrix[i])
+        sw = np.sqrt(copts.control_point_weight * problem.cp_weight[i])
+        out.append((int(problem.cp_track[i]), sw * A, problem.cp_point[i], copts.control_point_huber_delta, i))
+    return out
+
+
+def constraint_weight_matrix(r, delta):
+    """d(w r)/dr for Huber on s = |r|^2 (weight w = 1 inside, delta / sqrt(s) outside), along-r eigenvalue >= 0."""
+    s = float(r @ r)
+    w = 1.0 if s <= delta * delta else delta / np.sqrt(s)
+    if s == 0.0:
+        return w * np.eye(len(r))
+    mu = max(0.0, w + 4.0 * robust_curvature(s, 1, delta) * s)
+    Pm = np.outer(r, r) / s
+    return w * (np.eye(len(r)) - Pm) + mu * Pm
+
+
+def adjoint_reference(O, problem, state, masks, prep, copts, dl_dC, dl_dX=None, priors=None, with_control=False):
+    """dl/d(u, v) per observation ([num_observations, 2]) at `state` (dict of the oracle / workspace download);
+    with_control=True also returns dl/dP of every control point ([num_control_points, 3]). Problems with survey
+    constraints use the default parameterisation (world-frame rotation, unscaled point step)."""
+    ref_param = bool(copts.use_online_schur) and problem.num_control_points == 0 and problem.num_laser_planes == 0
     P = prep["primary_block_count"]
     Ne = prep["track_block_count"]
     n = P * P13 + 3 * Ne
@@ -47,7 +76,7 @@ def adjoint_reference(O, problem, state, masks, prep, copts, dl_dC, dl_dX=None, 
                 continue
             lin = O.linearize(state["cam_rotation"][c], state["cam_center"][c], state["cam_intrinsics"][c], 0, X,
                               problem.obs_uv[i, 0], problem.obs_uv[i, 1], problem.obs_weight[i],
-                              problem.obs_scale[i], copts.robust_loss, copts.robust_loss_scale, True,
+                              problem.obs_scale[i], copts.robust_loss, copts.robust_loss_scale, ref_param,
                               mask if masks["intrinsic_block_by_camera"][c] >= 0 else 0)
             assert lin is not None
             J = np.zeros((2, n))
@@ -63,6 +92,17 @@ def adjoint_reference(O, problem, state, masks, prep, copts, dl_dC, dl_dX=None, 
             M = weight_matrix(lin["residual"], w_obs, lin["weight"], copts.robust_loss, copts.robust_loss_scale)
             H += J.T @ M @ J
             rows.append((i, J, M))
+    cons = []
+    for t, Jc, target, delta, index in constraint_rows(O, problem, copts):
+        tb = masks["track_block"][t]
+        if not masks["active_track"][t]:
+            continue
+        r = Jc @ (state["track_point"][t] - target)
+        M = constraint_weight_matrix(r, delta)
+        if tb >= 0:
+            cols = slice(P * P13 + 3 * tb, P * P13 + 3 * tb + 3)
+            H[cols, cols] += Jc.T @ M @ Jc
+        cons.append((tb, Jc, M, index))
     for idx, h in (priors or {}).items():
         H[idx, idx] += h
     g = np.zeros(n)
@@ -76,11 +116,19 @@ def adjoint_reference(O, problem, state, masks, prep, copts, dl_dC, dl_dX=None, 
         for t in range(problem.num_tracks):
             tb = masks["track_block"][t]
             if tb >= 0:
-                g[P * P13 + 3 * tb: P * P13 + 3 * tb + 3] = Oo.point_scale(state["track_point"][t]) * dl_dX[t]
+                sc = Oo.point_scale(state["track_point"][t]) if ref_param else 1.0
+                g[P * P13 + 3 * tb: P * P13 + 3 * tb + 3] = sc * dl_dX[t]
     keep = np.diag(H) != 0.0
     lam = np.zeros(n)
     lam[keep] = np.linalg.solve(H[np.ix_(keep, keep)], g[keep])
     out = np.zeros((problem.num_observations, 2))
     for i, J, M in rows:
         out[i] = M @ (J @ lam)
-    return out
+    if not with_control:
+        return out
+    dctl = np.zeros((problem.num_control_points, 3))
+    for tb, Jc, M, index in cons:
+        if index >= 0 and tb >= 0:
+            lx = lam[P * P13 + 3 * tb: P * P13 + 3 * tb + 3]
+            dctl[index] = Jc.T @ (M @ (Jc @ lx))
+    return out, dctl

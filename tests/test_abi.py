@@ -43,8 +43,10 @@ def test_struct_layouts_match_header():
 #include "pb_hip.h"
 int main(void) {
   printf("%zu %zu %zu\n", sizeof(pb_problem_view), sizeof(pb_solve_options), sizeof(pb_solve_report));
-  printf("%zu %zu %zu\n", offsetof(pb_problem_view, gauge_baseline), offsetof(pb_solve_options, step_rms_tolerance),
-         offsetof(pb_solve_report, message));
+  printf("%zu %zu %zu %zu %zu %zu\n", offsetof(pb_problem_view, gauge_baseline),
+         offsetof(pb_solve_options, step_rms_tolerance), offsetof(pb_solve_report, message),
+         offsetof(pb_problem_view, cp_matrix), offsetof(pb_problem_view, lp_weight),
+         offsetof(pb_solve_options, control_point_huber_delta));
   printf("%zu %zu %zu %zu\n", sizeof(pb_batch_view), offsetof(pb_batch_view, windows), sizeof(pb_window_report),
          offsetof(pb_window_report, device_seconds));
   printf("%zu %zu %zu %zu\n", sizeof(pb_quality_options), offsetof(pb_quality_options, min_accepted_valid_track_ratio),
@@ -62,7 +64,8 @@ int main(void) {
     offs = [int(x) for x in lines[1].split()]
     assert sizes == [C.sizeof(_abi.ProblemView), C.sizeof(_abi.SolveOptionsC), C.sizeof(_abi.SolveReportC)]
     assert offs == [_abi.ProblemView.gauge_baseline.offset, _abi.SolveOptionsC.step_rms_tolerance.offset,
-                    _abi.SolveReportC.message.offset]
+                    _abi.SolveReportC.message.offset, _abi.ProblemView.cp_matrix.offset,
+                    _abi.ProblemView.lp_weight.offset, _abi.SolveOptionsC.control_point_huber_delta.offset]
     batch = [int(x) for x in lines[2].split()]
     assert batch == [C.sizeof(_abi.BatchView), _abi.BatchView.windows.offset, C.sizeof(_abi.WindowReportC),
                      _abi.WindowReportC.device_seconds.offset]

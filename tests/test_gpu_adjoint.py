@@ -112,7 +112,7 @@ def test_backward_matches_gpu_finite_differences(ws):
             assert abs(fd - g[i, k]) <= 2e-3 * np.abs(g).max(), (i, k, fd, g[i, k])
 
 
-def test_backward_rejects_control_points_and_needs_a_solve(ws):
+def test_backward_needs_a_solve(ws):
     import ctypes as C
 
     from plabundle_amd import _abi
@@ -124,7 +124,9 @@ def test_backward_rejects_control_points_and_needs_a_solve(ws):
     lib = _abi.load_library()
     assert lib.pb_hip_backward(fresh.handle, None, None, _abi.ptr(out, C.c_double), None) == _abi.PB_ERR_INVALID
     Solver().solve(p, o, fresh, precision=PO.Precision.Fp64)
-    ctrl = np.zeros(3)
+    ctrl = np.full(3, 7.0)
+    # a problem without control points leaves dl_dcontrol untouched
     assert lib.pb_hip_backward(fresh.handle, None, None, _abi.ptr(out, C.c_double),
-                               _abi.ptr(ctrl, C.c_double)) == _abi.PB_ERR_UNSUPPORTED
+                               _abi.ptr(ctrl, C.c_double)) == _abi.PB_OK
+    np.testing.assert_array_equal(ctrl, 7.0)
     fresh.close()
