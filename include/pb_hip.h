@@ -207,6 +207,12 @@ typedef struct pb_solve_options {
     double laser_huber_delta;            /* laserHuberDeltaMeters = 0.2 (scaled by sqrt(weight) per plane) */
     double control_point_weight;         /* controlPointWeight = 1 */
     double control_point_huber_delta;    /* controlPointHuberDeltaMeters = 0.2 (on the whitened residual) */
+
+    /* CalibrationOptions::useReferenceCalibrationTransitionPrior and referencePreviousIntrinsicParameterMask
+     * (options.h:118-119; BundleAdjustPlaMatrixModel.cpp:173-225): every refined slot but the aspect replaces its
+     * prior sigma by a transition weight, non-zero only when the refined set differs from the previous mask. */
+    int32_t use_transition_prior;        /* false */
+    uint32_t previous_intrinsic_mask;    /* bit i: slot i was refined in the previous stage */
 } pb_solve_options;
 
 /* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */

@@ -68,6 +68,8 @@ def lib():
     L.oracle_linearize.argtypes = [_dp, _dp, _dp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                    C.c_int, C.c_double, C.c_int, C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp]
     L.oracle_linearize.restype = C.c_int
+    L.oracle_groups.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint32)]
+    L.oracle_groups.restype = C.c_int
     L.oracle_control_whitening.argtypes = [C.c_int, C.c_double, _dp, _dp]
     L.oracle_control_whitening.restype = C.c_int
     _lib = L
@@ -114,6 +116,8 @@ def default_options() -> _abi.SolveOptionsC:
     o.laser_huber_delta = 0.2
     o.control_point_weight = 1.0
     o.control_point_huber_delta = 0.2
+    o.use_transition_prior = 0
+    o.previous_intrinsic_mask = 0
     return o
 
 
@@ -210,6 +214,19 @@ class Oracle:
         lib().oracle_quality(self.ctx, C.byref(qopts), C.byref(rep), _p(before), _p(after), _p(valid, C.c_int32))
         out = rep.as_dict()
         out.update(track_rms_before=before, track_rms_after=after, track_valid=valid)
+        return out
+
+    def groups(self) -> dict:
+        """Intrinsic groups after prepare(): params, prior, inv_sigma (transition weights when that prior is on),
+        lower, upper ([G, 13]) and the enabled slot masks ([G])."""
+        n = lib().oracle_groups(self.ctx, None, None, None, None, None, None)
+        if n < 0:
+            raise RuntimeError("oracle not prepared")
+        out = {k: np.zeros((n, 13)) for k in ("params", "prior", "inv_sigma", "lower", "upper")}
+        en = np.zeros(n, dtype=np.uint32)
+        lib().oracle_groups(self.ctx, *(_p(out[k]) for k in ("params", "prior", "inv_sigma", "lower", "upper")),
+                            _p(en, C.c_uint32))
+        out["enabled"] = en
         return out
 
     def initial_cost(self) -> float:

@@ -892,6 +892,7 @@ namespace
         struct Samples
         {
             std::array<std::vector<double>, kP> init, ref;
+            std::vector<double> widths, heights;
             SlotMask sup;
         };
         std::vector<Samples> smp(groups.size());
@@ -916,6 +917,10 @@ namespace
             }
             const SlotMask sup = supportedSlots(pb.cams[c].conv);
             for (int i = 0; i < kP; ++i) s.sup[i] = s.sup[i] && sup[i];
+            // appendSamples (Model.cpp:61-66): the image size, else twice the reference principal point
+            const bool sized = pb.cams[c].width > 0 || pb.cams[c].height > 0;
+            s.widths.push_back(sized ? pb.cams[c].width : 2.0 * r[2]);
+            s.heights.push_back(sized ? pb.cams[c].height : 2.0 * r[3]);
         }
         const pb_solve_options& o = opt.o;
         const double low = std::max(1.0, o.low_order_distortion_scale);
@@ -961,6 +966,31 @@ namespace
             {
                 g.params[i] = std::clamp(g.params[i], g.lower[i], g.upper[i]);
                 g.invSigma[i] = g.enabled[i] ? 1.0 / std::max(1e-9, sig[i]) : 0.0;
+            }
+            // useReferenceCalibrationTransitionPrior (Model.cpp:173-225). The transition weight replaces the
+            // inverse sigma of every slot but the aspect; its reference value is the prior (prior[0] is the focal
+            // reference), so the prior terms keep their form with this weight.
+            if (o.use_transition_prior)
+            {
+                bool changed = false;
+                for (int i = 0; i < kP; ++i)
+                    if (i != 1) changed = changed || g.enabled[i] != (((o.previous_intrinsic_mask >> i) & 1u) != 0);
+                const double w = medianOr(smp[gi].widths, 0.0), h = medianOr(smp[gi].heights, 0.0);
+                const double nx = 0.5 * w / g.focalRef, ny = 0.5 * h / (g.focalRef * g.aspectRef);
+                const double r = std::sqrt(nx * nx + ny * ny);
+                Slots tw{};
+                if (changed && std::isfinite(r) && r > 0.0)
+                {
+                    const double f = g.focalRef;
+                    const Slots base{1.0 / r, 0.0, 1.0, 1.0, 1.0 / (std::pow(r, 3) * f), 1.0 / (std::pow(r, 5) * f),
+                                     1.0 / (std::pow(r, 7) * f), 1.0 / (r * r * f), 1.0 / (r * r * f), 1.0 / r,
+                                     1.0 / (std::pow(r, 9) * f), 1.0, 1.0};
+                    for (int i = 0; i < kP; ++i)
+                        if (i != 1 && g.enabled[i])
+                            tw[i] = 1.0 / (base[i] * (((o.previous_intrinsic_mask >> i) & 1u) ? 1e6 : 0.01));
+                }
+                for (int i = 0; i < kP; ++i)
+                    if (i != 1) g.invSigma[i] = tw[i];
             }
         }
         return groups;
@@ -2608,6 +2638,31 @@ double oracle_cost_after_step(oracle_ctx* ctx, const double* primary, const doub
     std::vector<double> e(elim, elim + static_cast<std::size_t>(o.act.trackBlockCount) * kE);
     applyStep(o, p, e, &st, alpha);
     return evaluateObjective(o, st);
+}
+
+// Intrinsic groups after preparation: per group 13 values each of params, prior, inverse sigma (or transition
+// weight), lower, upper, and the enabled mask. Returns the group count (-1 when not prepared).
+int oracle_groups(oracle_ctx* ctx, double* params, double* prior, double* inv_sigma, double* lower, double* upper,
+                  uint32_t* enabled)
+{
+    Oracle& o = ctx->o;
+    if (!o.prepared) return -1;
+    const auto& gs = o.init.groups;
+    for (std::size_t g = 0; g < gs.size(); ++g)
+    {
+        uint32_t m = 0;
+        for (int i = 0; i < kP; ++i)
+        {
+            if (params) params[g * kP + i] = gs[g].params[i];
+            if (prior) prior[g * kP + i] = gs[g].prior[i];
+            if (inv_sigma) inv_sigma[g * kP + i] = gs[g].invSigma[i];
+            if (lower) lower[g * kP + i] = gs[g].lower[i];
+            if (upper) upper[g * kP + i] = gs[g].upper[i];
+            if (gs[g].enabled[i]) m |= 1u << i;
+        }
+        if (enabled) enabled[g] = m;
+    }
+    return static_cast<int>(gs.size());
 }
 
 double oracle_initial_cost(oracle_ctx* ctx)

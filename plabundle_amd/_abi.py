@@ -113,6 +113,8 @@ class SolveOptionsC(C.Structure):
         ("laser_huber_delta", C.c_double),
         ("control_point_weight", C.c_double),
         ("control_point_huber_delta", C.c_double),
+        ("use_transition_prior", C.c_int32),
+        ("previous_intrinsic_mask", C.c_uint32),
     ]
 
 

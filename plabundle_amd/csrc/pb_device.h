@@ -19,7 +19,7 @@
 namespace pbdev
 {
     constexpr int kSlots = 13;
-    constexpr int kMaxActive = 12;  // active intrinsic slots per group handled on device (2 units of 6)
+    constexpr int kMaxActive = 13;  // active intrinsic slots per group handled on device (all 13: 3 units of 6)
 
     // Effective camera used by every kernel of one pass (pose + intrinsics after the shared-group overlay).
     struct Camera
@@ -216,7 +216,7 @@ namespace pbdev
     }
 
     // Full linearisation of one observation. nActive/activeSlots select the intrinsic columns.
-    // KM: compile-time bound on the active intrinsic slots (0, 6 or 12) so that Ji lives in registers.
+    // KM: compile-time bound on the active intrinsic slots (0, 6, 12 or 13) so that Ji lives in registers.
     template <int KM>
     PB_HD bool linearize(const Camera& c, double X0, double X1, double X2, double ou, double ov,
                                               double ow, int loss, double lscale, int nActive,
@@ -411,8 +411,8 @@ namespace pbdev
         return finite(delta) && delta > 0.0 && robust(s, 1, delta, cost, weight) && finite(cost) && finite(weight);
     }
 
-    // Symmetric 3x3 inverse (row-major) with the singularity test |det| <= 1e-15 max|m|^3.
-    PB_HD bool inverse3(const double* m, double* inv)
+    // Symmetric 3x3 inverse (row-major) with the singularity test |det| <= tol max|m|^3 (tol = 1e-15).
+    PB_HD bool inverse3(const double* m, double* inv, double tol = 1e-15)
     {
         double mx = 0.0;
 #pragma unroll
@@ -425,7 +425,7 @@ namespace pbdev
         const double c01 = m[5] * m[6] - m[3] * m[8];
         const double c02 = m[3] * m[7] - m[4] * m[6];
         const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
-        if (!finite(det) || !(fabs(det) > 1e-15 * mx * mx * mx)) return false;
+        if (!finite(det) || !(fabs(det) > tol * mx * mx * mx)) return false;
         const double id = 1.0 / det;
         inv[0] = c00 * id;
         inv[1] = (m[2] * m[7] - m[1] * m[8]) * id;

@@ -61,6 +61,7 @@ namespace
         int nCam = 0, nTracks = 0;
         std::vector<double> R, C, slots, ref;
         std::vector<int32_t> conv, group;
+        std::vector<int32_t> imageSize;  // [nCam*2] width, height, or empty
         bool hasGroup = false;
         std::vector<double> X;
         std::vector<int64_t> off;
@@ -390,6 +391,7 @@ namespace
         for (int g = 0; g < G; ++g)
         {
             std::array<std::vector<double>, 13> init, ref;
+            std::vector<double> widths, heights;
             uint32_t sup = 0x1FFFu;
             for (int c = 0; c < in.nCam; ++c)
             {
@@ -410,6 +412,10 @@ namespace
                     ref[k].push_back(r[k]);
                 }
                 sup &= supportedMask(in.conv[c]);
+                // image size of the camera, else twice the reference principal point (Model.cpp:61-66)
+                const bool sized = !in.imageSize.empty();
+                widths.push_back(sized ? double(in.imageSize[2 * c]) : 2.0 * r[2]);
+                heights.push_back(sized ? double(in.imageSize[2 * c + 1]) : 2.0 * r[3]);
             }
             double* gp = &p->gParams[13 * g];
             double* pr = &p->gPrior[13 * g];
@@ -455,13 +461,39 @@ namespace
                 p->gInvSigma[13 * g + k] = on ? 1.0 / std::max(1e-9, sig[k]) : 0.0;
                 if (on)
                 {
-                    if (K >= pbdev::kMaxActive) throw std::invalid_argument("unsupported: more than 12 refined slots");
+                    if (K >= pbdev::kMaxActive) throw std::invalid_argument("unsupported: more refined slots than the 13-slot block");
                     p->gActive[pbdev::kMaxActive * g + K] = static_cast<int8_t>(k);
                     p->slotToCompact[13 * g + k] = K;
                     ++K;
                 }
             }
             p->gK[g] = K;
+            if (o.use_transition_prior)
+            {
+                // reference calibration transition prior (Model.cpp:173-225): every slot but the aspect trades
+                // its prior sigma for a transition weight, which is non-zero only when the refined set differs
+                // from the previous stage's; its scale follows the image half-diagonal r in normalised
+                // coordinates (slot effect at the image corner), times 1e6 for a slot refined before, 0.01 else
+                bool changed = false;
+                for (int k = 0; k < 13; ++k)
+                    if (k != 1) changed = changed || (((en >> k) & 1u) != ((o.previous_intrinsic_mask >> k) & 1u));
+                const double nx = 0.5 * median(widths, 0.0) / fref;
+                const double ny = 0.5 * median(heights, 0.0) / (fref * aref);
+                const double r = std::sqrt(nx * nx + ny * ny);
+                double tw[13] = {};
+                if (changed && std::isfinite(r) && r > 0.0)
+                {
+                    const double r2 = r * r, r3 = r2 * r, r5 = r3 * r2, r7 = r5 * r2, r9 = r7 * r2;
+                    const double base[13] = {1.0 / r, 0.0, 1.0, 1.0, 1.0 / (r3 * fref), 1.0 / (r5 * fref),
+                                             1.0 / (r7 * fref), 1.0 / (r2 * fref), 1.0 / (r2 * fref), 1.0 / r,
+                                             1.0 / (r9 * fref), 1.0, 1.0};
+                    for (int k = 0; k < 13; ++k)
+                        if (k != 1 && ((en >> k) & 1u))
+                            tw[k] = 1.0 / (base[k] * (((o.previous_intrinsic_mask >> k) & 1u) ? 1e6 : 0.01));
+                }
+                for (int k = 0; k < 13; ++k)
+                    if (k != 1) p->gInvSigma[13 * g + k] = tw[k];
+            }
         }
         // units: pose units first (ids = camera block ids), then ceil(K/6) per group
         int u = p->cameraBlocks;
@@ -803,7 +835,7 @@ namespace
         }
         // direct-part block ids
         L->camDiagBlock.assign(n, -1);
-        L->camGroupBlock.assign(2 * n, -1);
+        L->camGroupBlock.assign(size_t(pbk::kGroupUnits) * n, -1);
         for (int c = 0; c < n; ++c)
         {
             const int pu = L->camUnit[c];
@@ -811,9 +843,9 @@ namespace
             L->camDiagBlock[c] = blockId(pu, pu);
             int g0 = 0;
             const int m = groupUnits(c, &g0);
-            for (int j = 0; j < m; ++j) L->camGroupBlock[2 * c + j] = blockId(pu, g0 + j);
+            for (int j = 0; j < m; ++j) L->camGroupBlock[pbk::kGroupUnits * c + j] = blockId(pu, g0 + j);
         }
-        L->groupPairBlock.assign(4 * std::max(1, p.G), -1);
+        L->groupPairBlock.assign(size_t(pbk::kGroupUnits) * pbk::kGroupUnits * std::max(1, p.G), -1);
         L->groupCamOffset.assign(p.G + 1, 0);
         for (int g = 0; g < p.G; ++g)
         {
@@ -821,7 +853,9 @@ namespace
             {
                 const int m = (p.gK[g] + 5) / 6;
                 for (int i = 0; i < m; ++i)
-                    for (int j = i; j < m; ++j) L->groupPairBlock[4 * g + 2 * i + j] = blockId(p.gUnit[g] + i, p.gUnit[g] + j);
+                    for (int j = i; j < m; ++j)
+                        L->groupPairBlock[(pbk::kGroupUnits * g + i) * pbk::kGroupUnits + j] =
+                            blockId(p.gUnit[g] + i, p.gUnit[g] + j);
             }
             for (int c = 0; c < n; ++c)
                 if (L->camGroup[c] == g) L->groupCams.push_back(c);
@@ -916,9 +950,7 @@ struct pb_ctx
     bool uploaded = false;
     // prepared structure (kept while the problem and the structural options stay the same)
     bool structureValid = false;
-    uint32_t structureMask = 0;
-    int structureRefinePose = -1;
-    double structureMaxRms = -1.0;
+    pb_solve_options structureOptions{};  // options the prepared structure was built with
     Prepared prep;
     Layout lay;
     DeviceAlloc dstruct;  // structure + state
@@ -1120,6 +1152,7 @@ namespace
                                    : std::vector<int32_t>(n, PB_BROWN_OPENCV);
         r.hasGroup = v->cam_group != nullptr;
         if (r.hasGroup) r.group.assign(v->cam_group, v->cam_group + n);
+        if (v->cam_image_size) r.imageSize.assign(v->cam_image_size, v->cam_image_size + 2 * n);
         r.X.assign(v->track_point, v->track_point + 3 * static_cast<size_t>(r.nTracks));
         r.off.assign(v->track_offset, v->track_offset + r.nTracks + 1);
         const int64_t no = r.off.back();
@@ -1200,11 +1233,29 @@ namespace
     void releaseStructure(pb_ctx* ctx);
     void uploadStructure(pb_ctx* ctx, Prepared&& p, Layout&& L, const pb_solve_options& o);
 
+    // The options a prepared structure depends on: the active problem (refined pose and slots, initial-RMS
+    // gate) and the intrinsic groups (bounds, prior sigmas, transition prior).
+    bool sameStructureOptions(const pb_solve_options& a, const pb_solve_options& b)
+    {
+        bool same = a.shared_intrinsic_mask == b.shared_intrinsic_mask && a.refine_camera_pose == b.refine_camera_pose &&
+                    a.max_initial_track_rms == b.max_initial_track_rms && a.min_focal_scale == b.min_focal_scale &&
+                    a.max_focal_scale == b.max_focal_scale && a.min_aspect_scale == b.min_aspect_scale &&
+                    a.max_aspect_scale == b.max_aspect_scale &&
+                    a.max_principal_offset_fraction == b.max_principal_offset_fraction &&
+                    a.focal_prior_sigma == b.focal_prior_sigma &&
+                    a.principal_prior_sigma_fraction == b.principal_prior_sigma_fraction &&
+                    a.aspect_prior_sigma == b.aspect_prior_sigma &&
+                    a.low_order_distortion_scale == b.low_order_distortion_scale &&
+                    a.use_transition_prior == b.use_transition_prior &&
+                    a.previous_intrinsic_mask == b.previous_intrinsic_mask;
+        for (int i = 0; i < 9; ++i) same = same && a.max_abs[i] == b.max_abs[i] && a.prior_sigma[i] == b.prior_sigma[i];
+        return same;
+    }
+
     // Builds the prepared structure and uploads it (setup; once per problem and structural options).
     void ensureStructure(pb_ctx* ctx, const pb_solve_options& o)
     {
-        if (ctx->structureValid && ctx->structureMask == o.shared_intrinsic_mask &&
-            ctx->structureRefinePose == o.refine_camera_pose && ctx->structureMaxRms == o.max_initial_track_rms)
+        if (ctx->structureValid && sameStructureOptions(ctx->structureOptions, o))
             return;
         releaseStructure(ctx);
         Prepared p;
@@ -1348,9 +1399,7 @@ namespace
         ctx->prep = std::move(p);
         ctx->lay = std::move(L);
         ctx->structureValid = true;
-        ctx->structureMask = o.shared_intrinsic_mask;
-        ctx->structureRefinePose = o.refine_camera_pose;
-        ctx->structureMaxRms = o.max_initial_track_rms;
+        ctx->structureOptions = o;
         ctx->costBlocks = std::max(1, std::min(2048, (P.nObs + 255) / 256));
     }
 
@@ -1625,18 +1674,37 @@ namespace
         std::snprintf(r->message, sizeof(r->message), "%s", m.c_str());
     }
 
+    // Controller state of one solve, shared by the two parts of a mixed solve: when an FP32 step fails its line
+    // search (path B) or is rejected (path A), the FP32 part hands the state, damping and counters to an FP64
+    // continuation with the exact PCG stop of the FP64 path, which repeats the step from the same state (the FP32
+    // rounding or the truncated PCG failed, not the state).
+    struct Progress
+    {
+        bool started = false;
+        bool inexact = false;     // truncated-Newton PCG stop and cap (a mixed solve, both parts)
+        bool fallback = false;    // set by an FP32 part that hands over
+        bool fp64Finish = false;  // the FP64 continuation of a mixed solve ran
+        Clock::time_point t0{};
+        double current = std::numeric_limits<double>::quiet_NaN();
+        double damping = 0.0;
+        int accepted = 0, rejected = 0, iterations = 0;
+        bool converged = false, cancelled = false;
+    };
+
     template <typename T>
-    void runSolveAssembled(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel);
+    void runSolveAssembled(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel,
+                           Progress& pg);
     template <typename T>
-    void publishState(pb_ctx* ctx, pb_solve_report* rep, double current, bool converged, bool cancelled);
+    void publishState(pb_ctx* ctx, pb_solve_report* rep, const Progress& pg);
 
     // The online-Schur controller (BundleAdjustPlaMatrix.cpp:162-631 with useReferenceOnlineSchur).
     template <typename T>
-    void runSolve(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel)
+    void runSolve(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel,
+                  Progress& pg)
     {
         if (!onlineSchur(ctx, o))
         {
-            runSolveAssembled<T>(ctx, o, rep, cancel);
+            runSolveAssembled<T>(ctx, o, rep, cancel, pg);
             return;
         }
         using Lr = pbk::Launcher<T>;
@@ -1644,9 +1712,16 @@ namespace
         hipStream_t s = ctx->stream;
         const pbk::Problem& P = ctx->P;
         const Prepared& pr = ctx->prep;
-        resetState(ctx);
-        check(hipStreamSynchronize(s), "reset");
-        const bool mixed = std::is_same<T, float>::value;
+        const bool fp32 = std::is_same<T, float>::value;
+        if (!pg.started)
+        {
+            resetState(ctx);
+            check(hipStreamSynchronize(s), "reset");
+            pg.t0 = Clock::now();
+            pg.converged = pr.primaryBlocks == 0 && pr.trackBlocks == 0;
+            pg.started = true;
+        }
+        const bool mixed = pg.inexact;
         const double relTol = o.pcg_relative_tolerance > 0.0 ? o.pcg_relative_tolerance : (mixed ? 1e-3 : 2e-10);
         const double absTol = o.pcg_absolute_tolerance > 0.0 ? o.pcg_absolute_tolerance : 1e-12;
         // iteration cap: the reference's max(200, 30 P) for exact solves; for truncated-Newton steps 0.6 P, which
@@ -1659,19 +1734,18 @@ namespace
         const double modelEta = o.pcg_model_eta >= 0.0 ? o.pcg_model_eta : (mixed ? 0.1 : 0.0);
         const double rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
         rep->linear_tolerance = relTol;
-        rep->mixed_precision_used = mixed ? 1 : 0;
+        rep->mixed_precision_used = (mixed || pg.fp64Finish) ? 1 : 0;
         ctx->lastScalarBytes = sizeof(T);
         rep->online_schur_used = 1;
 
-        const auto t0 = Clock::now();
-        double current = std::numeric_limits<double>::quiet_NaN();
-        double damping = 0.0;
-        int accepted = 0, rejected = 0, iterations = 0;
-        bool converged = pr.primaryBlocks == 0 && pr.trackBlocks == 0;
-        bool cancelled = false;
+        double& current = pg.current;
+        double& damping = pg.damping;
+        int &accepted = pg.accepted, &rejected = pg.rejected, &iterations = pg.iterations;
+        bool& converged = pg.converged;
+        bool& cancelled = pg.cancelled;
         auto isCancelled = [&]() { return cancel && *cancel != 0; };
         auto raise = [&]() { damping = damping == 0.0 ? 1e-3 : damping * 10.0; };
-        if (converged)
+        if (converged && std::isnan(current))
         {
             const BuildResult b = build(ctx, W, 0.0);
             current = b.cost;
@@ -1802,26 +1876,33 @@ namespace
                 else
                 {
                     ++rejected;
+                    if (fp32)
+                    {
+                        pg.fallback = true;  // an FP64 continuation repeats the step from this state
+                        return;
+                    }
                     terminate = true;
                     relinearize = true;
                 }
             }
         }
         check(hipStreamSynchronize(s), "solve");
-        rep->solve_seconds = seconds(t0);
-        rep->accepted_steps = accepted;
-        rep->rejected_steps = rejected;
-        rep->iterations = iterations;
-        publishState<T>(ctx, rep, current, converged, cancelled);
+        publishState<T>(ctx, rep, pg);
     }
 
     // Final report fields and the refined state back to the host (both controllers).
     template <typename T>
-    void publishState(pb_ctx* ctx, pb_solve_report* rep, double current, bool converged, bool cancelled)
+    void publishState(pb_ctx* ctx, pb_solve_report* rep, const Progress& pg)
     {
         hipStream_t s = ctx->stream;
         const pbk::Problem& P = ctx->P;
         const Prepared& pr = ctx->prep;
+        const double current = pg.current;
+        const bool converged = pg.converged, cancelled = pg.cancelled;
+        rep->solve_seconds = seconds(pg.t0);
+        rep->accepted_steps = pg.accepted;
+        rep->rejected_steps = pg.rejected;
+        rep->iterations = pg.iterations;
         rep->final_cost = current;
         collectProfile(ctx);
         if (cancelled)
@@ -1835,9 +1916,11 @@ namespace
         rep->converged = converged ? 1 : 0;
         rep->solve_status = converged ? PB_SOLVE_SUCCESS : PB_SOLVE_NO_CONVERGENCE;
         rep->solution_usable = std::isfinite(current) ? 1 : 0;
-        setMessage(rep, converged ? (rep->online_schur_used ? "converged (online Schur + block-Jacobi PCG + Armijo)"
-                                                            : "converged (assembled Schur + block-Jacobi PCG + line search)")
-                                  : "iteration budget or line search exhausted");
+        std::string msg = converged ? (rep->online_schur_used ? "converged (online Schur + block-Jacobi PCG + Armijo)"
+                                                              : "converged (assembled Schur + block-Jacobi PCG + line search)")
+                                    : "iteration budget or line search exhausted";
+        if (pg.fp64Finish) msg += "; finished in FP64 with exact steps after a mixed step failed";
+        setMessage(rep, msg);
         // refined state back to the host
         ctx->outR.resize(ctx->in.R.size());
         ctx->outC.resize(ctx->in.C.size());
@@ -1865,16 +1948,24 @@ namespace
     // point block is a linear-solver exception, which this path does not retry (:357-364); a reduced diagonal
     // block that is not positive definite counts as a solve that did not converge.
     template <typename T>
-    void runSolveAssembled(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel)
+    void runSolveAssembled(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel,
+                           Progress& pg)
     {
         using Lr = pbk::Launcher<T>;
         pbk::Work<T>& W = work<T>(ctx);
         hipStream_t s = ctx->stream;
         const pbk::Problem& P = ctx->P;
         const Prepared& pr = ctx->prep;
-        resetState(ctx);
-        check(hipStreamSynchronize(s), "reset");
-        const bool mixed = std::is_same<T, float>::value;
+        const bool fp32 = std::is_same<T, float>::value;
+        if (!pg.started)
+        {
+            resetState(ctx);
+            check(hipStreamSynchronize(s), "reset");
+            pg.t0 = Clock::now();
+            pg.converged = pr.primaryBlocks == 0 && pr.trackBlocks == 0;
+            pg.started = true;
+        }
+        const bool mixed = pg.inexact;
         const double relTol = o.pcg_relative_tolerance > 0.0 ? o.pcg_relative_tolerance : (mixed ? 1e-3 : 1e-12);
         const double absTol = o.pcg_absolute_tolerance > 0.0 ? o.pcg_absolute_tolerance : 1e-12;
         const int maxPcg = o.pcg_max_iterations > 0 ? o.pcg_max_iterations
@@ -1886,16 +1977,15 @@ namespace
         const double rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
         constexpr double kProbe = 1e-4;
         rep->linear_tolerance = relTol;
-        rep->mixed_precision_used = mixed ? 1 : 0;
+        rep->mixed_precision_used = (mixed || pg.fp64Finish) ? 1 : 0;
         rep->online_schur_used = 0;
         ctx->lastScalarBytes = sizeof(T);
 
-        const auto t0 = Clock::now();
-        double current = std::numeric_limits<double>::quiet_NaN();
-        double damping = 0.0;
-        int accepted = 0, rejected = 0, iterations = 0;
-        bool converged = pr.primaryBlocks == 0 && pr.trackBlocks == 0;
-        bool cancelled = false;
+        double& current = pg.current;
+        double& damping = pg.damping;
+        int &accepted = pg.accepted, &rejected = pg.rejected, &iterations = pg.iterations;
+        bool& converged = pg.converged;
+        bool& cancelled = pg.cancelled;
         auto isCancelled = [&]() { return cancel && *cancel != 0; };
         auto raise = [&]() { damping = damping == 0.0 ? 1e-3 : damping * 10.0; };
         auto timedBuild = [&]() {
@@ -1905,7 +1995,7 @@ namespace
             if (b.invalid) throw std::runtime_error("assembled Schur linearisation produced an invalid projection");
             return b;
         };
-        if (converged)
+        if (converged && std::isnan(current))
         {
             const BuildResult b = build(ctx, W, 0.0);
             current = b.cost;
@@ -2031,16 +2121,17 @@ namespace
                 else
                 {
                     ++rejected;
+                    if (fp32)
+                    {
+                        pg.fallback = true;  // an FP64 continuation relinearises and repeats the step
+                        return;
+                    }
                     raise();
                 }
             }
         }
         check(hipStreamSynchronize(s), "solve");
-        rep->solve_seconds = seconds(t0);
-        rep->accepted_steps = accepted;
-        rep->rejected_steps = rejected;
-        rep->iterations = iterations;
-        publishState<T>(ctx, rep, current, converged, cancelled);
+        publishState<T>(ctx, rep, pg);
     }
 
     void fillReportCounts(const pb_ctx* ctx, pb_solve_report* r)
@@ -2705,6 +2796,8 @@ void pb_default_options(pb_solve_options* o)
     o->laser_huber_delta = 0.2;
     o->control_point_weight = 1.0;
     o->control_point_huber_delta = 0.2;
+    o->use_transition_prior = 0;
+    o->previous_intrinsic_mask = 0;
     o->refine_camera_pose = 1;
     o->shared_intrinsic_mask = 0;
     o->min_focal_scale = 0.5;
@@ -2873,10 +2966,24 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
         check(hipStreamSynchronize(ctx->stream), "setup");
         report->setup_seconds = seconds(ts);
         ctx->lastOptions = *options;
-        if (options->precision == PB_PRECISION_MIXED)
-            runSolve<float>(ctx, *options, report, cancel);
+        Progress pg;
+        pg.inexact = options->precision == PB_PRECISION_MIXED;
+        if (pg.inexact)
+        {
+            runSolve<float>(ctx, *options, report, cancel, pg);
+            if (pg.fallback)
+            {
+                (void)work<double>(ctx);
+                pg.fallback = false;
+                pg.fp64Finish = true;
+                pg.inexact = false;
+                runSolve<double>(ctx, *options, report, cancel, pg);
+            }
+        }
         else
-            runSolve<double>(ctx, *options, report, cancel);
+        {
+            runSolve<double>(ctx, *options, report, cancel, pg);
+        }
         ctx->lastSolveStatus = report->solve_status;
         ctx->lastUsable = report->solution_usable;
         if (report->solve_status == PB_SOLVE_CANCELLED) return PB_ERR_CANCELLED;
@@ -2938,10 +3045,15 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
             throw;
         }
         P.adjoint = 0;
-        if (br.invalid || br.singular || br.notSpd)
+        const long nRows = 6L * P.nUnits;
+        // a 6x6 diagonal block that is not positive definite only stops the PCG route; the dense route reports
+        // through its factorisation
+        if (br.invalid || br.singular || (br.notSpd && nRows > kDenseAdjointMax))
         {
             tmp.release();
-            return fail(ctx, PB_ERR_NUMERIC, "adjoint system is singular or has an invalid projection");
+            return fail(ctx, PB_ERR_NUMERIC, br.invalid    ? "adjoint linearisation has an invalid projection"
+                                             : br.singular ? "adjoint system has a singular point block"
+                                                           : "adjoint system has a diagonal block that is not positive definite");
         }
         check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
         Lr::adjointRhs(P, W, dC, dP, s);

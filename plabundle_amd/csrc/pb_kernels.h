@@ -18,7 +18,8 @@
 namespace pbk
 {
     constexpr int kMaxAlpha = 8;             // step lengths evaluated per trial-cost pass
-    constexpr int kGroupPartialStride = 96;  // >= K(K+1)/2 + K for K <= 12
+    constexpr int kGroupUnits = 3;           // 6-wide units of an intrinsic group with all 13 slots refined
+    constexpr int kGroupPartialStride = 104;  // >= K(K+1)/2 + K for K <= 13
     constexpr int kPairChunk = 2048;         // pairs per Schur work item
     constexpr int kXcds = 8;                 // accelerator dies (XCDs) of an MI355X, each with its own L2
     constexpr int kCgWaves = 8;              // waves per workgroup of the fused PCG iteration
@@ -30,6 +31,9 @@ namespace pbk
     constexpr int kCgPartLoads = 4;          // unrolled row-partial loads per thread (rows up to 4 * 64 * kCgWaves)
     constexpr int kCgDirect = 60;            // column ids per slot stored by slot (the prefetched steps)
     constexpr int kCgSegment = 20;           // stored blocks per wave before a row gets another wave
+    // PCG iterations before the truncated-Newton model test may stop (the first iterations of a badly scaled
+    // system barely move the model; a mixed step that still fails its line search is repeated in exact FP64)
+    constexpr int kCgModelMinIterations = 3;
 
     enum Flag
     {
@@ -109,7 +113,7 @@ namespace pbk
         const int32_t* camObsOffset = nullptr;
         const int32_t* camObs = nullptr;
         const int32_t* camDiagBlock = nullptr;
-        const int32_t* camGroupBlock = nullptr;  // [nCam*2]
+        const int32_t* camGroupBlock = nullptr;  // [nCam*kGroupUnits] (pose unit, group unit j) block ids
 
         // intrinsic groups
         double* groupParams = nullptr;  // [nGroup*13]
@@ -123,7 +127,7 @@ namespace pbk
         const int32_t* groupUnit = nullptr;
         const int32_t* groupCamOffset = nullptr;
         const int32_t* groupCams = nullptr;
-        const int32_t* groupPairBlock = nullptr;  // [nGroup*4] (unit i, unit j) block ids, i <= j < 2
+        const int32_t* groupPairBlock = nullptr;  // [nGroup*kGroupUnits^2] (unit i, unit j) block ids, i <= j
         const int32_t* slotToCompact = nullptr;   // [nGroup*13]
 
         // observations (usable observations of active tracks, track order)

@@ -251,7 +251,19 @@ namespace pbk
         Hd[4] *= 1.0 + damping;
         Hd[8] *= 1.0 + damping;
         double inv[9];
-        if (!pbdev::inverse3(Hd, inv))
+        bool invertible = pbdev::inverse3(Hd, inv);
+        if (!invertible && P.adjoint)
+        {
+            // the adjoint's weight matrices are rank-deficient for observations on the clamped branch of the
+            // loss (a two-view outlier track can leave its point block singular): a ridge of 1e-9 of the largest
+            // diagonal entry holds the directions the data leave free
+            const double ridge = 1e-9 * fmax(fabs(Hd[0]), fmax(fabs(Hd[4]), fabs(Hd[8])));
+            Hd[0] += ridge;
+            Hd[4] += ridge;
+            Hd[8] += ridge;
+            invertible = ridge > 0.0 && pbdev::inverse3(Hd, inv, 1e-30);
+        }
+        if (!invertible)
         {
             setFlag(flags, kFlagSingular);
 #pragma unroll
@@ -437,7 +449,7 @@ namespace pbk
                     const int j = q / 36, r = (q % 36) / 6, col = q % 6;
                     const int k2 = 6 * j + col;
                     if (6 * j < K)
-                        Wk.D[P.camGroupBlock[c * 2 + j] * 36 + r * 6 + col] = k2 < K ? red[21 + r * KM + k2] : 0.0;
+                        Wk.D[P.camGroupBlock[c * kGroupUnits + j] * 36 + r * 6 + col] = k2 < K ? red[21 + r * KM + k2] : 0.0;
                 }
             }
         }
@@ -502,7 +514,7 @@ namespace pbk
                 const int a = i < j ? i : j, bb = i < j ? j : i;
                 v = red[a * KM - a * (a - 1) / 2 + (bb - a)];
             }
-            Wk.D[P.groupPairBlock[g * 4 + bi * 2 + bj] * 36 + r * 6 + col] = v;
+            Wk.D[P.groupPairBlock[(g * kGroupUnits + bi) * kGroupUnits + bj] * 36 + r * 6 + col] = v;
         }
         for (int q = threadIdx.x; q < m * 6; q += blockDim.x)
             Wk.directRhs[(u0 + q / 6) * 6 + q % 6] = q < K ? red[kSym + q] : 0.0;
@@ -889,7 +901,8 @@ namespace pbk
             const bool conv = rho <= thrSq;
             // truncated-Newton test of the previous step on the quadratic model Q(x) = x'Sx/2 - b'x:
             // it (Q_it - Q_{it-1}) / Q_it < eta, multiplied out with Q_it < 0
-            const bool modelStop = !conv && it > 0 && modelEta > 0.0 && qIt < 0.0 && double(it) * dqPrev > modelEta * qIt;
+            const bool modelStop = !conv && it >= kCgModelMinIterations && modelEta > 0.0 && qIt < 0.0 &&
+                                   double(it) * dqPrev > modelEta * qIt;
             double den = delta;
             if (it > 0)
             {
@@ -1524,8 +1537,10 @@ namespace pbk
             k_linearize_points<T, 0><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
         else if (P.maxGroupK <= 6)
             k_linearize_points<T, 6><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
-        else
+        else if (P.maxGroupK <= 12)
             k_linearize_points<T, 12><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
+        else
+            k_linearize_points<T, 13><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
     }
 
     template <typename T>
@@ -1536,16 +1551,20 @@ namespace pbk
             k_camera_direct<T, 0><<<P.nCam, 256, 0, s>>>(P, W);
         else if (P.maxGroupK <= 6)
             k_camera_direct<T, 6><<<P.nCam, 256, 0, s>>>(P, W);
-        else
+        else if (P.maxGroupK <= 12)
             k_camera_direct<T, 12><<<P.nCam, 256, 0, s>>>(P, W);
+        else
+            k_camera_direct<T, 13><<<P.nCam, 256, 0, s>>>(P, W);
         if (P.nGroup > 0)
         {
             if (P.maxGroupK == 0)
                 ;
             else if (P.maxGroupK <= 6)
                 k_group_reduce<T, 6><<<P.nGroup, 128, 0, s>>>(P, W);
-            else
+            else if (P.maxGroupK <= 12)
                 k_group_reduce<T, 12><<<P.nGroup, 128, 0, s>>>(P, W);
+            else
+                k_group_reduce<T, 13><<<P.nGroup, 128, 0, s>>>(P, W);
         }
         k_finish_cost<T><<<1, 256, 0, s>>>(P, W);
     }
@@ -1617,8 +1636,10 @@ namespace pbk
             k_adjoint_observations<T, 0><<<nb, 128, 0, s>>>(P, W, out);
         else if (P.maxGroupK <= 6)
             k_adjoint_observations<T, 6><<<nb, 128, 0, s>>>(P, W, out);
-        else
+        else if (P.maxGroupK <= 12)
             k_adjoint_observations<T, 12><<<nb, 128, 0, s>>>(P, W, out);
+        else
+            k_adjoint_observations<T, 13><<<nb, 128, 0, s>>>(P, W, out);
     }
 
     template <typename T>

@@ -107,6 +107,8 @@ class CalibrationOptions:
     sharedTangentialP3PriorSigma: float = 0.15
     sharedTangentialP4PriorSigma: float = 0.15
     sharedLowOrderDistortionScale: float = 1.0
+    useReferenceCalibrationTransitionPrior: bool = False
+    referencePreviousIntrinsicParameterMask: list = field(default_factory=lambda: [False] * 13)
 
 
 @dataclass
@@ -214,6 +216,8 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
                            c.sharedTangentialP3PriorSigma, c.sharedTangentialP4PriorSigma)):
         o.prior_sigma[i] = v
     o.low_order_distortion_scale = c.sharedLowOrderDistortionScale
+    o.use_transition_prior = 1 if c.useReferenceCalibrationTransitionPrior else 0
+    o.previous_intrinsic_mask = sum(1 << i for i, v in enumerate(c.referencePreviousIntrinsicParameterMask) if v)
     o.device = int(b.plaMatrixDevice)
     if precision is None:
         precision = Precision.Mixed if b.enablePlaMatrixMixedPrecision else Precision.Fp64

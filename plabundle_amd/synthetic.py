@@ -132,20 +132,26 @@ def native_three_cameras() -> Problem:
 # --- ring of cameras around a cube (configuration 1) ------------------------------------------
 def ring_problem(seed: int = 1, num_cameras: int = 50, num_points: int = 2000, max_views: int = 8,
                  noise_px: float = 0.5, rot_sigma: float = 0.01, center_sigma: float = 0.05,
-                 point_sigma: float = 0.1) -> Problem:
+                 point_sigma: float = 0.1, metashape: bool = False) -> Problem:
+    """Cameras on a 10 m circle looking inward at points in a 4 m cube, one shared Brown group. With
+    metashape=True the cameras use the Metashape convention with skew b2, k4, p3 and p4 as well (the full
+    frame calibration)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     W, H, f = 4000, 3000, 3000.0
     ang = 2 * np.pi * np.arange(num_cameras) / num_cameras
     cen = np.stack([10 * np.cos(ang), 10 * np.sin(ang), np.zeros(num_cameras)], axis=1)
     rot = np.stack([look_at_rotation(c, (0, 0, 0)) for c in cen])
     dist = rng.normal(0.0, 1e-2, size=4)  # one shared Brown group: k1, k2, p1, p2 ~ N(0, sigma=1e-2)
-    intr1 = intrinsic_block(f, f, W / 2, H / 2, k1=dist[0], k2=dist[1], p1=dist[2], p2=dist[3])
+    extra = rng.normal(0.0, [0.5, 1e-3, 1e-2, 1e-2]) if metashape else np.zeros(4)  # b2 (px), k4, p3, p4
+    intr1 = intrinsic_block(f, 1.002 * f if metashape else f, W / 2, H / 2, k1=dist[0], k2=dist[1], p1=dist[2],
+                            p2=dist[3], b2=extra[0], k4=extra[1], p3=extra[2], p4=extra[3])
     intr = np.repeat(intr1[None], num_cameras, axis=0)
+    conv = np.full(num_cameras, 1 if metashape else 0, dtype=np.int32)
     truth = rng.uniform(-2.0, 2.0, size=(num_points, 3))
     tracks, pts, truth_index = [], [], []
     for p in range(num_points):
         X = np.repeat(truth[p][None], num_cameras, axis=0)
-        uv, z = project_points(rot, cen, intr, X)
+        uv, z = project_points(rot, cen, intr, X, metashape=conv == 1)
         vis = np.nonzero((z > 0) & (uv[:, 0] >= 0) & (uv[:, 0] < W) & (uv[:, 1] >= 0) & (uv[:, 1] < H))[0]
         vis = rng.permutation(vis)[:max_views]
         if len(vis) < 2:
@@ -165,6 +171,7 @@ def ring_problem(seed: int = 1, num_cameras: int = 50, num_points: int = 2000, m
     return _finish(rot0, cen0, intr, np.array(pts), tracks,
                    cam_image_size=np.tile(np.array([W, H], dtype=np.int32), (num_cameras, 1)),
                    cam_group=np.zeros(num_cameras, dtype=np.int32),
+                   cam_convention=conv if metashape else None,
                    fixed_cameras=np.array([0, 1], dtype=np.int32),
                    gauge_policy=GaugePolicy.RequireExplicitGauge,
                    meta={"workload": "ring", "seed": seed, "truth_rotation": rot, "truth_center": cen,
