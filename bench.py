@@ -142,6 +142,87 @@ def batch_leg(num_windows: int, precisions, oracle_sample: int) -> dict:
     return out
 
 
+def survey_block_leg(strips: int, cameras_per_strip: int, num_points: int, steps: int):
+    """The large aerial survey block (20 strips x 500 cameras, full Metashape calibration over 8 groups, 2M points,
+    about 15M observations with 2 % outliers, 50 control points, 10k LiDAR planes; path A): 20 LM iterations mixed,
+    and FP64 with the same PCG stopping rule as the precision reference. No CPU oracle at this size; the control
+    points' residuals before and after are reported as a check of the datum."""
+    import numpy as np
+
+    from plabundle_amd import options as PO
+    from plabundle_amd import synthetic as S
+    from plabundle_amd.solver import Solver, SolverWorkspace
+
+    t = time.time()
+    problem = S.large_aerial_problem(strips=strips, cameras_per_strip=cameras_per_strip, num_points=num_points)
+    gen_s = time.time() - t
+    log(f"survey block: {problem.num_cameras} cameras, {problem.num_tracks} tracks, {problem.num_observations} "
+        f"observations, {problem.num_control_points} control points, {problem.num_laser_planes} laser planes "
+        f"generated in {gen_s:.1f}s")
+    ws = SolverWorkspace(0)
+    solver = Solver()
+
+    def run(precision, inexact=False, profile=False):
+        o = S.large_aerial_options(PO.SolveOptions())
+        o.backend.profileKernels = profile
+        if inexact:
+            o.backend.pcgRelativeTolerance = 1e-3
+            o.backend.pcgModelEta = 0.1
+            o.backend.pcgMaxIterations = max(200, 3 * (problem.num_cameras + 24) // 5)
+        r = solver.solve(problem, o, ws, precision=precision)
+        if not r.usable():
+            raise RuntimeError(f"survey block solve failed: {r.status.name} {r.backendMessage}")
+        return r
+
+    def cp_rms(X):
+        d = X[problem.cp_track] - problem.cp_point
+        return float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+
+    first = run(PO.Precision.Mixed)
+    log(f"survey block warmup: {first.report['iterations']} it, setup {first.report['setup_seconds']:.2f}s, "
+        f"solve {first.report['solve_seconds']:.3f}s, cost {first.finalCost:.10g}")
+    mixed = []
+    for i in range(steps):
+        r = run(PO.Precision.Mixed, profile=(i == 0))
+        mixed.append(r)
+        log(f"survey block mixed {i}: {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
+            f"solve {r.report['solve_seconds']:.3f}s, cost {r.finalCost:.10g}")
+    fams = {}
+    for fam in ("linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost", "back_substitution"):
+        st = ws.kernel_stats(fam)
+        if st["launches"]:
+            avg = st["total_ms"] / st["launches"]
+            fams[fam] = {"timed_launches": st["launches"], "avg_ms": round(avg, 5),
+                         "GBps": round(st["bytes_per_launch"] / (avg * 1e-3) / 1e9, 1) if avg > 0 else 0.0}
+    ref = run(PO.Precision.Fp64, inexact=True)
+    log(f"survey block fp64 (same PCG stop): {ref.report['iterations']} it, "
+        f"{ref.report['linear_iterations']} PCG it, solve {ref.report['solve_seconds']:.3f}s, cost {ref.finalCost:.10g}")
+    mt = sum(r.report["solve_seconds"] for r in mixed) / len(mixed)
+    out = {
+        "workload": f"survey block: {strips} strips x {cameras_per_strip} cameras at 300 m, full Metashape calibration "
+                    f"over 8 groups, {num_points} points, 2% outliers, 50 control points, 10k LiDAR planes, seed 3",
+        "observations": problem.num_observations,
+        "generation_s": round(gen_s, 1),
+        "setup_s": round(first.report["setup_seconds"], 3),
+        "path": "assembled damped Schur (laser planes)" if not mixed[0].report["online_schur_used"] else "online Schur",
+        "mixed_time_s": round(mt, 4),
+        "mixed_lm_iter_per_s": round(mixed[0].report["iterations"] / mt, 3),
+        "mixed_pcg_iterations": mixed[0].report["linear_iterations"],
+        "mixed_message": mixed[0].report["message"],
+        "fp64_same_stop_time_s": round(ref.report["solve_seconds"], 4),
+        "fp64_same_stop_pcg_iterations": ref.report["linear_iterations"],
+        "initial_cost": mixed[0].initialCost,
+        "final_cost_mixed": mixed[0].finalCost,
+        "final_cost_fp64_same_stop": ref.finalCost,
+        "rel_cost_diff_mixed_vs_fp64": abs(mixed[0].finalCost - ref.finalCost) / ref.finalCost,
+        "control_rms_m": {"initial": round(cp_rms(problem.track_point), 5), "mixed": round(cp_rms(mixed[0].track_point), 5),
+                          "fp64": round(cp_rms(ref.track_point), 5)},
+        "families": fams,
+    }
+    ws.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +247,11 @@ def main():
                     help="windows of the batch checked against the FP64 oracle")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched-window leg and print its JSON (profiling the batch)")
+    ap.add_argument("--survey-block", action="store_true",
+                    help="run only the large survey-block leg (15M observations, path A) and print its JSON")
+    ap.add_argument("--survey-strips", type=int, default=20)
+    ap.add_argument("--survey-cameras", type=int, default=500, help="cameras per strip of the survey block")
+    ap.add_argument("--survey-points", type=int, default=2_000_000)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only copy this many MiB device-to-device three times (the counter calibration run)")
     args = ap.parse_args()
@@ -180,6 +266,11 @@ def main():
     from plabundle_amd import options as PO
     from plabundle_amd import synthetic as S
     from plabundle_amd.solver import Solver, SolverWorkspace
+
+    if args.survey_block:
+        print(json.dumps({"survey_block": survey_block_leg(args.survey_strips, args.survey_cameras,
+                                                            args.survey_points, args.steps)}), flush=True)
+        return
 
     if args.batch_only:
         precs = [PO.Precision.Mixed] + ([PO.Precision.Fp64] if args.batch_fp64 else [])

@@ -1658,8 +1658,10 @@ namespace
                 for (int j = 0; j <= kP - 1; ++j)
                 {
                     const std::size_t ci = colOf[b * kP + i], cj = colOf[b * kP + j];
-                    if (ci == SIZE_MAX || cj == SIZE_MAX || ci < cj) continue;
-                    at(ci, cj) = R.diag[static_cast<std::size_t>(b) * kP * kP + i * kP + j];
+                    const double v = R.diag[static_cast<std::size_t>(b) * kP * kP + i * kP + j];
+                    // zeros are not in the envelope (touch() registered nonzeros only) and need no store
+                    if (ci == SIZE_MAX || cj == SIZE_MAX || ci < cj || v == 0.0) continue;
+                    at(ci, cj) = v;
                 }
         for (std::size_t s = 0; s < R.offBlocks.size(); ++s)
         {
@@ -1668,8 +1670,8 @@ namespace
                 for (int j = 0; j < kP; ++j)
                 {
                     std::size_t ci = colOf[rb * kP + i], cj = colOf[cb * kP + j];
-                    if (ci == SIZE_MAX || cj == SIZE_MAX) continue;
                     const double v = R.off[s * kP * kP + i * kP + j];
+                    if (ci == SIZE_MAX || cj == SIZE_MAX || v == 0.0) continue;
                     if (ci < cj) std::swap(ci, cj);
                     at(ci, cj) = v;
                 }
@@ -1966,8 +1968,9 @@ namespace
     // with more damping; the directional derivative comes from an objective probe at alpha = 1e-4 (:473-486);
     // any decrease is accepted (:517-519); a small step ends the solve without being applied (:451-465).
     // Unpinned choices (PlaMatrix's solveDampedSchurComplement is external): the damped system is the online
-    // path's (Hpp and reduced diagonals times 1 + lambda), and a singular damped point block or a reduced
-    // system that is not positive definite is a linear-solver exception, which this path rethrows (:357-364).
+    // path's (Hpp and reduced diagonals times 1 + lambda); a singular damped point block is a linear-solver
+    // exception, which this path rethrows (:357-364); a reduced system that is not positive definite counts as a
+    // solve that did not converge (more damping, the same equations).
     void solveAssembled(Oracle& o, const volatile int32_t* cancel)
     {
         pb_solve_report& rep = o.report;
@@ -2023,12 +2026,12 @@ namespace
                     std::vector<double> primary;
                     double rel = 0.0;
                     const auto tl = std::chrono::steady_clock::now();
-                    if (!solveReduced(R, a.cameraBlockCount, &primary, &rel))
-                        throw std::runtime_error("reduced system not positive definite");
+                    const bool spd = solveReduced(R, a.cameraBlockCount, &primary, &rel);
                     rep.linear_solve_seconds +=
                         std::chrono::duration<double>(std::chrono::steady_clock::now() - tl).count();
                     ++iterations;
-                    if (!(rel <= 1e-12))
+                    // a reduced system that is not positive definite counts as a solve that did not converge
+                    if (!spd || !(rel <= 1e-12))
                     {
                         ++rejected;
                         raise();

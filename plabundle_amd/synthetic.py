@@ -236,7 +236,15 @@ def aerial_problem(seed: int = 2, cameras_per_strip=(334, 333, 333), num_points:
                    mean_track_length: float = 7.5, outlier_fraction: float = 0.01, noise_px: float = 0.5,
                    num_groups: int = 4, rot_sigma: float = 0.002, center_sigma: float = 0.2,
                    point_sigma: float = 0.2, focal_sigma: float = 0.002, distortion_sigma: float = 0.01,
-                   altitude: float = 300.0) -> Problem:
+                   altitude: float = 300.0, metashape: bool = False, strip_window: int | None = None,
+                   chunk: int | None = None) -> Problem:
+    """Aerial strip block: strips flown along +x at `altitude` with 80 % forward and 60 % side overlap, points
+    on an fBm terrain, tracks of geometric length, outliers uniform over the image.
+
+    metashape=True gives every group a full Metashape frame calibration (aspect, principal point, skew b2, k1..k4,
+    p1..p4 besides the focal length). strip_window limits the candidate cameras of a point to the strips within
+    that many strips of it, and chunk bounds the points drawn per batch (both needed for blocks of many strips;
+    the defaults keep the draws of the three-strip block)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     W, H, f = 4000, 3000, 3000.0
     along = altitude * H / f          # 300 m footprint along track (image v axis)
@@ -261,8 +269,17 @@ def aerial_problem(seed: int = 2, cameras_per_strip=(334, 333, 333), num_points:
     g_f = f * (1.0 + rng.normal(0.0, focal_sigma, size=num_groups))
     g_d = rng.normal(0.0, distortion_sigma, size=(num_groups, 2))           # k1, k2
     g_t = rng.normal(0.0, 1e-4, size=(num_groups, 2))                       # p1, p2
-    intr_true = np.stack([intrinsic_block(g_f[g], g_f[g], W / 2, H / 2, k1=g_d[g, 0], k2=g_d[g, 1],
-                                          p1=g_t[g, 0], p2=g_t[g, 1]) for g in group])
+    if metashape:
+        # aspect (1e-4), principal offsets (2 px), skew b2 (0.5 px), k3, k4 (1e-3), p3, p4 (1e-2)
+        g_m = rng.normal(0.0, [1e-4, 2.0, 2.0, 0.5, 1e-3, 1e-3, 1e-2, 1e-2], size=(num_groups, 8))
+        intr_true = np.stack([intrinsic_block(g_f[g], g_f[g] * np.exp(g_m[g, 0]), W / 2 + g_m[g, 1],
+                                              H / 2 + g_m[g, 2], k1=g_d[g, 0], k2=g_d[g, 1], k3=g_m[g, 4],
+                                              p1=g_t[g, 0], p2=g_t[g, 1], b2=g_m[g, 3], k4=g_m[g, 5],
+                                              p3=g_m[g, 6], p4=g_m[g, 7]) for g in group])
+    else:
+        intr_true = np.stack([intrinsic_block(g_f[g], g_f[g], W / 2, H / 2, k1=g_d[g, 0], k2=g_d[g, 1],
+                                              p1=g_t[g, 0], p2=g_t[g, 1]) for g in group])
+    conv = np.full(n_cam, 1 if metashape else 0, dtype=np.int32)
     intr_init = np.repeat(intrinsic_block(f, f, W / 2, H / 2)[None], n_cam, axis=0)
     # terrain
     octaves = []
@@ -281,26 +298,38 @@ def aerial_problem(seed: int = 2, cameras_per_strip=(334, 333, 333), num_points:
     half = int(np.ceil(along / base)) + 2
     win = 2 * half + 1
     strip_start = np.concatenate([[0], np.cumsum(cameras_per_strip)])
-    pts_true, cams_l, uv_l = [], [], []
+    pts_true, cams_l, uv_l, len_l = [], [], [], []
     have = 0
+    ns = strips if strip_window is None else min(strips, 2 * strip_window + 1)
     while have < num_points:
         m = int((num_points - have) * 1.2) + 64
+        if chunk is not None:
+            m = min(m, chunk)
         px = rng.uniform(x_lo, x_hi, size=m)
         py = rng.uniform(y_lo, y_hi, size=m)
         pz = _fbm_height(px, py, octaves)
         X = np.stack([px, py, pz], axis=1)
-        # candidate cameras: a window of along-track indices in every strip
+        # candidate cameras: a window of along-track indices in every strip (or in the strips near the point)
         kc = np.floor(px / base).astype(np.int64)
-        cand = np.full((m, strips * win), -1, dtype=np.int64)
-        for s in range(strips):
+        cand = np.full((m, ns * win), -1, dtype=np.int64)
+        s0 = None if strip_window is None else np.rint(py / spacing).astype(np.int64) - strip_window
+        for j in range(ns):
             k = kc[:, None] + np.arange(-half, half + 1)[None, :]
-            valid = (k >= 0) & (k < cameras_per_strip[s])
-            cand[:, s * win:(s + 1) * win] = np.where(valid, strip_start[s] + k, -1)
+            if s0 is None:
+                valid = (k >= 0) & (k < cameras_per_strip[j])
+                cand[:, j * win:(j + 1) * win] = np.where(valid, strip_start[j] + k, -1)
+            else:
+                sj = s0 + j
+                ok_s = (sj >= 0) & (sj < strips)
+                sjc = np.clip(sj, 0, strips - 1)
+                per = np.asarray(cameras_per_strip)[sjc]
+                valid = ok_s[:, None] & (k >= 0) & (k < per[:, None])
+                cand[:, j * win:(j + 1) * win] = np.where(valid, strip_start[sjc][:, None] + k, -1)
         flat = cand.reshape(-1)
         ok = flat >= 0
         cidx = np.where(ok, flat, 0)
-        Xr = np.repeat(X, strips * win, axis=0)
-        uv, z = project_points(rot[cidx], cen[cidx], intr_true[cidx], Xr)
+        Xr = np.repeat(X, ns * win, axis=0)
+        uv, z = project_points(rot[cidx], cen[cidx], intr_true[cidx], Xr, metashape=conv[cidx] == 1)
         vis = ok & (z > 0) & (uv[:, 0] >= 0) & (uv[:, 0] < W) & (uv[:, 1] >= 0) & (uv[:, 1] < H)
         vis = vis.reshape(m, -1)
         uv = uv.reshape(m, -1, 2)
@@ -323,20 +352,27 @@ def aerial_problem(seed: int = 2, cameras_per_strip=(334, 333, 333), num_points:
         noise = rng.normal(0.0, noise_px, size=uv.shape)
         outl = rng.random(size=vis.shape) < outlier_fraction
         outl_uv = rng.uniform(0, 1, size=uv.shape) * np.array([W, H])
-        for i in np.nonzero(nvis >= 2)[0]:
-            if have >= num_points:
-                break
-            pick = order[i, : L[i]]
-            pick = pick[np.argsort(cand[i, pick])]
-            obs = uv[i, pick] + noise[i, pick]
-            o_mask = outl[i, pick]
-            obs[o_mask] = outl_uv[i, pick][o_mask]
-            pts_true.append(X[i])
-            cams_l.append(cand[i, pick])
-            uv_l.append(obs)
-            have += 1
-    tracks = list(zip(cams_l, uv_l))
-    pts_true = np.array(pts_true)
+        # every visible point up to the target count: the first L views of its random order, by camera index
+        rows = np.nonzero(nvis >= 2)[0][: num_points - have]
+        Lr = L[rows]
+        o_r = order[rows]
+        sel = np.arange(o_r.shape[1])[None, :] < Lr[:, None]
+        cam_r = np.take_along_axis(cand[rows], o_r, axis=1)
+        srt = np.argsort(np.where(sel, cam_r, np.iinfo(np.int64).max), axis=1)
+        col = np.take_along_axis(o_r, srt, axis=1)[np.take_along_axis(sel, srt, axis=1)]
+        rr = np.repeat(rows, Lr)
+        obs = uv[rr, col] + noise[rr, col]
+        o_mask = outl[rr, col]
+        obs[o_mask] = outl_uv[rr, col][o_mask]
+        pts_true.append(X[rows])
+        cams_l.append(cand[rr, col])
+        uv_l.append(obs)
+        len_l.append(Lr)
+        have += len(rows)
+    pts_true = np.concatenate(pts_true)
+    lens = np.concatenate(len_l)
+    split = np.cumsum(lens)[:-1]
+    tracks = list(zip(np.split(np.concatenate(cams_l), split), np.split(np.concatenate(uv_l), split)))
     dR = exp_so3(rng.normal(0.0, rot_sigma, size=(n_cam, 3)))
     dC = rng.normal(0.0, center_sigma, size=(n_cam, 3))
     dR[:2] = np.eye(3)
@@ -347,10 +383,42 @@ def aerial_problem(seed: int = 2, cameras_per_strip=(334, 333, 333), num_points:
     return _finish(rot0, cen0, intr_init, pts0, tracks,
                    cam_image_size=np.tile(np.array([W, H], dtype=np.int32), (n_cam, 1)),
                    cam_group=group.astype(np.int32),
+                   cam_convention=conv if metashape else None,
                    fixed_cameras=np.array([0, 1], dtype=np.int32),
                    gauge_policy=GaugePolicy.RequireExplicitGauge,
                    meta={"workload": "aerial", "seed": seed, "geometric_q": q, "truth_rotation": rot, "truth_center": cen,
                          "truth_points": pts_true, "truth_intrinsics": intr_true})
+
+
+def large_aerial_problem(seed: int = 3, strips: int = 20, cameras_per_strip: int = 500, num_points: int = 2_000_000,
+                         num_control: int = 50, num_planes: int = 10_000, num_groups: int = 8,
+                         outlier_fraction: float = 0.02, mean_track_length: float = 7.5) -> Problem:
+    """Large aerial block with the full survey setup: a strip block (80 % / 60 % overlap) with a full Metashape
+    calibration per group, 0.5 px noise plus outliers, surveyed control points (sigma 0.02 m, three uncertainty
+    forms) and LiDAR point-to-plane constraints. The datum comes from the control points (no fixed camera).
+    Defaults: 20 strips x 500 cameras, 8 groups, 2M points, about 15M observations, 50 control points, 10k
+    planes."""
+    p = aerial_problem(seed=seed, cameras_per_strip=(cameras_per_strip,) * strips, num_points=num_points,
+                       mean_track_length=mean_track_length, outlier_fraction=outlier_fraction,
+                       num_groups=num_groups, metashape=True, strip_window=2, chunk=200_000)
+    p.fixed_cameras = None
+    p.gauge_policy = GaugePolicy.AutoAnchor
+    return add_survey_constraints(p, seed=seed + 100, num_control=num_control, num_planes=num_planes,
+                                  kinds=(0, 1, 2), control_sigma=0.02)
+
+
+def large_aerial_options(options=None):
+    """Solve options of the large aerial block: Cauchy 1 px, every calibration slot refined per group."""
+    from .options import ImageRobustLoss, SolveOptions
+
+    o = options or SolveOptions()
+    o.solver.imageRobustLoss = ImageRobustLoss.Cauchy
+    o.solver.imageRobustLossScalePixels = 1.0
+    o.solver.useReferenceOnlineSchur = True
+    c = o.calibration
+    c.refineSharedFocalLength = c.refineSharedFocalAspectRatio = c.refineSharedPrincipalPoint = True
+    c.refineSharedRadialDistortion = c.refineSharedHighOrderDistortion = c.refineSharedMetashapeParameters = True
+    return o
 
 
 def aerial_options(options=None):

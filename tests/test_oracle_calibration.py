@@ -182,3 +182,20 @@ def test_transition_prior_objective(oracle_mod):
     assert abs(res[0]) > 0 and abs(res[4]) > 0
     np.testing.assert_allclose(orc.initial_cost() - orc0.initial_cost(), 0.5 * (res @ res - res0 @ res0),
                                rtol=1e-9)
+
+
+def test_reduced_solve_with_zero_entries_in_intrinsic_blocks(oracle_mod):
+    # eight 13-slot groups on a survey block: some entries of the intrinsic blocks are exactly zero; the envelope
+    # Cholesky of the reduced solve must leave them out of its profile (a store of a zero outside the profile
+    # once overwrote a neighbouring row and made the factorisation fail)
+    p = S.large_aerial_problem(strips=3, cameras_per_strip=24, num_points=6000, num_control=20, num_planes=200)
+    copts = PO.to_c(S.large_aerial_options(PO.SolveOptions()))
+    orc = oracle_mod.Oracle(p, threads=4)
+    prep = orc.prepare(copts)
+    assert prep["prepared"] and prep["intrinsic_block_count"] == 8
+    P, Ne = prep["primary_block_count"], prep["track_block_count"]
+    for damping in (0.0, 1e-3):
+        A, b, _, _ = orc.reduced_system(P, damping)
+        assert (A == 0.0).sum() > 0
+        primary, _, _, _, _ = orc.compute_step(P, Ne, damping)
+        assert np.linalg.norm(A @ primary - b) <= 1e-10 * np.linalg.norm(b)
