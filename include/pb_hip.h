@@ -213,6 +213,11 @@ typedef struct pb_solve_options {
      * prior sigma by a transition weight, non-zero only when the refined set differs from the previous mask. */
     int32_t use_transition_prior;        /* false */
     uint32_t previous_intrinsic_mask;    /* bit i: slot i was refined in the previous stage */
+
+    /* Backend control of pb_hip_solve_batch: 1 (default) replays each round's fixed launch sequences (build,
+     * factorisation, back-substitution, trial passes, controller) as captured HIP graphs, 0 launches them directly.
+     * Both give bitwise the same results. */
+    int32_t batch_graphs;
 } pb_solve_options;
 
 /* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */

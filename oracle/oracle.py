@@ -118,6 +118,7 @@ def default_options() -> _abi.SolveOptionsC:
     o.control_point_huber_delta = 0.2
     o.use_transition_prior = 0
     o.previous_intrinsic_mask = 0
+    o.batch_graphs = 1
     return o
 
 

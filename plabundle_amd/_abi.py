@@ -115,6 +115,7 @@ class SolveOptionsC(C.Structure):
         ("control_point_huber_delta", C.c_double),
         ("use_transition_prior", C.c_int32),
         ("previous_intrinsic_mask", C.c_uint32),
+        ("batch_graphs", C.c_int32),
     ]
 
 

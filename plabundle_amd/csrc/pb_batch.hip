@@ -728,11 +728,19 @@ namespace pbk
         k_batch_after_build<T><<<gridFor(B.nWin, 4), 256, 0, s>>>(P, W, B);
     }
 
-    template <typename T>
-    void BatchLauncher<T>::factor(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s)
+    namespace
     {
-        const size_t lds = sizeof(double) * (size_t(B.maxDim) * kDensePanel + kDensePanel * kDensePanel + B.maxDim +
-                                             (kDenseThreads / kWave) * kDensePanel);
+        size_t factorLds(const Batch& B)
+        {
+            return sizeof(double) * (size_t(B.maxDim) * kDensePanel + kDensePanel * kDensePanel + B.maxDim +
+                                     (kDenseThreads / kWave) * kDensePanel);
+        }
+    }
+
+    template <typename T>
+    void BatchLauncher<T>::configure(const Batch& B)
+    {
+        const size_t lds = factorLds(B);
         if (lds > 64 * 1024)
         {
             // above the default dynamic LDS limit (kMaxDense = 512 needs 72 KiB of the CU's 160 KiB)
@@ -740,7 +748,12 @@ namespace pbk
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
             if (e != hipSuccess) throw std::runtime_error("hipFuncSetAttribute (dense window solve LDS)");
         }
-        k_batch_factor<T><<<B.nWin, kDenseThreads, lds, s>>>(P, W, B);
+    }
+
+    template <typename T>
+    void BatchLauncher<T>::factor(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s)
+    {
+        k_batch_factor<T><<<B.nWin, kDenseThreads, factorLds(B), s>>>(P, W, B);
     }
 
     template <typename T>

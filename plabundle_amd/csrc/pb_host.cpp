@@ -2524,33 +2524,95 @@ namespace
     }
 
     // One LM iteration of every running window of one precision phase (the kernels of that precision).
+    // The three fixed launch sequences of a batch round for one assembly precision: the head (build, window
+    // factorisations, back-substitution, the first trial pass of one step length), a further trial pass, and the
+    // tail (controllers, state update, running-window counts). Each ends with the readback of the counters the host
+    // polls. With graphs they are captured once per batch solve and replayed every round.
+    struct BatchRoundGraphs
+    {
+        hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+        ~BatchRoundGraphs()
+        {
+            for (hipGraphExec_t& e : exec)
+                if (e) (void)hipGraphExecDestroy(e);
+        }
+    };
+
     template <typename T>
-    int batchIteration(pb_ctx* ctx, pbk::Work<T>& W, int phase, const pb_solve_options& o, int passAlpha,
-                       int32_t* hc)
+    void batchSequence(pb_ctx* ctx, pbk::Work<T>& W, int phase, int passAlpha, int32_t* hc, int which)
     {
         using BL = pbk::BatchLauncher<T>;
         hipStream_t s = ctx->stream;
         const pbk::Problem& P = ctx->P;
         const pbk::Batch& B = ctx->bh.B;
-        BL::roundBegin(P, W, B, phase, s);
-        BL::build(P, W, s);
-        BL::afterBuild(P, W, B, s);
-        BL::factor(P, W, B, s);
-        BL::backSubstitute(P, W, B, s);
-        // trial passes until no window is still searching (each pass spends at least one of a window's
-        // line-search steps, so at most line_search_steps passes)
-        int passes = 0;
-        for (int pass = 0; pass <= o.line_search_steps; ++pass)
+        if (which == 0)
         {
-            BL::trialPass(P, W, B, pass == 0 ? 1 : passAlpha, s);
-            ++passes;
+            BL::roundBegin(P, W, B, phase, s);
+            BL::build(P, W, s);
+            BL::afterBuild(P, W, B, s);
+            BL::factor(P, W, B, s);
+            BL::backSubstitute(P, W, B, s);
+        }
+        if (which <= 1)
+        {
+            BL::trialPass(P, W, B, which == 0 ? 1 : passAlpha, s);
             check(hipMemcpyAsync(hc + 2, B.counts + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s), "readback");
-            check(hipStreamSynchronize(s), "trial pass");
-            if (hc[2] == 0) break;
+            return;
         }
         BL::finishRound(P, W, B, phase, s);
         BL::count(B, s);
         check(hipMemcpyAsync(hc, B.counts, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s), "readback");
+    }
+
+    template <typename T>
+    void batchRun(pb_ctx* ctx, pbk::Work<T>& W, int phase, int passAlpha, int32_t* hc, int which,
+                  BatchRoundGraphs* graphs)
+    {
+        hipStream_t s = ctx->stream;
+        if (!graphs)
+        {
+            batchSequence<T>(ctx, W, phase, passAlpha, hc, which);
+            return;
+        }
+        if (!graphs->exec[which])
+        {
+            hipGraph_t g = nullptr;
+            check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try
+            {
+                batchSequence<T>(ctx, W, phase, passAlpha, hc, which);
+            }
+            catch (...)
+            {
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+            const hipError_t e = hipGraphInstantiate(&graphs->exec[which], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            check(e, "hipGraphInstantiate");
+        }
+        check(hipGraphLaunch(graphs->exec[which], s), "hipGraphLaunch");
+    }
+
+    template <typename T>
+    int batchIteration(pb_ctx* ctx, pbk::Work<T>& W, int phase, const pb_solve_options& o, int passAlpha,
+                       int32_t* hc, BatchRoundGraphs* graphs)
+    {
+        hipStream_t s = ctx->stream;
+        // trial passes until no window is still searching (each pass spends at least one of a window's
+        // line-search steps, so at most line_search_steps passes)
+        batchRun<T>(ctx, W, phase, passAlpha, hc, 0, graphs);
+        check(hipStreamSynchronize(s), "trial pass");
+        int passes = 1;
+        for (int pass = 1; pass <= o.line_search_steps && hc[2] != 0; ++pass)
+        {
+            batchRun<T>(ctx, W, phase, passAlpha, hc, 1, graphs);
+            check(hipStreamSynchronize(s), "trial pass");
+            ++passes;
+        }
+        batchRun<T>(ctx, W, phase, passAlpha, hc, 2, graphs);
         check(hipStreamSynchronize(s), "round");
         return passes;
     }
@@ -2595,7 +2657,13 @@ namespace
         check(hipMemsetAsync(B.doneClock, 0, sizeof(unsigned long long) * (nWin + 1), s), "memset");
         zeroDirect(ctx, W64);
         if (W32) zeroDirect(ctx, *W32);
+        pbk::BatchLauncher<double>::configure(B);
+        if (W32) pbk::BatchLauncher<float>::configure(B);
         check(hipStreamSynchronize(s), "reset");
+        // captured once per batch solve (the launch arguments are this batch's), replayed every round
+        BatchRoundGraphs graphs64, graphs32;
+        BatchRoundGraphs* g64 = o.batch_graphs ? &graphs64 : nullptr;
+        BatchRoundGraphs* g32 = o.batch_graphs ? &graphs32 : nullptr;
         const auto t0 = Clock::now();
         pbk::BatchLauncher<double>::start(B, s);
         int32_t* hc = reinterpret_cast<int32_t*>(ctx->pinned + kPinFlags);
@@ -2611,8 +2679,8 @@ namespace
                 cancelled = true;
                 break;
             }
-            if (W32 && hc[0] > 0) passes += batchIteration<float>(ctx, *W32, 0, o, passAlpha, hc);
-            if (hc[1] > 0) passes += batchIteration<double>(ctx, W64, 1, o, passAlpha, hc);
+            if (W32 && hc[0] > 0) passes += batchIteration<float>(ctx, *W32, 0, o, passAlpha, hc, g32);
+            if (hc[1] > 0) passes += batchIteration<double>(ctx, W64, 1, o, passAlpha, hc, g64);
             ++rounds;
         }
         check(hipStreamSynchronize(s), "batch");
@@ -2798,6 +2866,7 @@ void pb_default_options(pb_solve_options* o)
     o->control_point_huber_delta = 0.2;
     o->use_transition_prior = 0;
     o->previous_intrinsic_mask = 0;
+    o->batch_graphs = 1;
     o->refine_camera_pose = 1;
     o->shared_intrinsic_mask = 0;
     o->min_focal_scale = 0.5;

@@ -364,6 +364,7 @@ namespace pbk
     template <typename T>
     struct BatchLauncher
     {
+        static void configure(const Batch& B);  // kernel attributes (outside any stream capture)
         static void start(const Batch& B, hipStream_t s);
         static void count(const Batch& B, hipStream_t s);  // running windows per phase -> counts[0], counts[1]
         static void roundBegin(const Problem& P, const Work<T>& W, const Batch& B, int phase, hipStream_t s);

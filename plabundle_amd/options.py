@@ -136,6 +136,7 @@ class BackendOptions:
     profileKernels: bool = False
     pcgModelEta: float = -1.0  # < 0: path default (off for FP64, 0.1 for mixed)
     stepRmsTolerance: float = 0.0  # 0: the reference's 1e-6
+    batchGraphs: bool = True  # batched windows: replay each round's launches as captured HIP graphs
 
 
 @dataclass
@@ -230,6 +231,7 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.profile_kernels = 1 if b.profileKernels else 0
     o.pcg_model_eta = float(b.pcgModelEta)
     o.step_rms_tolerance = float(b.stepRmsTolerance)
+    o.batch_graphs = 1 if b.batchGraphs else 0
     k = options.constraints
     o.laser_plane_weight = float(k.laserPlaneWeight)
     o.laser_huber_delta = float(k.laserHuberDeltaMeters)

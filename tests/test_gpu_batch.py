@@ -156,3 +156,25 @@ def test_batch_cancel_publishes_nothing(ws):
     res, summary = Solver().solve_batch(wins, o, ws, precision=PO.Precision.Fp64, cancel=True)
     assert summary["solve_status"] == int(SolveStatus.Cancelled)
     assert all(not r.usable() for r in res)
+
+
+@pytest.mark.parametrize("precision", [PO.Precision.Fp64, PO.Precision.Mixed])
+def test_batch_graph_replay_matches_direct_launches(ws, precision):
+    # every round replays the captured head / trial-pass / tail graphs, with the host's polls and syncs between
+    # replays; the outcome of every window must be bitwise that of direct launches, and the same on a second
+    # batch solve (new captures)
+    _say(f"case batch graphs vs direct launches precision={precision.name}")
+    wins = _windows(n=8, seed=13)
+    o = S.window_options(PO.SolveOptions(), max_iterations=12)
+    o.backend.batchGraphs = False
+    direct, sd = Solver().solve_batch(wins, o, ws, precision=precision)
+    o.backend.batchGraphs = True
+    for rep in range(2):
+        graph, sg = Solver().solve_batch(wins, o, ws, precision=precision)
+        assert sd["abi_status"] == sg["abi_status"] == _abi.PB_OK
+        assert sd["iterations"] == sg["iterations"]
+        for a, b in zip(direct, graph):
+            assert a.report["iterations"] == b.report["iterations"]
+            assert a.finalCost == b.finalCost
+            np.testing.assert_array_equal(a.cam_center, b.cam_center)
+            np.testing.assert_array_equal(a.track_point, b.track_point)
