@@ -218,6 +218,13 @@ typedef struct pb_solve_options {
      * factorisation, back-substitution, trial passes, controller) as captured HIP graphs, 0 launches them directly.
      * Both give bitwise the same results. */
     int32_t batch_graphs;
+    /* Backend control of pb_hip_solve_batch: launch shape of the dense window factorisation (threads of the
+     * window's workgroup and column-panel width, pb_hip_batch_factor_shape). -1 (default): every shape that fits
+     * the LDS is timed on the first round of a batch and the fastest is kept for the rest of the batch and for
+     * later batches of the same class on the context (window count rounded up to a power of two, largest window
+     * system rounded up to a multiple of 64 unknowns; pb_hip_batch_tuning reports the times);
+     * 0 .. PB_BATCH_FACTOR_SHAPES-1: that shape. Every shape gives bitwise the same results. */
+    int32_t batch_factor_shape;
 } pb_solve_options;
 
 /* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */
@@ -362,6 +369,17 @@ typedef struct pb_window_report {
  * window, pb_hip_download the concatenated state; pb_hip_solve and pb_hip_backward need a new pb_hip_upload. */
 pb_status pb_hip_solve_batch(pb_ctx* ctx, const pb_batch_view* batch, const pb_solve_options* options,
                              pb_window_report* reports, pb_solve_report* summary, const volatile int32_t* cancel);
+
+/* Launch shapes of the dense window factorisation of pb_hip_solve_batch (no reference counterpart). */
+#define PB_BATCH_FACTOR_SHAPES 6
+
+/* Threads of the window workgroup and column-panel width of factorisation shape `shape`. */
+pb_status pb_hip_batch_factor_shape(int32_t shape, int32_t* threads, int32_t* panel);
+
+/* Factorisation-shape timing of the batches solved on the context, for FP32 (fp64 = 0) or FP64 (fp64 = 1)
+ * assembly: *shape receives the timed choice (-1 if no batch was timed), ms [PB_BATCH_FACTOR_SHAPES] (may be
+ * NULL) the measured milliseconds of each shape's launch over the timed round (-1 for a shape not timed). */
+pb_status pb_hip_batch_tuning(pb_ctx* ctx, int32_t fp64, int32_t* shape, double* ms);
 
 /* Refined state of one window of the last batch, in the window's own numbering (as pb_hip_download). */
 pb_status pb_hip_batch_download(pb_ctx* ctx, int32_t window, double* cam_rotation, double* cam_center,

@@ -119,6 +119,7 @@ def default_options() -> _abi.SolveOptionsC:
     o.use_transition_prior = 0
     o.previous_intrinsic_mask = 0
     o.batch_graphs = 1
+    o.batch_factor_shape = -1
     return o
 
 

@@ -116,6 +116,7 @@ class SolveOptionsC(C.Structure):
         ("use_transition_prior", C.c_int32),
         ("previous_intrinsic_mask", C.c_uint32),
         ("batch_graphs", C.c_int32),
+        ("batch_factor_shape", C.c_int32),
     ]
 
 
@@ -294,6 +295,8 @@ class ComparisonC(C.Structure):
         return out
 
 
+PB_BATCH_FACTOR_SHAPES = 6
+
 # Exported symbols of the shared library, in header order.
 EXPORTED_SYMBOLS = (
     "pb_default_options",
@@ -309,6 +312,8 @@ EXPORTED_SYMBOLS = (
     "pb_hip_evaluate_cost",
     "pb_hip_backward",
     "pb_hip_solve_batch",
+    "pb_hip_batch_factor_shape",
+    "pb_hip_batch_tuning",
     "pb_hip_batch_download",
     "pb_default_quality_options",
     "pb_hip_quality",
@@ -364,6 +369,10 @@ def load_library(path: str | None = None):
     lib.pb_hip_solve_batch.argtypes = [C.c_void_p, C.POINTER(BatchView), C.POINTER(SolveOptionsC),
                                        C.POINTER(WindowReportC), C.POINTER(SolveReportC), _ip]
     lib.pb_hip_solve_batch.restype = C.c_int
+    lib.pb_hip_batch_factor_shape.argtypes = [C.c_int32, _ip, _ip]
+    lib.pb_hip_batch_factor_shape.restype = C.c_int
+    lib.pb_hip_batch_tuning.argtypes = [C.c_void_p, C.c_int32, _ip, _dp]
+    lib.pb_hip_batch_tuning.restype = C.c_int
     lib.pb_hip_batch_download.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _ip]
     lib.pb_hip_batch_download.restype = C.c_int
     lib.pb_default_quality_options.argtypes = [C.POINTER(QualityOptionsC)]

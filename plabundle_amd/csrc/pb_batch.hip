@@ -97,47 +97,72 @@ namespace pbk
     // ---------------------------------------------------------------------------------------------
     // Dense window solve. The window's reduced system (6 * units rows, both triangles of the block-sparse S)
     // is gathered into a column-major FP64 matrix A, factored in place by a left-looking blocked Cholesky with
-    // column panels of kDensePanel, and solved by blocked forward and backward substitution. Thread t owns
-    // the panel rows j0 + t + r * kDenseThreads (r < kDenseRows). A failed pivot (not positive, or not
-    // finite) rejects the step, as a breakdown of the iterative solve does on the single-problem path.
-    template <int R>
+    // column panels of NB, and solved by a blocked forward substitution and a right-looking blocked backward
+    // substitution. Thread t of the NT-thread workgroup owns the panel rows j0 + t + r * NT (r < R, R * NT =
+    // kMaxDense). Every value is computed by one thread in an order that NT and NB do not change:
+    //   L(i,j) = A(i,j) - L(i,0) L(j,0) - L(i,1) L(j,1) - ... - L(i,j-1) L(j,j-1), then sqrt (i = j) or / L(j,j)
+    //   y_i    = b_i - L(i,0) y_0 - ... - L(i,i-1) y_{i-1}, then / L(i,i)
+    //   x_i    = y_i - L(n-1,i) x_{n-1} - ... - L(i+1,i) x_{i+1}, then / L(i,i)
+    // (each "-" one fused multiply-subtract), so every launch shape gives bitwise the same step and the batch
+    // driver can pick the shape by time alone. A failed pivot (not positive, or not finite) rejects the step, as
+    // a breakdown of the iterative solve does on the single-problem path.
+
+    // acc[r][jj] -= sum_{k < j0} L(row_r, k) L(j0 + jj, k) for the RA active row slots of this thread, k ascending;
+    // Bt holds L(j0 + jj, k) at k * NB + jj. The loads of the next group of columns are issued before the
+    // arithmetic of the current one (two register buffers), so their latency overlaps it.
+    template <int NT, int NB, int RA, int R>
     __device__ __forceinline__ void panelUpdate(const double* A, const double* Bt, int n, int j0, int tid,
-                                                double (&acc)[kDenseRows][kDensePanel])
+                                                double (&acc)[R][NB])
     {
-        int row[R];
+        constexpr int L0 = (R * NB >= 32 || NT >= 512) ? 8 : 16;
+        constexpr int L = L0 < NB ? L0 : NB;  // columns per load group (j0 is a multiple of NB, so of L)
+        if (j0 == 0) return;
+        int row[RA];
 #pragma unroll
-        for (int r = 0; r < R; ++r) row[r] = j0 + tid + r * kDenseThreads;
-        // the loads of kDenseLoads columns are issued together, so their latency overlaps (j0 is a multiple of
-        // the panel width, itself a multiple of kDenseLoads)
-        for (int k0 = 0; k0 < j0; k0 += kDenseLoads)
-        {
-            double a[kDenseLoads][R];
+        for (int r = 0; r < RA; ++r) row[r] = j0 + tid + r * NT;
+        double a0[L][RA], a1[L][RA];
+        auto load = [&](double (&a)[L][RA], int k0) {
 #pragma unroll
-            for (int u = 0; u < kDenseLoads; ++u)
+            for (int u = 0; u < L; ++u)
 #pragma unroll
-                for (int r = 0; r < R; ++r) a[u][r] = row[r] < n ? A[size_t(k0 + u) * n + row[r]] : 0.0;
+                for (int r = 0; r < RA; ++r) a[u][r] = row[r] < n ? A[size_t(k0 + u) * n + row[r]] : 0.0;
+        };
+        auto update = [&](const double (&a)[L][RA], int k0) {
 #pragma unroll
-            for (int u = 0; u < kDenseLoads; ++u)
+            for (int u = 0; u < L; ++u)
             {
-                const double2* b2 = reinterpret_cast<const double2*>(Bt + (k0 + u) * kDensePanel);
+                const double2* b2 = reinterpret_cast<const double2*>(Bt + (k0 + u) * NB);
 #pragma unroll
-                for (int q = 0; q < kDensePanel / 2; ++q)
+                for (int q = 0; q < NB / 2; ++q)
                 {
                     const double2 b = b2[q];
 #pragma unroll
-                    for (int r = 0; r < R; ++r)
+                    for (int r = 0; r < RA; ++r)
                     {
                         acc[r][2 * q] -= a[u][r] * b.x;
                         acc[r][2 * q + 1] -= a[u][r] * b.y;
                     }
                 }
             }
+        };
+        load(a0, 0);
+        for (int k0 = 0; k0 < j0; k0 += 2 * L)
+        {
+            if (k0 + L < j0) load(a1, k0 + L);
+            update(a0, k0);
+            if (k0 + L >= j0) break;
+            if (k0 + 2 * L < j0) load(a0, k0 + 2 * L);
+            update(a1, k0 + L);
         }
     }
 
-    template <typename T>
-    __global__ void __launch_bounds__(kDenseThreads) k_batch_factor(Problem P, Work<T> Wk, Batch B)
+    // Two workgroups per CU (every window of a 512-window batch resident at once) need NT / 128 waves per SIMD.
+    template <typename T, int NT, int NB>
+    __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 256 ? NT / 128 : 1)))
+    k_batch_factor(Problem P, Work<T> Wk, Batch B)
     {
+        constexpr int R = kMaxDense / NT;
+        static_assert(R * NT == kMaxDense && NB % 2 == 0 && NB <= kWave, "factorisation shape");
         extern __shared__ double lds[];
         const int w = blockIdx.x;
         if (!B.active[w] || B.reject[w]) return;
@@ -148,12 +173,9 @@ namespace pbk
         const int n = 6 * nU;
         if (n == 0) return;
         const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-        constexpr int kWaves = kDenseThreads / kWave;
-        constexpr int NB = kDensePanel;
         double* Bt = lds;                         // [maxDim][NB]  L(j0 + jj, k) at k * NB + jj
         double* Dg = Bt + size_t(B.maxDim) * NB;  // [NB][NB]      diagonal block of the panel, row-major
         double* vec = Dg + NB * NB;               // [maxDim]      right-hand side / solution
-        double* red = vec + B.maxDim;             // [kWaves][NB]  backward-substitution partials
         __shared__ int failed;
         if (tid == 0) failed = 0;
         double* A = B.dense + B.denseOffset[w];
@@ -162,21 +184,21 @@ namespace pbk
 
         // gather: zero, then every element of the stored blocks of the window's rows (the pose rows and the
         // group rows are two contiguous ranges of the block-sparse rows) goes to its place, all loads independent
-        for (long i = tid; i < long(n) * n; i += kDenseThreads) A[i] = 0.0;
-        for (int i = tid; i < n; i += kDenseThreads) vec[i] = Wk.rhs[unitOf(i / 6) * 6 + i % 6];
+        for (long i = tid; i < long(n) * n; i += NT) A[i] = 0.0;
+        for (int i = tid; i < n; i += NT) vec[i] = Wk.rhs[unitOf(i / 6) * 6 + i % 6];
         __syncthreads();
         {
             const int kp0 = P.rowPtr[pu0], nbp = P.rowPtr[pu1] - kp0;
             const int kg0 = P.rowPtr[gu0], nbg = P.rowPtr[B.groupUnitOffset[w + 1]] - kg0;
             const long total = long(nbp + nbg) * 36;
-            for (long e0 = tid; e0 < total; e0 += kDenseLoads * kDenseThreads)
+            for (long e0 = tid; e0 < total; e0 += kDenseLoads * NT)
             {
                 double v[kDenseLoads];
                 size_t at[kDenseLoads];
 #pragma unroll
                 for (int u = 0; u < kDenseLoads; ++u)
                 {
-                    const long e = e0 + long(u) * kDenseThreads;
+                    const long e = e0 + long(u) * NT;
                     const long ec = e < total ? e : total - 1;
                     const int bi = int(ec / 36), el = int(ec % 36);
                     const int k = bi < nbp ? kp0 + bi : kg0 + (bi - nbp);
@@ -186,7 +208,7 @@ namespace pbk
                 }
 #pragma unroll
                 for (int u = 0; u < kDenseLoads; ++u)
-                    if (e0 + long(u) * kDenseThreads < total) A[at[u]] = v[u];
+                    if (e0 + long(u) * NT < total) A[at[u]] = v[u];
             }
         }
         __syncthreads();
@@ -196,38 +218,46 @@ namespace pbk
         {
             const int jb = min(NB, n - j0);
             // staging: groups of kDenseLoads independent loads per thread before the LDS stores
-            for (int q0 = tid; q0 < j0 * NB; q0 += kDenseLoads * kDenseThreads)
+            for (int q0 = tid; q0 < j0 * NB; q0 += kDenseLoads * NT)
             {
                 double v[kDenseLoads];
 #pragma unroll
                 for (int u = 0; u < kDenseLoads; ++u)
                 {
-                    const int q = q0 + u * kDenseThreads, k = q / NB, jj = q % NB;
+                    const int q = q0 + u * NT, k = q / NB, jj = q % NB;
                     v[u] = (q < j0 * NB && jj < jb) ? A[size_t(k) * n + j0 + jj] : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < kDenseLoads; ++u)
-                    if (q0 + u * kDenseThreads < j0 * NB) Bt[q0 + u * kDenseThreads] = v[u];
+                    if (q0 + u * NT < j0 * NB) Bt[q0 + u * NT] = v[u];
             }
             __syncthreads();
-            double acc[kDenseRows][NB];
+            double acc[R][NB];
 #pragma unroll
-            for (int r = 0; r < kDenseRows; ++r)
+            for (int r = 0; r < R; ++r)
             {
-                const int i = j0 + tid + r * kDenseThreads;
+                const int i = j0 + tid + r * NT;
 #pragma unroll
                 for (int jj = 0; jj < NB; ++jj) acc[r][jj] = (i < n && jj < jb) ? A[size_t(j0 + jj) * n + i] : 0.0;
             }
-            // rows of this wave beyond the first kDenseThreads of the panel (wave-uniform)
-            if (j0 + kDenseThreads + (tid & ~63) < n)
-                panelUpdate<2>(A, Bt, n, j0, tid, acc);
-            else if (j0 + (tid & ~63) < n)
-                panelUpdate<1>(A, Bt, n, j0, tid, acc);
+            // row slots of this wave that reach into the matrix (wave-uniform)
+            const int first = j0 + (tid & ~63);
+            int ra = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (first + r * NT < n) ra = r + 1;
+            if constexpr (R >= 4)
+                if (ra == 4) panelUpdate<NT, NB, 4>(A, Bt, n, j0, tid, acc);
+            if constexpr (R >= 3)
+                if (ra == 3) panelUpdate<NT, NB, 3>(A, Bt, n, j0, tid, acc);
+            if constexpr (R >= 2)
+                if (ra == 2) panelUpdate<NT, NB, 2>(A, Bt, n, j0, tid, acc);
+            if (ra == 1) panelUpdate<NT, NB, 1>(A, Bt, n, j0, tid, acc);
             // the diagonal block goes to LDS; wave 0 factors it in registers (lane l holds row l)
 #pragma unroll
-            for (int r = 0; r < kDenseRows; ++r)
+            for (int r = 0; r < R; ++r)
             {
-                const int i = j0 + tid + r * kDenseThreads;
+                const int i = j0 + tid + r * NT;
                 if (i < j0 + jb)
                 {
 #pragma unroll
@@ -269,9 +299,9 @@ namespace pbk
             __syncthreads();
             // rows below the diagonal block: L(i, panel) = acc L_D^-T (forward substitution over the panel)
 #pragma unroll
-            for (int r = 0; r < kDenseRows; ++r)
+            for (int r = 0; r < R; ++r)
             {
-                const int i = j0 + tid + r * kDenseThreads;
+                const int i = j0 + tid + r * NT;
                 if (i >= n) continue;
                 if (i < j0 + jb)
                 {
@@ -301,7 +331,8 @@ namespace pbk
             return;
         }
 
-        // forward substitution L y = b
+        // forward substitution L y = b: wave 0 solves the panel's triangle, then every row below subtracts the
+        // panel's terms
         for (int j0 = 0; j0 < n; j0 += NB)
         {
             const int jb = min(NB, n - j0);
@@ -325,7 +356,7 @@ namespace pbk
                 if (lane < jb) vec[j0 + lane] = val;
             }
             __syncthreads();
-            for (int i = j0 + jb + tid; i < n; i += kDenseThreads)
+            for (int i = j0 + jb + tid; i < n; i += NT)
             {
                 double s = vec[i];
 #pragma unroll
@@ -335,56 +366,43 @@ namespace pbk
             }
             __syncthreads();
         }
-        // backward substitution L^T x = y, panels from the last
+        // backward substitution L^T x = y, panels from the last: wave 0 solves the panel's triangle (lane l
+        // holds column l of the panel's L, i.e. row l of L^T), then every row above subtracts the panel's terms,
+        // highest index first
         for (int j0 = ((n - 1) / NB) * NB; j0 >= 0; j0 -= NB)
         {
             const int jb = min(NB, n - j0);
-            double part[NB];
-#pragma unroll
-            for (int jj = 0; jj < NB; ++jj) part[jj] = 0.0;
-            for (int i = j0 + jb + tid; i < n; i += kDenseThreads)
-            {
-                const double xi = vec[i];
-#pragma unroll
-                for (int jj = 0; jj < NB; ++jj)
-                    if (jj < jb) part[jj] += A[size_t(j0 + jj) * n + i] * xi;
-            }
-#pragma unroll
-            for (int jj = 0; jj < NB; ++jj)
-            {
-                const double s = waveSum(part[jj]);
-                if (lane == 0) red[wv * NB + jj] = s;
-            }
-            // the panel's diagonal block, row-major
-            for (int q = tid; q < NB * NB; q += kDenseThreads)
-            {
-                const int i = q / NB, j = q % NB;
-                Dg[q] = (i < jb && j <= i) ? A[size_t(j0 + j) * n + j0 + i] : 0.0;
-            }
-            __syncthreads();
             if (wv == 0)
             {
-                double val = 0.0;
-                if (lane < jb)
-                {
-                    val = vec[j0 + lane];
-                    for (int q = 0; q < kWaves; ++q) val -= red[q * NB + lane];
-                }
+                double colv[NB];  // colv[jj] = L(j0 + jj, j0 + lane) for jj >= lane
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj)
+                    colv[jj] = (lane < jb && jj < jb && jj >= lane) ? A[size_t(j0 + lane) * n + j0 + jj] : 0.0;
+                double val = lane < jb ? vec[j0 + lane] : 0.0;
 #pragma unroll
                 for (int j = NB - 1; j >= 0; --j)
                 {
                     if (j < jb)
                     {
-                        const double xj = __shfl(val, j, kWave) / Dg[j * NB + j];
-                        if (lane < j) val -= Dg[j * NB + (lane & (NB - 1))] * xj;
+                        const double xj = __shfl(val, j, kWave) / __shfl(colv[j], j, kWave);
+                        if (lane < j) val -= colv[j] * xj;
                         if (lane == j) val = xj;
                     }
                 }
                 if (lane < jb) vec[j0 + lane] = val;
             }
             __syncthreads();
+            for (int i = tid; i < j0; i += NT)
+            {
+                double s = vec[i];
+#pragma unroll
+                for (int jj = NB - 1; jj >= 0; --jj)
+                    if (jj < jb) s -= A[size_t(i) * n + j0 + jj] * vec[j0 + jj];
+                vec[i] = s;
+            }
+            __syncthreads();
         }
-        for (int i = tid; i < n; i += kDenseThreads) Wk.x[unitOf(i / 6) * 6 + i % 6] = T(vec[i]);
+        for (int i = tid; i < n; i += NT) Wk.x[unitOf(i / 6) * 6 + i % 6] = T(vec[i]);
     }
 
     // Directional decrease dc.directRhs + sum bp.dp and the two step RMS values of each window
@@ -730,30 +748,64 @@ namespace pbk
 
     namespace
     {
-        size_t factorLds(const Batch& B)
+        size_t factorLds(const Batch& B, int shape)
         {
-            return sizeof(double) * (size_t(B.maxDim) * kDensePanel + kDensePanel * kDensePanel + B.maxDim +
-                                     (kDenseThreads / kWave) * kDensePanel);
+            const size_t nb = size_t(kFactorShape[shape].panel);
+            return sizeof(double) * (size_t(B.maxDim) * nb + nb * nb + size_t(B.maxDim));
         }
+
+        template <int V, typename T>
+        constexpr auto factorKernel()
+        {
+            return &k_batch_factor<T, kFactorShape[V].threads, kFactorShape[V].panel>;
+        }
+
+        template <typename T>
+        const void* factorFunction(int shape)
+        {
+            switch (shape)
+            {
+                case 0: return reinterpret_cast<const void*>(factorKernel<0, T>());
+                case 1: return reinterpret_cast<const void*>(factorKernel<1, T>());
+                case 2: return reinterpret_cast<const void*>(factorKernel<2, T>());
+                case 3: return reinterpret_cast<const void*>(factorKernel<3, T>());
+                case 4: return reinterpret_cast<const void*>(factorKernel<4, T>());
+                default: return reinterpret_cast<const void*>(factorKernel<5, T>());
+            }
+        }
+        static_assert(kFactorShapes == 6, "factorFunction and BatchLauncher::factor list every shape");
+    }  // namespace
+
+    template <typename T>
+    void BatchLauncher<T>::configure(const Batch& B, bool (&usable)[kFactorShapes])
+    {
+        for (int v = 0; v < kFactorShapes; ++v)
+        {
+            const size_t lds = factorLds(B, v);
+            usable[v] = lds <= size_t(kFactorLdsLimit);
+            // above the default dynamic LDS limit (kMaxDense = 512 with 16-wide panels needs 66 KiB of the CU's
+            // 160 KiB)
+            if (usable[v] && lds > 64 * 1024)
+                usable[v] = hipFuncSetAttribute(factorFunction<T>(v), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                int(lds)) == hipSuccess;
+        }
+        if (!usable[0]) throw std::runtime_error("hipFuncSetAttribute (dense window solve LDS)");
     }
 
     template <typename T>
-    void BatchLauncher<T>::configure(const Batch& B)
+    void BatchLauncher<T>::factor(const Problem& P, const Work<T>& W, const Batch& B, int shape, hipStream_t s)
     {
-        const size_t lds = factorLds(B);
-        if (lds > 64 * 1024)
+        const size_t lds = factorLds(B, shape);
+        switch (shape)
         {
-            // above the default dynamic LDS limit (kMaxDense = 512 needs 72 KiB of the CU's 160 KiB)
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_factor<T>),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-            if (e != hipSuccess) throw std::runtime_error("hipFuncSetAttribute (dense window solve LDS)");
+            case 0: factorKernel<0, T>()<<<B.nWin, kFactorShape[0].threads, lds, s>>>(P, W, B); break;
+            case 1: factorKernel<1, T>()<<<B.nWin, kFactorShape[1].threads, lds, s>>>(P, W, B); break;
+            case 2: factorKernel<2, T>()<<<B.nWin, kFactorShape[2].threads, lds, s>>>(P, W, B); break;
+            case 3: factorKernel<3, T>()<<<B.nWin, kFactorShape[3].threads, lds, s>>>(P, W, B); break;
+            case 4: factorKernel<4, T>()<<<B.nWin, kFactorShape[4].threads, lds, s>>>(P, W, B); break;
+            case 5: factorKernel<5, T>()<<<B.nWin, kFactorShape[5].threads, lds, s>>>(P, W, B); break;
+            default: throw std::invalid_argument("factorisation shape out of range");
         }
-    }
-
-    template <typename T>
-    void BatchLauncher<T>::factor(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s)
-    {
-        k_batch_factor<T><<<B.nWin, kDenseThreads, factorLds(B), s>>>(P, W, B);
     }
 
     template <typename T>

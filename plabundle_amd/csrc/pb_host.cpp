@@ -938,6 +938,20 @@ namespace
         pbk::Batch B;
         double clockHz = 1e8;
     };
+
+    // Launch shape of the dense window factorisation per assembly precision ([0] FP32, [1] FP64 assembly): every
+    // usable shape is timed on the first round of a batch (each leaves bitwise the same result) and the fastest
+    // is kept for the later rounds and for later batches of the same class: window count rounded up to a power
+    // of two and largest window system rounded up to a multiple of 64 unknowns (the two figures that set the
+    // occupancy and the longest factorisation).
+    struct FactorTuning
+    {
+        std::vector<int32_t> key;
+        int shape[2] = {-1, -1};   // -1: not timed yet
+        double ms[2][pbk::kFactorShapes] = {};
+        bool usable[2][pbk::kFactorShapes] = {};
+        hipEvent_t ev[pbk::kFactorShapes][2] = {};
+    };
 }  // namespace
 
 struct pb_ctx
@@ -962,6 +976,7 @@ struct pb_ctx
     int lastSolveStatus = PB_SOLVE_NOT_RUN, lastUsable = 0;
     bool batchMode = false;  // the context holds a batch (pb_hip_solve_batch); single-problem entries refuse it
     BatchHost bh;
+    FactorTuning tune;
     bool work32 = false, work64 = false;
     pbk::Problem P;
     pbk::Work<float> W32;
@@ -1227,6 +1242,8 @@ namespace
         if (o->precision != PB_PRECISION_FP64 && o->precision != PB_PRECISION_MIXED) return "precision is invalid";
         if (o->pcg_relative_tolerance < 0.0 || o->pcg_absolute_tolerance < 0.0 || o->pcg_max_iterations < 0)
             return "PCG controls must be non-negative";
+        if (o->batch_factor_shape < -1 || o->batch_factor_shape >= pbk::kFactorShapes)
+            return "batch_factor_shape must be -1 (timed) or a shape index below " + std::to_string(pbk::kFactorShapes);
         return {};
     }
 
@@ -2539,7 +2556,15 @@ namespace
     };
 
     template <typename T>
-    void batchSequence(pb_ctx* ctx, pbk::Work<T>& W, int phase, int passAlpha, int32_t* hc, int which)
+    constexpr int precisionIndex()
+    {
+        return std::is_same<T, double>::value ? 1 : 0;
+    }
+
+    // shape < 0: every usable factorisation shape runs once, each between two events (the last one's result is
+    // the one used, and all of them are the same)
+    template <typename T>
+    void batchSequence(pb_ctx* ctx, pbk::Work<T>& W, int phase, int passAlpha, int32_t* hc, int which, int shape)
     {
         using BL = pbk::BatchLauncher<T>;
         hipStream_t s = ctx->stream;
@@ -2550,7 +2575,27 @@ namespace
             BL::roundBegin(P, W, B, phase, s);
             BL::build(P, W, s);
             BL::afterBuild(P, W, B, s);
-            BL::factor(P, W, B, s);
+            if (shape >= 0)
+                BL::factor(P, W, B, shape, s);
+            else
+            {
+                FactorTuning& ft = ctx->tune;
+                const int pi = precisionIndex<T>();
+                for (int v = 0; v < pbk::kFactorShapes; ++v)
+                {
+                    if (!ft.usable[pi][v]) continue;
+                    for (hipEvent_t& e : ft.ev[v])
+                        if (!e)
+                        {
+                            check(hipEventCreate(&e), "hipEventCreate");
+                            ctx->eventPool.push_back(e);  // destroyed with the context
+                            ++ctx->eventNext;
+                        }
+                    check(hipEventRecord(ft.ev[v][0], s), "hipEventRecord");
+                    BL::factor(P, W, B, v, s);
+                    check(hipEventRecord(ft.ev[v][1], s), "hipEventRecord");
+                }
+            }
             BL::backSubstitute(P, W, B, s);
         }
         if (which <= 1)
@@ -2566,12 +2611,12 @@ namespace
 
     template <typename T>
     void batchRun(pb_ctx* ctx, pbk::Work<T>& W, int phase, int passAlpha, int32_t* hc, int which,
-                  BatchRoundGraphs* graphs)
+                  BatchRoundGraphs* graphs, int shape)
     {
         hipStream_t s = ctx->stream;
-        if (!graphs)
+        if (!graphs || shape < 0)
         {
-            batchSequence<T>(ctx, W, phase, passAlpha, hc, which);
+            batchSequence<T>(ctx, W, phase, passAlpha, hc, which, shape);
             return;
         }
         if (!graphs->exec[which])
@@ -2580,7 +2625,7 @@ namespace
             check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try
             {
-                batchSequence<T>(ctx, W, phase, passAlpha, hc, which);
+                batchSequence<T>(ctx, W, phase, passAlpha, hc, which, shape);
             }
             catch (...)
             {
@@ -2596,23 +2641,46 @@ namespace
         check(hipGraphLaunch(graphs->exec[which], s), "hipGraphLaunch");
     }
 
+    // Keeps the fastest factorisation shape of the timed head (ties and untimed shapes: the lower index).
+    void pickFactorShape(pb_ctx* ctx, int pi)
+    {
+        FactorTuning& ft = ctx->tune;
+        int best = -1;
+        for (int v = 0; v < pbk::kFactorShapes; ++v)
+        {
+            ft.ms[pi][v] = -1.0;
+            if (!ft.usable[pi][v]) continue;
+            float ms = 0.0f;
+            check(hipEventElapsedTime(&ms, ft.ev[v][0], ft.ev[v][1]), "hipEventElapsedTime");
+            ft.ms[pi][v] = ms;
+            if (best < 0 || ms < ft.ms[pi][best]) best = v;
+        }
+        ft.shape[pi] = best < 0 ? 0 : best;
+    }
+
     template <typename T>
     int batchIteration(pb_ctx* ctx, pbk::Work<T>& W, int phase, const pb_solve_options& o, int passAlpha,
-                       int32_t* hc, BatchRoundGraphs* graphs)
+                       int32_t* hc, BatchRoundGraphs* graphs, int& shape)
     {
         hipStream_t s = ctx->stream;
         // trial passes until no window is still searching (each pass spends at least one of a window's
-        // line-search steps, so at most line_search_steps passes)
-        batchRun<T>(ctx, W, phase, passAlpha, hc, 0, graphs);
+        // line-search steps, so at most line_search_steps passes); a head with shape < 0 times the factorisation
+        // shapes and fixes the shape for the rest of the batch
+        batchRun<T>(ctx, W, phase, passAlpha, hc, 0, graphs, shape);
         check(hipStreamSynchronize(s), "trial pass");
+        if (shape < 0)
+        {
+            pickFactorShape(ctx, precisionIndex<T>());
+            shape = ctx->tune.shape[precisionIndex<T>()];
+        }
         int passes = 1;
         for (int pass = 1; pass <= o.line_search_steps && hc[2] != 0; ++pass)
         {
-            batchRun<T>(ctx, W, phase, passAlpha, hc, 1, graphs);
+            batchRun<T>(ctx, W, phase, passAlpha, hc, 1, graphs, shape);
             check(hipStreamSynchronize(s), "trial pass");
             ++passes;
         }
-        batchRun<T>(ctx, W, phase, passAlpha, hc, 2, graphs);
+        batchRun<T>(ctx, W, phase, passAlpha, hc, 2, graphs, shape);
         check(hipStreamSynchronize(s), "round");
         return passes;
     }
@@ -2657,8 +2725,27 @@ namespace
         check(hipMemsetAsync(B.doneClock, 0, sizeof(unsigned long long) * (nWin + 1), s), "memset");
         zeroDirect(ctx, W64);
         if (W32) zeroDirect(ctx, *W32);
-        pbk::BatchLauncher<double>::configure(B);
-        if (W32) pbk::BatchLauncher<float>::configure(B);
+        FactorTuning& ft = ctx->tune;
+        int winClass = 1;
+        while (winClass < nWin) winClass *= 2;
+        const std::vector<int32_t> key{winClass, (B.maxDim + 63) / 64};
+        if (ft.key != key)
+        {
+            ft.key = key;
+            ft.shape[0] = ft.shape[1] = -1;
+        }
+        pbk::BatchLauncher<double>::configure(B, ft.usable[1]);
+        if (W32) pbk::BatchLauncher<float>::configure(B, ft.usable[0]);
+        // shape per precision: fixed by the options, else the tuned one (-1: timed on this batch's first round)
+        int shape[2];
+        for (int pi = 0; pi < 2; ++pi)
+        {
+            shape[pi] = o.batch_factor_shape >= 0 ? o.batch_factor_shape : ft.shape[pi];
+            if (o.batch_factor_shape >= 0 && (pi == 1 || W32) && !ft.usable[pi][o.batch_factor_shape])
+                throw std::invalid_argument("unsupported: batch_factor_shape " + std::to_string(o.batch_factor_shape) +
+                                  " needs more LDS than a workgroup has for windows of " +
+                                  std::to_string(B.maxDim) + " unknowns");
+        }
         check(hipStreamSynchronize(s), "reset");
         // captured once per batch solve (the launch arguments are this batch's), replayed every round
         BatchRoundGraphs graphs64, graphs32;
@@ -2679,8 +2766,8 @@ namespace
                 cancelled = true;
                 break;
             }
-            if (W32 && hc[0] > 0) passes += batchIteration<float>(ctx, *W32, 0, o, passAlpha, hc, g32);
-            if (hc[1] > 0) passes += batchIteration<double>(ctx, W64, 1, o, passAlpha, hc, g64);
+            if (W32 && hc[0] > 0) passes += batchIteration<float>(ctx, *W32, 0, o, passAlpha, hc, g32, shape[0]);
+            if (hc[1] > 0) passes += batchIteration<double>(ctx, W64, 1, o, passAlpha, hc, g64, shape[1]);
             ++rounds;
         }
         check(hipStreamSynchronize(s), "batch");
@@ -2867,6 +2954,7 @@ void pb_default_options(pb_solve_options* o)
     o->use_transition_prior = 0;
     o->previous_intrinsic_mask = 0;
     o->batch_graphs = 1;
+    o->batch_factor_shape = -1;
     o->refine_camera_pose = 1;
     o->shared_intrinsic_mask = 0;
     o->min_focal_scale = 0.5;
@@ -3204,6 +3292,26 @@ pb_status pb_hip_batch_download(pb_ctx* ctx, int32_t window, double* R, double* 
     const BatchHost& bh = ctx->bh;
     downloadRange(ctx, bh.camBase[window], bh.camBase[window + 1], bh.trackBase[window], bh.trackBase[window + 1], R,
                   C, intr, X, valid);
+    return PB_OK;
+}
+
+static_assert(PB_BATCH_FACTOR_SHAPES == pbk::kFactorShapes, "factorisation shapes of the ABI and the kernels");
+
+pb_status pb_hip_batch_factor_shape(int32_t shape, int32_t* threads, int32_t* panel)
+{
+    if (shape < 0 || shape >= pbk::kFactorShapes) return PB_ERR_INVALID;
+    if (threads) *threads = pbk::kFactorShape[shape].threads;
+    if (panel) *panel = pbk::kFactorShape[shape].panel;
+    return PB_OK;
+}
+
+pb_status pb_hip_batch_tuning(pb_ctx* ctx, int32_t fp64, int32_t* shape, double* ms)
+{
+    if (!ctx || !shape || (fp64 != 0 && fp64 != 1)) return PB_ERR_INVALID;
+    const FactorTuning& ft = ctx->tune;
+    *shape = ft.shape[fp64];
+    if (ms)
+        for (int v = 0; v < pbk::kFactorShapes; ++v) ms[v] = ft.shape[fp64] >= 0 ? ft.ms[fp64][v] : -1.0;
     return PB_OK;
 }
 

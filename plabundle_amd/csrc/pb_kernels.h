@@ -212,12 +212,20 @@ namespace pbk
         kWinCounterCount = 8
     };
 
-    constexpr int kDensePanel = 16;  // column panel of the dense window factorisation
-    constexpr int kDenseLoads = 16;  // global loads a thread of the dense window solve issues together
-    static_assert(kDensePanel % kDenseLoads == 0, "panel starts are multiples of the load group");
-    constexpr int kDenseRows = 2;    // matrix rows per thread of the factorisation workgroup
-    constexpr int kDenseThreads = 256;
-    constexpr int kMaxDense = kDenseRows * kDenseThreads;  // largest window system (6 * primary units)
+    constexpr int kMaxDense = 512;     // largest window system (6 * primary units)
+    constexpr int kDenseLoads = 16;    // global loads a thread of the dense window gather issues together
+
+    // Launch shapes of the dense window factorisation: workgroup threads and column-panel width. Every shape
+    // computes bitwise the same factor and solution (each element is updated by one thread in a fixed order),
+    // so the batch driver picks among them by measured time alone. Shape 0 is the fixed default.
+    struct FactorShape
+    {
+        int threads, panel;
+    };
+    constexpr int kFactorShapes = 6;
+    constexpr FactorShape kFactorShape[kFactorShapes] = {{256, 16}, {128, 16}, {512, 16},
+                                                         {256, 8},  {128, 8},  {512, 8}};
+    constexpr int kFactorLdsLimit = 160 * 1024;  // LDS of one CU (gfx950)
     constexpr int kTrialChunk = 1024;  // observations per workgroup of a batched trial-cost pass
 
     struct Batch
@@ -364,13 +372,14 @@ namespace pbk
     template <typename T>
     struct BatchLauncher
     {
-        static void configure(const Batch& B);  // kernel attributes (outside any stream capture)
+        // kernel attributes (outside any stream capture); usable[v] = factorisation shape v can be launched
+        static void configure(const Batch& B, bool (&usable)[kFactorShapes]);
         static void start(const Batch& B, hipStream_t s);
         static void count(const Batch& B, hipStream_t s);  // running windows per phase -> counts[0], counts[1]
         static void roundBegin(const Problem& P, const Work<T>& W, const Batch& B, int phase, hipStream_t s);
         static void build(const Problem& P, const Work<T>& W, hipStream_t s);
         static void afterBuild(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
-        static void factor(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
+        static void factor(const Problem& P, const Work<T>& W, const Batch& B, int shape, hipStream_t s);
         static void backSubstitute(const Problem& P, const Work<T>& W, const Batch& B, hipStream_t s);
         static void trialPass(const Problem& P, const Work<T>& W, const Batch& B, int maxAlpha, hipStream_t s);
         static void finishRound(const Problem& P, const Work<T>& W, const Batch& B, int phase, hipStream_t s);

@@ -137,6 +137,7 @@ class BackendOptions:
     pcgModelEta: float = -1.0  # < 0: path default (off for FP64, 0.1 for mixed)
     stepRmsTolerance: float = 0.0  # 0: the reference's 1e-6
     batchGraphs: bool = True  # batched windows: replay each round's launches as captured HIP graphs
+    batchFactorShape: int = -1  # batched windows: dense factorisation launch shape (-1 = timed on the first round)
 
 
 @dataclass
@@ -232,6 +233,7 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.pcg_model_eta = float(b.pcgModelEta)
     o.step_rms_tolerance = float(b.stepRmsTolerance)
     o.batch_graphs = 1 if b.batchGraphs else 0
+    o.batch_factor_shape = int(b.batchFactorShape)
     k = options.constraints
     o.laser_plane_weight = float(k.laserPlaneWeight)
     o.laser_huber_delta = float(k.laserHuberDeltaMeters)

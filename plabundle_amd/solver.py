@@ -64,6 +64,14 @@ class Result:
         return self.report.get("final_cost", float("nan"))
 
 
+def factor_shape(shape: int):
+    """(threads, panel width) of a launch shape of the batched dense window factorisation."""
+    t, p = C.c_int32(), C.c_int32()
+    if _abi.load_library().pb_hip_batch_factor_shape(int(shape), C.byref(t), C.byref(p)) != _abi.PB_OK:
+        raise ValueError(f"no factorisation shape {shape}")
+    return t.value, p.value
+
+
 class SolverWorkspace:
     """Reusable device workspace (the SolverWorkspace analogue, include/plabundle/solver.h:13-32)."""
 
@@ -120,6 +128,23 @@ class SolverWorkspace:
         if st != _abi.PB_OK:
             raise RuntimeError(f"pb_hip_reduced_system: {self.last_error()}")
         return A, b, cost.value
+
+    def batch_tuning(self) -> dict:
+        """Factorisation-shape timing of the batches solved on this workspace (pb_hip_batch_tuning): per
+        assembly precision the chosen shape (threads, panel) and the measured milliseconds of every shape."""
+        out = {}
+        for name, fp64 in (("fp32", 0), ("fp64", 1)):
+            shape = C.c_int32()
+            ms = np.zeros(_abi.PB_BATCH_FACTOR_SHAPES)
+            if self._lib.pb_hip_batch_tuning(self._ctx, fp64, C.byref(shape), _abi.ptr(ms, C.c_double)) != _abi.PB_OK:
+                continue
+            if shape.value < 0:
+                out[name] = None
+                continue
+            out[name] = {"shape": shape.value, "threads_panel": factor_shape(shape.value),
+                         "ms": {f"{t}x{p}": round(float(m), 4) for (t, p), m in
+                                zip((factor_shape(v) for v in range(_abi.PB_BATCH_FACTOR_SHAPES)), ms) if m >= 0}}
+        return out
 
     def evaluate_cost(self, copts) -> float:
         cost = C.c_double()
@@ -292,6 +317,7 @@ class Solver:
         del keep
         out = summary.as_dict()
         out["abi_status"] = st
+        out["factor_tuning"] = ws.batch_tuning()
         for w, (p, r) in enumerate(zip(problems, results)):
             rep = reports[w]
             r.abi_status = st

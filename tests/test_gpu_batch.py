@@ -178,3 +178,57 @@ def test_batch_graph_replay_matches_direct_launches(ws, precision):
             assert a.finalCost == b.finalCost
             np.testing.assert_array_equal(a.cam_center, b.cam_center)
             np.testing.assert_array_equal(a.track_point, b.track_point)
+
+
+@pytest.mark.parametrize("precision", [PO.Precision.Fp64, PO.Precision.Mixed])
+def test_batch_factor_shapes_are_bitwise_equal_and_timed(ws, precision):
+    # every launch shape of the dense window factorisation updates each value in the same order, so fixed
+    # shapes and the timed choice give bitwise the same windows; the windows span 8 to 64 cameras (systems of
+    # up to 402 unknowns: every row slot of the 128-thread shapes is used)
+    _say(f"case batch factorisation shapes precision={precision.name}")
+    rng = np.random.Generator(np.random.PCG64(21))
+    wins = _windows(n=4, seed=17) + [S.window_problem(rng, 64, 1500), S.window_problem(rng, 41, 900)]
+    o = S.window_options(PO.SolveOptions(), max_iterations=6)
+    runs = []
+    for shape in range(_abi.PB_BATCH_FACTOR_SHAPES):
+        o.backend.batchFactorShape = shape
+        res, summ = Solver().solve_batch(wins, o, ws, precision=precision)
+        assert summ["abi_status"] == _abi.PB_OK, summ["message"]
+        runs.append(res)
+    for shape, res in enumerate(runs[1:], start=1):
+        for w, (a, b) in enumerate(zip(runs[0], res)):
+            assert a.report["iterations"] == b.report["iterations"], (shape, w)
+            assert a.finalCost == b.finalCost, (shape, w)
+            np.testing.assert_array_equal(a.cam_center, b.cam_center)
+            np.testing.assert_array_equal(a.cam_intrinsics, b.cam_intrinsics)
+            np.testing.assert_array_equal(a.track_point, b.track_point)
+    o.backend.batchFactorShape = -1
+    ws2 = SolverWorkspace(0)  # a fresh context times the shapes on its first batch
+    try:
+        for rep in range(2):
+            res, summ = Solver().solve_batch(wins, o, ws2, precision=precision)
+            assert summ["abi_status"] == _abi.PB_OK
+            tuning = summ["factor_tuning"]
+            _say(f"  solve {rep}: {tuning}")
+            names = ["fp64"] + (["fp32"] if precision == PO.Precision.Mixed else [])
+            for name in names:
+                t = tuning[name]
+                assert 0 <= t["shape"] < _abi.PB_BATCH_FACTOR_SHAPES
+                assert len(t["ms"]) == _abi.PB_BATCH_FACTOR_SHAPES and min(t["ms"].values()) > 0.0
+                assert t["ms"][f"{t['threads_panel'][0]}x{t['threads_panel'][1]}"] == min(t["ms"].values())
+            if rep == 0:
+                first = tuning
+            else:
+                assert tuning == first  # same batch class: the second batch keeps the timed shapes
+            for a, b in zip(runs[0], res):
+                assert a.finalCost == b.finalCost
+                np.testing.assert_array_equal(a.track_point, b.track_point)
+        # a batch of another class (2 windows, not 6) times the shapes again
+        res, summ = Solver().solve_batch(wins[-2:], o, ws2, precision=precision)
+        assert summ["abi_status"] == _abi.PB_OK
+        assert summ["factor_tuning"]["fp64"]["ms"] != first["fp64"]["ms"]
+    finally:
+        ws2.close()
+    o.backend.batchFactorShape = _abi.PB_BATCH_FACTOR_SHAPES
+    res, summ = Solver().solve_batch(wins, o, ws, precision=precision)
+    assert summ["abi_status"] == _abi.PB_ERR_INVALID
