@@ -225,6 +225,10 @@ typedef struct pb_solve_options {
      * system rounded up to a multiple of 64 unknowns; pb_hip_batch_tuning reports the times);
      * 0 .. PB_BATCH_FACTOR_SHAPES-1: that shape. Every shape gives bitwise the same results. */
     int32_t batch_factor_shape;
+    /* Backend control of pb_hip_solve and pb_hip_backward: 1 (default) replays the PCG iterations from a captured
+     * HIP graph of 16 launches (captured once per problem and stop parameters), 0 launches them one by one. Both
+     * give bitwise the same results. */
+    int32_t pcg_graphs;
 } pb_solve_options;
 
 /* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */

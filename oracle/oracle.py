@@ -120,6 +120,7 @@ def default_options() -> _abi.SolveOptionsC:
     o.previous_intrinsic_mask = 0
     o.batch_graphs = 1
     o.batch_factor_shape = -1
+    o.pcg_graphs = 1
     return o
 
 

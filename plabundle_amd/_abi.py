@@ -117,6 +117,7 @@ class SolveOptionsC(C.Structure):
         ("previous_intrinsic_mask", C.c_uint32),
         ("batch_graphs", C.c_int32),
         ("batch_factor_shape", C.c_int32),
+        ("pcg_graphs", C.c_int32),
     ]
 
 

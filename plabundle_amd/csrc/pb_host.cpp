@@ -977,6 +977,15 @@ struct pb_ctx
     bool batchMode = false;  // the context holds a batch (pb_hip_solve_batch); single-problem entries refuse it
     BatchHost bh;
     FactorTuning tune;
+    // Captured PCG iterations per work precision ([0] FP32, [1] FP64): kPcgGraphLaunches launches that take their
+    // absolute iteration from the device counter, then the counter's advance. Kept while the launch arguments
+    // (problem and work views, stop parameters) stay byte for byte the same; recaptured otherwise.
+    struct PcgGraph
+    {
+        hipGraphExec_t exec = nullptr;
+        std::vector<unsigned char> key;
+    } pcgGraph[2];
+    bool pcgGraphs = true;  // options.pcg_graphs of the current solve
     bool work32 = false, work64 = false;
     pbk::Problem P;
     pbk::Work<float> W32;
@@ -1529,6 +1538,7 @@ namespace
     void applySolveOptions(pb_ctx* ctx, const pb_solve_options& o)
     {
         pbk::Problem& P = ctx->P;
+        ctx->pcgGraphs = o.pcg_graphs != 0;
         P.loss = o.robust_loss;
         P.lossScale = o.robust_loss_scale;
         P.refParam = (o.use_online_schur && ctx->in.nControl == 0 && ctx->in.nPlanes == 0) ? 1 : 0;
@@ -1625,6 +1635,47 @@ namespace
         return r;
     }
 
+    constexpr int kPcgGraphLaunches = 16;  // PCG launches per captured graph (even: the counter stays even)
+    static_assert(kPcgGraphLaunches % 2 == 0, "a replay keeps the iteration parity");
+
+    // The captured PCG iterations for these launch arguments (captured on first use and after any change).
+    template <typename T>
+    hipGraphExec_t pcgGraph(pb_ctx* ctx, pbk::Work<T>& W, int maxIter, double relTol, double absTol, double modelEta)
+    {
+        using Lr = pbk::Launcher<T>;
+        const pbk::Problem& P = ctx->P;
+        hipStream_t s = ctx->stream;
+        auto& g = ctx->pcgGraph[std::is_same<T, double>::value ? 1 : 0];
+        std::vector<unsigned char> key(sizeof(P) + sizeof(W) + sizeof(int) + 3 * sizeof(double));
+        unsigned char* k = key.data();
+        std::memcpy(k, &P, sizeof(P));
+        k += sizeof(P);
+        std::memcpy(k, &W, sizeof(W));
+        k += sizeof(W);
+        std::memcpy(k, &maxIter, sizeof(int));
+        k += sizeof(int);
+        const double d[3] = {relTol, absTol, modelEta};
+        std::memcpy(k, d, sizeof(d));
+        if (g.exec && g.key == key) return g.exec;
+        if (g.exec)
+        {
+            (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        for (int j = 0; j < kPcgGraphLaunches; ++j)
+            Lr::pcgIteration(P, W, j, maxIter, relTol, absTol, modelEta, s, true);
+        Lr::pcgAdvance(P, W, kPcgGraphLaunches, s);
+        check(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) g.exec = nullptr;
+        check(e, "hipGraphInstantiate");
+        g.key = std::move(key);
+        return g.exec;
+    }
+
     struct PcgResult
     {
         bool converged = false, breakdown = false, modelStop = false;
@@ -1654,20 +1705,28 @@ namespace
         }
         // fused iterations; the host looks at the flags after chunks that grow with the iteration count
         // (launches after the stop return at once)
+        // with pcg_graphs, whole replays of the captured launches (a launch past the stop, or at maxIter where
+        // the kernel stops, returns at once)
+        hipGraphExec_t exec = ctx->pcgGraphs ? pcgGraph<T>(ctx, W, maxIter, relTol, absTol, modelEta) : nullptr;
         Lr::pcgInit(P, W, relTol, absTol, s);
         int it = 0;
         while (true)
         {
             const int chunk = std::min(std::max(checkEvery, it / 4), 256);
             const int last = std::min(it + chunk, maxIter + 1);
-            TimedSpan ts{nullptr, nullptr, last - it};
+            const int first = it;
+            TimedSpan ts{nullptr, nullptr, 0};
             if (ctx->profile)
             {
                 ts.start = takeEvent(ctx);
                 ts.stop = takeEvent(ctx);
                 check(hipEventRecord(ts.start, s), "hipEventRecord");
             }
-            for (; it < last; ++it) Lr::pcgIteration(P, W, it, maxIter, relTol, absTol, modelEta, s);
+            if (exec)
+                for (; it < last; it += kPcgGraphLaunches) check(hipGraphLaunch(exec, s), "hipGraphLaunch");
+            else
+                for (; it < last; ++it) Lr::pcgIteration(P, W, it, maxIter, relTol, absTol, modelEta, s);
+            ts.launches = it - first;
             if (ctx->profile) check(hipEventRecord(ts.stop, s), "hipEventRecord");
             readback();
             const bool done = hf[pbk::kIflagDone0] || hf[pbk::kIflagDone1] || it > maxIter;
@@ -2955,6 +3014,7 @@ void pb_default_options(pb_solve_options* o)
     o->previous_intrinsic_mask = 0;
     o->batch_graphs = 1;
     o->batch_factor_shape = -1;
+    o->pcg_graphs = 1;
     o->refine_camera_pose = 1;
     o->shared_intrinsic_mask = 0;
     o->min_focal_scale = 0.5;
@@ -3036,6 +3096,8 @@ pb_status pb_hip_destroy(pb_ctx* ctx)
     ctx->dbatch.release();
     ctx->dquality.release();
     for (hipEvent_t e : ctx->eventPool) (void)hipEventDestroy(e);
+    for (auto& g : ctx->pcgGraph)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

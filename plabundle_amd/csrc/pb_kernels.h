@@ -50,6 +50,7 @@ namespace pbk
         kIflagIters = 2,
         kIflagBreakdown = 3,
         kIflagModelStop = 4,  // PCG ended by the quadratic-model test
+        kIflagItBase = 5,     // iteration counter of graph-replayed PCG launches (even at every replay)
         kIflagCount = 8
     };
 
@@ -330,8 +331,10 @@ namespace pbk
         static void cameraDirect(const Problem& P, const Work<T>& W, hipStream_t s);
         static void schurBlocks(const Problem& P, const Work<T>& W, double damping, hipStream_t s);
         static void pcgInit(const Problem& P, const Work<T>& W, double relTol, double absTol, hipStream_t s);
+        // counted: the absolute iteration is it + the device counter (launches captured into a graph)
         static void pcgIteration(const Problem& P, const Work<T>& W, int it, int maxIt, double relTol, double absTol,
-                                 double modelEta, hipStream_t s);
+                                 double modelEta, hipStream_t s, bool counted = false);
+        static void pcgAdvance(const Problem& P, const Work<T>& W, int launches, hipStream_t s);
         static void backSubstitute(const Problem& P, const Work<T>& W, hipStream_t s);
         static void pointSteps(const Problem& P, const Work<T>& W, hipStream_t s);  // back-substitution only
         static void trialStates(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, hipStream_t s);

@@ -766,15 +766,22 @@ namespace pbk
         return acc;
     }
 
+    //
+    // Iteration number: itRel, plus *itBase when itBase is given (launches replayed from a captured graph take
+    // their absolute iteration from a device counter that k_pcg_advance moves on; the counter is even at every
+    // replay and the graph holds an even number of launches, so itRel fixes the parity). The counter is loaded
+    // with the other first-round-trip loads, and what depends on the iteration number comes after them.
     template <typename T>
-    __global__ void __launch_bounds__(64 * kCgWaves) k_pcg_iter(Problem P, Work<T> Wk, int it, int maxIt, int nParts,
-                                                                double relTol, double absTol, double modelEta)
+    __global__ void __launch_bounds__(64 * kCgWaves) k_pcg_iter(Problem P, Work<T> Wk, int itRel, const int32_t* itBase,
+                                                                int maxIt, int nParts, double relTol, double absTol,
+                                                                double modelEta)
     {
         __shared__ double red3[kCgWaves][3];
         __shared__ T red[kCgWaves][kWave];
         __shared__ T segq[kCgWaves][6];
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int rd = it & 1, wr = rd ^ 1;  // prime (it = -1) reads parity 1 and writes parity 0
+        const int rd = itRel & 1, wr = rd ^ 1;  // prime (it = -1) reads parity 1 and writes parity 0
+        const bool started = itBase != nullptr || itRel >= 0;  // not the priming launch (launch-uniform)
         const int i6 = lane % 6, bslot = lane / 6, base = 6 * bslot;
         constexpr int kCgPrefetch = cgPrefetch<T>();
 
@@ -785,6 +792,7 @@ namespace pbk
         // compiler issues them together instead of waiting at branch joins.
         const int slot = blockIdx.x * kCgWaves + wv;
         const int prevDone = Wk.iflags[kIflagDone0 + rd];
+        const int it = itRel + (itBase ? *itBase : 0);
         const int4 si = P.cgSlot[slot];
         int col[kCgPrefetch];
 #pragma unroll
@@ -800,7 +808,7 @@ namespace pbk
             pd[k] = part[3 * i + 1];
             pq[k] = part[3 * i + 2];
         }
-        if (it >= 0)
+        if (started)
         {
             thr2 = Wk.scalars[kScalarThreshold2];
             qIt = Wk.scalars[kScalarQ0 + rd];
@@ -846,7 +854,7 @@ namespace pbk
         for (int st = kCgAlways; st < kCgPrefetch; ++st)
             if (st < nSteps) loadStep(st);
         double g = 0.0, d = 0.0, q = 0.0;
-        if (it >= 0)
+        if (started)
         {
 #pragma unroll
             for (int k = 0; k < kCgPartLoads; ++k)
@@ -870,7 +878,7 @@ namespace pbk
             return;
         }
         double alpha = 0.0, beta = 0.0;
-        if (it >= 0)
+        if (started)
         {
             g = waveSum(g);
             d = waveSum(d);
@@ -1061,7 +1069,15 @@ namespace pbk
             Wk.iflags[kIflagIters] = 0;
             Wk.iflags[kIflagBreakdown] = 0;
             Wk.iflags[kIflagModelStop] = 0;
+            Wk.iflags[kIflagItBase] = 0;
         }
+    }
+
+    // Moves the iteration counter of graph-replayed PCG launches on by the launches of one replay.
+    template <typename T>
+    __global__ void k_pcg_advance(Work<T> Wk, int launches)
+    {
+        if (threadIdx.x == 0) Wk.iflags[kIflagItBase] += launches;
     }
 
     // ---------------------------------------------------------------------------------------------
@@ -1586,16 +1602,24 @@ namespace pbk
     {
         if (P.cgBlocks == 0) return;
         k_pcg_init<T><<<std::max(1, blocksFor(long(P.nUnits) * 6, 256)), 256, 0, s>>>(P, W);
-        k_pcg_iter<T><<<std::max(1, P.cgBlocks), 64 * kCgWaves, 0, s>>>(P, W, -1, 0, P.nUnits, relTol, absTol, 0.0);
+        k_pcg_iter<T><<<std::max(1, P.cgBlocks), 64 * kCgWaves, 0, s>>>(P, W, -1, nullptr, 0, P.nUnits, relTol, absTol,
+                                                                        0.0);
     }
 
     template <typename T>
     void Launcher<T>::pcgIteration(const Problem& P, const Work<T>& W, int it, int maxIt, double relTol, double absTol,
-                                   double modelEta, hipStream_t s)
+                                   double modelEta, hipStream_t s, bool counted)
     {
         if (P.cgBlocks == 0) return;
-        k_pcg_iter<T><<<std::max(1, P.cgBlocks), 64 * kCgWaves, 0, s>>>(P, W, it, maxIt, P.nUnits, relTol, absTol,
-                                                                        modelEta);
+        k_pcg_iter<T><<<std::max(1, P.cgBlocks), 64 * kCgWaves, 0, s>>>(
+            P, W, it, counted ? W.iflags + kIflagItBase : nullptr, maxIt, P.nUnits, relTol, absTol, modelEta);
+    }
+
+    template <typename T>
+    void Launcher<T>::pcgAdvance(const Problem& P, const Work<T>& W, int launches, hipStream_t s)
+    {
+        if (P.cgBlocks == 0) return;
+        k_pcg_advance<T><<<1, 64, 0, s>>>(W, launches);
     }
 
     template <typename T>

@@ -138,6 +138,7 @@ class BackendOptions:
     stepRmsTolerance: float = 0.0  # 0: the reference's 1e-6
     batchGraphs: bool = True  # batched windows: replay each round's launches as captured HIP graphs
     batchFactorShape: int = -1  # batched windows: dense factorisation launch shape (-1 = timed on the first round)
+    pcgGraphs: bool = True  # replay PCG iterations from a captured HIP graph
 
 
 @dataclass
@@ -234,6 +235,7 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.step_rms_tolerance = float(b.stepRmsTolerance)
     o.batch_graphs = 1 if b.batchGraphs else 0
     o.batch_factor_shape = int(b.batchFactorShape)
+    o.pcg_graphs = 1 if b.pcgGraphs else 0
     k = options.constraints
     o.laser_plane_weight = float(k.laserPlaneWeight)
     o.laser_huber_delta = float(k.laserHuberDeltaMeters)

@@ -167,3 +167,33 @@ def test_invalid_problem_rejected(ws):
     p2.fixed_cameras = np.array([0], dtype=np.int32)
     r = Solver().solve(p2, o, ws, precision=PO.Precision.Fp64)
     assert r.status == SolveStatus.UnsupportedConfiguration
+
+
+@pytest.mark.parametrize("precision", [PO.Precision.Fp64, PO.Precision.Mixed])
+def test_pcg_graph_replay_matches_direct_launches(ws, precision):
+    # the PCG iterations replayed from the captured graph (absolute iteration from the device counter) must give
+    # bitwise the solve of one-by-one launches, also on a second solve (graph reused) and after the stop
+    # parameters change (graph recaptured)
+    _say(f"case PCG graph replay vs direct launches precision={precision.name}")
+    p, o = _aerial(n=(12, 12, 12), pts=2000)
+    o.backend.pcgGraphs = False
+    direct = Solver().solve(p, o, ws, precision=precision)
+    assert direct.usable(), direct.backendMessage
+    o.backend.pcgGraphs = True
+    for rep in range(2):
+        r = Solver().solve(p, o, ws, precision=precision)
+        assert r.report["iterations"] == direct.report["iterations"]
+        assert r.report["linear_iterations"] == direct.report["linear_iterations"]
+        assert r.finalCost == direct.finalCost
+        np.testing.assert_array_equal(r.cam_center, direct.cam_center)
+        np.testing.assert_array_equal(r.cam_intrinsics, direct.cam_intrinsics)
+        np.testing.assert_array_equal(r.track_point, direct.track_point)
+    o.backend.pcgMaxIterations = 37  # odd cap: the stop falls inside a replay
+    o.backend.pcgGraphs = False
+    a = Solver().solve(p, o, ws, precision=precision)
+    o.backend.pcgGraphs = True
+    b = Solver().solve(p, o, ws, precision=precision)
+    _say(f"  capped: {a.report['linear_iterations']} PCG iterations, cost {a.finalCost:.12g}")
+    assert a.report["linear_iterations"] == b.report["linear_iterations"]
+    assert a.finalCost == b.finalCost
+    np.testing.assert_array_equal(a.track_point, b.track_point)
