@@ -11,8 +11,9 @@ Reported: the mixed-precision path (FP32 Jacobians/Schur/PCG, FP64 cost and
 acceptance) as `value` (LM iter/s) and `ms_per_step` (time-to-converge), the plain
 FP64 GPU path beside it, the relative final-cost error against the FP64 CPU oracle
 (golden result in tests/golden/aerial_oracle.json), the HBM roofline of the dominant
-kernel (HIP events inside the library, on its stream) and the CPU oracle timed on a
-bounded sample of the same workload.
+kernel (HIP events inside the library, on its stream), the CPU oracle timed on the
+same workload on all host cores (and a one-thread sample), and the differences of the
+GPU refined states from the oracle's (the metrics of comparePhotoAlignmentOutcomes).
 """
 import argparse
 import hashlib
@@ -236,7 +237,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=1)
-    ap.add_argument("--cpu-iterations", type=int, default=2, help="LM iterations of the CPU oracle sample")
+    ap.add_argument("--cpu-iterations", type=int, default=20,
+                    help="LM iterations of the CPU oracle run on all host cores (20: the whole workload, whose state "
+                         "the GPU states are compared with)")
+    ap.add_argument("--cpu-single-iterations", type=int, default=1,
+                    help="LM iterations of the one-thread CPU oracle sample (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="reduced problem for quick checks (not a bench line)")
     ap.add_argument("--mixed-rtol", type=float, default=0.0, help="PCG relative tolerance of the mixed path (0 = default)")
@@ -428,9 +433,14 @@ def main():
             log(f"batch leg failed: {batch['error']}")
 
     cpu = None
+    state_vs_oracle = None
     if not args.no_cpu_baseline:
+        import numpy as np
+
         from oracle import oracle as O
 
+        # the FP64 oracle on all host cores over the full workload (20 LM iterations): the CPU baseline, and the
+        # refined state the GPU states are compared with; then a one-thread sample of its first iterations
         o = PO.to_c(S.aerial_options(PO.SolveOptions()))
         o.max_iterations = args.cpu_iterations
         threads = O.lib().oracle_max_threads()
@@ -440,9 +450,51 @@ def main():
         wall = time.perf_counter() - t0
         cpu = {"value": round(rep["iterations"] / rep["solve_seconds"], 4), "unit": "LM iter/s", "cores": threads,
                "kind": "port",
-               "sample": f"FP64 oracle, first {rep['iterations']} LM iterations of the same workload "
+               "sample": f"FP64 oracle, {rep['iterations']} LM iterations of the same workload "
                          f"({rep['solve_seconds']:.2f}s solve, {wall:.2f}s with setup)"}
+        if args.cpu_single_iterations > 0:
+            o1 = PO.to_c(S.aerial_options(PO.SolveOptions()))
+            o1.max_iterations = args.cpu_single_iterations
+            rep1 = O.Oracle(problem, threads=1).solve(o1)
+            cpu["single_thread"] = {"value": round(rep1["iterations"] / rep1["solve_seconds"], 4), "cores": 1,
+                                    "sample": f"first {rep1['iterations']} LM iterations "
+                                              f"({rep1['solve_seconds']:.2f}s solve)"}
         log(f"cpu baseline: {cpu}")
+        if rep["iterations"] == mixed[0].report["iterations"]:
+            # metrics of comparePhotoAlignmentOutcomes (src/photo_alignment.cpp:225-318): largest centre distance,
+            # largest rotation difference (angle of R_ref^T R), largest point distance over tracks valid in both,
+            # tracks whose valid flag differs
+            ost = orc.state()
+
+            def compare(r, st=ost):
+                tr = np.einsum("ij,ij->i", st["cam_rotation"], r.cam_rotation)
+                rot = np.degrees(np.arccos(np.clip(0.5 * (tr - 1.0), -1.0, 1.0)))
+                va, vb = st["track_valid"] != 0, r.track_valid != 0
+                both = va & vb
+                pd = np.linalg.norm(st["track_point"][both] - r.track_point[both], axis=1)
+                f0 = st["cam_intrinsics"][:, 0]
+                return {"max_camera_center_difference_m": float(np.linalg.norm(st["cam_center"] - r.cam_center,
+                                                                               axis=1).max()),
+                        "mean_camera_height_difference_m": float(np.mean(r.cam_center[:, 2] - st["cam_center"][:, 2])),
+                        "max_focal_relative_difference": float(np.max(np.abs(r.cam_intrinsics[:, 0] - f0) /
+                                                                      np.abs(f0))),
+                        "max_camera_rotation_difference_deg": float(rot.max()),
+                        "max_point_difference_m": float(pd.max()) if pd.size else 0.0,
+                        "valid_track_mask_mismatches": int(np.count_nonzero(va != vb))}
+
+            state_vs_oracle = {"oracle_final_cost": rep["final_cost"], "mixed": compare(mixed[0])}
+            if fp64:
+                state_vs_oracle["fp64"] = compare(fp64[0])
+            if fp64i:
+                state_vs_oracle["fp64_same_stop_as_mixed"] = compare(fp64i[0])
+                # the precision effect alone: mixed against FP64 with the same stopping rule
+                ref = fp64i[0]
+                state_vs_oracle["mixed_vs_fp64_same_stop"] = compare(
+                    mixed[0], {"cam_rotation": ref.cam_rotation, "cam_center": ref.cam_center,
+                               "cam_intrinsics": ref.cam_intrinsics, "track_point": ref.track_point,
+                               "track_valid": ref.track_valid})
+            log(f"state vs oracle: {state_vs_oracle}")
+        orc.close()
 
     out = {
         "metric": METRIC,
@@ -490,6 +542,7 @@ def main():
         "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "state_vs_oracle": state_vs_oracle,
         "device": mixed[0].report["device_name"],
     }
     print(json.dumps(out), flush=True)
