@@ -147,6 +147,46 @@ def test_aerial_fp64_matches_oracle(ws, oracle_mod):
     assert rel <= 1e-9
 
 
+def _state_differences(a, b):
+    """Largest centre distance (m), rotation angle (deg) and point distance (m) over tracks valid in both (the
+    metrics of comparePhotoAlignmentOutcomes, src/photo_alignment.cpp:225-318), and valid-flag mismatches."""
+    tr = np.einsum("ij,ij->i", a["cam_rotation"], b["cam_rotation"])
+    rot = np.degrees(np.arccos(np.clip(0.5 * (tr - 1.0), -1.0, 1.0)))
+    va, vb = a["track_valid"] != 0, b["track_valid"] != 0
+    both = va & vb
+    return (float(np.linalg.norm(a["cam_center"] - b["cam_center"], axis=1).max()), float(rot.max()),
+            float(np.linalg.norm(a["track_point"][both] - b["track_point"][both], axis=1).max()),
+            int(np.count_nonzero(va != vb)))
+
+
+def _state(r):
+    return {"cam_rotation": r.cam_rotation, "cam_center": r.cam_center, "track_point": r.track_point,
+            "track_valid": r.track_valid}
+
+
+def test_aerial_states_within_stated_tolerances(ws, oracle_mod):
+    # FP64 (reference stop) against the oracle's refined state: 1e-6 m / 1e-5 deg; mixed against an FP64 solve with
+    # the same truncated-Newton stop (the precision effect alone): 0.1 m, same valid tracks
+    _say("case refined states aerial-small")
+    p, o = _aerial()
+    orc = oracle_mod.Oracle(p, threads=4)
+    orc.solve(PO.to_c(o))
+    ost = orc.state()
+    r64 = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
+    c, rdeg, pt, mm = _state_differences(ost, _state(r64))
+    _say(f"  fp64 vs oracle: centre {c:.2e} m, rotation {rdeg:.2e} deg, point {pt:.2e} m, mismatches {mm}")
+    assert c <= 1e-6 and rdeg <= 1e-5 and pt <= 1e-5 and mm == 0
+    rm = Solver().solve(p, o, ws, precision=PO.Precision.Mixed)
+    oi = S.aerial_options(PO.SolveOptions())
+    oi.backend.pcgModelEta = 0.1
+    oi.backend.pcgRelativeTolerance = 1e-3
+    oi.backend.pcgMaxIterations = max(200, 3 * rm.report["primary_block_count"] // 5)
+    ri = Solver().solve(p, oi, ws, precision=PO.Precision.Fp64)
+    c, rdeg, pt, mm = _state_differences(_state(ri), _state(rm))
+    _say(f"  mixed vs fp64 same stop: centre {c:.2e} m, rotation {rdeg:.2e} deg, point {pt:.2e} m, mismatches {mm}")
+    assert c <= 0.1 and pt <= 0.1 and mm == 0
+
+
 def test_cancel_publishes_nothing(ws):
     _say("case cancel")
     p, o = _ring()
