@@ -96,3 +96,21 @@ def test_invalid_calls_fail_cleanly_without_a_device(lib):
     ctx = C.c_void_p()
     if not lib.pb_hip_available(0):
         assert lib.pb_hip_create(0, C.byref(ctx)) == _abi.PB_ERR_DEVICE
+
+
+def test_batch_factor_shapes_and_tuning_queries(lib):
+    # the launch shapes of the dense window factorisation: every shape covers the largest window system
+    # (threads x rows per thread = 512) with an even panel width no wider than a wave; the tuning query needs a
+    # context and a precision index 0 or 1
+    t, p = C.c_int32(), C.c_int32()
+    shapes = []
+    for v in range(_abi.PB_BATCH_FACTOR_SHAPES):
+        assert lib.pb_hip_batch_factor_shape(v, C.byref(t), C.byref(p)) == _abi.PB_OK
+        assert t.value in (128, 256, 512) and 512 % t.value == 0
+        assert p.value % 2 == 0 and 2 <= p.value <= 64
+        shapes.append((t.value, p.value))
+    assert shapes[0] == (256, 16) and len(set(shapes)) == len(shapes)
+    assert lib.pb_hip_batch_factor_shape(_abi.PB_BATCH_FACTOR_SHAPES, C.byref(t), C.byref(p)) == _abi.PB_ERR_INVALID
+    assert lib.pb_hip_batch_factor_shape(-1, None, None) == _abi.PB_ERR_INVALID
+    shape = C.c_int32()
+    assert lib.pb_hip_batch_tuning(None, 1, C.byref(shape), None) == _abi.PB_ERR_INVALID
