@@ -189,14 +189,17 @@ namespace pbk
             const int eg = P.obsGroupEntry[o];
             if (eg >= 0)
             {
+                // predicated rather than cut short, so the loop unrolls and Ji stays in registers
 #pragma unroll
                 for (int k = 0; k < KM; ++k)
                 {
-                    if (k >= K) break;
-                    T* Wg = Wk.W + (eg + k / 6) * 18 + (k % 6) * 3;
+                    if (k < K)
+                    {
+                        T* Wg = Wk.W + (eg + k / 6) * 18 + (k % 6) * 3;
 #pragma unroll
-                    for (int i = 0; i < 3; ++i)
-                        Wg[i] += T(w * (L.Ji[k] * L.Jp[i] + L.Ji[kMaxActive + k] * L.Jp[3 + i]));
+                        for (int i = 0; i < 3; ++i)
+                            Wg[i] += T(w * (L.Ji[k] * L.Jp[i] + L.Ji[kMaxActive + k] * L.Jp[3 + i]));
+                    }
                 }
             }
         }
