@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define PB_ABI_VERSION 2
+#define PB_ABI_VERSION 3
 #define PB_INTRINSIC_COUNT 13
 
 typedef enum pb_status {
@@ -93,9 +93,22 @@ typedef enum pb_brown_convention {
 } pb_brown_convention;
 
 typedef enum pb_precision {
-    PB_PRECISION_FP64 = 0,  /* every stage in FP64 */
-    PB_PRECISION_MIXED = 1  /* FP32 Jacobians/Schur/PCG; FP64 cost, acceptance, point inverses */
+    PB_PRECISION_FP64 = 0,      /* every stage in FP64 */
+    PB_PRECISION_MIXED = 1,     /* FP32 Jacobians/Schur and FP32 reduced solve (dense: FP32 factorisation refined in
+                                   FP64 against the FP32 system; PCG: FP32 PCG); FP64 cost, acceptance, point
+                                   inverses; a step that fails its line search is repeated in FP64 */
+    PB_PRECISION_FP32_SEED = 2  /* enablePlaMatrixMixedPrecision (options.h:272-273): FP64 system, guarded FP32 seed
+                                   of the reduced solve (dense route: FP32 factorisation refined in FP64 to the
+                                   FP64 tolerance, else the step is solved with the FP64 factorisation) */
 } pb_precision;
+
+/* Reduced-system solver of each LM step (no reference option: the reference picks PlaMatrix's dense CPU or
+ * PCG GPU route by backend, BundleAdjustPlaMatrixRuntime.cpp:18-56). */
+typedef enum pb_reduced_solver {
+    PB_REDUCED_AUTO = 0,   /* dense when 6 * units <= dense_max_rows, else PCG */
+    PB_REDUCED_PCG = 1,    /* block-Jacobi preconditioned CG (fused iteration) */
+    PB_REDUCED_DENSE = 2   /* dense Cholesky of the reduced system (64 x 64 tiles, MFMA trailing updates) */
+} pb_reduced_solver;
 
 /* Uncertainty form of a control point (plabundle::ControlPointConstraint::Uncertainty,
  * include/plabundle/constraints.h:81-87). */
@@ -229,6 +242,13 @@ typedef struct pb_solve_options {
      * HIP graph of 16 launches (captured once per problem and stop parameters), 0 launches them one by one. Both
      * give bitwise the same results. */
     int32_t pcg_graphs;
+    /* pb_reduced_solver (default PB_REDUCED_AUTO). */
+    int32_t reduced_solver;
+    /* Largest reduced system (rows = 6 * units) solved densely under PB_REDUCED_AUTO (0 = 16384). */
+    int32_t dense_max_rows;
+    /* plaMatrixPreconditionerClusterSize (options.h:164,285; valid 1..16, src/options.cpp:393-394): consecutive
+     * primary units whose diagonal blocks form one block of the PCG preconditioner (1 = block-Jacobi). */
+    int32_t preconditioner_cluster_size;
 } pb_solve_options;
 
 /* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */
@@ -265,7 +285,33 @@ typedef struct pb_solve_report {
     double linear_tolerance;
     char device_name[64];
     char message[256];
+    /* further PlaMatrixDiagnostics fields (result.h:91-128) */
+    char linear_solver_name[32];         /* "dense-cholesky-fp64", "dense-cholesky-fp32-refined", "pcg" */
+    char preconditioner_name[32];        /* "block-jacobi-6", "cluster-jacobi", "none" */
+    int32_t dense_fallbacks;             /* FP32 factorisations rejected and solved again in FP64 */
+    int32_t mixed_seed_steps;            /* reduced solves whose FP32 seed was accepted */
+    int32_t schur_pattern_builds;        /* reduced-system patterns built for this solve (0 when reused) */
+    int32_t schur_pattern_reuses;
+    int32_t schur_assembly_on_device;    /* 1 */
+    double linear_tolerance_minimum;     /* smallest / largest linear stopping tolerance used */
+    double linear_tolerance_maximum;
+    double trial_state_seconds;          /* share of objective_seconds spent forming trial states */
+    double cholesky_factorization_seconds;
+    double triangular_solve_seconds;     /* dense solves incl. refinement */
+    double residual_check_seconds;
 } pb_solve_report;
+
+/* plabundle::IterationSummary (options.h:50-56), passed to the progress callback after every trial step (the
+ * calls of BundleAdjustPlaMatrix.cpp:455-464,544-554). */
+typedef struct pb_iteration_summary {
+    int32_t current_iteration;
+    int32_t max_iterations;
+    double average_rms_pixels;           /* sqrt(max(0, cost) / max(1, observation count)) */
+    int32_t valid_point_count;           /* active tracks */
+} pb_iteration_summary;
+
+/* Returns non-zero to continue, 0 to cancel (SolverControlOptions::progressCallback, options.h:72-74). */
+typedef int32_t (*pb_progress_fn)(const pb_iteration_summary* summary, void* user);
 
 typedef struct pb_ctx pb_ctx;
 
@@ -287,6 +333,10 @@ pb_status pb_hip_upload(pb_ctx* ctx, const pb_problem_view* problem);
  * the solve stops and publishes nothing (state stays at the input). */
 pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_report* report,
                        const volatile int32_t* cancel);
+
+/* Installs the progress callback of later solves on this context (NULL removes it). It is called on the
+ * calling thread of pb_hip_solve after every trial step; returning 0 cancels the solve like the cancel flag. */
+pb_status pb_hip_set_progress(pb_ctx* ctx, pb_progress_fn fn, void* user);
 
 /* Refined state of the last solve. Any pointer may be NULL.
  *   cam_rotation [num_cameras*9], cam_center [num_cameras*3],
@@ -501,6 +551,14 @@ pb_status pb_compare_outcomes(const pb_outcome_view* reference, const pb_outcome
                               const pb_comparison_tolerance* tolerance, pb_comparison* comparison,
                               double* camera_center_difference, double* camera_rotation_difference_degrees,
                               double* point_difference, int32_t* valid_mismatch);
+
+/* Test and measurement hook of the dense reduced solver (no reference counterpart): factors the symmetric
+ * positive-definite n x n matrix A (column-major; the lower triangle is read) on the context's device in FP64
+ * (fp32 = 0) or FP32 (fp32 = 1), the way a reduced system is factored per LM step, and solves A x = b (x in FP64;
+ * no refinement). info (may be NULL) receives 0, or 1 when a pivot was not positive. seconds (may be NULL)
+ * receives the device time of the factorisation [0] and of the two triangular sweeps [1]. */
+pb_status pb_hip_dense_solve(pb_ctx* ctx, int32_t n, int32_t fp32, const double* A, const double* b, double* x,
+                             int32_t* info, double* seconds);
 
 /* Last error message of the context (never NULL). */
 const char* pb_hip_last_error(const pb_ctx* ctx);

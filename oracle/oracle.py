@@ -65,6 +65,10 @@ def lib():
     L.oracle_point_scale.argtypes = [_dp]
     L.oracle_point_scale.restype = C.c_double
     L.oracle_retract.argtypes = [_dp, _dp, _dp, C.c_int]
+    L.oracle_gauge_basis.argtypes = [_dp, _dp, _dp]
+    L.oracle_gauge_basis.restype = None
+    L.oracle_gauge_step.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
+    L.oracle_gauge_step.restype = None
     L.oracle_linearize.argtypes = [_dp, _dp, _dp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                    C.c_int, C.c_double, C.c_int, C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp]
     L.oracle_linearize.restype = C.c_int
@@ -95,7 +99,7 @@ def default_options() -> _abi.SolveOptionsC:
     o.robust_loss_scale = 1.0
     o.armijo_coefficient = 1e-3
     o.line_search_steps = 34
-    o.use_online_schur = 1
+    o.use_online_schur = 0
     o.max_initial_track_rms = 100.0
     o.refine_camera_pose = 1
     o.shared_intrinsic_mask = 0
@@ -121,6 +125,9 @@ def default_options() -> _abi.SolveOptionsC:
     o.batch_graphs = 1
     o.batch_factor_shape = -1
     o.pcg_graphs = 1
+    o.reduced_solver = 0
+    o.dense_max_rows = 0
+    o.preconditioner_cluster_size = 1
     return o
 
 
@@ -295,3 +302,21 @@ def control_whitening(kind, sigma=1.0, matrix=None):
     A = np.zeros(9)
     ok = lib().oracle_control_whitening(int(kind), float(sigma), _p(m), _p(A))
     return A.reshape(3, 3) if ok else None
+
+
+def gauge_basis(anchor, scale):
+    """World tangent basis (2 x 3) of the gauge scale camera at `scale` around the anchor at `anchor`."""
+    a = np.ascontiguousarray(anchor, dtype=np.float64)
+    c = np.ascontiguousarray(scale, dtype=np.float64)
+    out = np.zeros(6)
+    lib().oracle_gauge_basis(_p(a), _p(c), _p(out))
+    return out.reshape(2, 3)
+
+
+def gauge_step(anchor, scale, baseline, first, second):
+    """World centre of the gauge scale camera after the tangent step (first, second)."""
+    a = np.ascontiguousarray(anchor, dtype=np.float64)
+    c = np.ascontiguousarray(scale, dtype=np.float64)
+    out = np.zeros(3)
+    lib().oracle_gauge_step(_p(a), _p(c), float(baseline), float(first), float(second), _p(out))
+    return out

@@ -2701,6 +2701,44 @@ int oracle_robust_loss(double s, int loss, double scale, double* cost, double* w
 
 double oracle_point_scale(const double* X) { return pointScale({X[0], X[1], X[2]}); }
 
+// Reference gauge tangent basis of a scale camera at world centre `scale` around the anchor at `anchor`
+// (referenceGaugeTargetTangentBasis, Assembly.cpp:69-99), written as world vectors: basis[0..2] the first,
+// basis[3..5] the second (the scale camera's Jacobian columns 3 and 4 are Jp times these, Assembly.cpp:295-306).
+void oracle_gauge_basis(const double* anchor, const double* scale, double* basis)
+{
+    Options opt;
+    opt.gaugeAnchor = 0;
+    opt.gaugeScale = 1;
+    opt.gaugeBaseline = 1.0;
+    State st;
+    st.cams.resize(2);
+    st.cams[0].C = {anchor[0], anchor[1], anchor[2]};
+    st.cams[1].C = {scale[0], scale[1], scale[2]};
+    const auto b = gaugeBasis(opt, st);
+    for (int k = 0; k < 2; ++k)
+    {
+        const Vec3 w = targetCoords(b[k]);
+        for (int i = 0; i < 3; ++i) basis[3 * k + i] = w[i];
+    }
+}
+
+// New world centre of the scale camera after the tangent step (first, second) at the given baseline
+// (applyReferenceGaugeTangentStep, Assembly.cpp:101-128).
+void oracle_gauge_step(const double* anchor, const double* scale, double baseline, double first, double second,
+                       double* center)
+{
+    Options opt;
+    opt.gaugeAnchor = 0;
+    opt.gaugeScale = 1;
+    opt.gaugeBaseline = baseline;
+    State st;
+    st.cams.resize(2);
+    st.cams[0].C = {anchor[0], anchor[1], anchor[2]};
+    st.cams[1].C = {scale[0], scale[1], scale[2]};
+    const Vec3 c = gaugeTangentStep(opt, st, first, second);
+    for (int i = 0; i < 3; ++i) center[i] = c[i];
+}
+
 void oracle_retract(double* R, double* C, const double* delta, int reference_param)
 {
     Mat3 r;

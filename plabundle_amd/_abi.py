@@ -118,6 +118,9 @@ class SolveOptionsC(C.Structure):
         ("batch_graphs", C.c_int32),
         ("batch_factor_shape", C.c_int32),
         ("pcg_graphs", C.c_int32),
+        ("reduced_solver", C.c_int32),
+        ("dense_max_rows", C.c_int32),
+        ("preconditioner_cluster_size", C.c_int32),
     ]
 
 
@@ -155,6 +158,19 @@ class SolveReportC(C.Structure):
         ("linear_tolerance", C.c_double),
         ("device_name", C.c_char * 64),
         ("message", C.c_char * 256),
+        ("linear_solver_name", C.c_char * 32),
+        ("preconditioner_name", C.c_char * 32),
+        ("dense_fallbacks", C.c_int32),
+        ("mixed_seed_steps", C.c_int32),
+        ("schur_pattern_builds", C.c_int32),
+        ("schur_pattern_reuses", C.c_int32),
+        ("schur_assembly_on_device", C.c_int32),
+        ("linear_tolerance_minimum", C.c_double),
+        ("linear_tolerance_maximum", C.c_double),
+        ("trial_state_seconds", C.c_double),
+        ("cholesky_factorization_seconds", C.c_double),
+        ("triangular_solve_seconds", C.c_double),
+        ("residual_check_seconds", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -165,6 +181,18 @@ class SolveReportC(C.Structure):
                 value = value.decode("utf-8", "replace")
             out[name] = value
         return out
+
+
+class IterationSummaryC(C.Structure):
+    _fields_ = [
+        ("current_iteration", C.c_int32),
+        ("max_iterations", C.c_int32),
+        ("average_rms_pixels", C.c_double),
+        ("valid_point_count", C.c_int32),
+    ]
+
+
+PROGRESS_FN = C.CFUNCTYPE(C.c_int32, C.POINTER(IterationSummaryC), C.c_void_p)
 
 
 class BatchView(C.Structure):
@@ -321,6 +349,8 @@ EXPORTED_SYMBOLS = (
     "pb_default_comparison_tolerance",
     "pb_validate_outcome",
     "pb_compare_outcomes",
+    "pb_hip_set_progress",
+    "pb_hip_dense_solve",
     "pb_hip_last_error",
 )
 
@@ -387,6 +417,10 @@ def load_library(path: str | None = None):
     lib.pb_compare_outcomes.argtypes = [C.POINTER(OutcomeView), C.POINTER(OutcomeView),
                                         C.POINTER(ComparisonToleranceC), C.POINTER(ComparisonC), _dp, _dp, _dp, _ip]
     lib.pb_compare_outcomes.restype = C.c_int
+    lib.pb_hip_set_progress.argtypes = [C.c_void_p, PROGRESS_FN, C.c_void_p]
+    lib.pb_hip_set_progress.restype = C.c_int
+    lib.pb_hip_dense_solve.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp, _ip, _dp]
+    lib.pb_hip_dense_solve.restype = C.c_int
     lib.pb_hip_last_error.argtypes = [C.c_void_p]
     lib.pb_hip_last_error.restype = C.c_char_p
     if path is None:

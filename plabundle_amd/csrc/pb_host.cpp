@@ -13,8 +13,6 @@
 #include "pb_kernels.h"
 
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <array>
@@ -921,7 +919,17 @@ namespace
     // per chunk, so the events do not stretch the solve); chunks that contain the stop are not counted, so
     // every timed launch did a full iteration
     constexpr int kFamilyCount = sizeof(kFamilies) / sizeof(kFamilies[0]);
-    constexpr long kDenseAdjointMax = 24576;  // rows of the dense adjoint factorisation (4.8 GB of FP64)
+    constexpr int kPinDense = 80;       // dense solver words (8 int32) and two norms
+    static_assert(kPinPcgFlags + pbk::kIflagCount / 2 <= kPinDense, "pinned layout");
+    // rows up to which the adjoint factors its reduced system densely (34 GB of FP64; FP64 PCG above that)
+    constexpr long kDenseAdjointMax = 65536;
+    // default largest reduced system solved densely per LM step (reduced_solver = auto): above it the
+    // factorisation (n^3/3 flops per step) costs more than block-Jacobi PCG on the survey-sized blocks
+    constexpr int kDenseSolveMaxRows = 16384;
+    constexpr int kDenseSeedRefinements = 10;  // FP64 refinement steps of an FP32 factorisation before it is rejected
+    // consecutive rejected FP32 seeds after which a solve factors in FP64 only (the system's conditioning is
+    // beyond FP32 and changes slowly between LM steps)
+    constexpr int kSeedRejectionsToStop = 2;
     // a mixed batch moves a window to FP64 assembly once an FP32 step lowers its objective by less than this
     // (relative): FP32 steps carry the early, large decreases; near the minimum the FP32 rounding of the
     // reduced system limits them, so the last iterations of every window assemble in FP64
@@ -986,6 +994,9 @@ struct pb_ctx
         std::vector<unsigned char> key;
     } pcgGraph[2];
     bool pcgGraphs = true;  // options.pcg_graphs of the current solve
+    pb_progress_fn progressFn = nullptr;  // SolverControlOptions::progressCallback (pb_hip_set_progress)
+    void* progressUser = nullptr;
+    int64_t patternBuilds = 0;  // reduced-system patterns built on this context
     bool work32 = false, work64 = false;
     pbk::Problem P;
     pbk::Work<float> W32;
@@ -996,7 +1007,19 @@ struct pb_ctx
     std::vector<double> outR, outC, outGroupParams, outPointX;
     bool haveResult = false;
     pb_solve_options lastOptions{};  // options of the last solve (the adjoint linearises with them)
-    rocblas_handle blas = nullptr;   // dense factorisation of the adjoint system (created on first use)
+    // Dense reduced solve: matrices of order 64 * ntCap per precision (allocated on first use and when a larger
+    // system comes), refinement vectors and the solver's state words and tile flags.
+    struct Dense
+    {
+        DeviceAlloc vec, mat32, mat64;
+        int ntCap = 0;
+        float* A32 = nullptr;
+        double* A64 = nullptr;
+        float* v32 = nullptr;
+        double* v64 = nullptr;
+        double *x = nullptr, *r = nullptr, *rowNorm = nullptr, *norm = nullptr;
+        pbk::DenseState st;
+    } dense;
     // profiling
     bool profile = false;
     int64_t pcgIterationsProfiled = 0;
@@ -1248,7 +1271,12 @@ namespace
         if (o->shared_intrinsic_mask & ~0x1FFFu) return "shared_intrinsic_mask has bits beyond slot 12";
         if ((o->shared_intrinsic_mask & 0x1FFEu) && !(o->shared_intrinsic_mask & 1u) && o->shared_intrinsic_mask != (1u << 4))
             return "refining aspect, principal point or distortion requires refining the shared focal length";
-        if (o->precision != PB_PRECISION_FP64 && o->precision != PB_PRECISION_MIXED) return "precision is invalid";
+        if (o->precision < PB_PRECISION_FP64 || o->precision > PB_PRECISION_FP32_SEED) return "precision is invalid";
+        if (o->reduced_solver < PB_REDUCED_AUTO || o->reduced_solver > PB_REDUCED_DENSE) return "reduced_solver is invalid";
+        if (o->dense_max_rows < 0) return "dense_max_rows must be non-negative";
+        // plaMatrixPreconditionerClusterSize must be in [1, 16] (src/options.cpp:393-394)
+        if (o->preconditioner_cluster_size < 1 || o->preconditioner_cluster_size > 16)
+            return "plaMatrixPreconditionerClusterSize must be in [1, 16]";
         if (o->pcg_relative_tolerance < 0.0 || o->pcg_absolute_tolerance < 0.0 || o->pcg_max_iterations < 0)
             return "PCG controls must be non-negative";
         if (o->batch_factor_shape < -1 || o->batch_factor_shape >= pbk::kFactorShapes)
@@ -1310,6 +1338,7 @@ namespace
     // Device copy of a prepared problem and its layout (setup).
     void uploadStructure(pb_ctx* ctx, Prepared&& p, Layout&& L, const pb_solve_options& o)
     {
+        ++ctx->patternBuilds;
         hipStream_t s = ctx->stream;
         DeviceAlloc& A = ctx->dstruct;
         pbk::Problem P;
@@ -1525,6 +1554,15 @@ namespace
     // canUseReferenceOnlineSchur (ReferenceSchur.cpp:667-673) for frame cameras without rigs, laser ranges, scale
     // bars, pose priors or a camera plane: the online point Schur (path B) unless laser planes are present or
     // nothing primary is refined; otherwise the assembled damped Schur (path A).
+    // Reduced-solve route of a solve: PB_REDUCED_AUTO takes the dense Cholesky up to dense_max_rows rows.
+    bool useDense(const pb_ctx* ctx, const pb_solve_options& o)
+    {
+        if (o.reduced_solver == PB_REDUCED_DENSE) return true;
+        if (o.reduced_solver == PB_REDUCED_PCG) return false;
+        const int maxRows = o.dense_max_rows > 0 ? o.dense_max_rows : kDenseSolveMaxRows;
+        return 6L * ctx->P.nUnits <= maxRows;
+    }
+
     bool onlineSchur(const pb_ctx* ctx, const pb_solve_options& o)
     {
         return o.use_online_schur != 0 && ctx->prep.primaryBlocks > 0 && ctx->in.nPlanes == 0;
@@ -1745,6 +1783,178 @@ namespace
         return res;
     }
 
+    // ------------------------------------------------------------------------------------------
+    // Dense reduced solve (pb_dense.hip): Cholesky of the gathered reduced system in FP64, or a guarded FP32
+    // factorisation whose solution is refined in FP64 against the stored system.
+    int denseTiles(const pbk::Problem& P) { return (6 * P.nUnits + pbk::kDenseTile - 1) / pbk::kDenseTile; }
+
+    void ensureDense(pb_ctx* ctx, int nt, bool fp32)
+    {
+        auto& D = ctx->dense;
+        hipStream_t s = ctx->stream;
+        const size_t npad = size_t(nt) * pbk::kDenseTile;
+        if (nt > D.ntCap)
+        {
+            D.vec.release();
+            D.mat32.release();
+            D.mat64.release();
+            D.A32 = nullptr;
+            D.A64 = nullptr;
+            D.ntCap = nt;
+            D.v32 = D.vec.alloc<float>(npad);
+            D.v64 = D.vec.alloc<double>(npad);
+            D.x = D.vec.alloc<double>(npad);
+            D.r = D.vec.alloc<double>(npad);
+            D.rowNorm = D.vec.alloc<double>(npad);
+            D.norm = D.vec.alloc<double>(4);
+            D.st.words = D.vec.alloc<int32_t>(pbk::kDenseWordCount);
+            D.st.progress = D.vec.alloc<unsigned long long>(2);
+            // words and progress start at zero once; every later solve advances the epoch on the device
+            check(hipMemsetAsync(D.st.words, 0, sizeof(int32_t) * pbk::kDenseWordCount, s), "memset");
+            check(hipMemsetAsync(D.st.progress, 0, sizeof(unsigned long long) * 2, s), "memset");
+        }
+        const size_t cap = size_t(D.ntCap) * pbk::kDenseTile;
+        if (fp32 && !D.A32) D.A32 = D.mat32.alloc<float>(cap * cap);
+        if (!fp32 && !D.A64) D.A64 = D.mat64.alloc<double>(cap * cap);
+    }
+
+    struct DenseOutcome
+    {
+        bool factored = false;   // the Cholesky factorisation found every pivot positive
+        bool converged = false;  // the refined relative residual reached the tolerance
+        int refinements = 0;
+        double relResidual = 0.0;
+    };
+
+    // Factors S (the work's reduced matrix, gathered in precision TA) and solves S x = rhs into dense.x (FP64):
+    // one solve, then up to maxRefine refinement steps x += (L L^T)^-1 (rhs - S x) while the relative residual
+    // is above relTol.
+    template <typename TA, typename T>
+    DenseOutcome denseAttempt(pb_ctx* ctx, const pbk::Work<T>& W, double relTol, int maxRefine, pb_solve_report* rep)
+    {
+        using DL = pbk::DenseLauncher<TA>;
+        const pbk::Problem& P = ctx->P;
+        hipStream_t s = ctx->stream;
+        const int nt = denseTiles(P);
+        const int n = 6 * P.nUnits, npad = nt * pbk::kDenseTile;
+        ensureDense(ctx, nt, std::is_same<TA, float>::value);
+        auto& D = ctx->dense;
+        TA* A = nullptr;
+        TA* v = nullptr;
+        if constexpr (std::is_same<TA, float>::value)
+        {
+            A = D.A32;
+            v = D.v32;
+        }
+        else
+        {
+            A = D.A64;
+            v = D.v64;
+        }
+        DenseOutcome out;
+        auto t0 = Clock::now();
+        DL::template gather<T>(P, W.S, A, nt, s);
+        DL::factor(A, nt, D.st, s);
+        DL::norm2(W.rhs, n, D.norm + 1, s);
+        int32_t* hw = reinterpret_cast<int32_t*>(ctx->pinned + kPinDense);
+        double* hn = ctx->pinned + kPinDense + 4;
+        auto readWords = [&]() {
+            check(hipMemcpyAsync(hw, D.st.words, sizeof(int32_t) * pbk::kDenseWordCount, hipMemcpyDeviceToHost, s),
+                  "readback");
+            check(hipMemcpyAsync(hn, D.norm, sizeof(double) * 2, hipMemcpyDeviceToHost, s), "readback");
+            check(hipStreamSynchronize(s), "dense solve");
+            if (hw[pbk::kDenseTimeout]) throw std::runtime_error("dense triangular solve stopped waiting for a tile");
+        };
+        readWords();
+        if (rep) rep->cholesky_factorization_seconds += seconds(t0);
+        if (hw[pbk::kDenseNotSpd]) return out;
+        out.factored = true;
+        const double bn = std::sqrt(hn[1]);
+        auto ts = Clock::now();
+        DL::toDense(W.rhs, n, npad, v, s);
+        DL::solve(A, nt, v, D.st, s);
+        DL::accumulate(D.x, v, n, true, s);
+        if (rep)
+        {
+            check(hipStreamSynchronize(s), "dense solve");
+            rep->triangular_solve_seconds += seconds(ts);
+        }
+        double previous = kInf;
+        for (int it = 0;; ++it)
+        {
+            auto tr = Clock::now();
+            DL::template residual<T>(P, W.S, W.rhs, D.x, D.r, D.rowNorm, D.norm, s);
+            readWords();
+            if (rep) rep->residual_check_seconds += seconds(tr);
+            out.relResidual = bn > 0.0 ? std::sqrt(hn[0]) / bn : 0.0;
+            if (!std::isfinite(out.relResidual)) return out;
+            // refinement that stops contracting (an FP32 factor of a system whose conditioning exceeds what FP32
+            // resolves) ends the attempt
+            const bool stalled = it > 0 && out.relResidual > 0.5 * previous;
+            previous = out.relResidual;
+            if (out.relResidual <= relTol || it >= maxRefine || stalled)
+            {
+                out.converged = out.relResidual <= relTol;
+                break;
+            }
+            auto ts = Clock::now();
+            DL::toDense(D.r, n, npad, v, s);
+            DL::solve(A, nt, v, D.st, s);
+            DL::accumulate(D.x, v, n, false, s);
+            if (rep)
+            {
+                check(hipStreamSynchronize(s), "dense solve");
+                rep->triangular_solve_seconds += seconds(ts);
+            }
+            ++out.refinements;
+        }
+        return out;
+    }
+
+    // The dense route of one LM step: FP64 Cholesky (fp64 work or a rejected seed), or with fp32Seed an FP32
+    // factorisation refined in FP64 to relTol, accepted only when that residual is reached (else the step is
+    // solved again with the FP64 factorisation). The solution goes to W.x.
+    template <typename T>
+    PcgResult denseSolve(pb_ctx* ctx, pbk::Work<T>& W, double relTol, bool fp32Seed, bool* seedAccepted,
+                         pb_solve_report* rep)
+    {
+        const pbk::Problem& P = ctx->P;
+        hipStream_t s = ctx->stream;
+        PcgResult res;
+        *seedAccepted = false;
+        if (P.nUnits == 0)
+        {
+            res.converged = true;
+            return res;
+        }
+        const int n = 6 * P.nUnits;
+        DenseOutcome o;
+        if (fp32Seed)
+        {
+            o = denseAttempt<float>(ctx, W, relTol, kDenseSeedRefinements, rep);
+            *seedAccepted = o.factored && o.converged;
+            res.iterations += o.refinements + (o.factored ? 1 : 0);
+        }
+        if (!*seedAccepted)
+        {
+            o = denseAttempt<double>(ctx, W, relTol, 0, rep);
+            res.iterations += o.factored ? 1 : 0;
+        }
+        res.breakdown = !o.factored;
+        res.converged = o.factored && std::isfinite(o.relResidual);
+        res.rnorm = o.relResidual;
+        if (res.converged) pbk::DenseLauncher<double>::store<T>(ctx->dense.x, n, W.x, s);
+        return res;
+    }
+
+    struct Progress;
+
+    // Reduced solve of one LM step by the route of the solve (dense Cholesky or PCG), with its report counters.
+    template <typename T>
+    PcgResult reducedSolve(pb_ctx* ctx, pbk::Work<T>& W, const pb_solve_options& o, Progress& pg,
+                           pb_solve_report* rep, double relTol, double absTol, int maxPcg, int checkEvery,
+                           double modelEta);
+
     void setMessage(pb_solve_report* r, const std::string& m)
     {
         std::snprintf(r->message, sizeof(r->message), "%s", m.c_str());
@@ -1760,6 +1970,11 @@ namespace
         bool inexact = false;     // truncated-Newton PCG stop and cap (a mixed solve, both parts)
         bool fallback = false;    // set by an FP32 part that hands over
         bool fp64Finish = false;  // the FP64 continuation of a mixed solve ran
+        int precision = PB_PRECISION_FP64;
+        bool dense = false;       // reduced systems solved by the dense Cholesky route
+        bool seed = false;        // dense route: FP32 factorisation refined in FP64 first (guarded)
+        int fp32Accepted = 0;     // accepted steps of the FP32 part of a mixed solve
+        int seedRejections = 0;   // consecutive FP32 seeds rejected (at kSeedRejectionsToStop the solve stops seeding)
         Clock::time_point t0{};
         double current = std::numeric_limits<double>::quiet_NaN();
         double damping = 0.0;
@@ -1772,6 +1987,59 @@ namespace
                            Progress& pg);
     template <typename T>
     void publishState(pb_ctx* ctx, pb_solve_report* rep, const Progress& pg);
+
+    template <typename T>
+    PcgResult reducedSolve(pb_ctx* ctx, pbk::Work<T>& W, const pb_solve_options& o, Progress& pg,
+                           pb_solve_report* rep, double relTol, double absTol, int maxPcg, int checkEvery,
+                           double modelEta)
+    {
+        auto tl = Clock::now();
+        PcgResult lin;
+        double tol = relTol;
+        if (pg.dense)
+        {
+            // an FP32-assembled system carries FP32 rounding itself, so its refinement stops at 1e-6; the
+            // reference's mixed mode refines an FP32 seed to the FP64 tolerance of the GPU route (2e-10,
+            // BundleAdjustPlaMatrix.cpp:326)
+            const bool fp32Work = std::is_same<T, float>::value;
+            tol = o.pcg_relative_tolerance > 0.0 ? o.pcg_relative_tolerance : (fp32Work ? 1e-6 : 2e-10);
+            bool seed = false;
+            lin = denseSolve(ctx, W, tol, pg.seed && pg.seedRejections < kSeedRejectionsToStop, &seed, rep);
+            if (seed)
+            {
+                ++rep->mixed_seed_steps;
+                pg.seedRejections = 0;
+            }
+            else if (pg.seed && pg.seedRejections < kSeedRejectionsToStop)
+            {
+                ++rep->dense_fallbacks;
+                ++pg.seedRejections;
+            }
+        }
+        else
+        {
+            lin = pcg(ctx, W, relTol, absTol, maxPcg, checkEvery, modelEta);
+        }
+        rep->linear_tolerance_minimum =
+            rep->linear_tolerance_minimum > 0.0 ? std::min(rep->linear_tolerance_minimum, tol) : tol;
+        rep->linear_tolerance_maximum = std::max(rep->linear_tolerance_maximum, tol);
+        rep->linear_solve_seconds += seconds(tl);
+        rep->linear_iterations += lin.iterations;
+        return lin;
+    }
+
+    // SolverControlOptions::progressCallback after a trial step (BundleAdjustPlaMatrix.cpp:455-464,544-554):
+    // false cancels the solve.
+    bool reportProgress(pb_ctx* ctx, const pb_solve_options& o, const Progress& pg)
+    {
+        if (!ctx->progressFn) return true;
+        pb_iteration_summary sm{};
+        sm.current_iteration = pg.iterations;
+        sm.max_iterations = std::max(1, o.max_iterations);
+        sm.average_rms_pixels = std::sqrt(std::max(0.0, pg.current) / std::max(1, ctx->prep.observations));
+        sm.valid_point_count = ctx->prep.activeTracks;
+        return ctx->progressFn(&sm, ctx->progressUser) != 0;
+    }
 
     // The online-Schur controller (BundleAdjustPlaMatrix.cpp:162-631 with useReferenceOnlineSchur).
     template <typename T>
@@ -1810,7 +2078,6 @@ namespace
         const double modelEta = o.pcg_model_eta >= 0.0 ? o.pcg_model_eta : (mixed ? 0.1 : 0.0);
         const double rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
         rep->linear_tolerance = relTol;
-        rep->mixed_precision_used = (mixed || pg.fp64Finish) ? 1 : 0;
         ctx->lastScalarBytes = sizeof(T);
         rep->online_schur_used = 1;
 
@@ -1819,7 +2086,7 @@ namespace
         int &accepted = pg.accepted, &rejected = pg.rejected, &iterations = pg.iterations;
         bool& converged = pg.converged;
         bool& cancelled = pg.cancelled;
-        auto isCancelled = [&]() { return cancel && *cancel != 0; };
+        auto isCancelled = [&]() { return cancelled || (cancel && *cancel != 0); };
         auto raise = [&]() { damping = damping == 0.0 ? 1e-3 : damping * 10.0; };
         if (converged && std::isnan(current))
         {
@@ -1863,10 +2130,7 @@ namespace
                     raise();
                     continue;
                 }
-                auto tl = Clock::now();
-                const PcgResult lin = pcg(ctx, W, relTol, absTol, maxPcg, checkEvery, modelEta);
-                rep->linear_solve_seconds += seconds(tl);
-                rep->linear_iterations += lin.iterations;
+                const PcgResult lin = reducedSolve(ctx, W, o, pg, rep, relTol, absTol, maxPcg, checkEvery, modelEta);
                 if (lin.breakdown)
                 {
                     ++iterations;
@@ -1945,6 +2209,7 @@ namespace
                     Lr::accept(P, W, acceptedIndex, alpha, s);
                     current = cand;
                     ++accepted;
+                    if (fp32) ++pg.fp32Accepted;
                     damping /= 10.0;
                     converged = alpha * rmsG <= rmsTol && alpha * rmsP <= rmsTol;
                     relinearize = !converged;
@@ -1959,6 +2224,11 @@ namespace
                     }
                     terminate = true;
                     relinearize = true;
+                }
+                if (!reportProgress(ctx, o, pg))
+                {
+                    cancelled = true;
+                    break;
                 }
             }
         }
@@ -1976,6 +2246,15 @@ namespace
         const double current = pg.current;
         const bool converged = pg.converged, cancelled = pg.cancelled;
         rep->solve_seconds = seconds(pg.t0);
+        // mixedPrecisionUsed (result.h:115-116): the reference mode counts accepted FP32 seeds; the FP32-assembly
+        // mode counts FP32 steps the line search accepted (a solve handed to FP64 before any reports 0)
+        rep->mixed_precision_used = pg.precision == PB_PRECISION_FP32_SEED ? (rep->mixed_seed_steps > 0 ? 1 : 0)
+                                    : pg.precision == PB_PRECISION_MIXED   ? (pg.fp32Accepted > 0 ? 1 : 0)
+                                                                           : 0;
+        std::snprintf(rep->linear_solver_name, sizeof(rep->linear_solver_name), "%s",
+                      !pg.dense ? "pcg" : pg.seed ? "dense-cholesky-fp32-refined" : "dense-cholesky-fp64");
+        std::snprintf(rep->preconditioner_name, sizeof(rep->preconditioner_name), "%s",
+                      pg.dense ? "none" : "block-jacobi-6");
         rep->accepted_steps = pg.accepted;
         rep->rejected_steps = pg.rejected;
         rep->iterations = pg.iterations;
@@ -2053,7 +2332,6 @@ namespace
         const double rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
         constexpr double kProbe = 1e-4;
         rep->linear_tolerance = relTol;
-        rep->mixed_precision_used = (mixed || pg.fp64Finish) ? 1 : 0;
         rep->online_schur_used = 0;
         ctx->lastScalarBytes = sizeof(T);
 
@@ -2062,7 +2340,7 @@ namespace
         int &accepted = pg.accepted, &rejected = pg.rejected, &iterations = pg.iterations;
         bool& converged = pg.converged;
         bool& cancelled = pg.cancelled;
-        auto isCancelled = [&]() { return cancel && *cancel != 0; };
+        auto isCancelled = [&]() { return cancelled || (cancel && *cancel != 0); };
         auto raise = [&]() { damping = damping == 0.0 ? 1e-3 : damping * 10.0; };
         auto timedBuild = [&]() {
             auto ta = Clock::now();
@@ -2105,13 +2383,7 @@ namespace
                 fresh = false;
                 if (b.singular) throw std::runtime_error("linear solve failed: singular damped point block");
                 PcgResult lin;
-                if (!b.notSpd)
-                {
-                    auto tl = Clock::now();
-                    lin = pcg(ctx, W, relTol, absTol, maxPcg, checkEvery, modelEta);
-                    rep->linear_solve_seconds += seconds(tl);
-                    rep->linear_iterations += lin.iterations;
-                }
+                if (!b.notSpd) lin = reducedSolve(ctx, W, o, pg, rep, relTol, absTol, maxPcg, checkEvery, modelEta);
                 ++iterations;
                 if (b.notSpd || lin.breakdown || !lin.converged)
                 {
@@ -2134,6 +2406,7 @@ namespace
                 if (rmsG <= rmsTol && rmsP <= rmsTol)
                 {
                     converged = true;  // the small step is not applied
+                    if (!reportProgress(ctx, o, pg)) cancelled = true;
                     continue;
                 }
                 // probe + line search: the first pass evaluates alpha = 1e-4 and 1, later passes `batch` halvings
@@ -2190,6 +2463,7 @@ namespace
                     Lr::accept(P, W, candIndex, candAlpha, s);
                     current = cand;
                     ++accepted;
+                    if (fp32) ++pg.fp32Accepted;
                     damping /= 10.0;
                     converged = alpha * rmsG <= rmsTol && alpha * rmsP <= rmsTol;
                     relinearize = !converged;
@@ -2203,6 +2477,11 @@ namespace
                         return;
                     }
                     raise();
+                }
+                if (!reportProgress(ctx, o, pg))
+                {
+                    cancelled = true;
+                    break;
                 }
             }
         }
@@ -3004,7 +3283,7 @@ void pb_default_options(pb_solve_options* o)
     o->robust_loss_scale = 1.0;
     o->armijo_coefficient = 1e-3;
     o->line_search_steps = 34;
-    o->use_online_schur = 1;
+    o->use_online_schur = 0;  // useReferenceOnlineSchur = false (options.h:65)
     o->max_initial_track_rms = 100.0;
     o->laser_plane_weight = 1.0;
     o->laser_huber_delta = 0.2;
@@ -3036,6 +3315,9 @@ void pb_default_options(pb_solve_options* o)
     o->device = 0;
     o->precision = PB_PRECISION_FP64;
     o->pcg_model_eta = -1.0;
+    o->reduced_solver = PB_REDUCED_AUTO;
+    o->dense_max_rows = 0;
+    o->preconditioner_cluster_size = 1;
 }
 
 int pb_hip_available(int device)
@@ -3098,7 +3380,9 @@ pb_status pb_hip_destroy(pb_ctx* ctx)
     for (hipEvent_t e : ctx->eventPool) (void)hipEventDestroy(e);
     for (auto& g : ctx->pcgGraph)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
+    ctx->dense.vec.release();
+    ctx->dense.mat32.release();
+    ctx->dense.mat64.release();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -3163,8 +3447,12 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
             return PB_ERR_CANCELLED;
         }
         const auto ts = Clock::now();
+        const int64_t builds = ctx->patternBuilds;
         ensureStructure(ctx, *options);
         fillReportCounts(ctx, report);
+        report->schur_pattern_builds = int32_t(ctx->patternBuilds - builds);
+        report->schur_pattern_reuses = report->schur_pattern_builds ? 0 : 1;
+        report->schur_assembly_on_device = 1;
         applySolveOptions(ctx, *options);
         if (ctx->in.nCam == 0 || ctx->in.nTracks == 0)
         {
@@ -3182,12 +3470,24 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
             (void)work<float>(ctx);
         else
             (void)work<double>(ctx);
+        Progress pg;
+        pg.precision = options->precision;
+        pg.dense = useDense(ctx, *options);
+        pg.seed = pg.dense && options->precision != PB_PRECISION_FP64;
+        if (pg.dense) ensureDense(ctx, denseTiles(ctx->P), pg.seed);
+        if (!pg.dense && options->preconditioner_cluster_size > 1)
+        {
+            report->solve_status = PB_SOLVE_UNSUPPORTED;
+            setMessage(report, "preconditioner clusters above 1 are not implemented on the PCG route "
+                               "(the dense route has no preconditioner)");
+            return fail(ctx, PB_ERR_UNSUPPORTED, report->message);
+        }
         check(hipStreamSynchronize(ctx->stream), "setup");
         report->setup_seconds = seconds(ts);
         ctx->lastOptions = *options;
-        Progress pg;
-        pg.inexact = options->precision == PB_PRECISION_MIXED;
-        if (pg.inexact)
+        // the truncated-Newton PCG stop is a property of the FP32-assembly mode on the PCG route
+        pg.inexact = options->precision == PB_PRECISION_MIXED && !pg.dense;
+        if (options->precision == PB_PRECISION_MIXED)
         {
             runSolve<float>(ctx, *options, report, cancel, pg);
             if (pg.fallback)
@@ -3196,6 +3496,7 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
                 pg.fallback = false;
                 pg.fp64Finish = true;
                 pg.inexact = false;
+                pg.seed = false;
                 runSolve<double>(ctx, *options, report, cancel, pg);
             }
         }
@@ -3277,35 +3578,20 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
         check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
         Lr::adjointRhs(P, W, dC, dP, s);
         Lr::schurBlocks(P, W, 0.0, s);
-        // H lambda = g is solved once per call, exactly: a dense FP64 Cholesky of the reduced system (rocSOLVER)
-        // up to kDenseAdjointMax rows (the aerial block has 6,024; its condition number near 1e14 keeps PCG
-        // from a tight residual), FP64 PCG to a 1e-10 relative residual above that
+        // H lambda = g is solved once per call, exactly: the dense FP64 Cholesky of the reduced system up to
+        // kDenseAdjointMax rows (the aerial block has 6,024; its condition number near 1e14 keeps PCG from a
+        // tight residual; 60k rows of a 10k-camera block take 29 GB of HBM), FP64 PCG to a 1e-10 relative
+        // residual above that
         const long n = 6L * P.nUnits;
         if (n > 0 && n <= kDenseAdjointMax)
         {
-            double* A = tmp.alloc<double>(size_t(n) * n);
-            double* rhs = tmp.alloc<double>(n);
-            rocblas_int* info = tmp.alloc<rocblas_int>(1);
-            check(hipMemsetAsync(A, 0, sizeof(double) * size_t(n) * n, s), "memset");
-            Lr::denseSystem(P, W, A, rhs, s);
-            if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success)
-                throw std::runtime_error("rocblas_create_handle failed");
-            if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success ||
-                rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, rocblas_int(n), A, rocblas_int(n), info) !=
-                    rocblas_status_success)
-                throw std::runtime_error("rocsolver_dpotrf failed");
-            rocblas_int hinfo = 0;
-            check(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s), "readback");
-            check(hipStreamSynchronize(s), "adjoint factorisation");
-            if (hinfo != 0)
+            bool seed = false;
+            const PcgResult lin = denseSolve(ctx, W, 0.0, false, &seed, nullptr);
+            if (lin.breakdown)
             {
                 tmp.release();
                 return fail(ctx, PB_ERR_NUMERIC, "adjoint system is not positive definite");
             }
-            if (rocsolver_dpotrs(ctx->blas, rocblas_fill_lower, rocblas_int(n), 1, A, rocblas_int(n), rhs,
-                                 rocblas_int(n)) != rocblas_status_success)
-                throw std::runtime_error("rocsolver_dpotrs failed");
-            check(hipMemcpyAsync(W.x, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
         }
         else
         {
@@ -3650,6 +3936,75 @@ pb_status pb_hip_quality(pb_ctx* ctx, const pb_quality_options* q, pb_quality_re
                 if (validOut) validOut[t] = 0;
                 if (rmsBefore) rmsBefore[t] = hostStrictRms(in, inCams, t);
             }
+        }
+        return PB_OK;
+    });
+}
+
+pb_status pb_hip_set_progress(pb_ctx* ctx, pb_progress_fn fn, void* user)
+{
+    if (!ctx) return PB_ERR_INVALID;
+    ctx->progressFn = fn;
+    ctx->progressUser = fn ? user : nullptr;
+    return PB_OK;
+}
+
+pb_status pb_hip_dense_solve(pb_ctx* ctx, int32_t n, int32_t fp32, const double* A, const double* b, double* x,
+                             int32_t* info, double* seconds)
+{
+    if (!ctx || n <= 0 || !A || !b || !x || (fp32 != 0 && fp32 != 1)) return fail(ctx, PB_ERR_INVALID, "invalid dense solve");
+    return guarded(ctx, [&]() -> pb_status {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        hipStream_t s = ctx->stream;
+        const int nt = (n + pbk::kDenseTile - 1) / pbk::kDenseTile;
+        const size_t npad = size_t(nt) * pbk::kDenseTile;
+        ensureDense(ctx, nt, fp32 != 0);
+        auto& D = ctx->dense;
+        // the solver works on the leading npad x npad block of the matrix buffer (leading dimension npad)
+        struct Events
+        {
+            hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+            ~Events()
+            {
+                for (hipEvent_t x : e)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } evs;
+        hipEvent_t* ev = evs.e;
+        for (int i = 0; i < 3; ++i) check(hipEventCreate(&ev[i]), "hipEventCreate");
+        auto run = [&](auto* Adev, auto* vdev) {
+            using TA = std::remove_pointer_t<decltype(Adev)>;
+            std::vector<TA> hA(npad * npad, TA(0)), hb(npad, TA(0));
+            for (size_t c = 0; c < npad; ++c)
+                for (size_t r = 0; r < npad; ++r)
+                    hA[c * npad + r] = (r < size_t(n) && c < size_t(n)) ? TA(A[c * n + r]) : TA(r == c ? 1 : 0);
+            for (int i = 0; i < n; ++i) hb[i] = TA(b[i]);
+            check(hipMemcpyAsync(Adev, hA.data(), sizeof(TA) * hA.size(), hipMemcpyHostToDevice, s), "upload");
+            check(hipMemcpyAsync(vdev, hb.data(), sizeof(TA) * hb.size(), hipMemcpyHostToDevice, s), "upload");
+            check(hipEventRecord(ev[0], s), "hipEventRecord");
+            pbk::DenseLauncher<TA>::factor(Adev, nt, D.st, s);
+            check(hipEventRecord(ev[1], s), "hipEventRecord");
+            pbk::DenseLauncher<TA>::solve(Adev, nt, vdev, D.st, s);
+            check(hipEventRecord(ev[2], s), "hipEventRecord");
+            check(hipMemcpyAsync(hb.data(), vdev, sizeof(TA) * npad, hipMemcpyDeviceToHost, s), "download");
+            int32_t words[pbk::kDenseWordCount];
+            check(hipMemcpyAsync(words, D.st.words, sizeof(words), hipMemcpyDeviceToHost, s), "download");
+            check(hipStreamSynchronize(s), "dense solve");
+            if (words[pbk::kDenseTimeout]) throw std::runtime_error("dense triangular solve stopped waiting for a tile");
+            for (int i = 0; i < n; ++i) x[i] = double(hb[i]);
+            if (info) *info = words[pbk::kDenseNotSpd] ? 1 : 0;
+        };
+        if (fp32)
+            run(D.A32, D.v32);
+        else
+            run(D.A64, D.v64);
+        float f = 0.0f, t = 0.0f;
+        check(hipEventElapsedTime(&f, ev[0], ev[1]), "hipEventElapsedTime");
+        check(hipEventElapsedTime(&t, ev[1], ev[2]), "hipEventElapsedTime");
+        if (seconds)
+        {
+            seconds[0] = f * 1e-3;
+            seconds[1] = t * 1e-3;
         }
         return PB_OK;
     });

@@ -350,6 +350,47 @@ namespace pbk
         static void accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s);
     };
 
+    // Dense reduced solve (pb_dense.hip): the reduced system gathered into a column-major matrix of order
+    // npad = 64 * nt (identity padding), factored L L^T in place in 64 x 64 tiles, and solved in two
+    // single-launch triangular sweeps.
+    constexpr int kDenseTile = 64;
+    constexpr int kDenseBlock = 4;  // tile columns per block of the factorisation (one trailing update per block)
+    enum DenseWord
+    {
+        kDenseEpoch = 0,     // advanced by every factorisation and solve; the progress words carry it
+        kDenseTicketLower,   // tile tickets of the two sweeps (dispatch order)
+        kDenseTicketUpper,
+        kDenseNotSpd,        // a pivot of the factorisation was not positive and finite
+        kDenseTimeout,       // a sweep stopped waiting for a tile (bounded spin; not expected to happen)
+        kDenseWordCount = 8
+    };
+    struct DenseState
+    {
+        int32_t* words = nullptr;               // [kDenseWordCount]
+        unsigned long long* progress = nullptr;  // [2] (epoch << 32) | tiles done, of the lower and the upper sweep
+    };
+
+    template <typename TA>
+    struct DenseLauncher
+    {
+        // lower tiles of the padded dense matrix (leading dimension 64 * nt) from the block-sparse rows of S
+        template <typename TW>
+        static void gather(const Problem& P, const TW* S, TA* A, int nt, hipStream_t s);
+        // A = L L^T in place (lower triangle); a pivot that is not positive sets words[kDenseNotSpd]
+        static void factor(TA* A, int nt, const DenseState& st, hipStream_t s);
+        // x <- (L L^T)^-1 x (x has 64 * nt entries, the padded tail zero)
+        static void solve(const TA* A, int nt, TA* x, const DenseState& st, hipStream_t s);
+        // r = b - S x in FP64 over the block-sparse rows, norm[0] = |r|^2 (fixed-order sum of rowNorm)
+        template <typename TW>
+        static void residual(const Problem& P, const TW* S, const double* b, const double* x, double* r,
+                             double* rowNorm, double* norm, hipStream_t s);
+        static void norm2(const double* v, int n, double* out, hipStream_t s);  // out[0] = |v|^2, fixed order
+        static void toDense(const double* src, int n, int npad, TA* dst, hipStream_t s);   // padded copy
+        static void accumulate(double* x, const TA* c, int n, bool overwrite, hipStream_t s);  // x += c (or = c)
+        template <typename TO>
+        static void store(const double* src, int n, TO* dst, hipStream_t s);
+    };
+
     // Post-solve quality buffers (device): strict track RMS at the input and refined state, sort keys, filter.
     struct Quality
     {

@@ -48,8 +48,20 @@ class IntrinsicParameter(enum.IntEnum):
 
 
 class Precision(enum.IntEnum):
+    """pb_precision: Fp64; Mixed (FP32 Jacobians, Schur and reduced solve, FP64 cost and acceptance); Fp32Seed
+    (enablePlaMatrixMixedPrecision, options.h:272-273: FP64 system, guarded FP32 seed of the reduced solve)."""
+
     Fp64 = 0
     Mixed = 1
+    Fp32Seed = 2
+
+
+class ReducedSolver(enum.IntEnum):
+    """pb_reduced_solver: Auto (dense up to denseMaxRows rows), Pcg (block-Jacobi PCG), Dense (Cholesky)."""
+
+    Auto = 0
+    Pcg = 1
+    Dense = 2
 
 
 @dataclass
@@ -59,13 +71,16 @@ class SolverControlOptions:
     imageRobustLossScalePixels: float = 1.0
     referenceArmijoCoefficient: float = 1.0e-3
     referenceLineSearchSteps: int = 34
-    useReferenceOnlineSchur: bool = True
+    useReferenceOnlineSchur: bool = False
     enablePointFilter: bool = True
     filterMaxReprojError: float = 2.5
     filterSigmaFactor: float = 3.0
     maxInitialTrackRms: float = 100.0
     numThreads: int = 0
     logIterationProgress: bool = True
+    # SolverControlOptions::progressCallback (options.h:74): called with an IterationSummary after every trial
+    # step; returning False cancels the solve
+    progressCallback: object = None
 
 
 @dataclass
@@ -126,6 +141,7 @@ class BackendOptions:
     requested: Backend = Backend.Mi355x
     plaMatrixDevice: int = 0
     enablePlaMatrixMixedPrecision: bool = False
+    plaMatrixPreconditionerClusterSize: int = 1
     allowFallback: bool = False
     # MI355X controls (no reference counterpart)
     pcgRelativeTolerance: float = 0.0
@@ -139,6 +155,8 @@ class BackendOptions:
     batchGraphs: bool = True  # batched windows: replay each round's launches as captured HIP graphs
     batchFactorShape: int = -1  # batched windows: dense factorisation launch shape (-1 = timed on the first round)
     pcgGraphs: bool = True  # replay PCG iterations from a captured HIP graph
+    reducedSolver: ReducedSolver = ReducedSolver.Auto
+    denseMaxRows: int = 0  # 0: 16384 rows
 
 
 @dataclass
@@ -223,7 +241,7 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.previous_intrinsic_mask = sum(1 << i for i, v in enumerate(c.referencePreviousIntrinsicParameterMask) if v)
     o.device = int(b.plaMatrixDevice)
     if precision is None:
-        precision = Precision.Mixed if b.enablePlaMatrixMixedPrecision else Precision.Fp64
+        precision = Precision.Fp32Seed if b.enablePlaMatrixMixedPrecision else Precision.Fp64
     o.precision = int(precision)
     o.pcg_relative_tolerance = float(b.pcgRelativeTolerance)
     o.pcg_absolute_tolerance = float(b.pcgAbsoluteTolerance)
@@ -236,6 +254,9 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.batch_graphs = 1 if b.batchGraphs else 0
     o.batch_factor_shape = int(b.batchFactorShape)
     o.pcg_graphs = 1 if b.pcgGraphs else 0
+    o.reduced_solver = int(b.reducedSolver)
+    o.dense_max_rows = int(b.denseMaxRows)
+    o.preconditioner_cluster_size = int(b.plaMatrixPreconditionerClusterSize)
     k = options.constraints
     o.laser_plane_weight = float(k.laserPlaneWeight)
     o.laser_huber_delta = float(k.laserHuberDeltaMeters)

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import enum
+import hashlib
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -64,6 +65,39 @@ class Result:
         return self.report.get("final_cost", float("nan"))
 
 
+@dataclass
+class IterationSummary:
+    """plabundle::IterationSummary (include/plabundle/options.h:50-56)."""
+
+    currentIteration: int = 0
+    maxIterations: int = 0
+    averageRmsPixels: float = 0.0
+    validPointCount: int = 0
+
+
+def problem_fingerprint(problem: Problem) -> bytes:
+    """Digest of everything pb_hip_upload copies from a problem: every array (bytes, dtype, shape) and the gauge
+    scalars. A workspace re-uploads whenever it changes, so arrays mutated in place are never solved from a stale
+    device copy."""
+    try:
+        import xxhash
+
+        h = xxhash.xxh3_128()
+    except ImportError:  # pragma: no cover - xxhash ships with the image
+        h = hashlib.blake2b(digest_size=16)
+    for name in problem.__dataclass_fields__:
+        if name == "meta":
+            continue
+        v = getattr(problem, name)
+        if isinstance(v, np.ndarray):
+            a = np.ascontiguousarray(v)
+            h.update(f"{name}:{a.dtype.str}:{a.shape}".encode())
+            h.update(a.view(np.uint8).reshape(-1) if a.size else b"")
+        else:
+            h.update(f"{name}={v!r}".encode())
+    return h.digest()
+
+
 def factor_shape(shape: int):
     """(threads, panel width) of a launch shape of the batched dense window factorisation."""
     t, p = C.c_int32(), C.c_int32()
@@ -79,7 +113,8 @@ class SolverWorkspace:
         self.device = device
         self._lib = _abi.load_library()
         self._ctx = C.c_void_p()
-        self._uploaded_id = None
+        self._uploaded = None  # fingerprint of the problem on the context
+        self._counts = None    # (cameras, tracks, observations, control points) of that problem
         self._num_tracks = 0
         st = self._lib.pb_hip_create(device, C.byref(self._ctx))
         if st != _abi.PB_OK:
@@ -93,17 +128,28 @@ class SolverWorkspace:
     def last_error(self) -> str:
         return self._lib.pb_hip_last_error(self._ctx).decode("utf-8", "replace") if self._ctx else "closed"
 
-    def upload(self, problem: Problem) -> int:
+    def upload(self, problem: Problem, fingerprint: bytes | None = None) -> int:
         view, keep = _abi.make_view(problem)
         st = self._lib.pb_hip_upload(self._ctx, C.byref(view))
         del keep
         if st == _abi.PB_OK:
-            self._uploaded_id = id(problem)
+            self._uploaded = fingerprint or problem_fingerprint(problem)
+            self._counts = (problem.num_cameras, problem.num_tracks, problem.num_observations,
+                            problem.num_control_points)
             self._num_tracks = problem.num_tracks
+        else:
+            self._uploaded = self._counts = None
         return st
 
+    def ensure_uploaded(self, problem: Problem) -> int:
+        """Uploads the problem unless the context already holds one with the same content."""
+        fp = problem_fingerprint(problem)
+        if self._uploaded == fp:
+            return _abi.PB_OK
+        return self.upload(problem, fp)
+
     def clear(self):
-        self._uploaded_id = None
+        self._uploaded = self._counts = None
 
     def close(self):
         if self._ctx:
@@ -169,19 +215,23 @@ class SolverWorkspace:
         """Adjoint of the last solve on this workspace: dl/d(u, v) per observation ([num_observations, 2])
         for a loss with gradients dl/dC ([num_cameras, 3]) and dl/dX ([num_tracks, 3]) at the refined state.
         with_control=True also returns dl/dP of every control point's surveyed position ([num_control_points, 3])."""
+        # pb_hip_backward reads and writes arrays sized by the problem on the context: refuse any other problem
+        if self._uploaded is None or self._uploaded != problem_fingerprint(problem):
+            raise ValueError("backward() needs the problem of the last solve on this workspace")
+        nc, nt, no, ncp = self._counts
         dc = None if dl_dcenter is None else np.ascontiguousarray(dl_dcenter, dtype=np.float64).reshape(-1, 3)
         dp = None if dl_dpoint is None else np.ascontiguousarray(dl_dpoint, dtype=np.float64).reshape(-1, 3)
-        if dc is not None and dc.shape[0] != problem.num_cameras:
+        if dc is not None and dc.shape[0] != nc:
             raise ValueError("dl_dcenter must have one row per camera")
-        if dp is not None and dp.shape[0] != problem.num_tracks:
+        if dp is not None and dp.shape[0] != nt:
             raise ValueError("dl_dpoint must have one row per track")
-        out = np.zeros((problem.num_observations, 2))
-        ctl = np.zeros((max(1, problem.num_control_points), 3)) if with_control else None
+        out = np.zeros((no, 2))
+        ctl = np.zeros((max(1, ncp), 3)) if with_control else None
         st = self._lib.pb_hip_backward(self._ctx, _abi.ptr(dc, C.c_double), _abi.ptr(dp, C.c_double),
                                        _abi.ptr(out, C.c_double), _abi.ptr(ctl, C.c_double))
         if st != _abi.PB_OK:
             raise RuntimeError(f"pb_hip_backward: {_abi.STATUS_NAMES.get(st, st)}: {self.last_error()}")
-        return (out, ctl[: problem.num_control_points]) if with_control else out
+        return (out, ctl[:ncp]) if with_control else out
 
     def quality(self, qopts) -> dict:
         """Post-solve quality of the last solve on this workspace (pb_hip_quality): the report fields plus
@@ -239,19 +289,36 @@ class Solver:
             result.backendMessage = f"no gfx950 device at index {options.backend.plaMatrixDevice}"
             return result
         ws = workspace or SolverWorkspace(options.backend.plaMatrixDevice)
-        if ws._uploaded_id != id(problem):
-            st = ws.upload(problem)
-            if st != _abi.PB_OK:
-                result.status = SolveStatus.InvalidInput
-                result.backendMessage = ws.last_error()
-                result.abi_status = st
-                return result
+        st = ws.ensure_uploaded(problem)
+        if st != _abi.PB_OK:
+            result.status = SolveStatus.InvalidInput
+            result.backendMessage = ws.last_error()
+            result.abi_status = st
+            return result
         copts = to_c(options, precision)
         rep = _abi.SolveReportC()
         flag = None
         if cancel is not None:
             flag = np.array([1 if cancel else 0], dtype=np.int32)
-        st = lib.pb_hip_solve(ws.handle, C.byref(copts), C.byref(rep), _abi.ptr(flag, C.c_int32))
+        callback = None
+        user_cb = options.solver.progressCallback
+        if user_cb is not None:
+            def _progress(summary_ptr, _user):
+                sm = summary_ptr.contents
+                try:
+                    go = user_cb(IterationSummary(sm.current_iteration, sm.max_iterations, sm.average_rms_pixels,
+                                                  sm.valid_point_count))
+                except Exception:  # a raising callback cancels, like a false return
+                    return 0
+                return 0 if go is False else 1
+
+            callback = _abi.PROGRESS_FN(_progress)
+            lib.pb_hip_set_progress(ws.handle, callback, None)
+        try:
+            st = lib.pb_hip_solve(ws.handle, C.byref(copts), C.byref(rep), _abi.ptr(flag, C.c_int32))
+        finally:
+            if callback is not None:
+                lib.pb_hip_set_progress(ws.handle, _abi.PROGRESS_FN(), None)
         result.abi_status = st
         result.report = rep.as_dict()
         result.status = SolveStatus(rep.solve_status)

@@ -74,17 +74,68 @@ int main(void) {
                        C.sizeof(_abi.QualityReportC), _abi.QualityReportC.message.offset]
 
 
+# Reference defaults written out from include/plabundle/options.h (SolverControlOptions :58-75, CalibrationOptions
+# :77-120, ConstraintOptions :122-135, BackendOptions :137-164), keyed by the pb_solve_options field that carries
+# them. A default changed on either side fails here.
+REFERENCE_DEFAULTS = {
+    "max_iterations": 20,                      # options.h:60 maxIterations
+    "robust_loss": 0,                          # options.h:61 ImageRobustLoss::LeastSquares
+    "robust_loss_scale": 1.0,                  # options.h:62 imageRobustLossScalePixels
+    "armijo_coefficient": 1.0e-3,              # options.h:63 referenceArmijoCoefficient
+    "line_search_steps": 34,                   # options.h:64 referenceLineSearchSteps
+    "use_online_schur": 0,                     # options.h:65 useReferenceOnlineSchur = false
+    "max_initial_track_rms": 100.0,            # options.h:69 maxInitialTrackRms
+    "refine_camera_pose": 1,                   # options.h:79 refineCameraPose
+    # options.h:80-86: only refineSharedHighOrderDistortion is on, which refines nothing without the radial
+    # switch (src/backend.cpp:8-47), so no slot is refined by default
+    "shared_intrinsic_mask": 0,
+    "min_focal_scale": 0.5,                    # options.h:91 minSharedFocalScale
+    "max_focal_scale": 4.0,                    # options.h:92 maxSharedFocalScale
+    "min_aspect_scale": 0.85,                  # options.h:93
+    "max_aspect_scale": 1.18,                  # options.h:94
+    "max_principal_offset_fraction": 0.08,     # options.h:95
+    "focal_prior_sigma": 0.35,                 # options.h:96
+    "principal_prior_sigma_fraction": 0.04,    # options.h:97
+    "aspect_prior_sigma": 0.08,                # options.h:98
+    # options.h:99-107 maxShared{RadialK1,K2,K3,TangentialP1,P2}Abs, SkewFraction, RadialK4, TangentialP3, P4
+    "max_abs": [0.35, 0.35, 0.35, 0.02, 0.02, 0.10, 0.35, 0.35, 0.35],
+    # options.h:108-116 the matching prior sigmas
+    "prior_sigma": [0.15, 0.15, 0.15, 0.01, 0.01, 0.02, 0.15, 0.15, 0.15],
+    "low_order_distortion_scale": 1.0,         # options.h:117 sharedLowOrderDistortionScale
+    "use_transition_prior": 0,                 # options.h:118 useReferenceCalibrationTransitionPrior
+    "previous_intrinsic_mask": 0,              # options.h:119 referencePreviousIntrinsicParameterMask{}
+    "laser_plane_weight": 1.0,                 # options.h:124
+    "laser_huber_delta": 0.2,                  # options.h:125
+    "control_point_weight": 1.0,               # options.h:128
+    "control_point_huber_delta": 0.2,          # options.h:129
+    "device": 0,                               # options.h:152 plaMatrixDevice
+    "precision": 0,                            # options.h:153 enablePlaMatrixMixedPrecision = false
+    "preconditioner_cluster_size": 1,          # options.h:164 plaMatrixPreconditionerClusterSize
+}
+
+
 def test_default_options_are_the_reference_defaults(lib):
+    o = _abi.default_options(lib)
+    for name, want in REFERENCE_DEFAULTS.items():
+        got = getattr(o, name)
+        if isinstance(want, list):
+            assert list(got) == want, name
+        else:
+            assert got == want, name
+    # the oracle's defaults (test infrastructure) carry the same reference values
     from oracle import oracle as O
 
-    o = _abi.default_options(lib)
     ref = O.default_options()
-    for name, _ in _abi.SolveOptionsC._fields_:
-        a, b = getattr(o, name), getattr(ref, name)
-        if hasattr(a, "__len__"):
-            assert list(a) == list(b), name
-        elif name not in ("precision",):
-            assert a == b, name
+    for name, want in REFERENCE_DEFAULTS.items():
+        got = getattr(ref, name)
+        assert (list(got) if isinstance(want, list) else got) == want, name
+    # the structured Python options flatten to the same values
+    from plabundle_amd import options as PO
+
+    flat = PO.to_c(PO.SolveOptions())
+    for name, want in REFERENCE_DEFAULTS.items():
+        got = getattr(flat, name)
+        assert (list(got) if isinstance(want, list) else got) == pytest.approx(want), name
 
 
 def test_invalid_calls_fail_cleanly_without_a_device(lib):

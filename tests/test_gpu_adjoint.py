@@ -33,6 +33,7 @@ def _setup(loss, intrinsics=False):
     p = S.ring_problem(seed=11, num_cameras=8, num_points=60, max_views=5)
     o = PO.SolveOptions()
     o.solver.imageRobustLoss = loss
+    o.solver.useReferenceOnlineSchur = True
     o.solver.maxIterations = 80
     o.backend.stepRmsTolerance = 1e-13
     if intrinsics:

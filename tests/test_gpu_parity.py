@@ -99,6 +99,7 @@ def test_native_three_camera_scene(ws, oracle_mod):
     p = S.native_three_cameras()
     o = PO.SolveOptions()
     o.solver.maxIterations = 8
+    o.solver.useReferenceOnlineSchur = True  # as the reference scene (test_native_problem.cpp:84)
     r = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
     assert r.report["online_schur_used"] == 1 and r.usable(), r.backendMessage
     assert r.finalCost < r.initialCost
@@ -125,14 +126,17 @@ def test_full_solve_fp64_matches_oracle(ws, oracle_mod, intrinsics):
     np.testing.assert_array_equal(r.track_valid, st["track_valid"])
 
 
-def test_full_solve_mixed_within_north_star_tolerance(ws, oracle_mod):
-    _say("case solve mixed aerial-small")
+@pytest.mark.parametrize("route", [PO.ReducedSolver.Dense, PO.ReducedSolver.Pcg])
+def test_full_solve_mixed_within_north_star_tolerance(ws, oracle_mod, route):
+    _say(f"case solve mixed aerial-small route={route.name}")
     p, o = _aerial()
+    o.backend.reducedSolver = route
     r = Solver().solve(p, o, ws, precision=PO.Precision.Mixed)
     ref = oracle_mod.Oracle(p, threads=4).solve(PO.to_c(o))
     assert r.usable(), r.backendMessage
     rel = abs(r.finalCost - ref["final_cost"]) / ref["final_cost"]
-    _say(f"mixed rel cost error {rel:.3e} iterations {r.report['iterations']} vs {ref['iterations']}")
+    _say(f"mixed rel cost error {rel:.3e} iterations {r.report['iterations']} vs {ref['iterations']} "
+         f"({r.report['linear_solver_name']}, seeds {r.report['mixed_seed_steps']})")
     assert rel <= 1e-6
 
 
@@ -176,8 +180,10 @@ def test_aerial_states_within_stated_tolerances(ws, oracle_mod):
     c, rdeg, pt, mm = _state_differences(ost, _state(r64))
     _say(f"  fp64 vs oracle: centre {c:.2e} m, rotation {rdeg:.2e} deg, point {pt:.2e} m, mismatches {mm}")
     assert c <= 1e-6 and rdeg <= 1e-5 and pt <= 1e-5 and mm == 0
+    o.backend.reducedSolver = PO.ReducedSolver.Pcg  # the truncated-Newton PCG stop of the FP32-assembly mode
     rm = Solver().solve(p, o, ws, precision=PO.Precision.Mixed)
     oi = S.aerial_options(PO.SolveOptions())
+    oi.backend.reducedSolver = PO.ReducedSolver.Pcg
     oi.backend.pcgModelEta = 0.1
     oi.backend.pcgRelativeTolerance = 1e-3
     oi.backend.pcgMaxIterations = max(200, 3 * rm.report["primary_block_count"] // 5)
@@ -216,6 +222,7 @@ def test_pcg_graph_replay_matches_direct_launches(ws, precision):
     # parameters change (graph recaptured)
     _say(f"case PCG graph replay vs direct launches precision={precision.name}")
     p, o = _aerial(n=(12, 12, 12), pts=2000)
+    o.backend.reducedSolver = PO.ReducedSolver.Pcg
     o.backend.pcgGraphs = False
     direct = Solver().solve(p, o, ws, precision=precision)
     assert direct.usable(), direct.backendMessage

@@ -11,6 +11,7 @@ def test_reference_native_scene(oracle_mod):
     orc = oracle_mod.Oracle(p, threads=1)
     o = oracle_mod.default_options()
     o.max_iterations = 8
+    o.use_online_schur = 1  # the reference scene opts in (test_native_problem.cpp:84)
     rep = orc.solve(o)
     assert rep["online_schur_used"] == 1
     assert rep["solution_usable"] == 1 and rep["solve_status"] in (1, 2)
