@@ -1,19 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark: time-to-converge and LM iter/s of the MI355X frame-BA backend.
 
-Workload: aerial block, 1000 cameras in 3 strips at 300 m, 200k points on an fBm
-terrain, ~1.5M observations (seed 2), Cauchy 1 px, shared focal + Brown
-distortion refined over 4 groups, 20 LM iterations (the reference default), online
-point Schur. A "step" is one complete solve from the initial state with the inputs
-resident in HBM (the problem structure is uploaded once, before the timed region).
+Workload: aerial block, 1000 cameras in 3 strips at 300 m, 200k points on an fBm terrain, ~1.5M observations
+(seed 2), Cauchy 1 px, shared focal + Brown distortion refined over 4 groups, online point Schur. A "step" is one
+complete solve from the initial state to the reference's convergence stop (step RMS <= 1e-6,
+BundleAdjustPlaMatrix.cpp:446-466,524-526; at most 400 LM iterations) with the inputs resident in HBM (the problem
+structure is uploaded once, before the timed region).
 
-Reported: the mixed-precision path (FP32 Jacobians/Schur/PCG, FP64 cost and
-acceptance) as `value` (LM iter/s) and `ms_per_step` (time-to-converge), the plain
-FP64 GPU path beside it, the relative final-cost error against the FP64 CPU oracle
-(golden result in tests/golden/aerial_oracle.json), the HBM roofline of the dominant
-kernel (HIP events inside the library, on its stream), the CPU oracle timed on the
-same workload on all host cores (and a one-thread sample), and the differences of the
-GPU refined states from the oracle's (the metrics of comparePhotoAlignmentOutcomes).
+Reported: the mixed-precision path (FP32 Jacobians/Schur, FP64 cost and acceptance; FP64 assembly once FP32 steps
+reach their rounding floor) as `value` (LM iter/s) and `ms_per_step` (time-to-converge), the plain FP64 GPU path
+beside it (both with the dense reduced solve), the time each needs to bring the objective within 1e-6 of the
+converged FP64 oracle's, the reference's GPU algorithm (FP64 block-Jacobi PCG to 2e-10) and the truncated-Newton
+FP32 PCG configuration over the reference's 20-iteration budget, the relative final-cost error and the refined
+states against the converged oracle (tests/golden), the roofline of the dominant kernel family (HIP events inside
+the library, on its stream), the CPU oracle timed on the same workload on all host cores (and a one-thread sample),
+and the batched-window, adjoint and quality legs.
 """
 import argparse
 import hashlib
@@ -26,6 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+FAMILIES = ("linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost", "back_substitution",
+            "dense_factor", "dense_solve")
+MFMA_FAMILIES = ("dense_factor", "dense_solve")
 METRIC = "time-to-converge + LM iter/s, 1.5M obs, mixed vs FP64 GPU at <=1e-6 rel cost"
 
 
@@ -173,6 +178,7 @@ def survey_block_leg(strips: int, cameras_per_strip: int, num_points: int, steps
     def run(precision, inexact=False, profile=False):
         o = S.large_aerial_options(PO.SolveOptions())
         o.backend.profileKernels = profile
+        o.backend.mixedSwitchDecrease = -1.0  # the truncated-Newton FP32 PCG configuration throughout
         if inexact:
             o.backend.pcgRelativeTolerance = 1e-3
             o.backend.pcgModelEta = 0.1
@@ -231,27 +237,44 @@ def survey_block_leg(strips: int, cameras_per_strip: int, num_points: int, steps
     return out
 
 
+def state_differences(ref, r):
+    """Metrics of comparePhotoAlignmentOutcomes (src/photo_alignment.cpp:225-318) against the converged golden
+    state: largest centre distance, largest rotation angle (of R_ref^T R), largest point distance over the sampled
+    tracks valid on both sides, mean height difference, and sampled valid-flag mismatches."""
+    import numpy as np
+
+    tr = np.einsum("ij,ij->i", ref["cam_rotation"], r.cam_rotation)
+    rot = np.degrees(np.arccos(np.clip(0.5 * (tr - 1.0), -1.0, 1.0)))
+    idx = ref["point_index"]
+    va, vb = ref["track_valid"] != 0, r.track_valid[idx] != 0
+    both = va & vb
+    pd = np.linalg.norm(ref["track_point"][both] - r.track_point[idx][both], axis=1)
+    gi = ref["group_intrinsics"]
+    f = r.cam_intrinsics[ref["group_cameras"], 0]
+    return {"max_camera_center_difference_m": float(np.linalg.norm(ref["cam_center"] - r.cam_center, axis=1).max()),
+            "mean_camera_height_difference_m": float(np.mean(r.cam_center[:, 2] - ref["cam_center"][:, 2])),
+            "max_camera_rotation_difference_deg": float(rot.max()),
+            "max_point_difference_m_sampled": float(pd.max()) if pd.size else 0.0,
+            "max_focal_relative_difference": float(np.max(np.abs(f - gi[:, 0]) / np.abs(gi[:, 0]))),
+            "valid_track_mask_mismatches_sampled": int(np.count_nonzero(va != vb))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=3, help="timed converged mixed solves")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--max-iterations", type=int, default=400,
+                    help="LM budget of the converged solves (the reference's stop ends them long before)")
+    ap.add_argument("--pcg-iterations", type=int, default=20,
+                    help="LM iterations of the FP64 PCG and truncated-Newton legs (0 = skip)")
     ap.add_argument("--cpu-iterations", type=int, default=20,
-                    help="LM iterations of the CPU oracle run on all host cores (20: the whole workload, whose state "
-                         "the GPU states are compared with)")
+                    help="LM iterations of the CPU oracle run on all host cores")
     ap.add_argument("--cpu-single-iterations", type=int, default=1,
                     help="LM iterations of the one-thread CPU oracle sample (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="reduced problem for quick checks (not a bench line)")
-    ap.add_argument("--mixed-rtol", type=float, default=0.0, help="PCG relative tolerance of the mixed path (0 = default)")
-    ap.add_argument("--mixed-eta", type=float, default=-1.0,
-                    help="quadratic-model PCG stop of the mixed path (<0 = default, 0 = off)")
-    ap.add_argument("--mixed-pcg-max", type=int, default=0,
-                    help="PCG iteration cap per LM step of the mixed path (0 = default)")
-    ap.add_argument("--fp64-inexact-steps", type=int, default=1,
-                    help="FP64 solves with the mixed path's PCG stopping rule (isolates the precision effect)")
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint and quality timing")
     ap.add_argument("--batch-windows", type=int, default=512,
                     help="windows of the batched local-BA leg (0 = skip)")
     ap.add_argument("--batch-fp64", type=int, default=1, help="also solve the batch in FP64")
@@ -268,6 +291,8 @@ def main():
     ap.add_argument("--survey-points", type=int, default=2_000_000)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only copy this many MiB device-to-device three times (the counter calibration run)")
+    ap.add_argument("--profile-only", choices=("mixed", "fp64"), default=None,
+                    help="only the warm-up and one converged solve of this path (profiler runs)")
     args = ap.parse_args()
     if args.copy_calibration_mb > 0:
         copy_calibration(args.copy_calibration_mb << 20)
@@ -276,6 +301,8 @@ def main():
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this backend runs on one MI355X; "
               "the multi-GPU sharding plan is design-only)", file=sys.stderr)
         sys.exit(2)
+
+    import numpy as np
 
     from plabundle_amd import options as PO
     from plabundle_amd import synthetic as S
@@ -302,76 +329,117 @@ def main():
     ws = SolverWorkspace(0)
     solver = Solver()
 
-    truncation_cap = {}
-
-    def solve(precision, profile, inexact=False):
+    def solve(precision, max_iterations=args.max_iterations, route=PO.ReducedSolver.Auto, profile=False,
+              trace=False, configure=None):
         o = S.aerial_options(PO.SolveOptions())
+        o.solver.maxIterations = max_iterations
+        o.backend.reducedSolver = route
         o.backend.profileKernels = profile
-        if precision == PO.Precision.Mixed or inexact:
-            if args.mixed_rtol > 0:
-                o.backend.pcgRelativeTolerance = args.mixed_rtol
-            elif inexact:
-                o.backend.pcgRelativeTolerance = 1e-3
-            o.backend.pcgModelEta = args.mixed_eta if args.mixed_eta >= 0 else 0.1
-            if args.mixed_pcg_max > 0:
-                o.backend.pcgMaxIterations = args.mixed_pcg_max
-            elif inexact and "cap" in truncation_cap:
-                o.backend.pcgMaxIterations = truncation_cap["cap"]  # the mixed path's default cap
+        if configure:
+            configure(o)
+        points = []
+        if trace:
+            def progress(sm):
+                points.append((sm.currentIteration, sm.averageRmsPixels, time.perf_counter()))
+                return True
+
+            o.solver.progressCallback = progress
         t0 = time.perf_counter()
         r = solver.solve(problem, o, ws, precision=precision)
-        wall = time.perf_counter() - t0
         if not r.usable():
             raise RuntimeError(f"solve failed: {r.status.name} {r.backendMessage}")
-        return r, wall
+        return r, [(it, rms * rms * r.report["observation_count"], tt - t0) for it, rms, tt in points]
+
+    def brief(r):
+        rp = r.report
+        return (f"{rp['iterations']} it ({rp['accepted_steps']} accepted), solve {rp['solve_seconds']:.4f}s, "
+                f"cost {r.finalCost:.12g}, {rp['linear_solver_name']}, status {r.status.name}")
 
     for i in range(args.warmup):
-        r, wall = solve(PO.Precision.Mixed, False)
-        log(f"warmup {i}: mixed {r.report['iterations']} it, solve {r.report['solve_seconds']:.4f}s "
-            f"(setup {r.report['setup_seconds']:.2f}s), cost {r.finalCost:.10g}")
+        for prec in (PO.Precision.Mixed, PO.Precision.Fp64):
+            r, _ = solve(prec, max_iterations=2)
+            log(f"warmup {i} {prec.name}: {brief(r)} (setup {r.report['setup_seconds']:.2f}s)")
 
+    if args.profile_only:
+        prec = PO.Precision.Mixed if args.profile_only == "mixed" else PO.Precision.Fp64
+        r, _ = solve(prec)
+        log(f"profile-only {prec.name}: {brief(r)}")
+        print(json.dumps({"profile_only": args.profile_only, "iterations": r.report["iterations"],
+                          "solve_seconds": r.report["solve_seconds"]}), flush=True)
+        return
+
+    # timed converged solves: the mixed path (the headline), then the FP64 path
     mixed = []
     for i in range(args.steps):
-        r, wall = solve(PO.Precision.Mixed, True)
+        r, _ = solve(PO.Precision.Mixed, profile=(i == 0))
         mixed.append(r)
-        log(f"step {i}: mixed {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
-            f"solve {r.report['solve_seconds']:.4f}s, cost {r.finalCost:.10g}")
-    stats = {}
-    for fam in ("linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost", "back_substitution"):
-        stats[fam] = ws.kernel_stats(fam)
+        log(f"step {i}: mixed {brief(r)}")
+    stats = {fam: ws.kernel_stats(fam) for fam in FAMILIES}
+    fp64, _ = solve(PO.Precision.Fp64, profile=True)
+    stats64 = {fam: ws.kernel_stats(fam) for fam in FAMILIES}
+    log(f"fp64: {brief(fp64)}")
+    seed, _ = solve(PO.Precision.Fp32Seed)
+    log(f"fp32-seed (reference mixed mode): {brief(seed)}; seeds accepted {seed.report['mixed_seed_steps']}, "
+        f"rejected {seed.report['dense_fallbacks']}")
 
-    fp64 = []
-    for i in range(args.fp64_steps):
-        r, wall = solve(PO.Precision.Fp64, False)
-        fp64.append(r)
-        log(f"fp64 {i}: {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
-            f"solve {r.report['solve_seconds']:.4f}s (setup {r.report['setup_seconds']:.2f}s), cost {r.finalCost:.10g}")
+    # time to bring the objective within 1e-6 of the converged oracle's (traced runs: a host callback per step)
+    golden = gstate = None
+    gpath = os.path.join(ROOT, "tests", "golden", "aerial_oracle.json")
+    if not args.small and os.path.exists(gpath):
+        golden = json.load(open(gpath))
+        if golden.get("fingerprint") != fingerprint(problem) or "converged_run" not in golden:
+            log("golden mismatch: the generated problem differs from the golden one")
+            golden = None
+        else:
+            gstate = dict(np.load(os.path.join(ROOT, "tests", "golden", "aerial_oracle_state.npz")))
+    f_star = golden["converged_run"]["final_cost"] if golden else None
 
-    # the FP64 leg with the mixed path's stopping rule uses the mixed default cap max(200, 0.6 P) too
-    truncation_cap["cap"] = max(200, 3 * mixed[0].report["primary_block_count"] // 5)
-    fp64i = []
-    for i in range(args.fp64_inexact_steps):
-        r, wall = solve(PO.Precision.Fp64, False, inexact=True)
-        fp64i.append(r)
-        log(f"fp64 inexact {i}: {r.report['iterations']} it, {r.report['linear_iterations']} PCG it, "
-            f"solve {r.report['solve_seconds']:.4f}s, cost {r.finalCost:.10g}")
+    def time_to_tolerance(points, tol=1e-6):
+        if f_star is None:
+            return None
+        for it, cost, tt in points:
+            if (cost - f_star) / f_star <= tol:
+                return {"seconds": round(tt, 6), "iteration": it}
+        return None
 
-    # adjoint at the last mixed solve's state: l = sum |C - T|^2, gradients for every observation
-    adjoint = None
+    _, tr_m = solve(PO.Precision.Mixed, trace=True)
+    _, tr_64 = solve(PO.Precision.Fp64, trace=True)
+    ttt_m, ttt_64 = time_to_tolerance(tr_m), time_to_tolerance(tr_64)
+    log(f"time to 1e-6 of the converged oracle cost: mixed {ttt_m}, fp64 {ttt_64}")
+
+    # the reference's GPU algorithm (FP64 block-Jacobi PCG to 2e-10) and the truncated-Newton FP32 PCG
+    # configuration, over the reference's 20-iteration budget
+    pcg = trunc = None
+    if args.pcg_iterations > 0:
+        rp, _ = solve(PO.Precision.Fp64, max_iterations=args.pcg_iterations, route=PO.ReducedSolver.Pcg)
+        log(f"fp64 PCG ({args.pcg_iterations} it budget): {brief(rp)}")
+        pcg = {"iterations": rp.report["iterations"], "solve_s": round(rp.report["solve_seconds"], 4),
+               "lm_iter_per_s": round(rp.report["iterations"] / rp.report["solve_seconds"], 3),
+               "pcg_iterations": rp.report["linear_iterations"], "final_cost": rp.finalCost}
+
+        def truncated(o):
+            o.backend.mixedSwitchDecrease = -1.0  # FP32 to the end
+
+        rt, _ = solve(PO.Precision.Mixed, max_iterations=args.pcg_iterations, route=PO.ReducedSolver.Pcg,
+                      configure=truncated)
+        log(f"mixed truncated-Newton PCG ({args.pcg_iterations} it budget): {brief(rt)}")
+        trunc = {"iterations": rt.report["iterations"], "solve_s": round(rt.report["solve_seconds"], 4),
+                 "lm_iter_per_s": round(rt.report["iterations"] / rt.report["solve_seconds"], 3),
+                 "pcg_iterations": rt.report["linear_iterations"], "final_cost": rt.finalCost,
+                 "rel_cost_err_vs_converged_oracle": abs(rt.finalCost - f_star) / f_star if f_star else None}
+
+    # adjoint at the converged mixed state: l = sum |C - T|^2, gradients for every observation
+    adjoint = quality = None
     if not args.no_adjoint:
-        import numpy as np
-
-        r = solve(PO.Precision.Mixed, False)[0]
+        r = mixed[-1]
+        solve(PO.Precision.Mixed)
         T = r.cam_center + np.random.default_rng(5).normal(0.0, 0.05, r.cam_center.shape)
         t0 = time.perf_counter()
         g = ws.backward(problem, 2.0 * (r.cam_center - T))
         adjoint = {"seconds": round(time.perf_counter() - t0, 4), "loss": "sum |C - T|^2 over refined centres",
-                   "gradient_entries": int(g.size), "nonzero": int(np.count_nonzero(g))}
+                   "state": "converged mixed solve", "gradient_entries": int(g.size),
+                   "nonzero": int(np.count_nonzero(g)), "finite": bool(np.all(np.isfinite(g)))}
         log(f"adjoint: {adjoint}")
-
-    # post-solve quality on the device after a mixed solve (finalizeBundleAdjustResult + image quality gate)
-    quality = None
-    if not args.no_adjoint:
-        r = solve(PO.Precision.Mixed, False)[0]
         qo = PO.quality_to_c(S.aerial_options(PO.SolveOptions()))
         times = []
         for _ in range(3):
@@ -387,41 +455,43 @@ def main():
 
     mixed_t = sum(r.report["solve_seconds"] for r in mixed) / len(mixed)
     mixed_it = sum(r.report["iterations"] for r in mixed) / len(mixed)
-    fp64_t = sum(r.report["solve_seconds"] for r in fp64) / len(fp64) if fp64 else None
-    fp64_it = sum(r.report["iterations"] for r in fp64) / len(fp64) if fp64 else None
+    fp64_t, fp64_it = fp64.report["solve_seconds"], fp64.report["iterations"]
 
-    # Dominant kernel family by estimated time in the timed region: the average timed launch times the number
-    # of launches. PCG iterations are timed per chunk of back-to-back launches (the stop chunk excluded), so
-    # their launch count is the iteration count of the timed solves; every other family is timed per launch.
-    launches_run = {k: v["launches"] for k, v in stats.items()}
-    launches_run["pcg_iteration"] = sum(r.report["linear_iterations"] for r in mixed)
-    avg = {k: (v["total_ms"] / v["launches"] if v["launches"] else 0.0) for k, v in stats.items()}
-    dom = max(stats, key=lambda k: avg[k] * launches_run[k])
-    s = stats[dom]
-    avg_ms = avg[dom]
-    achieved = s["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    traffic = profiled_traffic(dom, fingerprint(problem)) if not args.small else None
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "avg_launch_ms": round(avg_ms, 6), "timed_launches": s["launches"], "launches": launches_run[dom],
-                "bytes_per_launch": s["bytes_per_launch"],
-                "families": {k: {"launches": launches_run[k], "avg_ms": round(avg[k], 6),
-                                 "est_total_ms": round(avg[k] * launches_run[k], 3),
-                                 "GBps": round(v["bytes_per_launch"] / (avg[k] * 1e-3) / 1e9, 1) if avg[k] > 0 else 0.0}
-                             for k, v in stats.items()}}
+    # Roofline of the dominant kernel family of the timed mixed solve (average timed launch x launches). The dense
+    # factorisation is MFMA work (n^3 / 3 flops per factorisation; FP64 once the solve has moved to FP64 assembly),
+    # the other families move memory (algorithmic bytes).
+    def family_line(st):
+        out = {}
+        for k, v in st.items():
+            avg = v["total_ms"] / v["launches"] if v["launches"] else 0.0
+            rate = v["bytes_per_launch"] / (avg * 1e-3) if avg > 0 else 0.0
+            out[k] = {"launches": v["launches"], "avg_ms": round(avg, 6), "total_ms": round(v["total_ms"], 3),
+                      ("TFLOPs" if k in MFMA_FAMILIES else "GBps"): round(rate / (1e12 if k in MFMA_FAMILIES else 1e9), 2)}
+        return out
 
-    golden = None
-    gpath = os.path.join(ROOT, "tests", "golden", "aerial_oracle.json")
-    if not args.small and os.path.exists(gpath):
-        golden = json.load(open(gpath))
-        if golden.get("fingerprint") != fingerprint(problem):
-            log("golden fingerprint mismatch: the generated problem differs from the golden one")
-            golden = None
-    rel_oracle = abs(mixed[0].finalCost - golden["final_cost"]) / golden["final_cost"] if golden else None
-    rel_oracle64 = abs(fp64[0].finalCost - golden["final_cost"]) / golden["final_cost"] if golden and fp64 else None
-    rel_fp64 = abs(mixed[0].finalCost - fp64[0].finalCost) / fp64[0].finalCost if fp64 else None
-    fp64i_t = sum(r.report["solve_seconds"] for r in fp64i) / len(fp64i) if fp64i else None
-    rel_oracle64i = abs(fp64i[0].finalCost - golden["final_cost"]) / golden["final_cost"] if golden and fp64i else None
+    fam = family_line(stats)
+    dom = max(fam, key=lambda k: fam[k]["total_ms"])
+    s_dom = stats[dom]
+    avg_ms = fam[dom]["avg_ms"]
+    if dom in MFMA_FAMILIES:
+        achieved = s_dom["bytes_per_launch"] / (avg_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 5),
+                    "flops_per_launch": s_dom["bytes_per_launch"]}
+    else:
+        achieved = s_dom["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "bytes_per_launch": s_dom["bytes_per_launch"]}
+    roofline.update({"traffic": profiled_traffic(dom, fingerprint(problem)) if not args.small else None,
+                     "avg_launch_ms": avg_ms, "timed_launches": s_dom["launches"], "families": fam,
+                     "families_fp64_solve": family_line(stats64)})
+
+    rel = lambda r: abs(r.finalCost - f_star) / f_star if f_star else None  # noqa: E731
+    state_vs_oracle = None
+    if gstate is not None:
+        state_vs_oracle = {"oracle": "converged FP64 oracle (tests/golden/aerial_oracle_state.npz)",
+                           "mixed": state_differences(gstate, mixed[0]), "fp64": state_differences(gstate, fp64)}
+        log(f"state vs oracle: {state_vs_oracle}")
 
     batch = None
     if args.batch_windows > 0 and not args.small:
@@ -433,14 +503,11 @@ def main():
             log(f"batch leg failed: {batch['error']}")
 
     cpu = None
-    state_vs_oracle = None
     if not args.no_cpu_baseline:
-        import numpy as np
-
         from oracle import oracle as O
 
-        # the FP64 oracle on all host cores over the full workload (20 LM iterations): the CPU baseline, and the
-        # refined state the GPU states are compared with; then a one-thread sample of its first iterations
+        # the FP64 oracle on all host cores over the reference's 20-iteration budget (~7 s on 16 threads), and a
+        # one-thread sample of its first iteration
         o = PO.to_c(S.aerial_options(PO.SolveOptions()))
         o.max_iterations = args.cpu_iterations
         threads = O.lib().oracle_max_threads()
@@ -450,8 +517,9 @@ def main():
         wall = time.perf_counter() - t0
         cpu = {"value": round(rep["iterations"] / rep["solve_seconds"], 4), "unit": "LM iter/s", "cores": threads,
                "kind": "port",
-               "sample": f"FP64 oracle, {rep['iterations']} LM iterations of the same workload "
-                         f"({rep['solve_seconds']:.2f}s solve, {wall:.2f}s with setup)"}
+               "sample": f"FP64 oracle, first {rep['iterations']} LM iterations of the same workload "
+                         f"({rep['solve_seconds']:.2f}s solve, {wall:.2f}s with setup); to converge it needs "
+                         f"{golden['converged_run']['iterations'] if golden else '?'} iterations"}
         if args.cpu_single_iterations > 0:
             o1 = PO.to_c(S.aerial_options(PO.SolveOptions()))
             o1.max_iterations = args.cpu_single_iterations
@@ -460,40 +528,6 @@ def main():
                                     "sample": f"first {rep1['iterations']} LM iterations "
                                               f"({rep1['solve_seconds']:.2f}s solve)"}
         log(f"cpu baseline: {cpu}")
-        if rep["iterations"] == mixed[0].report["iterations"]:
-            # metrics of comparePhotoAlignmentOutcomes (src/photo_alignment.cpp:225-318): largest centre distance,
-            # largest rotation difference (angle of R_ref^T R), largest point distance over tracks valid in both,
-            # tracks whose valid flag differs
-            ost = orc.state()
-
-            def compare(r, st=ost):
-                tr = np.einsum("ij,ij->i", st["cam_rotation"], r.cam_rotation)
-                rot = np.degrees(np.arccos(np.clip(0.5 * (tr - 1.0), -1.0, 1.0)))
-                va, vb = st["track_valid"] != 0, r.track_valid != 0
-                both = va & vb
-                pd = np.linalg.norm(st["track_point"][both] - r.track_point[both], axis=1)
-                f0 = st["cam_intrinsics"][:, 0]
-                return {"max_camera_center_difference_m": float(np.linalg.norm(st["cam_center"] - r.cam_center,
-                                                                               axis=1).max()),
-                        "mean_camera_height_difference_m": float(np.mean(r.cam_center[:, 2] - st["cam_center"][:, 2])),
-                        "max_focal_relative_difference": float(np.max(np.abs(r.cam_intrinsics[:, 0] - f0) /
-                                                                      np.abs(f0))),
-                        "max_camera_rotation_difference_deg": float(rot.max()),
-                        "max_point_difference_m": float(pd.max()) if pd.size else 0.0,
-                        "valid_track_mask_mismatches": int(np.count_nonzero(va != vb))}
-
-            state_vs_oracle = {"oracle_final_cost": rep["final_cost"], "mixed": compare(mixed[0])}
-            if fp64:
-                state_vs_oracle["fp64"] = compare(fp64[0])
-            if fp64i:
-                state_vs_oracle["fp64_same_stop_as_mixed"] = compare(fp64i[0])
-                # the precision effect alone: mixed against FP64 with the same stopping rule
-                ref = fp64i[0]
-                state_vs_oracle["mixed_vs_fp64_same_stop"] = compare(
-                    mixed[0], {"cam_rotation": ref.cam_rotation, "cam_center": ref.cam_center,
-                               "cam_intrinsics": ref.cam_intrinsics, "track_point": ref.track_point,
-                               "track_valid": ref.track_valid})
-            log(f"state vs oracle: {state_vs_oracle}")
         orc.close()
 
     out = {
@@ -507,7 +541,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (Jacobians/Schur/PCG) + fp64 (cost, acceptance, point inverses)",
+        "dtype": "fp32 (Jacobians/Schur until the FP32 rounding floor) + fp64 (cost, acceptance, reduced solve, the "
+                 "converging steps)",
         "data": "synthetic",
         "config": {
             "workload": ("aerial block (reduced check size)" if args.small else
@@ -518,25 +553,33 @@ def main():
             "points": problem.num_tracks,
             "loss": "Cauchy 1 px",
             "refined_intrinsics": "shared focal + k1,k2,k3,p1,p2 over 4 groups",
-            "max_iterations": 20,
-            "path": "online point Schur, explicit 6x6-block reduced system, block-Jacobi PCG",
+            "stop": f"step RMS <= 1e-6 (reference), at most {args.max_iterations} LM iterations",
+            "path": "online point Schur, dense reduced solve (64x64-tile MFMA Cholesky)",
         },
         "time_to_converge_s": round(mixed_t, 6),
         "lm_iterations": mixed_it,
-        "pcg_iterations": mixed[0].report["linear_iterations"],
-        "fp64_time_to_converge_s": round(fp64_t, 6) if fp64_t else None,
-        "fp64_lm_iter_per_s": round(fp64_it / fp64_t, 4) if fp64_t else None,
-        "fp64_pcg_iterations": fp64[0].report["linear_iterations"] if fp64 else None,
-        "speedup_mixed_vs_fp64": round(fp64_t / mixed_t, 4) if fp64_t else None,
+        "mixed_fp32_accepted_steps_reported": bool(mixed[0].report["mixed_precision_used"]),
+        "mixed_message": mixed[0].report["message"],
+        "fp64_time_to_converge_s": round(fp64_t, 6),
+        "fp64_lm_iterations": fp64_it,
+        "fp64_lm_iter_per_s": round(fp64_it / fp64_t, 4),
+        "speedup_mixed_vs_fp64": round(fp64_t / mixed_t, 4),
+        "time_to_1e-6_cost": {"mixed": ttt_m, "fp64": ttt_64,
+                              "speedup_mixed_vs_fp64": round(ttt_64["seconds"] / ttt_m["seconds"], 4)
+                              if ttt_m and ttt_64 else None},
         "final_cost": mixed[0].finalCost,
-        "rel_cost_err_vs_oracle": rel_oracle,
-        "fp64_rel_cost_err_vs_oracle": rel_oracle64,
-        "rel_cost_err_vs_fp64_gpu": rel_fp64,
-        "meets_1e-6": (rel_oracle <= 1e-6) if rel_oracle is not None else None,
-        "fp64_inexact_time_to_converge_s": round(fp64i_t, 6) if fp64i_t else None,
-        "fp64_inexact_pcg_iterations": fp64i[0].report["linear_iterations"] if fp64i else None,
-        "fp64_inexact_rel_cost_err_vs_oracle": rel_oracle64i,
-        "speedup_mixed_vs_fp64_inexact": round(fp64i_t / mixed_t, 4) if fp64i_t else None,
+        "rel_cost_err_vs_oracle": rel(mixed[0]),
+        "fp64_rel_cost_err_vs_oracle": rel(fp64),
+        "meets_1e-6": (rel(mixed[0]) <= 1e-6) if f_star else None,
+        "oracle_converged": {"iterations": golden["converged_run"]["iterations"],
+                             "final_cost": f_star} if golden else None,
+        "fp32_seed_mode": {"time_to_converge_s": round(seed.report["solve_seconds"], 6),
+                           "iterations": seed.report["iterations"], "seeds_accepted": seed.report["mixed_seed_steps"],
+                           "seeds_rejected": seed.report["dense_fallbacks"],
+                           "mixed_precision_used": seed.report["mixed_precision_used"],
+                           "rel_cost_err_vs_oracle": rel(seed)},
+        "fp64_pcg_reference_algorithm": pcg,
+        "mixed_truncated_newton_pcg": trunc,
         "adjoint": adjoint,
         "quality": quality,
         "batch": batch,

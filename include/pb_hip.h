@@ -249,6 +249,10 @@ typedef struct pb_solve_options {
     /* plaMatrixPreconditionerClusterSize (options.h:164,285; valid 1..16, src/options.cpp:393-394): consecutive
      * primary units whose diagonal blocks form one block of the PCG preconditioner (1 = block-Jacobi). */
     int32_t preconditioner_cluster_size;
+    /* PB_PRECISION_MIXED: an accepted FP32 step that lowers the objective by less than this fraction, or that
+     * meets the step-RMS stop, hands the solve to FP64 assembly for the rest (0 = 1e-6, the FP32 rounding floor of
+     * the assembled model; negative = FP32 to the end, stopping only when a line search fails). */
+    double mixed_switch_decrease;
 } pb_solve_options;
 
 /* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */
@@ -351,8 +355,11 @@ pb_status pb_hip_get_masks(pb_ctx* ctx, int32_t* camera_block, int32_t* intrinsi
                            int32_t* calibration_group_by_camera, int32_t* track_block, int32_t* active_track);
 
 /* Kernel timing collected when options.profile_kernels was set.
- * name: kernel family ("linearize_points", "camera_direct", "schur_blocks",
- * "pcg_spmv", "cost"). Returns launch count and total milliseconds. */
+ * name: kernel family ("linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost",
+ * "back_substitution", "dense_factor", "dense_solve"). Returns launch count, total milliseconds and the algorithmic
+ * HBM bytes of one launch; for the two dense families a "launch" is one factorisation (gather included) or one
+ * pair of triangular sweeps, and bytes_per_launch receives their algorithmic floating-point operations
+ * (n^3 / 3 and 2 n^2 for the reduced order n). */
 pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, double* total_ms,
                               double* bytes_per_launch);
 

@@ -51,6 +51,8 @@ def lib():
     L.oracle_compute_step.restype = C.c_int
     L.oracle_cost_after_step.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
     L.oracle_cost_after_step.restype = C.c_double
+    L.oracle_decrease.argtypes = [C.c_void_p, _dp, _dp]
+    L.oracle_decrease.restype = C.c_double
     L.oracle_default_quality_options.argtypes = [C.POINTER(_abi.QualityOptionsC)]
     L.oracle_default_quality_options.restype = None
     L.oracle_quality.argtypes = [C.c_void_p, C.POINTER(_abi.QualityOptionsC), C.POINTER(_abi.QualityReportC), _dp,
@@ -212,6 +214,12 @@ class Oracle:
         primary = np.ascontiguousarray(primary, dtype=np.float64)
         elim = np.ascontiguousarray(elim, dtype=np.float64)
         return lib().oracle_cost_after_step(self.ctx, _p(primary), _p(elim), alpha)
+
+    def decrease(self, primary, elim) -> float:
+        """Analytic decrease of the objective at the initial state along (primary, elim) (after prepare())."""
+        primary = np.ascontiguousarray(primary, dtype=np.float64)
+        elim = np.ascontiguousarray(elim, dtype=np.float64)
+        return lib().oracle_decrease(self.ctx, _p(primary), _p(elim))
 
     def quality(self, qopts=None) -> dict:
         """finalizeBundleAdjustResult + the image quality gate on the last solve (per-track arrays included)."""

@@ -2631,6 +2631,32 @@ int oracle_compute_step(oracle_ctx* ctx, double damping, double* primary, double
     }
 }
 
+// Analytic decrease of the objective at the initial state along an arbitrary step (primary [P*13], elim
+// [Ne*3]): primary . b_c + sum_p elim_p . b_p, the form the back-substitution uses for the solved step
+// (ReferenceSchur.cpp:1004-1007,1060-1062). Returns NaN on failure.
+double oracle_decrease(oracle_ctx* ctx, const double* primary, const double* elim)
+{
+    Oracle& o = ctx->o;
+    if (!o.prepared) return std::numeric_limits<double>::quiet_NaN();
+    try
+    {
+        Reduced R = buildReduced(o, o.init, 0.0);
+        double dec = 0.0;
+        for (std::size_t i = 0; i < R.directRhs.size(); ++i) dec += primary[i] * R.directRhs[i];
+        for (std::size_t t = 0; t < o.pb.tracks.size(); ++t)
+        {
+            const int b = o.act.trackBlock[t];
+            if (!o.act.activeTrack[t] || b < 0) continue;
+            for (int r = 0; r < kE; ++r) dec += o.backSub[t].rhs[r] * elim[static_cast<std::size_t>(b) * kE + r];
+        }
+        return dec;
+    }
+    catch (const std::exception&)
+    {
+        return std::numeric_limits<double>::quiet_NaN();
+    }
+}
+
 // Objective at the initial state moved by alpha * (primary, elim) through the retraction.
 double oracle_cost_after_step(oracle_ctx* ctx, const double* primary, const double* elim, double alpha)
 {

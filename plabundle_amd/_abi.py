@@ -121,6 +121,7 @@ class SolveOptionsC(C.Structure):
         ("reduced_solver", C.c_int32),
         ("dense_max_rows", C.c_int32),
         ("preconditioner_cluster_size", C.c_int32),
+        ("mixed_switch_decrease", C.c_double),
     ]
 
 

@@ -909,7 +909,8 @@ namespace
     };
 
     const char* const kFamilies[] = {"linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost",
-                                     "back_substitution"};
+                                     "back_substitution", "dense_factor", "dense_solve"};
+    constexpr int kFamilyDenseFactor = 6, kFamilyDenseSolve = 7;
     // pinned readback area (in doubles): scalars of a build / step, flags, PCG scalars, PCG flags
     constexpr int kPinFlags = 32;
     constexpr int kPinPcg = 40;
@@ -934,6 +935,10 @@ namespace
     // (relative): FP32 steps carry the early, large decreases; near the minimum the FP32 rounding of the
     // reduced system limits them, so the last iterations of every window assemble in FP64
     constexpr double kMixedSwitchDecrease = 1e-2;
+    // a mixed single solve moves to FP64 assembly once an accepted FP32 step lowers the objective by less than this
+    // (relative): about ten times the FP32 rounding of the assembled model, below which FP32 steps stop being
+    // reliable descent steps along the weakly determined directions
+    constexpr double kMixedSolveSwitchDecrease = 1e-6;
 
     // Host side of a batched solve: window ranges of the combined input and the per-window counts.
     struct BatchHost
@@ -1853,8 +1858,11 @@ namespace
         }
         DenseOutcome out;
         auto t0 = Clock::now();
-        DL::template gather<T>(P, W.S, A, nt, s);
-        DL::factor(A, nt, D.st, s);
+        {
+            Span sp(ctx, kFamilyDenseFactor);
+            DL::template gather<T>(P, W.S, A, nt, s);
+            DL::factor(A, nt, D.st, s);
+        }
         DL::norm2(W.rhs, n, D.norm + 1, s);
         int32_t* hw = reinterpret_cast<int32_t*>(ctx->pinned + kPinDense);
         double* hn = ctx->pinned + kPinDense + 4;
@@ -1872,7 +1880,10 @@ namespace
         const double bn = std::sqrt(hn[1]);
         auto ts = Clock::now();
         DL::toDense(W.rhs, n, npad, v, s);
-        DL::solve(A, nt, v, D.st, s);
+        {
+            Span sp(ctx, kFamilyDenseSolve);
+            DL::solve(A, nt, v, D.st, s);
+        }
         DL::accumulate(D.x, v, n, true, s);
         if (rep)
         {
@@ -1899,7 +1910,10 @@ namespace
             }
             auto ts = Clock::now();
             DL::toDense(D.r, n, npad, v, s);
-            DL::solve(A, nt, v, D.st, s);
+            {
+                Span sp(ctx, kFamilyDenseSolve);
+                DL::solve(A, nt, v, D.st, s);
+            }
             DL::accumulate(D.x, v, n, false, s);
             if (rep)
             {
@@ -2077,6 +2091,7 @@ namespace
         const int batch = o.armijo_batch > 0 ? std::min(o.armijo_batch, pbk::kMaxAlpha) : 4;
         const double modelEta = o.pcg_model_eta >= 0.0 ? o.pcg_model_eta : (mixed ? 0.1 : 0.0);
         const double rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
+        const double switchDecrease = o.mixed_switch_decrease == 0.0 ? kMixedSolveSwitchDecrease : o.mixed_switch_decrease;
         rep->linear_tolerance = relTol;
         ctx->lastScalarBytes = sizeof(T);
         rep->online_schur_used = 1;
@@ -2207,12 +2222,22 @@ namespace
                 if (armijo && std::isfinite(cand) && cand < current)
                 {
                     Lr::accept(P, W, acceptedIndex, alpha, s);
+                    const double before = current;
                     current = cand;
                     ++accepted;
                     if (fp32) ++pg.fp32Accepted;
                     damping /= 10.0;
                     converged = alpha * rmsG <= rmsTol && alpha * rmsP <= rmsTol;
                     relinearize = !converged;
+                    // FP32 assembly has reached its rounding floor, or its step would stop the solve: the rest
+                    // is assembled in FP64 (the stop is only taken on FP64 steps)
+                    if (fp32 && switchDecrease > 0.0 && (converged || before - cand < switchDecrease * before))
+                    {
+                        converged = false;
+                        pg.fallback = true;
+                        if (!reportProgress(ctx, o, pg)) cancelled = true;
+                        return;
+                    }
                 }
                 else
                 {
@@ -2330,6 +2355,7 @@ namespace
         const int batch = o.armijo_batch > 0 ? std::min(o.armijo_batch, pbk::kMaxAlpha) : 4;
         const double modelEta = o.pcg_model_eta >= 0.0 ? o.pcg_model_eta : (mixed ? 0.1 : 0.0);
         const double rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
+        const double switchDecrease = o.mixed_switch_decrease == 0.0 ? kMixedSolveSwitchDecrease : o.mixed_switch_decrease;
         constexpr double kProbe = 1e-4;
         rep->linear_tolerance = relTol;
         rep->online_schur_used = 0;
@@ -2461,12 +2487,20 @@ namespace
                     // the state of the last length tried; the stop test uses alpha, which is half of it when no
                     // length met the Armijo condition (:505-529)
                     Lr::accept(P, W, candIndex, candAlpha, s);
+                    const double before = current;
                     current = cand;
                     ++accepted;
                     if (fp32) ++pg.fp32Accepted;
                     damping /= 10.0;
                     converged = alpha * rmsG <= rmsTol && alpha * rmsP <= rmsTol;
                     relinearize = !converged;
+                    if (fp32 && switchDecrease > 0.0 && (converged || before - cand < switchDecrease * before))
+                    {
+                        converged = false;
+                        pg.fallback = true;  // the rest is assembled in FP64 (as in runSolve)
+                        if (!reportProgress(ctx, o, pg)) cancelled = true;
+                        return;
+                    }
                 }
                 else
                 {
@@ -3779,6 +3813,12 @@ pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, 
             case 5:  // cross blocks U + units + point z, L, bp + dp out
                 b = P.nEntries * (18 * t + 4) + P.nElim * (24.0 + 6 * t + 24 + 24);
                 break;
+            case kFamilyDenseFactor:  // floating-point operations of the Cholesky factorisation, n^3 / 3
+                b = std::pow(6.0 * P.nUnits, 3) / 3.0;
+                break;
+            case kFamilyDenseSolve:  // floating-point operations of the two triangular sweeps, 2 n^2
+                b = 2.0 * std::pow(6.0 * P.nUnits, 2);
+                break;
             default:
                 break;
             }
@@ -4013,3 +4053,4 @@ pb_status pb_hip_dense_solve(pb_ctx* ctx, int32_t n, int32_t fp32, const double*
 const char* pb_hip_last_error(const pb_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
 
 }  // extern "C"
+

@@ -157,6 +157,7 @@ class BackendOptions:
     pcgGraphs: bool = True  # replay PCG iterations from a captured HIP graph
     reducedSolver: ReducedSolver = ReducedSolver.Auto
     denseMaxRows: int = 0  # 0: 16384 rows
+    mixedSwitchDecrease: float = 0.0  # Mixed: FP32 -> FP64 assembly below this relative decrease (0: 1e-6, < 0: never)
 
 
 @dataclass
@@ -257,6 +258,7 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.reduced_solver = int(b.reducedSolver)
     o.dense_max_rows = int(b.denseMaxRows)
     o.preconditioner_cluster_size = int(b.plaMatrixPreconditionerClusterSize)
+    o.mixed_switch_decrease = float(b.mixedSwitchDecrease)
     k = options.constraints
     o.laser_plane_weight = float(k.laserPlaneWeight)
     o.laser_huber_delta = float(k.laserHuberDeltaMeters)

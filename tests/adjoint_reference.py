@@ -57,18 +57,22 @@ def constraint_weight_matrix(r, delta):
 def adjoint_reference(O, problem, state, masks, prep, copts, dl_dC, dl_dX=None, priors=None, with_control=False):
     """dl/d(u, v) per observation ([num_observations, 2]) at `state` (dict of the oracle / workspace download);
     with_control=True also returns dl/dP of every control point ([num_control_points, 3]). Problems with survey
-    constraints use the default parameterisation (world-frame rotation, unscaled point step)."""
+    constraints use the default parameterisation (world-frame rotation, unscaled point step).
+
+    H lambda = dl/dtheta is solved through the point blocks: H = [[Hcc, W], [W^T, blockdiag(Hpp)]] is assembled
+    as its primary part (dense), one primary x 3 cross block and one 3 x 3 block per point, and reduced to the
+    primary system S = Hcc - sum W Hpp^-1 W^T (the same solution as the dense solve of H, without forming it)."""
     ref_param = bool(copts.use_online_schur) and problem.num_control_points == 0 and problem.num_laser_planes == 0
     P = prep["primary_block_count"]
     Ne = prep["track_block_count"]
-    n = P * P13 + 3 * Ne
-    H = np.zeros((n, n))
-    rows = []
+    PP = P * P13
     mask = copts.shared_intrinsic_mask
+    ids, Jcs, Jps, Ms, tbs = [], [], [], [], []
     for t in range(problem.num_tracks):
         if not masks["active_track"][t]:
             continue
         X = state["track_point"][t]
+        tb = masks["track_block"][t]
         for i in range(problem.track_offset[t], problem.track_offset[t + 1]):
             c = problem.obs_camera[i]
             w_obs = problem.obs_weight[i] / problem.obs_scale[i] ** 2
@@ -79,56 +83,72 @@ def adjoint_reference(O, problem, state, masks, prep, copts, dl_dC, dl_dX=None, 
                               problem.obs_scale[i], copts.robust_loss, copts.robust_loss_scale, ref_param,
                               mask if masks["intrinsic_block_by_camera"][c] >= 0 else 0)
             assert lin is not None
-            J = np.zeros((2, n))
+            Jc = np.zeros((2, PP))
             cb = masks["camera_block"][c]
             if cb >= 0:
-                J[:, cb * P13: cb * P13 + 6] = lin["Jpose"]
+                Jc[:, cb * P13: cb * P13 + 6] = lin["Jpose"]
             ib = masks["intrinsic_block_by_camera"][c]
             if ib >= 0:
-                J[:, ib * P13: ib * P13 + P13] = lin["Jint"]
-            tb = masks["track_block"][t]
-            if tb >= 0:
-                J[:, P * P13 + 3 * tb: P * P13 + 3 * tb + 3] = lin["Jp"]
-            M = weight_matrix(lin["residual"], w_obs, lin["weight"], copts.robust_loss, copts.robust_loss_scale)
-            H += J.T @ M @ J
-            rows.append((i, J, M))
+                Jc[:, ib * P13: ib * P13 + P13] = lin["Jint"]
+            ids.append(i)
+            Jcs.append(Jc)
+            Jps.append(lin["Jp"] if tb >= 0 else np.zeros((2, 3)))
+            Ms.append(weight_matrix(lin["residual"], w_obs, lin["weight"], copts.robust_loss, copts.robust_loss_scale))
+            tbs.append(tb)
+    Jc = np.array(Jcs).reshape(-1, 2, PP)
+    Jp = np.array(Jps).reshape(-1, 2, 3)
+    M = np.array(Ms).reshape(-1, 2, 2)
+    tb = np.array(tbs, dtype=np.int64)
+    MJc, MJp = M @ Jc, M @ Jp
+    Hcc = np.tensordot(Jc, MJc, axes=([0, 1], [0, 1]))
+    on = tb >= 0
+    W = np.zeros((max(Ne, 1), PP, 3))
+    Hpp = np.zeros((max(Ne, 1), 3, 3))
+    np.add.at(W, tb[on], np.einsum("nki,nkj->nij", Jc[on], MJp[on]))
+    np.add.at(Hpp, tb[on], np.einsum("nki,nkj->nij", Jp[on], MJp[on]))
     cons = []
-    for t, Jc, target, delta, index in constraint_rows(O, problem, copts):
-        tb = masks["track_block"][t]
+    for t, Jk, target, delta, index in constraint_rows(O, problem, copts):
+        b = masks["track_block"][t]
         if not masks["active_track"][t]:
             continue
-        r = Jc @ (state["track_point"][t] - target)
-        M = constraint_weight_matrix(r, delta)
-        if tb >= 0:
-            cols = slice(P * P13 + 3 * tb, P * P13 + 3 * tb + 3)
-            H[cols, cols] += Jc.T @ M @ Jc
-        cons.append((tb, Jc, M, index))
+        r = Jk @ (state["track_point"][t] - target)
+        Mk = constraint_weight_matrix(r, delta)
+        if b >= 0:
+            Hpp[b] += Jk.T @ Mk @ Jk
+        cons.append((b, Jk, Mk, index))
     for idx, h in (priors or {}).items():
-        H[idx, idx] += h
-    g = np.zeros(n)
+        Hcc[idx, idx] += h
+    gc = np.zeros(PP)
     for c in range(problem.num_cameras):
         cb = masks["camera_block"][c]
         if cb >= 0:
-            g[cb * P13 + 3: cb * P13 + 6] = dl_dC[c]
+            gc[cb * P13 + 3: cb * P13 + 6] = dl_dC[c]
+    gp = np.zeros((max(Ne, 1), 3))
     if dl_dX is not None:
         from oracle import oracle as Oo
 
         for t in range(problem.num_tracks):
-            tb = masks["track_block"][t]
-            if tb >= 0:
+            b = masks["track_block"][t]
+            if b >= 0:
                 sc = Oo.point_scale(state["track_point"][t]) if ref_param else 1.0
-                g[P * P13 + 3 * tb: P * P13 + 3 * tb + 3] = sc * dl_dX[t]
-    keep = np.diag(H) != 0.0
-    lam = np.zeros(n)
-    lam[keep] = np.linalg.solve(H[np.ix_(keep, keep)], g[keep])
+                gp[b] = sc * dl_dX[t]
+    Hinv = np.linalg.inv(Hpp[:Ne]) if Ne else np.zeros((0, 3, 3))
+    V = W[:Ne] @ Hinv                                  # W Hpp^-1 per point
+    S = Hcc - np.tensordot(V, W[:Ne], axes=([0, 2], [0, 2]))
+    rhs = gc - np.einsum("tij,tj->i", V, gp[:Ne])
+    keep = np.diag(Hcc) != 0.0
+    lam_c = np.zeros(PP)
+    lam_c[keep] = np.linalg.solve(S[np.ix_(keep, keep)], rhs[keep])
+    lam_p = np.einsum("tij,tj->ti", Hinv, gp[:Ne] - np.einsum("tij,i->tj", W[:Ne], lam_c))
     out = np.zeros((problem.num_observations, 2))
-    for i, J, M in rows:
-        out[i] = M @ (J @ lam)
+    lp = np.zeros((len(tb), 3))
+    lp[on] = lam_p[tb[on]]
+    vals = np.einsum("nij,nj->ni", M, Jc @ lam_c + np.einsum("nkj,nj->nk", Jp, lp))
+    out[np.array(ids, dtype=np.int64)] = vals
     if not with_control:
         return out
     dctl = np.zeros((problem.num_control_points, 3))
-    for tb, Jc, M, index in cons:
-        if index >= 0 and tb >= 0:
-            lx = lam[P * P13 + 3 * tb: P * P13 + 3 * tb + 3]
-            dctl[index] = Jc.T @ (M @ (Jc @ lx))
+    for b, Jk, Mk, index in cons:
+        if index >= 0 and b >= 0:
+            dctl[index] = Jk.T @ (Mk @ (Jk @ lam_p[b]))
     return out, dctl

@@ -244,3 +244,39 @@ def test_pcg_graph_replay_matches_direct_launches(ws, precision):
     assert a.report["linear_iterations"] == b.report["linear_iterations"]
     assert a.finalCost == b.finalCost
     np.testing.assert_array_equal(a.track_point, b.track_point)
+
+
+@pytest.mark.parametrize("route", [PO.ReducedSolver.Dense, PO.ReducedSolver.Pcg])
+def test_auto_anchor_with_a_gauge_scale_camera_matches_oracle(ws, oracle_mod, route):
+    # AutoAnchor without an absolute scale: camera 0 fixed, the farthest camera moves on the sphere of the baseline
+    # through its two tangent columns (Assembly.cpp:69-128, 204-206, 295-306)
+    _say(f"case AutoAnchor scale camera route={route.name}")
+    from plabundle_amd.problem import GaugePolicy
+
+    p = S.ring_problem(seed=23, num_cameras=14, num_points=400, max_views=6)
+    p.fixed_cameras = None
+    p.gauge_policy = GaugePolicy.AutoAnchor
+    o = S.ring_options(PO.SolveOptions(), max_iterations=12)
+    o.backend.reducedSolver = route
+    orc = oracle_mod.Oracle(p, threads=4)
+    ref = orc.solve(PO.to_c(o))
+    st = orc.state()
+    assert ref["gauge_scale_camera"] > 0 and ref["gauge_baseline"] > 0
+    r = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
+    assert r.usable(), r.backendMessage
+    rep = r.report
+    _say(f"  scale camera {rep['gauge_scale_camera']} baseline {rep['gauge_baseline']:.6f}; gpu {rep['iterations']} it "
+         f"{r.finalCost:.12g}, oracle {ref['iterations']} it {ref['final_cost']:.12g}")
+    assert (rep["gauge_anchor_camera"], rep["gauge_scale_camera"]) == (ref["gauge_anchor_camera"],
+                                                                         ref["gauge_scale_camera"])
+    assert rep["gauge_baseline"] == pytest.approx(ref["gauge_baseline"], rel=1e-14)
+    assert rep["iterations"] == ref["iterations"] and rep["accepted_steps"] == ref["accepted_steps"]
+    assert abs(r.finalCost - ref["final_cost"]) <= 1e-9 * ref["final_cost"]
+    np.testing.assert_allclose(r.cam_center, st["cam_center"], atol=1e-7)
+    # the scale camera stays on the sphere of the baseline around the anchor
+    a, sc = rep["gauge_anchor_camera"], rep["gauge_scale_camera"]
+    assert np.linalg.norm(r.cam_center[sc] - r.cam_center[a]) == pytest.approx(rep["gauge_baseline"], rel=1e-12)
+    masks = orc.masks()
+    gm = ws.masks(p)
+    for k in gm:
+        np.testing.assert_array_equal(gm[k], masks[k], err_msg=k)
