@@ -45,8 +45,10 @@ def ws():
 
 
 def _state_diff(ref, r, index=None):
-    tr = np.einsum("ij,ij->i", ref["cam_rotation"], r.cam_rotation)
-    rot = np.degrees(np.arccos(np.clip(0.5 * (tr - 1.0), -1.0, 1.0)))
+    # rotation angle between R and R' from ||R - R'||_F = 2 sqrt(2) sin(angle / 2): unlike
+    # acos((tr(R^T R') - 1) / 2) it keeps its precision near zero (acos has a 4e-8 rad floor in FP64)
+    fro = np.linalg.norm((ref["cam_rotation"] - r.cam_rotation).reshape(-1, 9), axis=1)
+    rot = np.degrees(2.0 * np.arcsin(np.clip(fro / (2.0 * np.sqrt(2.0)), 0.0, 1.0)))
     X = r.track_point if index is None else r.track_point[index]
     V = r.track_valid if index is None else r.track_valid[index]
     both = (ref["track_valid"] != 0) & (V != 0)
