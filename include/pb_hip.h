@@ -567,6 +567,21 @@ pb_status pb_compare_outcomes(const pb_outcome_view* reference, const pb_outcome
 pb_status pb_hip_dense_solve(pb_ctx* ctx, int32_t n, int32_t fp32, const double* A, const double* b, double* x,
                              int32_t* info, double* seconds);
 
+/* Largest system and scratch size of pb_hip_dense_solve_noalloc. */
+#define PB_NOALLOC_MAX_ROWS 1024
+#define PB_NOALLOC_SCRATCH_BYTES (8u * 1024u * 1024u + 16384u)
+
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The FP64 factor-and-solve of pb_hip_dense_solve on caller
+ * device memory: A (device, n x n column-major with leading dimension n; the lower triangle is read, A is not
+ * modified) and b (device, n doubles, overwritten by x = A^-1 b), on the context's stream, with every
+ * intermediate in the library's static per-device scratch array of PB_NOALLOC_SCRATCH_BYTES (held until the
+ * process ends). n above PB_NOALLOC_MAX_ROWS is refused with PB_ERR_INVALID. Allocates nothing; returns once the
+ * stream has finished. info (host, may be NULL) receives 0, or 1 when a pivot was not positive. Calls that overlap
+ * on one GPU share its scratch array and race. */
+pb_status pb_hip_dense_solve_noalloc(pb_ctx* ctx, int32_t n, const double* A_device, double* b_device,
+                                     int32_t* info);
+
 /* Last error message of the context (never NULL). */
 const char* pb_hip_last_error(const pb_ctx* ctx);
 

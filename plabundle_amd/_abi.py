@@ -21,6 +21,7 @@ PB_ERR_OOM = 3
 PB_ERR_NUMERIC = 4
 PB_ERR_CANCELLED = 5
 PB_ERR_UNSUPPORTED = 6
+PB_NOALLOC_MAX_ROWS = 1024  # include/pb_hip.h
 
 STATUS_NAMES = {
     PB_OK: "ok",
@@ -352,6 +353,7 @@ EXPORTED_SYMBOLS = (
     "pb_compare_outcomes",
     "pb_hip_set_progress",
     "pb_hip_dense_solve",
+    "pb_hip_dense_solve_noalloc",
     "pb_hip_last_error",
 )
 
@@ -422,6 +424,8 @@ def load_library(path: str | None = None):
     lib.pb_hip_set_progress.restype = C.c_int
     lib.pb_hip_dense_solve.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp, _ip, _dp]
     lib.pb_hip_dense_solve.restype = C.c_int
+    lib.pb_hip_dense_solve_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, _ip]
+    lib.pb_hip_dense_solve_noalloc.restype = C.c_int
     lib.pb_hip_last_error.argtypes = [C.c_void_p]
     lib.pb_hip_last_error.restype = C.c_char_p
     if path is None:

@@ -4054,3 +4054,15 @@ const char* pb_hip_last_error(const pb_ctx* ctx) { return ctx ? ctx->error.c_str
 
 }  // extern "C"
 
+// Context access for pb_hip_dense_solve_noalloc (pb_noalloc.hip): its device, its stream and the pinned readback
+// area allocated when the context was created.
+bool pbNoallocContext(pb_ctx* ctx, int* device, hipStream_t* stream, int32_t** pinnedWords)
+{
+    if (!ctx || !ctx->stream || !ctx->pinned) return false;
+    *device = ctx->device;
+    *stream = ctx->stream;
+    *pinnedWords = reinterpret_cast<int32_t*>(ctx->pinned + kPinDense);
+    return true;
+}
+
+pb_status pbNoallocFail(pb_ctx* ctx, pb_status st, const std::string& message) { return fail(ctx, st, message); }
