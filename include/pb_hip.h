@@ -105,9 +105,14 @@ typedef enum pb_precision {
 /* Reduced-system solver of each LM step (no reference option: the reference picks PlaMatrix's dense CPU or
  * PCG GPU route by backend, BundleAdjustPlaMatrixRuntime.cpp:18-56). */
 typedef enum pb_reduced_solver {
-    PB_REDUCED_AUTO = 0,   /* dense when 6 * units <= dense_max_rows, else PCG */
+    PB_REDUCED_AUTO = 0,   /* band when the pattern fits it (at least three super-blocks), else dense when
+                              6 * units <= dense_max_rows, else PCG */
     PB_REDUCED_PCG = 1,    /* block-Jacobi preconditioned CG (fused iteration) */
-    PB_REDUCED_DENSE = 2   /* dense Cholesky of the reduced system (64 x 64 tiles, MFMA trailing updates) */
+    PB_REDUCED_DENSE = 2,  /* dense Cholesky of the reduced system (64 x 64 tiles, MFMA trailing updates) */
+    PB_REDUCED_BAND = 3    /* exact block-tridiagonal solve: pose units in reverse Cuthill-McKee order cut into
+                              super-blocks of up to 128 rows, intrinsic-group units as a dense border, odd-even
+                              cyclic reduction; PB_ERR_UNSUPPORTED when the camera graph is too wide (bandwidth
+                              above 21 pose units) or the border above 256 rows */
 } pb_reduced_solver;
 
 /* Uncertainty form of a control point (plabundle::ControlPointConstraint::Uncertainty,
@@ -558,6 +563,17 @@ pb_status pb_compare_outcomes(const pb_outcome_view* reference, const pb_outcome
                               const pb_comparison_tolerance* tolerance, pb_comparison* comparison,
                               double* camera_center_difference, double* camera_rotation_difference_degrees,
                               double* point_difference, int32_t* valid_mismatch);
+
+/* Test hook of the direct reduced-system routes (no reference counterpart): the reduced system of the uploaded
+ * initial state in FP64 (damping as given) solved once by route (PB_REDUCED_DENSE or PB_REDUCED_BAND; the band
+ * route returns PB_ERR_UNSUPPORTED when the pattern does not fit it) into x [6 * units] (unit order). When not
+ * NULL: matrix [(6 units)^2] (column-major, unit order) and rhs [6 units] receive the system; info [8] receives
+ * the not-positive-definite flag, the relative residual in units of 1e-15 (clamped to 2^31 - 1), for the band
+ * route its super-block rows, super-blocks, border rows, levels and bandwidth (pose units), and the number of
+ * unknowns; arena (arena_capacity elements) receives the band route's arena after the solve. */
+pb_status pb_hip_direct_solve(pb_ctx* ctx, const pb_solve_options* options, double damping, int32_t route,
+                              double* x, double* matrix, double* rhs, int32_t* info, double* arena,
+                              int64_t arena_capacity);
 
 /* Test and measurement hook of the dense reduced solver (no reference counterpart): factors the symmetric
  * positive-definite n x n matrix A (column-major; the lower triangle is read) on the context's device in FP64

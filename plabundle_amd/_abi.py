@@ -353,6 +353,7 @@ EXPORTED_SYMBOLS = (
     "pb_compare_outcomes",
     "pb_hip_set_progress",
     "pb_hip_dense_solve",
+    "pb_hip_direct_solve",
     "pb_hip_dense_solve_noalloc",
     "pb_hip_last_error",
 )
@@ -422,6 +423,9 @@ def load_library(path: str | None = None):
     lib.pb_compare_outcomes.restype = C.c_int
     lib.pb_hip_set_progress.argtypes = [C.c_void_p, PROGRESS_FN, C.c_void_p]
     lib.pb_hip_set_progress.restype = C.c_int
+    lib.pb_hip_direct_solve.argtypes = [C.c_void_p, C.POINTER(SolveOptionsC), C.c_double, C.c_int32, _dp, _dp, _dp,
+                                        _ip, _dp, C.c_int64]
+    lib.pb_hip_direct_solve.restype = C.c_int
     lib.pb_hip_dense_solve.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp, _ip, _dp]
     lib.pb_hip_dense_solve.restype = C.c_int
     lib.pb_hip_dense_solve_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, _ip]

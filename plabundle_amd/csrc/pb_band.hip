@@ -1,0 +1,703 @@
+// pb_band.hip - block-tridiagonal reduced solve by odd-even cyclic reduction (gfx950, wave64), FP64 or FP32.
+//
+// The reduced system S of a strip or block of cameras couples each camera only with its neighbours in a
+// bandwidth-reducing order (the camera graph of an aerial block has a bandwidth of ~17 cameras in 1000), so the
+// pose rows cut into super-blocks of sp = 64 t rows form a block-tridiagonal matrix; the intrinsic-group rows form
+// a dense border coupled to all of them. Instead of the n^3 / 3 of the dense Cholesky this takes O(m sp^3):
+//   k_band_clear, k_band_gather   the super-blocks D_k, the first-level couplings K, the border columns BdT_k
+//                                 and the border block C from the block-sparse rows of S (identity padding)
+//   per level (every second remaining super-block e, all in parallel):
+//     k_band_factor               D_e = L L^T in LDS (64 x 64 tiles: one-wave tile Cholesky and inverse, tile
+//                                 products), W_e = L^-T written in place of D_e
+//     k_band_gemm (X tasks)       X = W_e^T K for the two couplings of e, XB_e = W_e^T BdT_e      (MFMA 16x16x4)
+//     k_band_gemm (U tasks)       D_s -= X^T X and BdT_s -= X^T XB for the survivors s, the new coupling
+//                                 -X_b^T X_a between the survivors, Pc_e = XB_e^T XB_e            (MFMA 16x16x4)
+//     k_band_csum                 C -= sum_e Pc_e in a fixed order
+//   k_band_final + the dense route  the last super-block with the border (order sp + nbp)
+// The solve runs the same levels forward (y_e = W_e^T (r_e - pending X^T y)) and backward
+// (x_e = W_e (y_e - X x_neighbours - XB_e x_border)), one workgroup per eliminated block and level.
+// Every reduction has a fixed order, so results are bitwise reproducible.
+#include "pb_common.h"
+#include "pb_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+namespace pbk
+{
+    namespace
+    {
+        constexpr int kT = kDenseTile;
+        constexpr int kLd = kT + 1;  // LDS tile leading dimension: buf[column][row]
+
+        template <typename T>
+        struct BandMfma;
+        // Result element e of lane l is row kLaneRow * (l >> 4) + kItemRow * e, column l & 15 of the 16 x 16 block
+        // (the two instructions lay their results out differently; measured on gfx950).
+        template <>
+        struct BandMfma<double>
+        {
+            static constexpr int kLaneRow = 1, kItemRow = 4;
+            typedef double V __attribute__((ext_vector_type(4)));
+            static __device__ __forceinline__ V op(double a, double b, V c)
+            {
+                return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+            }
+        };
+        template <>
+        struct BandMfma<float>
+        {
+            static constexpr int kLaneRow = 4, kItemRow = 1;
+            typedef float V __attribute__((ext_vector_type(4)));
+            static __device__ __forceinline__ V op(float a, float b, V c)
+            {
+                return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+            }
+        };
+
+        __device__ __forceinline__ double bandReadLane(double v, int j)
+        {
+            const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+            const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffu), j));
+            const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), j));
+            return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+        }
+        __device__ __forceinline__ float bandReadLane(float v, int j)
+        {
+            return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+        }
+
+        // 1/p from the hardware reciprocal refined by Newton's iteration (two steps in FP64, one in FP32)
+        __device__ __forceinline__ double bandRecip(double p)
+        {
+            double r = __builtin_amdgcn_rcp(p);
+            r = __builtin_fma(r, __builtin_fma(-p, r, 1.0), r);
+            return __builtin_fma(r, __builtin_fma(-p, r, 1.0), r);
+        }
+        __device__ __forceinline__ float bandRecip(float p)
+        {
+            const float r = __builtin_amdgcn_rcpf(p);
+            return __builtin_fmaf(r, __builtin_fmaf(-p, r, 1.0f), r);
+        }
+
+        // Orders LDS accesses of one wave (lanes exchange values through LDS without a workgroup barrier).
+        __device__ __forceinline__ void waveLdsFence()
+        {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+
+        // Cholesky of the 64 x 64 tile buf (lower triangle read; L written with zeros above the diagonal) by the
+        // whole workgroup, in four panels of 16 columns: wave 0 factors the panel with lane r holding row r's 16
+        // entries in registers (column j published in LDS, every lane subtracts L(r, j) L(c, j) from its entries
+        // c > j), then all waves apply the panel to the trailing columns (rank-16 update in LDS).
+        constexpr int kPanel = 16;
+        template <typename T>
+        __device__ void cholTile(T (*buf)[kLd], T* col, bool& bad)
+        {
+            const int lane = threadIdx.x & 63;
+            for (int c0 = 0; c0 < kT; c0 += kPanel)
+            {
+                if (threadIdx.x < 64)
+                {
+                    const int r = lane;
+                    T a[kPanel];
+#pragma unroll
+                    for (int q = 0; q < kPanel; ++q) a[q] = buf[c0 + q][r];
+#pragma unroll
+                    for (int j = 0; j < kPanel; ++j)
+                    {
+                        const int cj = c0 + j;
+                        T p = bandReadLane(a[j], cj);
+                        if (!(p > T(0) && p < T(INFINITY)))
+                        {
+                            bad = true;
+                            p = T(1);
+                        }
+                        const T s = sqrt(p);
+                        const T l = r > cj ? a[j] * bandRecip(s) : (r == cj ? s : T(0));
+                        a[j] = l;
+                        col[r] = l;
+                        waveLdsFence();
+#pragma unroll
+                        for (int q = j + 1; q < kPanel; ++q) a[q] -= l * col[c0 + q];
+                        waveLdsFence();
+                    }
+#pragma unroll
+                    for (int q = 0; q < kPanel; ++q) buf[c0 + q][r] = r >= c0 + q ? a[q] : T(0);
+                }
+                __syncthreads();
+                // trailing columns c >= c0 + 16, rows r >= c0 + 16 (both triangles of the trailing block)
+                const int t0 = c0 + kPanel, nt = kT - t0;
+                for (int idx = threadIdx.x; idx < nt * nt; idx += blockDim.x)
+                {
+                    const int r = t0 + idx % nt, c = t0 + idx / nt;
+                    T s = T(0);
+#pragma unroll
+                    for (int q = 0; q < kPanel; ++q) s += buf[c0 + q][r] * buf[c0 + q][c];
+                    buf[c][r] -= s;
+                }
+                __syncthreads();
+            }
+        }
+
+        // inv = L^-1 of the lower-triangular tile L (zeros above the diagonal) by the whole workgroup, in four row
+        // panels of 16: every thread subtracts the finished panels' contribution from its entries of the panel
+        // rows (inv(i, j) = [i == j] - sum_{k < i0} L(i, k) inv(k, j)), then wave 0 substitutes down the panel with
+        // lane j holding column j's 16 entries in registers.
+        template <typename T>
+        __device__ void invTile(T (*L)[kLd], T (*inv)[kLd], T* dinv)
+        {
+            if (threadIdx.x < 64) dinv[threadIdx.x] = bandRecip(L[threadIdx.x][threadIdx.x]);
+            for (int i0 = 0; i0 < kT; i0 += kPanel)
+            {
+                for (int idx = threadIdx.x; idx < kPanel * kT; idx += blockDim.x)
+                {
+                    const int i = i0 + idx % kPanel, j = idx / kPanel;
+                    T s = i == j ? T(1) : T(0);
+                    for (int k = 0; k < i0; ++k) s -= L[k][i] * inv[j][k];
+                    inv[j][i] = s;
+                }
+                __syncthreads();
+                if (threadIdx.x < 64)
+                {
+                    const int j = threadIdx.x;
+                    T x[kPanel];
+#pragma unroll
+                    for (int q = 0; q < kPanel; ++q)
+                    {
+                        T s = inv[j][i0 + q];
+#pragma unroll
+                        for (int p = 0; p < q; ++p) s -= L[i0 + p][i0 + q] * x[p];
+                        x[q] = s * dinv[i0 + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < kPanel; ++q) inv[j][i0 + q] = x[q];
+                }
+                __syncthreads();
+            }
+        }
+
+        // O = (neg ? -1 : 1) A B, or O -= A B with sub, on 64 x 64 LDS tiles; A(r, k) = transA ? a[r][k] : a[k][r],
+        // B(k, c) = transB ? b[k][c] : b[c][k]. Thread (tr, tc) of 256 forms rows 4 tr.., columns 4 tc...
+        template <typename T>
+        __device__ void tileProduct(T (*o)[kLd], T (*a)[kLd], bool transA, T (*b)[kLd], bool transB, bool sub,
+                                    bool neg)
+        {
+            const int tr = (threadIdx.x & 15) * 4, tc = (threadIdx.x >> 4) * 4;
+            T acc[4][4] = {};
+            for (int k = 0; k < kT; ++k)
+            {
+                T av[4], bv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    av[q] = transA ? a[tr + q][k] : a[k][tr + q];
+                    bv[q] = transB ? b[k][tc + q] : b[tc + q][k];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) acc[q][p] += av[q] * bv[p];
+            }
+            __syncthreads();  // every thread has read a and b before o (which may alias neither) is written
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int p = 0; p < 4; ++p)
+                {
+                    T& d = o[tc + p][tr + q];
+                    d = sub ? d - acc[q][p] : (neg ? -acc[q][p] : acc[q][p]);
+                }
+        }
+
+        template <typename T>
+        __device__ __forceinline__ void loadTile(T (*buf)[kLd], const T* M, int ld, int r0, int c0)
+        {
+            for (int idx = threadIdx.x; idx < kT * kT; idx += blockDim.x)
+            {
+                const int r = idx & 63, c = idx >> 6;
+                buf[c][r] = M[size_t(c0 + c) * ld + r0 + r];
+            }
+        }
+
+        int blocksOf(long n, int per) { return int((n + per - 1) / per); }
+    }  // namespace
+
+    // ---------------------------------------------------------------------------------------------
+    // Zero the super-blocks, the first-level couplings, the border columns and the border block; identity on
+    // the padding rows of each super-block (rows past its units) and of the border.
+    template <typename TA>
+    __global__ void k_band_clear(TA* Z, long long offD, long long offC, long long offEnd, int sp, int su, int nPose,
+                                 int nb, int nbp)
+    {
+        const long long total = offEnd - offD;
+        for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+             i += (long long)gridDim.x * blockDim.x)
+        {
+            const long long g = offD + i;
+            TA v = TA(0);
+            if (g < offD + (long long)(nPose + su - 1) / su * sp * sp)
+            {
+                const long long local = g - offD;
+                const int k = int(local / ((long long)sp * sp));
+                const int rc = int(local % ((long long)sp * sp));
+                const int r = rc % sp, c = rc / sp;
+                const int used = 6 * min(su, nPose - k * su);
+                if (r == c && r >= used) v = TA(1);
+            }
+            else if (g >= offC && g < offC + (long long)nbp * nbp)
+            {
+                const int rc = int(g - offC);
+                const int r = rc % nbp, c = rc / nbp;
+                if (r == c && r >= nb) v = TA(1);
+            }
+            Z[g] = v;
+        }
+    }
+
+    // Scatter of the stored 6 x 6 blocks (one wave per unit row): pose/pose in one super-block into D, pose rows of
+    // an odd super-block against a neighbouring super-block into the first-level coupling (rows = the odd block,
+    // eliminated first), pose rows against border columns into BdT, border against border into C. Blocks whose
+    // transpose carries the same values are skipped.
+    template <typename TW, typename TA>
+    __global__ void __launch_bounds__(256) k_band_gather(Problem P, const TW* S, BandDev B, TA* Z)
+    {
+        const int u = blockIdx.x * 4 + (threadIdx.x >> 6);
+        if (u >= P.nUnits) return;
+        const int lane = threadIdx.x & 63;
+        const int sp = B.sp, nbp = B.nbp;
+        const int bu = B.unitBlk[u], ou = B.unitOff[u];
+        for (int k = P.rowPtr[u] + lane; k < P.rowPtr[u + 1]; k += kWave)
+        {
+            const int v = P.colIdx[k];
+            const int bv = B.unitBlk[v], ov = B.unitOff[v];
+            TA* dst = nullptr;
+            int ld = 0;
+            if (bu >= 0 && bv >= 0)
+            {
+                if (bu == bv)
+                {
+                    dst = Z + B.offD + size_t(bu) * sp * sp;
+                    ld = sp;
+                }
+                else if ((bu & 1) && (bu == bv + 1 || bu + 1 == bv))
+                {
+                    dst = Z + B.offK + size_t(B.k0slot[min(bu, bv)]) * sp * sp;
+                    ld = sp;
+                }
+            }
+            else if (bu >= 0)
+            {
+                dst = Z + B.offB + size_t(bu) * sp * nbp;
+                ld = sp;
+            }
+            else if (bv < 0)
+            {
+                dst = Z + B.offC;
+                ld = nbp;
+            }
+            if (!dst) continue;
+            const TW* blk = S + size_t(k) * 36;
+#pragma unroll
+            for (int e = 0; e < 36; ++e) dst[size_t(ov + e % 6) * ld + ou + e / 6] = TA(blk[e]);
+        }
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    // One workgroup per eliminated super-block of a level: D_e = L L^T, then W_e = L^-T written over D_e.
+    // sp = 128: L00 = chol(A00), I0 = L00^-1, L10 = A10 I0^T, A11 -= L10 L10^T, L11 = chol(A11), I1 = L11^-1,
+    // L^-1 = [I0, 0; -I1 L10 I0, I1].
+    // Dynamic LDS: four 64 x 65 tiles and a 64-entry column (133 KiB in FP64 for sp = 128, above the default
+    // 64 KiB limit; the launcher raises the kernel's limit once).
+    template <typename T>
+    constexpr size_t bandFactorLds(int sp)
+    {
+        return (size_t(sp > kT ? 4 : 2) * kT * kLd + kT) * sizeof(T);
+    }
+
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_band_factor(T* Z, const BandElim* elims, long long offD, int sp,
+                                                         int32_t* words)
+    {
+        extern __shared__ __align__(16) unsigned char bandLds[];
+        T(*b0)[kLd] = reinterpret_cast<T(*)[kLd]>(bandLds);
+        T(*b1)[kLd] = b0 + kT;
+        T(*b2)[kLd] = b1 + kT;  // only with sp > 64
+        T(*b3)[kLd] = b2 + kT;
+        T* col = reinterpret_cast<T*>(b0 + (sp > kT ? 4 : 2) * kT);
+        const int e = elims[blockIdx.x].e;
+        T* M = Z + offD + size_t(e) * sp * sp;
+        bool bad = false;
+        loadTile(b0, M, sp, 0, 0);
+        if (sp > kT)
+        {
+            loadTile(b2, M, sp, kT, 0);
+            loadTile(b3, M, sp, kT, kT);
+        }
+        __syncthreads();
+        cholTile(b0, col, bad);
+        invTile(b0, b1, col);
+        if (sp > kT)
+        {
+            tileProduct(b0, b2, false, b1, true, false, false);  // L10 = A10 I0^T
+            __syncthreads();
+            tileProduct(b3, b0, false, b0, true, true, false);   // A11 -= L10 L10^T
+            __syncthreads();
+            cholTile(b3, col, bad);
+            invTile(b3, b2, col);                                 // I1
+            tileProduct(b3, b0, false, b1, false, false, false);  // L10 I0
+            __syncthreads();
+            tileProduct(b0, b2, false, b3, false, false, true);   // Linv10 = -I1 (L10 I0)
+            __syncthreads();
+        }
+        // W(r, c) = Linv(c, r)
+        for (int idx = threadIdx.x; idx < sp * sp; idx += blockDim.x)
+        {
+            const int r = idx % sp, c = idx / sp;
+            T v;
+            if (c < kT)
+                v = r < kT ? b1[r][c] : T(0);
+            else
+                v = r < kT ? b0[r][c - kT] : b2[r - kT][c - kT];
+            M[size_t(c) * sp + r] = v;
+        }
+        if (bad && threadIdx.x == 0) atomicOr(words + kDenseNotSpd, 1);
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    // One workgroup per 64 x 64 output tile of a task: out = beta out +- sum_terms P^T Q (MFMA 16x16x4, operands
+    // staged in LDS 32 rows of k at a time). Wave w forms the 32 x 32 quadrant (w >> 1, w & 1) as 2 x 2 blocks.
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_band_gemm(T* Z, const BandTask* tasks, const BandTile* tiles, int sp)
+    {
+        using V = typename BandMfma<T>::V;
+        constexpr int kStage = 32;
+        __shared__ T sa[kStage][kT + 2], sb[kStage][kT + 2];  // [k][column of P / Q]
+        const BandTile tl = tiles[blockIdx.x];
+        const BandTask tk = tasks[tl.task];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int qi = (w >> 1) * 32, qj = (w & 1) * 32;
+        const int li = lane & 15, lk = lane >> 4;
+        constexpr int kLR = BandMfma<T>::kLaneRow, kIR = BandMfma<T>::kItemRow;
+        T* out = Z + tk.out + size_t(tl.tj) * kT * tk.ldo + size_t(tl.ti) * kT;
+        V acc[2][2];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+            {
+                const T* src = out + size_t(qj + 16 * bj + li) * tk.ldo + qi + 16 * bi + kLR * lk;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[bi][bj][e] = tk.beta ? src[kIR * e] : T(0);
+            }
+        const int kEnd = tk.kTri ? min(sp, (tl.ti + 1) * kT) : sp;
+        const T sign = tk.alphaNeg ? T(-1) : T(1);
+        for (int term = 0; term < tk.terms; ++term)
+        {
+            const T* Pm = Z + (term ? tk.p1 : tk.p0) + size_t(tl.ti) * kT * sp;
+            const T* Qm = Z + (term ? tk.q1 : tk.q0) + size_t(tl.tj) * kT * sp;
+            for (int k0 = 0; k0 < kEnd; k0 += kStage)
+            {
+                __syncthreads();  // every wave is done with the previous slice
+                for (int idx = threadIdx.x; idx < kStage * kT; idx += 256)
+                {
+                    const int kk = idx % kStage, i = idx / kStage;
+                    sa[kk][i] = Pm[size_t(i) * sp + k0 + kk];
+                    sb[kk][i] = Qm[size_t(i) * sp + k0 + kk];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int kk = 0; kk < kStage; kk += 4)
+                {
+                    const T a0 = sign * sa[kk + lk][qi + li], a1 = sign * sa[kk + lk][qi + 16 + li];
+                    const T bb0 = sb[kk + lk][qj + li], bb1 = sb[kk + lk][qj + 16 + li];
+                    acc[0][0] = BandMfma<T>::op(a0, bb0, acc[0][0]);
+                    acc[0][1] = BandMfma<T>::op(a0, bb1, acc[0][1]);
+                    acc[1][0] = BandMfma<T>::op(a1, bb0, acc[1][0]);
+                    acc[1][1] = BandMfma<T>::op(a1, bb1, acc[1][1]);
+                }
+            }
+        }
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+            {
+                T* dst = out + size_t(qj + 16 * bj + li) * tk.ldo + qi + 16 * bi + kLR * lk;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) dst[kIR * e] = acc[bi][bj][e];
+            }
+    }
+
+    // C -= sum over the level's eliminated blocks e of Pc_e, in elimination order (one thread per entry).
+    template <typename T>
+    __global__ void k_band_csum(T* Z, const BandElim* elims, int count, long long offC, long long offPc, int nbp)
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= nbp * nbp) return;
+        T s = T(0);
+        for (int q = 0; q < count; ++q) s += Z[offPc + size_t(elims[q].e) * nbp * nbp + i];
+        Z[offC + i] -= s;
+    }
+
+    // F = [D_0, BdT_0; BdT_0^T, C] (order nF = sp + nbp, leading dimension nF), from the lower triangle of D_0.
+    template <typename T>
+    __global__ void k_band_final(T* Z, BandDev B)
+    {
+        const int nF = B.nF, sp = B.sp;
+        const long long total = (long long)nF * nF;
+        for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+             idx += (long long)gridDim.x * blockDim.x)
+        {
+            const int r = int(idx % nF), c = int(idx / nF);
+            T v;
+            if (r < sp && c < sp)
+                v = r >= c ? Z[B.offD + size_t(c) * sp + r] : Z[B.offD + size_t(r) * sp + c];
+            else if (r < sp)
+                v = Z[B.offB + size_t(c - sp) * sp + r];
+            else if (c < sp)
+                v = Z[B.offB + size_t(r - sp) * sp + c];
+            else
+                v = Z[B.offC + size_t(c - sp) * B.nbp + (r - sp)];
+            Z[B.offF + idx] = v;
+        }
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    // Solve helpers for one workgroup of 256 threads: matrix-vector products with a column-major matrix M
+    // (leading dimension ld) and a vector in LDS.
+    //   T-form: out_i = sum_{k < K} M(k, i) x_k (upper-triangular M with tri: k <= i)
+    //   N-form: out_i = sum_{k < K} M(i, k) x_k (upper-triangular M with tri: k >= i)
+    template <typename T>
+    __device__ __forceinline__ T gemvT(const T* M, int ld, int i, int K, bool tri, const T* x)
+    {
+        const int kEnd = tri ? min(K, i + 1) : K;
+        const T* col = M + size_t(i) * ld;
+        T s0 = T(0), s1 = T(0);
+        int k = 0;
+        for (; k + 1 < kEnd; k += 2)
+        {
+            s0 += col[k] * x[k];
+            s1 += col[k + 1] * x[k + 1];
+        }
+        if (k < kEnd) s0 += col[k] * x[k];
+        return s0 + s1;
+    }
+
+    template <typename T>
+    __device__ __forceinline__ T gemvN(const T* M, int ld, int i, int K, bool tri, const T* x)
+    {
+        T s0 = T(0), s1 = T(0);
+        int k = tri ? i : 0;
+        for (; k + 1 < K; k += 2)
+        {
+            s0 += M[size_t(k) * ld + i] * x[k];
+            s1 += M[size_t(k + 1) * ld + i] * x[k + 1];
+        }
+        if (k < K) s0 += M[size_t(k) * ld + i] * x[k];
+        return s0 + s1;
+    }
+
+    // Forward level: r_e = v_e - sum of its pending contributions, y_e = W_e^T r_e into v_e, then the
+    // contributions X_slot^T y_e to its two survivors and XB_e^T y_e to the border.
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_band_forward(T* Z, BandDev B, const BandElim* elims)
+    {
+        __shared__ T r[kBandMaxRows], y[kBandMaxRows];
+        const BandElim el = elims[blockIdx.x];
+        const int sp = B.sp, nbp = B.nbp;
+        const int i = threadIdx.x;
+        T* v = Z + B.offV + size_t(el.e) * sp;
+        if (i < sp)
+        {
+            T s = v[i];
+            for (int p = el.pendBegin; p < el.pendEnd; ++p) s -= Z[B.offPV + size_t(B.pend[p]) * sp + i];
+            r[i] = s;
+        }
+        __syncthreads();
+        const T* W = Z + B.offD + size_t(el.e) * sp * sp;
+        if (i < sp)
+        {
+            const T yi = gemvT(W, sp, i, sp, true, r);
+            y[i] = yi;
+            v[i] = yi;
+        }
+        __syncthreads();
+        if (i < sp)
+        {
+            Z[B.offPV + size_t(el.slotA) * sp + i] = gemvT(Z + B.offX + size_t(el.slotA) * sp * sp, sp, i, sp, false, y);
+            if (el.slotB >= 0)
+                Z[B.offPV + size_t(el.slotB) * sp + i] =
+                    gemvT(Z + B.offX + size_t(el.slotB) * sp * sp, sp, i, sp, false, y);
+        }
+        if (i < nbp) Z[B.offPB + size_t(el.e) * nbp + i] = gemvT(Z + B.offXB + size_t(el.e) * sp * nbp, sp, i, sp, false, y);
+    }
+
+    // The last system's right-hand side: block 0 minus its pending contributions, the border minus every
+    // eliminated block's contribution (in block order). One workgroup.
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_band_final_rhs(T* Z, BandDev B)
+    {
+        const int sp = B.sp, nbp = B.nbp;
+        for (int i = threadIdx.x; i < sp + nbp; i += blockDim.x)
+        {
+            T s;
+            if (i < sp)
+            {
+                s = Z[B.offV + i];
+                for (int p = B.lastPendBegin; p < B.lastPendEnd; ++p) s -= Z[B.offPV + size_t(B.pend[p]) * sp + i];
+            }
+            else
+            {
+                const int j = i - sp;
+                s = Z[B.offV + size_t(B.m) * sp + j];
+                for (int e = 1; e < B.m; ++e) s -= Z[B.offPB + size_t(e) * nbp + j];
+            }
+            Z[B.offFV + i] = s;
+        }
+    }
+
+    // The last system's solution back into the band vector (block 0 and the border).
+    template <typename T>
+    __global__ void k_band_final_x(T* Z, BandDev B)
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= B.sp + B.nbp) return;
+        const size_t dst = i < B.sp ? size_t(i) : size_t(B.m) * B.sp + (i - B.sp);
+        Z[B.offV + dst] = Z[B.offFV + i];
+    }
+
+    // Backward level: x_e = W_e (y_e - X_a x_a - X_b x_b - XB_e x_border).
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_band_backward(T* Z, BandDev B, const BandElim* elims)
+    {
+        __shared__ T xa[kBandMaxRows], xb[kBandMaxRows], xB[kBandMaxBorder], t[kBandMaxRows];
+        const BandElim el = elims[blockIdx.x];
+        const int sp = B.sp, nbp = B.nbp;
+        const int i = threadIdx.x;
+        const T* V = Z + B.offV;
+        if (i < sp)
+        {
+            xa[i] = V[size_t(el.a) * sp + i];
+            xb[i] = el.b >= 0 ? V[size_t(el.b) * sp + i] : T(0);
+        }
+        if (i < nbp) xB[i] = V[size_t(B.m) * sp + i];
+        __syncthreads();
+        if (i < sp)
+        {
+            T s = V[size_t(el.e) * sp + i];
+            s -= gemvN(Z + B.offX + size_t(el.slotA) * sp * sp, sp, i, sp, false, xa);
+            if (el.slotB >= 0) s -= gemvN(Z + B.offX + size_t(el.slotB) * sp * sp, sp, i, sp, false, xb);
+            if (nbp > 0) s -= gemvN(Z + B.offXB + size_t(el.e) * sp * nbp, sp, i, nbp, false, xB);
+            t[i] = s;
+        }
+        __syncthreads();
+        if (i < sp) Z[B.offV + size_t(el.e) * sp + i] = gemvN(Z + B.offD + size_t(el.e) * sp * sp, sp, i, sp, true, t);
+    }
+
+    template <typename TA>
+    __global__ void k_band_to(BandDev B, const double* src, TA* Z)
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= B.vN) return;
+        const int j = B.srcOf[i];
+        Z[B.offV + i] = j >= 0 ? TA(src[j]) : TA(0);
+    }
+
+    template <typename TA>
+    __global__ void k_band_from(BandDev B, double* x, const TA* Z, int overwrite)
+    {
+        const int j = blockIdx.x * blockDim.x + threadIdx.x;
+        if (j >= B.n) return;
+        const double v = double(Z[B.offV + B.rowOf[j]]);
+        x[j] = overwrite ? v : x[j] + v;
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    template <typename TA>
+    template <typename TW>
+    void BandLauncher<TA>::gather(const Problem& P, const TW* S, const BandView& B, TA* Z, hipStream_t s)
+    {
+        const BandDev& D = B.d;
+        const long long clearEnd = D.offC + (long long)D.nbp * D.nbp;  // D, K, B, C are contiguous
+        k_band_clear<TA><<<2048, 256, 0, s>>>(Z, D.offD, D.offC, clearEnd, D.sp, D.su, D.nPose, D.nb, D.nbp);
+        k_band_gather<TW, TA><<<blocksOf(P.nUnits, 4), 256, 0, s>>>(P, S, D, Z);
+    }
+
+    template <typename TA>
+    void BandLauncher<TA>::factor(const BandView& B, TA* Z, const DenseState& st, hipStream_t s)
+    {
+        const BandDev& D = B.d;
+        const size_t lds = bandFactorLds<TA>(D.sp);
+        static const bool ldsRaised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_band_factor<TA>),
+                                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                          int(bandFactorLds<TA>(kBandMaxRows))) == hipSuccess;
+        if (!ldsRaised && lds > 64 * 1024) throw std::runtime_error("hipFuncSetAttribute (band factor LDS)");
+        DenseLauncher<TA>::begin(st, s);
+        for (int l = 0; l < B.levels; ++l)
+        {
+            const int e0 = B.elimStart[l], ne = B.elimStart[l + 1] - e0;
+            k_band_factor<TA><<<ne, 256, lds, s>>>(Z, D.elims + e0, D.offD, D.sp, st.words);
+            const int x0 = B.tileXStart[l], nx = B.tileXStart[l + 1] - x0;
+            if (nx > 0) k_band_gemm<TA><<<nx, 256, 0, s>>>(Z, D.tasks, D.tiles + x0, D.sp);
+            const int u0 = B.tileUStart[l], nu = B.tileUStart[l + 1] - u0;
+            if (nu > 0) k_band_gemm<TA><<<nu, 256, 0, s>>>(Z, D.tasks, D.tiles + u0, D.sp);
+            if (D.nbp > 0)
+                k_band_csum<TA><<<blocksOf((long)D.nbp * D.nbp, 256), 256, 0, s>>>(Z, D.elims + e0, ne, D.offC,
+                                                                                      D.offPc, D.nbp);
+        }
+        k_band_final<TA><<<blocksOf((long)D.nF * D.nF, 256), 256, 0, s>>>(Z, D);
+        DenseLauncher<TA>::factor(Z + D.offF, D.nF / kT, st, s, false);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw std::runtime_error(std::string("band factorisation launch: ") + hipGetErrorString(e));
+    }
+
+    template <typename TA>
+    void BandLauncher<TA>::solve(const BandView& B, TA* Z, const DenseState& st, hipStream_t s)
+    {
+        const BandDev& D = B.d;
+        for (int l = 0; l < B.levels; ++l)
+        {
+            const int e0 = B.elimStart[l], ne = B.elimStart[l + 1] - e0;
+            k_band_forward<TA><<<ne, 256, 0, s>>>(Z, D, D.elims + e0);
+        }
+        k_band_final_rhs<TA><<<1, 256, 0, s>>>(Z, D);
+        DenseLauncher<TA>::solve(Z + D.offF, D.nF / kT, Z + D.offFV, st, s);
+        k_band_final_x<TA><<<blocksOf(D.nF, 256), 256, 0, s>>>(Z, D);
+        for (int l = B.levels - 1; l >= 0; --l)
+        {
+            const int e0 = B.elimStart[l], ne = B.elimStart[l + 1] - e0;
+            k_band_backward<TA><<<ne, 256, 0, s>>>(Z, D, D.elims + e0);
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw std::runtime_error(std::string("band solve launch: ") + hipGetErrorString(e));
+    }
+
+    template <typename TA>
+    void BandLauncher<TA>::toBand(const BandView& B, const double* src, TA* Z, hipStream_t s)
+    {
+        k_band_to<TA><<<blocksOf(B.d.vN, 256), 256, 0, s>>>(B.d, src, Z);
+    }
+
+    template <typename TA>
+    void BandLauncher<TA>::fromBand(const BandView& B, double* x, const TA* Z, bool overwrite, hipStream_t s)
+    {
+        k_band_from<TA><<<blocksOf(B.d.n, 256), 256, 0, s>>>(B.d, x, Z, overwrite ? 1 : 0);
+    }
+
+    template struct BandLauncher<float>;
+    template struct BandLauncher<double>;
+    template void BandLauncher<float>::gather<float>(const Problem&, const float*, const BandView&, float*,
+                                                     hipStream_t);
+    template void BandLauncher<float>::gather<double>(const Problem&, const double*, const BandView&, float*,
+                                                      hipStream_t);
+    template void BandLauncher<double>::gather<float>(const Problem&, const float*, const BandView&, double*,
+                                                      hipStream_t);
+    template void BandLauncher<double>::gather<double>(const Problem&, const double*, const BandView&, double*,
+                                                       hipStream_t);
+}  // namespace pbk

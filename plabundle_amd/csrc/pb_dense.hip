@@ -566,10 +566,16 @@ namespace pbk
     // per block with the block's kDenseBlock * 64 columns (a quarter of the passes over the trailing matrix of
     // one update per tile column).
     template <typename TA>
-    void DenseLauncher<TA>::factor(TA* A, int nt, const DenseState& st, hipStream_t s)
+    void DenseLauncher<TA>::begin(const DenseState& st, hipStream_t s)
+    {
+        k_dense_begin<<<1, 64, 0, s>>>(st.words, 1);
+    }
+
+    template <typename TA>
+    void DenseLauncher<TA>::factor(TA* A, int nt, const DenseState& st, hipStream_t s, bool reset)
     {
         const int lda = nt * kT;
-        k_dense_begin<<<1, 64, 0, s>>>(st.words, 1);
+        k_dense_begin<<<1, 64, 0, s>>>(st.words, reset ? 1 : 0);
         for (int kb = 0; kb < nt; kb += kDenseBlock)
         {
             const int ke = std::min(nt, kb + kDenseBlock);

@@ -1025,6 +1025,20 @@ struct pb_ctx
         double *x = nullptr, *r = nullptr, *rowNorm = nullptr, *norm = nullptr;
         pbk::DenseState st;
     } dense;
+    // Block-tridiagonal reduced solve (pb_band.hip): the plan of the current reduced-system pattern (made on
+    // first use after a pattern build) and its arenas per precision.
+    struct Band
+    {
+        int64_t planBuild = -1;  // patternBuilds value the plan belongs to
+        bool ok = false;         // the pattern fits the band route
+        bool active = false;     // the current solve (or adjoint) takes the band route
+        std::string why;         // why the pattern does not fit, when it does not
+        int bandwidth = 0;       // pose units, in the bandwidth-reducing order
+        pbk::BandView view;
+        DeviceAlloc tables, arena32, arena64;
+        float* Z32 = nullptr;
+        double* Z64 = nullptr;
+    } band;
     // profiling
     bool profile = false;
     int64_t pcgIterationsProfiled = 0;
@@ -1277,7 +1291,7 @@ namespace
         if ((o->shared_intrinsic_mask & 0x1FFEu) && !(o->shared_intrinsic_mask & 1u) && o->shared_intrinsic_mask != (1u << 4))
             return "refining aspect, principal point or distortion requires refining the shared focal length";
         if (o->precision < PB_PRECISION_FP64 || o->precision > PB_PRECISION_FP32_SEED) return "precision is invalid";
-        if (o->reduced_solver < PB_REDUCED_AUTO || o->reduced_solver > PB_REDUCED_DENSE) return "reduced_solver is invalid";
+        if (o->reduced_solver < PB_REDUCED_AUTO || o->reduced_solver > PB_REDUCED_BAND) return "reduced_solver is invalid";
         if (o->dense_max_rows < 0) return "dense_max_rows must be non-negative";
         // plaMatrixPreconditionerClusterSize must be in [1, 16] (src/options.cpp:393-394)
         if (o->preconditioner_cluster_size < 1 || o->preconditioner_cluster_size > 16)
@@ -1559,14 +1573,13 @@ namespace
     // canUseReferenceOnlineSchur (ReferenceSchur.cpp:667-673) for frame cameras without rigs, laser ranges, scale
     // bars, pose priors or a camera plane: the online point Schur (path B) unless laser planes are present or
     // nothing primary is refined; otherwise the assembled damped Schur (path A).
-    // Reduced-solve route of a solve: PB_REDUCED_AUTO takes the dense Cholesky up to dense_max_rows rows.
-    bool useDense(const pb_ctx* ctx, const pb_solve_options& o)
+    enum Route
     {
-        if (o.reduced_solver == PB_REDUCED_DENSE) return true;
-        if (o.reduced_solver == PB_REDUCED_PCG) return false;
-        const int maxRows = o.dense_max_rows > 0 ? o.dense_max_rows : kDenseSolveMaxRows;
-        return 6L * ctx->P.nUnits <= maxRows;
-    }
+        kRoutePcg,
+        kRouteDense,
+        kRouteBand
+    };
+    bool planBand(pb_ctx* ctx);
 
     bool onlineSchur(const pb_ctx* ctx, const pb_solve_options& o)
     {
@@ -1793,7 +1806,9 @@ namespace
     // factorisation whose solution is refined in FP64 against the stored system.
     int denseTiles(const pbk::Problem& P) { return (6 * P.nUnits + pbk::kDenseTile - 1) / pbk::kDenseTile; }
 
-    void ensureDense(pb_ctx* ctx, int nt, bool fp32)
+    // Vectors and state words of a direct reduced solve; with matrix, the dense matrix of precision fp32/fp64 too
+    // (the band route keeps its factors in its own arena).
+    void ensureDense(pb_ctx* ctx, int nt, bool fp32, bool matrix = true)
     {
         auto& D = ctx->dense;
         hipStream_t s = ctx->stream;
@@ -1818,9 +1833,501 @@ namespace
             check(hipMemsetAsync(D.st.words, 0, sizeof(int32_t) * pbk::kDenseWordCount, s), "memset");
             check(hipMemsetAsync(D.st.progress, 0, sizeof(unsigned long long) * 2, s), "memset");
         }
+        if (!matrix) return;
         const size_t cap = size_t(D.ntCap) * pbk::kDenseTile;
         if (fp32 && !D.A32) D.A32 = D.mat32.alloc<float>(cap * cap);
         if (!fp32 && !D.A64) D.A64 = D.mat64.alloc<double>(cap * cap);
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // Band route (pb_band.hip): plan of the block-tridiagonal cyclic reduction for the current pattern.
+
+    // Reverse Cuthill-McKee order of an undirected graph (no self loops): per connected component a
+    // breadth-first search from a pseudo-peripheral vertex (repeated searches from the least-degree vertex of
+    // the last level while the eccentricity grows), neighbours visited by increasing degree, then the whole
+    // order reversed.
+    std::vector<int> reverseCuthillMcKee(const std::vector<std::vector<int>>& adj)
+    {
+        const int n = static_cast<int>(adj.size());
+        std::vector<int> order;
+        order.reserve(n);
+        std::vector<char> placed(n, 0);
+        std::vector<int> mark(n, -1);
+        int stamp = 0;
+        // vertices of the component of root by levels; returns the last level
+        auto levels = [&](int root, int& depth) {
+            ++stamp;
+            std::vector<int> cur{root}, last;
+            mark[root] = stamp;
+            depth = 0;
+            while (!cur.empty())
+            {
+                last = cur;
+                std::vector<int> next;
+                for (int v : cur)
+                    for (int w : adj[v])
+                        if (mark[w] != stamp && !placed[w])
+                        {
+                            mark[w] = stamp;
+                            next.push_back(w);
+                        }
+                if (!next.empty()) ++depth;
+                cur.swap(next);
+            }
+            return last;
+        };
+        auto leastDegree = [&](const std::vector<int>& vs) {
+            int best = vs.front();
+            for (int v : vs)
+                if (adj[v].size() < adj[best].size() || (adj[v].size() == adj[best].size() && v < best)) best = v;
+            return best;
+        };
+        for (int s0 = 0; s0 < n; ++s0)
+        {
+            if (placed[s0]) continue;
+            int root = s0, depth = 0;
+            std::vector<int> last = levels(root, depth);
+            for (int round = 0; round < 8; ++round)
+            {
+                const int cand = leastDegree(last);
+                int d2 = 0;
+                std::vector<int> l2 = levels(cand, d2);
+                if (d2 <= depth) break;
+                root = cand;
+                depth = d2;
+                last.swap(l2);
+            }
+            // Cuthill-McKee from root
+            size_t head = order.size();
+            order.push_back(root);
+            placed[root] = 1;
+            while (head < order.size())
+            {
+                const int v = order[head++];
+                std::vector<int> nb;
+                for (int w : adj[v])
+                    if (!placed[w]) nb.push_back(w);
+                std::sort(nb.begin(), nb.end(), [&](int x, int y) {
+                    return adj[x].size() != adj[y].size() ? adj[x].size() < adj[y].size() : x < y;
+                });
+                nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+                for (int w : nb)
+                    if (!placed[w])
+                    {
+                        placed[w] = 1;
+                        order.push_back(w);
+                    }
+            }
+        }
+        std::reverse(order.begin(), order.end());
+        return order;
+    }
+
+    // Plans the band route for the current pattern (once per pattern build). The pose units are ordered by
+    // reverse Cuthill-McKee over the pose-pose blocks of S, the bandwidth w (in units) fixes the super-block
+    // order sp = 64 ceil(6 w / 64) <= 64 kBandMaxTiles, and each super-block takes floor(sp / 6) units, so no
+    // stored block couples super-blocks further apart than one. The group units form the border. The cyclic
+    // reduction schedule (levels, coupling slots, tasks) follows; every arena offset is checked against the
+    // arena size here, so the kernels index only inside it.
+    bool planBand(pb_ctx* ctx)
+    {
+        auto& Bd = ctx->band;
+        if (Bd.planBuild == ctx->patternBuilds) return Bd.ok;
+        Bd.tables.release();
+        Bd.arena32.release();
+        Bd.arena64.release();
+        Bd.Z32 = nullptr;
+        Bd.Z64 = nullptr;
+        Bd.view = pbk::BandView{};
+        Bd.ok = false;
+        Bd.bandwidth = 0;
+        Bd.planBuild = ctx->patternBuilds;
+        const Layout& L = ctx->lay;
+        const int U = ctx->prep.nUnits, nPose = ctx->prep.cameraBlocks;
+        if (nPose < 2)
+        {
+            Bd.why = "fewer than two pose units";
+            return false;
+        }
+        std::vector<std::vector<int>> adj(nPose);
+        for (int u = 0; u < nPose; ++u)
+            for (int k = L.rowPtr[u]; k < L.rowPtr[u + 1]; ++k)
+            {
+                const int v = L.colIdx[k];
+                if (v < nPose && v != u) adj[u].push_back(v);
+            }
+        const std::vector<int> order = reverseCuthillMcKee(adj);
+        std::vector<int> pos(nPose);
+        for (int i = 0; i < nPose; ++i) pos[order[i]] = i;
+        int w = 1;
+        for (int u = 0; u < nPose; ++u)
+            for (int v : adj[u]) w = std::max(w, std::abs(pos[u] - pos[v]));
+        Bd.bandwidth = w;
+        const int t = (6 * w + pbk::kDenseTile - 1) / pbk::kDenseTile;
+        if (t > pbk::kBandMaxTiles)
+        {
+            Bd.why = "bandwidth of " + std::to_string(w) + " pose units exceeds " +
+                     std::to_string(pbk::kBandMaxRows / 6);
+            return false;
+        }
+        pbk::BandView V;
+        pbk::BandDev& D = V.d;
+        D.sp = pbk::kDenseTile * t;
+        D.su = D.sp / 6;
+        D.nPose = nPose;
+        D.m = (nPose + D.su - 1) / D.su;
+        D.nb = 6 * (U - nPose);
+        D.nbp = (D.nb + pbk::kDenseTile - 1) / pbk::kDenseTile * pbk::kDenseTile;
+        if (D.nbp > pbk::kBandMaxBorder)
+        {
+            Bd.why = "border of " + std::to_string(D.nb) + " intrinsic rows exceeds " +
+                     std::to_string(pbk::kBandMaxBorder);
+            return false;
+        }
+        if (D.m < 2)
+        {
+            Bd.why = "a single super-block";
+            return false;
+        }
+        const int sp = D.sp, m = D.m, nbp = D.nbp, tb = nbp / pbk::kDenseTile;
+        D.nF = sp + nbp;
+        D.n = 6 * U;
+        D.vN = m * sp + nbp;
+        // unit placement
+        std::vector<int32_t> unitBlk(U), unitOff(U), rowOf(D.n), srcOf(D.vN, -1);
+        for (int u = 0; u < U; ++u)
+        {
+            if (u < nPose)
+            {
+                unitBlk[u] = pos[u] / D.su;
+                unitOff[u] = 6 * (pos[u] % D.su);
+            }
+            else
+            {
+                unitBlk[u] = -1;
+                unitOff[u] = 6 * (u - nPose);
+            }
+            const int base = u < nPose ? unitBlk[u] * sp + unitOff[u] : m * sp + unitOff[u];
+            for (int a = 0; a < 6; ++a)
+            {
+                rowOf[6 * u + a] = base + a;
+                srcOf[base + a] = 6 * u + a;
+            }
+        }
+        // the pattern must be block-tridiagonal over the super-blocks
+        for (int u = 0; u < nPose; ++u)
+            for (int v : adj[u])
+                if (std::abs(unitBlk[u] - unitBlk[v]) > 1) throw std::logic_error("band plan: coupling beyond a neighbour");
+        // cyclic reduction schedule; a slot couples two consecutive active super-blocks, stored with the rows of
+        // the one eliminated at that level
+        struct Slot
+        {
+            int rows, cols;
+        };
+        std::vector<Slot> slots;
+        std::vector<int32_t> k0slot(m - 1);
+        std::vector<int> act(m), actSlot(m - 1);  // active blocks and the slots between neighbours
+        for (int k = 0; k < m; ++k) act[k] = k;
+        for (int p = 0; p + 1 < m; ++p)
+        {
+            const bool oddFirst = (p & 1) != 0;  // positions p (odd: eliminated) and p + 1
+            slots.push_back(oddFirst ? Slot{p, p + 1} : Slot{p + 1, p});
+            k0slot[p] = p;
+            actSlot[p] = p;
+        }
+        struct ElimRec
+        {
+            int e, a, b, sA, sB;
+        };
+        std::vector<std::vector<ElimRec>> levelElims;
+        std::vector<pbk::BandTask> tasks;
+        std::vector<pbk::BandTile> tiles;
+        V.tileXStart.push_back(0);
+        V.tileUStart.push_back(0);
+        const long long spp = (long long)sp * sp, spb = (long long)sp * nbp, bb = (long long)nbp * nbp;
+        // arena offsets are fixed once the slot count is known; tasks are written with symbolic slot ids first
+        // and resolved below
+        struct TaskSym
+        {
+            int kind;  // 0: X of slot, 1: XB of block, 2: D of survivor, 3: BdT of survivor, 4: new slot, 5: Pc
+            int a0, a1, a2, a3, a4, a5;
+            int terms;
+        };
+        std::vector<std::vector<TaskSym>> symX, symU;
+        while (act.size() > 1)
+        {
+            const int na = static_cast<int>(act.size());
+            std::vector<ElimRec> el;
+            std::vector<TaskSym> sx, su;
+            for (int j = 0; 2 * j + 1 < na; ++j)
+            {
+                ElimRec r;
+                r.e = act[2 * j + 1];
+                r.a = act[2 * j];
+                r.sA = actSlot[2 * j];
+                if (2 * j + 2 < na)
+                {
+                    r.b = act[2 * j + 2];
+                    r.sB = actSlot[2 * j + 1];
+                }
+                else
+                {
+                    r.b = -1;
+                    r.sB = -1;
+                }
+                el.push_back(r);
+                sx.push_back({0, r.sA, r.e, 0, 0, 0, 0, 1});
+                if (r.sB >= 0) sx.push_back({0, r.sB, r.e, 0, 0, 0, 0, 1});
+                if (nbp > 0) sx.push_back({1, r.e, 0, 0, 0, 0, 0, 1});
+            }
+            std::vector<int> next, nextSlot;
+            for (int q = 0; 2 * q < na; ++q) next.push_back(act[2 * q]);
+            for (int q = 0; q < static_cast<int>(next.size()); ++q)
+            {
+                const int s = next[q];
+                const bool hasL = q > 0, hasR = 2 * q + 1 < na;
+                const int eL = hasL ? act[2 * q - 1] : -1, sL = hasL ? actSlot[2 * q - 1] : -1;
+                const int eR = hasR ? act[2 * q + 1] : -1, sR = hasR ? actSlot[2 * q] : -1;
+                if (!hasL && !hasR) continue;
+                const int terms = (hasL ? 1 : 0) + (hasR ? 1 : 0);
+                const int s0 = hasL ? sL : sR, e0 = hasL ? eL : eR;
+                const int s1 = hasL && hasR ? sR : -1, e1 = hasL && hasR ? eR : -1;
+                su.push_back({2, s, s0, s1, 0, 0, 0, terms});
+                if (nbp > 0) su.push_back({3, s, s0, e0, s1, e1, 0, terms});
+            }
+            for (int q = 0; q + 1 < static_cast<int>(next.size()); ++q)
+            {
+                const int xa = actSlot[2 * q], xb = actSlot[2 * q + 1];
+                const int id = static_cast<int>(slots.size());
+                if (q % 2 == 0)
+                {
+                    slots.push_back({next[q + 1], next[q]});  // eliminated next: next[q + 1]
+                    su.push_back({4, id, xb, xa, 0, 0, 0, 1});
+                }
+                else
+                {
+                    slots.push_back({next[q], next[q + 1]});
+                    su.push_back({4, id, xa, xb, 0, 0, 0, 1});
+                }
+                nextSlot.push_back(id);
+            }
+            if (nbp > 0)
+                for (const ElimRec& r : el) su.push_back({5, r.e, 0, 0, 0, 0, 0, 1});
+            levelElims.push_back(el);
+            symX.push_back(sx);
+            symU.push_back(su);
+            act.swap(next);
+            actSlot.swap(nextSlot);
+        }
+        V.levels = static_cast<int>(levelElims.size());
+        D.nSlots = static_cast<int>(slots.size());
+        auto align = [](long long x) { return (x + 63) / 64 * 64; };
+        D.offD = 0;
+        D.offK = align(D.offD + m * spp);
+        D.offB = align(D.offK + D.nSlots * spp);
+        D.offC = align(D.offB + m * spb);
+        D.offX = align(D.offC + bb);
+        D.offXB = align(D.offX + D.nSlots * spp);
+        D.offPc = align(D.offXB + m * spb);
+        D.offF = align(D.offPc + m * bb);
+        D.offV = align(D.offF + (long long)D.nF * D.nF);
+        D.offFV = align(D.offV + D.vN);
+        D.offPV = align(D.offFV + D.nF);
+        D.offPB = align(D.offPV + (long long)D.nSlots * sp);
+        D.size = align(D.offPB + (long long)m * nbp);
+        // tasks and tiles
+        auto addTask = [&](pbk::BandTask tk, int rowTiles, int colTiles, bool lower, long long outExtent,
+                           std::vector<pbk::BandTile>& out) {
+            // every element a tile touches lies inside the arena
+            auto inside = [&](long long off, long long extent) {
+                if (off < 0 || off + extent > D.size) throw std::logic_error("band plan: task outside the arena");
+            };
+            inside(tk.out, outExtent);
+            for (int q = 0; q < tk.terms; ++q)
+            {
+                inside(q ? tk.p1 : tk.p0, (long long)rowTiles * pbk::kDenseTile * sp);
+                inside(q ? tk.q1 : tk.q0, (long long)colTiles * pbk::kDenseTile * sp);
+            }
+            const int id = static_cast<int>(tasks.size());
+            tasks.push_back(tk);
+            for (int tj = 0; tj < colTiles; ++tj)
+                for (int ti = 0; ti < rowTiles; ++ti)
+                    if (!lower || ti >= tj) out.push_back({id, ti, tj, 0});
+        };
+        auto X = [&](int slot) { return D.offX + slot * spp; };
+        auto XB = [&](int blk) { return D.offXB + blk * spb; };
+        for (int l = 0; l < V.levels; ++l)
+        {
+            for (const TaskSym& ts : symX[l])
+            {
+                // kind 0: a0 = slot, a1 = eliminated block; kind 1: a0 = eliminated block
+                pbk::BandTask tk{};
+                tk.terms = 1;
+                tk.kTri = 1;
+                tk.ldo = sp;
+                if (ts.kind == 0)
+                {
+                    tk.p0 = D.offD + ts.a1 * spp;  // W_e
+                    tk.out = X(ts.a0);
+                    tk.q0 = D.offK + ts.a0 * spp;
+                    addTask(tk, t, t, false, spp, tiles);
+                }
+                else
+                {
+                    tk.p0 = D.offD + ts.a0 * spp;
+                    tk.out = XB(ts.a0);
+                    tk.q0 = D.offB + ts.a0 * spb;
+                    addTask(tk, t, tb, false, spb, tiles);
+                }
+            }
+            V.tileXStart.push_back(static_cast<int>(tiles.size()));
+            V.tileUStart.push_back(0);
+        }
+        // the update tiles follow all X tiles in the tile array
+        V.tileUStart[0] = static_cast<int>(tiles.size());
+        for (int l = 0; l < V.levels; ++l)
+        {
+            for (const TaskSym& ts : symU[l])
+            {
+                pbk::BandTask tk{};
+                tk.terms = ts.terms;
+                tk.ldo = sp;
+                switch (ts.kind)
+                {
+                case 2:  // D_s -= sum X^T X
+                    tk.out = D.offD + ts.a0 * spp;
+                    tk.p0 = X(ts.a1);
+                    tk.q0 = X(ts.a1);
+                    if (ts.terms > 1)
+                    {
+                        tk.p1 = X(ts.a2);
+                        tk.q1 = X(ts.a2);
+                    }
+                    tk.alphaNeg = 1;
+                    tk.beta = 1;
+                    addTask(tk, t, t, true, spp, tiles);
+                    break;
+                case 3:  // BdT_s -= sum X^T XB
+                    tk.out = D.offB + ts.a0 * spb;
+                    tk.p0 = X(ts.a1);
+                    tk.q0 = XB(ts.a2);
+                    if (ts.terms > 1)
+                    {
+                        tk.p1 = X(ts.a3);
+                        tk.q1 = XB(ts.a4);
+                    }
+                    tk.alphaNeg = 1;
+                    tk.beta = 1;
+                    addTask(tk, t, tb, false, spb, tiles);
+                    break;
+                case 4:  // new coupling = -P^T Q
+                    tk.out = D.offK + ts.a0 * spp;
+                    tk.p0 = X(ts.a1);
+                    tk.q0 = X(ts.a2);
+                    tk.alphaNeg = 1;
+                    addTask(tk, t, t, false, spp, tiles);
+                    break;
+                default:  // Pc_e = XB^T XB
+                    tk.out = D.offPc + ts.a0 * bb;
+                    tk.ldo = nbp;
+                    tk.p0 = XB(ts.a0);
+                    tk.q0 = XB(ts.a0);
+                    addTask(tk, tb, tb, false, bb, tiles);
+                    break;
+                }
+            }
+            V.tileUStart[l + 1] = static_cast<int>(tiles.size());
+        }
+        // pending slots per block (forward sweep): the slots whose columns are the block, in creation order
+        std::vector<std::vector<int>> pendOf(m);
+        for (int sId = 0; sId < D.nSlots; ++sId) pendOf[slots[sId].cols].push_back(sId);
+        std::vector<int32_t> pend;
+        std::vector<pbk::BandElim> elims;
+        for (int l = 0; l < V.levels; ++l)
+        {
+            V.elimStart.push_back(static_cast<int>(elims.size()));
+            for (const ElimRec& r : levelElims[l])
+            {
+                pbk::BandElim be{};
+                be.e = r.e;
+                be.a = r.a;
+                be.b = r.b;
+                be.slotA = r.sA;
+                be.slotB = r.sB;
+                be.pendBegin = static_cast<int32_t>(pend.size());
+                for (int sId : pendOf[r.e]) pend.push_back(sId);
+                be.pendEnd = static_cast<int32_t>(pend.size());
+                if (slots[r.sA].rows != r.e || (r.sB >= 0 && slots[r.sB].rows != r.e))
+                    throw std::logic_error("band plan: slot orientation");
+                elims.push_back(be);
+            }
+        }
+        V.elimStart.push_back(static_cast<int>(elims.size()));
+        D.lastPendBegin = static_cast<int32_t>(pend.size());
+        for (int sId : pendOf[0]) pend.push_back(sId);
+        D.lastPendEnd = static_cast<int32_t>(pend.size());
+        if (static_cast<int>(elims.size()) != m - 1) throw std::logic_error("band plan: every block but one eliminated");
+        // every index the kernels follow stays inside its table or the arena
+        for (const pbk::BandElim& be : elims)
+            if (be.e <= 0 || be.e >= m || be.a < 0 || be.a >= m || be.b < -1 || be.b >= m || be.slotA < 0 ||
+                be.slotA >= D.nSlots || be.slotB < -1 || be.slotB >= D.nSlots || (be.b < 0) != (be.slotB < 0))
+                throw std::logic_error("band plan: elimination record out of range");
+        for (int32_t sId : pend)
+            if (sId < 0 || sId >= D.nSlots) throw std::logic_error("band plan: pending slot out of range");
+        for (int u = 0; u < U; ++u)
+            if (unitBlk[u] < -1 || unitBlk[u] >= m || unitOff[u] < 0 ||
+                unitOff[u] + 6 > (unitBlk[u] >= 0 ? sp : D.nb))
+                throw std::logic_error("band plan: unit placement out of range");
+        hipStream_t s = ctx->stream;
+        D.unitBlk = Bd.tables.upload(unitBlk, s);
+        D.unitOff = Bd.tables.upload(unitOff, s);
+        D.k0slot = Bd.tables.upload(k0slot, s);
+        D.rowOf = Bd.tables.upload(rowOf, s);
+        D.srcOf = Bd.tables.upload(srcOf, s);
+        D.elims = Bd.tables.upload(elims, s);
+        D.pend = Bd.tables.upload(pend, s);
+        D.tasks = Bd.tables.upload(tasks, s);
+        D.tiles = Bd.tables.upload(tiles, s);
+        check(hipStreamSynchronize(s), "band plan upload");
+        Bd.view = std::move(V);
+        Bd.ok = true;
+        Bd.why.clear();
+        return true;
+    }
+
+    // Reduced-solve route of a solve. PB_REDUCED_AUTO takes the band route when the pattern is block-tridiagonal
+    // over at least three super-blocks, else the dense Cholesky up to dense_max_rows rows, else PCG.
+    Route reducedRoute(pb_ctx* ctx, const pb_solve_options& o)
+    {
+        if (o.reduced_solver == PB_REDUCED_PCG) return kRoutePcg;
+        if (o.reduced_solver == PB_REDUCED_DENSE) return kRouteDense;
+        const bool band = planBand(ctx);
+        if (o.reduced_solver == PB_REDUCED_BAND)
+        {
+            if (!band) throw std::invalid_argument("unsupported: the band route does not fit this reduced system: " + ctx->band.why);
+            return kRouteBand;
+        }
+        if (band && ctx->band.view.d.m >= 3) return kRouteBand;
+        const int maxRows = o.dense_max_rows > 0 ? o.dense_max_rows : kDenseSolveMaxRows;
+        return 6L * ctx->P.nUnits <= maxRows ? kRouteDense : kRoutePcg;
+    }
+
+    // Arena of the band route in the factor precision (allocated once per plan and precision).
+    template <typename TA>
+    TA* bandArena(pb_ctx* ctx)
+    {
+        auto& Bd = ctx->band;
+        const size_t n = static_cast<size_t>(Bd.view.d.size);
+        if constexpr (std::is_same<TA, float>::value)
+        {
+            if (!Bd.Z32) Bd.Z32 = Bd.arena32.alloc<float>(n);
+            return Bd.Z32;
+        }
+        else
+        {
+            if (!Bd.Z64) Bd.Z64 = Bd.arena64.alloc<double>(n);
+            return Bd.Z64;
+        }
     }
 
     struct DenseOutcome
@@ -1838,14 +2345,18 @@ namespace
     DenseOutcome denseAttempt(pb_ctx* ctx, const pbk::Work<T>& W, double relTol, int maxRefine, pb_solve_report* rep)
     {
         using DL = pbk::DenseLauncher<TA>;
+        using BL = pbk::BandLauncher<TA>;
         const pbk::Problem& P = ctx->P;
         hipStream_t s = ctx->stream;
+        const bool band = ctx->band.active;
+        const pbk::BandView& BV = ctx->band.view;
         const int nt = denseTiles(P);
         const int n = 6 * P.nUnits, npad = nt * pbk::kDenseTile;
-        ensureDense(ctx, nt, std::is_same<TA, float>::value);
+        ensureDense(ctx, nt, std::is_same<TA, float>::value, !band);
         auto& D = ctx->dense;
         TA* A = nullptr;
         TA* v = nullptr;
+        TA* Z = band ? bandArena<TA>(ctx) : nullptr;
         if constexpr (std::is_same<TA, float>::value)
         {
             A = D.A32;
@@ -1856,12 +2367,36 @@ namespace
             A = D.A64;
             v = D.v64;
         }
+        // one solve of (L L^T) c = src into D.x (overwrite) or added to it
+        auto solveInto = [&](const double* src, bool overwrite) {
+            Span sp(ctx, kFamilyDenseSolve);
+            if (band)
+            {
+                BL::toBand(BV, src, Z, s);
+                BL::solve(BV, Z, D.st, s);
+                BL::fromBand(BV, D.x, Z, overwrite, s);
+            }
+            else
+            {
+                DL::toDense(src, n, npad, v, s);
+                DL::solve(A, nt, v, D.st, s);
+                DL::accumulate(D.x, v, n, overwrite, s);
+            }
+        };
         DenseOutcome out;
         auto t0 = Clock::now();
         {
             Span sp(ctx, kFamilyDenseFactor);
-            DL::template gather<T>(P, W.S, A, nt, s);
-            DL::factor(A, nt, D.st, s);
+            if (band)
+            {
+                BL::template gather<T>(P, W.S, BV, Z, s);
+                BL::factor(BV, Z, D.st, s);
+            }
+            else
+            {
+                DL::template gather<T>(P, W.S, A, nt, s);
+                DL::factor(A, nt, D.st, s);
+            }
         }
         DL::norm2(W.rhs, n, D.norm + 1, s);
         int32_t* hw = reinterpret_cast<int32_t*>(ctx->pinned + kPinDense);
@@ -1879,12 +2414,7 @@ namespace
         out.factored = true;
         const double bn = std::sqrt(hn[1]);
         auto ts = Clock::now();
-        DL::toDense(W.rhs, n, npad, v, s);
-        {
-            Span sp(ctx, kFamilyDenseSolve);
-            DL::solve(A, nt, v, D.st, s);
-        }
-        DL::accumulate(D.x, v, n, true, s);
+        solveInto(W.rhs, true);
         if (rep)
         {
             check(hipStreamSynchronize(s), "dense solve");
@@ -1908,17 +2438,12 @@ namespace
                 out.converged = out.relResidual <= relTol;
                 break;
             }
-            auto ts = Clock::now();
-            DL::toDense(D.r, n, npad, v, s);
-            {
-                Span sp(ctx, kFamilyDenseSolve);
-                DL::solve(A, nt, v, D.st, s);
-            }
-            DL::accumulate(D.x, v, n, false, s);
+            auto ts2 = Clock::now();
+            solveInto(D.r, false);
             if (rep)
             {
                 check(hipStreamSynchronize(s), "dense solve");
-                rep->triangular_solve_seconds += seconds(ts);
+                rep->triangular_solve_seconds += seconds(ts2);
             }
             ++out.refinements;
         }
@@ -2277,7 +2802,10 @@ namespace
                                     : pg.precision == PB_PRECISION_MIXED   ? (pg.fp32Accepted > 0 ? 1 : 0)
                                                                            : 0;
         std::snprintf(rep->linear_solver_name, sizeof(rep->linear_solver_name), "%s",
-                      !pg.dense ? "pcg" : pg.seed ? "dense-cholesky-fp32-refined" : "dense-cholesky-fp64");
+                      !pg.dense                ? "pcg"
+                      : ctx->band.active       ? (pg.seed ? "band-cr-fp32-refined" : "band-cr-fp64")
+                      : pg.seed                ? "dense-cholesky-fp32-refined"
+                                               : "dense-cholesky-fp64");
         std::snprintf(rep->preconditioner_name, sizeof(rep->preconditioner_name), "%s",
                       pg.dense ? "none" : "block-jacobi-6");
         rep->accepted_steps = pg.accepted;
@@ -3506,9 +4034,16 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
             (void)work<double>(ctx);
         Progress pg;
         pg.precision = options->precision;
-        pg.dense = useDense(ctx, *options);
+        const Route route = reducedRoute(ctx, *options);
+        pg.dense = route != kRoutePcg;
+        ctx->band.active = route == kRouteBand;
         pg.seed = pg.dense && options->precision != PB_PRECISION_FP64;
-        if (pg.dense) ensureDense(ctx, denseTiles(ctx->P), pg.seed);
+        if (pg.dense) ensureDense(ctx, denseTiles(ctx->P), pg.seed, !ctx->band.active);
+        if (ctx->band.active)
+        {
+            (void)bandArena<double>(ctx);
+            if (options->precision != PB_PRECISION_FP64) (void)bandArena<float>(ctx);
+        }
         if (!pg.dense && options->preconditioner_cluster_size > 1)
         {
             report->solve_status = PB_SOLVE_UNSUPPORTED;
@@ -3600,9 +4135,12 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
         }
         P.adjoint = 0;
         const long nRows = 6L * P.nUnits;
-        // a 6x6 diagonal block that is not positive definite only stops the PCG route; the dense route reports
-        // through its factorisation
-        if (br.invalid || br.singular || (br.notSpd && nRows > kDenseAdjointMax))
+        // the route of the last solve when it was forced to the dense Cholesky, else band when it fits
+        ctx->band.active = ctx->lastOptions.reduced_solver != PB_REDUCED_DENSE && planBand(ctx);
+        const bool direct = ctx->band.active || (nRows > 0 && nRows <= kDenseAdjointMax);
+        // a 6x6 diagonal block that is not positive definite only stops the PCG route; the direct routes report
+        // through their factorisation
+        if (br.invalid || br.singular || (br.notSpd && !direct))
         {
             tmp.release();
             return fail(ctx, PB_ERR_NUMERIC, br.invalid    ? "adjoint linearisation has an invalid projection"
@@ -3616,8 +4154,7 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
         // kDenseAdjointMax rows (the aerial block has 6,024; its condition number near 1e14 keeps PCG from a
         // tight residual; 60k rows of a 10k-camera block take 29 GB of HBM), FP64 PCG to a 1e-10 relative
         // residual above that
-        const long n = 6L * P.nUnits;
-        if (n > 0 && n <= kDenseAdjointMax)
+        if (direct)
         {
             bool seed = false;
             const PcgResult lin = denseSolve(ctx, W, 0.0, false, &seed, nullptr);
@@ -3987,6 +4524,82 @@ pb_status pb_hip_set_progress(pb_ctx* ctx, pb_progress_fn fn, void* user)
     ctx->progressFn = fn;
     ctx->progressUser = fn ? user : nullptr;
     return PB_OK;
+}
+
+pb_status pb_hip_direct_solve(pb_ctx* ctx, const pb_solve_options* options, double damping, int32_t route,
+                              double* x, double* matrix, double* rhs, int32_t* info, double* arena,
+                              int64_t arena_capacity)
+{
+    if (!ctx || !ctx->uploaded || !x) return PB_ERR_INVALID;
+    if (ctx->batchMode) return fail(ctx, PB_ERR_INVALID, "the context holds a batch; upload a problem first");
+    if (route != PB_REDUCED_DENSE && route != PB_REDUCED_BAND) return fail(ctx, PB_ERR_INVALID, "route must be dense or band");
+    const std::string oerr = validateOptions(options);
+    if (!oerr.empty()) return fail(ctx, PB_ERR_INVALID, oerr);
+    return guarded(ctx, [&]() -> pb_status {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        ensureStructure(ctx, *options);
+        applySolveOptions(ctx, *options);
+        ctx->profile = false;
+        hipStream_t s = ctx->stream;
+        pbk::Work<double>& W = work<double>(ctx);
+        resetState(ctx);
+        const BuildResult b = build(ctx, W, damping);
+        if (b.invalid) return fail(ctx, PB_ERR_NUMERIC, "invalid projection at the initial state");
+        const bool band = route == PB_REDUCED_BAND;
+        if (band && !planBand(ctx))
+            return fail(ctx, PB_ERR_UNSUPPORTED, "the band route does not fit this reduced system: " + ctx->band.why);
+        ctx->band.active = band;
+        const pbk::Problem& P = ctx->P;
+        const int n = 6 * P.nUnits;
+        bool seed = false;
+        const PcgResult lin = denseSolve<double>(ctx, W, 0.0, false, &seed, nullptr);
+        check(hipStreamSynchronize(s), "direct solve");
+        std::vector<double> hx(n, 0.0);
+        if (n > 0 && lin.converged)
+            check(hipMemcpy(hx.data(), W.x, sizeof(double) * n, hipMemcpyDeviceToHost), "download");
+        std::copy(hx.begin(), hx.end(), x);
+        if (info)
+        {
+            std::fill(info, info + 8, 0);
+            info[0] = lin.breakdown ? 1 : 0;
+            info[1] = int32_t(std::min(2147483647.0, std::max(0.0, lin.rnorm * 1e15)));
+            if (band)
+            {
+                const pbk::BandView& V = ctx->band.view;
+                info[2] = V.d.sp;
+                info[3] = V.d.m;
+                info[4] = V.d.nb;
+                info[5] = V.levels;
+                info[6] = ctx->band.bandwidth;
+            }
+            info[7] = n;
+        }
+        if (matrix || rhs)
+        {
+            std::vector<double> S(size_t(P.nNnz) * 36), r(n);
+            if (!S.empty()) check(hipMemcpy(S.data(), W.S, sizeof(double) * S.size(), hipMemcpyDeviceToHost), "download");
+            if (n > 0) check(hipMemcpy(r.data(), W.rhs, sizeof(double) * n, hipMemcpyDeviceToHost), "download");
+            if (rhs) std::copy(r.begin(), r.end(), rhs);
+            if (matrix)
+            {
+                std::fill(matrix, matrix + size_t(n) * n, 0.0);
+                for (int u = 0; u < P.nUnits; ++u)
+                    for (int k = ctx->lay.rowPtr[u]; k < ctx->lay.rowPtr[u + 1]; ++k)
+                    {
+                        const int v = ctx->lay.colIdx[k];
+                        for (int e = 0; e < 36; ++e)
+                            matrix[size_t(6 * v + e % 6) * n + 6 * u + e / 6] = S[size_t(k) * 36 + e];
+                    }
+            }
+        }
+        if (arena && band && ctx->band.Z64)
+        {
+            const long long size = ctx->band.view.d.size;
+            if (arena_capacity < size) return fail(ctx, PB_ERR_INVALID, "arena capacity below the band arena size");
+            check(hipMemcpy(arena, ctx->band.Z64, sizeof(double) * size_t(size), hipMemcpyDeviceToHost), "download");
+        }
+        return PB_OK;
+    });
 }
 
 pb_status pb_hip_dense_solve(pb_ctx* ctx, int32_t n, int32_t fp32, const double* A, const double* b, double* x,

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace pbk
 {
@@ -377,7 +378,9 @@ namespace pbk
         template <typename TW>
         static void gather(const Problem& P, const TW* S, TA* A, int nt, hipStream_t s);
         // A = L L^T in place (lower triangle); a pivot that is not positive sets words[kDenseNotSpd]
-        static void factor(TA* A, int nt, const DenseState& st, hipStream_t s);
+        // with reset false the failure words are kept (a factorisation that continues an earlier one)
+        static void factor(TA* A, int nt, const DenseState& st, hipStream_t s, bool reset = true);
+        static void begin(const DenseState& st, hipStream_t s);  // new epoch, cleared failure words
         // x <- (L L^T)^-1 x (x has 64 * nt entries, the padded tail zero)
         static void solve(const TA* A, int nt, TA* x, const DenseState& st, hipStream_t s);
         // r = b - S x in FP64 over the block-sparse rows, norm[0] = |r|^2 (fixed-order sum of rowNorm)
@@ -389,6 +392,78 @@ namespace pbk
         static void accumulate(double* x, const TA* c, int n, bool overwrite, hipStream_t s);  // x += c (or = c)
         template <typename TO>
         static void store(const double* src, int n, TO* dst, hipStream_t s);
+    };
+
+    // Block-tridiagonal reduced solve (pb_band.hip). The pose units, in a bandwidth-reducing order, are cut into
+    // m super-blocks of sp = 64 t rows (t <= kBandMaxTiles) such that S couples only neighbouring super-blocks;
+    // the intrinsic-group units form a dense border of nbp rows coupled to every super-block. Odd-even cyclic
+    // reduction eliminates every second remaining super-block per level (all of a level in parallel): the
+    // eliminated block e is factored, W_e = L_e^-T kept, its couplings X = W_e^T K to its two surviving
+    // neighbours and to the border formed, and the survivors, the new coupling between them and the border
+    // updated with -X^T X products. The last super-block and the border form a dense system of sp + nbp rows
+    // solved by the dense route. Every matrix lives in one arena (offsets in elements).
+    constexpr int kBandMaxTiles = 2;
+    constexpr int kBandMaxRows = kBandMaxTiles * kDenseTile;
+    struct BandElim
+    {
+        int32_t e, a, b;             // eliminated super-block, its surviving neighbours (b = -1 at the end)
+        int32_t slotA, slotB;        // coupling slots (rows e, columns a / b); slotB = -1 when b = -1
+        int32_t pendBegin, pendEnd;  // range in the pending list: slots whose X^T y the forward sweep subtracts
+        int32_t pad;
+    };
+    // out = beta * out + (alphaNeg ? -1 : 1) * sum over terms of P^T Q (P, Q column-major with leading dimension
+    // sp and sp rows; with kTri P is upper triangular, so row tile i of the result needs rows k < 64 (i + 1))
+    struct BandTask
+    {
+        long long out, p0, q0, p1, q1;
+        int32_t ldo, terms, alphaNeg, beta, kTri, pad;
+    };
+    struct BandTile
+    {
+        int32_t task, ti, tj, pad;
+    };
+    constexpr int kBandMaxBorder = 256;  // border rows (nbp) the solve kernels hold in LDS
+    // Device-visible description of a band plan (passed to the kernels by value).
+    struct BandDev
+    {
+        int sp = 0, m = 0, nb = 0, nbp = 0, nF = 0, su = 0, nPose = 0, nSlots = 0;
+        int n = 0, vN = 0;  // unknowns of S and rows of the band vector (m sp + nbp)
+        // arena offsets (elements): D, K, B, C first (cleared before every gather), then X, XB, Pc, F and the
+        // solve vectors
+        long long offD = 0, offK = 0, offB = 0, offC = 0, offX = 0, offXB = 0, offPc = 0, offF = 0;
+        long long offV = 0, offFV = 0, offPV = 0, offPB = 0, size = 0;
+        int lastPendBegin = 0, lastPendEnd = 0;  // pending slots of super-block 0 (the last system)
+        const int32_t* unitBlk = nullptr;  // [nUnits] super-block of a pose unit, -1 for a border unit
+        const int32_t* unitOff = nullptr;  // [nUnits] first row in its super-block or in the border
+        const int32_t* k0slot = nullptr;   // [m - 1] slot of the first-level coupling of super-blocks k, k + 1
+        const int32_t* rowOf = nullptr;    // [n] band-vector row of unknown j
+        const int32_t* srcOf = nullptr;    // [vN] unknown of band-vector row i, or -1
+        const BandElim* elims = nullptr;   // every level's eliminations, level after level
+        const int32_t* pend = nullptr;     // pending slots, grouped per eliminated block (and super-block 0)
+        const BandTask* tasks = nullptr;
+        const BandTile* tiles = nullptr;
+    };
+    struct BandView
+    {
+        BandDev d;
+        int levels = 0;
+        std::vector<int> elimStart;   // [levels + 1]
+        std::vector<int> tileXStart;  // [levels + 1] tiles forming the couplings X of a level
+        std::vector<int> tileUStart;  // [levels + 1] tiles of the updates of a level
+    };
+
+    template <typename TA>
+    struct BandLauncher
+    {
+        template <typename TW>
+        static void gather(const Problem& P, const TW* S, const BandView& B, TA* Z, hipStream_t s);
+        // cyclic reduction and the dense factorisation of the last system; a pivot that is not positive sets
+        // words[kDenseNotSpd]
+        static void factor(const BandView& B, TA* Z, const DenseState& st, hipStream_t s);
+        // the band vector Z + offV <- (S)^-1 of itself (padding rows zero)
+        static void solve(const BandView& B, TA* Z, const DenseState& st, hipStream_t s);
+        static void toBand(const BandView& B, const double* src, TA* Z, hipStream_t s);
+        static void fromBand(const BandView& B, double* x, const TA* Z, bool overwrite, hipStream_t s);
     };
 
     // Post-solve quality buffers (device): strict track RMS at the input and refined state, sort keys, filter.

@@ -57,11 +57,13 @@ class Precision(enum.IntEnum):
 
 
 class ReducedSolver(enum.IntEnum):
-    """pb_reduced_solver: Auto (dense up to denseMaxRows rows), Pcg (block-Jacobi PCG), Dense (Cholesky)."""
+    """pb_reduced_solver: Auto (band when the camera graph fits it, else dense up to denseMaxRows rows, else PCG),
+    Pcg (block-Jacobi PCG), Dense (Cholesky), Band (block-tridiagonal cyclic reduction)."""
 
     Auto = 0
     Pcg = 1
     Dense = 2
+    Band = 3
 
 
 @dataclass
