@@ -155,12 +155,24 @@ namespace pbk
             if (threadIdx.x < 64) dinv[threadIdx.x] = bandRecip(L[threadIdx.x][threadIdx.x]);
             for (int i0 = 0; i0 < kT; i0 += kPanel)
             {
-                for (int idx = threadIdx.x; idx < kPanel * kT; idx += blockDim.x)
                 {
-                    const int i = i0 + idx % kPanel, j = idx / kPanel;
-                    T s = i == j ? T(1) : T(0);
-                    for (int k = 0; k < i0; ++k) s -= L[k][i] * inv[j][k];
-                    inv[j][i] = s;
+                    // thread t: row i = i0 + t % 16 of the panel, columns j = t / 16 + 16 q (q < 4)
+                    const int i = i0 + (threadIdx.x & (kPanel - 1)), j0 = threadIdx.x / kPanel;
+                    T s[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) s[q] = i == j0 + 16 * q ? T(1) : T(0);
+                    for (int k = 0; k < i0; k += 4)
+                    {
+                        T l[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) l[u] = L[k + u][i];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) s[q] -= l[u] * inv[j0 + 16 * q][k + u];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) inv[j0 + 16 * q][i] = s[q];
                 }
                 __syncthreads();
                 if (threadIdx.x < 64)
@@ -183,46 +195,78 @@ namespace pbk
         }
 
         // O = (neg ? -1 : 1) A B, or O -= A B with sub, on 64 x 64 LDS tiles; A(r, k) = transA ? a[r][k] : a[k][r],
-        // B(k, c) = transB ? b[k][c] : b[c][k]. Thread (tr, tc) of 256 forms rows 4 tr.., columns 4 tc...
+        // B(k, c) = transB ? b[k][c] : b[c][k]. Wave w forms the 32 x 32 quadrant (w >> 1, w & 1) as 2 x 2 MFMA
+        // 16x16x4 blocks (operand A(i, kk) from lane i + 16 kk, B(kk, j) from lane j + 16 kk).
         template <typename T>
         __device__ void tileProduct(T (*o)[kLd], T (*a)[kLd], bool transA, T (*b)[kLd], bool transB, bool sub,
                                     bool neg)
         {
-            const int tr = (threadIdx.x & 15) * 4, tc = (threadIdx.x >> 4) * 4;
-            T acc[4][4] = {};
-            for (int k = 0; k < kT; ++k)
+            using V = typename BandMfma<T>::V;
+            constexpr int kLR = BandMfma<T>::kLaneRow, kIR = BandMfma<T>::kItemRow;
+            const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+            const int qi = (w >> 1) * 32, qj = (w & 1) * 32;
+            const int li = lane & 15, lk = lane >> 4;
+            V acc[2][2];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[bi][bj][e] = T(0);
+#pragma unroll 4
+            for (int k = 0; k < kT; k += 4)
             {
-                T av[4], bv[4];
+                const int kk = k + lk;
+                T av[2], bv[2];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
+                for (int x = 0; x < 2; ++x)
                 {
-                    av[q] = transA ? a[tr + q][k] : a[k][tr + q];
-                    bv[q] = transB ? b[k][tc + q] : b[tc + q][k];
+                    const int r = qi + 16 * x + li, c = qj + 16 * x + li;
+                    av[x] = transA ? a[r][kk] : a[kk][r];
+                    bv[x] = transB ? b[kk][c] : b[c][kk];
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
+                for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                    for (int p = 0; p < 4; ++p) acc[q][p] += av[q] * bv[p];
+                    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = BandMfma<T>::op(av[bi], bv[bj], acc[bi][bj]);
             }
-            __syncthreads();  // every thread has read a and b before o (which may alias neither) is written
+            __syncthreads();  // every wave has read a and b before o (which may alias neither) is written
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                for (int p = 0; p < 4; ++p)
-                {
-                    T& d = o[tc + p][tr + q];
-                    d = sub ? d - acc[q][p] : (neg ? -acc[q][p] : acc[q][p]);
-                }
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                    {
+                        const int r = qi + 16 * bi + kLR * lk + kIR * e, c = qj + 16 * bj + li;
+                        T& d = o[c][r];
+                        const T v = acc[bi][bj][e];
+                        d = sub ? d - v : (neg ? -v : v);
+                    }
         }
 
+        // 64 x 64 tile of M (leading dimension ld, from row r0, column c0) into buf[column][row]; every thread
+        // issues its 16 loads before it stores (the loads of a workgroup overlap instead of queueing one by one).
         template <typename T>
         __device__ __forceinline__ void loadTile(T (*buf)[kLd], const T* M, int ld, int r0, int c0)
         {
-            for (int idx = threadIdx.x; idx < kT * kT; idx += blockDim.x)
-            {
-                const int r = idx & 63, c = idx >> 6;
-                buf[c][r] = M[size_t(c0 + c) * ld + r0 + r];
-            }
+            constexpr int kPer = kT * kT / 256;
+            const int r = threadIdx.x & 63, c = threadIdx.x >> 6;
+            T v[kPer];
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) v[q] = M[size_t(c0 + c + 4 * q) * ld + r0 + r];
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) buf[c + 4 * q][r] = v[q];
+        }
+
+        // M(r0 + r, c0 + c) = buf(c, r) (the transpose of the LDS tile), or 0 when buf is null
+        template <typename T>
+        __device__ __forceinline__ void storeTransposed(T* M, int ld, int r0, int c0, T (*buf)[kLd])
+        {
+            constexpr int kPer = kT * kT / 256;
+            const int r = threadIdx.x & 63, c = threadIdx.x >> 6;
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) M[size_t(c0 + c + 4 * q) * ld + r0 + r] = buf ? buf[r][c + 4 * q] : T(0);
         }
 
         int blocksOf(long n, int per) { return int((n + per - 1) / per); }
@@ -355,16 +399,13 @@ namespace pbk
             tileProduct(b0, b2, false, b3, false, false, true);   // Linv10 = -I1 (L10 I0)
             __syncthreads();
         }
-        // W(r, c) = Linv(c, r)
-        for (int idx = threadIdx.x; idx < sp * sp; idx += blockDim.x)
+        // W(r, c) = Linv(c, r), tile by tile: W_00 = I0^T, W_01 = Linv10^T, W_11 = I1^T, W_10 = 0
+        storeTransposed(M, sp, 0, 0, b1);
+        if (sp > kT)
         {
-            const int r = idx % sp, c = idx / sp;
-            T v;
-            if (c < kT)
-                v = r < kT ? b1[r][c] : T(0);
-            else
-                v = r < kT ? b0[r][c - kT] : b2[r - kT][c - kT];
-            M[size_t(c) * sp + r] = v;
+            storeTransposed(M, sp, 0, kT, b0);
+            storeTransposed(M, sp, kT, kT, b2);
+            storeTransposed(M, sp, kT, 0, static_cast<T(*)[kLd]>(nullptr));
         }
         if (bad && threadIdx.x == 0) atomicOr(words + kDenseNotSpd, 1);
     }
@@ -470,37 +511,63 @@ namespace pbk
 
     // ---------------------------------------------------------------------------------------------
     // Solve helpers for one workgroup of 256 threads: matrix-vector products with a column-major matrix M
-    // (leading dimension ld) and a vector in LDS.
-    //   T-form: out_i = sum_{k < K} M(k, i) x_k (upper-triangular M with tri: k <= i)
-    //   N-form: out_i = sum_{k < K} M(i, k) x_k (upper-triangular M with tri: k >= i)
-    template <typename T>
-    __device__ __forceinline__ T gemvT(const T* M, int ld, int i, int K, bool tri, const T* x)
+    // (leading dimension ld, at most 128 rows) and a vector x in LDS.
+    // T-form: out_i = sum_k M(k, i) x_k (k <= i with tri, M upper triangular) for i < nOut: 16 lanes per output,
+    // each summing 8 consecutive k of column i (one 64-byte run), then a 16-lane butterfly; store(i, out_i) runs
+    // on the first lane of each output.
+    template <typename T, typename F>
+    __device__ __forceinline__ void blockGemvT(const T* M, int ld, int nOut, int K, bool tri, const T* x, F&& store)
     {
-        const int kEnd = tri ? min(K, i + 1) : K;
-        const T* col = M + size_t(i) * ld;
-        T s0 = T(0), s1 = T(0);
-        int k = 0;
-        for (; k + 1 < kEnd; k += 2)
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int g = lane >> 4, h = lane & 15;
+        for (int i0 = 4 * w; i0 < nOut; i0 += 16)
         {
-            s0 += col[k] * x[k];
-            s1 += col[k + 1] * x[k + 1];
+            const int i = i0 + g;
+            T s = T(0);
+            if (i < nOut)
+            {
+                const int kEnd = tri ? min(K, i + 1) : K;
+                const T* col = M + size_t(i) * ld + 8 * h;
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (8 * h + j < kEnd) s += col[j] * x[8 * h + j];
+            }
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
+            if (h == 0 && i < nOut) store(i, s);
         }
-        if (k < kEnd) s0 += col[k] * x[k];
-        return s0 + s1;
     }
 
+    // N-form: out_i = sum_k M(i, k) x_k (k >= i with tri) for i < nOut <= 128: two threads per output (halves of
+    // k, four running sums each), combined in a fixed order through part (LDS, 256 entries). Every thread calls
+    // it; the result is valid on threads i < nOut.
     template <typename T>
-    __device__ __forceinline__ T gemvN(const T* M, int ld, int i, int K, bool tri, const T* x)
+    __device__ __forceinline__ T blockGemvN(const T* M, int ld, int nOut, int K, bool tri, const T* x, T* part)
     {
-        T s0 = T(0), s1 = T(0);
-        int k = tri ? i : 0;
-        for (; k + 1 < K; k += 2)
+        const int i = threadIdx.x & 127, half = threadIdx.x >> 7;
+        T s = T(0);
+        if (i < nOut)
         {
-            s0 += M[size_t(k) * ld + i] * x[k];
-            s1 += M[size_t(k + 1) * ld + i] * x[k + 1];
+            const int kh = (K + 1) / 2;
+            int k = half ? kh : 0;
+            const int k1 = half ? K : kh;
+            if (tri) k = max(k, i);
+            T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+            for (; k + 3 < k1; k += 4)
+            {
+                s0 += M[size_t(k) * ld + i] * x[k];
+                s1 += M[size_t(k + 1) * ld + i] * x[k + 1];
+                s2 += M[size_t(k + 2) * ld + i] * x[k + 2];
+                s3 += M[size_t(k + 3) * ld + i] * x[k + 3];
+            }
+            for (; k < k1; ++k) s0 += M[size_t(k) * ld + i] * x[k];
+            s = (s0 + s1) + (s2 + s3);
         }
-        if (k < K) s0 += M[size_t(k) * ld + i] * x[k];
-        return s0 + s1;
+        part[threadIdx.x] = s;
+        __syncthreads();
+        const T r = threadIdx.x < 128 ? part[threadIdx.x] + part[threadIdx.x + 128] : T(0);
+        __syncthreads();
+        return r;
     }
 
     // Forward level: r_e = v_e - sum of its pending contributions, y_e = W_e^T r_e into v_e, then the
@@ -520,22 +587,23 @@ namespace pbk
             r[i] = s;
         }
         __syncthreads();
-        const T* W = Z + B.offD + size_t(el.e) * sp * sp;
-        if (i < sp)
-        {
-            const T yi = gemvT(W, sp, i, sp, true, r);
-            y[i] = yi;
-            v[i] = yi;
-        }
+        blockGemvT(Z + B.offD + size_t(el.e) * sp * sp, sp, sp, sp, true, r, [&](int k, T s) {
+            y[k] = s;
+            v[k] = s;
+        });
         __syncthreads();
-        if (i < sp)
+        T* pvA = Z + B.offPV + size_t(el.slotA) * sp;
+        blockGemvT(Z + B.offX + size_t(el.slotA) * sp * sp, sp, sp, sp, false, y, [&](int k, T s) { pvA[k] = s; });
+        if (el.slotB >= 0)
         {
-            Z[B.offPV + size_t(el.slotA) * sp + i] = gemvT(Z + B.offX + size_t(el.slotA) * sp * sp, sp, i, sp, false, y);
-            if (el.slotB >= 0)
-                Z[B.offPV + size_t(el.slotB) * sp + i] =
-                    gemvT(Z + B.offX + size_t(el.slotB) * sp * sp, sp, i, sp, false, y);
+            T* pvB = Z + B.offPV + size_t(el.slotB) * sp;
+            blockGemvT(Z + B.offX + size_t(el.slotB) * sp * sp, sp, sp, sp, false, y, [&](int k, T s) { pvB[k] = s; });
         }
-        if (i < nbp) Z[B.offPB + size_t(el.e) * nbp + i] = gemvT(Z + B.offXB + size_t(el.e) * sp * nbp, sp, i, sp, false, y);
+        if (nbp > 0)
+        {
+            T* pB = Z + B.offPB + size_t(el.e) * nbp;
+            blockGemvT(Z + B.offXB + size_t(el.e) * sp * nbp, sp, nbp, sp, false, y, [&](int k, T s) { pB[k] = s; });
+        }
     }
 
     // The last system's right-hand side: block 0 minus its pending contributions, the border minus every
@@ -576,11 +644,11 @@ namespace pbk
     template <typename T>
     __global__ void __launch_bounds__(256) k_band_backward(T* Z, BandDev B, const BandElim* elims)
     {
-        __shared__ T xa[kBandMaxRows], xb[kBandMaxRows], xB[kBandMaxBorder], t[kBandMaxRows];
+        __shared__ T xa[kBandMaxRows], xb[kBandMaxRows], xB[kBandMaxBorder], t[kBandMaxRows], part[256];
         const BandElim el = elims[blockIdx.x];
         const int sp = B.sp, nbp = B.nbp;
         const int i = threadIdx.x;
-        const T* V = Z + B.offV;
+        T* V = Z + B.offV;
         if (i < sp)
         {
             xa[i] = V[size_t(el.a) * sp + i];
@@ -588,16 +656,14 @@ namespace pbk
         }
         if (i < nbp) xB[i] = V[size_t(B.m) * sp + i];
         __syncthreads();
-        if (i < sp)
-        {
-            T s = V[size_t(el.e) * sp + i];
-            s -= gemvN(Z + B.offX + size_t(el.slotA) * sp * sp, sp, i, sp, false, xa);
-            if (el.slotB >= 0) s -= gemvN(Z + B.offX + size_t(el.slotB) * sp * sp, sp, i, sp, false, xb);
-            if (nbp > 0) s -= gemvN(Z + B.offXB + size_t(el.e) * sp * nbp, sp, i, nbp, false, xB);
-            t[i] = s;
-        }
+        const T ya = blockGemvN(Z + B.offX + size_t(el.slotA) * sp * sp, sp, sp, sp, false, xa, part);
+        const T yb = el.slotB >= 0 ? blockGemvN(Z + B.offX + size_t(el.slotB) * sp * sp, sp, sp, sp, false, xb, part)
+                                   : T(0);
+        const T yB = nbp > 0 ? blockGemvN(Z + B.offXB + size_t(el.e) * sp * nbp, sp, sp, nbp, false, xB, part) : T(0);
+        if (i < sp) t[i] = ((V[size_t(el.e) * sp + i] - ya) - yb) - yB;
         __syncthreads();
-        if (i < sp) Z[B.offV + size_t(el.e) * sp + i] = gemvN(Z + B.offD + size_t(el.e) * sp * sp, sp, i, sp, true, t);
+        const T xe = blockGemvN(Z + B.offD + size_t(el.e) * sp * sp, sp, sp, sp, true, t, part);
+        if (i < sp) V[size_t(el.e) * sp + i] = xe;
     }
 
     template <typename TA>
