@@ -438,30 +438,45 @@ namespace pbk
             }
         const int kEnd = tk.kTri ? min(sp, (tl.ti + 1) * kT) : sp;
         const T sign = tk.alphaNeg ? T(-1) : T(1);
-        for (int term = 0; term < tk.terms; ++term)
-        {
-            const T* Pm = Z + (term ? tk.p1 : tk.p0) + size_t(tl.ti) * kT * sp;
-            const T* Qm = Z + (term ? tk.q1 : tk.q0) + size_t(tl.tj) * kT * sp;
-            for (int k0 = 0; k0 < kEnd; k0 += kStage)
-            {
-                __syncthreads();  // every wave is done with the previous slice
-                for (int idx = threadIdx.x; idx < kStage * kT; idx += 256)
-                {
-                    const int kk = idx % kStage, i = idx / kStage;
-                    sa[kk][i] = Pm[size_t(i) * sp + k0 + kk];
-                    sb[kk][i] = Qm[size_t(i) * sp + k0 + kk];
-                }
-                __syncthreads();
+        // steps of kStage rows of k over every term; the next step's operands are loaded into registers while
+        // the current one is multiplied
+        constexpr int kPer = kStage * kT / 256;
+        const int perTerm = kEnd / kStage, steps = perTerm * tk.terms;
+        T pa[kPer], pb[kPer];
+        auto load = [&](int st) {
+            const int term = st / perTerm, k0 = (st % perTerm) * kStage;
+            const T* Pm = Z + (term ? tk.p1 : tk.p0) + size_t(tl.ti) * kT * sp + k0;
+            const T* Qm = Z + (term ? tk.q1 : tk.q0) + size_t(tl.tj) * kT * sp + k0;
 #pragma unroll
-                for (int kk = 0; kk < kStage; kk += 4)
-                {
-                    const T a0 = sign * sa[kk + lk][qi + li], a1 = sign * sa[kk + lk][qi + 16 + li];
-                    const T bb0 = sb[kk + lk][qj + li], bb1 = sb[kk + lk][qj + 16 + li];
-                    acc[0][0] = BandMfma<T>::op(a0, bb0, acc[0][0]);
-                    acc[0][1] = BandMfma<T>::op(a0, bb1, acc[0][1]);
-                    acc[1][0] = BandMfma<T>::op(a1, bb0, acc[1][0]);
-                    acc[1][1] = BandMfma<T>::op(a1, bb1, acc[1][1]);
-                }
+            for (int q = 0; q < kPer; ++q)
+            {
+                const int idx = threadIdx.x + 256 * q;
+                pa[q] = Pm[size_t(idx / kStage) * sp + idx % kStage];
+                pb[q] = Qm[size_t(idx / kStage) * sp + idx % kStage];
+            }
+        };
+        if (steps > 0) load(0);
+        for (int st = 0; st < steps; ++st)
+        {
+            __syncthreads();  // every wave is done with the previous slice
+#pragma unroll
+            for (int q = 0; q < kPer; ++q)
+            {
+                const int idx = threadIdx.x + 256 * q;
+                sa[idx % kStage][idx / kStage] = pa[q];
+                sb[idx % kStage][idx / kStage] = pb[q];
+            }
+            __syncthreads();
+            if (st + 1 < steps) load(st + 1);
+#pragma unroll
+            for (int kk = 0; kk < kStage; kk += 4)
+            {
+                const T a0 = sign * sa[kk + lk][qi + li], a1 = sign * sa[kk + lk][qi + 16 + li];
+                const T bb0 = sb[kk + lk][qj + li], bb1 = sb[kk + lk][qj + 16 + li];
+                acc[0][0] = BandMfma<T>::op(a0, bb0, acc[0][0]);
+                acc[0][1] = BandMfma<T>::op(a0, bb1, acc[0][1]);
+                acc[1][0] = BandMfma<T>::op(a1, bb0, acc[1][0]);
+                acc[1][1] = BandMfma<T>::op(a1, bb1, acc[1][1]);
             }
         }
 #pragma unroll
