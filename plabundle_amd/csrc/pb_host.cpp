@@ -1046,10 +1046,18 @@ struct pb_ctx
     std::array<Family, kFamilyCount> fam;
     std::vector<hipEvent_t> eventPool;
     size_t eventNext = 0;
+    hipEvent_t trialEv[2] = {nullptr, nullptr};  // brackets the trial-state kernels of a line-search pass
 };
 
 namespace
 {
+    // elapsed device time between two recorded events, once the later one has completed
+    double eventSeconds(hipEvent_t a, hipEvent_t b)
+    {
+        float ms = 0.0f;
+        return hipEventElapsedTime(&ms, a, b) == hipSuccess ? 1e-3 * ms : 0.0;
+    }
+
     hipEvent_t takeEvent(pb_ctx* ctx)
     {
         if (ctx->eventNext == ctx->eventPool.size())
@@ -2509,6 +2517,7 @@ namespace
         bool inexact = false;     // truncated-Newton PCG stop and cap (a mixed solve, both parts)
         bool fallback = false;    // set by an FP32 part that hands over
         bool fp64Finish = false;  // the FP64 continuation of a mixed solve ran
+        bool handOverBySwitch = false;  // the FP32 part handed over at its rounding floor (not after a failed step)
         int precision = PB_PRECISION_FP64;
         bool dense = false;       // reduced systems solved by the dense Cholesky route
         bool seed = false;        // dense route: FP32 factorisation refined in FP64 first (guarded)
@@ -2718,10 +2727,14 @@ namespace
                     check(hipMemcpyAsync(W.alphas, alphas, sizeof(double) * n, hipMemcpyHostToDevice, s), "alphas");
                     {
                         Span sp(ctx, 4);
-                        Lr::trialCosts(P, W, W.alphas, n, ctx->costBlocks, s);
+                        check(hipEventRecord(ctx->trialEv[0], s), "event");
+                        Lr::trialStates(P, W, W.alphas, n, s);
+                        check(hipEventRecord(ctx->trialEv[1], s), "event");
+                        Lr::trialObjective(P, W, W.alphas, n, ctx->costBlocks, s);
                     }
                     check(hipMemcpyAsync(h, W.trialCost, sizeof(double) * n, hipMemcpyDeviceToHost, s), "readback");
                     check(hipStreamSynchronize(s), "trial cost");
+                    rep->trial_state_seconds += eventSeconds(ctx->trialEv[0], ctx->trialEv[1]);
                     for (int k = 0; k < n; ++k)
                     {
                         ++rep->objective_evaluations;
@@ -2760,6 +2773,7 @@ namespace
                     {
                         converged = false;
                         pg.fallback = true;
+                        pg.handOverBySwitch = true;
                         if (!reportProgress(ctx, o, pg)) cancelled = true;
                         return;
                     }
@@ -2827,7 +2841,9 @@ namespace
         std::string msg = converged ? (rep->online_schur_used ? "converged (online Schur + block-Jacobi PCG + Armijo)"
                                                               : "converged (assembled Schur + block-Jacobi PCG + line search)")
                                     : "iteration budget or line search exhausted";
-        if (pg.fp64Finish) msg += "; finished in FP64 with exact steps after a mixed step failed";
+        if (pg.fp64Finish)
+            msg += pg.handOverBySwitch ? "; FP32 assembly handed over to FP64 at its rounding floor"
+                                       : "; finished in FP64 with exact steps after a mixed step failed";
         setMessage(rep, msg);
         // refined state back to the host
         ctx->outR.resize(ctx->in.R.size());
@@ -2977,10 +2993,14 @@ namespace
                     check(hipMemcpyAsync(W.alphas, alphas, sizeof(double) * n, hipMemcpyHostToDevice, s), "alphas");
                     {
                         Span sp(ctx, 4);
-                        Lr::trialCosts(P, W, W.alphas, n, ctx->costBlocks, s);
+                        check(hipEventRecord(ctx->trialEv[0], s), "event");
+                        Lr::trialStates(P, W, W.alphas, n, s);
+                        check(hipEventRecord(ctx->trialEv[1], s), "event");
+                        Lr::trialObjective(P, W, W.alphas, n, ctx->costBlocks, s);
                     }
                     check(hipMemcpyAsync(h, W.trialCost, sizeof(double) * n, hipMemcpyDeviceToHost, s), "readback");
                     check(hipStreamSynchronize(s), "trial cost");
+                    rep->trial_state_seconds += eventSeconds(ctx->trialEv[0], ctx->trialEv[1]);
                     int k = 0;
                     if (!probed)
                     {
@@ -3026,6 +3046,7 @@ namespace
                     {
                         converged = false;
                         pg.fallback = true;  // the rest is assembled in FP64 (as in runSolve)
+                        pg.handOverBySwitch = true;
                         if (!reportProgress(ctx, o, pg)) cancelled = true;
                         return;
                     }
@@ -3910,8 +3931,15 @@ pb_status pb_hip_create(int device, pb_ctx** out)
         check(hipSetDevice(device), "hipSetDevice");
         hipDeviceProp_t prop;
         check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-        ctx->deviceName = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+        // the marketing name can be empty on some systems; the architecture name and the compute-unit count
+        // identify the device then
+        const std::string marketing = prop.name;
+        ctx->deviceName = (marketing.empty() ? std::string("AMD Instinct (") + std::to_string(prop.multiProcessorCount) +
+                                                   " CUs)"
+                                             : marketing) +
+                          " (" + prop.gcnArchName + ")";
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (hipEvent_t& e : ctx->trialEv) check(hipEventCreate(&e), "hipEventCreate");
         void* p = nullptr;
         check(hipHostMalloc(&p, 4096, hipHostMallocDefault), "hipHostMalloc");
         ctx->pinned = static_cast<double*>(p);
@@ -3940,6 +3968,11 @@ pb_status pb_hip_destroy(pb_ctx* ctx)
     ctx->dbatch.release();
     ctx->dquality.release();
     for (hipEvent_t e : ctx->eventPool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->trialEv)
+        if (e) (void)hipEventDestroy(e);
+    ctx->band.tables.release();
+    ctx->band.arena32.release();
+    ctx->band.arena64.release();
     for (auto& g : ctx->pcgGraph)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     ctx->dense.vec.release();

@@ -348,6 +348,9 @@ namespace pbk
         static void denseSystem(const Problem& P, const Work<T>& W, double* A, double* b, hipStream_t s);
         static void trialCosts(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, int costBlocks,
                                hipStream_t s);
+        // the objective part of trialCosts (trial states already formed by trialStates)
+        static void trialObjective(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha,
+                                   int costBlocks, hipStream_t s);
         static void accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s);
     };
 

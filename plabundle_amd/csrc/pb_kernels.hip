@@ -1686,6 +1686,13 @@ namespace pbk
                                  int costBlocks, hipStream_t s)
     {
         trialStates(P, W, alphas, nAlpha, s);
+        trialObjective(P, W, alphas, nAlpha, costBlocks, s);
+    }
+
+    template <typename T>
+    void Launcher<T>::trialObjective(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha,
+                                     int costBlocks, hipStream_t s)
+    {
         k_trial_cost<T, kMaxAlpha><<<costBlocks, 256, 0, s>>>(P, W, alphas, nAlpha);
         k_trial_reduce<T, kMaxAlpha><<<1, 64 * kMaxAlpha, 0, s>>>(P, W, nAlpha, costBlocks);
     }
