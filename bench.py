@@ -29,8 +29,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
 FAMILIES = ("linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost", "back_substitution",
-            "dense_factor", "dense_solve")
-MFMA_FAMILIES = ("dense_factor", "dense_solve")
+            "dense_factor", "dense_solve", "band_factor", "band_solve")
+MFMA_FAMILIES = ("dense_factor", "dense_solve", "band_factor", "band_solve")
 METRIC = "time-to-converge + LM iter/s, 1.5M obs, mixed vs FP64 GPU at <=1e-6 rel cost"
 
 
@@ -457,9 +457,9 @@ def main():
     mixed_it = sum(r.report["iterations"] for r in mixed) / len(mixed)
     fp64_t, fp64_it = fp64.report["solve_seconds"], fp64.report["iterations"]
 
-    # Roofline of the dominant kernel family of the timed mixed solve (average timed launch x launches). The dense
-    # factorisation is MFMA work (n^3 / 3 flops per factorisation; FP64 once the solve has moved to FP64 assembly),
-    # the other families move memory (algorithmic bytes).
+    # Roofline of the dominant kernel family of the timed mixed solve (average timed launch x launches). The direct
+    # factorisations are MFMA work (dense: n^3 / 3 flops; band: the flops of the planned cyclic reduction; FP64
+    # once the solve has moved to FP64 assembly), the other families move memory (algorithmic bytes).
     def family_line(st):
         out = {}
         for k, v in st.items():

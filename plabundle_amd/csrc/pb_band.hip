@@ -84,6 +84,23 @@ namespace pbk
             return __builtin_fmaf(r, __builtin_fmaf(-p, r, 1.0f), r);
         }
 
+        // 1/sqrt(p) from the hardware estimate refined by Newton's iteration y <- y + y (1 - p y^2) / 2 (two steps in
+        // FP64, one in FP32)
+        __device__ __forceinline__ double bandRsqrt(double p)
+        {
+            double y = __builtin_amdgcn_rsq(p);
+            y = __builtin_fma(0.5 * y, __builtin_fma(-(p * y), y, 1.0), y);
+            return __builtin_fma(0.5 * y, __builtin_fma(-(p * y), y, 1.0), y);
+        }
+        __device__ __forceinline__ float bandRsqrt(float p)
+        {
+            const float y = __builtin_amdgcn_rsqf(p);
+            return __builtin_fmaf(0.5f * y, __builtin_fmaf(-(p * y), y, 1.0f), y);
+        }
+
+        __device__ __forceinline__ double bandFma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+        __device__ __forceinline__ float bandFma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
         // Orders LDS accesses of one wave (lanes exchange values through LDS without a workgroup barrier).
         __device__ __forceinline__ void waveLdsFence()
         {
@@ -104,27 +121,31 @@ namespace pbk
             {
                 if (threadIdx.x < 64)
                 {
+                    // the pivot of the next column is formed by its own lane from its own value, so the serial
+                    // chain is readlane -> 1/sqrt -> scale -> fma -> readlane; the LDS exchange of the column
+                    // feeds the other lanes' updates off that chain
                     const int r = lane;
                     T a[kPanel];
 #pragma unroll
                     for (int q = 0; q < kPanel; ++q) a[q] = buf[c0 + q][r];
+                    T p = bandReadLane(a[0], c0);
 #pragma unroll
                     for (int j = 0; j < kPanel; ++j)
                     {
                         const int cj = c0 + j;
-                        T p = bandReadLane(a[j], cj);
                         if (!(p > T(0) && p < T(INFINITY)))
                         {
                             bad = true;
                             p = T(1);
                         }
-                        const T s = sqrt(p);
-                        const T l = r > cj ? a[j] * bandRecip(s) : (r == cj ? s : T(0));
+                        const T rs = bandRsqrt(p);
+                        const T l = r > cj ? a[j] * rs : (r == cj ? p * rs : T(0));
                         a[j] = l;
                         col[r] = l;
+                        if (j + 1 < kPanel) p = bandReadLane(bandFma(-l, l, a[j + 1]), cj + 1);
                         waveLdsFence();
 #pragma unroll
-                        for (int q = j + 1; q < kPanel; ++q) a[q] -= l * col[c0 + q];
+                        for (int q = j + 1; q < kPanel; ++q) a[q] = bandFma(-l, col[c0 + q], a[q]);
                         waveLdsFence();
                     }
 #pragma unroll
@@ -177,15 +198,18 @@ namespace pbk
                 __syncthreads();
                 if (threadIdx.x < 64)
                 {
+                    // right-looking substitution: once x_p is known it is removed from every later row, so the
+                    // serial chain per row is one multiply and one fma
                     const int j = threadIdx.x;
                     T x[kPanel];
 #pragma unroll
-                    for (int q = 0; q < kPanel; ++q)
-                    {
-                        T s = inv[j][i0 + q];
+                    for (int q = 0; q < kPanel; ++q) x[q] = inv[j][i0 + q];
 #pragma unroll
-                        for (int p = 0; p < q; ++p) s -= L[i0 + p][i0 + q] * x[p];
-                        x[q] = s * dinv[i0 + q];
+                    for (int p = 0; p < kPanel; ++p)
+                    {
+                        x[p] *= dinv[i0 + p];
+#pragma unroll
+                        for (int q = p + 1; q < kPanel; ++q) x[q] = bandFma(-L[i0 + p][i0 + q], x[p], x[q]);
                     }
 #pragma unroll
                     for (int q = 0; q < kPanel; ++q) inv[j][i0 + q] = x[q];

@@ -909,8 +909,8 @@ namespace
     };
 
     const char* const kFamilies[] = {"linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost",
-                                     "back_substitution", "dense_factor", "dense_solve"};
-    constexpr int kFamilyDenseFactor = 6, kFamilyDenseSolve = 7;
+                                     "back_substitution", "dense_factor", "dense_solve", "band_factor", "band_solve"};
+    constexpr int kFamilyDenseFactor = 6, kFamilyDenseSolve = 7, kFamilyBandFactor = 8, kFamilyBandSolve = 9;
     // pinned readback area (in doubles): scalars of a build / step, flags, PCG scalars, PCG flags
     constexpr int kPinFlags = 32;
     constexpr int kPinPcg = 40;
@@ -1395,6 +1395,21 @@ namespace
         P.cams = A.alloc<pbdev::Camera>(P.nCam);
         P.camObsOffset = A.upload(L.camObsOffset, s);
         P.camObs = A.upload(L.camObs, s);
+        {
+            std::vector<int32_t> cp(L.nObs);
+            std::vector<double> cuv(size_t(L.nObs) * 2), cw(L.nObs);
+            for (int k = 0; k < L.nObs; ++k)
+            {
+                const int o = L.camObs[k];
+                cp[k] = L.obsPoint[o];
+                cuv[2 * size_t(k)] = L.obsUV[2 * size_t(o)];
+                cuv[2 * size_t(k) + 1] = L.obsUV[2 * size_t(o) + 1];
+                cw[k] = L.obsW[o];
+            }
+            P.camObsPoint = A.upload(cp, s);
+            P.camObsUV = A.upload(cuv, s);
+            P.camObsW = A.upload(cw, s);
+        }
         P.camDiagBlock = A.upload(L.camDiagBlock, s);
         P.camGroupBlock = A.upload(L.camGroupBlock, s);
         P.groupParams = A.upload(p.gParams, s);
@@ -2286,6 +2301,24 @@ namespace
             if (unitBlk[u] < -1 || unitBlk[u] >= m || unitOff[u] < 0 ||
                 unitOff[u] + 6 > (unitBlk[u] >= 0 ? sp : D.nb))
                 throw std::logic_error("band plan: unit placement out of range");
+        // algorithmic flops: per eliminated block a Cholesky and a triangular inverse of order sp (sp^3 / 3
+        // each), every task tile 2 * 64 * 64 * k per term, the border sums, the last dense system (nF^3 / 3);
+        // the solve: W^T r and W t, the coupling and border products of both sweeps, the last system's sweeps
+        {
+            const double spd = sp, nbd = nbp, nFd = D.nF;
+            double f = (m - 1) * (2.0 * spd * spd * spd / 3.0) + (m - 1) * nbd * nbd + nFd * nFd * nFd / 3.0;
+            for (const pbk::BandTile& tl : tiles)
+            {
+                const pbk::BandTask& tk = tasks[tl.task];
+                const double kEnd = tk.kTri ? std::min(spd, (tl.ti + 1) * 64.0) : spd;
+                f += 2.0 * 64.0 * 64.0 * kEnd * tk.terms;
+            }
+            V.factorFlops = f;
+            double g = 2.0 * nFd * nFd;
+            for (const pbk::BandElim& be : elims)
+                g += 2.0 * (2.0 * spd * spd / 2.0 + (be.slotB >= 0 ? 4.0 : 2.0) * spd * spd + 2.0 * spd * nbd);
+            V.solveFlops = g;
+        }
         hipStream_t s = ctx->stream;
         D.unitBlk = Bd.tables.upload(unitBlk, s);
         D.unitOff = Bd.tables.upload(unitOff, s);
@@ -2377,7 +2410,7 @@ namespace
         }
         // one solve of (L L^T) c = src into D.x (overwrite) or added to it
         auto solveInto = [&](const double* src, bool overwrite) {
-            Span sp(ctx, kFamilyDenseSolve);
+            Span sp(ctx, band ? kFamilyBandSolve : kFamilyDenseSolve);
             if (band)
             {
                 BL::toBand(BV, src, Z, s);
@@ -2394,7 +2427,7 @@ namespace
         DenseOutcome out;
         auto t0 = Clock::now();
         {
-            Span sp(ctx, kFamilyDenseFactor);
+            Span sp(ctx, band ? kFamilyBandFactor : kFamilyDenseFactor);
             if (band)
             {
                 BL::template gather<T>(P, W.S, BV, Z, s);
@@ -4388,6 +4421,12 @@ pb_status pb_hip_kernel_stats(pb_ctx* ctx, const char* name, int64_t* launches, 
                 break;
             case kFamilyDenseSolve:  // floating-point operations of the two triangular sweeps, 2 n^2
                 b = 2.0 * std::pow(6.0 * P.nUnits, 2);
+                break;
+            case kFamilyBandFactor:  // floating-point operations of the cyclic reduction as planned
+                b = ctx->band.view.factorFlops;
+                break;
+            case kFamilyBandSolve:
+                b = ctx->band.view.solveFlops;
                 break;
             default:
                 break;

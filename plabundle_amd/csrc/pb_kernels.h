@@ -114,6 +114,9 @@ namespace pbk
         pbdev::Camera* cams = nullptr;      // effective cameras of the current pass
         const int32_t* camObsOffset = nullptr;
         const int32_t* camObs = nullptr;
+        const int32_t* camObsPoint = nullptr;  // [nObs] camera-major copies of obsPoint, obsUV, obsW
+        const double* camObsUV = nullptr;
+        const double* camObsW = nullptr;
         const int32_t* camDiagBlock = nullptr;
         const int32_t* camGroupBlock = nullptr;  // [nCam*kGroupUnits] (pose unit, group unit j) block ids
 
@@ -453,6 +456,8 @@ namespace pbk
         std::vector<int> elimStart;   // [levels + 1]
         std::vector<int> tileXStart;  // [levels + 1] tiles forming the couplings X of a level
         std::vector<int> tileUStart;  // [levels + 1] tiles of the updates of a level
+        double factorFlops = 0.0;     // floating-point operations of one factorisation (as planned)
+        double solveFlops = 0.0;      // of one forward and backward solve
     };
 
     template <typename TA>

@@ -350,12 +350,14 @@ namespace pbk
         A.cost = 0.0;
         const Camera& cam = P.cams[c];
         Linearization L;
+        // the camera's observations in camera-major copies (point, pixel, weight read contiguously); only the
+        // points are gathered
         for (int k = P.camObsOffset[c] + threadIdx.x; k < P.camObsOffset[c + 1]; k += blockDim.x)
         {
-            const int o = P.camObs[k];
-            const int t = P.obsPoint[o];
-            if (!pbdev::linearize<KM>(cam, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2], P.obsUV[2 * o],
-                                  P.obsUV[2 * o + 1], P.obsW[o], P.loss, P.lossScale, KM > 0 ? K : 0, act, L,
+            const int t = P.camObsPoint[k];
+            const double ow = P.camObsW[k];
+            if (!pbdev::linearize<KM>(cam, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2], P.camObsUV[2 * k],
+                                  P.camObsUV[2 * k + 1], ow, P.loss, P.lossScale, KM > 0 ? K : 0, act, L,
                                   P.refParam != 0))
             {
                 setFlag(flagsOf(P, Wk, P.camWin, c), kFlagInvalid);
@@ -364,7 +366,7 @@ namespace pbk
             if (P.adjoint)
             {
                 double q[3];
-                pbdev::adjointWeightRoot(L, P.obsW[o], P.loss, P.lossScale, q);
+                pbdev::adjointWeightRoot(L, ow, P.loss, P.lossScale, q);
                 pbdev::applyWeightRoot<KM>(q, L);
             }
             const double w = L.w;
@@ -480,12 +482,18 @@ namespace pbk
         constexpr int kSym = KM * (KM + 1) / 2;
         __shared__ double red[kSym + KM];
         const double* gp = P.groupParams + g * 13;
-        for (int q = threadIdx.x; q < kSym + KM; q += blockDim.x)
+        // one wave per packed entry at a time: lane l sums the group's cameras l, l + 64, ... (their loads all in
+        // flight), then a butterfly over the lanes; the order is fixed, so the sum is reproducible
         {
-            double v = 0.0;
-            for (int k = P.groupCamOffset[g]; k < P.groupCamOffset[g + 1]; ++k)
-                v += Wk.camGroupPartial[P.groupCams[k] * kGroupPartialStride + q];
-            red[q] = v;
+            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+            const int k0 = P.groupCamOffset[g], k1 = P.groupCamOffset[g + 1];
+            for (int q = wv; q < kSym + KM; q += nw)
+            {
+                double v = 0.0;
+                for (int k = k0 + lane; k < k1; k += kWave) v += Wk.camGroupPartial[P.groupCams[k] * kGroupPartialStride + q];
+                v = waveSum(v);
+                if (lane == 0) red[q] = v;
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0)
@@ -1579,11 +1587,11 @@ namespace pbk
             if (P.maxGroupK == 0)
                 ;
             else if (P.maxGroupK <= 6)
-                k_group_reduce<T, 6><<<P.nGroup, 128, 0, s>>>(P, W);
+                k_group_reduce<T, 6><<<P.nGroup, 512, 0, s>>>(P, W);
             else if (P.maxGroupK <= 12)
-                k_group_reduce<T, 12><<<P.nGroup, 128, 0, s>>>(P, W);
+                k_group_reduce<T, 12><<<P.nGroup, 512, 0, s>>>(P, W);
             else
-                k_group_reduce<T, 13><<<P.nGroup, 128, 0, s>>>(P, W);
+                k_group_reduce<T, 13><<<P.nGroup, 512, 0, s>>>(P, W);
         }
         k_finish_cost<T><<<1, 256, 0, s>>>(P, W);
     }
