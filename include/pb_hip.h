@@ -392,6 +392,14 @@ pb_status pb_hip_evaluate_cost(pb_ctx* ctx, const pb_solve_options* options, dou
 pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* dl_dpoint, double* dl_dobs,
                           double* dl_dcontrol);
 
+/* pb_hip_backward for a loss of the whole refined pose (no reference counterpart, like pb_hip_backward): also
+ * takes dl/dR of every camera's refined rotation (dl_drotation [num_cameras*9] or NULL, row-major, for the
+ * camera-to-world matrices that pb_hip_download returns). The rotation enters through the step's rotation
+ * parameters (dR/dw = -D [e_k]x D R for the reference retraction, [e_k]x R for the default one), so a gradient
+ * along a direction R cannot move (outside the tangent space of rotations) contributes nothing. */
+pb_status pb_hip_backward_pose(pb_ctx* ctx, const double* dl_dcenter, const double* dl_drotation,
+                               const double* dl_dpoint, double* dl_dobs, double* dl_dcontrol);
+
 /* Batched local windows: independent problems solved in one call. Each window is a complete problem (its own
  * camera and track numbering, calibration groups, fixed blocks and gauge) and is solved as pb_hip_upload +
  * pb_hip_solve would solve it alone, with the same options, except that each LM step solves the window's

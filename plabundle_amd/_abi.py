@@ -342,6 +342,7 @@ EXPORTED_SYMBOLS = (
     "pb_hip_reduced_system",
     "pb_hip_evaluate_cost",
     "pb_hip_backward",
+    "pb_hip_backward_pose",
     "pb_hip_solve_batch",
     "pb_hip_batch_factor_shape",
     "pb_hip_batch_tuning",
@@ -401,6 +402,8 @@ def load_library(path: str | None = None):
     lib.pb_hip_evaluate_cost.restype = C.c_int
     lib.pb_hip_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     lib.pb_hip_backward.restype = C.c_int
+    lib.pb_hip_backward_pose.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+    lib.pb_hip_backward_pose.restype = C.c_int
     lib.pb_hip_solve_batch.argtypes = [C.c_void_p, C.POINTER(BatchView), C.POINTER(SolveOptionsC),
                                        C.POINTER(WindowReportC), C.POINTER(SolveReportC), _ip]
     lib.pb_hip_solve_batch.restype = C.c_int

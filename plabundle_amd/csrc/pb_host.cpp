@@ -4164,6 +4164,12 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
 pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* dl_dpoint, double* dl_dobs,
                           double* dl_dcontrol)
 {
+    return pb_hip_backward_pose(ctx, dl_dcenter, nullptr, dl_dpoint, dl_dobs, dl_dcontrol);
+}
+
+pb_status pb_hip_backward_pose(pb_ctx* ctx, const double* dl_dcenter, const double* dl_drotation,
+                               const double* dl_dpoint, double* dl_dobs, double* dl_dcontrol)
+{
     if (!ctx || !dl_dobs) return PB_ERR_INVALID;
     if (!ctx->uploaded || !ctx->haveResult || ctx->batchMode)
         return fail(ctx, PB_ERR_INVALID, "pb_hip_backward needs the result of a pb_hip_solve on this context");
@@ -4177,12 +4183,15 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
         applySolveOptions(ctx, ctx->lastOptions);
         // loss gradients in device order (cameras as given, points in elimination order)
         std::vector<double> hc(3 * size_t(std::max(1, P.nCam)), 0.0), hp(3 * size_t(std::max(1, P.nElim)), 0.0);
+        std::vector<double> hr(9 * size_t(std::max(1, P.nCam)), 0.0);
         if (dl_dcenter) std::copy_n(dl_dcenter, 3 * size_t(P.nCam), hc.begin());
+        if (dl_drotation) std::copy_n(dl_drotation, 9 * size_t(P.nCam), hr.begin());
         if (dl_dpoint)
             for (int b = 0; b < P.nElim; ++b)
                 std::copy_n(dl_dpoint + 3 * size_t(L.activeTrackOf[L.elimTrack[b]]), 3, hp.begin() + 3 * b);
         DeviceAlloc tmp;
         const double* dC = tmp.upload(hc, s);
+        const double* dR = tmp.upload(hr, s);
         const double* dP = tmp.upload(hp, s);
         double* dOut = tmp.alloc<double>(2 * size_t(std::max(1, P.nObs)));
         double* dCtl = tmp.alloc<double>(3 * size_t(std::max(1, ctx->in.nControl)));
@@ -4214,7 +4223,7 @@ pb_status pb_hip_backward(pb_ctx* ctx, const double* dl_dcenter, const double* d
                                                            : "adjoint system has a diagonal block that is not positive definite");
         }
         check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
-        Lr::adjointRhs(P, W, dC, dP, s);
+        Lr::adjointRhs(P, W, dC, dR, dP, s);
         Lr::schurBlocks(P, W, 0.0, s);
         // H lambda = g is solved once per call, exactly: the dense FP64 Cholesky of the reduced system up to
         // kDenseAdjointMax rows (the aerial block has 6,024; its condition number near 1e14 keeps PCG from a

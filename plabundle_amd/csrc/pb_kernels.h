@@ -342,8 +342,10 @@ namespace pbk
         static void backSubstitute(const Problem& P, const Work<T>& W, hipStream_t s);
         static void pointSteps(const Problem& P, const Work<T>& W, hipStream_t s);  // back-substitution only
         static void trialStates(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, hipStream_t s);
-        static void adjointRhs(const Problem& P, const Work<T>& W, const double* dlCenter, const double* dlPoint,
-                               hipStream_t s);
+        // dl/dC [nCam*3], dl/dR [nCam*9] (row-major, camera-to-world) and dl/dX (elimination order) into the
+        // adjoint right-hand side in step parameters
+        static void adjointRhs(const Problem& P, const Work<T>& W, const double* dlCenter, const double* dlRot,
+                               const double* dlPoint, hipStream_t s);
         static void adjointObservations(const Problem& P, const Work<T>& W, double* out, hipStream_t s);
         // d loss / d target of every control point (out [control points * 3], zero-filled by the caller)
         static void adjointConstraints(const Problem& P, const Work<T>& W, double* out, hipStream_t s);

@@ -1383,7 +1383,7 @@ namespace pbk
     // loss gradient in the step parameters, the point side enters as z = L^T (s grad_X) (the point step is
     // X += s dp); then d loss / d y_i = M_i J_i lambda for every observation.
     template <typename T>
-    __global__ void k_adjoint_rhs_cameras(Problem P, Work<T> Wk, const double* dlCenter)
+    __global__ void k_adjoint_rhs_cameras(Problem P, Work<T> Wk, const double* dlCenter, const double* dlRot)
     {
         const int c = blockIdx.x * blockDim.x + threadIdx.x;
         if (c >= P.nCam) return;
@@ -1391,7 +1391,39 @@ namespace pbk
         if (u < 0) return;
         const double g0 = dlCenter[3 * c], g1 = dlCenter[3 * c + 1], g2 = dlCenter[3 * c + 2];
         double* out = Wk.directRhs + 6 * u;
-        out[0] = out[1] = out[2] = 0.0;  // no rotation gradient
+        // rotation step k: dR/dw_k at w = 0 is -D [e_k]x D R for the reference retraction (R' = D E^T D R,
+        // E = Rx Ry Rz, D = diag(1, -1, -1)) and [e_k]x R for the default one (R' = Exp([w]x) R); dl/dw_k is its
+        // inner product with dl/dR
+        {
+            const double* R = P.camR + 9 * c;
+            const double* G = dlRot + 9 * c;
+            const double sg[3] = {1.0, -1.0, -1.0};
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx)
+            {
+                // [e_k]x: entry (r, q) = -eps(k, r, q) ... written out: (k+1 mod 3, k+2 mod 3) = -1, (k+2, k+1) = +1
+                double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                const int i1 = (kx + 1) % 3, i2 = (kx + 2) % 3;
+                S[i1 * 3 + i2] = -1.0;
+                S[i2 * 3 + i1] = 1.0;
+                double acc = 0.0;
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q)
+                    {
+                        double d = 0.0;
+#pragma unroll
+                        for (int m = 0; m < 3; ++m)
+                        {
+                            const double s = P.refParam ? -sg[r] * S[r * 3 + m] * sg[m] : S[r * 3 + m];
+                            d += s * R[m * 3 + q];
+                        }
+                        acc += G[r * 3 + q] * d;
+                    }
+                out[kx] = acc;
+            }
+        }
         const Camera& cam = P.cams[c];
         if (cam.tangent)
         {
@@ -1655,10 +1687,10 @@ namespace pbk
     }
 
     template <typename T>
-    void Launcher<T>::adjointRhs(const Problem& P, const Work<T>& W, const double* dlCenter, const double* dlPoint,
-                                 hipStream_t s)
+    void Launcher<T>::adjointRhs(const Problem& P, const Work<T>& W, const double* dlCenter, const double* dlRot,
+                                 const double* dlPoint, hipStream_t s)
     {
-        if (P.nCam > 0) k_adjoint_rhs_cameras<T><<<blocksFor(P.nCam, 128), 128, 0, s>>>(P, W, dlCenter);
+        if (P.nCam > 0) k_adjoint_rhs_cameras<T><<<blocksFor(P.nCam, 128), 128, 0, s>>>(P, W, dlCenter, dlRot);
         if (P.nElim > 0) k_adjoint_rhs_points<T><<<blocksFor(P.nElim, 256), 256, 0, s>>>(P, W, dlPoint);
     }
 

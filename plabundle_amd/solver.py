@@ -211,9 +211,11 @@ class SolverWorkspace:
             raise RuntimeError(f"pb_hip_get_masks: {self.last_error()}")
         return out
 
-    def backward(self, problem: Problem, dl_dcenter=None, dl_dpoint=None, with_control: bool = False):
+    def backward(self, problem: Problem, dl_dcenter=None, dl_dpoint=None, with_control: bool = False,
+                 dl_drotation=None):
         """Adjoint of the last solve on this workspace: dl/d(u, v) per observation ([num_observations, 2])
-        for a loss with gradients dl/dC ([num_cameras, 3]) and dl/dX ([num_tracks, 3]) at the refined state.
+        for a loss with gradients dl/dC ([num_cameras, 3]), dl/dR ([num_cameras, 3, 3] or [num_cameras, 9],
+        camera-to-world rotations as returned) and dl/dX ([num_tracks, 3]) at the refined state.
         with_control=True also returns dl/dP of every control point's surveyed position ([num_control_points, 3])."""
         # pb_hip_backward reads and writes arrays sized by the problem on the context: refuse any other problem
         if self._uploaded is None or self._uploaded != problem_fingerprint(problem):
@@ -225,12 +227,16 @@ class SolverWorkspace:
             raise ValueError("dl_dcenter must have one row per camera")
         if dp is not None and dp.shape[0] != nt:
             raise ValueError("dl_dpoint must have one row per track")
+        dr = None if dl_drotation is None else np.ascontiguousarray(dl_drotation, dtype=np.float64).reshape(-1, 9)
+        if dr is not None and dr.shape[0] != nc:
+            raise ValueError("dl_drotation must have one 3x3 matrix per camera")
         out = np.zeros((no, 2))
         ctl = np.zeros((max(1, ncp), 3)) if with_control else None
-        st = self._lib.pb_hip_backward(self._ctx, _abi.ptr(dc, C.c_double), _abi.ptr(dp, C.c_double),
-                                       _abi.ptr(out, C.c_double), _abi.ptr(ctl, C.c_double))
+        st = self._lib.pb_hip_backward_pose(self._ctx, _abi.ptr(dc, C.c_double), _abi.ptr(dr, C.c_double),
+                                            _abi.ptr(dp, C.c_double), _abi.ptr(out, C.c_double),
+                                            _abi.ptr(ctl, C.c_double))
         if st != _abi.PB_OK:
-            raise RuntimeError(f"pb_hip_backward: {_abi.STATUS_NAMES.get(st, st)}: {self.last_error()}")
+            raise RuntimeError(f"pb_hip_backward_pose: {_abi.STATUS_NAMES.get(st, st)}: {self.last_error()}")
         return (out, ctl[:ncp]) if with_control else out
 
     def quality(self, qopts) -> dict:

@@ -54,7 +54,8 @@ def constraint_weight_matrix(r, delta):
     return w * (np.eye(len(r)) - Pm) + mu * Pm
 
 
-def adjoint_reference(O, problem, state, masks, prep, copts, dl_dC, dl_dX=None, priors=None, with_control=False):
+def adjoint_reference(O, problem, state, masks, prep, copts, dl_dC, dl_dX=None, priors=None, with_control=False,
+                      stats=None):
     """dl/d(u, v) per observation ([num_observations, 2]) at `state` (dict of the oracle / workspace download);
     with_control=True also returns dl/dP of every control point ([num_control_points, 3]). Problems with survey
     constraints use the default parameterisation (world-frame rotation, unscaled point step).
@@ -132,7 +133,20 @@ def adjoint_reference(O, problem, state, masks, prep, copts, dl_dC, dl_dX=None, 
             if b >= 0:
                 sc = Oo.point_scale(state["track_point"][t]) if ref_param else 1.0
                 gp[b] = sc * dl_dX[t]
-    Hinv = np.linalg.inv(Hpp[:Ne]) if Ne else np.zeros((0, 3, 3))
+    # a point block that the 3x3 cofactor inverse rejects (|det| <= 1e-15 max|m|^3, rank-deficient weight matrices
+    # of observations on the clamped branch of the loss) gets a ridge of 1e-9 of its largest diagonal entry, the
+    # rule of the GPU's adjoint linearisation
+    Hb = Hpp[:Ne].copy()
+    if Ne:
+        mx = np.abs(Hb).reshape(Ne, 9).max(axis=1)
+        det = np.linalg.det(Hb)
+        sing = ~(np.abs(det) > 1e-15 * mx**3)
+        if np.any(sing):
+            ridge = 1e-9 * np.abs(np.diagonal(Hb[sing], axis1=1, axis2=2)).max(axis=1)
+            Hb[sing] += ridge[:, None, None] * np.eye(3)[None]
+        if stats is not None:
+            stats["ridged_point_blocks"] = int(np.count_nonzero(sing))
+    Hinv = np.linalg.inv(Hb) if Ne else np.zeros((0, 3, 3))
     V = W[:Ne] @ Hinv                                  # W Hpp^-1 per point
     S = Hcc - np.tensordot(V, W[:Ne], axes=([0, 2], [0, 2]))
     rhs = gc - np.einsum("tij,tj->i", V, gp[:Ne])

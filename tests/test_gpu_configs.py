@@ -239,3 +239,87 @@ def test_c5_adjoint_on_a_converged_subproblem(ws, oracle_mod, noise_px, fd_toler
         fd = (vals[0] - vals[1]) / (2 * h)
         _say(f"  obs {i} axis {k}: adjoint {g[i, k]:.6e}, finite difference {fd:.6e}")
         assert abs(fd - g[i, k]) <= fd_tolerance * np.abs(g).max()
+
+
+def test_c5_rotation_gradient_noise_free(ws):
+    """dl/dR through pb_hip_backward_pose: l = sum_c <A_c, R_c> + |C - T|^2 on the noise-free C5 problem, where the
+    Gauss-Newton adjoint is exact, against central differences of full re-solves."""
+    _say("case C5: adjoint with a rotation loss, 20 cameras / 2000 points, noise-free, FD h = 1e-4 px")
+    p = S.ring_problem(seed=5, num_cameras=20, num_points=2000, max_views=8, noise_px=0.0)
+    o = S.ring_options(PO.SolveOptions(), max_iterations=100)
+    o.backend.stepRmsTolerance = 1e-13
+    solver = Solver()
+    r = solver.solve(p, o, ws, precision=PO.Precision.Fp64)
+    assert r.usable(), r.backendMessage
+    rng = np.random.default_rng(6)
+    A = rng.normal(0.0, 1.0, (p.num_cameras, 9))
+    T = p.meta["truth_center"] + rng.normal(0.0, 0.05, r.cam_center.shape)
+
+    def loss(res):
+        return float(np.sum(A * res.cam_rotation.reshape(-1, 9)) + np.sum((res.cam_center - T) ** 2))
+
+    g = ws.backward(p, 2.0 * (r.cam_center - T), dl_drotation=A)
+    g_center_only = ws.backward(p, 2.0 * (r.cam_center - T))
+    assert np.abs(g - g_center_only).max() > 0.1 * np.abs(g).max()  # the rotation term matters
+    h = 1e-4
+    order = np.argsort(-np.abs(g).ravel())[:4]
+    for flat in order:
+        i, k = divmod(int(flat), 2)
+        vals = []
+        for sgn in (1.0, -1.0):
+            q = p.copy()
+            q.obs_uv[i, k] += sgn * h
+            rq = solver.solve(q, o, ws, precision=PO.Precision.Fp64)
+            assert rq.usable()
+            vals.append(loss(rq))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        _say(f"  obs {i} axis {k}: adjoint {g[i, k]:.6e}, finite difference {fd:.6e}")
+        assert abs(fd - g[i, k]) <= 1e-4 * np.abs(g).max()
+
+
+def test_adjoint_ridge_path_on_an_aerial_block(ws, oracle_mod):
+    """Cauchy loss: an observation whose residual exceeds the loss scale sits on the branch where the curvature
+    along the residual is negative and clamped, so it adds a rank-1 term; a two-view track with both residuals
+    above 1 px leaves its point block singular, and the adjoint gives it a 1e-9 ridge. Such tracks are planted in
+    a 120-camera strip block; at convergence Cauchy moves the whole disagreement onto one observation, so the
+    adjoint is taken after two LM iterations, where both residuals are still large. The check is the linear
+    algebra of the ridge path: the GPU adjoint (band route) against the CPU implicit-function reference with the
+    same ridge rule, at the same state."""
+    _say("case adjoint ridge path: aerial strips, 120 cameras, Cauchy, planted two-view disagreements")
+    p = S.aerial_problem(seed=7, cameras_per_strip=(40, 40, 40), num_points=12_000)
+    # two-view tracks whose observations disagree by several pixels: at convergence both residuals stay above the
+    # Cauchy scale (1 px), where the along-residual curvature is negative and clamped, so each observation adds a
+    # rank-1 term and the point block is singular
+    two = np.nonzero(np.diff(p.track_offset) == 2)[0][:12]
+    assert len(two) == 12
+    for t in two:
+        o0 = p.track_offset[t]
+        p.obs_uv[o0] += (7.0, -5.0)
+        p.obs_uv[o0 + 1] += (-7.0, 5.0)
+    o = S.aerial_options(PO.SolveOptions())
+    c = o.calibration
+    c.refineSharedFocalLength = c.refineSharedRadialDistortion = c.refineSharedHighOrderDistortion = False
+    o.solver.maxIterations = 2
+    r = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
+    assert r.report["iterations"] == 2 and r.report["linear_solver_name"] == "band-cr-fp64", r.backendMessage
+    rng = np.random.default_rng(8)
+    T = r.cam_center + rng.normal(0.0, 0.05, r.cam_center.shape)
+    g = ws.backward(p, 2.0 * (r.cam_center - T))
+    copts = PO.to_c(o, PO.Precision.Fp64)
+    orc = oracle_mod.Oracle(p, threads=8)
+    prep = orc.prepare(copts)
+    masks = orc.masks()
+    orc.close()
+    state = {"cam_rotation": r.cam_rotation, "cam_center": r.cam_center, "cam_intrinsics": r.cam_intrinsics,
+             "track_point": r.track_point}
+    stats = {}
+    t = time.time()
+    ref = adjoint_reference(oracle_mod, p, state, masks, prep, copts, 2.0 * (r.cam_center - T), None, {},
+                            stats=stats)
+    err = np.abs(g - ref).max() / np.abs(ref).max()
+    _say(f"  reference in {time.time() - t:.1f}s, ridged point blocks {stats['ridged_point_blocks']}, "
+         f"max |gpu - reference| / max |reference| = {err:.2e}")
+    assert stats["ridged_point_blocks"] > 0
+    # a ridged block has a condition number up to 1e9, so two FP64 evaluations in different orders agree to
+    # about 1e-7..1e-6 of the largest gradient
+    assert err <= 5e-6
