@@ -212,5 +212,6 @@ def test_survey_block_matches_oracle(ws, oracle_mod):
     m = Solver().solve(p, o, ws, precision=PO.Precision.Mixed)
     rel = abs(m.finalCost - ref["final_cost"]) / ref["final_cost"]
     _say(f"  mixed rel cost error after 20 iterations {rel:.3e} ({m.report['message']})")
-    # the bar applies to converged solves; at the budget the mixed trajectory is within 1e-5 of the oracle's
-    assert m.usable() and rel <= 1e-5
+    # the FP32 assembly hands over to FP64 at its rounding floor, so even at the budget the mixed solve meets the
+    # 1e-6 bar (measured 1.9e-10)
+    assert m.usable() and rel <= 1e-6
