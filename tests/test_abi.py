@@ -165,3 +165,26 @@ def test_batch_factor_shapes_and_tuning_queries(lib):
     assert lib.pb_hip_batch_factor_shape(-1, None, None) == _abi.PB_ERR_INVALID
     shape = C.c_int32()
     assert lib.pb_hip_batch_tuning(None, 1, C.byref(shape), None) == _abi.PB_ERR_INVALID
+
+
+def test_problem_fingerprint_sees_in_place_changes():
+    # a workspace re-uploads when the fingerprint changes: every array edited in place, and the gauge scalars, must
+    # change it, while an equal copy must not
+    import numpy as np
+
+    from plabundle_amd import synthetic as S
+    from plabundle_amd.solver import problem_fingerprint
+
+    p = S.ring_problem(seed=3, num_cameras=8, num_points=100, max_views=4)
+    base = problem_fingerprint(p)
+    assert problem_fingerprint(p.copy()) == base
+    for name in ("cam_center", "cam_rotation", "cam_intrinsics", "track_point", "obs_uv"):
+        a = getattr(p, name)
+        keep = a.flat[0]
+        a.flat[0] = keep + 1e-9
+        assert problem_fingerprint(p) != base, name
+        a.flat[0] = keep
+        assert problem_fingerprint(p) == base, name
+    q = p.copy()
+    q.cam_center = np.asfortranarray(q.cam_center)  # same values, other layout: same upload
+    assert problem_fingerprint(q) == base

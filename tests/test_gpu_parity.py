@@ -280,3 +280,27 @@ def test_auto_anchor_with_a_gauge_scale_camera_matches_oracle(ws, oracle_mod, ro
     gm = ws.masks(p)
     for k in gm:
         np.testing.assert_array_equal(gm[k], masks[k], err_msg=k)
+
+
+def test_in_place_edit_between_solves_on_one_workspace(ws):
+    # a caller that edits the problem's arrays in place between two solves on one workspace must get the solve of
+    # the edited problem (the workspace re-uploads), identical to a fresh workspace's
+    _say("case in-place edit between solves")
+    p, o = _ring(n=12, pts=300, seed=11)
+    first = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
+    assert first.usable(), first.backendMessage
+    p.cam_center[5] += np.array([0.3, -0.2, 0.1])
+    p.track_point[:40] *= 1.01
+    again = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
+    fresh_ws = SolverWorkspace(0)
+    try:
+        fresh = Solver().solve(p, o, fresh_ws, precision=PO.Precision.Fp64)
+    finally:
+        fresh_ws.close()
+    _say(f"  first {first.finalCost:.12g} ({first.report['initial_cost']:.6g} initial), edited "
+         f"{again.finalCost:.12g} ({again.report['initial_cost']:.6g} initial), fresh {fresh.finalCost:.12g}")
+    assert again.report["initial_cost"] != first.report["initial_cost"]
+    assert again.report["iterations"] == fresh.report["iterations"]
+    assert again.finalCost == fresh.finalCost
+    np.testing.assert_array_equal(again.cam_center, fresh.cam_center)
+    np.testing.assert_array_equal(again.track_point, fresh.track_point)
