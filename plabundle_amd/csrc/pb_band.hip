@@ -101,13 +101,6 @@ namespace pbk
         __device__ __forceinline__ double bandFma(double a, double b, double c) { return __builtin_fma(a, b, c); }
         __device__ __forceinline__ float bandFma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-        // Orders LDS accesses of one wave (lanes exchange values through LDS without a workgroup barrier).
-        __device__ __forceinline__ void waveLdsFence()
-        {
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-        }
-
         // Cholesky of the 64 x 64 tile buf (lower triangle read; L written with zeros above the diagonal) by the
         // whole workgroup, in four panels of 16 columns: wave 0 factors the panel with lane r holding row r's 16
         // entries in registers (column j published in LDS, every lane subtracts L(r, j) L(c, j) from its entries

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace pbk
 {
@@ -19,6 +20,13 @@ namespace pbk
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
         return v;
+    }
+
+    // Orders LDS accesses of one wave (lanes exchange values through LDS without a workgroup barrier).
+    __device__ __forceinline__ void waveLdsFence()
+    {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 
     __device__ __forceinline__ void setFlag(int32_t* flags, int index)

@@ -21,7 +21,12 @@ namespace pbk
     constexpr int kMaxAlpha = 8;             // step lengths evaluated per trial-cost pass
     constexpr int kGroupUnits = 3;           // 6-wide units of an intrinsic group with all 13 slots refined
     constexpr int kGroupPartialStride = 104;  // >= K(K+1)/2 + K for K <= 13
-    constexpr int kPairChunk = 2048;         // pairs per Schur work item
+    // Pairs per Schur work item. Short items keep the pairs that the waves of one XCD hold at once within a few
+    // block rows, whose cross blocks then fit that XCD's L2.
+    constexpr int kPairChunk = 256;
+    // blocks with more work items than this (intrinsic groups: one item per 256 points they see) sum their item
+    // partials with the whole wave instead of one lane per element
+    constexpr int kFinalizeSerialItems = 16;
     constexpr int kXcds = 8;                 // accelerator dies (XCDs) of an MI355X, each with its own L2
     constexpr int kCgWaves = 8;              // waves per workgroup of the fused PCG iteration
     // SpMV steps per wave whose loads are issued before the scalars are known (more steps cost registers and

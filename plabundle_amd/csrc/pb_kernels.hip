@@ -628,11 +628,44 @@ namespace pbk
         if (idleWindow(P, P.unitWin, u)) return;
         damping = dampingOf(P, P.unitWin, u, damping);
         const bool diag = u == v;
+        const int it0 = P.blockItemOffset[blk], it1 = P.blockItemOffset[blk + 1];
+        if (it1 - it0 > kFinalizeSerialItems)
+        {
+            // many items: lane l sums items l, l + 64, ..., then a butterfly over the lanes; lane i keeps element i
+            // (i < 36) or right-hand side entry i - 36 (36 <= i < 42)
+            double acc[42];
+#pragma unroll
+            for (int i = 0; i < 42; ++i) acc[i] = 0.0;
+            for (int it = it0 + lane; it < it1; it += kWave)
+            {
+#pragma unroll
+                for (int i = 0; i < 36; ++i) acc[i] += double(Wk.itemPartial[it * 36 + i]);
+                if (diag)
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) acc[36 + a] += Wk.itemPartialRhs[it * 6 + a];
+            }
+            double mine = 0.0;
+#pragma unroll
+            for (int i = 0; i < 42; ++i)
+            {
+                const double t = waveSum(acc[i]);
+                if (lane == i) mine = t;
+            }
+            if (lane < 36)
+            {
+                double s = Wk.D[blk * 36 + lane] + mine;
+                if (diag && (lane / 6) == (lane % 6)) s = s == 0.0 ? 1.0 : s * (1.0 + damping);
+                Wk.S[P.blockPosUp[blk] * 36 + lane] = T(s);
+                if (!diag) Wk.S[P.blockPosLo[blk] * 36 + (lane % 6) * 6 + lane / 6] = T(s);
+            }
+            else if (diag && lane < 42)
+                Wk.rhs[u * 6 + lane - 36] = Wk.directRhs[u * 6 + lane - 36] - mine;
+            return;
+        }
         if (lane < 36)
         {
             double s = Wk.D[blk * 36 + lane];
-            for (int it = P.blockItemOffset[blk]; it < P.blockItemOffset[blk + 1]; ++it)
-                s += double(Wk.itemPartial[it * 36 + lane]);
+            for (int it = it0; it < it1; ++it) s += double(Wk.itemPartial[it * 36 + lane]);
             if (diag && (lane / 6) == (lane % 6)) s = s == 0.0 ? 1.0 : s * (1.0 + damping);
             Wk.S[P.blockPosUp[blk] * 36 + lane] = T(s);
             if (!diag) Wk.S[P.blockPosLo[blk] * 36 + (lane % 6) * 6 + lane / 6] = T(s);
@@ -641,8 +674,7 @@ namespace pbk
         {
             const int a = lane - 36;
             double s = 0.0;
-            for (int it = P.blockItemOffset[blk]; it < P.blockItemOffset[blk + 1]; ++it)
-                s += Wk.itemPartialRhs[it * 6 + a];
+            for (int it = it0; it < it1; ++it) s += Wk.itemPartialRhs[it * 6 + a];
             Wk.rhs[u * 6 + a] = Wk.directRhs[u * 6 + a] - s;
         }
     }
