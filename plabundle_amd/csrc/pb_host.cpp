@@ -529,6 +529,9 @@ namespace
         std::vector<int32_t> itemBlock, itemBegin, pairE1, pairE2;
         std::vector<int32_t> rowPtr, colIdx, diagPos;
         std::vector<int32_t> cgWaveRow, cgWaveBegin, cgWaveEnd, cgWaveLead;
+        // lane-per-observation point linearisation: observation and eliminated point of each lane of each wave
+        // (-1 past a wave's last point); the points it cannot take go to the thread-per-point kernel
+        std::vector<int32_t> linLaneObs, linLanePoint, linFallback;
     };
 
     void buildLayout(const Input& in, const Prepared& p, Layout* L)
@@ -651,6 +654,39 @@ namespace
             }
         }
         L->nEntries = static_cast<int>(L->entryUnit.size());
+        // Points packed whole, in order, into waves of 64 observation lanes. A point with no observation, with
+        // more than 64, or with two observations in one camera (two lanes would write one cross block) goes to the
+        // thread-per-point kernel.
+        {
+            int fill = pbk::kWaveLanes;
+            std::vector<int32_t> poses;
+            for (int b = 0; b < L->nElim; ++b)
+            {
+                const int a = L->elimTrack[b];
+                const int o0 = L->obsOffset[a], o1 = L->obsOffset[a + 1], n = o1 - o0;
+                poses.clear();
+                for (int o = o0; o < o1; ++o)
+                    if (L->obsPoseEntry[o] >= 0) poses.push_back(L->obsPoseEntry[o]);
+                std::sort(poses.begin(), poses.end());
+                if (n == 0 || n > pbk::kWaveLanes || std::adjacent_find(poses.begin(), poses.end()) != poses.end())
+                {
+                    L->linFallback.push_back(b);
+                    continue;
+                }
+                if (fill + n > pbk::kWaveLanes)
+                {
+                    L->linLaneObs.insert(L->linLaneObs.end(), pbk::kWaveLanes, -1);
+                    L->linLanePoint.insert(L->linLanePoint.end(), pbk::kWaveLanes, -1);
+                    fill = 0;
+                }
+                const size_t base = L->linLaneObs.size() - pbk::kWaveLanes;
+                for (int o = o0; o < o1; ++o, ++fill)
+                {
+                    L->linLaneObs[base + fill] = o;
+                    L->linLanePoint[base + fill] = b;
+                }
+            }
+        }
         // Schur pairs bucketed by row unit, then sorted by (column unit, first entry)
         const int U = p.nUnits;
         std::vector<int64_t> rowCount(U + 1, 0);
@@ -1438,6 +1474,11 @@ namespace
         P.entryOffset = A.upload(L.entryOffset, s);
         P.entryUnit = A.upload(L.entryUnit, s);
         P.entryPoint = A.upload(L.entryPoint, s);
+        P.nLinWaves = static_cast<int>(L.linLaneObs.size() / pbk::kWaveLanes);
+        P.nLinFallback = static_cast<int>(L.linFallback.size());
+        P.linLaneObs = A.upload(L.linLaneObs, s);
+        P.linLanePoint = A.upload(L.linLanePoint, s);
+        P.linFallback = A.upload(L.linFallback, s);
         P.blockRow = A.upload(L.blockRow, s);
         P.blockCol = A.upload(L.blockCol, s);
         P.blockPosUp = A.upload(L.blockPosUp, s);
@@ -1492,6 +1533,8 @@ namespace
         L.pairE1.shrink_to_fit();
         L.pairE2.clear();
         L.pairE2.shrink_to_fit();
+        L.linLaneObs = std::vector<int32_t>();
+        L.linLanePoint = std::vector<int32_t>();
         ctx->P = P;
         ctx->prep = std::move(p);
         ctx->lay = std::move(L);

@@ -28,6 +28,7 @@ namespace pbk
     // partials with the whole wave instead of one lane per element
     constexpr int kFinalizeSerialItems = 16;
     constexpr int kXcds = 8;                 // accelerator dies (XCDs) of an MI355X, each with its own L2
+    constexpr int kWaveLanes = 64;           // lanes of a wavefront
     constexpr int kCgWaves = 8;              // waves per workgroup of the fused PCG iteration
     // SpMV steps per wave whose loads are issued before the scalars are known (more steps cost registers and
     // with them the occupancy that lets the whole grid be resident at once)
@@ -158,6 +159,12 @@ namespace pbk
         const int32_t* entryOffset = nullptr;  // [nElim+1]
         const int32_t* entryUnit = nullptr;
         const int32_t* entryPoint = nullptr;  // eliminated index
+
+        // point linearisation: lanes of the lane-per-observation waves, and the points left to one thread each
+        int nLinWaves = 0, nLinFallback = 0;
+        const int32_t* linLaneObs = nullptr;    // [nLinWaves*kWaveLanes] observation or -1
+        const int32_t* linLanePoint = nullptr;  // [nLinWaves*kWaveLanes] eliminated point or -1
+        const int32_t* linFallback = nullptr;   // [nLinFallback] eliminated points
 
         // reduced system structure
         const int32_t* blockRow = nullptr;  // upper blocks (row <= col)

@@ -135,74 +135,15 @@ namespace pbk
 
     // ---------------------------------------------------------------------------------------------
     // Point side of the linearisation: Hpp, bp, cross blocks W (6x3 per entry), damped inverse.
-    template <typename T, int KM>
-    __global__ void __launch_bounds__(128) k_linearize_points(Problem P, Work<T> Wk, double damping)
+
+    // The end of one point's linearisation, given the sums H (3x3) and bp over its observations: the point-only
+    // constraints, the damped inverse, its factor Lf (L00, L10, L11, L20, L21, L22 of Hpp^-1 = L L^T), and the
+    // stored Hinv (the factor), bp and y = L^T bp.
+    template <typename T>
+    __device__ __forceinline__ void finishPoint(const Problem& P, const Work<T>& Wk, int b, int t, double X0, double X1,
+                                                double X2, double damping, int32_t* flags, double H[9], double bp[3],
+                                                double Lf[6])
     {
-        const int b = blockIdx.x * blockDim.x + threadIdx.x;
-        if (b >= P.nElim || idleWindow(P, P.elimWin, b)) return;
-        damping = dampingOf(P, P.elimWin, b, damping);
-        int32_t* flags = flagsOf(P, Wk, P.elimWin, b);
-        const int t = P.elimTrack[b];
-        const double X0 = P.pointX[3 * t], X1 = P.pointX[3 * t + 1], X2 = P.pointX[3 * t + 2];
-        const int e0 = P.entryOffset[b], e1 = P.entryOffset[b + 1];
-        for (int e = e0; e < e1; ++e)
-            for (int i = 0; i < 18; ++i) Wk.W[e * 18 + i] = T(0);
-        double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        double bp[3] = {0, 0, 0};
-        Linearization L;
-        for (int o = P.obsOffset[t]; o < P.obsOffset[t + 1]; ++o)
-        {
-            const int c = P.obsCam[o];
-            const int g = P.camGroup[c];
-            const int K = g >= 0 ? P.groupK[g] : 0;
-            const int8_t* act = g >= 0 ? P.groupActive + g * kMaxActive : nullptr;
-            if (!pbdev::linearize<KM>(P.cams[c], X0, X1, X2, P.obsUV[2 * o], P.obsUV[2 * o + 1], P.obsW[o], P.loss,
-                                  P.lossScale, K, act, L, P.refParam != 0))
-            {
-                setFlag(flags, kFlagInvalid);
-                continue;
-            }
-            if (P.adjoint)
-            {
-                double q[3];
-                pbdev::adjointWeightRoot(L, P.obsW[o], P.loss, P.lossScale, q);
-                pbdev::applyWeightRoot<KM>(q, L);
-            }
-            const double w = L.w;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-            {
-                bp[i] -= w * (L.Jp[i] * L.r[0] + L.Jp[3 + i] * L.r[1]);
-#pragma unroll
-                for (int j = 0; j < 3; ++j) H[i * 3 + j] += w * (L.Jp[i] * L.Jp[j] + L.Jp[3 + i] * L.Jp[3 + j]);
-            }
-            const int ep = P.obsPoseEntry[o];
-            if (ep >= 0)
-            {
-                T* Wp = Wk.W + ep * 18;
-#pragma unroll
-                for (int a = 0; a < 6; ++a)
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-                        Wp[a * 3 + i] += T(w * (L.Jc[a] * L.Jp[i] + L.Jc[6 + a] * L.Jp[3 + i]));
-            }
-            const int eg = P.obsGroupEntry[o];
-            if (eg >= 0)
-            {
-                // predicated rather than cut short, so the loop unrolls and Ji stays in registers
-#pragma unroll
-                for (int k = 0; k < KM; ++k)
-                {
-                    if (k < K)
-                    {
-                        T* Wg = Wk.W + (eg + k / 6) * 18 + (k % 6) * 3;
-#pragma unroll
-                        for (int i = 0; i < 3; ++i)
-                            Wg[i] += T(w * (L.Ji[k] * L.Jp[i] + L.Ji[kMaxActive + k] * L.Jp[3 + i]));
-                    }
-                }
-            }
-        }
         // point-only survey constraints (accumulatePointOnlyConstraint, ReferenceSchur.cpp:241-259): w J^T J and
         // -w J^T r; the adjoint build uses the root Q of the exact weight matrix instead (J <- Q J, weight 1)
         if (P.nCons)
@@ -274,7 +215,6 @@ namespace pbk
         }
         // Factor the inverse, Hpp^-1 = L L^T (L lower), and store U = W L in place of each cross block W: then
         // W Hpp^-1 W'^T = U U'^T, W Hpp^-1 bp = U z with z = L^T bp, and Hpp^-1 (bp - W^T dc) = L (z - U^T dc).
-        double Lf[6];  // L00, L10, L11, L20, L21, L22
         {
             const double l00 = sqrt(inv[0]);
             const double l10 = inv[3] / l00, l20 = inv[6] / l00;
@@ -305,16 +245,262 @@ namespace pbk
         Wk.y[3 * b] = Lf[0] * bp[0] + Lf[1] * bp[1] + Lf[3] * bp[2];
         Wk.y[3 * b + 1] = Lf[2] * bp[1] + Lf[4] * bp[2];
         Wk.y[3 * b + 2] = Lf[5] * bp[2];
+    }
+
+    // Cross block U = W L of one entry row: w0 w1 w2 the row of W.
+    __device__ __forceinline__ void crossRow(double w0, double w1, double w2, const double Lf[6], double u[3])
+    {
+        u[0] = w0 * Lf[0] + w1 * Lf[1] + w2 * Lf[3];
+        u[1] = w1 * Lf[2] + w2 * Lf[4];
+        u[2] = w2 * Lf[5];
+    }
+
+    // One thread per point: the points the lane-per-observation kernel does not take (no observation, more than
+    // a wave's lanes of observations, two observations in one camera).
+    template <typename T, int KM>
+    __global__ void __launch_bounds__(128) k_linearize_points(Problem P, Work<T> Wk, double damping)
+    {
+        const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+        if (idx >= P.nLinFallback) return;
+        const int b = P.linFallback[idx];
+        if (idleWindow(P, P.elimWin, b)) return;
+        damping = dampingOf(P, P.elimWin, b, damping);
+        int32_t* flags = flagsOf(P, Wk, P.elimWin, b);
+        const int t = P.elimTrack[b];
+        const double X0 = P.pointX[3 * t], X1 = P.pointX[3 * t + 1], X2 = P.pointX[3 * t + 2];
+        const int e0 = P.entryOffset[b], e1 = P.entryOffset[b + 1];
+        for (int e = e0; e < e1; ++e)
+            for (int i = 0; i < 18; ++i) Wk.W[e * 18 + i] = T(0);
+        double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double bp[3] = {0, 0, 0};
+        Linearization L;
+        for (int o = P.obsOffset[t]; o < P.obsOffset[t + 1]; ++o)
+        {
+            const int c = P.obsCam[o];
+            const int g = P.camGroup[c];
+            const int K = g >= 0 ? P.groupK[g] : 0;
+            const int8_t* act = g >= 0 ? P.groupActive + g * kMaxActive : nullptr;
+            if (!pbdev::linearize<KM>(P.cams[c], X0, X1, X2, P.obsUV[2 * o], P.obsUV[2 * o + 1], P.obsW[o], P.loss,
+                                      P.lossScale, K, act, L, P.refParam != 0))
+            {
+                setFlag(flags, kFlagInvalid);
+                continue;
+            }
+            if (P.adjoint)
+            {
+                double q[3];
+                pbdev::adjointWeightRoot(L, P.obsW[o], P.loss, P.lossScale, q);
+                pbdev::applyWeightRoot<KM>(q, L);
+            }
+            const double w = L.w;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                bp[i] -= w * (L.Jp[i] * L.r[0] + L.Jp[3 + i] * L.r[1]);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) H[i * 3 + j] += w * (L.Jp[i] * L.Jp[j] + L.Jp[3 + i] * L.Jp[3 + j]);
+            }
+            const int ep = P.obsPoseEntry[o];
+            if (ep >= 0)
+            {
+                T* Wp = Wk.W + ep * 18;
+#pragma unroll
+                for (int a = 0; a < 6; ++a)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+                        Wp[a * 3 + i] += T(w * (L.Jc[a] * L.Jp[i] + L.Jc[6 + a] * L.Jp[3 + i]));
+            }
+            const int eg = P.obsGroupEntry[o];
+            if (eg >= 0)
+            {
+                // predicated rather than cut short, so the loop unrolls and Ji stays in registers
+#pragma unroll
+                for (int k = 0; k < KM; ++k)
+                {
+                    if (k < K)
+                    {
+                        T* Wg = Wk.W + (eg + k / 6) * 18 + (k % 6) * 3;
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+                            Wg[i] += T(w * (L.Ji[k] * L.Jp[i] + L.Ji[kMaxActive + k] * L.Jp[3 + i]));
+                    }
+                }
+            }
+        }
+        double Lf[6];
+        finishPoint(P, Wk, b, t, X0, X1, X2, damping, flags, H, bp, Lf);
         for (int e = e0; e < e1; ++e)
         {
             T* We = Wk.W + e * 18;
 #pragma unroll
             for (int a = 0; a < 6; ++a)
             {
-                const double w0 = double(We[a * 3]), w1 = double(We[a * 3 + 1]), w2 = double(We[a * 3 + 2]);
-                We[a * 3] = T(w0 * Lf[0] + w1 * Lf[1] + w2 * Lf[3]);
-                We[a * 3 + 1] = T(w1 * Lf[2] + w2 * Lf[4]);
-                We[a * 3 + 2] = T(w2 * Lf[5]);
+                double u[3];
+                crossRow(double(We[a * 3]), double(We[a * 3 + 1]), double(We[a * 3 + 2]), Lf, u);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) We[a * 3 + i] = T(u[i]);
+            }
+        }
+    }
+
+    // One lane per observation: each wave holds whole points (its lanes are their observations, in track order).
+    // A lane linearises its observation; the point's first lane sums H and bp over the point's lanes in lane
+    // order (the order of the thread-per-point kernel) and finishes the point; then each lane writes its cross
+    // block U = W L straight to its pose entry (one observation per camera and point), and the first lane of
+    // each intrinsic group of the point sums the group's contributions in lane order and writes the group's
+    // entries. Each cross block is written once: no clearing and no read-modify-write pass over W.
+    constexpr int kLinThreads = 128;
+
+    template <typename T, int KM>
+    __global__ void __launch_bounds__(kLinThreads) k_linearize_lanes(Problem P, Work<T> Wk, double damping)
+    {
+        constexpr int kWaves = kLinThreads / kWave;
+        constexpr int kG = KM > 0 ? 3 * KM : 1;  // group contribution: K rows of 3
+        // H upper triangle (00 01 02 11 12 22) and bp terms of each lane; then the point factor at its first lane
+        __shared__ double sHb[kWaves][9][kWave];
+        __shared__ T sG[kWaves][kG][kWave];
+        __shared__ int32_t sKey[kWaves][kWave];
+        const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+        const int wv = blockIdx.x * kWaves + wave;
+        if (wv >= P.nLinWaves) return;
+        const int o = P.linLaneObs[wv * kWave + lane];
+        const int b = P.linLanePoint[wv * kWave + lane];
+        // segment of this lane's point: [first, end) lanes
+        const int prev = __shfl(b, lane > 0 ? lane - 1 : 0, kWave);
+        const unsigned long long heads = __ballot(b >= 0 && (lane == 0 || prev != b));
+        const unsigned long long filled = __ballot(b >= 0);
+        const unsigned long long upTo = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1;
+        const int first = kWave - 1 - __clzll(heads & upTo);
+        const unsigned long long after = heads & ~upTo;
+        const int end = after ? __ffsll(static_cast<long long>(after)) - 1 : __popcll(filled);
+        const bool live = b >= 0 && !idleWindow(P, P.elimWin, b);
+
+        double hb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        T wp[18];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) wp[i] = T(0);
+        int ep = -1, eg = -1, K = 0, t = 0;
+        double X0 = 0.0, X1 = 0.0, X2 = 0.0;
+        int32_t* flags = nullptr;
+        if (live)
+        {
+            flags = flagsOf(P, Wk, P.elimWin, b);
+            t = P.elimTrack[b];
+            X0 = P.pointX[3 * t];
+            X1 = P.pointX[3 * t + 1];
+            X2 = P.pointX[3 * t + 2];
+            const int c = P.obsCam[o];
+            const int g = P.camGroup[c];
+            K = g >= 0 ? P.groupK[g] : 0;
+            ep = P.obsPoseEntry[o];
+            eg = P.obsGroupEntry[o];
+            const int8_t* act = g >= 0 ? P.groupActive + g * kMaxActive : nullptr;
+            Linearization L;
+            if (!pbdev::linearize<KM>(P.cams[c], X0, X1, X2, P.obsUV[2 * o], P.obsUV[2 * o + 1], P.obsW[o], P.loss,
+                                      P.lossScale, K, act, L, P.refParam != 0))
+            {
+                setFlag(flags, kFlagInvalid);
+#pragma unroll
+                for (int k = 0; k < kG; ++k) sG[wave][k][lane] = T(0);
+            }
+            else
+            {
+                if (P.adjoint)
+                {
+                    double q[3];
+                    pbdev::adjointWeightRoot(L, P.obsW[o], P.loss, P.lossScale, q);
+                    pbdev::applyWeightRoot<KM>(q, L);
+                }
+                const double w = L.w;
+                int h = 0;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = i; j < 3; ++j) hb[h++] = w * (L.Jp[i] * L.Jp[j] + L.Jp[3 + i] * L.Jp[3 + j]);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) hb[6 + i] = w * (L.Jp[i] * L.r[0] + L.Jp[3 + i] * L.r[1]);
+#pragma unroll
+                for (int a = 0; a < 6; ++a)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) wp[a * 3 + i] = T(w * (L.Jc[a] * L.Jp[i] + L.Jc[6 + a] * L.Jp[3 + i]));
+#pragma unroll
+                for (int k = 0; k < KM; ++k)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+                        sG[wave][k * 3 + i][lane] =
+                            k < K ? T(w * (L.Ji[k] * L.Jp[i] + L.Ji[kMaxActive + k] * L.Jp[3 + i])) : T(0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sHb[wave][i][lane] = hb[i];
+        sKey[wave][lane] = eg;
+        waveLdsFence();
+        if (live && lane == first)
+        {
+            double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            double bp[3] = {0, 0, 0};
+            for (int j = first; j < end; ++j)
+            {
+                int h = 0;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int m = i; m < 3; ++m) H[i * 3 + m] += sHb[wave][h++][j];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) bp[i] -= sHb[wave][6 + i][j];
+            }
+            H[3] = H[1];
+            H[6] = H[2];
+            H[7] = H[5];
+            double Lf[6];
+            finishPoint(P, Wk, b, t, X0, X1, X2, dampingOf(P, P.elimWin, b, damping), flags, H, bp, Lf);
+            // only this lane reads its point's slots: they take the factor
+#pragma unroll
+            for (int i = 0; i < 6; ++i) sHb[wave][i][lane] = Lf[i];
+        }
+        waveLdsFence();
+        if (!live) return;
+        double Lf[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) Lf[i] = sHb[wave][i][first];
+        if (ep >= 0)
+        {
+            T* We = Wk.W + size_t(ep) * 18;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+            {
+                double u[3];
+                crossRow(double(wp[a * 3]), double(wp[a * 3 + 1]), double(wp[a * 3 + 2]), Lf, u);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) We[a * 3 + i] = T(u[i]);
+            }
+        }
+        if (KM > 0 && eg >= 0)
+        {
+            bool head = true;
+            for (int j = first; j < lane; ++j) head = head && sKey[wave][j] != eg;
+            if (head)
+            {
+                T g[kG];
+#pragma unroll
+                for (int k = 0; k < kG; ++k) g[k] = T(0);
+                for (int j = lane; j < end; ++j)
+                    if (sKey[wave][j] == eg)
+#pragma unroll
+                        for (int k = 0; k < kG; ++k) g[k] += sG[wave][k][j];
+                // the group's units hold rows k = 0 .. K-1 of the contribution, six per unit; the rest are zero
+                constexpr int kRows = (KM + 5) / 6 * 6;
+                const int units = (K + 5) / 6;
+#pragma unroll
+                for (int k = 0; k < kRows; ++k)
+                {
+                    if (k / 6 >= units) break;
+                    double u[3] = {0.0, 0.0, 0.0};
+                    if (k < KM && k < K) crossRow(double(g[3 * k]), double(g[3 * k + 1]), double(g[3 * k + 2]), Lf, u);
+                    T* We = Wk.W + size_t(eg + k / 6) * 18 + (k % 6) * 3;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) We[i] = T(u[i]);
+                }
             }
         }
     }
@@ -1623,15 +1809,24 @@ namespace pbk
     template <typename T>
     void Launcher<T>::linearizePoints(const Problem& P, const Work<T>& W, double damping, hipStream_t s)
     {
+        auto launch = [&](auto km)
+        {
+            constexpr int KM = decltype(km)::value;
+            if (P.nLinWaves > 0)
+                k_linearize_lanes<T, KM>
+                    <<<blocksFor(P.nLinWaves, kLinThreads / kWave), kLinThreads, 0, s>>>(P, W, damping);
+            if (P.nLinFallback > 0)
+                k_linearize_points<T, KM><<<blocksFor(P.nLinFallback, 128), 128, 0, s>>>(P, W, damping);
+        };
         if (P.nElim == 0) return;
         if (P.maxGroupK == 0)
-            k_linearize_points<T, 0><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
+            launch(std::integral_constant<int, 0>());
         else if (P.maxGroupK <= 6)
-            k_linearize_points<T, 6><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
+            launch(std::integral_constant<int, 6>());
         else if (P.maxGroupK <= 12)
-            k_linearize_points<T, 12><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
+            launch(std::integral_constant<int, 12>());
         else
-            k_linearize_points<T, 13><<<blocksFor(P.nElim, 128), 128, 0, s>>>(P, W, damping);
+            launch(std::integral_constant<int, 13>());
     }
 
     template <typename T>
