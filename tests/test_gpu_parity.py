@@ -126,6 +126,57 @@ def test_full_solve_fp64_matches_oracle(ws, oracle_mod, intrinsics):
     np.testing.assert_array_equal(r.track_valid, st["track_valid"])
 
 
+def _track_length_mix(seed=5):
+    # a ring scene whose tracks cover both point linearisation kernels: long tracks (more than a wave's 64 lanes
+    # of observations) and tracks that see one camera twice go one thread per point, short tracks one lane per
+    # observation
+    p = S.ring_problem(seed=seed, num_cameras=72, num_points=240, max_views=72)
+    off = p.track_offset
+    cams, uvs, lens = [], [], []
+    for t in range(len(off) - 1):
+        c = p.obs_camera[off[t]:off[t + 1]]
+        uv = p.obs_uv[off[t]:off[t + 1]]
+        if t % 3 == 1:
+            c, uv = c[:6], uv[:6]
+        elif t % 3 == 2:  # a second observation in the first camera, half a pixel away
+            c = np.concatenate([c[:5], c[:1]])
+            uv = np.concatenate([uv[:5], uv[:1] + 0.5])
+        cams.append(c)
+        uvs.append(uv)
+        lens.append(len(c))
+    q = p.copy()
+    q.track_offset = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    q.obs_camera = np.concatenate(cams).astype(np.int32)
+    q.obs_uv = np.ascontiguousarray(np.concatenate(uvs))
+    q.obs_weight = np.ones(len(q.obs_camera))
+    q.obs_scale = np.ones(len(q.obs_camera))
+    assert max(lens) > 64
+    return q
+
+
+@pytest.mark.parametrize("intrinsics", [False, True])
+def test_long_and_repeated_camera_tracks_match_oracle(ws, oracle_mod, intrinsics):
+    _say(f"case track length mix intrinsics={intrinsics}")
+    p = _track_length_mix()
+    o = S.ring_options(PO.SolveOptions(), max_iterations=8)
+    if intrinsics:
+        o.calibration.refineSharedFocalLength = True
+        o.calibration.refineSharedRadialDistortion = True
+    r = Solver().solve(p, o, ws, precision=PO.Precision.Fp64)
+    orc = oracle_mod.Oracle(p, threads=8)
+    ref = orc.solve(PO.to_c(o))
+    st = orc.state()
+    orc.close()
+    _say(f"  {r.report['iterations']} it, cost {r.finalCost:.12g} vs {ref['final_cost']:.12g}")
+    assert r.usable(), r.backendMessage
+    assert r.report["iterations"] == ref["iterations"]
+    assert r.report["accepted_steps"] == ref["accepted_steps"]
+    assert abs(r.finalCost - ref["final_cost"]) <= 1e-9 * ref["final_cost"]
+    np.testing.assert_allclose(r.cam_center, st["cam_center"], atol=1e-7)
+    np.testing.assert_allclose(r.track_point, st["track_point"], atol=1e-6)
+    np.testing.assert_allclose(r.cam_intrinsics, st["cam_intrinsics"], rtol=1e-7, atol=1e-9)
+
+
 @pytest.mark.parametrize("route", [PO.ReducedSolver.Dense, PO.ReducedSolver.Pcg])
 def test_full_solve_mixed_within_north_star_tolerance(ws, oracle_mod, route):
     _say(f"case solve mixed aerial-small route={route.name}")
