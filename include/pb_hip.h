@@ -173,6 +173,24 @@ typedef struct pb_problem_view {
     const double* lp_point;         /* [n*3] point on the plane */
     const double* lp_normal;        /* [n*3] plane normal (non-zero) */
     const double* lp_weight;        /* [n] weight (> 0), or NULL = 1 */
+
+    /* Camera rig (Problem::rig, problem.h:52; placamera::RigTopology): captures (rig poses), sensors (camera
+     * poses in the rig) and one binding (capture, sensor) per bound camera. A bound camera's pose is the
+     * composition of its capture and its sensor: camera-to-world R = R_rig R_sensor, centre C = C_rig + R_rig
+     * c_sensor (runConcreteBackend composes the rig before the solve, solver.cpp:282); cam_rotation /
+     * cam_center of bound cameras are ignored. The solve refines capture and sensor poses instead of camera
+     * poses (BundleAdjustPlaMatrixProblem.cpp:157-199, Assembly.cpp:207-284, 1152-1208) on the assembled Schur
+     * path. With no captures and no sensors the problem has no rig. */
+    int32_t num_rig_captures;
+    const double* rig_capture_rotation;  /* [n*9] rig-to-world, row-major (RigCapture::rigToWorldRotation) */
+    const double* rig_capture_center;    /* [n*3] rig centre in the world (RigCapture::rigCenterInWorld) */
+    const int32_t* rig_capture_fixed;    /* [n] RigCapture::fixedPose, or NULL = none fixed */
+    int32_t num_rig_sensors;
+    const double* rig_sensor_rotation;   /* [n*9] camera-to-rig, row-major (RigSensor::cameraToRigRotation) */
+    const double* rig_sensor_center;     /* [n*3] camera centre in the rig frame (RigSensor::cameraCenterInRig) */
+    const int32_t* rig_sensor_fixed;     /* [n] RigSensor::fixedExtrinsic, or NULL = none fixed */
+    const int32_t* cam_rig_capture;      /* [num_cameras] capture of the camera's binding, -1 = unbound */
+    const int32_t* cam_rig_sensor;       /* [num_cameras] sensor of the camera's binding, -1 = unbound */
 } pb_problem_view;
 
 /* Solve options. Mirrors the fields of plabundle::Options
@@ -358,6 +376,19 @@ pb_status pb_hip_download(pb_ctx* ctx, double* cam_rotation, double* cam_center,
  * Arrays sized [num_cameras] / [num_tracks]; any pointer may be NULL. */
 pb_status pb_hip_get_masks(pb_ctx* ctx, int32_t* camera_block, int32_t* intrinsic_block_by_camera,
                            int32_t* calibration_group_by_camera, int32_t* track_block, int32_t* active_track);
+
+/* Rig block numbering of the last solve: ActiveProblem::rigCaptureBlockByCamera and rigSensorBlockByCamera
+ * (BundleAdjustPlaMatrixProblem.h:13-39, numbered by BundleAdjustPlaMatrixProblem.cpp:157-199), arrays sized
+ * [num_cameras], -1 where a camera is unbound or its capture / sensor is not refined; any pointer may be NULL.
+ * camera_block of pb_hip_get_masks is -1 for every camera of a rig problem, as in the reference. */
+pb_status pb_hip_get_rig_masks(pb_ctx* ctx, int32_t* capture_block, int32_t* sensor_block);
+
+/* Refined rig of the last solve (BAResult::refinedRig, BundleAdjustPlaMatrix.cpp:601): capture rotations
+ * [num_rig_captures*9] and centres [*3], sensor rotations [num_rig_sensors*9] and centres [*3], in the layout
+ * of pb_problem_view. Any pointer may be NULL. Without a solve (or when the solve published nothing) the input
+ * rig is returned, as the reference does (:172, :574). */
+pb_status pb_hip_download_rig(pb_ctx* ctx, double* capture_rotation, double* capture_center,
+                              double* sensor_rotation, double* sensor_center);
 
 /* Kernel timing collected when options.profile_kernels was set.
  * name: kernel family ("linearize_points", "camera_direct", "schur_blocks", "pcg_iteration", "cost",

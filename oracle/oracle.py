@@ -43,6 +43,10 @@ def lib():
     L.oracle_solve.restype = C.c_int
     L.oracle_download.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _ip]
     L.oracle_masks.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip, _ip]
+    L.oracle_rig_masks.argtypes = [C.c_void_p, _ip, _ip]
+    L.oracle_rig_masks.restype = None
+    L.oracle_download_rig.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.oracle_download_rig.restype = None
     L.oracle_prepare.argtypes = [C.c_void_p, C.POINTER(_abi.SolveOptionsC), C.POINTER(_abi.SolveReportC)]
     L.oracle_prepare.restype = C.c_int
     L.oracle_reduced_system.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp, _ip]
@@ -189,6 +193,21 @@ class Oracle:
         lib().oracle_masks(self.ctx, *(_p(a, C.c_int32) for a in (cb, ib, gb, tb, at)))
         return {"camera_block": cb, "intrinsic_block_by_camera": ib, "calibration_group_by_camera": gb,
                 "track_block": tb, "active_track": at}
+
+    def rig_masks(self):
+        n = self.problem.num_cameras
+        kb = np.zeros(n, dtype=np.int32)
+        sb = np.zeros(n, dtype=np.int32)
+        lib().oracle_rig_masks(self.ctx, _p(kb, C.c_int32), _p(sb, C.c_int32))
+        return {"capture_block": kb, "sensor_block": sb}
+
+    def rig_state(self):
+        p = self.problem
+        out = {"capture_rotation": np.zeros((p.num_rig_captures, 9)), "capture_center": np.zeros((p.num_rig_captures, 3)),
+               "sensor_rotation": np.zeros((p.num_rig_sensors, 9)), "sensor_center": np.zeros((p.num_rig_sensors, 3))}
+        lib().oracle_download_rig(self.ctx, *(_p(out[k]) for k in ("capture_rotation", "capture_center",
+                                                                    "sensor_rotation", "sensor_center")))
+        return out
 
     def reduced_system(self, primary_blocks: int, damping: float = 0.0, dense: bool = True):
         n = primary_blocks * 13

@@ -600,8 +600,9 @@ namespace
         int gaugePolicy = PB_GAUGE_AUTO_ANCHOR;
         bool controlPoints = false;  // enableControlPointConstraints (Conversion.cpp:106)
         bool laserPlanes = false;    // enableLaserPlaneConstraints (Conversion.cpp:103)
+        bool rig = false;            // !options.rig.empty() (Conversion.cpp:144)
         // useReferencePointParameterization (Assembly.cpp:24-29)
-        bool referenceParam() const { return o.use_online_schur != 0 && !controlPoints && !laserPlanes; }
+        bool referenceParam() const { return o.use_online_schur != 0 && !rig && !controlPoints && !laserPlanes; }
         bool enabled(int slot) const { return (o.shared_intrinsic_mask >> slot) & 1u; }
     };
 
@@ -609,7 +610,10 @@ namespace
     {
         std::vector<char> activeTrack;
         std::vector<int> cameraBlock, intrinsicBlockByCamera, calibrationGroupByCamera, trackBlock;
+        std::vector<int> rigCaptureBlockByCamera, rigSensorBlockByCamera;
+        std::vector<int> rigCaptureBlock, rigSensorBlock;  // per capture / sensor, -1 when not refined
         int primaryBlockCount = 0, cameraBlockCount = 0, intrinsicBlockCount = 0, trackBlockCount = 0;
+        int rigCaptureBlockCount = 0, rigSensorBlockCount = 0;
         int activeTrackCount = 0, observationCount = 0, rejectedInitialTracks = 0;
     };
 
@@ -620,11 +624,21 @@ namespace
         double focalRef = 1.0, aspectRef = 1.0;
     };
 
+    // Rig poses of the state (OptimizationState::rig, BundleAdjustPlaMatrixAssembly.h:27): captures carry the
+    // rig-to-world rotation and the rig centre, sensors the camera-to-rig rotation and the camera centre in the
+    // rig frame.
+    struct RigPoses
+    {
+        std::vector<Mat3> capR, senR;
+        std::vector<Vec3> capC, senC;
+    };
+
     struct State
     {
         std::vector<Camera> cams;
         std::vector<Vec3> points;
         std::vector<Group> groups;
+        RigPoses rig;
     };
 
     // finiteMedian: unpinned choice - odd count: middle value; even count: mean of the two middle values.
@@ -637,8 +651,45 @@ namespace
         return n % 2 ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
     }
 
+    // Rig topology of the problem: poses, fixed flags and per camera its binding (capture, sensor), -1 when
+    // unbound.
+    struct Rig
+    {
+        RigPoses poses;
+        std::vector<char> capFixed, senFixed;
+        std::vector<int> camCapture, camSensor;
+        bool empty() const { return poses.capR.empty() && poses.senR.empty(); }
+        bool bound(int c) const
+        {
+            return !empty() && c >= 0 && c < static_cast<int>(camCapture.size()) && camCapture[c] >= 0 &&
+                   camSensor[c] >= 0;
+        }
+    };
+
+    // composeRigCameras (placamera, called at solver.cpp:282 and Assembly.cpp:1203): a bound camera's
+    // camera-to-world rotation is R_rig R_sensor and its centre C_rig + R_rig c_sensor. The composition is
+    // PlaCamera's and not in the tree; this form is the one the rig Jacobians of cameraPosePrimaryTerms
+    // (Assembly.cpp:224-281) differentiate: the lever arm R_rig c_sensor and the rig-frame sensor axes R_rig.
+    void composeRig(const Rig& rig, const RigPoses& poses, std::vector<Camera>* cams)
+    {
+        for (std::size_t c = 0; c < cams->size(); ++c)
+        {
+            if (!rig.bound(static_cast<int>(c))) continue;
+            const int k = rig.camCapture[c], s = rig.camSensor[c];
+            Camera& cam = (*cams)[c];
+            cam.R = mul3(poses.capR[k], poses.senR[s]);
+            for (int i = 0; i < 3; ++i)
+            {
+                double v = poses.capC[k][i];
+                for (int j = 0; j < 3; ++j) v += poses.capR[k][i * 3 + j] * poses.senC[s][j];
+                cam.C[i] = v;
+            }
+        }
+    }
+
     struct Problem
     {
+        Rig rig;
         std::vector<Camera> cams;
         std::vector<Slots> refIntr;
         std::vector<int> groupIds;
@@ -650,6 +701,7 @@ namespace
         int gaugeAnchor = -1, gaugeScale = -1;
         double gaugeBaseline = 0.0;
         std::string constraintError;  // validateProblem's verdict on the survey constraints (problem.cpp:241-259)
+        std::string rigError;         // the rig could not be composed (runConcreteBackend, solver.cpp:282-287)
     };
 
     bool hasSharedIntrinsics(const Options& opt)
@@ -839,14 +891,48 @@ namespace
             for (const Observation& o : tr.obs)
                 if (observationUsable(o, nc)) hasResidual[o.cam] = 1;
         }
-        if (opt.o.refine_camera_pose)
+        auto isFixed = [&](int c)
+        { return std::find(opt.fixedCameras.begin(), opt.fixedCameras.end(), c) != opt.fixedCameras.end(); };
+        const Rig& rig = pb.rig;
+        a.rigCaptureBlockByCamera.assign(nc, -1);
+        a.rigSensorBlockByCamera.assign(nc, -1);
+        a.rigCaptureBlock.assign(rig.poses.capR.size(), -1);
+        a.rigSensorBlock.assign(rig.poses.senR.size(), -1);
+        if (opt.o.refine_camera_pose && rig.empty())
         {
             for (std::size_t c = 0; c < nc; ++c)
+                if (hasResidual[c] && !isFixed(static_cast<int>(c))) a.cameraBlock[c] = a.cameraBlockCount++;
+        }
+        else if (opt.o.refine_camera_pose)
+        {
+            // BundleAdjustPlaMatrixProblem.cpp:157-199: a capture is refined when one of its cameras has a residual
+            // and it is not fixed, none of its cameras is fixed and none is the gauge scale camera; a sensor when
+            // one of its cameras has a residual and its extrinsic is not fixed; sensor blocks follow the captures
+            for (std::size_t k = 0; k < rig.poses.capR.size(); ++k)
             {
-                const bool fixed = std::find(opt.fixedCameras.begin(), opt.fixedCameras.end(),
-                                             static_cast<int>(c)) != opt.fixedCameras.end();
-                if (hasResidual[c] && !fixed) a.cameraBlock[c] = a.cameraBlockCount++;
+                bool residual = false, fixed = rig.capFixed[k] != 0;
+                for (std::size_t c = 0; c < nc; ++c)
+                    if (rig.camCapture[c] == static_cast<int>(k))
+                    {
+                        residual = residual || hasResidual[c];
+                        fixed = fixed || isFixed(static_cast<int>(c)) || opt.gaugeScale == static_cast<int>(c);
+                    }
+                if (residual && !fixed) a.rigCaptureBlock[k] = a.rigCaptureBlockCount++;
             }
+            for (std::size_t k = 0; k < rig.poses.senR.size(); ++k)
+            {
+                bool residual = false;
+                for (std::size_t c = 0; c < nc; ++c)
+                    residual = residual || (rig.camSensor[c] == static_cast<int>(k) && hasResidual[c]);
+                if (residual && !rig.senFixed[k])
+                    a.rigSensorBlock[k] = a.rigCaptureBlockCount + a.rigSensorBlockCount++;
+            }
+            for (std::size_t c = 0; c < nc; ++c)
+            {
+                if (rig.camCapture[c] >= 0) a.rigCaptureBlockByCamera[c] = a.rigCaptureBlock[rig.camCapture[c]];
+                if (rig.camSensor[c] >= 0) a.rigSensorBlockByCamera[c] = a.rigSensorBlock[rig.camSensor[c]];
+            }
+            a.cameraBlockCount = a.rigCaptureBlockCount + a.rigSensorBlockCount;
         }
         a.primaryBlockCount = a.cameraBlockCount;
         if (hasSharedIntrinsics(opt))
@@ -1071,19 +1157,68 @@ namespace
                              origin[2] + dir[2] * il * opt.gaugeBaseline});
     }
 
-    // observationPrimaryTerms (Assembly.cpp:186-342): up to two (block, 2x13 Jacobian) terms.
+    // observationPrimaryTerms (Assembly.cpp:186-342): up to three (block, 2x13 Jacobian) terms: the camera
+    // pose (or a rig camera's capture and sensor) and the intrinsic group.
     struct Terms
     {
         int count = 0;
-        int blocks[2];
-        double J[2][2 * kP];
+        int blocks[3];
+        double J[3][2 * kP];
     };
 
     Terms primaryTerms(const Options& opt, const Active& a, const State& st, int cam, const Linearization& L,
-                       const std::array<Vec3, 2>* basis)
+                       const std::array<Vec3, 2>* basis, const Rig* rig = nullptr)
     {
         Terms t;
-        if (a.cameraBlock[cam] >= 0)
+        if (rig && !rig->empty())
+        {
+            // cameraPosePrimaryTerms, rig branch (Assembly.cpp:207-284): the camera's pose Jacobian [rotation |
+            // centre] (world-frame increments) chained to its capture (rotation moves the lever arm
+            // l = R_rig c_sensor: dC/dw_a = e_a x l) and to its sensor (increments in rig axes: columns R_rig e_a)
+            if (rig->bound(cam))
+            {
+                const Mat3& Rr = st.rig.capR[rig->camCapture[cam]];
+                const Vec3& cs = st.rig.senC[rig->camSensor[cam]];
+                Vec3 lever{};
+                for (int w = 0; w < 3; ++w)
+                    for (int r = 0; r < 3; ++r) lever[w] += Rr[w * 3 + r] * cs[r];
+                if (a.rigCaptureBlockByCamera[cam] >= 0)
+                {
+                    double* J = t.J[t.count];
+                    std::fill(J, J + 2 * kP, 0.0);
+                    for (int r = 0; r < 2; ++r)
+                    {
+                        const double* src = L.Jpose + r * 6;
+                        for (int ax = 0; ax < 3; ++ax)
+                        {
+                            Vec3 e{};
+                            e[ax] = 1.0;
+                            const Vec3 d = cross(e, lever);
+                            J[r * kP + ax] = src[ax] + src[3] * d[0] + src[4] * d[1] + src[5] * d[2];
+                            J[r * kP + 3 + ax] = src[3 + ax];
+                        }
+                    }
+                    t.blocks[t.count++] = a.rigCaptureBlockByCamera[cam];
+                }
+                if (a.rigSensorBlockByCamera[cam] >= 0)
+                {
+                    double* J = t.J[t.count];
+                    std::fill(J, J + 2 * kP, 0.0);
+                    for (int r = 0; r < 2; ++r)
+                    {
+                        const double* src = L.Jpose + r * 6;
+                        for (int ax = 0; ax < 3; ++ax)
+                            for (int w = 0; w < 3; ++w)
+                            {
+                                J[r * kP + ax] += src[w] * Rr[w * 3 + ax];
+                                J[r * kP + 3 + ax] += src[3 + w] * Rr[w * 3 + ax];
+                            }
+                    }
+                    t.blocks[t.count++] = a.rigSensorBlockByCamera[cam];
+                }
+            }
+        }
+        else if (a.cameraBlock[cam] >= 0)
         {
             double* J = t.J[0];
             std::fill(J, J + 2 * kP, 0.0);
@@ -1178,13 +1313,15 @@ namespace
             for (const Observation& ob : o.pb.tracks[t].obs)
             {
                 if (!observationUsable(ob, o.pb.cams.size())) continue;
-                const int b0 = a.cameraBlock[ob.cam], b1 = a.intrinsicBlockByCamera[ob.cam];
-                connect(b0, b1);
+                // the primary blocks of the observation: camera pose (or rig capture and sensor), intrinsics
+                const int bs[3] = {o.pb.rig.empty() ? a.cameraBlock[ob.cam] : a.rigCaptureBlockByCamera[ob.cam],
+                                   o.pb.rig.empty() ? -1 : a.rigSensorBlockByCamera[ob.cam],
+                                   a.intrinsicBlockByCamera[ob.cam]};
+                for (int i = 0; i < 3; ++i)
+                    for (int j = i + 1; j < 3; ++j) connect(bs[i], bs[j]);
                 if (a.trackBlock[t] >= 0)
-                {
-                    if (b0 >= 0) blocks.push_back(b0);
-                    if (b1 >= 0) blocks.push_back(b1);
-                }
+                    for (int b : bs)
+                        if (b >= 0) blocks.push_back(b);
             }
             if (a.trackBlock[t] < 0) continue;
             std::sort(blocks.begin(), blocks.end());
@@ -1320,10 +1457,10 @@ namespace
                         continue;
                     }
                     cost += L.cost;
-                    const Terms terms = primaryTerms(opt, a, st, ob.cam, L, tangent ? &basis : nullptr);
+                    const Terms terms = primaryTerms(opt, a, st, ob.cam, L, tangent ? &basis : nullptr, &o.pb.rig);
                     // accumulateObservationPrimaryTerms (ReferenceSchur.cpp:140-193)
-                    int act_idx[2][kP];
-                    int act_n[2] = {0, 0};
+                    int act_idx[3][kP];
+                    int act_n[3] = {0, 0, 0};
                     for (int k = 0; k < terms.count; ++k)
                     {
                         for (int q = 0; q < kP; ++q)
@@ -1823,7 +1960,21 @@ namespace
         if (alpha != 1.0)
             for (double& v : sp) v *= alpha;
         const State before = *st;
-        for (std::size_t c = 0; c < st->cams.size(); ++c)
+        if (!o.pb.rig.empty())
+        {
+            // rig branch (Assembly.cpp:1152-1208): applyPoseDelta on every refined capture and sensor (the
+            // world-frame / rig-frame left retraction of the rig Jacobians), then the cameras are composed again
+            for (std::size_t k = 0; k < a.rigCaptureBlock.size(); ++k)
+                if (a.rigCaptureBlock[k] >= 0)
+                    retractExp(&st->rig.capR[k], &st->rig.capC[k],
+                               sp.data() + static_cast<std::size_t>(a.rigCaptureBlock[k]) * kP);
+            for (std::size_t k = 0; k < a.rigSensorBlock.size(); ++k)
+                if (a.rigSensorBlock[k] >= 0)
+                    retractExp(&st->rig.senR[k], &st->rig.senC[k],
+                               sp.data() + static_cast<std::size_t>(a.rigSensorBlock[k]) * kP);
+            composeRig(o.pb.rig, st->rig, &st->cams);
+        }
+        for (std::size_t c = 0; c < st->cams.size() && o.pb.rig.empty(); ++c)
         {
             const int b = a.cameraBlock[c];
             if (b < 0) continue;
@@ -1930,7 +2081,7 @@ namespace
     // scale bars, pose priors or a camera plane.
     bool onlineSchur(const Oracle& o)
     {
-        return o.opt.o.use_online_schur != 0 && o.act.primaryBlockCount > 0 && !o.opt.laserPlanes;
+        return o.opt.o.use_online_schur != 0 && o.act.primaryBlockCount > 0 && !o.opt.laserPlanes && !o.opt.rig;
     }
 
     void finishReport(Oracle& o, const State& st, double current, bool converged, bool cancelled, bool failed,
@@ -2262,12 +2413,13 @@ namespace
             setMessage(&o.report, "invalid options");
             return false;
         }
-        if (!o.pb.constraintError.empty())
+        if (!o.pb.constraintError.empty() || !o.pb.rigError.empty())
         {
             o.report.solve_status = PB_SOLVE_INVALID_INPUT;
-            setMessage(&o.report, o.pb.constraintError);
+            setMessage(&o.report, o.pb.rigError.empty() ? o.pb.constraintError : o.pb.rigError);
             return false;
         }
+        o.opt.rig = !o.pb.rig.empty();
         for (const Track& tr : o.pb.tracks)
         {
             o.opt.controlPoints = o.opt.controlPoints || !tr.controls.empty();
@@ -2290,6 +2442,7 @@ namespace
         }
         o.act = prepareActive(o.pb, o.opt);
         o.init.cams = o.pb.cams;
+        o.init.rig = o.pb.rig.poses;
         o.init.points.resize(o.pb.tracks.size());
         for (std::size_t t = 0; t < o.pb.tracks.size(); ++t) o.init.points[t] = o.pb.tracks[t].X;
         o.init.groups = initGroups(o.pb, o.opt, o.act);
@@ -2416,6 +2569,88 @@ oracle_ctx* oracle_create(const pb_problem_view* v)
         }
     }
     if (v->fixed_cameras) pb.fixedCameras.assign(v->fixed_cameras, v->fixed_cameras + v->num_fixed_cameras);
+    // camera rig: every camera bound to a capture and a sensor, rotations orthonormal; the bound cameras are
+    // composed from it (solver.cpp:282); without calibration group ids each sensor is a group and unbound cameras
+    // take group 0 (Conversion.cpp:83-98); a camera whose capture and sensor are both fixed is a fixed camera
+    // (Conversion.cpp:120-142)
+    if (v->num_rig_captures > 0 || v->num_rig_sensors > 0)
+    {
+        Rig& rig = pb.rig;
+        auto rotationOk = [](const double* R)
+        {
+            for (int i = 0; i < 9; ++i)
+                if (!std::isfinite(R[i])) return false;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                {
+                    double d = 0.0;
+                    for (int k = 0; k < 3; ++k) d += R[k * 3 + i] * R[k * 3 + j];
+                    if (std::abs(d - (i == j ? 1.0 : 0.0)) > 1e-6) return false;
+                }
+            const double det = R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+                               R[2] * (R[3] * R[7] - R[4] * R[6]);
+            return det > 0.0;
+        };
+        const int nk = std::max(0, v->num_rig_captures), ns = std::max(0, v->num_rig_sensors);
+        rig.poses.capR.resize(nk);
+        rig.poses.capC.resize(nk);
+        rig.capFixed.assign(nk, 0);
+        rig.poses.senR.resize(ns);
+        rig.poses.senC.resize(ns);
+        rig.senFixed.assign(ns, 0);
+        for (int k = 0; k < nk; ++k)
+        {
+            std::copy(v->rig_capture_rotation + 9 * k, v->rig_capture_rotation + 9 * k + 9, rig.poses.capR[k].begin());
+            std::copy(v->rig_capture_center + 3 * k, v->rig_capture_center + 3 * k + 3, rig.poses.capC[k].begin());
+            rig.capFixed[k] = v->rig_capture_fixed && v->rig_capture_fixed[k] ? 1 : 0;
+            if (!rotationOk(rig.poses.capR[k].data()) || !std::isfinite(rig.poses.capC[k][0]) ||
+                !std::isfinite(rig.poses.capC[k][1]) || !std::isfinite(rig.poses.capC[k][2]))
+                pb.rigError = "rig capture " + std::to_string(k) + " is not a valid rigid transform";
+        }
+        for (int k = 0; k < ns; ++k)
+        {
+            std::copy(v->rig_sensor_rotation + 9 * k, v->rig_sensor_rotation + 9 * k + 9, rig.poses.senR[k].begin());
+            std::copy(v->rig_sensor_center + 3 * k, v->rig_sensor_center + 3 * k + 3, rig.poses.senC[k].begin());
+            rig.senFixed[k] = v->rig_sensor_fixed && v->rig_sensor_fixed[k] ? 1 : 0;
+            if (!rotationOk(rig.poses.senR[k].data()) || !std::isfinite(rig.poses.senC[k][0]) ||
+                !std::isfinite(rig.poses.senC[k][1]) || !std::isfinite(rig.poses.senC[k][2]))
+                pb.rigError = "rig sensor " + std::to_string(k) + " is not a valid rigid transform";
+        }
+        rig.camCapture.assign(v->num_cameras, -1);
+        rig.camSensor.assign(v->num_cameras, -1);
+        for (int c = 0; c < v->num_cameras; ++c)
+        {
+            const int k = v->cam_rig_capture ? v->cam_rig_capture[c] : -1;
+            const int q = v->cam_rig_sensor ? v->cam_rig_sensor[c] : -1;
+            if (k < 0 || k >= nk || q < 0 || q >= ns)
+            {
+                pb.rigError = "rig binding of camera " + std::to_string(c) + " is missing or out of range";
+                continue;
+            }
+            rig.camCapture[c] = k;
+            rig.camSensor[c] = q;
+        }
+        if (pb.rigError.empty())
+        {
+            composeRig(rig, rig.poses, &pb.cams);
+            if (!pb.hasGroupIds)
+            {
+                pb.hasGroupIds = true;
+                pb.groupIds.assign(v->num_cameras, 0);
+                std::vector<int> keys;
+                for (int c = 0; c < v->num_cameras; ++c)
+                {
+                    auto it = std::find(keys.begin(), keys.end(), rig.camSensor[c]);
+                    if (it == keys.end()) it = keys.insert(keys.end(), rig.camSensor[c]);
+                    pb.groupIds[c] = static_cast<int>(it - keys.begin());
+                }
+            }
+            for (int c = 0; c < v->num_cameras; ++c)
+                if (rig.capFixed[rig.camCapture[c]] && rig.senFixed[rig.camSensor[c]] &&
+                    std::find(pb.fixedCameras.begin(), pb.fixedCameras.end(), c) == pb.fixedCameras.end())
+                    pb.fixedCameras.push_back(c);
+        }
+    }
     if (v->fixed_tracks) pb.fixedTracks.assign(v->fixed_tracks, v->fixed_tracks + v->num_fixed_tracks);
     pb.gaugePolicy = v->gauge_policy;
     pb.gaugeAnchor = v->gauge_anchor_camera;
@@ -2563,6 +2798,35 @@ void oracle_masks(oracle_ctx* ctx, int32_t* cameraBlock, int32_t* intrBlock, int
     {
         if (trackBlock) trackBlock[t] = o.prepared ? o.act.trackBlock[t] : -1;
         if (activeTrack) activeTrack[t] = o.prepared ? o.act.activeTrack[t] : 0;
+    }
+}
+
+// Rig block numbering (rigCaptureBlockByCamera, rigSensorBlockByCamera) of the prepared problem.
+void oracle_rig_masks(oracle_ctx* ctx, int32_t* captureBlock, int32_t* sensorBlock)
+{
+    Oracle& o = ctx->o;
+    for (std::size_t c = 0; c < o.pb.cams.size(); ++c)
+    {
+        const bool rig = o.prepared && !o.act.rigCaptureBlockByCamera.empty();
+        if (captureBlock) captureBlock[c] = rig ? o.act.rigCaptureBlockByCamera[c] : -1;
+        if (sensorBlock) sensorBlock[c] = rig ? o.act.rigSensorBlockByCamera[c] : -1;
+    }
+}
+
+// Rig poses of the current state (the input rig before a solve).
+void oracle_download_rig(oracle_ctx* ctx, double* capR, double* capC, double* senR, double* senC)
+{
+    Oracle& o = ctx->o;
+    const RigPoses& r = o.state.rig.capR.size() == o.pb.rig.poses.capR.size() && o.prepared ? o.state.rig : o.pb.rig.poses;
+    for (std::size_t k = 0; k < r.capR.size(); ++k)
+    {
+        if (capR) std::copy(r.capR[k].begin(), r.capR[k].end(), capR + 9 * k);
+        if (capC) std::copy(r.capC[k].begin(), r.capC[k].end(), capC + 3 * k);
+    }
+    for (std::size_t k = 0; k < r.senR.size(); ++k)
+    {
+        if (senR) std::copy(r.senR[k].begin(), r.senR[k].end(), senR + 9 * k);
+        if (senC) std::copy(r.senC[k].begin(), r.senC[k].end(), senC + 3 * k);
     }
 }
 

@@ -75,6 +75,16 @@ class ProblemView(C.Structure):
         ("lp_point", _dp),
         ("lp_normal", _dp),
         ("lp_weight", _dp),
+        ("num_rig_captures", C.c_int32),
+        ("rig_capture_rotation", _dp),
+        ("rig_capture_center", _dp),
+        ("rig_capture_fixed", _ip),
+        ("num_rig_sensors", C.c_int32),
+        ("rig_sensor_rotation", _dp),
+        ("rig_sensor_center", _dp),
+        ("rig_sensor_fixed", _ip),
+        ("cam_rig_capture", _ip),
+        ("cam_rig_sensor", _ip),
     ]
 
 
@@ -338,6 +348,8 @@ EXPORTED_SYMBOLS = (
     "pb_hip_solve",
     "pb_hip_download",
     "pb_hip_get_masks",
+    "pb_hip_get_rig_masks",
+    "pb_hip_download_rig",
     "pb_hip_kernel_stats",
     "pb_hip_reduced_system",
     "pb_hip_evaluate_cost",
@@ -394,6 +406,10 @@ def load_library(path: str | None = None):
     lib.pb_hip_download.restype = C.c_int
     lib.pb_hip_get_masks.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip, _ip]
     lib.pb_hip_get_masks.restype = C.c_int
+    lib.pb_hip_get_rig_masks.argtypes = [C.c_void_p, _ip, _ip]
+    lib.pb_hip_get_rig_masks.restype = C.c_int
+    lib.pb_hip_download_rig.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    lib.pb_hip_download_rig.restype = C.c_int
     lib.pb_hip_kernel_stats.argtypes = [C.c_void_p, C.c_char_p, _lp, _dp, _dp]
     lib.pb_hip_kernel_stats.restype = C.c_int
     lib.pb_hip_reduced_system.argtypes = [C.c_void_p, C.POINTER(SolveOptionsC), C.c_double, _dp, _dp, _dp, _ip]
@@ -520,4 +536,16 @@ def make_view(problem) -> tuple[ProblemView, list]:
     v.lp_point = ptr(arr(getattr(problem, "lp_point", None), np.float64), C.c_double)
     v.lp_normal = ptr(arr(getattr(problem, "lp_normal", None), np.float64), C.c_double)
     v.lp_weight = ptr(arr(getattr(problem, "lp_weight", None), np.float64), C.c_double)
+    kr = arr(getattr(problem, "rig_capture_rotation", None), np.float64)
+    sr = arr(getattr(problem, "rig_sensor_rotation", None), np.float64)
+    v.num_rig_captures = 0 if kr is None else int(kr.size // 9)
+    v.rig_capture_rotation = ptr(kr, C.c_double)
+    v.rig_capture_center = ptr(arr(getattr(problem, "rig_capture_center", None), np.float64), C.c_double)
+    v.rig_capture_fixed = ptr(arr(getattr(problem, "rig_capture_fixed", None), np.int32), C.c_int32)
+    v.num_rig_sensors = 0 if sr is None else int(sr.size // 9)
+    v.rig_sensor_rotation = ptr(sr, C.c_double)
+    v.rig_sensor_center = ptr(arr(getattr(problem, "rig_sensor_center", None), np.float64), C.c_double)
+    v.rig_sensor_fixed = ptr(arr(getattr(problem, "rig_sensor_fixed", None), np.int32), C.c_int32)
+    v.cam_rig_capture = ptr(arr(getattr(problem, "cam_rig_capture", None), np.int32), C.c_int32)
+    v.cam_rig_sensor = ptr(arr(getattr(problem, "cam_rig_sensor", None), np.int32), C.c_int32)
     return v, keep

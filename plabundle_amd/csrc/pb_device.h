@@ -405,6 +405,56 @@ namespace pbdev
                 out[r * 3 + k] = E[r * 3 + 0] * R[0 * 3 + k] + E[r * 3 + 1] * R[1 * 3 + k] + E[r * 3 + 2] * R[2 * 3 + k];
     }
 
+    // Rig camera pose (composeRigCameras, as runConcreteBackend applies it, solver.cpp:282): camera-to-world
+    // R = R_rig R_sensor, centre C = C_rig + R_rig c_sensor.
+    PB_HD void composeRig(const double* capR, const double* capC, const double* senR, const double* senC, double* R,
+                          double* C)
+    {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                R[i * 3 + j] = capR[i * 3 + 0] * senR[0 * 3 + j] + capR[i * 3 + 1] * senR[1 * 3 + j] +
+                               capR[i * 3 + 2] * senR[2 * 3 + j];
+            C[i] = capC[i] + (capR[i * 3 + 0] * senC[0] + capR[i * 3 + 1] * senC[1] + capR[i * 3 + 2] * senC[2]);
+        }
+    }
+
+    // Chain maps of a rig camera (cameraPosePrimaryTerms, Assembly.cpp:224-281): with the camera's pose Jacobian
+    // J = [rotation | centre] in world-frame increments, the capture's Jacobian is J Mk and the sensor's J Ms
+    // (6 x 6, row-major). Mk = [[I, 0], [K, I]] with column a of K the centre's motion e_a x l under a rotation of
+    // the rig about axis a (lever arm l = R_rig c_sensor); Ms = diag(R_rig, R_rig) (sensor increments in rig axes).
+    PB_HD void rigMaps(const double* capR, const double* senC, double* Mk, double* Ms)
+    {
+        double l[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) l[i] = capR[i * 3 + 0] * senC[0] + capR[i * 3 + 1] * senC[1] + capR[i * 3 + 2] * senC[2];
+#pragma unroll
+        for (int i = 0; i < 36; ++i)
+        {
+            Mk[i] = 0.0;
+            Ms[i] = 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) Mk[i * 6 + i] = 1.0;
+        // e_a x l for a = 0, 1, 2 as the columns of K (rows 3..5, columns 0..2)
+        Mk[4 * 6 + 0] = -l[2];
+        Mk[5 * 6 + 0] = l[1];
+        Mk[3 * 6 + 1] = l[2];
+        Mk[5 * 6 + 1] = -l[0];
+        Mk[3 * 6 + 2] = -l[1];
+        Mk[4 * 6 + 2] = l[0];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+            {
+                Ms[i * 6 + j] = capR[i * 3 + j];
+                Ms[(3 + i) * 6 + 3 + j] = capR[i * 3 + j];
+            }
+    }
+
     // Huber loss of a survey constraint (BundleAdjustPlaMatrixConstraints.cpp:17-29): s = |r|^2.
     PB_HD bool huber(double s, double delta, double& cost, double& weight)
     {

@@ -80,6 +80,12 @@ namespace
         };
         std::vector<Constraint> cons;
         int nControl = 0, nPlanes = 0;
+        // camera rig: capture poses (rig-to-world), sensor poses (camera-to-rig), fixed flags and each camera's
+        // binding; the bound cameras' R and C above hold the composed poses
+        int nCap = 0, nSen = 0;
+        std::vector<double> capR, capC, senR, senC;
+        std::vector<int32_t> capFixed, senFixed, camCapture, camSensor;
+        bool rig() const { return nCap > 0 || nSen > 0; }
     };
 
     double sanitizedWeight(double w) { return std::isfinite(w) && w > 0.0 ? w : 0.0; }
@@ -139,6 +145,14 @@ namespace
         // units
         int nUnits = 0, globalDim = 0;
         std::vector<int32_t> unitWidth;
+        // camera rig, in the reference's numbering (BundleAdjustPlaMatrixProblem.cpp:157-199): the primary block of
+        // each refined capture and sensor (-1 otherwise) and per camera its capture's and sensor's block. The
+        // reference's camera blocks are then the rig blocks; cameraBlock / cameraBlocks above number the
+        // device's camera-level pose units instead (cameras whose capture or sensor is refined).
+        bool rig = false;
+        int rigCapBlocks = 0, rigSenBlocks = 0;
+        std::vector<int32_t> rigCapBlock, rigSenBlock, rigCapBlockByCam, rigSenBlockByCam;
+        int refCameraBlocks() const { return rig ? rigCapBlocks + rigSenBlocks : cameraBlocks; }
     };
 
     // Two distinct cameras with a finite pixel and a positive weight (hasUsableTrackObservations,
@@ -326,13 +340,48 @@ namespace
             for (int64_t i = in.off[t]; i < in.off[t + 1]; ++i)
                 if (usable(in, i)) hasResidual[in.obsCam[i]] = 1;
         }
-        if (o.refine_camera_pose)
+        p->rig = in.rig();
+        if (o.refine_camera_pose && !p->rig)
         {
             std::set<int> fixed(p->fixedCams.begin(), p->fixedCams.end());
             for (int c = 0; c < n; ++c)
                 if (hasResidual[c] && !fixed.count(c)) p->cameraBlock[c] = p->cameraBlocks++;
         }
-        p->primaryBlocks = p->cameraBlocks;
+        else if (p->rig)
+        {
+            // a capture is refined when one of its cameras has a residual and neither the capture, nor one of its
+            // cameras is fixed, nor one of them is the gauge scale camera; a sensor when one of its cameras has a
+            // residual and its extrinsic is not fixed; sensor blocks follow the capture blocks
+            std::set<int> fixed(p->fixedCams.begin(), p->fixedCams.end());
+            p->rigCapBlock.assign(in.nCap, -1);
+            p->rigSenBlock.assign(in.nSen, -1);
+            p->rigCapBlockByCam.assign(n, -1);
+            p->rigSenBlockByCam.assign(n, -1);
+            if (o.refine_camera_pose)
+            {
+                std::vector<char> capResidual(in.nCap, 0), capFixed(in.nCap, 0), senResidual(in.nSen, 0);
+                for (int k = 0; k < in.nCap; ++k) capFixed[k] = in.capFixed[k] != 0;
+                for (int c = 0; c < n; ++c)
+                {
+                    const int k = in.camCapture[c], q = in.camSensor[c];
+                    capResidual[k] = capResidual[k] || hasResidual[c];
+                    capFixed[k] = capFixed[k] || fixed.count(c) || p->scaleCam == c;
+                    senResidual[q] = senResidual[q] || hasResidual[c];
+                }
+                for (int k = 0; k < in.nCap; ++k)
+                    if (capResidual[k] && !capFixed[k]) p->rigCapBlock[k] = p->rigCapBlocks++;
+                for (int q = 0; q < in.nSen; ++q)
+                    if (senResidual[q] && !in.senFixed[q]) p->rigSenBlock[q] = p->rigCapBlocks + p->rigSenBlocks++;
+                for (int c = 0; c < n; ++c)
+                {
+                    p->rigCapBlockByCam[c] = p->rigCapBlock[in.camCapture[c]];
+                    p->rigSenBlockByCam[c] = p->rigSenBlock[in.camSensor[c]];
+                    if (hasResidual[c] && (p->rigCapBlockByCam[c] >= 0 || p->rigSenBlockByCam[c] >= 0))
+                        p->cameraBlock[c] = p->cameraBlocks++;
+                }
+            }
+        }
+        p->primaryBlocks = p->refCameraBlocks();
         if (o.shared_intrinsic_mask & 0x1FFFu)
         {
             std::vector<int> ids;
@@ -511,6 +560,14 @@ namespace
 
     // ------------------------------------------------------------------------------------------
     // Device layout of one prepared problem.
+    // Block-sparse rows of a reduced system (both triangles) and its fused-PCG wave slots.
+    struct Rows
+    {
+        std::vector<int32_t> rowPtr, colIdx, diagPos, blockPosUp, blockPosLo;
+        std::vector<int32_t> cgWaveRow, cgWaveBegin, cgWaveEnd, cgWaveLead;
+        int nNnz = 0;
+    };
+
     struct Layout
     {
         int nActive = 0, nObs = 0, nElim = 0, nEntries = 0, nBlocks = 0, nNnz = 0, nItems = 0;
@@ -525,14 +582,241 @@ namespace
         std::vector<int32_t> camObsOffset, camObs, camUnit, camGroup, camDiagBlock, camGroupBlock;
         std::vector<int32_t> groupCamOffset, groupCams, groupPairBlock;
         std::vector<int32_t> entryOffset, entryUnit, entryPoint;
-        std::vector<int32_t> blockRow, blockCol, blockPosUp, blockPosLo, blockItemOffset;
+        std::vector<int32_t> blockRow, blockCol, blockItemOffset;
         std::vector<int32_t> itemBlock, itemBegin, pairE1, pairE2;
-        std::vector<int32_t> rowPtr, colIdx, diagPos;
-        std::vector<int32_t> cgWaveRow, cgWaveBegin, cgWaveEnd, cgWaveLead;
+        Rows rows;  // block-sparse rows of S (both triangles) and the fused-PCG wave slots
+        // camera rig: the rig system (units: refined captures, refined sensors, the intrinsic-group units), its
+        // upper blocks with the camera-level contributions of each, the right-hand-side and expansion maps
+        int nSysUnits = 0, sysGlobalDim = 0;
+        std::vector<int32_t> sysUnitWidth, sysBlockRow, sysBlockCol;
+        Rows sysRows;
+        std::vector<int32_t> projOffset, projPos, projMapL, projMapR;
+        std::vector<int32_t> sysRhsOffset, sysRhsUnit, sysRhsMap, expUnit, expMap;
         // lane-per-observation point linearisation: observation and eliminated point of each lane of each wave
         // (-1 past a wave's last point); the points it cannot take go to the thread-per-point kernel
         std::vector<int32_t> linLaneObs, linLanePoint, linFallback;
     };
+
+    // Block-sparse rows of a reduced system (both triangles) with the fused-PCG wave slots, from its upper
+    // blocks sorted by (row, column); blockStart[u] is the first upper block of row u.
+    void blockRows(int U, const std::vector<int32_t>& blockRow, const std::vector<int32_t>& blockCol,
+                   const std::vector<int32_t>& blockStart, Rows* R)
+    {
+        const int nBlocks = static_cast<int>(blockRow.size());
+        // full block-sparse rows: lower part (from upper blocks (w, u), w < u) then the upper part
+        std::vector<std::vector<int32_t>> lower(U);
+        for (int blk = 0; blk < nBlocks; ++blk)
+            if (blockRow[blk] != blockCol[blk]) lower[blockCol[blk]].push_back(blk);
+        R->rowPtr.assign(U + 1, 0);
+        R->blockPosUp.assign(nBlocks, -1);
+        R->blockPosLo.assign(nBlocks, -1);
+        R->diagPos.assign(U, -1);
+        R->colIdx.clear();
+        for (int u = 0; u < U; ++u)
+        {
+            for (int blk : lower[u])
+            {
+                R->blockPosLo[blk] = static_cast<int32_t>(R->colIdx.size());
+                R->colIdx.push_back(blockRow[blk]);
+            }
+            for (int blk = blockStart[u]; blk < blockStart[u + 1]; ++blk)
+            {
+                R->blockPosUp[blk] = static_cast<int32_t>(R->colIdx.size());
+                if (blockCol[blk] == u)
+                {
+                    R->diagPos[u] = R->blockPosUp[blk];
+                    R->blockPosLo[blk] = R->blockPosUp[blk];
+                }
+                R->colIdx.push_back(blockCol[blk]);
+            }
+            R->rowPtr[u + 1] = static_cast<int32_t>(R->colIdx.size());
+        }
+        R->nNnz = static_cast<int>(R->colIdx.size());
+        // fused-PCG wave slots: a row gets one wave per kCgSegment stored blocks (at most kCgWaves, all in
+        // one workgroup); rows are packed into workgroups of kCgWaves slots in row order
+        R->cgWaveRow.clear();
+        R->cgWaveBegin.clear();
+        R->cgWaveEnd.clear();
+        R->cgWaveLead.clear();
+        auto padPack = [&]() {
+            while (R->cgWaveRow.size() % pbk::kCgWaves)
+            {
+                R->cgWaveRow.push_back(-1);
+                R->cgWaveBegin.push_back(0);
+                R->cgWaveEnd.push_back(0);
+                R->cgWaveLead.push_back(0);
+            }
+        };
+        for (int u = 0; u < U; ++u)
+        {
+            const int b0 = R->rowPtr[u], n = R->rowPtr[u + 1] - b0;
+            const int w = std::max(1, std::min(pbk::kCgWaves, (n + pbk::kCgSegment - 1) / pbk::kCgSegment));
+            const int len = (n + w - 1) / w;
+            const int used = static_cast<int>(R->cgWaveRow.size() % pbk::kCgWaves);
+            if (used + w > pbk::kCgWaves) padPack();
+            for (int j = 0; j < w; ++j)
+            {
+                R->cgWaveRow.push_back(u);
+                R->cgWaveBegin.push_back(b0 + std::min(n, j * len));
+                R->cgWaveEnd.push_back(b0 + std::min(n, (j + 1) * len));
+                R->cgWaveLead.push_back(j == 0 ? w : 0);
+            }
+        }
+        padPack();
+        // the kernel relies on these: every row owned by exactly one lead slot whose row slots follow it in the
+        // same workgroup, and slot ranges inside the row
+        std::vector<int> owners(U, 0);
+        const int nSlots = static_cast<int>(R->cgWaveRow.size());
+        for (int sl = 0; sl < nSlots; ++sl)
+        {
+            const int u = R->cgWaveRow[sl], lead = R->cgWaveLead[sl];
+            if (u < 0) continue;
+            if (u >= U || R->cgWaveBegin[sl] < R->rowPtr[u] || R->cgWaveEnd[sl] > R->rowPtr[u + 1] ||
+                R->cgWaveBegin[sl] > R->cgWaveEnd[sl])
+                throw std::logic_error("PCG wave slot range");
+            if (lead > 0)
+            {
+                ++owners[u];
+                if (sl % pbk::kCgWaves + lead > pbk::kCgWaves) throw std::logic_error("PCG row crosses workgroup");
+                for (int j = 1; j < lead; ++j)
+                    if (R->cgWaveRow[sl + j] != u || R->cgWaveLead[sl + j] != 0)
+                        throw std::logic_error("PCG row slots not contiguous");
+            }
+        }
+        for (int u = 0; u < U; ++u)
+            if (owners[u] != 1) throw std::logic_error("PCG row without a unique owner");
+    }
+
+    // The rig system of a camera rig problem. A camera-level pose unit a (camera c) maps to its capture's rig
+    // unit through chain map 2c and to its sensor's through 2c + 1 (k_rig_maps); a group unit maps to its rig
+    // unit unchanged (-1). Every stored camera-level block (a, b) adds M_l^T S_ab M_r to the rig block (u, v),
+    // u <= v, of each pair of its images; the contributions of a rig block are kept in camera-level storage
+    // order, so every build sums them in the same order.
+    void buildSysLayout(const Input& in, const Prepared& p, Layout* L)
+    {
+        const int rigPose = p.rigCapBlocks + p.rigSenBlocks;
+        const int U = p.nUnits, V = rigPose + (p.nUnits - p.cameraBlocks);
+        L->nSysUnits = V;
+        L->sysUnitWidth.assign(rigPose, 6);
+        for (int a = p.cameraBlocks; a < U; ++a) L->sysUnitWidth.push_back(p.unitWidth[a]);
+        L->sysGlobalDim = 6 * rigPose;
+        for (int g = 0; g < p.G; ++g) L->sysGlobalDim += p.gK[g];
+        std::vector<int> camOfUnit(p.cameraBlocks, -1);
+        for (int c = 0; c < in.nCam; ++c)
+            if (p.cameraBlock[c] >= 0) camOfUnit[p.cameraBlock[c]] = c;
+        L->expUnit.assign(2 * size_t(U), -1);
+        L->expMap.assign(2 * size_t(U), -1);
+        for (int a = 0; a < U; ++a)
+        {
+            if (a < p.cameraBlocks)
+            {
+                const int c = camOfUnit[a];
+                int e = 0;
+                if (p.rigCapBlockByCam[c] >= 0)
+                {
+                    L->expUnit[2 * a + e] = p.rigCapBlockByCam[c];
+                    L->expMap[2 * a + e++] = 2 * c;
+                }
+                if (p.rigSenBlockByCam[c] >= 0)
+                {
+                    L->expUnit[2 * a + e] = p.rigSenBlockByCam[c];
+                    L->expMap[2 * a + e++] = 2 * c + 1;
+                }
+            }
+            else
+            {
+                L->expUnit[2 * a] = a - p.cameraBlocks + rigPose;
+            }
+        }
+        // right-hand side: the camera-level units of each rig unit, in unit order
+        std::vector<std::vector<std::pair<int32_t, int32_t>>> src(V);
+        for (int a = 0; a < U; ++a)
+            for (int e = 0; e < 2; ++e)
+                if (L->expUnit[2 * a + e] >= 0) src[L->expUnit[2 * a + e]].push_back({a, L->expMap[2 * a + e]});
+        L->sysRhsOffset.assign(V + 1, 0);
+        for (int u = 0; u < V; ++u)
+        {
+            for (const auto& [a, m] : src[u])
+            {
+                L->sysRhsUnit.push_back(a);
+                L->sysRhsMap.push_back(m);
+            }
+            L->sysRhsOffset[u + 1] = static_cast<int32_t>(L->sysRhsUnit.size());
+        }
+        // projection: contributions keyed by rig upper block
+        struct Contribution
+        {
+            int32_t u, v, pos, ml, mr;
+        };
+        std::vector<Contribution> cs;
+        for (int a = 0; a < U; ++a)
+            for (int k = L->rows.rowPtr[a]; k < L->rows.rowPtr[a + 1]; ++k)
+            {
+                const int b = L->rows.colIdx[k];
+                for (int ea = 0; ea < 2; ++ea)
+                {
+                    const int u = L->expUnit[2 * a + ea];
+                    if (u < 0) continue;
+                    for (int eb = 0; eb < 2; ++eb)
+                    {
+                        const int v = L->expUnit[2 * b + eb];
+                        if (v < 0 || u > v) continue;
+                        cs.push_back({u, v, k, L->expMap[2 * a + ea], L->expMap[2 * b + eb]});
+                    }
+                }
+            }
+        std::stable_sort(cs.begin(), cs.end(),
+                         [](const Contribution& x, const Contribution& y) { return x.u != y.u ? x.u < y.u : x.v < y.v; });
+        std::vector<int32_t> blockStart(V + 1, 0);
+        L->projOffset.assign(1, 0);
+        for (size_t i = 0; i < cs.size(); ++i)
+        {
+            if (i == 0 || cs[i].u != cs[i - 1].u || cs[i].v != cs[i - 1].v)
+            {
+                if (i > 0) L->projOffset.push_back(static_cast<int32_t>(i));
+                L->sysBlockRow.push_back(cs[i].u);
+                L->sysBlockCol.push_back(cs[i].v);
+            }
+            L->projPos.push_back(cs[i].pos);
+            L->projMapL.push_back(cs[i].ml);
+            L->projMapR.push_back(cs[i].mr);
+        }
+        if (!cs.empty()) L->projOffset.push_back(static_cast<int32_t>(cs.size()));
+        // every rig unit keeps its diagonal block (an empty one finalizes to the identity)
+        std::vector<int32_t> row = L->sysBlockRow, col = L->sysBlockCol, off = L->projOffset;
+        L->sysBlockRow.clear();
+        L->sysBlockCol.clear();
+        L->projOffset.assign(1, 0);
+        std::vector<int32_t> pos, ml, mr;
+        size_t b = 0;
+        for (int u = 0; u < V; ++u)
+        {
+            const bool hasDiag = b < row.size() && row[b] == u && col[b] == u;
+            if (!hasDiag)
+            {
+                L->sysBlockRow.push_back(u);
+                L->sysBlockCol.push_back(u);
+                L->projOffset.push_back(static_cast<int32_t>(pos.size()));
+            }
+            for (; b < row.size() && row[b] == u; ++b)
+            {
+                L->sysBlockRow.push_back(row[b]);
+                L->sysBlockCol.push_back(col[b]);
+                for (int j = off[b]; j < off[b + 1]; ++j)
+                {
+                    pos.push_back(L->projPos[j]);
+                    ml.push_back(L->projMapL[j]);
+                    mr.push_back(L->projMapR[j]);
+                }
+                L->projOffset.push_back(static_cast<int32_t>(pos.size()));
+            }
+            blockStart[u + 1] = static_cast<int32_t>(L->sysBlockRow.size());
+        }
+        L->projPos.swap(pos);
+        L->projMapL.swap(ml);
+        L->projMapR.swap(mr);
+        blockRows(V, L->sysBlockRow, L->sysBlockCol, blockStart, &L->sysRows);
+    }
 
     void buildLayout(const Input& in, const Prepared& p, Layout* L)
     {
@@ -784,89 +1068,8 @@ namespace
             const auto it = std::lower_bound(b, e, v);
             return (it != e && *it == v) ? static_cast<int>(it - L->blockCol.begin()) : -1;
         };
-        // full block-sparse rows: lower part (from upper blocks (w, u), w < u) then the upper part
-        std::vector<std::vector<int32_t>> lower(U);
-        for (int blk = 0; blk < L->nBlocks; ++blk)
-            if (L->blockRow[blk] != L->blockCol[blk]) lower[L->blockCol[blk]].push_back(blk);
-        L->rowPtr.assign(U + 1, 0);
-        L->blockPosUp.assign(L->nBlocks, -1);
-        L->blockPosLo.assign(L->nBlocks, -1);
-        L->diagPos.assign(U, -1);
-        for (int u = 0; u < U; ++u)
-        {
-            for (int blk : lower[u])
-            {
-                L->blockPosLo[blk] = static_cast<int32_t>(L->colIdx.size());
-                L->colIdx.push_back(L->blockRow[blk]);
-            }
-            for (int blk = blockStart[u]; blk < blockStart[u + 1]; ++blk)
-            {
-                L->blockPosUp[blk] = static_cast<int32_t>(L->colIdx.size());
-                if (L->blockCol[blk] == u)
-                {
-                    L->diagPos[u] = L->blockPosUp[blk];
-                    L->blockPosLo[blk] = L->blockPosUp[blk];
-                }
-                L->colIdx.push_back(L->blockCol[blk]);
-            }
-            L->rowPtr[u + 1] = static_cast<int32_t>(L->colIdx.size());
-        }
-        L->nNnz = static_cast<int>(L->colIdx.size());
-        // fused-PCG wave slots: a row gets one wave per kCgSegment stored blocks (at most kCgWaves, all in
-        // one workgroup); rows are packed into workgroups of kCgWaves slots in row order
-        L->cgWaveRow.clear();
-        L->cgWaveBegin.clear();
-        L->cgWaveEnd.clear();
-        L->cgWaveLead.clear();
-        auto padPack = [&]() {
-            while (L->cgWaveRow.size() % pbk::kCgWaves)
-            {
-                L->cgWaveRow.push_back(-1);
-                L->cgWaveBegin.push_back(0);
-                L->cgWaveEnd.push_back(0);
-                L->cgWaveLead.push_back(0);
-            }
-        };
-        for (int u = 0; u < U; ++u)
-        {
-            const int b0 = L->rowPtr[u], n = L->rowPtr[u + 1] - b0;
-            const int w = std::max(1, std::min(pbk::kCgWaves, (n + pbk::kCgSegment - 1) / pbk::kCgSegment));
-            const int len = (n + w - 1) / w;
-            const int used = static_cast<int>(L->cgWaveRow.size() % pbk::kCgWaves);
-            if (used + w > pbk::kCgWaves) padPack();
-            for (int j = 0; j < w; ++j)
-            {
-                L->cgWaveRow.push_back(u);
-                L->cgWaveBegin.push_back(b0 + std::min(n, j * len));
-                L->cgWaveEnd.push_back(b0 + std::min(n, (j + 1) * len));
-                L->cgWaveLead.push_back(j == 0 ? w : 0);
-            }
-        }
-        padPack();
-        // the kernel relies on these: every row owned by exactly one lead slot whose row slots follow it in the
-        // same workgroup, and slot ranges inside the row
-        {
-            std::vector<int> owners(U, 0);
-            const int nSlots = static_cast<int>(L->cgWaveRow.size());
-            for (int sl = 0; sl < nSlots; ++sl)
-            {
-                const int u = L->cgWaveRow[sl], lead = L->cgWaveLead[sl];
-                if (u < 0) continue;
-                if (u >= U || L->cgWaveBegin[sl] < L->rowPtr[u] || L->cgWaveEnd[sl] > L->rowPtr[u + 1] ||
-                    L->cgWaveBegin[sl] > L->cgWaveEnd[sl])
-                    throw std::logic_error("PCG wave slot range");
-                if (lead > 0)
-                {
-                    ++owners[u];
-                    if (sl % pbk::kCgWaves + lead > pbk::kCgWaves) throw std::logic_error("PCG row crosses workgroup");
-                    for (int j = 1; j < lead; ++j)
-                        if (L->cgWaveRow[sl + j] != u || L->cgWaveLead[sl + j] != 0)
-                            throw std::logic_error("PCG row slots not contiguous");
-                }
-            }
-            for (int u = 0; u < U; ++u)
-                if (owners[u] != 1) throw std::logic_error("PCG row without a unique owner");
-        }
+        blockRows(U, L->blockRow, L->blockCol, blockStart, &L->rows);
+        L->nNnz = L->rows.nNnz;
         // direct-part block ids
         L->camDiagBlock.assign(n, -1);
         L->camGroupBlock.assign(size_t(pbk::kGroupUnits) * n, -1);
@@ -1042,6 +1245,13 @@ struct pb_ctx
     pbk::Problem P;
     pbk::Work<float> W32;
     pbk::Work<double> W64;
+    // camera rig: the rig system's structure (a Problem value whose reduced-system fields are the rig's) and its
+    // work vectors per precision; the reduced-solve routes run on these
+    bool rigSys = false;
+    pbk::Problem Psys;
+    pbk::Work<float> Wsys32;
+    pbk::Work<double> Wsys64;
+    std::vector<double> outCapR, outCapC, outSenR, outSenC;
     int costBlocks = 0;
     double* pinned = nullptr;  // host-pinned readback area
     // last results
@@ -1248,6 +1458,86 @@ namespace
         return {};
     }
 
+    // A rig's poses and bindings (validated by validateInput). As runConcreteBackend and makeSolverOptions do
+    // before the solve: the bound cameras are composed from their capture and sensor (solver.cpp:282), each sensor
+    // is a calibration group when no group ids are given and unbound cameras take group 0 (Conversion.cpp:83-98),
+    // and a camera whose capture and sensor are both fixed joins the fixed cameras (Conversion.cpp:120-142).
+    void copyRig(const pb_problem_view* v, Input* r)
+    {
+        r->nCap = v->num_rig_captures;
+        r->nSen = v->num_rig_sensors;
+        r->capR.assign(v->rig_capture_rotation, v->rig_capture_rotation + 9 * size_t(r->nCap));
+        r->capC.assign(v->rig_capture_center, v->rig_capture_center + 3 * size_t(r->nCap));
+        r->senR.assign(v->rig_sensor_rotation, v->rig_sensor_rotation + 9 * size_t(r->nSen));
+        r->senC.assign(v->rig_sensor_center, v->rig_sensor_center + 3 * size_t(r->nSen));
+        r->capFixed.assign(r->nCap, 0);
+        r->senFixed.assign(r->nSen, 0);
+        for (int k = 0; k < r->nCap; ++k) r->capFixed[k] = v->rig_capture_fixed && v->rig_capture_fixed[k] ? 1 : 0;
+        for (int k = 0; k < r->nSen; ++k) r->senFixed[k] = v->rig_sensor_fixed && v->rig_sensor_fixed[k] ? 1 : 0;
+        r->camCapture.assign(v->cam_rig_capture, v->cam_rig_capture + r->nCam);
+        r->camSensor.assign(v->cam_rig_sensor, v->cam_rig_sensor + r->nCam);
+        for (int c = 0; c < r->nCam; ++c)
+            pbdev::composeRig(&r->capR[9 * r->camCapture[c]], &r->capC[3 * r->camCapture[c]],
+                              &r->senR[9 * r->camSensor[c]], &r->senC[3 * r->camSensor[c]], &r->R[9 * c], &r->C[3 * c]);
+        if (!r->hasGroup)
+        {
+            r->hasGroup = true;
+            r->group.assign(r->nCam, 0);
+            std::vector<int32_t> keys;
+            for (int c = 0; c < r->nCam; ++c)
+            {
+                auto it = std::find(keys.begin(), keys.end(), r->camSensor[c]);
+                if (it == keys.end()) it = keys.insert(keys.end(), r->camSensor[c]);
+                r->group[c] = static_cast<int32_t>(it - keys.begin());
+            }
+        }
+        for (int c = 0; c < r->nCam; ++c)
+            if (r->capFixed[r->camCapture[c]] && r->senFixed[r->camSensor[c]] &&
+                std::find(r->fixedCams.begin(), r->fixedCams.end(), c) == r->fixedCams.end())
+                r->fixedCams.push_back(c);
+    }
+
+    // A rigid rotation: finite, orthonormal to 1e-6 and proper.
+    bool rotationValid(const double* R)
+    {
+        for (int i = 0; i < 9; ++i)
+            if (!std::isfinite(R[i])) return false;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+            {
+                const double d = R[i] * R[j] + R[3 + i] * R[3 + j] + R[6 + i] * R[6 + j];
+                if (std::abs(d - (i == j ? 1.0 : 0.0)) > 1e-6) return false;
+            }
+        return R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+                   R[2] * (R[3] * R[7] - R[4] * R[6]) >
+               0.0;
+    }
+
+    // The rig must compose: every camera bound to a capture and a sensor in range (the reference indexes its
+    // block tables with the binding, BundleAdjustPlaMatrixProblem.cpp:192-198), and every pose a rigid transform.
+    std::string validateRig(const pb_problem_view* v)
+    {
+        if (v->num_rig_captures < 0 || v->num_rig_sensors < 0) return "negative rig counts";
+        if (v->num_rig_captures == 0 && v->num_rig_sensors == 0) return {};
+        if (!v->rig_capture_rotation || !v->rig_capture_center || !v->rig_sensor_rotation || !v->rig_sensor_center ||
+            !v->cam_rig_capture || !v->cam_rig_sensor)
+            return "rig arrays are NULL";
+        auto finite3 = [](const double* q) { return std::isfinite(q[0]) && std::isfinite(q[1]) && std::isfinite(q[2]); };
+        for (int k = 0; k < v->num_rig_captures; ++k)
+            if (!rotationValid(v->rig_capture_rotation + 9 * k) || !finite3(v->rig_capture_center + 3 * k))
+                return "rig capture " + std::to_string(k) + " is not a valid rigid transform";
+        for (int k = 0; k < v->num_rig_sensors; ++k)
+            if (!rotationValid(v->rig_sensor_rotation + 9 * k) || !finite3(v->rig_sensor_center + 3 * k))
+                return "rig sensor " + std::to_string(k) + " is not a valid rigid transform";
+        for (int c = 0; c < v->num_cameras; ++c)
+        {
+            const int k = v->cam_rig_capture[c], q = v->cam_rig_sensor[c];
+            if (k < 0 || k >= v->num_rig_captures || q < 0 || q >= v->num_rig_sensors)
+                return "rig binding of camera " + std::to_string(c) + " is missing or out of range";
+        }
+        return {};
+    }
+
     void copyIn(const pb_problem_view* v, Input* in)
     {
         Input r;
@@ -1278,6 +1568,7 @@ namespace
         r.gaugeBaseline = v->gauge_baseline;
         const std::string err = copyConstraints(v, &r);
         if (!err.empty()) throw std::invalid_argument(err);
+        if (v->num_rig_captures > 0 || v->num_rig_sensors > 0) copyRig(v, &r);
         *in = std::move(r);
     }
 
@@ -1295,11 +1586,15 @@ namespace
             for (int t = 0; t < v->num_tracks; ++t)
                 if (v->track_offset[t + 1] < v->track_offset[t]) return "track_offset must be non-decreasing";
         }
+        const std::string rerr = validateRig(v);
+        if (!rerr.empty()) return rerr;
+        const bool rig = v->num_rig_captures > 0 || v->num_rig_sensors > 0;
         for (int c = 0; c < v->num_cameras; ++c)
         {
-            for (int k = 0; k < 9; ++k)
+            // a rig camera's pose comes from its capture and sensor
+            for (int k = 0; k < 9 && !rig; ++k)
                 if (!std::isfinite(v->cam_rotation[9 * c + k])) return "camera pose must be finite";
-            for (int k = 0; k < 3; ++k)
+            for (int k = 0; k < 3 && !rig; ++k)
                 if (!std::isfinite(v->cam_center[3 * c + k])) return "camera pose must be finite";
             const double fx = v->cam_intrinsics[13 * c];
             if (!std::isfinite(fx) || fx <= 0.0 || !std::isfinite(v->cam_intrinsics[13 * c + 1]))
@@ -1382,6 +1677,7 @@ namespace
         prepareGroups(ctx->in, o, &p);
         Layout L;
         buildLayout(ctx->in, p, &L);
+        if (p.rig) buildSysLayout(ctx->in, p, &L);
         uploadStructure(ctx, std::move(p), std::move(L), o);
     }
 
@@ -1399,6 +1695,26 @@ namespace
     }
 
     // Device copy of a prepared problem and its layout (setup).
+    // The fused-PCG wave slots of a reduced system's rows (slot table and the first column ids of each slot).
+    void uploadCgSlots(DeviceAlloc& A, const Rows& R, pbk::Problem* P, hipStream_t s)
+    {
+        P->cgBlocks = static_cast<int>(R.cgWaveRow.size()) / pbk::kCgWaves;
+        std::vector<int32_t> slots(4 * R.cgWaveRow.size());
+        for (size_t i = 0; i < R.cgWaveRow.size(); ++i)
+        {
+            slots[4 * i] = R.cgWaveRow[i];
+            slots[4 * i + 1] = R.cgWaveBegin[i];
+            slots[4 * i + 2] = R.cgWaveEnd[i];
+            slots[4 * i + 3] = R.cgWaveLead[i];
+        }
+        P->cgSlot = reinterpret_cast<const int4*>(A.upload(slots, s));
+        std::vector<int32_t> direct(pbk::kCgDirect * R.cgWaveRow.size(), -1);
+        for (size_t i = 0; i < R.cgWaveRow.size(); ++i)
+            for (int j = 0; j < pbk::kCgDirect && R.cgWaveBegin[i] + j < R.cgWaveEnd[i]; ++j)
+                direct[pbk::kCgDirect * i + j] = R.colIdx[R.cgWaveBegin[i] + j];
+        P->cgColDirect = A.upload(direct, s);
+    }
+
     void uploadStructure(pb_ctx* ctx, Prepared&& p, Layout&& L, const pb_solve_options& o)
     {
         ++ctx->patternBuilds;
@@ -1481,16 +1797,16 @@ namespace
         P.linFallback = A.upload(L.linFallback, s);
         P.blockRow = A.upload(L.blockRow, s);
         P.blockCol = A.upload(L.blockCol, s);
-        P.blockPosUp = A.upload(L.blockPosUp, s);
-        P.blockPosLo = A.upload(L.blockPosLo, s);
+        P.blockPosUp = A.upload(L.rows.blockPosUp, s);
+        P.blockPosLo = A.upload(L.rows.blockPosLo, s);
         P.blockItemOffset = A.upload(L.blockItemOffset, s);
         P.itemBlock = A.upload(L.itemBlock, s);
         P.itemBegin = A.upload(L.itemBegin, s);
         P.pairE1 = A.upload(L.pairE1, s);
         P.pairE2 = A.upload(L.pairE2, s);
-        P.rowPtr = A.upload(L.rowPtr, s);
-        P.colIdx = A.upload(L.colIdx, s);
-        P.diagPos = A.upload(L.diagPos, s);
+        P.rowPtr = A.upload(L.rows.rowPtr, s);
+        P.colIdx = A.upload(L.rows.colIdx, s);
+        P.diagPos = A.upload(L.rows.diagPos, s);
         P.unitWidth = A.upload(p.unitWidth, s);
         {
             // post-solve quality: input cameras (own intrinsics), input points, strict-RMS exclusions
@@ -1511,23 +1827,7 @@ namespace
             }
             P.trackStrictBad = A.upload(bad, s);
         }
-        P.cgBlocks = static_cast<int>(L.cgWaveRow.size()) / pbk::kCgWaves;
-        {
-            std::vector<int32_t> slots(4 * L.cgWaveRow.size());
-            for (size_t i = 0; i < L.cgWaveRow.size(); ++i)
-            {
-                slots[4 * i] = L.cgWaveRow[i];
-                slots[4 * i + 1] = L.cgWaveBegin[i];
-                slots[4 * i + 2] = L.cgWaveEnd[i];
-                slots[4 * i + 3] = L.cgWaveLead[i];
-            }
-            P.cgSlot = reinterpret_cast<const int4*>(A.upload(slots, s));
-            std::vector<int32_t> direct(pbk::kCgDirect * L.cgWaveRow.size(), -1);
-            for (size_t i = 0; i < L.cgWaveRow.size(); ++i)
-                for (int j = 0; j < pbk::kCgDirect && L.cgWaveBegin[i] + j < L.cgWaveEnd[i]; ++j)
-                    direct[pbk::kCgDirect * i + j] = L.colIdx[L.cgWaveBegin[i] + j];
-            P.cgColDirect = A.upload(direct, s);
-        }
+        uploadCgSlots(A, L.rows, &P, s);
         check(hipStreamSynchronize(s), "structure upload");
         L.pairE1.clear();
         L.pairE1.shrink_to_fit();
@@ -1535,6 +1835,55 @@ namespace
         L.pairE2.shrink_to_fit();
         L.linLaneObs = std::vector<int32_t>();
         L.linLanePoint = std::vector<int32_t>();
+        ctx->rigSys = p.rig;
+        if (p.rig)
+        {
+            const Input& in = ctx->in;
+            P.nCap = in.nCap;
+            P.nSen = in.nSen;
+            P.capR = A.upload(in.capR, s);
+            P.capC = A.upload(in.capC, s);
+            P.senR = A.upload(in.senR, s);
+            P.senC = A.upload(in.senC, s);
+            P.camCapture = A.upload(in.camCapture, s);
+            P.camSensor = A.upload(in.camSensor, s);
+            P.capUnit = A.upload(p.rigCapBlock, s);  // rig units are the reference's rig block ids
+            P.senUnit = A.upload(p.rigSenBlock, s);
+            P.rigMap = A.alloc<double>(72 * size_t(std::max(1, P.nCam)));
+            P.nSysUnits = L.nSysUnits;
+            P.nSysBlocks = static_cast<int>(L.sysBlockRow.size());
+            P.sysBlockRow = A.upload(L.sysBlockRow, s);
+            P.sysBlockCol = A.upload(L.sysBlockCol, s);
+            P.sysPosUp = A.upload(L.sysRows.blockPosUp, s);
+            P.sysPosLo = A.upload(L.sysRows.blockPosLo, s);
+            P.projOffset = A.upload(L.projOffset, s);
+            P.projPos = A.upload(L.projPos, s);
+            P.projMapL = A.upload(L.projMapL, s);
+            P.projMapR = A.upload(L.projMapR, s);
+            P.sysRhsOffset = A.upload(L.sysRhsOffset, s);
+            P.sysRhsUnit = A.upload(L.sysRhsUnit, s);
+            P.sysRhsMap = A.upload(L.sysRhsMap, s);
+            P.expUnit = A.upload(L.expUnit, s);
+            P.expMap = A.upload(L.expMap, s);
+            // a rig has no gauge tangent: the scale camera's capture is held (Assembly.cpp:204)
+            P.gaugeScale = -1;
+            pbk::Problem Ps = P;
+            Ps.nUnits = L.nSysUnits;
+            Ps.nBlocks = P.nSysBlocks;
+            Ps.nNnz = L.sysRows.nNnz;
+            Ps.blockRow = P.sysBlockRow;
+            Ps.blockCol = P.sysBlockCol;
+            Ps.blockPosUp = P.sysPosUp;
+            Ps.blockPosLo = P.sysPosLo;
+            Ps.rowPtr = A.upload(L.sysRows.rowPtr, s);
+            Ps.colIdx = A.upload(L.sysRows.colIdx, s);
+            Ps.diagPos = A.upload(L.sysRows.diagPos, s);
+            Ps.unitWidth = A.upload(L.sysUnitWidth, s);
+            Ps.globalDim = L.sysGlobalDim;
+            uploadCgSlots(A, L.sysRows, &Ps, s);
+            check(hipStreamSynchronize(s), "rig structure upload");
+            ctx->Psys = Ps;
+        }
         ctx->P = P;
         ctx->prep = std::move(p);
         ctx->lay = std::move(L);
@@ -1596,6 +1945,40 @@ namespace
               "memset");
     }
 
+    // Work of a rig problem's rig system: its own matrix, preconditioner, solution and PCG vectors; scalars,
+    // flags and the trial buffers are the camera-level work's. The camera-level work gets the candidate rig poses
+    // and the rig solution it reads in the line search.
+    template <typename T>
+    void allocSysWork(pb_ctx* ctx, DeviceAlloc& A, pbk::Work<T>& W, pbk::Work<T>& Ws)
+    {
+        const pbk::Problem& P = ctx->P;
+        const pbk::Problem& Ps = ctx->Psys;
+        const size_t nu = size_t(std::max(1, Ps.nUnits));
+        W.candCapR = A.alloc<double>(size_t(pbk::kMaxAlpha) * std::max(1, P.nCap) * 9);
+        W.candCapC = A.alloc<double>(size_t(pbk::kMaxAlpha) * std::max(1, P.nCap) * 3);
+        W.candSenR = A.alloc<double>(size_t(pbk::kMaxAlpha) * std::max(1, P.nSen) * 9);
+        W.candSenC = A.alloc<double>(size_t(pbk::kMaxAlpha) * std::max(1, P.nSen) * 3);
+        Ws = W;
+        Ws.S = A.alloc<T>(size_t(std::max(1, Ps.nNnz)) * 36);
+        Ws.Minv = A.alloc<T>(nu * 36);
+        Ws.x = A.alloc<T>(nu * 6);
+        Ws.p = A.alloc<T>(nu * 6);
+        Ws.rhs = A.alloc<double>(nu * 6);
+        Ws.directRhs = nullptr;
+        for (int k = 0; k < 2; ++k)
+        {
+            Ws.cgR[k] = A.alloc<T>(nu * 6);
+            Ws.cgU[k] = A.alloc<T>(nu * 6);
+            Ws.cgW[k] = A.alloc<T>(nu * 6);
+            Ws.cgMW[k] = A.alloc<T>(nu * 6);
+            Ws.cgS[k] = A.alloc<T>(nu * 6);
+            Ws.cgMS[k] = A.alloc<T>(nu * 6);
+            Ws.cgPart[k] = A.alloc<double>(nu * 3);
+        }
+        W.rigX = Ws.x;
+        Ws.rigX = Ws.x;
+    }
+
     template <typename T>
     pbk::Work<T>& work(pb_ctx* ctx);
     template <>
@@ -1604,6 +1987,7 @@ namespace
         if (!ctx->work32)
         {
             allocWork(ctx, ctx->dwork32, ctx->W32);
+            if (ctx->rigSys) allocSysWork(ctx, ctx->dwork32, ctx->W32, ctx->Wsys32);
             ctx->work32 = true;
         }
         return ctx->W32;
@@ -1614,9 +1998,23 @@ namespace
         if (!ctx->work64)
         {
             allocWork(ctx, ctx->dwork64, ctx->W64);
+            if (ctx->rigSys) allocSysWork(ctx, ctx->dwork64, ctx->W64, ctx->Wsys64);
             ctx->work64 = true;
         }
         return ctx->W64;
+    }
+
+    // The reduced system the solve routes take: the rig system of a camera rig problem, else the camera-level one.
+    const pbk::Problem& sysProblem(const pb_ctx* ctx) { return ctx->rigSys ? ctx->Psys : ctx->P; }
+    template <typename T>
+    pbk::Work<T>& sysWork(pb_ctx* ctx)
+    {
+        pbk::Work<T>& W = work<T>(ctx);
+        if (!ctx->rigSys) return W;
+        if constexpr (std::is_same<T, float>::value)
+            return ctx->Wsys32;
+        else
+            return ctx->Wsys64;
     }
 
     // Resets the device state (cameras, groups, points) to the uploaded initial state.
@@ -1634,6 +2032,14 @@ namespace
             check(hipMemcpyAsync(P.pointX, ctx->lay.pointX.data(), sizeof(double) * ctx->lay.pointX.size(),
                                  hipMemcpyHostToDevice, s),
                   "state");
+        if (ctx->rigSys)
+        {
+            const Input& in = ctx->in;
+            check(hipMemcpyAsync(P.capR, in.capR.data(), sizeof(double) * in.capR.size(), hipMemcpyHostToDevice, s), "state");
+            check(hipMemcpyAsync(P.capC, in.capC.data(), sizeof(double) * in.capC.size(), hipMemcpyHostToDevice, s), "state");
+            check(hipMemcpyAsync(P.senR, in.senR.data(), sizeof(double) * in.senR.size(), hipMemcpyHostToDevice, s), "state");
+            check(hipMemcpyAsync(P.senC, in.senC.data(), sizeof(double) * in.senC.size(), hipMemcpyHostToDevice, s), "state");
+        }
     }
 
     // canUseReferenceOnlineSchur (ReferenceSchur.cpp:667-673) for frame cameras without rigs, laser ranges, scale
@@ -1649,7 +2055,7 @@ namespace
 
     bool onlineSchur(const pb_ctx* ctx, const pb_solve_options& o)
     {
-        return o.use_online_schur != 0 && ctx->prep.primaryBlocks > 0 && ctx->in.nPlanes == 0;
+        return o.use_online_schur != 0 && ctx->prep.primaryBlocks > 0 && ctx->in.nPlanes == 0 && !ctx->in.rig();
     }
 
     // Per-solve device settings that follow the options: robust loss, the parameterisation
@@ -1663,7 +2069,7 @@ namespace
         ctx->pcgGraphs = o.pcg_graphs != 0;
         P.loss = o.robust_loss;
         P.lossScale = o.robust_loss_scale;
-        P.refParam = (o.use_online_schur && ctx->in.nControl == 0 && ctx->in.nPlanes == 0) ? 1 : 0;
+        P.refParam = (o.use_online_schur && ctx->in.nControl == 0 && ctx->in.nPlanes == 0 && !ctx->in.rig()) ? 1 : 0;
         ctx->dcons.release();
         P.nCons = 0;
         P.consOffset = P.consPoint = P.consRows = P.consIndex = nullptr;
@@ -1730,6 +2136,7 @@ namespace
         check(hipMemsetAsync(W.flags, 0, sizeof(int32_t) * pbk::kFlagCount, s), "memset");
         check(hipMemsetAsync(W.D, 0, sizeof(double) * size_t(std::max(1, P.nBlocks)) * 36, s), "memset");
         check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
+        if (ctx->rigSys) Lr::rigMaps(P, s);
         Lr::prepareCameras(P, s);
         {
             Span sp(ctx, 0);
@@ -1742,6 +2149,7 @@ namespace
         {
             Span sp(ctx, 2);
             Lr::schurBlocks(P, W, damping, s);
+            if (ctx->rigSys) Lr::rigProject(P, ctx->Psys, W, sysWork<T>(ctx), damping, s);
         }
         double* h = ctx->pinned;
         check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s), "readback");
@@ -1765,7 +2173,7 @@ namespace
     hipGraphExec_t pcgGraph(pb_ctx* ctx, pbk::Work<T>& W, int maxIter, double relTol, double absTol, double modelEta)
     {
         using Lr = pbk::Launcher<T>;
-        const pbk::Problem& P = ctx->P;
+        const pbk::Problem& P = sysProblem(ctx);
         hipStream_t s = ctx->stream;
         auto& g = ctx->pcgGraph[std::is_same<T, double>::value ? 1 : 0];
         std::vector<unsigned char> key(sizeof(P) + sizeof(W) + sizeof(int) + 3 * sizeof(double));
@@ -1811,7 +2219,7 @@ namespace
     {
         using Lr = pbk::Launcher<T>;
         hipStream_t s = ctx->stream;
-        const pbk::Problem& P = ctx->P;
+        const pbk::Problem& P = sysProblem(ctx);
         PcgResult res;
         double* h = ctx->pinned + kPinPcg;
         int32_t* hf = reinterpret_cast<int32_t*>(ctx->pinned + kPinPcgFlags);
@@ -2017,9 +2425,9 @@ namespace
         }
         std::vector<std::vector<int>> adj(nPose);
         for (int u = 0; u < nPose; ++u)
-            for (int k = L.rowPtr[u]; k < L.rowPtr[u + 1]; ++k)
+            for (int k = L.rows.rowPtr[u]; k < L.rows.rowPtr[u + 1]; ++k)
             {
-                const int v = L.colIdx[k];
+                const int v = L.rows.colIdx[k];
                 if (v < nPose && v != u) adj[u].push_back(v);
             }
         const std::vector<int> order = reverseCuthillMcKee(adj);
@@ -2385,6 +2793,14 @@ namespace
     {
         if (o.reduced_solver == PB_REDUCED_PCG) return kRoutePcg;
         if (o.reduced_solver == PB_REDUCED_DENSE) return kRouteDense;
+        if (ctx->rigSys)
+        {
+            // the band plan orders camera-level pose units; a rig system takes the dense or the PCG route
+            if (o.reduced_solver == PB_REDUCED_BAND)
+                throw std::invalid_argument("unsupported: the band route does not take a camera rig's system");
+            const int maxRows = o.dense_max_rows > 0 ? o.dense_max_rows : kDenseSolveMaxRows;
+            return 6L * ctx->Psys.nUnits <= maxRows ? kRouteDense : kRoutePcg;
+        }
         const bool band = planBand(ctx);
         if (o.reduced_solver == PB_REDUCED_BAND)
         {
@@ -2430,7 +2846,7 @@ namespace
     {
         using DL = pbk::DenseLauncher<TA>;
         using BL = pbk::BandLauncher<TA>;
-        const pbk::Problem& P = ctx->P;
+        const pbk::Problem& P = sysProblem(ctx);
         hipStream_t s = ctx->stream;
         const bool band = ctx->band.active;
         const pbk::BandView& BV = ctx->band.view;
@@ -2541,7 +2957,7 @@ namespace
     PcgResult denseSolve(pb_ctx* ctx, pbk::Work<T>& W, double relTol, bool fp32Seed, bool* seedAccepted,
                          pb_solve_report* rep)
     {
-        const pbk::Problem& P = ctx->P;
+        const pbk::Problem& P = sysProblem(ctx);
         hipStream_t s = ctx->stream;
         PcgResult res;
         *seedAccepted = false;
@@ -2936,6 +3352,18 @@ namespace
             check(hipMemcpyAsync(ctx->outPointX.data(), P.pointX, sizeof(double) * ctx->outPointX.size(),
                                  hipMemcpyDeviceToHost, s),
                   "download");
+        if (ctx->rigSys)
+        {
+            const Input& in = ctx->in;
+            ctx->outCapR.resize(in.capR.size());
+            ctx->outCapC.resize(in.capC.size());
+            ctx->outSenR.resize(in.senR.size());
+            ctx->outSenC.resize(in.senC.size());
+            check(hipMemcpyAsync(ctx->outCapR.data(), P.capR, sizeof(double) * in.capR.size(), hipMemcpyDeviceToHost, s), "download");
+            check(hipMemcpyAsync(ctx->outCapC.data(), P.capC, sizeof(double) * in.capC.size(), hipMemcpyDeviceToHost, s), "download");
+            check(hipMemcpyAsync(ctx->outSenR.data(), P.senR, sizeof(double) * in.senR.size(), hipMemcpyDeviceToHost, s), "download");
+            check(hipMemcpyAsync(ctx->outSenC.data(), P.senC, sizeof(double) * in.senC.size(), hipMemcpyDeviceToHost, s), "download");
+        }
         check(hipStreamSynchronize(s), "download");
         ctx->haveResult = true;
     }
@@ -3029,7 +3457,8 @@ namespace
                 fresh = false;
                 if (b.singular) throw std::runtime_error("linear solve failed: singular damped point block");
                 PcgResult lin;
-                if (!b.notSpd) lin = reducedSolve(ctx, W, o, pg, rep, relTol, absTol, maxPcg, checkEvery, modelEta);
+                if (!b.notSpd)
+                    lin = reducedSolve(ctx, sysWork<T>(ctx), o, pg, rep, relTol, absTol, maxPcg, checkEvery, modelEta);
                 ++iterations;
                 if (b.notSpd || lin.breakdown || !lin.converged)
                 {
@@ -3040,7 +3469,10 @@ namespace
                 auto tb = Clock::now();
                 {
                     Span sp(ctx, 5);
+                    // a rig's step is solved over its captures and sensors; the cameras' steps follow from it
+                    if (ctx->rigSys) Lr::rigExpand(P, sysWork<T>(ctx), W, s);
                     Lr::backSubstitute(P, W, s);
+                    if (ctx->rigSys) Lr::rigStepRms(ctx->Psys, sysWork<T>(ctx), W, s);
                 }
                 check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
                       "readback");
@@ -3154,7 +3586,7 @@ namespace
         r->observation_count = p.observations;
         r->active_track_count = p.activeTracks;
         r->rejected_initial_tracks = p.rejectedInitial;
-        r->camera_block_count = p.cameraBlocks;
+        r->camera_block_count = p.refCameraBlocks();
         r->intrinsic_block_count = p.intrinsicBlocks;
         r->track_block_count = p.trackBlocks;
         r->primary_block_count = p.primaryBlocks;
@@ -3201,24 +3633,28 @@ namespace
     }
 
     // Unit -> (reference 13-padded block, slot) mapping for dense exports.
+    // Units are those of the reduced system the routes solve: a rig's units (rig blocks, then the group units) or
+    // the camera-level ones.
     void unitSlots(const pb_ctx* ctx, int u, int* block, int slots[6])
     {
         const Prepared& p = ctx->prep;
         for (int a = 0; a < 6; ++a) slots[a] = -1;
-        if (u < p.cameraBlocks)
+        const int pose = p.refCameraBlocks();
+        if (u < pose)
         {
             *block = u;
             for (int a = 0; a < 6; ++a) slots[a] = a;
             return;
         }
+        const int cu = u - pose + p.cameraBlocks;  // the camera-level group unit
         for (int g = 0; g < p.G; ++g)
         {
             if (p.gK[g] == 0) continue;
             const int m = (p.gK[g] + 5) / 6;
-            if (u >= p.gUnit[g] && u < p.gUnit[g] + m)
+            if (cu >= p.gUnit[g] && cu < p.gUnit[g] + m)
             {
-                *block = p.cameraBlocks + g;
-                const int j = u - p.gUnit[g];
+                *block = pose + g;
+                const int j = cu - p.gUnit[g];
                 for (int a = 0; a < 6; ++a)
                     if (6 * j + a < p.gK[g]) slots[a] = p.gActive[pbdev::kMaxActive * g + 6 * j + a];
                 return;
@@ -3231,14 +3667,16 @@ namespace
     pb_status exportReduced(pb_ctx* ctx, const pb_solve_options& o, double damping, double* matrix, double* rhs,
                             double* cost, int32_t* primaryBlocks)
     {
-        pbk::Work<T>& W = work<T>(ctx);
+        pbk::Work<T>& Wc = work<T>(ctx);
         resetState(ctx);
-        const BuildResult b = build(ctx, W, damping);
+        const BuildResult b = build(ctx, Wc, damping);
         if (b.invalid) return fail(ctx, PB_ERR_NUMERIC, "invalid projection at the initial state");
         if (cost) *cost = b.cost;
         const int P13 = ctx->prep.primaryBlocks * 13;
         if (primaryBlocks) *primaryBlocks = ctx->prep.primaryBlocks;
-        const pbk::Problem& P = ctx->P;
+        const pbk::Problem& P = sysProblem(ctx);
+        const pbk::Work<T>& W = sysWork<T>(ctx);
+        const Rows& rows = ctx->rigSys ? ctx->lay.sysRows : ctx->lay.rows;
         std::vector<T> S(size_t(P.nNnz) * 36);
         std::vector<double> r(size_t(P.nUnits) * 6);
         if (!S.empty())
@@ -3253,9 +3691,9 @@ namespace
             {
                 int bu, su[6];
                 unitSlots(ctx, u, &bu, su);
-                for (int k = ctx->lay.rowPtr[u]; k < ctx->lay.rowPtr[u + 1]; ++k)
+                for (int k = rows.rowPtr[u]; k < rows.rowPtr[u + 1]; ++k)
                 {
-                    const int v = ctx->lay.colIdx[k];
+                    const int v = rows.colIdx[k];
                     int bv, sv[6];
                     unitSlots(ctx, v, &bv, sv);
                     for (int a = 0; a < 6; ++a)
@@ -3317,7 +3755,9 @@ namespace
                 return std::invalid_argument((unsupported ? "unsupported: window " : "window ") + std::to_string(w) +
                                              ": " + m);
             };
-            const std::string verr = validateInput(v);
+            const std::string verr = v && (v->num_rig_captures > 0 || v->num_rig_sensors > 0)
+                                         ? std::string("unsupported: camera rigs in batched windows")
+                                         : validateInput(v);
             if (!verr.empty()) throw windowError(verr);
             if (v->num_control_points > 0 || v->num_laser_planes > 0)
                 throw windowError("unsupported: survey constraints in a batched window");
@@ -3483,7 +3923,7 @@ namespace
         {
             std::vector<int32_t> nnzRow(std::max(1, LL.nNnz));
             for (int u = 0; u < pp.nUnits; ++u)
-                for (int k = LL.rowPtr[u]; k < LL.rowPtr[u + 1]; ++k) nnzRow[k] = u;
+                for (int k = LL.rows.rowPtr[u]; k < LL.rows.rowPtr[u + 1]; ++k) nnzRow[k] = u;
             P.nnzRow = A.upload(nnzRow, s);
         }
         B.nWin = nWin;
@@ -4216,6 +4656,7 @@ pb_status pb_hip_backward_pose(pb_ctx* ctx, const double* dl_dcenter, const doub
     if (!ctx || !dl_dobs) return PB_ERR_INVALID;
     if (!ctx->uploaded || !ctx->haveResult || ctx->batchMode)
         return fail(ctx, PB_ERR_INVALID, "pb_hip_backward needs the result of a pb_hip_solve on this context");
+    if (ctx->in.rig()) return fail(ctx, PB_ERR_UNSUPPORTED, "unsupported: the adjoint of a camera rig problem");
     return guarded(ctx, [&]() -> pb_status {
         check(hipSetDevice(ctx->device), "hipSetDevice");
         using Lr = pbk::Launcher<double>;
@@ -4311,6 +4752,33 @@ pb_status pb_hip_backward_pose(pb_ctx* ctx, const double* dl_dcenter, const doub
         }
         return PB_OK;
     });
+}
+
+pb_status pb_hip_get_rig_masks(pb_ctx* ctx, int32_t* captureBlock, int32_t* sensorBlock)
+{
+    if (!ctx || !ctx->structureValid) return fail(ctx, PB_ERR_INVALID, "no prepared problem (solve first)");
+    const Prepared& p = ctx->prep;
+    for (int c = 0; c < ctx->in.nCam; ++c)
+    {
+        if (captureBlock) captureBlock[c] = p.rig ? p.rigCapBlockByCam[c] : -1;
+        if (sensorBlock) sensorBlock[c] = p.rig ? p.rigSenBlockByCam[c] : -1;
+    }
+    return PB_OK;
+}
+
+pb_status pb_hip_download_rig(pb_ctx* ctx, double* capR, double* capC, double* senR, double* senC)
+{
+    if (!ctx || !ctx->uploaded) return fail(ctx, PB_ERR_INVALID, "no uploaded problem");
+    const Input& in = ctx->in;
+    const bool refined = ctx->haveResult && ctx->rigSys && !ctx->batchMode;
+    auto put = [](double* dst, const std::vector<double>& src) {
+        if (dst && !src.empty()) std::copy(src.begin(), src.end(), dst);
+    };
+    put(capR, refined ? ctx->outCapR : in.capR);
+    put(capC, refined ? ctx->outCapC : in.capC);
+    put(senR, refined ? ctx->outSenR : in.senR);
+    put(senC, refined ? ctx->outSenC : in.senC);
+    return PB_OK;
 }
 
 pb_status pb_hip_download(pb_ctx* ctx, double* R, double* C, double* intr, double* X, int32_t* valid)
@@ -4419,7 +4887,7 @@ pb_status pb_hip_get_masks(pb_ctx* ctx, int32_t* cameraBlock, int32_t* intrinsic
     const Prepared& p = ctx->prep;
     for (int c = 0; c < ctx->in.nCam; ++c)
     {
-        if (cameraBlock) cameraBlock[c] = p.cameraBlock[c];
+        if (cameraBlock) cameraBlock[c] = p.rig ? -1 : p.cameraBlock[c];  // a rig's blocks are its captures and sensors
         if (intrinsicBlock) intrinsicBlock[c] = p.intrinsicBlock[c];
         if (groupOfCam) groupOfCam[c] = p.groupOfCam[c];
     }
@@ -4657,6 +5125,7 @@ pb_status pb_hip_direct_solve(pb_ctx* ctx, const pb_solve_options* options, doub
     if (!ctx || !ctx->uploaded || !x) return PB_ERR_INVALID;
     if (ctx->batchMode) return fail(ctx, PB_ERR_INVALID, "the context holds a batch; upload a problem first");
     if (route != PB_REDUCED_DENSE && route != PB_REDUCED_BAND) return fail(ctx, PB_ERR_INVALID, "route must be dense or band");
+    if (ctx->in.rig()) return fail(ctx, PB_ERR_UNSUPPORTED, "unsupported: the direct-solve hook takes problems without a rig");
     const std::string oerr = validateOptions(options);
     if (!oerr.empty()) return fail(ctx, PB_ERR_INVALID, oerr);
     return guarded(ctx, [&]() -> pb_status {
@@ -4708,9 +5177,9 @@ pb_status pb_hip_direct_solve(pb_ctx* ctx, const pb_solve_options* options, doub
             {
                 std::fill(matrix, matrix + size_t(n) * n, 0.0);
                 for (int u = 0; u < P.nUnits; ++u)
-                    for (int k = ctx->lay.rowPtr[u]; k < ctx->lay.rowPtr[u + 1]; ++k)
+                    for (int k = ctx->lay.rows.rowPtr[u]; k < ctx->lay.rows.rowPtr[u + 1]; ++k)
                     {
-                        const int v = ctx->lay.colIdx[k];
+                        const int v = ctx->lay.rows.colIdx[k];
                         for (int e = 0; e < 36; ++e)
                             matrix[size_t(6 * v + e % 6) * n + 6 * u + e / 6] = S[size_t(k) * 36 + e];
                     }

@@ -207,6 +207,38 @@ namespace pbk
         const double* winAlpha = nullptr;         // [nWin*kMaxAlpha] trial step lengths of the current pass
         const int32_t* winNAlpha = nullptr;       // [nWin] trial step lengths of the window in the current pass
         const int32_t* nnzRow = nullptr;          // [nNnz] block row (unit) of each stored block
+
+        // Camera rig (nCap + nSen > 0). Every camera is bound to a capture and a sensor and its pose is their
+        // composition. The linearisation, Schur and back-substitution kernels work on camera-level pose units
+        // (one per camera whose capture or sensor is refined); the reduced system is solved over the rig units
+        // (refined captures, then refined sensors, then the intrinsic-group units): S_rig = T^T S_cam T and
+        // rhs_rig = T^T rhs_cam with the chain maps T of each camera (pbdev::rigMaps), and the camera-level step
+        // is T x_rig. The reduced-system fields above (nUnits, rowPtr, ...) stay camera-level; the rig system's
+        // structure is a second Problem value whose reduced-system fields are the rig's (sysProblem on the host).
+        int nCap = 0, nSen = 0;
+        double* capR = nullptr;  // state: rig-to-world rotations [nCap*9], rig centres [nCap*3]
+        double* capC = nullptr;
+        double* senR = nullptr;  // state: camera-to-rig rotations [nSen*9], centres in the rig [nSen*3]
+        double* senC = nullptr;
+        const int32_t* camCapture = nullptr;  // [nCam]
+        const int32_t* camSensor = nullptr;   // [nCam]
+        const int32_t* capUnit = nullptr;     // [nCap] rig-level unit of a refined capture, else -1
+        const int32_t* senUnit = nullptr;     // [nSen]
+        double* rigMap = nullptr;             // [nCam*72] chain maps (capture, sensor) of the current state
+        int nSysUnits = 0, nSysBlocks = 0;
+        const int32_t* sysBlockRow = nullptr;   // [nSysBlocks] upper blocks of the rig system
+        const int32_t* sysBlockCol = nullptr;
+        const int32_t* sysPosUp = nullptr;      // stored position of (row, col) / (col, row) in the rig rows
+        const int32_t* sysPosLo = nullptr;
+        const int32_t* projOffset = nullptr;    // [nSysBlocks+1] contributions of each rig block
+        const int32_t* projPos = nullptr;       // camera-level stored block (row a, column b)
+        const int32_t* projMapL = nullptr;      // chain map of a (2 camera + 0 capture / 1 sensor), -1 identity
+        const int32_t* projMapR = nullptr;      // chain map of b
+        const int32_t* sysRhsOffset = nullptr;  // [nSysUnits+1] camera-level units of each rig unit
+        const int32_t* sysRhsUnit = nullptr;
+        const int32_t* sysRhsMap = nullptr;
+        const int32_t* expUnit = nullptr;       // [nUnits*2] rig units of a camera-level unit (-1: none)
+        const int32_t* expMap = nullptr;        // [nUnits*2] their chain maps
     };
 
     // Per-window LM controller state and window-local structure of a batched solve (device arrays).
@@ -337,6 +369,12 @@ namespace pbk
         double* trialPart = nullptr;        // [costBlocks*kMaxAlpha]
         int32_t* trialBad = nullptr;
         double* trialCost = nullptr;        // [kMaxAlpha]
+        // camera rig: the rig system's solution and the candidate rig poses of a line-search pass
+        const T* rigX = nullptr;     // [nSysUnits*6]
+        double* candCapR = nullptr;  // [kMaxAlpha*nCap*9]
+        double* candCapC = nullptr;
+        double* candSenR = nullptr;  // [kMaxAlpha*nSen*9]
+        double* candSenC = nullptr;
     };
 
     template <typename T>
@@ -369,6 +407,14 @@ namespace pbk
         static void trialObjective(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha,
                                    int costBlocks, hipStream_t s);
         static void accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s);
+        // camera rig: chain maps of the current state; the rig system from the camera-level one (damped diagonal,
+        // exact zeros -> 1) and its right-hand side; the camera-level step from the rig step; the step RMS over the
+        // rig units (scalars[kScalarRmsGlobal], after backSubstitute)
+        static void rigMaps(const Problem& P, hipStream_t s);
+        static void rigProject(const Problem& P, const Problem& Pr, const Work<T>& W, const Work<T>& Wr,
+                               double damping, hipStream_t s);
+        static void rigExpand(const Problem& P, const Work<T>& Wr, const Work<T>& W, hipStream_t s);
+        static void rigStepRms(const Problem& Pr, const Work<T>& Wr, const Work<T>& W, hipStream_t s);
     };
 
     // Dense reduced solve (pb_dense.hip): the reduced system gathered into a column-major matrix of order

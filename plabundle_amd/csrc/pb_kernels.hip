@@ -802,8 +802,11 @@ namespace pbk
         if (lane >= 36 && lane < 42) outr[lane - 36] = mr;
     }
 
-    // Direct part + Schur items per block, diagonal finalize (scale by 1 + lambda, exact zeros -> 1),
-    // both triangles of the block-sparse matrix, and the reduced right-hand side.
+    __device__ __forceinline__ bool rigMode(const Problem& P) { return P.nCap + P.nSen > 0; }
+
+    // Direct part + Schur items per block, diagonal finalize (scale by 1 + lambda, exact zeros -> 1; a rig
+    // problem keeps the camera-level sums and finalizes its rig system instead), both triangles of the
+    // block-sparse matrix, and the reduced right-hand side.
     template <typename T>
     __global__ void k_schur_finalize(Problem P, Work<T> Wk, double damping)
     {
@@ -840,7 +843,7 @@ namespace pbk
             if (lane < 36)
             {
                 double s = Wk.D[blk * 36 + lane] + mine;
-                if (diag && (lane / 6) == (lane % 6)) s = s == 0.0 ? 1.0 : s * (1.0 + damping);
+                if (diag && (lane / 6) == (lane % 6) && !rigMode(P)) s = s == 0.0 ? 1.0 : s * (1.0 + damping);
                 Wk.S[P.blockPosUp[blk] * 36 + lane] = T(s);
                 if (!diag) Wk.S[P.blockPosLo[blk] * 36 + (lane % 6) * 6 + lane / 6] = T(s);
             }
@@ -852,7 +855,7 @@ namespace pbk
         {
             double s = Wk.D[blk * 36 + lane];
             for (int it = it0; it < it1; ++it) s += double(Wk.itemPartial[it * 36 + lane]);
-            if (diag && (lane / 6) == (lane % 6)) s = s == 0.0 ? 1.0 : s * (1.0 + damping);
+            if (diag && (lane / 6) == (lane % 6) && !rigMode(P)) s = s == 0.0 ? 1.0 : s * (1.0 + damping);
             Wk.S[P.blockPosUp[blk] * 36 + lane] = T(s);
             if (!diag) Wk.S[P.blockPosLo[blk] * 36 + (lane % 6) * 6 + lane / 6] = T(s);
         }
@@ -1447,7 +1450,24 @@ namespace pbk
                 scaleC[i] = C0[i];
             }
         }
-        if (u >= 0)
+        if (rigMode(P))
+        {
+            // the composition of the camera's candidate capture and sensor (a camera whose capture and sensor
+            // are both held keeps its pose)
+            const int k = P.camCapture[c], q = P.camSensor[c];
+            if (P.capUnit[k] >= 0 || P.senUnit[q] >= 0)
+            {
+                const long ck = long(ai) * P.nCap + k, cq = long(ai) * P.nSen + q;
+                pbdev::composeRig(Wk.candCapR + ck * 9, Wk.candCapC + ck * 3, Wk.candSenR + cq * 9,
+                                  Wk.candSenC + cq * 3, R, Cc);
+            }
+            else
+            {
+                for (int i = 0; i < 9; ++i) R[i] = R0[i];
+                for (int i = 0; i < 3; ++i) Cc[i] = C0[i];
+            }
+        }
+        else if (u >= 0)
         {
             double d[6];
 #pragma unroll
@@ -1587,12 +1607,208 @@ namespace pbk
             for (int i = 0; i < 3; ++i) P.camC[3 * id + i] = Wk.candC[(ai * P.nCam + id) * 3 + i];
         }
         if (id < P.nGroup * 13) P.groupParams[id] = Wk.candGroupParams[ai * P.nGroup * 13 + id];
+        if (id < P.nCap)
+        {
+            for (int i = 0; i < 9; ++i) P.capR[9 * id + i] = Wk.candCapR[(long(ai) * P.nCap + id) * 9 + i];
+            for (int i = 0; i < 3; ++i) P.capC[3 * id + i] = Wk.candCapC[(long(ai) * P.nCap + id) * 3 + i];
+        }
+        if (id < P.nSen)
+        {
+            for (int i = 0; i < 9; ++i) P.senR[9 * id + i] = Wk.candSenR[(long(ai) * P.nSen + id) * 9 + i];
+            for (int i = 0; i < 3; ++i) P.senC[3 * id + i] = Wk.candSenC[(long(ai) * P.nSen + id) * 3 + i];
+        }
         if (id < P.nElim)
         {
             const int t = P.elimTrack[id];
             const double s = stepScale(P, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2]);
             for (int i = 0; i < 3; ++i) P.pointX[3 * t + i] += s * alpha * Wk.dp[3 * id + i];
         }
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    // Camera rig. The camera-level reduced system S_cam (one pose unit per camera whose capture or sensor is
+    // refined, then the intrinsic-group units) is mapped onto the rig's units: with T the chain maps (a pose
+    // unit's rows go to its capture's unit through Mk and to its sensor's unit through Ms; group units map to
+    // themselves), S_rig = T^T S_cam T and rhs_rig = T^T rhs_cam. Point elimination commutes with T, so this is
+    // the Schur complement of the system assembled over captures and sensors (ReferenceSchur.cpp, Assembly.cpp
+    // :207-284); the damped diagonal is applied to the rig system (path A damps the primary diagonal of the
+    // equations it solves). The camera-level step is T x_rig.
+
+    // Chain maps of every camera at the current rig state.
+    __global__ void k_rig_maps(Problem P)
+    {
+        const int c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= P.nCam) return;
+        const int k = P.camCapture[c], q = P.camSensor[c];
+        double Mk[36], Ms[36];
+        pbdev::rigMaps(P.capR + 9 * k, P.senC + 3 * q, Mk, Ms);
+        double* out = P.rigMap + 72L * c;
+        for (int i = 0; i < 36; ++i)
+        {
+            out[i] = Mk[i];
+            out[36 + i] = Ms[i];
+        }
+    }
+
+    // One wave per upper block (u, v) of the rig system: lane (p, q) sums (M_l^T S_ab M_r)(p, q) over the
+    // block's camera-level contributions in a fixed order, then the diagonal finalize of path A.
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_rig_project(Problem P, Work<T> Wk, Work<T> Wr, double damping)
+    {
+        const int blk = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+        if (blk >= P.nSysBlocks) return;
+        const int lane = threadIdx.x & 63;
+        if (lane >= 36) return;
+        const int p = lane / 6, q = lane % 6;
+        double acc = 0.0;
+        for (int j = P.projOffset[blk]; j < P.projOffset[blk + 1]; ++j)
+        {
+            const T* Sb = Wk.S + long(P.projPos[j]) * 36;
+            const int ml = P.projMapL[j], mr = P.projMapR[j];
+            const double* ML = ml >= 0 ? P.rigMap + 36L * ml : nullptr;
+            const double* MR = mr >= 0 ? P.rigMap + 36L * mr : nullptr;
+            double t[6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+            {
+                double v = 0.0;
+                if (MR)
+                {
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) v += double(Sb[i * 6 + k]) * MR[k * 6 + q];
+                }
+                else
+                {
+                    v = double(Sb[i * 6 + q]);
+                }
+                t[i] = v;
+            }
+            if (ML)
+            {
+#pragma unroll
+                for (int i = 0; i < 6; ++i) acc += ML[i * 6 + p] * t[i];
+            }
+            else
+            {
+                acc += t[p];
+            }
+        }
+        const int u = P.sysBlockRow[blk], v = P.sysBlockCol[blk];
+        if (u == v && p == q) acc = acc == 0.0 ? 1.0 : acc * (1.0 + damping);
+        Wr.S[long(P.sysPosUp[blk]) * 36 + lane] = T(acc);
+        if (u != v) Wr.S[long(P.sysPosLo[blk]) * 36 + q * 6 + p] = T(acc);
+    }
+
+    // rhs_rig(u) = sum over the camera-level units a mapped to u of M^T rhs_cam(a).
+    template <typename T>
+    __global__ void k_rig_rhs(Problem P, Work<T> Wk, Work<T> Wr)
+    {
+        const int id = blockIdx.x * blockDim.x + threadIdx.x;
+        if (id >= P.nSysUnits * 6) return;
+        const int u = id / 6, r = id % 6;
+        double acc = 0.0;
+        for (int j = P.sysRhsOffset[u]; j < P.sysRhsOffset[u + 1]; ++j)
+        {
+            const int a = P.sysRhsUnit[j], m = P.sysRhsMap[j];
+            const double* rc = Wk.rhs + 6L * a;
+            if (m < 0)
+            {
+                acc += rc[r];
+            }
+            else
+            {
+                const double* M = P.rigMap + 36L * m;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) acc += M[i * 6 + r] * rc[i];
+            }
+        }
+        Wr.rhs[id] = acc;
+    }
+
+    // x_cam(a) = sum over the rig units of a of M x_rig(u).
+    template <typename T>
+    __global__ void k_rig_expand(Problem P, Work<T> Wr, Work<T> Wk)
+    {
+        const int id = blockIdx.x * blockDim.x + threadIdx.x;
+        if (id >= P.nUnits * 6) return;
+        const int a = id / 6, i = id % 6;
+        double acc = 0.0;
+        for (int e = 0; e < 2; ++e)
+        {
+            const int u = P.expUnit[2 * a + e];
+            if (u < 0) continue;
+            const int m = P.expMap[2 * a + e];
+            if (m < 0)
+            {
+                acc += double(Wr.x[6L * u + i]);
+            }
+            else
+            {
+                const double* M = P.rigMap + 36L * m;
+#pragma unroll
+                for (int r = 0; r < 6; ++r) acc += M[i * 6 + r] * double(Wr.x[6L * u + r]);
+            }
+        }
+        Wk.x[id] = T(acc);
+    }
+
+    // Step RMS over the rig units (referenceStepRms, BundleAdjustPlaMatrix.cpp:77-158: six entries per capture
+    // and sensor block, the active slots of each group); fixed-order reduction.
+    template <typename T>
+    __global__ void k_rig_step_rms(Problem Pr, Work<T> Wr, double* scalars)
+    {
+        __shared__ double lds[4];
+        double gs = 0.0;
+        for (int u = threadIdx.x; u < Pr.nUnits; u += blockDim.x)
+        {
+            const int lim = Pr.unitWidth[u];
+            for (int a = 0; a < lim; ++a)
+            {
+                const double x = double(Wr.x[u * 6 + a]);
+                gs += x * x;
+            }
+        }
+        gs = waveSum(gs);
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = gs;
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            double t = 0.0;
+            for (int i = 0; i < int(blockDim.x >> 6); ++i) t += lds[i];
+            scalars[kScalarRmsGlobal] = Pr.globalDim > 0 ? sqrt(t / Pr.globalDim) : 0.0;
+        }
+    }
+
+    // Candidate capture and sensor poses of each step length: applyPoseDelta (world-frame / rig-frame left
+    // retraction) of the refined ones (Assembly.cpp:1152-1198), the others unchanged.
+    template <typename T>
+    __global__ void k_rig_trial(Problem P, Work<T> Wk, const double* alphas, int nAlpha)
+    {
+        const int n = P.nCap + P.nSen;
+        const int id = blockIdx.x * blockDim.x + threadIdx.x;
+        if (id >= nAlpha * n) return;
+        const int ai = id / n, j = id % n;
+        const double a = alphas[ai];
+        const bool capture = j < P.nCap;
+        const int k = capture ? j : j - P.nCap;
+        const double* R0 = capture ? P.capR + 9L * k : P.senR + 9L * k;
+        const double* C0 = capture ? P.capC + 3L * k : P.senC + 3L * k;
+        const int unit = capture ? P.capUnit[k] : P.senUnit[k];
+        double* outR = capture ? Wk.candCapR + (long(ai) * P.nCap + k) * 9 : Wk.candSenR + (long(ai) * P.nSen + k) * 9;
+        double* outC = capture ? Wk.candCapC + (long(ai) * P.nCap + k) * 3 : Wk.candSenC + (long(ai) * P.nSen + k) * 3;
+        if (unit < 0)
+        {
+            for (int i = 0; i < 9; ++i) outR[i] = R0[i];
+            for (int i = 0; i < 3; ++i) outC[i] = C0[i];
+            return;
+        }
+        double d[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) d[i] = a * double(Wk.rigX[unit * 6 + i]);
+        double R[9];
+        pbdev::retractExp(R0, d[0], d[1], d[2], R);
+        for (int i = 0; i < 9; ++i) outR[i] = R[i];
+        for (int i = 0; i < 3; ++i) outC[i] = C0[i] + d[3 + i];
     }
 
     // ---------------------------------------------------------------------------------------------
@@ -1864,7 +2080,8 @@ namespace pbk
             k_schur_partial<T><<<nb, 256, 0, s>>>(P, W);
         }
         if (P.nBlocks > 0) k_schur_finalize<T><<<blocksFor(P.nBlocks, 4), 256, 0, s>>>(P, W, damping);
-        if (P.nUnits > 0) k_block_jacobi<T><<<blocksFor(P.nUnits, 64), 64, 0, s>>>(P, W);
+        // a rig problem's preconditioner belongs to its rig system (rigProject)
+        if (P.nUnits > 0 && P.nCap + P.nSen == 0) k_block_jacobi<T><<<blocksFor(P.nUnits, 64), 64, 0, s>>>(P, W);
     }
 
     template <typename T>
@@ -1910,6 +2127,8 @@ namespace pbk
     void Launcher<T>::trialStates(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha, hipStream_t s)
     {
         if (P.nGroup > 0) k_trial_groups<T><<<blocksFor(long(nAlpha) * P.nGroup, 64), 64, 0, s>>>(P, W, alphas, nAlpha);
+        if (P.nCap + P.nSen > 0)
+            k_rig_trial<T><<<blocksFor(long(nAlpha) * (P.nCap + P.nSen), 64), 64, 0, s>>>(P, W, alphas, nAlpha);
         k_trial_cameras<T><<<blocksFor(long(nAlpha) * P.nCam, 64), 64, 0, s>>>(P, W, alphas, nAlpha);
     }
 
@@ -1967,8 +2186,39 @@ namespace pbk
     template <typename T>
     void Launcher<T>::accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s)
     {
-        const long n = std::max<long>(std::max<long>(P.nCam, long(P.nGroup) * 13), P.nElim);
+        const long n = std::max<long>(std::max<long>(std::max<long>(P.nCam, long(P.nGroup) * 13), P.nElim),
+                                      std::max(P.nCap, P.nSen));
         if (n > 0) k_accept<T><<<blocksFor(n, 256), 256, 0, s>>>(P, W, ai, alpha);
+    }
+
+    template <typename T>
+    void Launcher<T>::rigMaps(const Problem& P, hipStream_t s)
+    {
+        if (P.nCam > 0) k_rig_maps<<<blocksFor(P.nCam, 64), 64, 0, s>>>(P);
+    }
+
+    template <typename T>
+    void Launcher<T>::rigProject(const Problem& P, const Problem& Pr, const Work<T>& W, const Work<T>& Wr,
+                                 double damping, hipStream_t s)
+    {
+        if (P.nSysBlocks > 0) k_rig_project<T><<<blocksFor(P.nSysBlocks, 4), 256, 0, s>>>(P, W, Wr, damping);
+        if (P.nSysUnits > 0)
+        {
+            k_rig_rhs<T><<<blocksFor(long(P.nSysUnits) * 6, 128), 128, 0, s>>>(P, W, Wr);
+            k_block_jacobi<T><<<blocksFor(Pr.nUnits, 64), 64, 0, s>>>(Pr, Wr);
+        }
+    }
+
+    template <typename T>
+    void Launcher<T>::rigExpand(const Problem& P, const Work<T>& Wr, const Work<T>& W, hipStream_t s)
+    {
+        if (P.nUnits > 0) k_rig_expand<T><<<blocksFor(long(P.nUnits) * 6, 128), 128, 0, s>>>(P, Wr, W);
+    }
+
+    template <typename T>
+    void Launcher<T>::rigStepRms(const Problem& Pr, const Work<T>& Wr, const Work<T>& W, hipStream_t s)
+    {
+        k_rig_step_rms<T><<<1, 256, 0, s>>>(Pr, Wr, W.scalars);
     }
 
     template struct Launcher<float>;

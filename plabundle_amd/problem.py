@@ -64,7 +64,30 @@ class Problem:
     lp_point: np.ndarray | None = None        # (Nlp, 3) point on the plane
     lp_normal: np.ndarray | None = None       # (Nlp, 3) plane normal
     lp_weight: np.ndarray | None = None       # (Nlp,)
+    # Camera rig (Problem::rig, problem.h:52; placamera::RigTopology): capture poses (rig-to-world rotation, rig
+    # centre), sensor poses (camera-to-rig rotation, camera centre in the rig) and each camera's binding. A bound
+    # camera's pose is R_rig R_sensor, C_rig + R_rig c_sensor; the solve refines captures and sensors.
+    rig_capture_rotation: np.ndarray | None = None  # (Nk, 9) row-major
+    rig_capture_center: np.ndarray | None = None    # (Nk, 3)
+    rig_capture_fixed: np.ndarray | None = None     # (Nk,) int32, RigCapture::fixedPose
+    rig_sensor_rotation: np.ndarray | None = None   # (Ns, 9) row-major
+    rig_sensor_center: np.ndarray | None = None     # (Ns, 3)
+    rig_sensor_fixed: np.ndarray | None = None      # (Ns,) int32, RigSensor::fixedExtrinsic
+    cam_rig_capture: np.ndarray | None = None       # (Nc,) int32 capture of each camera's binding
+    cam_rig_sensor: np.ndarray | None = None        # (Nc,) int32 sensor of each camera's binding
     meta: dict = field(default_factory=dict)
+
+    @property
+    def num_rig_captures(self) -> int:
+        return 0 if self.rig_capture_rotation is None else int(np.asarray(self.rig_capture_rotation).shape[0])
+
+    @property
+    def num_rig_sensors(self) -> int:
+        return 0 if self.rig_sensor_rotation is None else int(np.asarray(self.rig_sensor_rotation).shape[0])
+
+    @property
+    def has_rig(self) -> bool:
+        return self.num_rig_captures > 0 or self.num_rig_sensors > 0
 
     @property
     def num_control_points(self) -> int:

@@ -52,6 +52,8 @@ class Result:
     quality: dict | None = None
     qualityGateRejected: bool = False
     qualityGateMessage: str = ""
+    # Result::refinedRig (result.h:149): capture rotations / centres and sensor rotations / centres, or None
+    refinedRig: dict | None = None
 
     def usable(self) -> bool:
         return self.solutionUsable and self.status in (SolveStatus.Success, SolveStatus.NoConvergence)
@@ -211,6 +213,28 @@ class SolverWorkspace:
             raise RuntimeError(f"pb_hip_get_masks: {self.last_error()}")
         return out
 
+    def rig_masks(self, problem: Problem) -> dict:
+        """ActiveProblem::rigCaptureBlockByCamera and rigSensorBlockByCamera of the last solve."""
+        nc = problem.num_cameras
+        out = {"capture_block": np.zeros(nc, dtype=np.int32), "sensor_block": np.zeros(nc, dtype=np.int32)}
+        st = self._lib.pb_hip_get_rig_masks(self._ctx, _abi.ptr(out["capture_block"], C.c_int32),
+                                            _abi.ptr(out["sensor_block"], C.c_int32))
+        if st != _abi.PB_OK:
+            raise RuntimeError(f"pb_hip_get_rig_masks: {self.last_error()}")
+        return out
+
+    def rig_state(self, problem: Problem) -> dict:
+        """The refined rig of the last solve (the input rig when nothing was published)."""
+        out = {"capture_rotation": np.zeros((problem.num_rig_captures, 9)),
+               "capture_center": np.zeros((problem.num_rig_captures, 3)),
+               "sensor_rotation": np.zeros((problem.num_rig_sensors, 9)),
+               "sensor_center": np.zeros((problem.num_rig_sensors, 3))}
+        st = self._lib.pb_hip_download_rig(self._ctx, *(_abi.ptr(out[k], C.c_double) for k in (
+            "capture_rotation", "capture_center", "sensor_rotation", "sensor_center")))
+        if st != _abi.PB_OK:
+            raise RuntimeError(f"pb_hip_download_rig: {self.last_error()}")
+        return out
+
     def backward(self, problem: Problem, dl_dcenter=None, dl_dpoint=None, with_control: bool = False,
                  dl_drotation=None):
         """Adjoint of the last solve on this workspace: dl/d(u, v) per observation ([num_observations, 2])
@@ -342,6 +366,8 @@ class Solver:
                             _abi.ptr(X, C.c_double), _abi.ptr(V, C.c_int32))
         result.cam_rotation, result.cam_center, result.cam_intrinsics = R, Cc, I
         result.track_point, result.track_valid = X, V
+        if problem.has_rig:
+            result.refinedRig = ws.rig_state(problem)
         if quality and st == _abi.PB_OK and result.usable():
             from .options import quality_to_c
 

@@ -178,6 +178,98 @@ def ring_problem(seed: int = 1, num_cameras: int = 50, num_points: int = 2000, m
                          "truth_points": truth[np.array(truth_index, dtype=np.int64)]})
 
 
+def rig_problem(seed: int = 6, num_captures: int = 16, num_points: int = 600, max_views: int = 8,
+                sensor_yaw_deg=(0.0, 14.0, -14.0), sensor_offset: float = 0.3, noise_px: float = 0.5,
+                capture_rot_sigma: float = 0.01, capture_center_sigma: float = 0.05, sensor_rot_sigma: float = 0.004,
+                sensor_center_sigma: float = 0.02, fixed_sensors=(0,), distinct_groups: bool = False) -> Problem:
+    """A multi-camera rig on a 10 m circle: each capture is a rig pose looking inward at points in a 4 m cube, and
+    every capture carries the same sensors (cameras at sensor_yaw_deg about the rig's down axis, spaced
+    sensor_offset m apart along the rig's right axis). Every camera is bound to its (capture, sensor). Capture
+    and sensor poses are perturbed (fixed sensors stay at the truth), observations get N(0, noise_px) pixels.
+    The cameras' own pose arrays hold the composed initial poses. With distinct_groups each sensor gets its own
+    calibration group id explicitly (otherwise the rig's default grouping applies: one group per sensor)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    W, H, f = 4000, 3000, 3000.0
+    S = len(sensor_yaw_deg)
+    ang = 2 * np.pi * np.arange(num_captures) / num_captures
+    capC = np.stack([10 * np.cos(ang), 10 * np.sin(ang), 0.3 * np.sin(3 * ang)], axis=1)
+    capR = np.stack([look_at_rotation(c, (0, 0, 0)) for c in capC])
+    senR = np.stack([exp_so3(np.array([0.0, np.deg2rad(y), 0.0])) for y in sensor_yaw_deg])
+    senC = np.array([[sensor_offset * (k - (S - 1) / 2), 0.02 * k, 0.0] for k in range(S)])
+    nc = num_captures * S
+    cam_cap = np.repeat(np.arange(num_captures), S).astype(np.int32)
+    cam_sen = np.tile(np.arange(S), num_captures).astype(np.int32)
+    rot = np.einsum("nij,njk->nik", capR[cam_cap], senR[cam_sen])
+    cen = capC[cam_cap] + np.einsum("nij,nj->ni", capR[cam_cap], senC[cam_sen])
+    # radial k1, k2 ~ N(0, 1e-2); tangential p1, p2 ~ N(0, 2e-3), inside the default bound |p| <= 0.02
+    dist = rng.normal(0.0, [1e-2, 1e-2, 2e-3, 2e-3])
+    intr1 = intrinsic_block(f, f, W / 2, H / 2, k1=dist[0], k2=dist[1], p1=dist[2], p2=dist[3])
+    intr = np.repeat(intr1[None], nc, axis=0)
+    truth = rng.uniform(-2.0, 2.0, size=(num_points, 3))
+    tracks, pts, truth_index = [], [], []
+    for p in range(num_points):
+        X = np.repeat(truth[p][None], nc, axis=0)
+        uv, z = project_points(rot, cen, intr, X)
+        vis = np.nonzero((z > 0) & (uv[:, 0] >= 0) & (uv[:, 0] < W) & (uv[:, 1] >= 0) & (uv[:, 1] < H))[0]
+        vis = rng.permutation(vis)[:max_views]
+        if len(vis) < 2:
+            continue
+        vis = np.sort(vis)
+        tracks.append((vis, uv[vis] + rng.normal(0.0, noise_px, size=(len(vis), 2))))
+        pts.append(truth[p] + rng.normal(0.0, 0.1, size=3))
+        truth_index.append(p)
+    fixed_s = np.zeros(S, dtype=np.int32)
+    fixed_s[list(fixed_sensors)] = 1
+    # captures 0 and num_captures / 2 stay at the truth: the automatic gauge fixes the capture of camera 0 and
+    # the capture of the camera farthest from it
+    dRk = exp_so3(rng.normal(0.0, capture_rot_sigma, size=(num_captures, 3)))
+    dCk = rng.normal(0.0, capture_center_sigma, size=(num_captures, 3))
+    dRk[[0, num_captures // 2]] = np.eye(3)
+    dCk[[0, num_captures // 2]] = 0.0
+    capR0 = dRk @ capR
+    capC0 = capC + dCk
+    dRs = exp_so3(rng.normal(0.0, sensor_rot_sigma, size=(S, 3)))
+    dCs = rng.normal(0.0, sensor_center_sigma, size=(S, 3))
+    dRs[fixed_s == 1] = np.eye(3)
+    dCs[fixed_s == 1] = 0.0
+    senR0 = dRs @ senR
+    senC0 = senC + dCs
+    rot0 = np.einsum("nij,njk->nik", capR0[cam_cap], senR0[cam_sen])
+    cen0 = capC0[cam_cap] + np.einsum("nij,nj->ni", capR0[cam_cap], senC0[cam_sen])
+    return _finish(rot0, cen0, intr, np.array(pts), tracks,
+                   cam_image_size=np.tile(np.array([W, H], dtype=np.int32), (nc, 1)),
+                   cam_group=(cam_sen.copy() if distinct_groups else None),
+                   gauge_policy=GaugePolicy.AutoAnchor,
+                   rig_capture_rotation=np.ascontiguousarray(capR0.reshape(-1, 9)),
+                   rig_capture_center=np.ascontiguousarray(capC0),
+                   rig_capture_fixed=np.zeros(num_captures, dtype=np.int32),
+                   rig_sensor_rotation=np.ascontiguousarray(senR0.reshape(-1, 9)),
+                   rig_sensor_center=np.ascontiguousarray(senC0),
+                   rig_sensor_fixed=fixed_s,
+                   cam_rig_capture=cam_cap,
+                   cam_rig_sensor=cam_sen,
+                   meta={"workload": "rig", "seed": seed, "truth_capture_rotation": capR,
+                         "truth_capture_center": capC, "truth_sensor_rotation": senR, "truth_sensor_center": senC,
+                         "truth_points": truth[np.array(truth_index, dtype=np.int64)]})
+
+
+def rig_options(options=None, max_iterations: int = 10, intrinsics: bool = True):
+    """Options of the rig workload: Huber 1 px and, with intrinsics, the shared focal length and radial
+    distortion refined per group. A rig always takes the assembled Schur path with the default
+    parameterisation (Assembly.cpp:24-29), whatever useReferenceOnlineSchur says."""
+    from .options import ImageRobustLoss, SolveOptions
+
+    o = options or SolveOptions()
+    o.solver.imageRobustLoss = ImageRobustLoss.Huber
+    o.solver.imageRobustLossScalePixels = 1.0
+    o.solver.maxIterations = max_iterations
+    o.solver.useReferenceOnlineSchur = True
+    if intrinsics:
+        o.calibration.refineSharedFocalLength = True
+        o.calibration.refineSharedRadialDistortion = True
+    return o
+
+
 def add_survey_constraints(problem: Problem, seed: int = 11, num_control: int = 6, num_planes: int = 0,
                            control_sigma: float = 0.02, plane_noise: float = 0.01, kinds=(0,),
                            truth_points=None) -> Problem:
