@@ -554,7 +554,7 @@ def main():
             "loss": "Cauchy 1 px",
             "refined_intrinsics": "shared focal + k1,k2,k3,p1,p2 over 4 groups",
             "stop": f"step RMS <= 1e-6 (reference), at most {args.max_iterations} LM iterations",
-            "path": "online point Schur, dense reduced solve (64x64-tile MFMA Cholesky)",
+            "path": f"online point Schur, reduced solve: {mixed[0].report['linear_solver_name']}",
         },
         "time_to_converge_s": round(mixed_t, 6),
         "lm_iterations": mixed_it,

@@ -1252,6 +1252,17 @@ struct pb_ctx
     pbk::Work<float> Wsys32;
     pbk::Work<double> Wsys64;
     std::vector<double> outCapR, outCapC, outSenR, outSenC;
+    // Cluster block-Jacobi PCG (pb_cluster.hip): the plan for the current structure and cluster size, used by the
+    // PCG route whenever a preconditioner block spans more than one unit.
+    struct Cluster
+    {
+        bool active = false;  // the current solve takes the cluster PCG
+        bool wide = false;    // the plan has a cluster of more than one unit
+        int size = 0;
+        int64_t planBuild = -1;
+        DeviceAlloc mem;
+        pbk::ClusterDev dev;
+    } cluster;
     int costBlocks = 0;
     double* pinned = nullptr;  // host-pinned readback area
     // last results
@@ -1692,6 +1703,8 @@ namespace
         ctx->dcons.release();
         ctx->Q = pbk::Quality{};
         ctx->work32 = ctx->work64 = false;
+        ctx->cluster.mem.release();
+        ctx->cluster = pb_ctx::Cluster{};
     }
 
     // Device copy of a prepared problem and its layout (setup).
@@ -2150,6 +2163,8 @@ namespace
             Span sp(ctx, 2);
             Lr::schurBlocks(P, W, damping, s);
             if (ctx->rigSys) Lr::rigProject(P, ctx->Psys, W, sysWork<T>(ctx), damping, s);
+            if (ctx->cluster.active)
+                pbk::ClusterLauncher<T>::factor(sysProblem(ctx), sysWork<T>(ctx).S, ctx->cluster.dev, W.flags, s);
         }
         double* h = ctx->pinned;
         check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s), "readback");
@@ -2213,10 +2228,52 @@ namespace
         double rnorm = 0.0;
     };
 
+    // PCG with the cluster block-Jacobi preconditioner (four launches per iteration, chunks between host polls).
+    template <typename T>
+    PcgResult clusterPcg(pb_ctx* ctx, pbk::Work<T>& W, double relTol, double absTol, int maxIter, int checkEvery,
+                         double modelEta)
+    {
+        using CL = pbk::ClusterLauncher<T>;
+        hipStream_t s = ctx->stream;
+        const pbk::Problem& P = sysProblem(ctx);
+        PcgResult res;
+        if (P.nUnits == 0)
+        {
+            res.converged = true;
+            return res;
+        }
+        double* h = ctx->pinned + kPinPcg;
+        int32_t* hf = reinterpret_cast<int32_t*>(ctx->pinned + kPinPcgFlags);
+        auto readback = [&]() {
+            check(hipMemcpyAsync(hf, W.iflags, sizeof(int32_t) * pbk::kIflagCount, hipMemcpyDeviceToHost, s), "readback");
+            check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s), "readback");
+            check(hipStreamSynchronize(s), "pcg");
+        };
+        CL::init(P, W, ctx->cluster.dev, relTol, absTol, s);
+        int it = 0;
+        while (true)
+        {
+            const int chunk = std::min(std::max(checkEvery, it / 4), 256);
+            const int last = std::min(it + chunk, maxIter);
+            for (; it < last; ++it) CL::iteration(P, W, ctx->cluster.dev, it, maxIter, modelEta, s);
+            readback();
+            if (hf[pbk::kIflagDone0] || it >= maxIter) break;
+        }
+        res.breakdown = hf[pbk::kIflagBreakdown] != 0;
+        res.iterations = hf[pbk::kIflagIters];
+        res.rnorm = h[pbk::kScalarRnorm];
+        res.modelStop = hf[pbk::kIflagModelStop] != 0;
+        const bool truncated = modelEta > 0.0 && res.iterations >= maxIter;
+        res.converged = !res.breakdown && (res.modelStop || truncated || res.rnorm <= h[pbk::kScalarThreshold]);
+        if (ctx->profile) ctx->pcgIterationsProfiled += res.iterations;
+        return res;
+    }
+
     template <typename T>
     PcgResult pcg(pb_ctx* ctx, pbk::Work<T>& W, double relTol, double absTol, int maxIter, int checkEvery,
                   double modelEta)
     {
+        if (ctx->cluster.active) return clusterPcg(ctx, W, relTol, absTol, maxIter, checkEvery, modelEta);
         using Lr = pbk::Launcher<T>;
         hipStream_t s = ctx->stream;
         const pbk::Problem& P = sysProblem(ctx);
@@ -2789,6 +2846,61 @@ namespace
 
     // Reduced-solve route of a solve. PB_REDUCED_AUTO takes the band route when the pattern is block-tridiagonal
     // over at least three super-blocks, else the dense Cholesky up to dense_max_rows rows, else PCG.
+    // Clusters of `size` consecutive primary blocks of the reduced system the routes solve (a pose unit, or all
+    // units of one intrinsic group), as contiguous unit ranges. The cluster PCG runs when some cluster spans more
+    // than one unit; otherwise the fused PCG's per-unit blocks are that preconditioner.
+    bool planClusters(pb_ctx* ctx, int size)
+    {
+        auto& Cl = ctx->cluster;
+        if (Cl.planBuild == ctx->patternBuilds && Cl.size == size)
+        {
+            Cl.active = Cl.wide;
+            return Cl.active;
+        }
+        Cl.mem.release();
+        Cl.dev = pbk::ClusterDev{};
+        Cl.active = Cl.wide = false;
+        Cl.size = size;
+        Cl.planBuild = ctx->patternBuilds;
+        const Prepared& p = ctx->prep;
+        const int pose = p.refCameraBlocks();
+        std::vector<std::pair<int, int>> primary;  // unit ranges of the primary blocks, in order
+        for (int u = 0; u < pose; ++u) primary.push_back({u, u + 1});
+        for (int g = 0; g < p.G; ++g)
+            if (p.gK[g] > 0)
+            {
+                const int first = p.gUnit[g] - p.cameraBlocks + pose;
+                primary.push_back({first, first + (p.gK[g] + 5) / 6});
+            }
+        std::vector<int32_t> begin{0}, unitCluster;
+        std::vector<long long> off{0};
+        bool wide = false;
+        int maxDim = 0;
+        for (size_t b = 0; b < primary.size(); b += size)
+        {
+            const int u0 = primary[b].first, u1 = primary[std::min(primary.size(), b + size) - 1].second;
+            wide = wide || u1 - u0 > 1;
+            const int d = 6 * (u1 - u0);
+            maxDim = std::max(maxDim, d);
+            for (int u = u0; u < u1; ++u) unitCluster.push_back(static_cast<int32_t>(begin.size() - 1));
+            begin.push_back(u1);
+            off.push_back(off.back() + static_cast<long long>(d) * d);
+        }
+        if (!wide) return false;
+        hipStream_t s = ctx->stream;
+        pbk::ClusterDev& D = Cl.dev;
+        D.n = static_cast<int>(begin.size()) - 1;
+        D.maxDim = maxDim;
+        D.unitBegin = Cl.mem.upload(begin, s);
+        D.unitCluster = Cl.mem.upload(unitCluster, s);
+        D.matOff = Cl.mem.upload(off, s);
+        D.inv = Cl.mem.alloc<double>(static_cast<size_t>(off.back()));
+        D.work = Cl.mem.alloc<double>(static_cast<size_t>(off.back()));
+        check(hipStreamSynchronize(s), "cluster plan");
+        Cl.active = Cl.wide = true;
+        return true;
+    }
+
     Route reducedRoute(pb_ctx* ctx, const pb_solve_options& o)
     {
         if (o.reduced_solver == PB_REDUCED_PCG) return kRoutePcg;
@@ -3312,8 +3424,13 @@ namespace
                       : ctx->band.active       ? (pg.seed ? "band-cr-fp32-refined" : "band-cr-fp64")
                       : pg.seed                ? "dense-cholesky-fp32-refined"
                                                : "dense-cholesky-fp64");
-        std::snprintf(rep->preconditioner_name, sizeof(rep->preconditioner_name), "%s",
-                      pg.dense ? "none" : "block-jacobi-6");
+        if (pg.dense)
+            std::snprintf(rep->preconditioner_name, sizeof(rep->preconditioner_name), "none");
+        else if (ctx->cluster.active)
+            std::snprintf(rep->preconditioner_name, sizeof(rep->preconditioner_name), "cluster-jacobi-%d",
+                          ctx->cluster.size);
+        else
+            std::snprintf(rep->preconditioner_name, sizeof(rep->preconditioner_name), "block-jacobi-6");
         rep->accepted_steps = pg.accepted;
         rep->rejected_steps = pg.rejected;
         rep->iterations = pg.iterations;
@@ -3330,8 +3447,10 @@ namespace
         rep->converged = converged ? 1 : 0;
         rep->solve_status = converged ? PB_SOLVE_SUCCESS : PB_SOLVE_NO_CONVERGENCE;
         rep->solution_usable = std::isfinite(current) ? 1 : 0;
-        std::string msg = converged ? (rep->online_schur_used ? "converged (online Schur + block-Jacobi PCG + Armijo)"
-                                                              : "converged (assembled Schur + block-Jacobi PCG + line search)")
+        const std::string route = !pg.dense ? "block-Jacobi PCG" : ctx->band.active ? "band cyclic reduction"
+                                                                                    : "dense Cholesky";
+        std::string msg = converged ? (rep->online_schur_used ? "converged (online Schur + " + route + " + Armijo)"
+                                                              : "converged (assembled Schur + " + route + " + line search)")
                                     : "iteration budget or line search exhausted";
         if (pg.fp64Finish)
             msg += pg.handOverBySwitch ? "; FP32 assembly handed over to FP64 at its rounding floor"
@@ -4593,13 +4712,10 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
             (void)bandArena<double>(ctx);
             if (options->precision != PB_PRECISION_FP64) (void)bandArena<float>(ctx);
         }
-        if (!pg.dense && options->preconditioner_cluster_size > 1)
-        {
-            report->solve_status = PB_SOLVE_UNSUPPORTED;
-            setMessage(report, "preconditioner clusters above 1 are not implemented on the PCG route "
-                               "(the dense route has no preconditioner)");
-            return fail(ctx, PB_ERR_UNSUPPORTED, report->message);
-        }
+        // the PCG route's preconditioner: per-unit blocks in the fused iteration, or clusters of primary blocks
+        ctx->cluster.active = false;
+        if (!pg.dense) planClusters(ctx, options->preconditioner_cluster_size);
+        if (pg.dense) ctx->cluster.active = false;
         check(hipStreamSynchronize(ctx->stream), "setup");
         report->setup_seconds = seconds(ts);
         ctx->lastOptions = *options;

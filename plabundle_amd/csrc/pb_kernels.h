@@ -534,6 +534,39 @@ namespace pbk
         static void fromBand(const BandView& B, double* x, const TA* Z, bool overwrite, hipStream_t s);
     };
 
+    // Cluster block-Jacobi PCG (plaMatrixPreconditionerClusterSize, options.h:164; used whenever a preconditioner
+    // block spans more than one unit: clusters of several primary blocks, or one intrinsic group of more than six
+    // refined slots). A cluster is a run of consecutive primary blocks (a pose unit, or every unit of one group),
+    // so its units are a contiguous range; its d x d diagonal block of S (d = 6 units) is gathered, factored and
+    // inverted once per build, and the PCG applies the inverse as a dense product. The iterations run as four
+    // launches (S p, the step, the preconditioner, the direction), every dot product reduced from per-block
+    // partials in a fixed order.
+    struct ClusterDev
+    {
+        int n = 0;                          // clusters
+        int maxDim = 0;
+        const int32_t* unitBegin = nullptr;  // [n+1] unit range of cluster c
+        const int32_t* unitCluster = nullptr;  // [nUnits]
+        const long long* matOff = nullptr;   // [n+1] offset (doubles) of the cluster's d x d block
+        double* inv = nullptr;               // the inverses (symmetric, row-major)
+        double* work = nullptr;              // factorisation scratch, same layout
+    };
+
+    template <typename T>
+    struct ClusterLauncher
+    {
+        // gather, Cholesky and inverse of every cluster's diagonal block of S (flags[kFlagNotSpd] on a pivot that
+        // is not positive)
+        static void factor(const Problem& P, const T* S, const ClusterDev& C, int32_t* flags, hipStream_t s);
+        // x = 0, r = rhs, z = M r, p = z, the stop threshold max(relTol |rhs|, absTol)
+        static void init(const Problem& P, const Work<T>& W, const ClusterDev& C, double relTol, double absTol,
+                         hipStream_t s);
+        // iteration it (stops at maxIt, on the residual threshold, or on the truncated-Newton model test when
+        // modelEta > 0); launches after the stop return at once
+        static void iteration(const Problem& P, const Work<T>& W, const ClusterDev& C, int it, int maxIt,
+                              double modelEta, hipStream_t s);
+    };
+
     // Post-solve quality buffers (device): strict track RMS at the input and refined state, sort keys, filter.
     struct Quality
     {
