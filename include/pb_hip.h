@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define PB_ABI_VERSION 3
+#define PB_ABI_VERSION 4
 #define PB_INTRINSIC_COUNT 13
 
 typedef enum pb_status {
@@ -125,9 +125,10 @@ typedef enum pb_control_uncertainty {
 
 /* Problem, structure-of-arrays, observations grouped by track.
  * Mirrors plabundle::Problem (include/plabundle/problem.h:37-54) restricted to
- * frame-pinhole cameras without rigs, with the per-track survey constraints of
+ * frame-pinhole cameras, with the per-track survey constraints of
  * plabundle::Track (control points and LiDAR point-to-plane constraints,
- * include/plabundle/constraints.h:136-141) as flat arrays keyed by track. */
+ * include/plabundle/constraints.h:136-141) as flat arrays keyed by track and the
+ * camera rig as capture / sensor poses with one binding per camera. */
 typedef struct pb_problem_view {
     int32_t num_cameras;
     const double* cam_rotation;     /* [num_cameras*9] camera-to-world, row-major */
@@ -276,6 +277,10 @@ typedef struct pb_solve_options {
      * meets the step-RMS stop, hands the solve to FP64 assembly for the rest (0 = 1e-6, the FP32 rounding floor of
      * the assembled model; negative = FP32 to the end, stopping only when a line search fails). */
     double mixed_switch_decrease;
+    /* Batched windows: 1 (default) factors each window's system with the MFMA tile Cholesky (64 x 64 tiles, one
+     * fixed launch); 0 uses the left-looking panel factorisation whose launch shape batch_factor_shape fixes or
+     * the autotuner times. The two order their sums differently and agree to rounding. */
+    int32_t batch_factor_tiles;
 } pb_solve_options;
 
 /* Per-solve report. Mirrors plabundle::PlaMatrixDiagnostics (include/plabundle/result.h:91-128). */

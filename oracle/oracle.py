@@ -134,6 +134,7 @@ def default_options() -> _abi.SolveOptionsC:
     o.reduced_solver = 0
     o.dense_max_rows = 0
     o.preconditioner_cluster_size = 1
+    o.batch_factor_tiles = 1
     return o
 
 

@@ -133,6 +133,7 @@ class SolveOptionsC(C.Structure):
         ("dense_max_rows", C.c_int32),
         ("preconditioner_cluster_size", C.c_int32),
         ("mixed_switch_decrease", C.c_double),
+        ("batch_factor_tiles", C.c_int32),
     ]
 
 

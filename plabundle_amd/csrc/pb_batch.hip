@@ -10,7 +10,8 @@
 //   k_batch_round_begin   activity, flags and line-search state of the round
 //   (shared build)        effective cameras, linearisation, camera terms, Schur blocks with the window damping
 //   k_batch_after_build   objective of the linearisation point, build failures -> reject or stop
-//   k_batch_factor        dense S of the window, Cholesky, forward/backward substitution -> camera step
+//   k_batch_tiles         dense S of the window, MFMA tile Cholesky, tile forward/backward solves -> camera step
+//                         (k_batch_factor: the left-looking panel factorisation with autotuned launch shapes)
 //   (shared back-subst.)  point steps
 //   k_batch_step_stats    directional derivative and step RMS values per window
 //   trial passes          k_batch_alphas, trial states (shared), k_batch_trial_cost, k_batch_decide
@@ -21,6 +22,7 @@
 #include "pb_common.h"
 #include "pb_device.h"
 #include "pb_kernels.h"
+#include "pb_tile.h"
 
 #include <hip/hip_runtime.h>
 
@@ -405,6 +407,152 @@ namespace pbk
         for (int i = tid; i < n; i += NT) Wk.x[unitOf(i / 6) * 6 + i % 6] = T(vec[i]);
     }
 
+    // Tile factorisation of each window's system (the default): the system gathered into a column-major matrix of
+    // order np = 64 nt with identity padding, factored right-looking in 64 x 64 tiles by the LDS tile routines of
+    // the band route (one-wave panel Cholesky of the diagonal tile, its triangular inverse, MFMA 16x16x4 products
+    // written straight to memory); each diagonal tile is replaced by the inverse of its factor, so the forward
+    // and backward solves are tile products: y_k = Linv_kk (b_k - sum_{j<k} L_kj y_j), x_k = Linv_kk^T (y_k -
+    // sum_{i>k} L_ik^T x_i). Two LDS tiles per workgroup, two workgroups per CU. Each value is computed in one
+    // fixed order (tile k ascending), so repeated batches are bitwise identical; the panel factorisation
+    // (k_batch_factor) orders its sums differently and agrees to rounding.
+    constexpr int kTileLdsBytes = int(2 * kDenseTile * (kDenseTile + 1) * sizeof(double) +
+                                      (kDenseTile + kMaxDense + 256) * sizeof(double));
+
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_batch_tiles(Problem P, Work<T> Wk, Batch B)
+    {
+        using namespace tile;
+        extern __shared__ __align__(16) unsigned char tileLds[];
+        double(*b0)[kLd] = reinterpret_cast<double(*)[kLd]>(tileLds);
+        double(*b1)[kLd] = b0 + kT;
+        double* col = reinterpret_cast<double*>(b1 + kT);  // [64] pivot column / tile vector
+        double* vec = col + kT;                            // [kMaxDense] right-hand side, then solution
+        double* part = vec + kMaxDense;                    // [256] partial sums
+        const int w = blockIdx.x;
+        if (!B.active[w] || B.reject[w]) return;
+        const int pu0 = B.poseUnitOffset[w], pu1 = B.poseUnitOffset[w + 1];
+        const int gu0 = B.groupUnitOffset[w];
+        const int nPose = pu1 - pu0;
+        const int nU = nPose + (B.groupUnitOffset[w + 1] - gu0);
+        const int n = 6 * nU;
+        if (n == 0) return;
+        const int nt = (n + kT - 1) / kT, np = nt * kT;
+        const int tid = threadIdx.x;
+        double* A = B.dense + B.denseOffset[w];
+        auto unitOf = [&](int r) { return r < nPose ? pu0 + r : gu0 + (r - nPose); };
+        auto localOf = [&](int u) { return (u >= pu0 && u < pu1) ? u - pu0 : nPose + (u - gu0); };
+
+        // gather: identity, then every element of the stored blocks of the window's rows to its place
+        for (long i = tid; i < long(np) * np; i += 256) A[i] = (i % np == i / np && i % np >= n) ? 1.0 : 0.0;
+        for (int i = tid; i < np; i += 256) vec[i] = i < n ? Wk.rhs[unitOf(i / 6) * 6 + i % 6] : 0.0;
+        __syncthreads();
+        {
+            const int kp0 = P.rowPtr[pu0], nbp = P.rowPtr[pu1] - kp0;
+            const int kg0 = P.rowPtr[gu0], nbg = P.rowPtr[B.groupUnitOffset[w + 1]] - kg0;
+            const long total = long(nbp + nbg) * 36;
+            for (long e0 = tid; e0 < total; e0 += kDenseLoads * 256)
+            {
+                double v[kDenseLoads];
+                size_t at[kDenseLoads];
+#pragma unroll
+                for (int u = 0; u < kDenseLoads; ++u)
+                {
+                    const long e = e0 + long(u) * 256;
+                    const long ec = e < total ? e : total - 1;
+                    const int bi = int(ec / 36), el = int(ec % 36);
+                    const int k = bi < nbp ? kp0 + bi : kg0 + (bi - nbp);
+                    const int r = localOf(P.nnzRow[k]), c = localOf(P.colIdx[k]);
+                    at[u] = size_t(6 * c + el % 6) * np + 6 * r + el / 6;
+                    v[u] = double(Wk.S[size_t(k) * 36 + el]);
+                }
+#pragma unroll
+                for (int u = 0; u < kDenseLoads; ++u)
+                    if (e0 + long(u) * 256 < total) A[at[u]] = v[u];
+            }
+        }
+        __syncthreads();
+
+        // right-looking tile Cholesky
+        bool bad = false;
+        for (int k = 0; k < nt; ++k)
+        {
+            loadTile(b0, A, np, k * kT, k * kT);
+            __syncthreads();
+            cholTile(b0, col, bad);
+            invTile(b0, b1, col);  // b1 = L_kk^-1 (synchronised inside)
+            storeTile(A, np, k * kT, k * kT, b1);
+            for (int i = k + 1; i < nt; ++i)
+            {
+                __syncthreads();
+                loadTile(b0, A, np, i * kT, k * kT);
+                __syncthreads();
+                tileProductGlobal(A + size_t(k * kT) * np + i * kT, np, b0, false, b1, true, false);  // L_ik
+            }
+            for (int j = k + 1; j < nt; ++j)
+            {
+                __syncthreads();
+                loadTile(b1, A, np, j * kT, k * kT);  // L_jk
+                for (int i = j; i < nt; ++i)
+                {
+                    __syncthreads();
+                    loadTile(b0, A, np, i * kT, k * kT);  // L_ik
+                    __syncthreads();
+                    tileProductGlobal(A + size_t(j * kT) * np + i * kT, np, b0, false, b1, true, true);
+                }
+            }
+            __syncthreads();
+        }
+        __shared__ int failed;
+        if (tid == 0) failed = 0;
+        __syncthreads();
+        if (bad) failed = 1;
+        __syncthreads();
+        if (failed)
+        {
+            if (tid == 0) B.reject[w] = 1;
+            return;
+        }
+
+        // forward: y_k = Linv_kk (b_k - sum_{j<k} L_kj y_j); thread t sums row t & 63 over every fourth column
+        const int rr = tid & 63, pp = tid >> 6;
+        for (int k = 0; k < nt; ++k)
+        {
+            const int r0 = k * kT;
+            double sacc = 0.0;
+            for (int c = pp; c < r0; c += 4) sacc += A[size_t(c) * np + r0 + rr] * vec[c];
+            part[tid] = sacc;
+            __syncthreads();
+            if (tid < kT) col[tid] = vec[r0 + tid] - ((part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]));
+            __syncthreads();
+            if (tid < kT)
+            {
+                double y = 0.0;
+                for (int c = 0; c <= tid; ++c) y += A[size_t(r0 + c) * np + r0 + tid] * col[c];
+                vec[r0 + tid] = y;
+            }
+            __syncthreads();
+        }
+        // backward: x_k = Linv_kk^T (y_k - sum_{i>k} L_ik^T x_i)
+        for (int k = nt - 1; k >= 0; --k)
+        {
+            const int r0 = k * kT;
+            double sacc = 0.0;
+            for (int i = r0 + kT + pp; i < np; i += 4) sacc += A[size_t(r0 + rr) * np + i] * vec[i];
+            part[tid] = sacc;
+            __syncthreads();
+            if (tid < kT) col[tid] = vec[r0 + tid] - ((part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]));
+            __syncthreads();
+            if (tid < kT)
+            {
+                double x = 0.0;
+                for (int c = tid; c < kT; ++c) x += A[size_t(r0 + tid) * np + r0 + c] * col[c];
+                vec[r0 + tid] = x;
+            }
+            __syncthreads();
+        }
+        for (int i = tid; i < n; i += 256) Wk.x[unitOf(i / 6) * 6 + i % 6] = T(vec[i]);
+    }
+
     // Directional decrease dc.directRhs + sum bp.dp and the two step RMS values of each window
     // (BundleAdjustPlaMatrix.cpp:77-158), in a fixed reduction order.
     template <typename T>
@@ -779,6 +927,9 @@ namespace pbk
     template <typename T>
     void BatchLauncher<T>::configure(const Batch& B, bool (&usable)[kFactorShapes])
     {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_tiles<T>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsBytes) != hipSuccess)
+            throw std::runtime_error("hipFuncSetAttribute (tile window factorisation LDS)");
         for (int v = 0; v < kFactorShapes; ++v)
         {
             const size_t lds = factorLds(B, v);
@@ -795,6 +946,11 @@ namespace pbk
     template <typename T>
     void BatchLauncher<T>::factor(const Problem& P, const Work<T>& W, const Batch& B, int shape, hipStream_t s)
     {
+        if (shape == kFactorTiles)
+        {
+            k_batch_tiles<T><<<B.nWin, 256, kTileLdsBytes, s>>>(P, W, B);
+            return;
+        }
         const size_t lds = factorLds(B, shape);
         switch (shape)
         {

@@ -1648,6 +1648,7 @@ namespace
             return "plaMatrixPreconditionerClusterSize must be in [1, 16]";
         if (o->pcg_relative_tolerance < 0.0 || o->pcg_absolute_tolerance < 0.0 || o->pcg_max_iterations < 0)
             return "PCG controls must be non-negative";
+        if (o->batch_factor_tiles != 0 && o->batch_factor_tiles != 1) return "batch_factor_tiles must be 0 or 1";
         if (o->batch_factor_shape < -1 || o->batch_factor_shape >= pbk::kFactorShapes)
             return "batch_factor_shape must be -1 (timed) or a shape index below " + std::to_string(pbk::kFactorShapes);
         return {};
@@ -4010,7 +4011,8 @@ namespace
                                             " primary units; the batched dense solve takes at most " +
                                             std::to_string(pbk::kMaxDense / 6));
             maxDim = std::max(maxDim, n);
-            denseOffset.push_back(denseOffset.back() + int64_t(n) * n);
+            const int64_t np = (n + pbk::kDenseTile - 1) / pbk::kDenseTile * pbk::kDenseTile;  // tile padding
+            denseOffset.push_back(denseOffset.back() + np * np);
             for (int b = obsOffset[w]; b < obsOffset[w + 1]; b += pbk::kTrialChunk)
             {
                 chunkWin.push_back(w);
@@ -4296,8 +4298,11 @@ namespace
         int shape[2];
         for (int pi = 0; pi < 2; ++pi)
         {
-            shape[pi] = o.batch_factor_shape >= 0 ? o.batch_factor_shape : ft.shape[pi];
-            if (o.batch_factor_shape >= 0 && (pi == 1 || W32) && !ft.usable[pi][o.batch_factor_shape])
+            shape[pi] = o.batch_factor_tiles       ? pbk::kFactorTiles
+                        : o.batch_factor_shape >= 0 ? o.batch_factor_shape
+                                                    : ft.shape[pi];
+            if (!o.batch_factor_tiles && o.batch_factor_shape >= 0 && (pi == 1 || W32) &&
+                !ft.usable[pi][o.batch_factor_shape])
                 throw std::invalid_argument("unsupported: batch_factor_shape " + std::to_string(o.batch_factor_shape) +
                                   " needs more LDS than a workgroup has for windows of " +
                                   std::to_string(B.maxDim) + " unknowns");
@@ -4511,6 +4516,7 @@ void pb_default_options(pb_solve_options* o)
     o->previous_intrinsic_mask = 0;
     o->batch_graphs = 1;
     o->batch_factor_shape = -1;
+    o->batch_factor_tiles = 1;
     o->pcg_graphs = 1;
     o->refine_camera_pose = 1;
     o->shared_intrinsic_mask = 0;

@@ -272,6 +272,7 @@ namespace pbk
         int threads, panel;
     };
     constexpr int kFactorShapes = 6;
+    constexpr int kFactorTiles = kFactorShapes;  // BatchLauncher::factor: the MFMA tile factorisation instead
     constexpr FactorShape kFactorShape[kFactorShapes] = {{256, 16}, {128, 16}, {512, 16},
                                                          {256, 8},  {128, 8},  {512, 8}};
     constexpr int kFactorLdsLimit = 160 * 1024;  // LDS of one CU (gfx950)

@@ -155,7 +155,8 @@ class BackendOptions:
     pcgModelEta: float = -1.0  # < 0: path default (off for FP64, 0.1 for mixed)
     stepRmsTolerance: float = 0.0  # 0: the reference's 1e-6
     batchGraphs: bool = True  # batched windows: replay each round's launches as captured HIP graphs
-    batchFactorShape: int = -1  # batched windows: dense factorisation launch shape (-1 = timed on the first round)
+    batchFactorShape: int = -1  # batched windows: panel factorisation launch shape (-1 = timed on the first round)
+    batchFactorTiles: bool = True  # batched windows: MFMA tile factorisation (False: the panel factorisation)
     pcgGraphs: bool = True  # replay PCG iterations from a captured HIP graph
     reducedSolver: ReducedSolver = ReducedSolver.Auto
     denseMaxRows: int = 0  # 0: 16384 rows
@@ -261,6 +262,7 @@ def to_c(options: SolveOptions, precision: Precision | None = None) -> _abi.Solv
     o.dense_max_rows = int(b.denseMaxRows)
     o.preconditioner_cluster_size = int(b.plaMatrixPreconditionerClusterSize)
     o.mixed_switch_decrease = float(b.mixedSwitchDecrease)
+    o.batch_factor_tiles = 1 if b.batchFactorTiles else 0
     k = options.constraints
     o.laser_plane_weight = float(k.laserPlaneWeight)
     o.laser_huber_delta = float(k.laserHuberDeltaMeters)

@@ -189,6 +189,7 @@ def test_batch_factor_shapes_are_bitwise_equal_and_timed(ws, precision):
     rng = np.random.Generator(np.random.PCG64(21))
     wins = _windows(n=4, seed=17) + [S.window_problem(rng, 64, 1500), S.window_problem(rng, 41, 900)]
     o = S.window_options(PO.SolveOptions(), max_iterations=6)
+    o.backend.batchFactorTiles = False
     runs = []
     for shape in range(_abi.PB_BATCH_FACTOR_SHAPES):
         o.backend.batchFactorShape = shape
@@ -232,3 +233,34 @@ def test_batch_factor_shapes_are_bitwise_equal_and_timed(ws, precision):
     o.backend.batchFactorShape = _abi.PB_BATCH_FACTOR_SHAPES
     res, summ = Solver().solve_batch(wins, o, ws, precision=precision)
     assert summ["abi_status"] == _abi.PB_ERR_INVALID
+
+
+@pytest.mark.parametrize("precision", [PO.Precision.Fp64, PO.Precision.Mixed])
+def test_tile_factorisation_matches_panels_and_the_oracle(ws, oracle_mod, precision):
+    # the MFMA tile factorisation (default) and the panel factorisation order their sums differently: windows of 8
+    # to 64 cameras (systems of 48 to 402 unknowns: one to seven 64-row tiles, padded) must reach the same
+    # trajectory and final costs within rounding, the FP64 ones the oracle's; repeated tile batches are bitwise
+    # identical
+    _say(f"case batch tile factorisation precision={precision.name}")
+    rng = np.random.Generator(np.random.PCG64(23))
+    wins = _windows(n=4, seed=19) + [S.window_problem(rng, 64, 1500), S.window_problem(rng, 41, 900),
+                                      S.window_problem(rng, 8, 500), S.window_problem(rng, 11, 600)]
+    o = S.window_options(PO.SolveOptions(), max_iterations=12)
+    o.backend.batchFactorTiles = True
+    tiles, st = Solver().solve_batch(wins, o, ws, precision=precision)
+    again, _ = Solver().solve_batch(wins, o, ws, precision=precision)
+    o.backend.batchFactorTiles = False
+    o.backend.batchFactorShape = 0
+    panels, sp = Solver().solve_batch(wins, o, ws, precision=precision)
+    assert st["abi_status"] == sp["abi_status"] == _abi.PB_OK, st["message"]
+    assert st["factor_tuning"]["fp64"] is None  # nothing to time
+    for w, (a, b, c) in enumerate(zip(tiles, again, panels)):
+        assert a.finalCost == b.finalCost and np.array_equal(a.track_point, b.track_point), w
+        assert a.report["iterations"] == c.report["iterations"], w
+        assert abs(a.finalCost - c.finalCost) <= 1e-10 * c.finalCost, w
+        np.testing.assert_allclose(a.cam_center, c.cam_center, atol=1e-8)
+        if precision == PO.Precision.Fp64:
+            ref = oracle_mod.Oracle(wins[w], threads=4).solve(PO.to_c(S.window_options(PO.SolveOptions(),
+                                                                                       max_iterations=12)))
+            assert a.report["iterations"] == ref["iterations"], w
+            assert abs(a.finalCost - ref["final_cost"]) <= 1e-9 * ref["final_cost"], w
