@@ -140,56 +140,99 @@ namespace pbk
             }
         }
 
-        // inv = L^-1 of the lower-triangular tile L (zeros above the diagonal) by the whole workgroup, in four row
-        // panels of 16: every thread subtracts the finished panels' contribution from its entries of the panel
-        // rows (inv(i, j) = [i == j] - sum_{k < i0} L(i, k) inv(k, j)), then wave 0 substitutes down the panel with
-        // lane j holding column j's 16 entries in registers.
+        // acc += A B for one 16 x 16 block with MFMA 16x16x4 steps over k = k0 .. k1 - 1 (a multiple of 4 apart);
+        // A(i, k) = a[ac + k][ar + i], B(k, j) = b[bc + j][br + k] (LDS tiles stored buf[column][row])
+        template <typename T>
+        __device__ __forceinline__ void mfmaBlock(typename BandMfma<T>::V& acc, T (*a)[kLd], int ar, int ac,
+                                                  T (*b)[kLd], int br, int bc, int k0, int k1)
+        {
+            const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+            for (int k = k0; k < k1; k += 4)
+                acc = BandMfma<T>::op(a[ac + k + lk][ar + li], b[bc + li][br + k + lk], acc);
+        }
+
+        // o(r0 + i, c0 + j) = (neg ? -1 : 1) acc(i, j) for the 16 x 16 block held in MFMA accumulators
+        template <typename T>
+        __device__ __forceinline__ void putBlock(T (*o)[kLd], int r0, int c0, const typename BandMfma<T>::V& acc,
+                                                 bool neg)
+        {
+            constexpr int kLR = BandMfma<T>::kLaneRow, kIR = BandMfma<T>::kItemRow;
+            const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) o[c0 + li][r0 + kLR * lk + kIR * e] = neg ? -acc[e] : acc[e];
+        }
+
+        // inv = L^-1 of the lower-triangular tile L (zeros above the diagonal) by the whole workgroup, blockwise in
+        // 16 x 16 blocks: wave w inverts diagonal block w by substitution (lane j holds column j; once x_p is known
+        // it is removed from every later row, so the serial chain per row is one multiply and one fma); then the
+        // off-diagonal block of each 32 x 32 half, X = -D1^-1 (B D0^-1), and the lower-left 32 x 32 block,
+        // X = -C^-1 (B A^-1), are MFMA products (the recursive form of a triangular inverse). The blocks above the
+        // diagonal are zero.
         template <typename T>
         __device__ void invTile(T (*L)[kLd], T (*inv)[kLd], T* dinv)
         {
+            using V = typename BandMfma<T>::V;
+            const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+            const int li = lane & 15, lk = lane >> 4;
             if (threadIdx.x < 64) dinv[threadIdx.x] = bandRecip(L[threadIdx.x][threadIdx.x]);
-            for (int i0 = 0; i0 < kT; i0 += kPanel)
+            for (int idx = threadIdx.x; idx < kT * kT; idx += blockDim.x)
             {
-                {
-                    // thread t: row i = i0 + t % 16 of the panel, columns j = t / 16 + 16 q (q < 4)
-                    const int i = i0 + (threadIdx.x & (kPanel - 1)), j0 = threadIdx.x / kPanel;
-                    T s[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) s[q] = i == j0 + 16 * q ? T(1) : T(0);
-                    for (int k = 0; k < i0; k += 4)
-                    {
-                        T l[4];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) l[u] = L[k + u][i];
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) s[q] -= l[u] * inv[j0 + 16 * q][k + u];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) inv[j0 + 16 * q][i] = s[q];
-                }
-                __syncthreads();
-                if (threadIdx.x < 64)
-                {
-                    // right-looking substitution: once x_p is known it is removed from every later row, so the
-                    // serial chain per row is one multiply and one fma
-                    const int j = threadIdx.x;
-                    T x[kPanel];
-#pragma unroll
-                    for (int q = 0; q < kPanel; ++q) x[q] = inv[j][i0 + q];
-#pragma unroll
-                    for (int p = 0; p < kPanel; ++p)
-                    {
-                        x[p] *= dinv[i0 + p];
-#pragma unroll
-                        for (int q = p + 1; q < kPanel; ++q) x[q] = bandFma(-L[i0 + p][i0 + q], x[p], x[q]);
-                    }
-#pragma unroll
-                    for (int q = 0; q < kPanel; ++q) inv[j][i0 + q] = x[q];
-                }
-                __syncthreads();
+                const int r = idx & 63, c = idx >> 6;
+                if ((c >> 4) > (r >> 4)) inv[c][r] = T(0);
             }
+            __syncthreads();
+            {
+                const int b0 = 16 * w, j = li;
+                T x[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) x[q] = q == j ? T(1) : T(0);
+#pragma unroll
+                for (int p = 0; p < 16; ++p)
+                {
+                    x[p] *= dinv[b0 + p];
+#pragma unroll
+                    for (int q = p + 1; q < 16; ++q) x[q] = bandFma(-L[b0 + p][b0 + q], x[p], x[q]);
+                }
+                if (lk == 0)
+                {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) inv[b0 + j][b0 + q] = x[q];
+                }
+            }
+            __syncthreads();
+            V acc;
+            if (w < 2)
+            {
+                // wave w: rows r0 .. r0 + 15, columns c0 .. c0 + 15 of the 32 x 32 half w; the product B D0^-1
+                // is parked in the block it becomes and read back by the same wave
+                const int r0 = 32 * w + 16, c0 = 32 * w;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[e] = T(0);
+                mfmaBlock(acc, L, r0, c0, inv, c0, c0, 0, 16);
+                putBlock(inv, r0, c0, acc, false);
+                waveLdsFence();
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[e] = T(0);
+                mfmaBlock(acc, inv, r0, r0, inv, r0, c0, 0, 16);
+                waveLdsFence();
+                putBlock(inv, r0, c0, acc, true);
+            }
+            __syncthreads();
+            // wave w: block (bi, bj) of the lower-left 32 x 32 block; A^-1 is zero above its diagonal blocks and
+            // C^-1 likewise, which bounds the k ranges
+            const int bi = w >> 1, bj = w & 1;
+            const int r0 = 32 + 16 * bi, c0 = 16 * bj;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[e] = T(0);
+            mfmaBlock(acc, L, r0, 0, inv, 0, c0, 16 * bj, 32);
+            putBlock(inv, r0, c0, acc, false);
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[e] = T(0);
+            mfmaBlock(acc, inv, 32 + 16 * bi, 32, inv, 32, c0, 0, 16 * (bi + 1));
+            __syncthreads();
+            putBlock(inv, r0, c0, acc, true);
+            __syncthreads();
         }
 
         // O = (neg ? -1 : 1) A B, or O -= A B with sub, on 64 x 64 LDS tiles; A(r, k) = transA ? a[r][k] : a[k][r],
