@@ -290,99 +290,102 @@ namespace pbk
     }
 
     // ---------------------------------------------------------------------------------------------
-    // Solve helpers for one workgroup of 256 threads: matrix-vector products with a column-major matrix M
-    // (leading dimension ld, at most 128 rows) and a vector x in LDS.
-    // T-form: out_i = sum_k M(k, i) x_k (k <= i with tri, M upper triangular) for i < nOut: 16 lanes per output,
-    // each summing 8 consecutive k of column i (one 64-byte run), then a 16-lane butterfly; store(i, out_i) runs
-    // on the first lane of each output.
+    // Solve helpers for one workgroup of 256 threads and 16 outputs of a matrix-vector product with a
+    // column-major matrix M (leading dimension ld); a level's products are spread over sp / 16 workgroups per
+    // eliminated block, so each workgroup issues its loads together and waits about one memory latency.
+    // T-form: out_k = sum_i M(i, k) x_i for k = k0 .. k0 + 15 (k < nOut), i < K <= 128 (i <= k with tri, M upper
+    // triangular): 16 lanes per output, each summing 8 consecutive i of column k (one 64-byte run), then a 16-lane
+    // butterfly; store(k, out_k) runs on the first lane of each output.
     template <typename T, typename F>
-    __device__ __forceinline__ void blockGemvT(const T* M, int ld, int nOut, int K, bool tri, const T* x, F&& store)
+    __device__ __forceinline__ void gemvT16(const T* M, int ld, int k0, int nOut, int K, bool tri, const T* x,
+                                            F&& store)
     {
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        const int g = lane >> 4, h = lane & 15;
-        for (int i0 = 4 * w; i0 < nOut; i0 += 16)
-        {
-            const int i = i0 + g;
-            T s = T(0);
-            if (i < nOut)
-            {
-                const int kEnd = tri ? min(K, i + 1) : K;
-                const T* col = M + size_t(i) * ld + 8 * h;
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (8 * h + j < kEnd) s += col[j] * x[8 * h + j];
-            }
-#pragma unroll
-            for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
-            if (h == 0 && i < nOut) store(i, s);
-        }
-    }
-
-    // N-form: out_i = sum_k M(i, k) x_k (k >= i with tri) for i < nOut <= 128: two threads per output (halves of
-    // k, four running sums each), combined in a fixed order through part (LDS, 256 entries). Every thread calls
-    // it; the result is valid on threads i < nOut.
-    template <typename T>
-    __device__ __forceinline__ T blockGemvN(const T* M, int ld, int nOut, int K, bool tri, const T* x, T* part)
-    {
-        const int i = threadIdx.x & 127, half = threadIdx.x >> 7;
+        const int g = threadIdx.x >> 4, h = threadIdx.x & 15;
+        const int k = k0 + g;
         T s = T(0);
-        if (i < nOut)
+        if (k < nOut)
         {
-            const int kh = (K + 1) / 2;
-            int k = half ? kh : 0;
-            const int k1 = half ? K : kh;
-            if (tri) k = max(k, i);
-            T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
-            for (; k + 3 < k1; k += 4)
-            {
-                s0 += M[size_t(k) * ld + i] * x[k];
-                s1 += M[size_t(k + 1) * ld + i] * x[k + 1];
-                s2 += M[size_t(k + 2) * ld + i] * x[k + 2];
-                s3 += M[size_t(k + 3) * ld + i] * x[k + 3];
-            }
-            for (; k < k1; ++k) s0 += M[size_t(k) * ld + i] * x[k];
-            s = (s0 + s1) + (s2 + s3);
+            const int iEnd = tri ? min(K, k + 1) : K;
+            const T* col = M + size_t(k) * ld + 8 * h;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (8 * h + j < iEnd) s += col[j] * x[8 * h + j];
         }
-        part[threadIdx.x] = s;
-        __syncthreads();
-        const T r = threadIdx.x < 128 ? part[threadIdx.x] + part[threadIdx.x + 128] : T(0);
-        __syncthreads();
-        return r;
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
+        if (h == 0 && k < nOut) store(k, s);
     }
 
-    // Forward level: r_e = v_e - sum of its pending contributions, y_e = W_e^T r_e into v_e, then the
-    // contributions X_slot^T y_e to its two survivors and XB_e^T y_e to the border.
+    // N-form part: thread (r = t & 15, kq = t >> 4) sums M(i, k) x_k over k = kq, kq + 16, ..., kq + 16 (J - 1) in
+    // order (K = 16 J, J <= 16); the caller adds the 16 parts of row i in kq order. Triangular factors are stored
+    // with their zeros, so a triangular product is a full one.
     template <typename T>
-    __global__ void __launch_bounds__(256) k_band_forward(T* Z, BandDev B, const BandElim* elims)
+    __device__ __forceinline__ T rowPart(const T* M, int ld, int i, int J, const T* x)
     {
-        __shared__ T r[kBandMaxRows], y[kBandMaxRows];
-        const BandElim el = elims[blockIdx.x];
-        const int sp = B.sp, nbp = B.nbp;
+        const int kq = threadIdx.x >> 4;
+        T v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (j < J) v[j] = M[size_t(kq + 16 * j) * ld + i];
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (j < J) s += v[j] * x[kq + 16 * j];
+        return s;
+    }
+
+    // Forward level, first launch (sp / 16 workgroups per eliminated block): r_e = v_e minus its pending
+    // contributions, and 16 entries of y_e = W_e^T r_e into Y.
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_band_forward_y(T* Z, BandDev B, const BandElim* elims)
+    {
+        __shared__ T r[kBandMaxRows];
+        const int sp = B.sp, parts = sp / 16;
+        const BandElim el = elims[blockIdx.x / parts];
+        const int p = blockIdx.x % parts;
         const int i = threadIdx.x;
-        T* v = Z + B.offV + size_t(el.e) * sp;
         if (i < sp)
         {
-            T s = v[i];
-            for (int p = el.pendBegin; p < el.pendEnd; ++p) s -= Z[B.offPV + size_t(B.pend[p]) * sp + i];
+            T s = Z[B.offV + size_t(el.e) * sp + i];
+            for (int q = el.pendBegin; q < el.pendEnd; ++q) s -= Z[B.offPV + size_t(B.pend[q]) * sp + i];
             r[i] = s;
         }
         __syncthreads();
-        blockGemvT(Z + B.offD + size_t(el.e) * sp * sp, sp, sp, sp, true, r, [&](int k, T s) {
-            y[k] = s;
-            v[k] = s;
-        });
-        __syncthreads();
-        T* pvA = Z + B.offPV + size_t(el.slotA) * sp;
-        blockGemvT(Z + B.offX + size_t(el.slotA) * sp * sp, sp, sp, sp, false, y, [&](int k, T s) { pvA[k] = s; });
-        if (el.slotB >= 0)
+        T* y = Z + B.offY + size_t(el.e) * sp;
+        gemvT16(Z + B.offD + size_t(el.e) * sp * sp, sp, 16 * p, sp, sp, true, r, [&](int k, T s) { y[k] = s; });
+    }
+
+    // Forward level, second launch (2 sp / 16 + nbp / 16 workgroups per eliminated block): v_e = y_e, and 16
+    // entries of one contribution: X_a^T y_e or X_b^T y_e to a survivor's pending slot, or XB_e^T y_e to the
+    // border.
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_band_forward_c(T* Z, BandDev B, const BandElim* elims)
+    {
+        __shared__ T y[kBandMaxRows];
+        const int sp = B.sp, nbp = B.nbp, ps = sp / 16, parts = 2 * ps + nbp / 16;
+        const BandElim el = elims[blockIdx.x / parts];
+        const int q = blockIdx.x % parts;
+        if (q >= ps && q < 2 * ps && el.slotB < 0) return;  // the last block of a level has one survivor
+        const int i = threadIdx.x;
+        if (i < sp)
         {
-            T* pvB = Z + B.offPV + size_t(el.slotB) * sp;
-            blockGemvT(Z + B.offX + size_t(el.slotB) * sp * sp, sp, sp, sp, false, y, [&](int k, T s) { pvB[k] = s; });
+            const T v = Z[B.offY + size_t(el.e) * sp + i];
+            y[i] = v;
+            if (q == 0) Z[B.offV + size_t(el.e) * sp + i] = v;
         }
-        if (nbp > 0)
+        __syncthreads();
+        if (q < 2 * ps)
+        {
+            const int slot = q < ps ? el.slotA : el.slotB;
+            T* pv = Z + B.offPV + size_t(slot) * sp;
+            gemvT16(Z + B.offX + size_t(slot) * sp * sp, sp, 16 * (q % ps), sp, sp, false, y,
+                    [&](int k, T s) { pv[k] = s; });
+        }
+        else
         {
             T* pB = Z + B.offPB + size_t(el.e) * nbp;
-            blockGemvT(Z + B.offXB + size_t(el.e) * sp * nbp, sp, nbp, sp, false, y, [&](int k, T s) { pB[k] = s; });
+            gemvT16(Z + B.offXB + size_t(el.e) * sp * nbp, sp, 16 * (q - 2 * ps), nbp, sp, false, y,
+                    [&](int k, T s) { pB[k] = s; });
         }
     }
 
@@ -420,30 +423,62 @@ namespace pbk
         Z[B.offV + dst] = Z[B.offFV + i];
     }
 
-    // Backward level: x_e = W_e (y_e - X_a x_a - X_b x_b - XB_e x_border).
+    // Backward level, first launch (sp / 16 workgroups per eliminated block): 16 rows of
+    // t_e = ((y_e - X_a x_a) - X_b x_b) - XB_e x_border into Y (y_e is in v_e).
     template <typename T>
-    __global__ void __launch_bounds__(256) k_band_backward(T* Z, BandDev B, const BandElim* elims)
+    __global__ void __launch_bounds__(256) k_band_backward_t(T* Z, BandDev B, const BandElim* elims)
     {
-        __shared__ T xa[kBandMaxRows], xb[kBandMaxRows], xB[kBandMaxBorder], t[kBandMaxRows], part[256];
-        const BandElim el = elims[blockIdx.x];
-        const int sp = B.sp, nbp = B.nbp;
-        const int i = threadIdx.x;
-        T* V = Z + B.offV;
-        if (i < sp)
+        __shared__ T xa[kBandMaxRows], xb[kBandMaxRows], xB[kBandMaxBorder];
+        __shared__ T part[3][16][17];
+        const int sp = B.sp, nbp = B.nbp, parts = sp / 16;
+        const BandElim el = elims[blockIdx.x / parts];
+        const int p = blockIdx.x % parts;
+        const T* V = Z + B.offV;
+        for (int k = threadIdx.x; k < sp; k += blockDim.x)
         {
-            xa[i] = V[size_t(el.a) * sp + i];
-            xb[i] = el.b >= 0 ? V[size_t(el.b) * sp + i] : T(0);
+            xa[k] = V[size_t(el.a) * sp + k];
+            xb[k] = el.b >= 0 ? V[size_t(el.b) * sp + k] : T(0);
         }
-        if (i < nbp) xB[i] = V[size_t(B.m) * sp + i];
+        for (int k = threadIdx.x; k < nbp; k += blockDim.x) xB[k] = V[size_t(B.m) * sp + k];
         __syncthreads();
-        const T ya = blockGemvN(Z + B.offX + size_t(el.slotA) * sp * sp, sp, sp, sp, false, xa, part);
-        const T yb = el.slotB >= 0 ? blockGemvN(Z + B.offX + size_t(el.slotB) * sp * sp, sp, sp, sp, false, xb, part)
-                                   : T(0);
-        const T yB = nbp > 0 ? blockGemvN(Z + B.offXB + size_t(el.e) * sp * nbp, sp, sp, nbp, false, xB, part) : T(0);
-        if (i < sp) t[i] = ((V[size_t(el.e) * sp + i] - ya) - yb) - yB;
+        const int r = threadIdx.x & 15, kq = threadIdx.x >> 4, i = 16 * p + r;
+        part[0][kq][r] = rowPart(Z + B.offX + size_t(el.slotA) * sp * sp, sp, i, sp / 16, xa);
+        part[1][kq][r] = el.slotB >= 0 ? rowPart(Z + B.offX + size_t(el.slotB) * sp * sp, sp, i, sp / 16, xb) : T(0);
+        part[2][kq][r] = nbp > 0 ? rowPart(Z + B.offXB + size_t(el.e) * sp * nbp, sp, i, nbp / 16, xB) : T(0);
         __syncthreads();
-        const T xe = blockGemvN(Z + B.offD + size_t(el.e) * sp * sp, sp, sp, sp, true, t, part);
-        if (i < sp) V[size_t(el.e) * sp + i] = xe;
+        if (threadIdx.x < 16)
+        {
+            T ya = T(0), yb = T(0), yB = T(0);
+            for (int q = 0; q < 16; ++q)
+            {
+                ya += part[0][q][r];
+                yb += part[1][q][r];
+                yB += part[2][q][r];
+            }
+            Z[B.offY + size_t(el.e) * sp + i] = ((V[size_t(el.e) * sp + i] - ya) - yb) - yB;
+        }
+    }
+
+    // Backward level, second launch (sp / 16 workgroups per eliminated block): 16 rows of x_e = W_e t_e into v_e.
+    template <typename T>
+    __global__ void __launch_bounds__(256) k_band_backward_x(T* Z, BandDev B, const BandElim* elims)
+    {
+        __shared__ T t[kBandMaxRows];
+        __shared__ T part[16][17];
+        const int sp = B.sp, parts = sp / 16;
+        const BandElim el = elims[blockIdx.x / parts];
+        const int p = blockIdx.x % parts;
+        for (int k = threadIdx.x; k < sp; k += blockDim.x) t[k] = Z[B.offY + size_t(el.e) * sp + k];
+        __syncthreads();
+        const int r = threadIdx.x & 15, kq = threadIdx.x >> 4, i = 16 * p + r;
+        part[kq][r] = rowPart(Z + B.offD + size_t(el.e) * sp * sp, sp, i, sp / 16, t);
+        __syncthreads();
+        if (threadIdx.x < 16)
+        {
+            T x = T(0);
+            for (int q = 0; q < 16; ++q) x += part[q][r];
+            Z[B.offV + size_t(el.e) * sp + i] = x;
+        }
     }
 
     template <typename TA>
@@ -507,10 +542,12 @@ namespace pbk
     void BandLauncher<TA>::solve(const BandView& B, TA* Z, const DenseState& st, hipStream_t s)
     {
         const BandDev& D = B.d;
+        const int ps = D.sp / 16;
         for (int l = 0; l < B.levels; ++l)
         {
             const int e0 = B.elimStart[l], ne = B.elimStart[l + 1] - e0;
-            k_band_forward<TA><<<ne, 256, 0, s>>>(Z, D, D.elims + e0);
+            k_band_forward_y<TA><<<ne * ps, 256, 0, s>>>(Z, D, D.elims + e0);
+            k_band_forward_c<TA><<<ne * (2 * ps + D.nbp / 16), 256, 0, s>>>(Z, D, D.elims + e0);
         }
         k_band_final_rhs<TA><<<1, 256, 0, s>>>(Z, D);
         DenseLauncher<TA>::solve(Z + D.offF, D.nF / kT, Z + D.offFV, st, s);
@@ -518,7 +555,8 @@ namespace pbk
         for (int l = B.levels - 1; l >= 0; --l)
         {
             const int e0 = B.elimStart[l], ne = B.elimStart[l + 1] - e0;
-            k_band_backward<TA><<<ne, 256, 0, s>>>(Z, D, D.elims + e0);
+            k_band_backward_t<TA><<<ne * ps, 256, 0, s>>>(Z, D, D.elims + e0);
+            k_band_backward_x<TA><<<ne * ps, 256, 0, s>>>(Z, D, D.elims + e0);
         }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) throw std::runtime_error(std::string("band solve launch: ") + hipGetErrorString(e));

@@ -1161,6 +1161,10 @@ namespace
     constexpr int kFamilyCount = sizeof(kFamilies) / sizeof(kFamilies[0]);
     constexpr int kPinDense = 80;       // dense solver words (8 int32) and two norms
     static_assert(kPinPcgFlags + pbk::kIflagCount / 2 <= kPinDense, "pinned layout");
+    // a step enqueued with one host wait: the scalars after back-substitution, the first trial cost and the
+    // unit step length it is taken at (the source of an asynchronous upload)
+    constexpr int kPinStep = 96, kPinTrial = kPinStep + pbk::kScalarCount, kPinAlphaOne = kPinTrial + 1;
+    static_assert(kPinDense + 6 <= kPinStep && kPinAlphaOne < 512, "pinned layout");
     // rows up to which the adjoint factors its reduced system densely (34 GB of FP64; FP64 PCG above that)
     constexpr long kDenseAdjointMax = 65536;
     // default largest reduced system solved densely per LM step (reduced_solver = auto): above it the
@@ -1304,6 +1308,7 @@ struct pb_ctx
     std::vector<hipEvent_t> eventPool;
     size_t eventNext = 0;
     hipEvent_t trialEv[2] = {nullptr, nullptr};  // brackets the trial-state kernels of a line-search pass
+    hipEvent_t stepEv[7] = {};  // phase boundaries of a step enqueued with one host wait (fusedStep)
 };
 
 namespace
@@ -2141,8 +2146,10 @@ namespace
         bool invalid = false, singular = false, notSpd = false;
     };
 
+    // Enqueues the linearisation, the camera terms and the Schur assembly at damping, and the copies of the
+    // objective and the flags to the pinned area (read by buildRead after the stream is synchronised).
     template <typename T>
-    BuildResult build(pb_ctx* ctx, pbk::Work<T>& W, double damping)
+    void buildLaunch(pb_ctx* ctx, pbk::Work<T>& W, double damping)
     {
         using Lr = pbk::Launcher<T>;
         hipStream_t s = ctx->stream;
@@ -2171,7 +2178,11 @@ namespace
         check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s), "readback");
         check(hipMemcpyAsync(h + kPinFlags, W.flags, sizeof(int32_t) * pbk::kFlagCount, hipMemcpyDeviceToHost, s),
               "readback");
-        check(hipStreamSynchronize(s), "build");
+    }
+
+    BuildResult buildRead(const pb_ctx* ctx)
+    {
+        const double* h = ctx->pinned;
         const int32_t* f = reinterpret_cast<const int32_t*>(h + kPinFlags);
         BuildResult r;
         r.cost = h[pbk::kScalarCost];
@@ -2179,6 +2190,14 @@ namespace
         r.singular = f[pbk::kFlagSingular] != 0;
         r.notSpd = f[pbk::kFlagNotSpd] != 0;
         return r;
+    }
+
+    template <typename T>
+    BuildResult build(pb_ctx* ctx, pbk::Work<T>& W, double damping)
+    {
+        buildLaunch(ctx, W, damping);
+        check(hipStreamSynchronize(ctx->stream), "build");
+        return buildRead(ctx);
     }
 
     constexpr int kPcgGraphLaunches = 16;  // PCG launches per captured graph (even: the counter stays even)
@@ -2666,7 +2685,8 @@ namespace
         D.offFV = align(D.offV + D.vN);
         D.offPV = align(D.offFV + D.nF);
         D.offPB = align(D.offPV + (long long)D.nSlots * sp);
-        D.size = align(D.offPB + (long long)m * nbp);
+        D.offY = align(D.offPB + (long long)m * nbp);
+        D.size = align(D.offY + (long long)m * sp);
         // tasks and tiles
         auto addTask = [&](pbk::BandTask tk, int rowTiles, int colTiles, bool lower, long long outExtent,
                            std::vector<pbk::BandTile>& out) {
@@ -3063,6 +3083,84 @@ namespace
         return out;
     }
 
+    // The FP64 attempt without refinement (the route of an FP64 step), enqueued without a host wait: the
+    // factorisation, one solve of the reduced right-hand side, the residual and the copy of the solution to W.x,
+    // with the solver words and the two norms copied to the pinned area for denseRead. The solve runs whatever
+    // the factorisation found (a pivot that is not positive is replaced by 1, so it completes); a step whose
+    // factorisation failed is discarded by the caller. ev[0], ev[1] and ev[2] mark the ends of the factorisation,
+    // the solve and the residual check.
+    template <typename T>
+    void denseEnqueue(pb_ctx* ctx, pbk::Work<T>& W, const hipEvent_t* ev)
+    {
+        using DL = pbk::DenseLauncher<double>;
+        using BL = pbk::BandLauncher<double>;
+        const pbk::Problem& P = sysProblem(ctx);
+        hipStream_t s = ctx->stream;
+        const bool band = ctx->band.active;
+        const pbk::BandView& BV = ctx->band.view;
+        const int nt = denseTiles(P);
+        const int n = 6 * P.nUnits, npad = nt * pbk::kDenseTile;
+        ensureDense(ctx, nt, false, !band);
+        auto& D = ctx->dense;
+        double* A = D.A64;
+        double* v = D.v64;
+        double* Z = band ? bandArena<double>(ctx) : nullptr;
+        {
+            Span sp(ctx, band ? kFamilyBandFactor : kFamilyDenseFactor);
+            if (band)
+            {
+                BL::template gather<T>(P, W.S, BV, Z, s);
+                BL::factor(BV, Z, D.st, s);
+            }
+            else
+            {
+                DL::template gather<T>(P, W.S, A, nt, s);
+                DL::factor(A, nt, D.st, s);
+            }
+        }
+        DL::norm2(W.rhs, n, D.norm + 1, s);
+        check(hipEventRecord(ev[0], s), "event");
+        {
+            Span sp(ctx, band ? kFamilyBandSolve : kFamilyDenseSolve);
+            if (band)
+            {
+                BL::toBand(BV, W.rhs, Z, s);
+                BL::solve(BV, Z, D.st, s);
+                BL::fromBand(BV, D.x, Z, true, s);
+            }
+            else
+            {
+                DL::toDense(W.rhs, n, npad, v, s);
+                DL::solve(A, nt, v, D.st, s);
+                DL::accumulate(D.x, v, n, true, s);
+            }
+        }
+        check(hipEventRecord(ev[1], s), "event");
+        DL::template residual<T>(P, W.S, W.rhs, D.x, D.r, D.rowNorm, D.norm, s);
+        check(hipMemcpyAsync(ctx->pinned + kPinDense, D.st.words, sizeof(int32_t) * pbk::kDenseWordCount,
+                             hipMemcpyDeviceToHost, s),
+              "readback");
+        check(hipMemcpyAsync(ctx->pinned + kPinDense + 4, D.norm, sizeof(double) * 2, hipMemcpyDeviceToHost, s),
+              "readback");
+        DL::store<T>(D.x, n, W.x, s);
+        check(hipEventRecord(ev[2], s), "event");
+    }
+
+    // The outcome of denseEnqueue once the stream has been synchronised.
+    DenseOutcome denseRead(const pb_ctx* ctx, double relTol)
+    {
+        const int32_t* hw = reinterpret_cast<const int32_t*>(ctx->pinned + kPinDense);
+        const double* hn = ctx->pinned + kPinDense + 4;
+        DenseOutcome out;
+        if (hw[pbk::kDenseNotSpd]) return out;
+        if (hw[pbk::kDenseTimeout]) throw std::runtime_error("dense triangular solve stopped waiting for a tile");
+        out.factored = true;
+        const double bn = std::sqrt(hn[1]);
+        out.relResidual = bn > 0.0 ? std::sqrt(hn[0]) / bn : 0.0;
+        out.converged = out.relResidual <= relTol;
+        return out;
+    }
+
     // The dense route of one LM step: FP64 Cholesky (fp64 work or a rejected seed), or with fp32Seed an FP32
     // factorisation refined in FP64 to relTol, accepted only when that residual is reached (else the step is
     // solved again with the FP64 factorisation). The solution goes to W.x.
@@ -3194,6 +3292,91 @@ namespace
         return ctx->progressFn(&sm, ctx->progressUser) != 0;
     }
 
+    // One LM step of the FP64 dense or band route (no FP32 seed) enqueued with one host wait: the assembly at
+    // the damping, the reduced solve, back-substitution and the objective at the full step, each phase's results
+    // copied to its own slot of the pinned area. The controller reads them in the order of the separate phases
+    // and discards what follows a phase that failed; the kernels and their inputs are those of the separate
+    // phases, so the step's numbers are the same. Phase times come from events between the phases.
+    struct FusedStep
+    {
+        BuildResult build;
+        DenseOutcome dense;
+        double dd = 0.0, rmsG = 0.0, rmsP = 0.0, trialCost = 0.0;
+        double assembly = 0.0, factor = 0.0, solve = 0.0, residual = 0.0, back = 0.0, trialStates = 0.0,
+               objective = 0.0;
+    };
+
+    template <typename T>
+    FusedStep fusedStep(pb_ctx* ctx, pbk::Work<T>& W, double damping, double relTol)
+    {
+        using Lr = pbk::Launcher<T>;
+        const pbk::Problem& P = ctx->P;
+        hipStream_t s = ctx->stream;
+        hipEvent_t* ev = ctx->stepEv;
+        double* h = ctx->pinned;
+        check(hipEventRecord(ev[0], s), "event");
+        buildLaunch(ctx, W, damping);
+        check(hipEventRecord(ev[1], s), "event");
+        denseEnqueue(ctx, W, ev + 2);
+        {
+            Span sp(ctx, 5);
+            Lr::backSubstitute(P, W, s);
+        }
+        check(hipMemcpyAsync(h + kPinStep, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
+              "readback");
+        check(hipEventRecord(ev[5], s), "event");
+        h[kPinAlphaOne] = 1.0;
+        check(hipMemcpyAsync(W.alphas, h + kPinAlphaOne, sizeof(double), hipMemcpyHostToDevice, s), "alphas");
+        {
+            Span sp(ctx, 4);
+            check(hipEventRecord(ctx->trialEv[0], s), "event");
+            Lr::trialStates(P, W, W.alphas, 1, s);
+            check(hipEventRecord(ctx->trialEv[1], s), "event");
+            Lr::trialObjective(P, W, W.alphas, 1, ctx->costBlocks, s);
+        }
+        check(hipMemcpyAsync(h + kPinTrial, W.trialCost, sizeof(double), hipMemcpyDeviceToHost, s), "readback");
+        check(hipEventRecord(ev[6], s), "event");
+        check(hipStreamSynchronize(s), "step");
+        FusedStep f;
+        f.build = buildRead(ctx);
+        f.dense = denseRead(ctx, relTol);
+        f.dd = h[kPinStep + pbk::kScalarDd];
+        f.rmsG = h[kPinStep + pbk::kScalarRmsGlobal];
+        f.rmsP = h[kPinStep + pbk::kScalarRmsPoint];
+        f.trialCost = h[kPinTrial];
+        f.assembly = eventSeconds(ev[0], ev[1]);
+        f.factor = eventSeconds(ev[1], ev[2]);
+        f.solve = eventSeconds(ev[2], ev[3]);
+        f.residual = eventSeconds(ev[3], ev[4]);
+        f.back = eventSeconds(ev[4], ev[5]);
+        f.trialStates = eventSeconds(ctx->trialEv[0], ctx->trialEv[1]);
+        f.objective = eventSeconds(ev[5], ev[6]);
+        return f;
+    }
+
+    // The reduced-solve outcome and report counters of a fused step (those of reducedSolve on the dense route
+    // without a seed).
+    PcgResult fusedLinear(const FusedStep& f, double tol, pb_solve_report* rep)
+    {
+        PcgResult lin;
+        lin.iterations = f.dense.factored ? 1 : 0;
+        lin.breakdown = !f.dense.factored;
+        lin.converged = f.dense.factored && std::isfinite(f.dense.relResidual);
+        lin.rnorm = f.dense.relResidual;
+        rep->cholesky_factorization_seconds += f.factor;
+        if (f.dense.factored)
+        {
+            rep->triangular_solve_seconds += f.solve;
+            rep->residual_check_seconds += f.residual;
+        }
+        rep->linear_tolerance_minimum =
+            rep->linear_tolerance_minimum > 0.0 ? std::min(rep->linear_tolerance_minimum, tol) : tol;
+        rep->linear_tolerance_maximum = std::max(rep->linear_tolerance_maximum, tol);
+        rep->linear_solve_seconds += f.factor + f.solve + f.residual;
+        rep->linear_iterations += lin.iterations;
+        return lin;
+    }
+
     // The online-Schur controller (BundleAdjustPlaMatrix.cpp:162-631 with useReferenceOnlineSchur).
     template <typename T>
     void runSolve(pb_ctx* ctx, const pb_solve_options& o, pb_solve_report* rep, const volatile int32_t* cancel,
@@ -3231,6 +3414,8 @@ namespace
         const double modelEta = o.pcg_model_eta >= 0.0 ? o.pcg_model_eta : (mixed ? 0.1 : 0.0);
         const double rmsTol = o.step_rms_tolerance > 0.0 ? o.step_rms_tolerance : 1e-6;
         const double switchDecrease = o.mixed_switch_decrease == 0.0 ? kMixedSolveSwitchDecrease : o.mixed_switch_decrease;
+        // the dense route's residual tolerance (as in reducedSolve)
+        const double denseTol = o.pcg_relative_tolerance > 0.0 ? o.pcg_relative_tolerance : (fp32 ? 1e-6 : 2e-10);
         rep->linear_tolerance = relTol;
         ctx->lastScalarBytes = sizeof(T);
         rep->online_schur_used = 1;
@@ -3268,8 +3453,22 @@ namespace
                     break;
                 }
                 auto ta = Clock::now();
-                const BuildResult b = build(ctx, W, damping);
-                rep->assembly_seconds += seconds(ta);
+                // the FP64 direct route without a seed takes the whole step with one host wait
+                const bool fused = pg.dense && !(pg.seed && pg.seedRejections < kSeedRejectionsToStop) &&
+                                   P.nUnits > 0 && o.line_search_steps > 0;
+                FusedStep fs;
+                BuildResult b;
+                if (fused)
+                {
+                    fs = fusedStep(ctx, W, damping, denseTol);
+                    b = fs.build;
+                    rep->assembly_seconds += fs.assembly;
+                }
+                else
+                {
+                    b = build(ctx, W, damping);
+                    rep->assembly_seconds += seconds(ta);
+                }
                 ++rep->linearizations;
                 ++rep->objective_evaluations;
                 current = b.cost;
@@ -3284,7 +3483,9 @@ namespace
                     raise();
                     continue;
                 }
-                const PcgResult lin = reducedSolve(ctx, W, o, pg, rep, relTol, absTol, maxPcg, checkEvery, modelEta);
+                const PcgResult lin = fused ? fusedLinear(fs, denseTol, rep)
+                                            : reducedSolve(ctx, W, o, pg, rep, relTol, absTol, maxPcg, checkEvery,
+                                                           modelEta);
                 if (lin.breakdown)
                 {
                     ++iterations;
@@ -3299,18 +3500,27 @@ namespace
                     raise();
                     continue;
                 }
-                auto tb = Clock::now();
-                {
-                    Span sp(ctx, 5);
-                    Lr::backSubstitute(P, W, s);
-                }
                 double* h = ctx->pinned;
-                check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
-                      "readback");
-                check(hipStreamSynchronize(s), "back-substitution");
-                rep->back_substitution_seconds += seconds(tb);
-                const double dd = h[pbk::kScalarDd];
-                const double rmsG = h[pbk::kScalarRmsGlobal], rmsP = h[pbk::kScalarRmsPoint];
+                double dd = fs.dd, rmsG = fs.rmsG, rmsP = fs.rmsP;
+                if (fused)
+                {
+                    rep->back_substitution_seconds += fs.back;
+                }
+                else
+                {
+                    auto tb = Clock::now();
+                    {
+                        Span sp(ctx, 5);
+                        Lr::backSubstitute(P, W, s);
+                    }
+                    check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
+                          "readback");
+                    check(hipStreamSynchronize(s), "back-substitution");
+                    rep->back_substitution_seconds += seconds(tb);
+                    dd = h[pbk::kScalarDd];
+                    rmsG = h[pbk::kScalarRmsGlobal];
+                    rmsP = h[pbk::kScalarRmsPoint];
+                }
                 if (!std::isfinite(dd))
                 {
                     ++rejected;
@@ -3327,24 +3537,35 @@ namespace
                 while (ls < o.line_search_steps && !armijo)
                 {
                     const int n = std::min(firstPass ? 1 : batch, o.line_search_steps - ls);
-                    firstPass = false;
                     for (int k = 0; k < n; ++k) alphas[k] = alpha * std::ldexp(1.0, -k);
-                    check(hipMemcpyAsync(W.alphas, alphas, sizeof(double) * n, hipMemcpyHostToDevice, s), "alphas");
+                    const double* costs = h;
+                    if (fused && firstPass)
                     {
-                        Span sp(ctx, 4);
-                        check(hipEventRecord(ctx->trialEv[0], s), "event");
-                        Lr::trialStates(P, W, W.alphas, n, s);
-                        check(hipEventRecord(ctx->trialEv[1], s), "event");
-                        Lr::trialObjective(P, W, W.alphas, n, ctx->costBlocks, s);
+                        // the unit step's objective was evaluated with the step
+                        costs = &fs.trialCost;
+                        rep->trial_state_seconds += fs.trialStates;
+                        rep->objective_seconds += fs.objective;
                     }
-                    check(hipMemcpyAsync(h, W.trialCost, sizeof(double) * n, hipMemcpyDeviceToHost, s), "readback");
-                    check(hipStreamSynchronize(s), "trial cost");
-                    rep->trial_state_seconds += eventSeconds(ctx->trialEv[0], ctx->trialEv[1]);
+                    else
+                    {
+                        check(hipMemcpyAsync(W.alphas, alphas, sizeof(double) * n, hipMemcpyHostToDevice, s), "alphas");
+                        {
+                            Span sp(ctx, 4);
+                            check(hipEventRecord(ctx->trialEv[0], s), "event");
+                            Lr::trialStates(P, W, W.alphas, n, s);
+                            check(hipEventRecord(ctx->trialEv[1], s), "event");
+                            Lr::trialObjective(P, W, W.alphas, n, ctx->costBlocks, s);
+                        }
+                        check(hipMemcpyAsync(h, W.trialCost, sizeof(double) * n, hipMemcpyDeviceToHost, s), "readback");
+                        check(hipStreamSynchronize(s), "trial cost");
+                        rep->trial_state_seconds += eventSeconds(ctx->trialEv[0], ctx->trialEv[1]);
+                    }
+                    firstPass = false;
                     for (int k = 0; k < n; ++k)
                     {
                         ++rep->objective_evaluations;
                         ++ls;
-                        cand = h[k];
+                        cand = costs[k];
                         const double limit = current - o.armijo_coefficient * alphas[k] * dd;
                         if (std::isfinite(cand) && cand <= limit && cand < current)
                         {
@@ -4581,6 +4802,7 @@ pb_status pb_hip_create(int device, pb_ctx** out)
                           " (" + prop.gcnArchName + ")";
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (hipEvent_t& e : ctx->trialEv) check(hipEventCreate(&e), "hipEventCreate");
+        for (hipEvent_t& e : ctx->stepEv) check(hipEventCreate(&e), "hipEventCreate");
         void* p = nullptr;
         check(hipHostMalloc(&p, 4096, hipHostMallocDefault), "hipHostMalloc");
         ctx->pinned = static_cast<double*>(p);
@@ -4610,6 +4832,8 @@ pb_status pb_hip_destroy(pb_ctx* ctx)
     ctx->dquality.release();
     for (hipEvent_t e : ctx->eventPool) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->trialEv)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->stepEv)
         if (e) (void)hipEventDestroy(e);
     ctx->band.tables.release();
     ctx->band.arena32.release();

@@ -499,6 +499,8 @@ namespace pbk
         // solve vectors
         long long offD = 0, offK = 0, offB = 0, offC = 0, offX = 0, offXB = 0, offPc = 0, offF = 0;
         long long offV = 0, offFV = 0, offPV = 0, offPB = 0, size = 0;
+        long long offY = 0;  // [m sp] a super-block's y_e between the two launches of a forward level, its t_e in
+                             // the backward sweep
         int lastPendBegin = 0, lastPendEnd = 0;  // pending slots of super-block 0 (the last system)
         const int32_t* unitBlk = nullptr;  // [nUnits] super-block of a pose unit, -1 for a border unit
         const int32_t* unitOff = nullptr;  // [nUnits] first row in its super-block or in the border
