@@ -176,42 +176,43 @@ namespace pbk
     }
 
     // ---------------------------------------------------------------------------------------------
-    // One workgroup per 64 x 64 output tile of a task: out = beta out +- sum_terms P^T Q (MFMA 16x16x4, operands
-    // staged in LDS 32 rows of k at a time). Wave w forms the 32 x 32 quadrant (w >> 1, w & 1) as 2 x 2 blocks.
+    // Four workgroups per 64 x 64 output tile of a task, one per 32 x 32 quarter: out = beta out +- sum_terms
+    // P^T Q (MFMA 16x16x4, operands staged in LDS 32 rows of k at a time). Wave w forms the 16 x 16 block
+    // (w >> 1, w & 1) of the quarter. Quarters spread a level's products over four times as many CUs; every
+    // output element is accumulated over k in the same order as with whole tiles.
+    constexpr int kBandGemmQuarters = 4;
     template <typename T>
     __global__ void __launch_bounds__(256) k_band_gemm(T* Z, const BandTask* tasks, const BandTile* tiles, int sp)
     {
         using V = typename BandMfma<T>::V;
-        constexpr int kStage = 32;
-        __shared__ T sa[kStage][kT + 2], sb[kStage][kT + 2];  // [k][column of P / Q]
-        const BandTile tl = tiles[blockIdx.x];
+        constexpr int kStage = 32, kG = 32;
+        __shared__ T sa[kStage][kG + 2], sb[kStage][kG + 2];  // [k][column of P / Q]
+        const BandTile tl = tiles[blockIdx.x / kBandGemmQuarters];
+        const int quarter = blockIdx.x % kBandGemmQuarters;
         const BandTask tk = tasks[tl.task];
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        const int qi = (w >> 1) * 32, qj = (w & 1) * 32;
+        const int qi = (w >> 1) * 16, qj = (w & 1) * 16;
         const int li = lane & 15, lk = lane >> 4;
         constexpr int kLR = BandMfma<T>::kLaneRow, kIR = BandMfma<T>::kItemRow;
-        T* out = Z + tk.out + size_t(tl.tj) * kT * tk.ldo + size_t(tl.ti) * kT;
-        V acc[2][2];
+        const int r0 = tl.ti * kT + (quarter >> 1) * kG, c0 = tl.tj * kT + (quarter & 1) * kG;
+        T* out = Z + tk.out + size_t(c0) * tk.ldo + r0;
+        V acc;
+        {
+            const T* src = out + size_t(qj + li) * tk.ldo + qi + kLR * lk;
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-            {
-                const T* src = out + size_t(qj + 16 * bj + li) * tk.ldo + qi + 16 * bi + kLR * lk;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[bi][bj][e] = tk.beta ? src[kIR * e] : T(0);
-            }
-        const int kEnd = tk.kTri ? min(sp, (tl.ti + 1) * kT) : sp;
+            for (int e = 0; e < 4; ++e) acc[e] = tk.beta ? src[kIR * e] : T(0);
+        }
+        const int kEnd = tk.kTri ? min(sp, r0 + kG) : sp;
         const T sign = tk.alphaNeg ? T(-1) : T(1);
         // steps of kStage rows of k over every term; the next step's operands are loaded into registers while
         // the current one is multiplied
-        constexpr int kPer = kStage * kT / 256;
+        constexpr int kPer = kStage * kG / 256;
         const int perTerm = kEnd / kStage, steps = perTerm * tk.terms;
         T pa[kPer], pb[kPer];
         auto load = [&](int st) {
             const int term = st / perTerm, k0 = (st % perTerm) * kStage;
-            const T* Pm = Z + (term ? tk.p1 : tk.p0) + size_t(tl.ti) * kT * sp + k0;
-            const T* Qm = Z + (term ? tk.q1 : tk.q0) + size_t(tl.tj) * kT * sp + k0;
+            const T* Pm = Z + (term ? tk.p1 : tk.p0) + size_t(r0) * sp + k0;
+            const T* Qm = Z + (term ? tk.q1 : tk.q0) + size_t(c0) * sp + k0;
 #pragma unroll
             for (int q = 0; q < kPer; ++q)
             {
@@ -235,24 +236,11 @@ namespace pbk
             if (st + 1 < steps) load(st + 1);
 #pragma unroll
             for (int kk = 0; kk < kStage; kk += 4)
-            {
-                const T a0 = sign * sa[kk + lk][qi + li], a1 = sign * sa[kk + lk][qi + 16 + li];
-                const T bb0 = sb[kk + lk][qj + li], bb1 = sb[kk + lk][qj + 16 + li];
-                acc[0][0] = BandMfma<T>::op(a0, bb0, acc[0][0]);
-                acc[0][1] = BandMfma<T>::op(a0, bb1, acc[0][1]);
-                acc[1][0] = BandMfma<T>::op(a1, bb0, acc[1][0]);
-                acc[1][1] = BandMfma<T>::op(a1, bb1, acc[1][1]);
-            }
+                acc = BandMfma<T>::op(sign * sa[kk + lk][qi + li], sb[kk + lk][qj + li], acc);
         }
+        T* dst = out + size_t(qj + li) * tk.ldo + qi + kLR * lk;
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-            {
-                T* dst = out + size_t(qj + 16 * bj + li) * tk.ldo + qi + 16 * bi + kLR * lk;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) dst[kIR * e] = acc[bi][bj][e];
-            }
+        for (int e = 0; e < 4; ++e) dst[kIR * e] = acc[e];
     }
 
     // C -= sum over the level's eliminated blocks e of Pc_e, in elimination order (one thread per entry).
@@ -525,9 +513,9 @@ namespace pbk
             const int e0 = B.elimStart[l], ne = B.elimStart[l + 1] - e0;
             k_band_factor<TA><<<ne, 256, lds, s>>>(Z, D.elims + e0, D.offD, D.sp, st.words);
             const int x0 = B.tileXStart[l], nx = B.tileXStart[l + 1] - x0;
-            if (nx > 0) k_band_gemm<TA><<<nx, 256, 0, s>>>(Z, D.tasks, D.tiles + x0, D.sp);
+            if (nx > 0) k_band_gemm<TA><<<nx * kBandGemmQuarters, 256, 0, s>>>(Z, D.tasks, D.tiles + x0, D.sp);
             const int u0 = B.tileUStart[l], nu = B.tileUStart[l + 1] - u0;
-            if (nu > 0) k_band_gemm<TA><<<nu, 256, 0, s>>>(Z, D.tasks, D.tiles + u0, D.sp);
+            if (nu > 0) k_band_gemm<TA><<<nu * kBandGemmQuarters, 256, 0, s>>>(Z, D.tasks, D.tiles + u0, D.sp);
             if (D.nbp > 0)
                 k_band_csum<TA><<<blocksOf((long)D.nbp * D.nbp, 256), 256, 0, s>>>(Z, D.elims + e0, ne, D.offC,
                                                                                       D.offPc, D.nbp);
