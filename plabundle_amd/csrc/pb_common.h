@@ -22,6 +22,69 @@ namespace pbk
         return v;
     }
 
+    // Wave sums of N values held by every lane, by recursive halving: at the stage with partner lane l ^ O
+    // (O = 32, 16, ..., 1) each lane keeps one half of its remaining values (the upper half on lanes with bit O
+    // set; zero-padded to ceil(n / 2)) and adds the partner's copy of that half. After six stages lane l holds
+    // halvingCount<N>() values, value j being the sum of value halvingSlot<N>(l, j) (padding where that is -1).
+    // About N exchanges instead of the 6 N of N butterflies; the order of the additions is fixed, so the sums
+    // are reproducible.
+    template <int N>
+    constexpr int halvingCount()
+    {
+        int n = N;
+        for (int s = 0; s < 6; ++s) n = (n + 1) / 2;
+        return n;
+    }
+
+    // (the values are exchanged in their own type VI and added in VO)
+    template <int N, int O, typename VO, typename VI>
+    __device__ __forceinline__ void halvingSum(const VI* v, int lane, VO* out)
+    {
+        constexpr int H = (N + 1) / 2;
+        VO a[H];
+        const bool up = (lane & O) != 0;
+#pragma unroll
+        for (int k = 0; k < H; ++k)
+        {
+            const VI lo = v[k], hi = k + H < N ? v[k + H] : VI(0);
+            a[k] = VO(up ? hi : lo) + VO(__shfl_xor(up ? lo : hi, O, kWave));
+        }
+        if constexpr (O == 1)
+        {
+#pragma unroll
+            for (int k = 0; k < H; ++k) out[k] = a[k];
+        }
+        else
+        {
+            halvingSum<H, O / 2, VO, VO>(a, lane, out);
+        }
+    }
+
+    template <int N>
+    __device__ __forceinline__ int halvingSlot(int lane, int j)
+    {
+        int ns[6], offs[6];
+        int n = N;
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+        {
+            const int h = (n + 1) / 2;
+            ns[s] = n;
+            offs[s] = (lane & (32 >> s)) ? h : 0;
+            n = h;
+        }
+        // value j after the last stage is value j + sum(offs[s..5]) of stage s, which exists when below ns[s]
+        int slot = j;
+        bool held = true;
+#pragma unroll
+        for (int s = 5; s >= 0; --s)
+        {
+            slot += offs[s];
+            held = held && slot < ns[s];
+        }
+        return held ? slot : -1;
+    }
+
     // Orders LDS accesses of one wave (lanes exchange values through LDS without a workgroup barrier).
     __device__ __forceinline__ void waveLdsFence()
     {

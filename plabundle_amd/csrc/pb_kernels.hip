@@ -589,25 +589,36 @@ namespace pbk
                 }
             }
         }
-        // deterministic block reduction: wave butterflies, then the four wave results in order
+        // deterministic block reduction: wave sums by recursive halving, then the four wave results in order
         __shared__ double lds[4][Acc::kCount + 6 + KM + 1];
         const int lane = threadIdx.x & 63;
         const int wv = threadIdx.x >> 6;
+        {
+            constexpr int kR = halvingCount<Acc::kCount>();
+            double hs[kR];
+            halvingSum<Acc::kCount, 32, double, T>(A.h, lane, hs);
 #pragma unroll
-        for (int i = 0; i < Acc::kCount; ++i)
-        {
-            const double v = waveSum(double(A.h[i]));
-            if (lane == 0) lds[wv][i] = v;
+            for (int j = 0; j < kR; ++j)
+            {
+                const int slot = halvingSlot<Acc::kCount>(lane, j);
+                if (slot >= 0) lds[wv][slot] = hs[j];
+            }
         }
+        {
+            constexpr int kB = 6 + KM + 1;
+            double bc[kB];
 #pragma unroll
-        for (int i = 0; i < 6 + KM; ++i)
-        {
-            const double v = waveSum(A.b[i]);
-            if (lane == 0) lds[wv][Acc::kCount + i] = v;
-        }
-        {
-            const double v = waveSum(A.cost);
-            if (lane == 0) lds[wv][Acc::kCount + 6 + KM] = v;
+            for (int i = 0; i < 6 + KM; ++i) bc[i] = A.b[i];
+            bc[6 + KM] = A.cost;
+            constexpr int kR = halvingCount<kB>();
+            double bs[kR];
+            halvingSum<kB, 32, double, double>(bc, lane, bs);
+#pragma unroll
+            for (int j = 0; j < kR; ++j)
+            {
+                const int slot = halvingSlot<kB>(lane, j);
+                if (slot >= 0) lds[wv][Acc::kCount + slot] = bs[j];
+            }
         }
         __syncthreads();
         constexpr int kTotal = Acc::kCount + 6 + KM + 1;
@@ -783,23 +794,14 @@ namespace pbk
                     accr[a] += double(u1[a * 3]) * z0 + double(u1[a * 3 + 1]) * z1 + double(u1[a * 3 + 2]) * z2;
             }
         }
-#pragma unroll
-        for (int i = 0; i < 36; ++i) acc[i] = waveSum(acc[i]);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) accr[i] = waveSum(accr[i]);
-        T* out = Wk.itemPartial + item * 36;
-        double* outr = Wk.itemPartialRhs + item * 6;
-        // every lane holds every sum; lanes write disjoint elements
-        T mine = T(0);
-#pragma unroll
-        for (int i = 0; i < 36; ++i)
-            if (lane == i) mine = acc[i];
-        if (lane < 36) out[lane] = mine;
-        double mr = 0.0;
-#pragma unroll
-        for (int i = 0; i < 6; ++i)
-            if (lane == 36 + i) mr = accr[i];
-        if (lane >= 36 && lane < 42) outr[lane - 36] = mr;
+        // recursive halving: each of 36 lanes (6 for the right-hand side) ends with one finished sum
+        T mine;
+        double mr;
+        halvingSum<36, 32, T, T>(acc, lane, &mine);
+        halvingSum<6, 32, double, double>(accr, lane, &mr);
+        const int slot = halvingSlot<36>(lane, 0), slotR = halvingSlot<6>(lane, 0);
+        if (slot >= 0) Wk.itemPartial[item * 36 + slot] = mine;
+        if (slotR >= 0) Wk.itemPartialRhs[item * 6 + slotR] = mr;
     }
 
     __device__ __forceinline__ bool rigMode(const Problem& P) { return P.nCap + P.nSen > 0; }
