@@ -657,16 +657,17 @@ namespace pbk
         __shared__ double lds[NA][4];
         __shared__ int lbad[NA][4];
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        {
+            double s;
+            halvingSum<NA, 32, double, double>(acc, lane, &s);
+            const int a = halvingSlot<NA>(lane, 0);
+            if (a >= 0) lds[a][wv] = s;
+        }
 #pragma unroll
         for (int a = 0; a < NA; ++a)
         {
-            const double s = waveSum(acc[a]);
             const int anyBad = __any(bad[a]) ? 1 : 0;
-            if (lane == 0)
-            {
-                lds[a][wv] = s;
-                lbad[a][wv] = anyBad;
-            }
+            if (lane == 0) lbad[a][wv] = anyBad;
         }
         __syncthreads();
         if (threadIdx.x < NA)

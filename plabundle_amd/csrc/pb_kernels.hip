@@ -822,8 +822,8 @@ namespace pbk
         const int it0 = P.blockItemOffset[blk], it1 = P.blockItemOffset[blk + 1];
         if (it1 - it0 > kFinalizeSerialItems)
         {
-            // many items: lane l sums items l, l + 64, ..., then a butterfly over the lanes; lane i keeps element i
-            // (i < 36) or right-hand side entry i - 36 (36 <= i < 42)
+            // many items: lane l sums items l, l + 64, ..., then a recursive-halving sum over the lanes; the lane
+            // holding value i keeps element i (i < 36) or right-hand side entry i - 36 (36 <= i < 42)
             double acc[42];
 #pragma unroll
             for (int i = 0; i < 42; ++i) acc[i] = 0.0;
@@ -835,22 +835,18 @@ namespace pbk
 #pragma unroll
                     for (int a = 0; a < 6; ++a) acc[36 + a] += Wk.itemPartialRhs[it * 6 + a];
             }
-            double mine = 0.0;
-#pragma unroll
-            for (int i = 0; i < 42; ++i)
+            double mine;
+            halvingSum<42, 32, double, double>(acc, lane, &mine);
+            const int i = halvingSlot<42>(lane, 0);
+            if (i >= 0 && i < 36)
             {
-                const double t = waveSum(acc[i]);
-                if (lane == i) mine = t;
+                double s = Wk.D[blk * 36 + i] + mine;
+                if (diag && (i / 6) == (i % 6) && !rigMode(P)) s = s == 0.0 ? 1.0 : s * (1.0 + damping);
+                Wk.S[P.blockPosUp[blk] * 36 + i] = T(s);
+                if (!diag) Wk.S[P.blockPosLo[blk] * 36 + (i % 6) * 6 + i / 6] = T(s);
             }
-            if (lane < 36)
-            {
-                double s = Wk.D[blk * 36 + lane] + mine;
-                if (diag && (lane / 6) == (lane % 6) && !rigMode(P)) s = s == 0.0 ? 1.0 : s * (1.0 + damping);
-                Wk.S[P.blockPosUp[blk] * 36 + lane] = T(s);
-                if (!diag) Wk.S[P.blockPosLo[blk] * 36 + (lane % 6) * 6 + lane / 6] = T(s);
-            }
-            else if (diag && lane < 42)
-                Wk.rhs[u * 6 + lane - 36] = Wk.directRhs[u * 6 + lane - 36] - mine;
+            else if (diag && i >= 36)
+                Wk.rhs[u * 6 + i - 36] = Wk.directRhs[u * 6 + i - 36] - mine;
             return;
         }
         if (lane < 36)
@@ -1549,16 +1545,17 @@ namespace pbk
         __shared__ double lds[NA][4];
         __shared__ int lbad[NA][4];
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        {
+            double s;
+            halvingSum<NA, 32, double, double>(acc, lane, &s);
+            const int a = halvingSlot<NA>(lane, 0);
+            if (a >= 0) lds[a][wv] = s;
+        }
 #pragma unroll
         for (int a = 0; a < NA; ++a)
         {
-            const double s = waveSum(acc[a]);
             const int anyBad = __any(bad[a]) ? 1 : 0;
-            if (lane == 0)
-            {
-                lds[a][wv] = s;
-                lbad[a][wv] = anyBad;
-            }
+            if (lane == 0) lbad[a][wv] = anyBad;
         }
         __syncthreads();
         if (threadIdx.x < NA && int(threadIdx.x) < nAlpha)
