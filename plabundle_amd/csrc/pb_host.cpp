@@ -1161,10 +1161,9 @@ namespace
     constexpr int kFamilyCount = sizeof(kFamilies) / sizeof(kFamilies[0]);
     constexpr int kPinDense = 80;       // dense solver words (8 int32) and two norms
     static_assert(kPinPcgFlags + pbk::kIflagCount / 2 <= kPinDense, "pinned layout");
-    // a step enqueued with one host wait: the scalars after back-substitution, the first trial cost and the
-    // unit step length it is taken at (the source of an asynchronous upload)
-    constexpr int kPinStep = 96, kPinTrial = kPinStep + pbk::kScalarCount, kPinAlphaOne = kPinTrial + 1;
-    static_assert(kPinDense + 6 <= kPinStep && kPinAlphaOne < 512, "pinned layout");
+    // a step enqueued with one host wait: the scalars after back-substitution and the first trial cost
+    constexpr int kPinStep = 96, kPinTrial = kPinStep + pbk::kScalarCount;
+    static_assert(kPinDense + 6 <= kPinStep && kPinTrial < 512, "pinned layout");
     // rows up to which the adjoint factors its reduced system densely (34 GB of FP64; FP64 PCG above that)
     constexpr long kDenseAdjointMax = 65536;
     // default largest reduced system solved densely per LM step (reduced_solver = auto): above it the
@@ -2146,17 +2145,16 @@ namespace
         bool invalid = false, singular = false, notSpd = false;
     };
 
-    // Enqueues the linearisation, the camera terms and the Schur assembly at damping, and the copies of the
+    // Enqueues the linearisation, the camera terms and the Schur assembly at damping, and (unless fused, where
+    // the step publishes everything at its end and the unit step length is set here) the copies of the
     // objective and the flags to the pinned area (read by buildRead after the stream is synchronised).
     template <typename T>
-    void buildLaunch(pb_ctx* ctx, pbk::Work<T>& W, double damping)
+    void buildLaunch(pb_ctx* ctx, pbk::Work<T>& W, double damping, bool fused = false)
     {
         using Lr = pbk::Launcher<T>;
         hipStream_t s = ctx->stream;
         const pbk::Problem& P = ctx->P;
-        check(hipMemsetAsync(W.flags, 0, sizeof(int32_t) * pbk::kFlagCount, s), "memset");
-        check(hipMemsetAsync(W.D, 0, sizeof(double) * size_t(std::max(1, P.nBlocks)) * 36, s), "memset");
-        check(hipMemsetAsync(W.directRhs, 0, sizeof(double) * size_t(std::max(1, P.nUnits)) * 6, s), "memset");
+        Lr::clearBuild(P, W, fused, s);
         if (ctx->rigSys) Lr::rigMaps(P, s);
         Lr::prepareCameras(P, s);
         {
@@ -2174,6 +2172,7 @@ namespace
             if (ctx->cluster.active)
                 pbk::ClusterLauncher<T>::factor(sysProblem(ctx), sysWork<T>(ctx).S, ctx->cluster.dev, W.flags, s);
         }
+        if (fused) return;
         double* h = ctx->pinned;
         check(hipMemcpyAsync(h, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s), "readback");
         check(hipMemcpyAsync(h + kPinFlags, W.flags, sizeof(int32_t) * pbk::kFlagCount, hipMemcpyDeviceToHost, s),
@@ -3084,8 +3083,8 @@ namespace
     }
 
     // The FP64 attempt without refinement (the route of an FP64 step), enqueued without a host wait: the
-    // factorisation, one solve of the reduced right-hand side, the residual and the copy of the solution to W.x,
-    // with the solver words and the two norms copied to the pinned area for denseRead. The solve runs whatever
+    // factorisation, one solve of the reduced right-hand side, the residual and the copy of the solution to W.x;
+    // the caller publishes the solver words and the two norms to the pinned area for denseRead. The solve runs whatever
     // the factorisation found (a pivot that is not positive is replaced by 1, so it completes); a step whose
     // factorisation failed is discarded by the caller. ev[0], ev[1] and ev[2] mark the ends of the factorisation,
     // the solve and the residual check.
@@ -3137,11 +3136,6 @@ namespace
         }
         check(hipEventRecord(ev[1], s), "event");
         DL::template residual<T>(P, W.S, W.rhs, D.x, D.r, D.rowNorm, D.norm, s);
-        check(hipMemcpyAsync(ctx->pinned + kPinDense, D.st.words, sizeof(int32_t) * pbk::kDenseWordCount,
-                             hipMemcpyDeviceToHost, s),
-              "readback");
-        check(hipMemcpyAsync(ctx->pinned + kPinDense + 4, D.norm, sizeof(double) * 2, hipMemcpyDeviceToHost, s),
-              "readback");
         DL::store<T>(D.x, n, W.x, s);
         check(hipEventRecord(ev[2], s), "event");
     }
@@ -3315,18 +3309,14 @@ namespace
         hipEvent_t* ev = ctx->stepEv;
         double* h = ctx->pinned;
         check(hipEventRecord(ev[0], s), "event");
-        buildLaunch(ctx, W, damping);
+        buildLaunch(ctx, W, damping, true);
         check(hipEventRecord(ev[1], s), "event");
         denseEnqueue(ctx, W, ev + 2);
         {
             Span sp(ctx, 5);
             Lr::backSubstitute(P, W, s);
         }
-        check(hipMemcpyAsync(h + kPinStep, W.scalars, sizeof(double) * pbk::kScalarCount, hipMemcpyDeviceToHost, s),
-              "readback");
         check(hipEventRecord(ev[5], s), "event");
-        h[kPinAlphaOne] = 1.0;
-        check(hipMemcpyAsync(W.alphas, h + kPinAlphaOne, sizeof(double), hipMemcpyHostToDevice, s), "alphas");
         {
             Span sp(ctx, 4);
             check(hipEventRecord(ctx->trialEv[0], s), "event");
@@ -3334,7 +3324,9 @@ namespace
             check(hipEventRecord(ctx->trialEv[1], s), "event");
             Lr::trialObjective(P, W, W.alphas, 1, ctx->costBlocks, s);
         }
-        check(hipMemcpyAsync(h + kPinTrial, W.trialCost, sizeof(double), hipMemcpyDeviceToHost, s), "readback");
+        // the build's objective and flags, the solver words and norms, the step's scalars and the unit step's
+        // objective, straight to the pinned area (no kernel after the build writes the objective or the flags)
+        Lr::publishStep(W, ctx->dense.st.words, ctx->dense.norm, h, kPinFlags, kPinDense, kPinStep, kPinTrial, s);
         check(hipEventRecord(ev[6], s), "event");
         check(hipStreamSynchronize(s), "step");
         FusedStep f;
@@ -4804,7 +4796,7 @@ pb_status pb_hip_create(int device, pb_ctx** out)
         for (hipEvent_t& e : ctx->trialEv) check(hipEventCreate(&e), "hipEventCreate");
         for (hipEvent_t& e : ctx->stepEv) check(hipEventCreate(&e), "hipEventCreate");
         void* p = nullptr;
-        check(hipHostMalloc(&p, 4096, hipHostMallocDefault), "hipHostMalloc");
+        check(hipHostMalloc(&p, 4096, hipHostMallocCoherent), "hipHostMalloc");  // kernels publish step results here
         ctx->pinned = static_cast<double*>(p);
         return PB_OK;
     });

@@ -408,6 +408,14 @@ namespace pbk
         static void trialObjective(const Problem& P, const Work<T>& W, const double* alphas, int nAlpha,
                                    int costBlocks, hipStream_t s);
         static void accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s);
+        // zeroes the flags, the direct blocks and the direct right-hand side before an assembly (in one launch);
+        // with unitAlpha also sets alphas[0] = 1
+        static void clearBuild(const Problem& P, const Work<T>& W, bool unitAlpha, hipStream_t s);
+        // the host-visible results of a step enqueued with one host wait, written to host memory hostPin
+        // (pinned, device-accessible): the scalars at pin[0] and pin[step], the flags at pin[flags] (int32), the
+        // dense solver words (int32) and two norms at pin[dense] and pin[dense + 4], trialCost[0] at pin[trial]
+        static void publishStep(const Work<T>& W, const int32_t* denseWords, const double* denseNorm, double* hostPin,
+                                int flags, int dense, int step, int trial, hipStream_t s);
         // camera rig: chain maps of the current state; the rig system from the camera-level one (damped diagonal,
         // exact zeros -> 1) and its right-hand side; the camera-level step from the rig step; the step RMS over the
         // rig units (scalars[kScalarRmsGlobal], after backSubstitute)

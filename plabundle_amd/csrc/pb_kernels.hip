@@ -2183,6 +2183,48 @@ namespace pbk
     }
 
     template <typename T>
+    __global__ void k_clear_build(Work<T> Wk, int nD, int nRhs, int unitAlpha)
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < kFlagCount) Wk.flags[i] = 0;
+        if (i < nD) Wk.D[i] = 0.0;
+        if (i < nRhs) Wk.directRhs[i] = 0.0;
+        if (i == 0 && unitAlpha) Wk.alphas[0] = 1.0;
+    }
+
+    template <typename T>
+    void Launcher<T>::clearBuild(const Problem& P, const Work<T>& W, bool unitAlpha, hipStream_t s)
+    {
+        const int nD = std::max(1, P.nBlocks) * 36, nRhs = std::max(1, P.nUnits) * 6;
+        const int n = std::max(std::max(nD, nRhs), int(kFlagCount));
+        k_clear_build<T><<<blocksFor(n, 256), 256, 0, s>>>(W, nD, nRhs, unitAlpha ? 1 : 0);
+    }
+
+    template <typename T>
+    __global__ void k_publish_step(Work<T> Wk, const int32_t* words, const double* norm, double* pin, int flags,
+                                   int dense, int step, int trial)
+    {
+        const int i = threadIdx.x;
+        if (i < kScalarCount)
+        {
+            pin[i] = Wk.scalars[i];
+            pin[step + i] = Wk.scalars[i];
+        }
+        if (i < kFlagCount) reinterpret_cast<int32_t*>(pin + flags)[i] = Wk.flags[i];
+        if (i < kDenseWordCount) reinterpret_cast<int32_t*>(pin + dense)[i] = words[i];
+        if (i < 2) pin[dense + 4 + i] = norm[i];
+        if (i == 0) pin[trial] = Wk.trialCost[0];
+    }
+
+    template <typename T>
+    void Launcher<T>::publishStep(const Work<T>& W, const int32_t* denseWords, const double* denseNorm,
+                                  double* hostPin, int flags, int dense, int step, int trial, hipStream_t s)
+    {
+        static_assert(kScalarCount <= 64 && kFlagCount <= 64 && kDenseWordCount <= 64, "one wave publishes");
+        k_publish_step<T><<<1, 64, 0, s>>>(W, denseWords, denseNorm, hostPin, flags, dense, step, trial);
+    }
+
+    template <typename T>
     void Launcher<T>::accept(const Problem& P, const Work<T>& W, int ai, double alpha, hipStream_t s)
     {
         const long n = std::max<long>(std::max<long>(std::max<long>(P.nCam, long(P.nGroup) * 13), P.nElim),
