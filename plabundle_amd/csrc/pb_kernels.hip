@@ -770,15 +770,31 @@ namespace pbk
         double accr[6] = {0, 0, 0, 0, 0, 0};
         // the cross blocks hold U = W L (Hpp^-1 = L L^T), so each pair contributes -U1 U2^T and the diagonal blocks
         // the right-hand side correction U1 z (z = L^T bp)
-        for (int k = P.itemBegin[item] + lane; k < P.itemBegin[item + 1]; k += kWave)
+        // the next round's pair indices are loaded while this round's cross blocks are multiplied
+        const int k1 = P.itemBegin[item + 1];
+        int k = P.itemBegin[item] + lane;
+        int e1n = k < k1 ? P.pairE1[k] : 0, e2n = k < k1 ? P.pairE2[k] : 0;
+        for (; k < k1; k += kWave)
         {
-            const int e1 = P.pairE1[k], e2 = P.pairE2[k];
+            const int e1 = e1n, e2 = e2n;
+            if (k + kWave < k1)
+            {
+                e1n = P.pairE1[k + kWave];
+                e2n = P.pairE2[k + kWave];
+            }
             T u1[18], u2[18];
 #pragma unroll
-            for (int i = 0; i < 18; ++i)
+            for (int i = 0; i < 18; ++i) u1[i] = Wk.W[e1 * 18 + i];
+            // a diagonal block's pairs are mostly an observation with itself: its cross block is read once
+            if (e2 != e1)
             {
-                u1[i] = Wk.W[e1 * 18 + i];
-                u2[i] = Wk.W[e2 * 18 + i];
+#pragma unroll
+                for (int i = 0; i < 18; ++i) u2[i] = Wk.W[e2 * 18 + i];
+            }
+            else
+            {
+#pragma unroll
+                for (int i = 0; i < 18; ++i) u2[i] = u1[i];
             }
 #pragma unroll
             for (int a = 0; a < 6; ++a)
