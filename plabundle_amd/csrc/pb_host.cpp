@@ -4927,7 +4927,11 @@ pb_status pb_hip_solve(pb_ctx* ctx, const pb_solve_options* options, pb_solve_re
         const Route route = reducedRoute(ctx, *options);
         pg.dense = route != kRoutePcg;
         ctx->band.active = route == kRouteBand;
-        pg.seed = pg.dense && options->precision != PB_PRECISION_FP64;
+        // the FP32-seed mode always seeds; the mixed mode factors its FP32 system in FP32 (refined to 1e-6) on the
+        // dense route, where that halves the MFMA time, and in FP64 on the band route, whose FP32 chain is only a
+        // quarter shorter and would add refinement solves and host waits to every step
+        pg.seed = pg.dense && (options->precision == PB_PRECISION_FP32_SEED ||
+                               (options->precision == PB_PRECISION_MIXED && !ctx->band.active));
         if (pg.dense) ensureDense(ctx, denseTiles(ctx->P), pg.seed, !ctx->band.active);
         if (ctx->band.active)
         {

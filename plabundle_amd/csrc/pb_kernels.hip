@@ -537,10 +537,14 @@ namespace pbk
         const Camera& cam = P.cams[c];
         Linearization L;
         // the camera's observations in camera-major copies (point, pixel, weight read contiguously); only the
-        // points are gathered
-        for (int k = P.camObsOffset[c] + threadIdx.x; k < P.camObsOffset[c + 1]; k += blockDim.x)
+        // points are gathered. The next observation's point index is loaded while this one is linearised.
+        const int kEnd = P.camObsOffset[c + 1];
+        int k = P.camObsOffset[c] + threadIdx.x;
+        int tNext = k < kEnd ? P.camObsPoint[k] : 0;
+        for (; k < kEnd; k += blockDim.x)
         {
-            const int t = P.camObsPoint[k];
+            const int t = tNext;
+            if (k + int(blockDim.x) < kEnd) tNext = P.camObsPoint[k + blockDim.x];
             const double ow = P.camObsW[k];
             if (!pbdev::linearize<KM>(cam, P.pointX[3 * t], P.pointX[3 * t + 1], P.pointX[3 * t + 2], P.camObsUV[2 * k],
                                   P.camObsUV[2 * k + 1], ow, P.loss, P.lossScale, KM > 0 ? K : 0, act, L,
