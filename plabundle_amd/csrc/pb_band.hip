@@ -250,7 +250,17 @@ namespace pbk
         const int i = blockIdx.x * blockDim.x + threadIdx.x;
         if (i >= nbp * nbp) return;
         T s = T(0);
-        for (int q = 0; q < count; ++q) s += Z[offPc + size_t(elims[q].e) * nbp * nbp + i];
+        // eight loads in flight per step, summed in elimination order
+        int q = 0;
+        for (; q + 8 <= count; q += 8)
+        {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = Z[offPc + size_t(elims[q + u].e) * nbp * nbp + i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; q < count; ++q) s += Z[offPc + size_t(elims[q].e) * nbp * nbp + i];
         Z[offC + i] -= s;
     }
 
@@ -395,7 +405,16 @@ namespace pbk
             {
                 const int j = i - sp;
                 s = Z[B.offV + size_t(B.m) * sp + j];
-                for (int e = 1; e < B.m; ++e) s -= Z[B.offPB + size_t(e) * nbp + j];
+                int e = 1;
+                for (; e + 8 <= B.m; e += 8)
+                {
+                    T v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = Z[B.offPB + size_t(e + u) * nbp + j];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) s -= v[u];
+                }
+                for (; e < B.m; ++e) s -= Z[B.offPB + size_t(e) * nbp + j];
             }
             Z[B.offFV + i] = s;
         }
