@@ -872,7 +872,14 @@ namespace pbk
         if (lane < 36)
         {
             double s = Wk.D[blk * 36 + lane];
-            for (int it = it0; it < it1; ++it) s += double(Wk.itemPartial[it * 36 + lane]);
+            // at most kFinalizeSerialItems items: their loads all in flight, then summed in item order
+            T v[kFinalizeSerialItems];
+#pragma unroll
+            for (int q = 0; q < kFinalizeSerialItems; ++q)
+                if (it0 + q < it1) v[q] = Wk.itemPartial[(it0 + q) * 36 + lane];
+#pragma unroll
+            for (int q = 0; q < kFinalizeSerialItems; ++q)
+                if (it0 + q < it1) s += double(v[q]);
             if (diag && (lane / 6) == (lane % 6) && !rigMode(P)) s = s == 0.0 ? 1.0 : s * (1.0 + damping);
             Wk.S[P.blockPosUp[blk] * 36 + lane] = T(s);
             if (!diag) Wk.S[P.blockPosLo[blk] * 36 + (lane % 6) * 6 + lane / 6] = T(s);
