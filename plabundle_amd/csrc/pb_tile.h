@@ -126,15 +126,24 @@ namespace pbk
                     for (int q = 0; q < kPanel; ++q) buf[c0 + q][r] = r >= c0 + q ? a[q] : T(0);
                 }
                 __syncthreads();
-                // trailing columns c >= c0 + 16, rows r >= c0 + 16 (both triangles of the trailing block)
-                const int t0 = c0 + kPanel, nt = kT - t0;
-                for (int idx = threadIdx.x; idx < nt * nt; idx += blockDim.x)
+                // trailing block (rows and columns >= c0 + 16) minus the panel's rank-16 product, in 16 x 16 blocks
+                // on or below the diagonal, one block per wave with MFMA: A(i, k) = L(r0 + i, c0 + k),
+                // B(k, j) = L(s0 + j, c0 + k). Blocks above the diagonal are not needed: each panel reads its
+                // columns on and below the diagonal and writes zeros above it.
+                const int t0 = c0 + kPanel, nb = (kT - t0) / kPanel;
+                const int li = lane & 15, lk = lane >> 4;
+                for (int b = threadIdx.x >> 6; b < nb * (nb + 1) / 2; b += blockDim.x >> 6)
                 {
-                    const int r = t0 + idx % nt, c = t0 + idx / nt;
-                    T s = T(0);
+                    int bi = 0;
+                    while ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
+                    const int r0 = t0 + kPanel * bi, s0 = t0 + kPanel * (b - bi * (bi + 1) / 2);
+                    typename BandMfma<T>::V acc = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-                    for (int q = 0; q < kPanel; ++q) s += buf[c0 + q][r] * buf[c0 + q][c];
-                    buf[c][r] -= s;
+                    for (int k = 0; k < kPanel; k += 4)
+                        acc = BandMfma<T>::op(buf[c0 + k + lk][r0 + li], buf[c0 + k + lk][s0 + li], acc);
+                    constexpr int kLR = BandMfma<T>::kLaneRow, kIR = BandMfma<T>::kItemRow;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) buf[s0 + li][r0 + kLR * lk + kIR * e] -= acc[e];
                 }
                 __syncthreads();
             }
